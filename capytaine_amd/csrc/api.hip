@@ -1,0 +1,464 @@
+// C-ABI of libcapytaine_amd: plain pointers and sizes, int status codes
+// (0 = success), no exceptions across the boundary.  Declarations and the
+// reference interface each entry point replaces are in
+// include/capytaine_amd.h.
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+#include "cpt_internal.hpp"
+#include "../../include/capytaine_amd.h"
+
+namespace cpt {
+
+int launch_tabulation(const double* r_range, int nr, const double* z_range, int nz, int npts, double* tab,
+                      hipStream_t stream);
+int launch_table_relayout(const double* src, int nr, int nz, double* dst, hipStream_t stream);
+int launch_zgemm(int M, int N, int K, double alpha, const cplx* A, int64_t lda, int64_t sA, const cplx* B,
+                 int64_t ldb, int64_t sB, cplx* C, int64_t ldc, int64_t sC, int beta_zero, int batch,
+                 hipStream_t stream);
+int launch_getrf(cplx* A, int N, int64_t lda, int64_t strideA, int* ipiv, int64_t strideP, int* info, int batch,
+                 hipStream_t stream);
+int launch_getrs(const cplx* LU, int N, int64_t lda, int64_t strideA, const int* ipiv, int64_t strideP, cplx* B,
+                 int nrhs, int64_t ldb, int64_t strideB, int batch, hipStream_t stream);
+
+static thread_local std::string g_last_error;
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+// Growable device buffer owned by a context.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    int reserve(size_t need)
+    {
+        if (need <= bytes) return 0;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        if (hipMalloc(&p, need) != hipSuccess) return -1;
+        bytes = need;
+        return 0;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+};
+
+}  // namespace cpt
+
+struct cpt_context {
+    int device = 0;
+    // tabulation
+    cpt::DevBuf r_range, z_range, table;
+    int nr = 0, nz = 0, grid = 1, npts = 1001;
+    double rmax = 0, zmax_abs = 0, dz = 0;
+    int index_of_1 = 81;
+    bool has_table = false;
+    // per-call workspace
+    cpt::DevBuf faces, quad, freq, prony;
+    cpt::DevBuf lu_work;
+};
+
+using namespace cpt;
+
+#define CPT_CHECK_HIP(expr, what)                                                            \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess) {                                                              \
+            set_error(std::string(what) + ": " + hipGetErrorString(e_));                     \
+            return CPT_ERR_HIP;                                                              \
+        }                                                                                    \
+    } while (0)
+
+namespace {
+
+__global__ void pack_faces_kernel(const double* __restrict__ vertices, int nv, const int32_t* __restrict__ faces,
+                                  const double* __restrict__ centers, const double* __restrict__ normals,
+                                  const double* __restrict__ areas, const double* __restrict__ radii,
+                                  const double* __restrict__ qp, const double* __restrict__ qw, int m, int nq,
+                                  Face* __restrict__ out, double* __restrict__ quad)
+{
+    const int J = blockIdx.x * blockDim.x + threadIdx.x;
+    if (J >= m) return;
+    Face F;
+#pragma unroll
+    for (int L = 0; L < 4; ++L) {
+        // Indices are validated by the host wrapper; clamping keeps a bad
+        // index from reading outside the vertex array.
+        int v = faces[4 * (size_t)J + L];
+        v = (v < 0 || v >= nv) ? 0 : v;
+        for (int c = 0; c < 3; ++c) F.v[L][c] = vertices[3 * (size_t)v + c];
+    }
+    for (int c = 0; c < 3; ++c) {
+        F.c[c] = centers[3 * (size_t)J + c];
+        F.n[c] = normals[3 * (size_t)J + c];
+    }
+    F.area = areas[J];
+    F.radius = radii[J];
+    for (int c = 0; c < 4; ++c) F.pad[c] = 0.0;
+    out[J] = F;
+    for (int q = 0; q < nq; ++q) {
+        const size_t s = (size_t)J * nq + q;
+        quad[4 * s + 0] = qp[3 * s + 0];
+        quad[4 * s + 1] = qp[3 * s + 1];
+        quad[4 * s + 2] = qp[3 * s + 2];
+        quad[4 * s + 3] = qw[s];
+    }
+}
+
+__global__ void wave_points_kernel(Table T, int np, const double* __restrict__ x0i, const double* __restrict__ x0j,
+                                   double k, int gfs, cplx* __restrict__ g, cplx* __restrict__ ng)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    cplx G, nG[3];
+    wave_point(T, k, gfs, x0i[3 * (size_t)p] - x0j[3 * (size_t)p], x0i[3 * (size_t)p + 1] - x0j[3 * (size_t)p + 1],
+               x0i[3 * (size_t)p + 2] + x0j[3 * (size_t)p + 2], G, nG);
+    g[p] = G;
+    for (int c = 0; c < 3; ++c) ng[3 * (size_t)p + c] = nG[c];
+}
+
+__global__ void zero_int_kernel(int* __restrict__ p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+Table table_of(const cpt_context* h)
+{
+    Table T;
+    T.tab = (const double*)h->table.p;
+    T.r_range = (const double*)h->r_range.p;
+    T.z_range = (const double*)h->z_range.p;
+    T.nr = h->nr;
+    T.nz = h->nz;
+    T.grid = h->grid;
+    T.npts = h->npts;
+    T.rmax = h->rmax;
+    T.zmax_abs = h->zmax_abs;
+    T.dz = h->dz;
+    T.index_of_1 = h->index_of_1;
+    return T;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* cpt_last_error(void) { return g_last_error.c_str(); }
+
+int cpt_version(void) { return CPT_ABI_VERSION; }
+
+int cpt_create(int device, cpt_handle* out)
+{
+    if (!out) {
+        set_error("cpt_create: null output pointer");
+        return CPT_ERR_ARG;
+    }
+    int count = 0;
+    CPT_CHECK_HIP(hipGetDeviceCount(&count), "hipGetDeviceCount");
+    if (device < 0 || device >= count) {
+        set_error("cpt_create: invalid device index " + std::to_string(device));
+        return CPT_ERR_ARG;
+    }
+    cpt_context* h = new cpt_context();
+    h->device = device;
+    *out = h;
+    return CPT_OK;
+}
+
+int cpt_destroy(cpt_handle h)
+{
+    if (!h) return CPT_OK;
+    (void)hipSetDevice(h->device);
+    h->r_range.release();
+    h->z_range.release();
+    h->table.release();
+    h->faces.release();
+    h->quad.release();
+    h->freq.release();
+    h->prony.release();
+    h->lu_work.release();
+    delete h;
+    return CPT_OK;
+}
+
+int cpt_tabulate(cpt_handle h, const double* r_range, int nr, const double* z_range, int nz,
+                 int nb_integration_points, double* tab, hipStream_t stream)
+{
+    if (!h || !r_range || !z_range || !tab || nr <= 0 || nz <= 0 || nb_integration_points <= 0) {
+        set_error("cpt_tabulate: invalid arguments");
+        return CPT_ERR_ARG;
+    }
+    CPT_CHECK_HIP(hipSetDevice(h->device), "hipSetDevice");
+    int e = launch_tabulation(r_range, nr, z_range, nz, nb_integration_points, tab, stream);
+    CPT_CHECK_HIP((hipError_t)e, "tabulate kernel launch");
+    return CPT_OK;
+}
+
+int cpt_set_tabulation(cpt_handle h, const double* r_range, int nr, const double* z_range, int nz,
+                       const double* tab, int grid_shape, int nb_integration_points, hipStream_t stream)
+{
+    if (!h || nr < 0 || nz < 0 || (grid_shape != CPT_LEGACY_GRID && grid_shape != CPT_SCALED_NEMOH3_GRID)) {
+        set_error("cpt_set_tabulation: invalid arguments");
+        return CPT_ERR_ARG;
+    }
+    if (nr > 1 && nz > 1 && (!r_range || !z_range || !tab)) {
+        set_error("cpt_set_tabulation: null table pointer");
+        return CPT_ERR_ARG;
+    }
+    if (nr <= 1 || nz <= 1) {
+        set_error("cpt_set_tabulation: untabulated evaluation (nr or nz <= 1) is not supported on the device");
+        return CPT_ERR_UNSUPPORTED;
+    }
+    CPT_CHECK_HIP(hipSetDevice(h->device), "hipSetDevice");
+    if (h->r_range.reserve(sizeof(double) * nr) || h->z_range.reserve(sizeof(double) * nz) ||
+        h->table.reserve(sizeof(double) * 5 * (size_t)nr * nz)) {
+        set_error("cpt_set_tabulation: device allocation failed");
+        return CPT_ERR_ALLOC;
+    }
+    CPT_CHECK_HIP(hipMemcpyAsync(h->r_range.p, r_range, sizeof(double) * nr, hipMemcpyDeviceToDevice, stream),
+                  "copy r_range");
+    CPT_CHECK_HIP(hipMemcpyAsync(h->z_range.p, z_range, sizeof(double) * nz, hipMemcpyDeviceToDevice, stream),
+                  "copy z_range");
+    int e = launch_table_relayout(tab, nr, nz, (double*)h->table.p, stream);
+    CPT_CHECK_HIP((hipError_t)e, "table relayout launch");
+    double last[2];
+    CPT_CHECK_HIP(hipMemcpyAsync(&last[0], (const double*)r_range + (nr - 1), sizeof(double),
+                                 hipMemcpyDeviceToHost, stream),
+                  "read rmax");
+    CPT_CHECK_HIP(hipMemcpyAsync(&last[1], (const double*)z_range + (nz - 1), sizeof(double),
+                                 hipMemcpyDeviceToHost, stream),
+                  "read zmin");
+    CPT_CHECK_HIP(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    h->nr = nr;
+    h->nz = nz;
+    h->grid = grid_shape;
+    h->npts = nb_integration_points;
+    h->rmax = last[0];
+    h->zmax_abs = std::fabs(last[1]);
+    // Delhommeau_integrals.f90:360 and :324 (the latter in default real)
+    h->dz = (std::log10(std::fabs(last[1])) + 10.0) / nz;
+    h->index_of_1 = (int)std::lround((float)(nr * 81) / 676.0f);
+    h->has_table = true;
+    return CPT_OK;
+}
+
+int cpt_build_matrices(cpt_handle h, int nb_collocation_points, const double* collocation_points,
+                       const double* dot_product_normals, int nb_vertices, const double* vertices, int nb_faces,
+                       const int32_t* faces, const double* centers, const double* normals, const double* areas,
+                       const double* radiuses, int nb_quad_points, const double* quad_points,
+                       const double* quad_weights, int nb_wavenumbers, const double* wavenumbers,
+                       const int* gf_singularities, double depth, int finite_depth_method, int nexp,
+                       const double* prony, int adjoint_double_layer, int early_dot_product,
+                       int diagonal_term_in_double_layer, double free_surface, void* S, int64_t ldS,
+                       int64_t strideS, void* K, int64_t ldK, int64_t strideK, hipStream_t stream)
+{
+    if (!h) {
+        set_error("cpt_build_matrices: null handle");
+        return CPT_ERR_ARG;
+    }
+    const int n = nb_collocation_points, m = nb_faces, nq = nb_quad_points, nb = nb_wavenumbers;
+    if (n < 0 || m < 0 || nq < 1 || nq > 16 || nb < 1 || nb_vertices < 0 || ldS < n || ldK < n ||
+        (nb > 1 && (strideS < ldS * m || strideK < ldK * m * (early_dot_product ? 1 : 3)))) {
+        set_error("cpt_build_matrices: invalid sizes or strides");
+        return CPT_ERR_ARG;
+    }
+    if (n == 0 || m == 0) return CPT_OK;
+    if (!collocation_points || !dot_product_normals || !vertices || !faces || !centers || !normals || !areas ||
+        !radiuses || !quad_points || !quad_weights || !wavenumbers || !gf_singularities || !S || !K) {
+        set_error("cpt_build_matrices: null pointer argument");
+        return CPT_ERR_ARG;
+    }
+    const bool rankine_only = std::isinf(free_surface);
+    const bool finite_depth = std::isfinite(depth);
+    if (rankine_only && finite_depth) {
+        set_error("cpt_build_matrices: free_surface=inf requires an infinite water depth");
+        return CPT_ERR_ARG;
+    }
+    if (finite_depth && finite_depth_method != CPT_LEGACY_FINITE_DEPTH &&
+        finite_depth_method != CPT_NEWER_FINITE_DEPTH) {
+        set_error("cpt_build_matrices: only Delhommeau's finite-depth methods are implemented on the device");
+        return CPT_ERR_UNSUPPORTED;
+    }
+    if (finite_depth && nexp > 0 && !prony) {
+        set_error("cpt_build_matrices: finite depth needs the Prony decomposition");
+        return CPT_ERR_ARG;
+    }
+    bool need_table = false;
+    for (int b = 0; b < nb; ++b) {
+        const double k = wavenumbers[b];
+        const int g = gf_singularities[b];
+        if (g != CPT_HIGH_FREQ && g != CPT_LOW_FREQ && g != CPT_LOW_FREQ_WITH_RANKINE_PART) {
+            set_error("cpt_build_matrices: invalid gf_singularities value");
+            return CPT_ERR_ARG;
+        }
+        if (std::isnan(k) || k < 0) {
+            set_error("cpt_build_matrices: invalid wavenumber");
+            return CPT_ERR_ARG;
+        }
+        if (!rankine_only && k > 0 && std::isfinite(k)) need_table = true;
+    }
+    if (need_table && !h->has_table) {
+        set_error("cpt_build_matrices: no tabulation set (call cpt_set_tabulation first)");
+        return CPT_ERR_STATE;
+    }
+    CPT_CHECK_HIP(hipSetDevice(h->device), "hipSetDevice");
+
+    // Pack panels and quadrature points.
+    const int ne = finite_depth ? std::max(nexp, 0) : 0;
+    if (h->faces.reserve(sizeof(Face) * (size_t)m) ||
+        h->quad.reserve(sizeof(double) * 4 * (size_t)m * nq) || h->freq.reserve(sizeof(FreqArgs) * nb) ||
+        h->prony.reserve(sizeof(double) * (2 * (size_t)nb * ne + 1))) {
+        set_error("cpt_build_matrices: device allocation failed");
+        return CPT_ERR_ALLOC;
+    }
+    hipLaunchKernelGGL(pack_faces_kernel, dim3((m + 255) / 256), dim3(256), 0, stream, vertices, nb_vertices, faces,
+                       centers, normals, areas, radiuses, quad_points, quad_weights, m, nq, (Face*)h->faces.p,
+                       (double*)h->quad.p);
+    CPT_CHECK_HIP(hipGetLastError(), "pack kernel launch");
+
+    std::vector<FreqArgs> fa(nb);
+    const double* prony_dev = (const double*)h->prony.p;
+    if (ne > 0)
+        CPT_CHECK_HIP(hipMemcpyAsync(h->prony.p, prony, sizeof(double) * 2 * (size_t)nb * ne, hipMemcpyHostToDevice,
+                                     stream),
+                      "copy prony");
+    for (int b = 0; b < nb; ++b) {
+        const double k = wavenumbers[b];
+        fa[b].k = k;
+        fa[b].gfs = gf_singularities[b];
+        fa[b].wave = (0.0 < k && std::isfinite(k)) ? 1 : 0;
+        fa[b].nexp = ne;
+        fa[b].pad = 0;
+        fa[b].prony_lambda = prony_dev + (size_t)b * 2 * ne;
+        fa[b].prony_a = prony_dev + (size_t)b * 2 * ne + ne;
+    }
+    CPT_CHECK_HIP(hipMemcpyAsync(h->freq.p, fa.data(), sizeof(FreqArgs) * nb, hipMemcpyHostToDevice, stream),
+                  "copy frequency table");
+
+    AssemblyArgs A;
+    std::memset(&A, 0, sizeof(A));
+    A.colloc = collocation_points;
+    A.dpn = dot_product_normals;
+    A.n = n;
+    A.m = m;
+    A.nq = nq;
+    A.nk = early_dot_product ? 1 : 3;
+    A.faces = (const Face*)h->faces.p;
+    A.quad = (const double*)h->quad.p;
+    A.T = table_of(h);
+    A.depth = depth;
+    A.finite_depth = finite_depth ? 1 : 0;
+    A.fd_method = finite_depth_method;
+    A.adjoint = adjoint_double_layer ? 1 : 0;
+    A.rankine_only = rankine_only ? 1 : 0;
+    A.diag = diagonal_term_in_double_layer ? 1 : 0;
+    A.n_diag = std::min(n, m);
+    A.free_surface = free_surface;
+    A.nb = nb;
+    // Enough work-groups to fill 256 CUs several times over.
+    const int gx = (n + 255) / 256;
+    int chunk = (int)std::max<int64_t>(1, ((int64_t)gx * m) / 4096);
+    A.j_chunk = std::min(chunk, 64);
+    A.freq = (const FreqArgs*)h->freq.p;
+    A.S = (cplx*)S;
+    A.ldS = ldS;
+    A.strideS = strideS;
+    A.K = (cplx*)K;
+    A.ldK = ldK;
+    A.strideK = strideK;
+    int e = launch_assembly(A, stream);
+    CPT_CHECK_HIP((hipError_t)e, "assembly kernel launch");
+    return CPT_OK;
+}
+
+int cpt_wave_part_infinite_depth(cpt_handle h, int np, const double* x0i, const double* x0j, double wavenumber,
+                                 int gf_singularities, void* g, void* nablag, hipStream_t stream)
+{
+    if (!h || np < 0 || (np > 0 && (!x0i || !x0j || !g || !nablag))) {
+        set_error("cpt_wave_part_infinite_depth: invalid arguments");
+        return CPT_ERR_ARG;
+    }
+    if (!h->has_table) {
+        set_error("cpt_wave_part_infinite_depth: no tabulation set");
+        return CPT_ERR_STATE;
+    }
+    if (np == 0) return CPT_OK;
+    CPT_CHECK_HIP(hipSetDevice(h->device), "hipSetDevice");
+    hipLaunchKernelGGL(wave_points_kernel, dim3((np + 255) / 256), dim3(256), 0, stream, table_of(h), np, x0i, x0j,
+                       wavenumber, gf_singularities, (cplx*)g, (cplx*)nablag);
+    CPT_CHECK_HIP(hipGetLastError(), "wave kernel launch");
+    return CPT_OK;
+}
+
+int cpt_lu_factor(cpt_handle h, void* A, int n, int64_t lda, int64_t strideA, int* ipiv, int64_t strideP,
+                  int* info, int batch, hipStream_t stream)
+{
+    if (!h || n < 0 || batch < 0 || lda < n || (batch > 1 && (strideA < lda * n || strideP < n))) {
+        set_error("cpt_lu_factor: invalid sizes or strides");
+        return CPT_ERR_ARG;
+    }
+    if (n == 0 || batch == 0) return CPT_OK;
+    if (!A || !ipiv || !info) {
+        set_error("cpt_lu_factor: null pointer argument");
+        return CPT_ERR_ARG;
+    }
+    if (n > 24576) {
+        set_error("cpt_lu_factor: n > 24576 is not supported");
+        return CPT_ERR_UNSUPPORTED;
+    }
+    CPT_CHECK_HIP(hipSetDevice(h->device), "hipSetDevice");
+    hipLaunchKernelGGL(zero_int_kernel, dim3((batch + 255) / 256), dim3(256), 0, stream, info, batch);
+    CPT_CHECK_HIP(hipGetLastError(), "zero info launch");
+    CPT_CHECK_HIP((hipError_t)launch_getrf((cplx*)A, n, lda, strideA, ipiv, strideP, info, batch, stream),
+                  "getrf launch");
+    return CPT_OK;
+}
+
+int cpt_lu_solve(cpt_handle h, const void* LU, int n, int64_t lda, int64_t strideA, const int* ipiv,
+                 int64_t strideP, void* B, int nrhs, int64_t ldb, int64_t strideB, int batch, hipStream_t stream)
+{
+    if (!h || n < 0 || nrhs < 0 || batch < 0 || lda < n || ldb < n ||
+        (batch > 1 && (strideA < lda * n || strideP < n || strideB < ldb * nrhs))) {
+        set_error("cpt_lu_solve: invalid sizes or strides");
+        return CPT_ERR_ARG;
+    }
+    if (n == 0 || nrhs == 0 || batch == 0) return CPT_OK;
+    if (!LU || !ipiv || !B) {
+        set_error("cpt_lu_solve: null pointer argument");
+        return CPT_ERR_ARG;
+    }
+    CPT_CHECK_HIP(hipSetDevice(h->device), "hipSetDevice");
+    CPT_CHECK_HIP((hipError_t)launch_getrs((const cplx*)LU, n, lda, strideA, ipiv, strideP, (cplx*)B, nrhs, ldb,
+                                           strideB, batch, stream),
+                  "getrs launch");
+    return CPT_OK;
+}
+
+int cpt_zgemm(cpt_handle h, int M, int N, int K, double alpha, const void* A, int64_t lda, int64_t strideA,
+              const void* B, int64_t ldb, int64_t strideB, void* C, int64_t ldc, int64_t strideC, int beta_zero,
+              int batch, hipStream_t stream)
+{
+    if (!h || M < 0 || N < 0 || K < 0 || batch < 0 || lda < M || ldb < K || ldc < M) {
+        set_error("cpt_zgemm: invalid sizes");
+        return CPT_ERR_ARG;
+    }
+    if (M == 0 || N == 0 || batch == 0) return CPT_OK;
+    if (!A || !B || !C) {
+        set_error("cpt_zgemm: null pointer argument");
+        return CPT_ERR_ARG;
+    }
+    CPT_CHECK_HIP(hipSetDevice(h->device), "hipSetDevice");
+    CPT_CHECK_HIP((hipError_t)launch_zgemm(M, N, K, alpha, (const cplx*)A, lda, strideA, (const cplx*)B, ldb, strideB,
+                                           (cplx*)C, ldc, strideC, beta_zero, batch, stream),
+                  "zgemm launch");
+    return CPT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,147 @@
+/*
+ * libcapytaine_amd -- C-ABI of the MI355X (gfx950) implementation of the
+ * Capytaine boundary-element hot path: Delhommeau Green-function
+ * evaluation, assembly of the influence matrices S and K, and their dense
+ * LU solve.
+ *
+ * Conventions
+ *   - Every function returns 0 (CPT_OK) on success or a negative CPT_ERR_*
+ *     code; cpt_last_error() then describes the failure (thread-local).
+ *   - Array arguments are DEVICE pointers (hipMalloc'd or torch CUDA/HIP
+ *     tensors), owned by the caller, unless the parameter says "host".
+ *   - Work is enqueued on the caller's stream and is asynchronous unless
+ *     stated otherwise.  A handle owns only workspace and the device copy
+ *     of the tabulation; one handle per host thread (or per stream).
+ *   - Complex numbers are interleaved (re, im) doubles: numpy complex128.
+ *   - Matrices are column-major ("Fortran order"), as in the reference.
+ *   - Point/vector arrays are row-major (n, 3): numpy's default layout.
+ *
+ * Reference citations are relative to the capytaine 3.0.0 source tree.
+ */
+#ifndef CAPYTAINE_AMD_H
+#define CAPYTAINE_AMD_H
+
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CPT_ABI_VERSION 1
+
+enum {
+    CPT_OK = 0,
+    CPT_ERR_ARG = -1,         /* invalid argument */
+    CPT_ERR_HIP = -2,         /* HIP runtime error */
+    CPT_ERR_ALLOC = -3,       /* device allocation failed */
+    CPT_ERR_STATE = -4,       /* call order violated (e.g. no tabulation) */
+    CPT_ERR_UNSUPPORTED = -5  /* option of the reference not implemented here */
+};
+
+/* Values of the integer options, equal to the reference's Fortran
+ * enumerations (libs/libDelhommeau/src/constants.f90:40-52). */
+enum { CPT_LEGACY_GRID = 0, CPT_SCALED_NEMOH3_GRID = 1 };
+enum { CPT_LEGACY_FINITE_DEPTH = 0, CPT_NEWER_FINITE_DEPTH = 1 };
+enum { CPT_HIGH_FREQ = 0, CPT_LOW_FREQ = 1, CPT_LOW_FREQ_WITH_RANKINE_PART = 2 };
+
+typedef struct cpt_context* cpt_handle;
+
+/* Last error message of the calling thread ("" if none). */
+const char* cpt_last_error(void);
+/* CPT_ABI_VERSION of the library. */
+int cpt_version(void);
+
+/* Create / destroy a context bound to HIP device `device`.
+ * Replaces the module state of the f2py extension
+ * capytaine.green_functions.Delhommeau_float64 (meson.build:39-47,96-112). */
+int cpt_create(int device, cpt_handle* out);
+int cpt_destroy(cpt_handle h);
+
+/* Tabulated Delhommeau integrals I1..I5 on the grid (r_range, z_range):
+ * writes tab, shape (nr, nz, 5) column-major, the reference table layout.
+ * Replaces delhommeau_integrals.construct_tabulation
+ * (libs/libDelhommeau/src/Delhommeau_integrals.f90:190-204, with
+ * numerical_integration :43-153), called from
+ * src/capytaine/green_functions/delhommeau.py:259-261. */
+int cpt_tabulate(cpt_handle h, const double* r_range, int nr, const double* z_range, int nz,
+                 int nb_integration_points, double* tab, hipStream_t stream);
+
+/* Copy a tabulation into the context (re-laid out for the assembly kernel).
+ * Takes the arguments the reference passes to build_matrices as
+ * `*self.all_tabulation_parameters` (delhommeau.py:263-267, 505):
+ * nb_integration_points, grid_shape, r_range, z_range, tabulated_integrals.
+ * Synchronises `stream`. */
+int cpt_set_tabulation(cpt_handle h, const double* r_range, int nr, const double* z_range, int nz,
+                       const double* tab, int grid_shape, int nb_integration_points, hipStream_t stream);
+
+/* Assemble S and K for nb_wavenumbers wavenumbers in one launch.
+ * Replaces matrices.build_matrices (libs/libDelhommeau/src/matrices.f90:28-325)
+ * followed by matrices.add_diagonal_term (:329-364) when
+ * diagonal_term_in_double_layer, and matrices.add_rankine_term_only
+ * (:368-432) when free_surface is +inf; called from
+ * src/capytaine/green_functions/delhommeau.py:498-515 and :377-390.
+ *
+ *   collocation_points  (n,3); dot_product_normals (n,3) if adjoint else (m,3)
+ *   vertices (nv,3); faces (m,4) int32, 0-based (the reference passes faces+1)
+ *   centers, normals (m,3); areas, radiuses (m)
+ *   quad_points (m,nq,3); quad_weights (m,nq)
+ *   wavenumbers, gf_singularities: HOST arrays (nb); gf_singularities after
+ *       the overrides of delhommeau.py:466-490
+ *   depth: +inf for infinite depth; finite_depth_method: CPT_*_FINITE_DEPTH
+ *   prony: HOST array (nb, 2, nexp) of (lambda_k, a_k), the reference's
+ *       prony_decomposition (delhommeau.py:492-495); unused in infinite depth
+ *   early_dot_product: K is (n,m) if non-zero, else (3,n,m)
+ *   S: complex128, S[i + ldS*j] (+ b*strideS for wavenumber b), OVERWRITTEN
+ *   K: complex128, K[c + nk*(i + ldK*j)] (+ b*strideK), nk = 1 or 3, OVERWRITTEN
+ * (the reference accumulates into caller-zeroed arrays: same values). */
+int cpt_build_matrices(cpt_handle h, int nb_collocation_points, const double* collocation_points,
+                       const double* dot_product_normals, int nb_vertices, const double* vertices, int nb_faces,
+                       const int32_t* faces, const double* centers, const double* normals, const double* areas,
+                       const double* radiuses, int nb_quad_points, const double* quad_points,
+                       const double* quad_weights, int nb_wavenumbers, const double* wavenumbers,
+                       const int* gf_singularities, double depth, int finite_depth_method, int nexp,
+                       const double* prony, int adjoint_double_layer, int early_dot_product,
+                       int diagonal_term_in_double_layer, double free_surface, void* S, int64_t ldS,
+                       int64_t strideS, void* K, int64_t ldK, int64_t strideK, hipStream_t stream);
+
+/* Wave part of the infinite-depth Green function at np point pairs.
+ * Replaces interface.vectorized_wave_part_infinite_depth
+ * (libs/libDelhommeau/src/interface.f90:29-67) and
+ * green_wave.wave_part_infinite_depth (Green_wave.f90:162-271), the hooks the
+ * reference's tests call (pytest/test_green_functions_fortran_core.py:37-114).
+ * x0i, x0j (np,3); g (np) complex128; nablag (np,3) complex128 row-major.
+ * Uses the tabulation set by cpt_set_tabulation. */
+int cpt_wave_part_infinite_depth(cpt_handle h, int np, const double* x0i, const double* x0j, double wavenumber,
+                                 int gf_singularities, void* g, void* nablag, hipStream_t stream);
+
+/* Batched LU factorisation with partial pivoting of `batch` complex128
+ * n x n matrices A + b*strideA (column-major, leading dimension lda), in
+ * place.  ipiv[b*strideP + i] (0-based) and info[b] as LAPACK zgetrf
+ * (info = 1 + index of the first exactly-zero pivot, 0 if none).
+ * Replaces scipy.linalg.lu_factor (LAPACK zgetrf) as called by
+ * src/capytaine/tools/block_circulant_matrices.py:361-378 from
+ * src/capytaine/bem/engines.py:326-335. */
+int cpt_lu_factor(cpt_handle h, void* A, int n, int64_t lda, int64_t strideA, int* ipiv, int64_t strideP,
+                  int* info, int batch, hipStream_t stream);
+
+/* Solve A X = B for nrhs right-hand sides at once with the factors of
+ * cpt_lu_factor; B (n x nrhs, column-major, ldb) is overwritten by X.
+ * Replaces scipy.linalg.lu_solve (LAPACK zgetrs),
+ * src/capytaine/tools/block_circulant_matrices.py:380-385, which the
+ * reference calls once per right-hand side (engines.py:338). */
+int cpt_lu_solve(cpt_handle h, const void* LU, int n, int64_t lda, int64_t strideA, const int* ipiv,
+                 int64_t strideP, void* B, int nrhs, int64_t ldb, int64_t strideB, int batch, hipStream_t stream);
+
+/* C = alpha*A*B + (beta_zero ? 0 : C), complex128, column-major, batched.
+ * Replaces the products `S @ sources` of src/capytaine/bem/solver.py:197-198
+ * (numpy matmul), done here for all right-hand sides at once. */
+int cpt_zgemm(cpt_handle h, int M, int N, int K, double alpha, const void* A, int64_t lda, int64_t strideA,
+              const void* B, int64_t ldb, int64_t strideB, void* C, int64_t ldc, int64_t strideC, int beta_zero,
+              int batch, hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CAPYTAINE_AMD_H */
