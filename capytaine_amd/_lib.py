@@ -1,0 +1,117 @@
+"""ctypes binding of libcapytaine_amd.so (declared in include/capytaine_amd.h).
+
+The library is built in place by ``capytaine_amd.build_lib``.  There is no
+CPU fallback: every entry point that computes requires the HIP library and a
+GPU, and raises otherwise.
+"""
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcapytaine_amd.so")
+
+CPT_LEGACY_GRID, CPT_SCALED_NEMOH3_GRID = 0, 1
+CPT_LEGACY_FINITE_DEPTH, CPT_NEWER_FINITE_DEPTH = 0, 1
+CPT_HIGH_FREQ, CPT_LOW_FREQ, CPT_LOW_FREQ_WITH_RANKINE_PART = 0, 1, 2
+
+# Every symbol of include/capytaine_amd.h with its argument types.
+_P, _I, _L, _D = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_double
+SIGNATURES = {
+    "cpt_last_error": (ctypes.c_char_p, []),
+    "cpt_version": (_I, []),
+    "cpt_create": (_I, [_I, ctypes.POINTER(_P)]),
+    "cpt_destroy": (_I, [_P]),
+    "cpt_tabulate": (_I, [_P, _P, _I, _P, _I, _I, _P, _P]),
+    "cpt_set_tabulation": (_I, [_P, _P, _I, _P, _I, _P, _I, _I, _P]),
+    "cpt_build_matrices": (_I, [_P, _I, _P, _P, _I, _P, _I, _P, _P, _P, _P, _P, _I, _P, _P, _I, _P, _P, _D, _I, _I,
+                                _P, _I, _I, _I, _D, _P, _L, _L, _P, _L, _L, _P]),
+    "cpt_wave_part_infinite_depth": (_I, [_P, _I, _P, _P, _D, _I, _P, _P, _P]),
+    "cpt_lu_factor": (_I, [_P, _P, _I, _L, _L, _P, _L, _P, _I, _P]),
+    "cpt_lu_solve": (_I, [_P, _P, _I, _L, _L, _P, _L, _P, _I, _L, _L, _I, _P]),
+    "cpt_zgemm": (_I, [_P, _I, _I, _I, _D, _P, _L, _L, _P, _L, _L, _P, _L, _L, _I, _I, _P]),
+}
+
+_lib = None
+_lock = threading.Lock()
+
+
+class CptError(RuntimeError):
+    """Error reported by libcapytaine_amd."""
+
+
+def load():
+    """Load the shared library (no GPU needed) and declare its signatures."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise CptError(f"{LIB_PATH} is missing: build it with `python -m capytaine_amd.build_lib`.")
+            lib = ctypes.CDLL(LIB_PATH)
+            for name, (res, args) in SIGNATURES.items():
+                fn = getattr(lib, name)
+                fn.restype = res
+                fn.argtypes = args
+            _lib = lib
+    return _lib
+
+
+def check(status, what):
+    if status != 0:
+        msg = load().cpt_last_error().decode(errors="replace")
+        raise CptError(f"{what} failed with status {status}: {msg}")
+
+
+def ptr(t):
+    """Device (or host) address of a torch tensor or numpy array."""
+    if isinstance(t, np.ndarray):
+        return ctypes.c_void_p(t.ctypes.data)
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def stream_handle(device):
+    import torch
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+class Context:
+    """One library context (cpt_handle) per device, created lazily."""
+
+    _contexts = {}
+
+    def __init__(self, device_index):
+        self.device_index = device_index
+        h = ctypes.c_void_p()
+        check(load().cpt_create(device_index, ctypes.byref(h)), "cpt_create")
+        self.handle = h
+        self.table_key = None
+
+    @classmethod
+    def get(cls, device):
+        import torch
+        dev = torch.device(device)
+        if dev.type != "cuda":
+            raise CptError(f"capytaine_amd computes on a GPU; got device {dev}")
+        idx = dev.index if dev.index is not None else torch.cuda.current_device()
+        with _lock:
+            ctx = cls._contexts.get(idx)
+            if ctx is None:
+                ctx = cls(idx)
+                cls._contexts[idx] = ctx
+        return ctx
+
+
+def require_gpu():
+    import torch
+    if not torch.cuda.is_available():
+        raise CptError("capytaine_amd needs a ROCm GPU (torch.cuda.is_available() is False); "
+                       "there is no CPU fallback.")
+    load()
+
+
+def default_device():
+    import torch
+    require_gpu()
+    return torch.device("cuda", torch.cuda.current_device())

@@ -1,0 +1,267 @@
+"""Matrix engine keeping the influence matrices and their LU factors on the GPU.
+
+Drop-in for ``capytaine.bem.engines.DefaultMatrixEngine``
+(src/capytaine/bem/engines.py:95-398): ``build_matrices`` with the
+last-computed cache keyed by (mesh1, mesh2, gf_params) (:258-298),
+``linear_solver(A, b)`` with the LU factorisation cached in place of K
+(:300-346), ``build_S_matrix`` / ``build_fullK_matrix`` (:142-176),
+``compute_ram_estimation`` (:348-398) and the NaN check (:400-405).  The
+matrices it returns are :class:`DeviceMatrix` objects: they expose
+``shape``, ``dtype``, ``@`` and ``__array__`` (a Fortran-ordered numpy copy),
+which is all the solver needs from them.
+"""
+import logging
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, ptr, stream_handle
+from .green_functions import Delhommeau, GreenFunctionEvaluationError
+
+LOG = logging.getLogger(__name__)
+
+
+class DeviceMatrix:
+    """Dense complex128 matrix (or batch of matrices) on the GPU.
+
+    `data` is a torch complex128 tensor of shape (nb, cols, rows) (or
+    (cols, rows)): each matrix is stored column-major, leading dimension
+    `rows`, like LAPACK and the reference's Fortran arrays.
+    """
+
+    def __init__(self, data, device=None):
+        self.data = data if data.dim() == 3 else data.unsqueeze(0)
+        self.device = self.data.device if device is None else device
+
+    @property
+    def batch(self):
+        return self.data.shape[0]
+
+    @property
+    def shape(self):
+        if self.batch == 1:
+            return (self.data.shape[2], self.data.shape[1])
+        return (self.batch, self.data.shape[2], self.data.shape[1])
+
+    @property
+    def dtype(self):
+        return np.dtype(np.complex128)
+
+    ndim = property(lambda self: len(self.shape))
+
+    def __array__(self, dtype=None, copy=None):
+        a = self.data.cpu().numpy().transpose(0, 2, 1)
+        if self.batch == 1:
+            a = a[0]
+        return a if dtype is None else a.astype(dtype)
+
+    def numpy(self):
+        return self.__array__()
+
+    def matmul_device(self, x):
+        """self @ x for a device tensor x of shape (nb, nrhs, cols)
+        (column-major (cols, nrhs) blocks); returns (nb, nrhs, rows)."""
+        import torch
+        nb, cols, rows = self.data.shape
+        nrhs = x.shape[1]
+        y = torch.empty((nb, nrhs, rows), dtype=torch.complex128, device=self.data.device)
+        ctx = _lib.Context.get(self.data.device)
+        check(_lib.load().cpt_zgemm(ctx.handle, rows, nrhs, cols, 1.0, ptr(self.data), rows, rows * cols, ptr(x),
+                                    cols, cols * nrhs, ptr(y), rows, rows * nrhs, 1, nb,
+                                    stream_handle(self.data.device)), "cpt_zgemm")
+        return y
+
+    def __matmul__(self, x):
+        import torch
+        x = np.asarray(x)
+        vec = x.ndim == 1
+        xb = x.reshape(x.shape[0], -1) if self.batch == 1 else x
+        if self.batch == 1:
+            xd = torch.as_tensor(np.ascontiguousarray(xb.T), dtype=torch.complex128).to(self.data.device)[None]
+        else:
+            xd = torch.as_tensor(np.ascontiguousarray(np.swapaxes(xb, -1, -2)),
+                                 dtype=torch.complex128).to(self.data.device)
+        y = self.matmul_device(xd).cpu().numpy()
+        y = np.swapaxes(y, -1, -2)
+        if self.batch == 1:
+            y = y[0]
+            return y[:, 0] if vec else y
+        return y
+
+
+class LUDecomposedDeviceMatrix:
+    """LU factors P A = L U of a batch of DeviceMatrix, computed by
+    cpt_lu_factor; `solve` replaces LUDecomposedMatrix.solve
+    (tools/block_circulant_matrices.py:374-385)."""
+
+    def __init__(self, A: DeviceMatrix, overwrite_a=False):
+        import torch
+        data = A.data if overwrite_a else A.data.clone()
+        nb, cols, rows = data.shape
+        if rows != cols:
+            raise ValueError("LU decomposition of a non-square matrix")
+        self.n = rows
+        self.lu = DeviceMatrix(data)
+        self.ipiv = torch.empty((nb, rows), dtype=torch.int32, device=data.device)
+        self.info = torch.empty((nb,), dtype=torch.int32, device=data.device)
+        ctx = _lib.Context.get(data.device)
+        check(_lib.load().cpt_lu_factor(ctx.handle, ptr(data), rows, rows, rows * rows, ptr(self.ipiv), rows,
+                                        ptr(self.info), nb, stream_handle(data.device)), "cpt_lu_factor")
+        self._checked = False
+
+    shape = property(lambda self: self.lu.shape)
+    dtype = property(lambda self: self.lu.dtype)
+
+    def check_info(self):
+        if not self._checked:
+            info = self.info.cpu().numpy()
+            if np.any(info != 0):
+                LOG.warning("LU factorisation: exactly singular matrix (info=%s)", info.tolist())
+            self._checked = True
+
+    def solve_device(self, b):
+        """Solve in place for a device tensor b of shape (nb, nrhs, n)."""
+        nb, nrhs, n = b.shape
+        ctx = _lib.Context.get(b.device)
+        check(_lib.load().cpt_lu_solve(ctx.handle, ptr(self.lu.data), n, n, n * n, ptr(self.ipiv), n, ptr(b),
+                                       nrhs, n, n * nrhs, nb, stream_handle(b.device)), "cpt_lu_solve")
+        return b
+
+    def solve(self, b):
+        import torch
+        b = np.asarray(b)
+        vec = b.ndim == 1
+        bb = b.reshape(b.shape[0], -1) if self.lu.batch == 1 else b
+        if self.lu.batch == 1:
+            bd = torch.as_tensor(np.ascontiguousarray(bb.T), dtype=torch.complex128).to(self.lu.device)[None]
+        else:
+            bd = torch.as_tensor(np.ascontiguousarray(np.swapaxes(bb, -1, -2)),
+                                 dtype=torch.complex128).to(self.lu.device)
+        x = self.solve_device(bd).cpu().numpy()
+        self.check_info()
+        x = np.swapaxes(x, -1, -2)
+        if self.lu.batch == 1:
+            x = x[0]
+            return x[:, 0] if vec else x
+        return x
+
+
+def has_been_lu_decomposed(A):
+    return isinstance(A, LUDecomposedDeviceMatrix)
+
+
+def lu_decompose(A, overwrite_a=False):
+    if not isinstance(A, DeviceMatrix):
+        raise TypeError("The device engine factorises DeviceMatrix objects")
+    return LUDecomposedDeviceMatrix(A, overwrite_a=overwrite_a)
+
+
+def check_if_nan_in_matrix(matrices):
+    import torch
+    for m in matrices:
+        data = m.data if isinstance(m, DeviceMatrix) else m
+        bad = bool(torch.isnan(torch.view_as_real(data)).any().item()) if hasattr(data, "device") else \
+            bool(np.any(np.isnan(data)))
+        if bad:
+            raise GreenFunctionEvaluationError("Green function returned a NaN in the interaction matrix.\n"
+                                               "It could be due to overlapping panels.")
+
+
+class DefaultMatrixEngine:
+    """GPU matrix engine.
+
+    Parameters
+    ----------
+    green_function: Delhommeau (GPU), default: Delhommeau()
+    linear_solver: "lu_decomposition" (default), "lu_decomposition_with_overwrite",
+        or a callable (A, b) -> x that receives numpy arrays.
+    check_nan: scan the matrices for NaN after assembly (engines.py:255).
+    """
+
+    def __init__(self, *, green_function=None, linear_solver='lu_decomposition', check_nan=True):
+        self.green_function = Delhommeau() if green_function is None else green_function
+        self._linear_solver = linear_solver
+        self.check_nan = check_nan
+        self.last_computed_inputs = None
+        self.last_computed_matrices = None
+        self.exportable_settings = {
+            'engine': 'DefaultMatrixEngine',
+            'linear_solver': str(linear_solver),
+            **self.green_function.exportable_settings,
+        }
+
+    def __str__(self):
+        return f"DefaultMatrixEngine(green_function={self.green_function}, linear_solver={self._linear_solver!r})"
+
+    __repr__ = __str__
+
+    @property
+    def device(self):
+        return self.green_function.device
+
+    def build_S_matrix(self, points, mesh, **gf_params):
+        S, _ = self.green_function.evaluate(points, mesh.merged(), **gf_params)
+        check_if_nan_in_matrix([S])
+        return S
+
+    def build_fullK_matrix(self, mesh1, mesh2, **gf_params):
+        gf_params.setdefault("diagonal_term_in_double_layer", True)
+        gf_params.setdefault("adjoint_double_layer", True)
+        gf_params.setdefault("early_dot_product", False)
+        _, fullK = self.green_function.evaluate(mesh1, mesh2.merged(), **gf_params)
+        check_if_nan_in_matrix([fullK])
+        return fullK
+
+    def _build(self, mesh1, mesh2, *, diagonal_term_in_double_layer=True, **gf_params):
+        gf_params.setdefault("early_dot_product", True)
+        if not gf_params["early_dot_product"]:
+            S, K = self.green_function.evaluate(mesh1, mesh2, diagonal_term_in_double_layer=diagonal_term_in_double_layer,
+                                                **gf_params)
+            check_if_nan_in_matrix([S, K])
+            return S, K
+        S, K = self.green_function.evaluate_on_device(mesh1, mesh2,
+                                                      diagonal_term_in_double_layer=diagonal_term_in_double_layer,
+                                                      **gf_params)
+        S, K = DeviceMatrix(S), DeviceMatrix(K)
+        if self.check_nan:
+            check_if_nan_in_matrix([S, K])
+        return S, K
+
+    def build_matrices(self, mesh1, mesh2, **gf_params):
+        """engines.py:258-298: returns the cached pair for identical inputs."""
+        if (mesh1, mesh2, gf_params) == self.last_computed_inputs:
+            LOG.debug("%s: reading cache.", self.__class__.__name__)
+            return self.last_computed_matrices
+        self.last_computed_matrices = None
+        S, K = self._build(mesh1, mesh2, **dict(gf_params))
+        self.last_computed_inputs = (mesh1, mesh2, dict(gf_params))
+        self.last_computed_matrices = (S, K)
+        return self.last_computed_matrices
+
+    def linear_solver(self, A, b):
+        """engines.py:300-346."""
+        if not isinstance(self._linear_solver, str):
+            x = self._linear_solver(np.asarray(A), b)
+            if not x.shape == b.shape:
+                raise ValueError(f"Error in linear solver of {self}: the shape of the output ({x.shape}) "
+                                 f"does not match the expected shape ({b.shape})")
+            return x
+        if self._linear_solver in ("lu_decomposition", "lu_decomposition_with_overwrite"):
+            overwrite_a = (self._linear_solver == "lu_decomposition_with_overwrite")
+            if not has_been_lu_decomposed(A):
+                luA = lu_decompose(A, overwrite_a=overwrite_a)
+                if self.last_computed_matrices is not None and A is self.last_computed_matrices[1]:
+                    self.last_computed_matrices = (self.last_computed_matrices[0], luA)
+            else:
+                luA = A
+            return luA.solve(b)
+        raise NotImplementedError(f"Unknown `linear_solver` in DefaultMatrixEngine: {self._linear_solver}")
+
+    def compute_ram_estimation(self, problem):
+        """Device memory estimate in GB (engines.py:348-398, plain meshes)."""
+        nb_faces = problem.body.mesh.nb_faces
+        nb_matrices = 3 if self._linear_solver == "lu_decomposition" else 2
+        return nb_faces ** 2 * nb_matrices * 16 / 1e9
+
+
+BasicMatrixEngine = DefaultMatrixEngine

@@ -1,0 +1,254 @@
+"""Bodies, problems and results: the host-side data around the device path.
+
+Restates the parts of capytaine the BEM solve needs:
+* rigid-body degrees of freedom (bodies/dofs.py:60-172) and
+  ``FloatingBody.integrate_pressure`` (bodies/bodies.py:238-250);
+* frequency/wavenumber relations (bem/problems_and_results.py:115-177);
+* radiation and diffraction Neumann conditions (:371-381, :434-449) with the
+  Airy incident wave (bem/airy_waves.py:19-102);
+* results: added mass, radiation damping (:583-592), diffraction and
+  Froude-Krylov forces (airy_waves.py:101-102).
+"""
+import logging
+
+import numpy as np
+from scipy.optimize import newton
+
+LOG = logging.getLogger(__name__)
+
+_default_parameters = dict(rho=1000.0, g=9.81, omega=1.0, free_surface=0.0, water_depth=np.inf,
+                           wave_direction=0.0, forward_speed=0.0)
+
+
+class TranslationDof:
+    def __init__(self, direction):
+        self.direction = np.asarray(direction, dtype=float)
+
+    def evaluate_motion(self, mesh):
+        return np.tile(self.direction, (mesh.nb_faces, 1))
+
+
+class RotationDof:
+    def __init__(self, rotation_center, direction):
+        self.direction = np.asarray(direction, dtype=float)
+        self.rotation_center = np.zeros(3) if rotation_center is None else np.asarray(rotation_center, float)
+
+    def evaluate_motion(self, mesh):
+        return np.cross(self.direction, mesh.faces_centers - self.rotation_center)
+
+
+def rigid_body_dofs(only=None, rotation_center=None):
+    """dofs.py:131-172."""
+    rc = np.zeros(3) if rotation_center is None else np.asarray(rotation_center, float)
+    dofs = {
+        "Surge": TranslationDof((1, 0, 0)),
+        "Sway": TranslationDof((0, 1, 0)),
+        "Heave": TranslationDof((0, 0, 1)),
+        "Roll": RotationDof(rc, (1, 0, 0)),
+        "Pitch": RotationDof(rc, (0, 1, 0)),
+        "Yaw": RotationDof(rc, (0, 0, 1)),
+    }
+    if only is not None:
+        dofs = {k: v for k, v in dofs.items() if k in only}
+    return dofs
+
+
+class FloatingBody:
+    """A mesh with degrees of freedom (bodies/bodies.py:85-250, no lid)."""
+
+    def __init__(self, mesh, dofs=None, name=None):
+        self.mesh = mesh
+        self.dofs = {} if dofs is None else dict(dofs)
+        self.name = name or mesh.name
+
+    @property
+    def mesh_including_lid(self):
+        return self.mesh
+
+    @property
+    def hull_mask(self):
+        return np.ones(self.mesh.nb_faces, dtype=bool)
+
+    def add_all_rigid_body_dofs(self, rotation_center=None):
+        self.dofs.update(rigid_body_dofs(rotation_center=rotation_center))
+        return self
+
+    def dof_motion(self, dof_name):
+        dof = self.dofs[dof_name]
+        return dof.evaluate_motion(self.mesh) if hasattr(dof, "evaluate_motion") else np.asarray(dof)
+
+    def dof_normal_matrix(self):
+        """(ndof, m) matrix of dof . n on each face, dofs in insertion order."""
+        n = self.mesh.faces_normals
+        return np.array([np.sum(self.dof_motion(d) * n, axis=1) for d in self.dofs])
+
+    def integrate_pressure(self, pressure):
+        """bodies.py:238-250: force on each dof, F = -sum p (dof . n) A."""
+        forces = {}
+        for dof_name in self.dofs:
+            dof = self.dof_motion(dof_name)
+            normal_dof_amplitude_on_face = - np.sum(dof * self.mesh.faces_normals, axis=1)
+            forces[dof_name] = np.sum(pressure * normal_dof_amplitude_on_face * self.mesh.faces_areas)
+        return forces
+
+    def __repr__(self):
+        return f"FloatingBody(name={self.name!r}, nb_faces={self.mesh.nb_faces}, dofs={list(self.dofs)})"
+
+
+def wavenumber_from_omega(omega, g, water_depth):
+    """problems_and_results.py:159-162."""
+    if water_depth == np.inf:
+        return omega ** 2 / g
+    return newton(lambda k: k * np.tanh(k * water_depth) - omega ** 2 / g, x0=1.0)
+
+
+class LinearPotentialFlowProblem:
+    def __init__(self, *, body=None, free_surface=0.0, water_depth=np.inf, omega=None, wavenumber=None,
+                 rho=1000.0, g=9.81, wave_direction=0.0):
+        self.body = body
+        self.free_surface = float(free_surface)
+        self.water_depth = float(water_depth)
+        self.rho = float(rho)
+        self.g = float(g)
+        self.wave_direction = float(wave_direction)
+        self.forward_speed = 0.0
+        if omega is not None and wavenumber is not None:
+            raise ValueError("Provide omega or wavenumber, not both")
+        if wavenumber is not None:
+            self.wavenumber = float(wavenumber)
+            self.omega = float(np.sqrt(self.g * self.wavenumber * np.tanh(self.wavenumber * self.water_depth)))
+        else:
+            self.omega = float(_default_parameters["omega"] if omega is None else omega)
+            if self.omega in (0.0, np.inf):
+                self.wavenumber = self.omega
+            else:
+                self.wavenumber = float(wavenumber_from_omega(self.omega, self.g, self.water_depth))
+        self.encounter_omega = self.omega
+        self.encounter_wavenumber = self.wavenumber
+        self.encounter_wave_direction = self.wave_direction
+
+    @property
+    def influenced_dofs(self):
+        return self.body.dofs
+
+    def _group_key(self):
+        """problems_and_results.py:266-275 grouping key."""
+        return (id(self.body), self.water_depth, self.omega, self.rho, self.g, self.free_surface)
+
+
+def airy_waves_velocity(points, pb):
+    """airy_waves.py:56-94."""
+    x, y, z = np.asarray(points).T
+    k, h, beta = pb.wavenumber, pb.water_depth, pb.encounter_wave_direction
+    wbar = x * np.cos(beta) + y * np.sin(beta)
+    if 0 <= k * h < 20:
+        cih = np.cosh(k * (z + h)) / np.cosh(k * h)
+        sih = np.sinh(k * (z + h)) / np.cosh(k * h)
+    else:
+        cih = np.exp(k * z)
+        sih = np.exp(k * z)
+    v = pb.g * k / pb.omega * np.exp(1j * k * wbar) * \
+        np.array([np.cos(pb.wave_direction) * cih, np.sin(pb.wave_direction) * cih, -1j * sih])
+    return v.T
+
+
+def airy_waves_potential(points, pb):
+    """airy_waves.py:19-53."""
+    x, y, z = np.asarray(points).T
+    k, h, beta = pb.wavenumber, pb.water_depth, pb.encounter_wave_direction
+    wbar = x * np.cos(beta) + y * np.sin(beta)
+    cih = np.cosh(k * (z + h)) / np.cosh(k * h) if 0 <= k * h < 20 else np.exp(k * z)
+    return -1j * pb.g / pb.omega * cih * np.exp(1j * k * wbar)
+
+
+def froude_krylov_force(pb):
+    """airy_waves.py:97-102."""
+    pressure = 1j * pb.omega * pb.rho * airy_waves_potential(pb.body.mesh.faces_centers, pb)
+    return pb.body.integrate_pressure(pressure)
+
+
+class RadiationProblem(LinearPotentialFlowProblem):
+    def __init__(self, *, body=None, radiating_dof=None, **kwargs):
+        super().__init__(body=body, **kwargs)
+        if body is None or len(body.dofs) == 0:
+            raise ValueError("A radiation problem needs a body with degrees of freedom")
+        self.radiating_dof = next(iter(body.dofs)) if radiating_dof is None else radiating_dof
+        if self.radiating_dof not in body.dofs:
+            raise ValueError(f"Unknown radiating dof {self.radiating_dof!r}")
+        displacement = np.sum(body.dof_motion(self.radiating_dof) * body.mesh.faces_normals, axis=1)
+        self.boundary_condition = -1j * self.encounter_omega * displacement.astype(np.complex128)
+
+    def make_results_container(self, forces=None, sources=None, potential=None, pressure=None):
+        return RadiationResult(self, forces, sources, potential, pressure)
+
+    def make_failed_results_container(self, exception):
+        r = RadiationResult(self, {d: np.nan for d in self.body.dofs})
+        r.exception = exception
+        return r
+
+    def __repr__(self):
+        return f"RadiationProblem(body={self.body.name}, omega={self.omega:.3f}, radiating_dof={self.radiating_dof})"
+
+
+class DiffractionProblem(LinearPotentialFlowProblem):
+    def __init__(self, *, body=None, **kwargs):
+        super().__init__(body=body, **kwargs)
+        self.boundary_condition = -(airy_waves_velocity(body.mesh.faces_centers, self)
+                                    * body.mesh.faces_normals).sum(axis=1)
+
+    def make_results_container(self, forces=None, sources=None, potential=None, pressure=None):
+        return DiffractionResult(self, forces, sources, potential, pressure)
+
+    def make_failed_results_container(self, exception):
+        r = DiffractionResult(self, {d: np.nan for d in self.body.dofs})
+        r.exception = exception
+        return r
+
+    def __repr__(self):
+        return f"DiffractionProblem(body={self.body.name}, omega={self.omega:.3f}, wave_direction={self.wave_direction:.3f})"
+
+
+class LinearPotentialFlowResult:
+    def __init__(self, problem, forces=None, sources=None, potential=None, pressure=None):
+        self.problem = problem
+        self.forces = forces if forces is not None else {}
+        self.sources = sources
+        self.potential = potential
+        self.pressure = pressure
+        self.omega = problem.omega
+        self.encounter_omega = problem.encounter_omega
+        self.wavenumber = problem.wavenumber
+        self.body = problem.body
+
+
+class RadiationResult(LinearPotentialFlowResult):
+    @property
+    def radiating_dof(self):
+        return self.problem.radiating_dof
+
+    @property
+    def added_mass(self):
+        w = self.encounter_omega
+        return {dof: float(np.real(f) / (w * w)) for dof, f in self.forces.items()}
+
+    @property
+    def radiation_damping(self):
+        w = self.encounter_omega
+        if float(w) in {0.0, np.inf}:
+            return {dof: 0.0 for dof in self.forces}
+        return {dof: float(np.imag(f) / w) for dof, f in self.forces.items()}
+
+
+class DiffractionResult(LinearPotentialFlowResult):
+    @property
+    def diffraction_force(self):
+        return dict(self.forces)
+
+    @property
+    def froude_krylov_force(self):
+        return froude_krylov_force(self.problem)
+
+    @property
+    def excitation_force(self):
+        fk = self.froude_krylov_force
+        return {dof: self.forces[dof] + fk[dof] for dof in self.forces}

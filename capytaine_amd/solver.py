@@ -1,0 +1,234 @@
+"""BEM solver on the GPU engine.
+
+Drop-in for ``capytaine.bem.solver.BEMSolver`` (src/capytaine/bem/solver.py:
+59-423) for the problems the hot path covers: ``solve`` follows ``_solve``
+(:145-215) step by step (build S and K through the engine, LU solve of the
+boundary condition, potential = S @ sources, pressure, forces), and
+``solve_all`` (:268-372) solves the problems of a group -- same body, depth,
+frequency, rho and g (problems_and_results.py:266-275) -- with one assembly,
+one factorisation and one multi-right-hand-side solve.  ``solve_batched``
+goes further: every frequency of a body in one assembly launch, one batched
+LU and one batched solve.
+"""
+import logging
+import time
+from collections import OrderedDict, defaultdict
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, ptr, stream_handle
+from .engines import DefaultMatrixEngine, DeviceMatrix, LUDecomposedDeviceMatrix, check_if_nan_in_matrix
+from .problems import DiffractionProblem, RadiationProblem
+
+LOG = logging.getLogger(__name__)
+
+
+class Timer:
+    """Accumulated wall time per step, with the reference's step names
+    (tools/timer.py, solver.py:176-198)."""
+
+    def __init__(self):
+        self.totals = defaultdict(float)
+        self.counts = defaultdict(int)
+
+    def __call__(self, step):
+        timer = self
+
+        class _Ctx:
+            def __enter__(self_):
+                self_.t = time.perf_counter()
+
+            def __exit__(self_, *exc):
+                timer.totals[step] += time.perf_counter() - self_.t
+                timer.counts[step] += 1
+        return _Ctx()
+
+    def summary(self):
+        return dict(self.totals)
+
+
+class BEMSolver:
+    def __init__(self, *, green_function=None, engine=None, method="indirect"):
+        if engine is None:
+            self.engine = DefaultMatrixEngine(green_function=green_function)
+        else:
+            if green_function is not None:
+                raise ValueError("If you are not using the default engine, set the Green function in the engine.")
+            self.engine = engine
+        if method.lower() not in {"direct", "indirect"}:
+            raise ValueError(f"Unrecognized method when initializing solver: {method!r}.")
+        self.method = method.lower()
+        self.timer = Timer()
+        self.exportable_settings = {**self.engine.exportable_settings, "method": self.method}
+
+    def __str__(self):
+        return f"BEMSolver(engine={self.engine}, method={self.method})"
+
+    __repr__ = __str__
+
+    # ------------------------------------------------------------------
+    def _solve(self, problem, method=None, keep_details=True):
+        if isinstance(problem, DiffractionProblem) and float(problem.encounter_omega) in {0.0, np.inf}:
+            raise ValueError("Diffraction problems at zero or infinite frequency are not defined")
+        omega, wavenumber = problem.omega, problem.wavenumber
+        gf_params = dict(free_surface=problem.free_surface, water_depth=problem.water_depth, wavenumber=wavenumber)
+        mesh = problem.body.mesh_including_lid
+        method = method if method is not None else self.method
+        if method == "direct":
+            with self.timer("Green function"):
+                S, D = self.engine.build_matrices(mesh, mesh, **gf_params, adjoint_double_layer=False,
+                                                  diagonal_term_in_double_layer=True)
+            with self.timer("Matrix-vector product"):
+                rhs = S @ problem.boundary_condition
+            with self.timer("Linear solver"):
+                potential = self.engine.linear_solver(D, rhs.astype(np.complex128))
+            sources = None
+        else:
+            with self.timer("Green function"):
+                S, K = self.engine.build_matrices(mesh, mesh, **gf_params, adjoint_double_layer=True,
+                                                  diagonal_term_in_double_layer=True)
+            with self.timer("Linear solver"):
+                sources = self.engine.linear_solver(K, problem.boundary_condition.astype(np.complex128))
+            with self.timer("Matrix-vector product"):
+                potential = S @ sources
+        pressure = 1j * omega * problem.rho * potential
+        forces = problem.body.integrate_pressure(pressure[problem.body.hull_mask])
+        if not keep_details:
+            return problem.make_results_container(forces)
+        return problem.make_results_container(forces, sources, potential, pressure)
+
+    def solve(self, problem, method=None, keep_details=True):
+        with self.timer("Total solve function"):
+            return self._solve(problem, method=method, keep_details=keep_details)
+
+    def _solve_and_catch_errors(self, problem, **kwargs):
+        try:
+            return self.solve(problem, **kwargs)
+        except Exception as e:  # solver.py:256-265
+            LOG.warning("Problem %s failed: %s", problem, e)
+            return problem.make_failed_results_container(e)
+
+    # ------------------------------------------------------------------
+    def solve_all(self, problems, *, method=None, keep_details=False, batch_frequencies=True,
+                  max_batch_bytes=None):
+        """Solve several problems.  Problems sharing a body, depth, rho, g and
+        free surface are solved together: with batch_frequencies, all their
+        frequencies at once (`solve_batched`); otherwise one frequency at a
+        time with all its right-hand sides at once.  Results come back in
+        the order of `problems`."""
+        method = method if method is not None else self.method
+        if method != "indirect":
+            return [self._solve_and_catch_errors(pb, method=method, keep_details=keep_details) for pb in problems]
+        results = [None] * len(problems)
+        groups = OrderedDict()
+        for i, pb in enumerate(problems):
+            groups.setdefault((id(pb.body), pb.water_depth, pb.rho, pb.g, pb.free_surface), []).append(i)
+        for key, idx in groups.items():
+            sub = [problems[i] for i in idx]
+            if batch_frequencies:
+                res = self.solve_batched(sub, keep_details=keep_details, max_batch_bytes=max_batch_bytes)
+            else:
+                res = []
+                by_omega = OrderedDict()
+                for j, pb in enumerate(sub):
+                    by_omega.setdefault(pb.omega, []).append(j)
+                res = [None] * len(sub)
+                for om, js in by_omega.items():
+                    out = self.solve_batched([sub[j] for j in js], keep_details=keep_details,
+                                             max_batch_bytes=max_batch_bytes)
+                    for j, r in zip(js, out):
+                        res[j] = r
+            for i, r in zip(idx, res):
+                results[i] = r
+        return results
+
+    def solve_batched(self, problems, *, keep_details=False, max_batch_bytes=None, return_device=False):
+        """All problems share body, depth, rho, g, free surface; frequencies may
+        differ.  Per chunk of frequencies: one assembly launch for all of
+        them, one batched LU of the K matrices (overwritten), one batched
+        solve for all right-hand sides, one batched S @ sources, and the
+        forces as a (dof x panels) product on the device."""
+        import torch
+        body = problems[0].body
+        mesh = body.mesh_including_lid
+        gf = self.engine.green_function
+        dev = gf.device
+        n = mesh.nb_faces
+        omegas = OrderedDict()
+        for j, pb in enumerate(problems):
+            if isinstance(pb, DiffractionProblem) and float(pb.omega) in {0.0, np.inf}:
+                raise ValueError("Diffraction problems at zero or infinite frequency are not defined")
+            omegas.setdefault(pb.omega, []).append(j)
+        omega_list = list(omegas)
+        if max_batch_bytes is None:
+            free, _ = torch.cuda.mem_get_info(dev)
+            max_batch_bytes = int(0.6 * free)
+        per_freq = 2 * n * n * 16
+        chunk = max(1, min(len(omega_list), max_batch_bytes // per_freq))
+        # W[i, d] = -(dof_d . n_i) A_i  (integrate_pressure, bodies.py:238-250)
+        dofs = list(body.dofs)
+        W = -(body.dof_normal_matrix() * mesh.faces_areas[None, :]).T  # (n, ndof)
+        W_d = torch.as_tensor(np.ascontiguousarray(W), dtype=torch.complex128).to(dev)
+        ndof = len(dofs)
+        results = [None] * len(problems)
+        ctx = _lib.Context.get(dev)
+        lib = _lib.load()
+        for c0 in range(0, len(omega_list), chunk):
+            oms = omega_list[c0:c0 + chunk]
+            ks = [problems[omegas[om][0]].wavenumber for om in oms]
+            pb0 = problems[omegas[oms[0]][0]]
+            with self.timer("Green function"):
+                S, K = gf.evaluate_on_device(mesh, mesh, free_surface=pb0.free_surface,
+                                             water_depth=pb0.water_depth, wavenumber=ks,
+                                             adjoint_double_layer=True, diagonal_term_in_double_layer=True)
+                if self.engine.check_nan:
+                    check_if_nan_in_matrix([DeviceMatrix(S), DeviceMatrix(K)])
+            nb = len(oms)
+            R = max(len(omegas[om]) for om in oms)
+            bc = np.zeros((nb, R, n), dtype=np.complex128)
+            for b, om in enumerate(oms):
+                for r, j in enumerate(omegas[om]):
+                    bc[b, r] = problems[j].boundary_condition
+            with self.timer("Linear solver"):
+                lu = LUDecomposedDeviceMatrix(DeviceMatrix(K), overwrite_a=True)
+                X = torch.as_tensor(bc).to(dev)
+                lu.solve_device(X)
+            with self.timer("Matrix-vector product"):
+                Phi = DeviceMatrix(S).matmul_device(X)
+                # G (ndof x R) = W^T (ndof x n) Phi (n x R), per frequency
+                G = torch.empty((nb, R, ndof), dtype=torch.complex128, device=dev)
+                check(lib.cpt_zgemm(ctx.handle, ndof, R, n, 1.0, ptr(W_d), ndof, 0, ptr(Phi), n, n * R, ptr(G),
+                                    ndof, ndof * R, 1, nb, stream_handle(dev)), "cpt_zgemm")
+            lu.check_info()
+            G_h = G.cpu().numpy()
+            if keep_details:
+                X_h, Phi_h = X.cpu().numpy(), Phi.cpu().numpy()
+            for b, om in enumerate(oms):
+                for r, j in enumerate(omegas[om]):
+                    pb = problems[j]
+                    f = 1j * pb.omega * pb.rho * G_h[b, r]
+                    forces = {d: f[q] for q, d in enumerate(dofs)}
+                    if keep_details:
+                        pot = Phi_h[b, r]
+                        results[j] = pb.make_results_container(forces, X_h[b, r], pot,
+                                                               1j * pb.omega * pb.rho * pot)
+                    else:
+                        results[j] = pb.make_results_container(forces)
+            del S, K, lu, X, Phi
+        return results
+
+
+def problems_for_body(body, omegas, *, headings=(), rho=1000.0, g=9.81, water_depth=np.inf, dofs=None):
+    """Radiation problems for every dof plus diffraction problems for every
+    heading (radians), for every frequency: the problem set of a
+    fill_dataset call over (omega, radiating_dof, wave_direction)."""
+    out = []
+    dofs = list(body.dofs) if dofs is None else list(dofs)
+    for om in omegas:
+        for d in dofs:
+            out.append(RadiationProblem(body=body, radiating_dof=d, omega=om, rho=rho, g=g, water_depth=water_depth))
+        for beta in headings:
+            out.append(DiffractionProblem(body=body, omega=om, wave_direction=beta, rho=rho, g=g,
+                                          water_depth=water_depth))
+    return out

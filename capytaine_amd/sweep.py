@@ -1,0 +1,102 @@
+"""Device-resident frequency sweep: the whole per-body BEM job on the GPU.
+
+For one body and a list of wavenumbers, with the boundary conditions of all
+problems already on the device, ``FrequencySweep.run`` does what capytaine's
+``BEMSolver.solve_all`` does for the same problems (solver.py:268-372 ->
+_solve :145-215 per problem), batched:
+
+  1. one assembly launch for every wavenumber (cpt_build_matrices),
+  2. one batched LU of the K matrices, in place (cpt_lu_factor),
+  3. one batched solve for all right-hand sides (cpt_lu_solve),
+  4. potentials Phi = S @ sources for all of them (cpt_zgemm),
+  5. generalised forces W^T Phi with W = -(dof . n) A (cpt_zgemm), to be
+     scaled by i omega rho per frequency (bodies.py:238-250).
+
+Buffers are allocated once and reused across calls.
+"""
+import numpy as np
+
+from . import _lib
+from ._lib import check, ptr, stream_handle
+
+
+class FrequencySweep:
+    def __init__(self, green_function, body, wavenumbers, nrhs, *, free_surface=0.0, water_depth=np.inf):
+        import torch
+        self.gf = green_function
+        self.body = body
+        self.mesh = body.mesh_including_lid
+        self.dev = green_function.device
+        self.ks = np.asarray(wavenumbers, dtype=np.float64)
+        self.nb = len(self.ks)
+        self.n = self.mesh.nb_faces
+        self.nrhs = nrhs
+        self.free_surface = free_surface
+        self.water_depth = water_depth
+        n, nb = self.n, self.nb
+        cplx = torch.complex128
+        self.S = torch.empty((nb, n, n), dtype=cplx, device=self.dev)
+        self.K = torch.empty((nb, n, n), dtype=cplx, device=self.dev)
+        self.X = torch.empty((nb, nrhs, n), dtype=cplx, device=self.dev)
+        self.Phi = torch.empty((nb, nrhs, n), dtype=cplx, device=self.dev)
+        self.ipiv = torch.empty((nb, n), dtype=torch.int32, device=self.dev)
+        self.info = torch.empty((nb,), dtype=torch.int32, device=self.dev)
+        dofs = list(body.dofs)
+        self.dofs = dofs
+        W = -(body.dof_normal_matrix() * self.mesh.faces_areas[None, :]).T  # (n, ndof)
+        self.W = torch.as_tensor(np.ascontiguousarray(W), dtype=cplx).to(self.dev)
+        self.G = torch.empty((nb, nrhs, len(dofs)), dtype=cplx, device=self.dev)
+        self.ctx = _lib.Context.get(self.dev)
+        self.stream = stream_handle(self.dev)
+        # warm the mesh and table uploads
+        self.mesh.device_arrays(self.dev)
+        self.gf._context()
+
+    # individual phases, so that callers can time them
+    def assemble(self):
+        self.gf.evaluate_on_device(self.mesh, self.mesh, free_surface=self.free_surface,
+                                   water_depth=self.water_depth, wavenumber=self.ks, adjoint_double_layer=True,
+                                   diagonal_term_in_double_layer=True, out=(self.S, self.K))
+
+    def factor(self):
+        n = self.n
+        check(_lib.load().cpt_lu_factor(self.ctx.handle, ptr(self.K), n, n, n * n, ptr(self.ipiv), n,
+                                        ptr(self.info), self.nb, self.stream), "cpt_lu_factor")
+
+    def solve(self, bc):
+        """bc: device tensor (nb, nrhs, n) of boundary conditions (not modified)."""
+        n = self.n
+        self.X.copy_(bc)
+        lib = _lib.load()
+        check(lib.cpt_lu_solve(self.ctx.handle, ptr(self.K), n, n, n * n, ptr(self.ipiv), n, ptr(self.X),
+                               self.nrhs, n, n * self.nrhs, self.nb, self.stream), "cpt_lu_solve")
+
+    def post(self):
+        n, r, nd = self.n, self.nrhs, len(self.dofs)
+        lib = _lib.load()
+        check(lib.cpt_zgemm(self.ctx.handle, n, r, n, 1.0, ptr(self.S), n, n * n, ptr(self.X), n, n * r,
+                            ptr(self.Phi), n, n * r, 1, self.nb, self.stream), "cpt_zgemm S@sources")
+        check(lib.cpt_zgemm(self.ctx.handle, nd, r, n, 1.0, ptr(self.W), nd, 0, ptr(self.Phi), n, n * r,
+                            ptr(self.G), nd, nd * r, 1, self.nb, self.stream), "cpt_zgemm forces")
+
+    def run(self, bc):
+        self.assemble()
+        self.factor()
+        self.solve(bc)
+        self.post()
+        return self.G
+
+    def forces(self, omegas, rho):
+        """Host copy of the forces (nb, nrhs, ndof), scaled by i omega rho."""
+        G = self.G.cpu().numpy()
+        return 1j * np.asarray(omegas)[:, None, None] * rho * G
+
+
+def boundary_conditions(problems_by_freq, nrhs, n, device):
+    """(nb, nrhs, n) device tensor from lists of problems per frequency."""
+    import torch
+    bc = np.zeros((len(problems_by_freq), nrhs, n), dtype=np.complex128)
+    for b, pbs in enumerate(problems_by_freq):
+        for r, pb in enumerate(pbs):
+            bc[b, r] = pb.boundary_condition
+    return torch.as_tensor(bc).to(device)

@@ -1,0 +1,111 @@
+"""End-to-end parity of the device solver: added mass, radiation damping,
+diffraction and excitation forces against the oracle pipeline (oracle S, K
+with the same table + scipy LU, as BEMSolver._solve does) and against the
+Nemoh-2 known answers of the reference test-suite.
+
+Tolerance: 1e-10 relative on every coefficient (FP64 path)."""
+import numpy as np
+import pytest
+
+from gpu_helpers import device, ref_table_gf
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return abs(a - b) / max(abs(b), 1e-300)
+
+
+def test_sphere_heave_vs_oracle(default_table, orc):
+    """Sphere R = 1, 450 panels, heave radiation at omega = 1, rho = 1000."""
+    import capytaine_amd as cpa
+    from oracle import mesh_arrays, solve_coefficients
+    print("[gpu] sphere heave end to end", flush=True)
+    mesh = cpa.mesh_sphere(radius=1.0, resolution=(30, 30)).immersed_part()
+    body = cpa.FloatingBody(mesh, cpa.rigid_body_dofs(only=["Heave", "Surge", "Pitch"]))
+    solver = cpa.BEMSolver(engine=cpa.DefaultMatrixEngine(green_function=ref_table_gf(default_table)))
+    pb = cpa.RadiationProblem(body=body, radiating_dof="Heave", omega=1.0, rho=1000.0)
+    res = solver.solve(pb)
+    S, K = orc.build_matrices(**mesh_arrays(mesh), wavenumber=pb.wavenumber, depth=np.inf, table=default_table)
+    (f_ref,) = solve_coefficients(S, K, body, [pb])
+    for dof in body.dofs:
+        assert _rel(res.forces[dof], f_ref[dof]) < 1e-10 or abs(res.forces[dof] - f_ref[dof]) < 1e-10 * abs(
+            f_ref["Heave"]), dof
+    am, bd = res.added_mass["Heave"], res.radiation_damping["Heave"]
+    assert _rel(am, np.real(f_ref["Heave"])) < 1e-10 and _rel(bd, np.imag(f_ref["Heave"])) < 1e-10
+    # physical sanity: added mass of a heaving hemisphere ~ 0.5-1 x displaced mass
+    assert 0.3 * 1000 * 2 / 3 * np.pi < am < 1.5 * 1000 * 2 / 3 * np.pi
+
+
+def test_batched_solve_matches_single(default_table, orc):
+    """solve_all over frequencies (one assembly launch, batched LU and solve)
+    gives the per-problem solve results and the oracle's."""
+    import capytaine_amd as cpa
+    from oracle import mesh_arrays, solve_coefficients
+    print("[gpu] batched solve", flush=True)
+    mesh = cpa.mesh_parallelepiped(size=(4.0, 2.0, 1.0), center=(0, 0, -0.5), resolution=(16, 8, 4),
+                                   missing_sides={"top"})
+    body = cpa.FloatingBody(mesh).add_all_rigid_body_dofs(rotation_center=(0, 0, -0.2))
+    gf = ref_table_gf(default_table)
+    solver = cpa.BEMSolver(engine=cpa.DefaultMatrixEngine(green_function=gf))
+    omegas = [0.5, 1.3, 2.2]
+    pbs = cpa.problems_for_body(body, omegas, headings=[0.0, np.pi / 4], rho=1025.0)
+    batched = solver.solve_all(pbs)
+    for om in omegas:
+        group = [p for p in pbs if p.omega == om]
+        S, K = orc.build_matrices(**mesh_arrays(mesh), wavenumber=group[0].wavenumber, depth=np.inf,
+                                  table=default_table)
+        refs = solve_coefficients(S, K, body, group)
+        for pb, ref in zip(group, refs):
+            res = batched[pbs.index(pb)]
+            scale = max(abs(v) for v in ref.values())
+            for dof in body.dofs:
+                assert abs(res.forces[dof] - ref[dof]) < 1e-10 * scale, (pb, dof)
+    single = solver.solve(pbs[1])
+    for dof in body.dofs:
+        assert abs(single.forces[dof] - batched[1].forces[dof]) < 1e-12 * max(abs(v) for v in single.forces.values())
+
+
+def test_nemoh2_floating_sphere(legacy_table):
+    """pytest/test_consistency_with_Nemoh_2.py:102-153: legacy grid (328 x 46,
+    251 points), high_freq singularities, heaving sphere at omega = 1 and 2;
+    tolerance 1e-3 rho V for the coefficients, 1e-3 relative for the
+    diffraction force (as in the reference test)."""
+    import capytaine_amd as cpa
+    from numpy.testing import assert_allclose
+    print("[gpu] Nemoh-2 floating sphere", flush=True)
+    gf = cpa.Delhommeau(tabulation_nr=328, tabulation_nz=46, tabulation_grid_shape="legacy",
+                        tabulation_nb_integration_points=251, finite_depth_method="legacy",
+                        gf_singularities="high_freq", tabulation_cache_dir=None, device=device())
+    gf._set_tabulation(legacy_table[0], legacy_table[1], np.asfortranarray(legacy_table[2]))
+    solver = cpa.BEMSolver(green_function=gf)
+    mesh = cpa.mesh_sphere(radius=1.0, resolution=(6, 12)).immersed_part()
+    body = cpa.FloatingBody(mesh, {"Heave": cpa.TranslationDof((0, 0, 1))})
+    atol = 1e-3 * 1000 * (2 / 3 * np.pi)
+    from conftest import load_golden
+    ka = load_golden("nemoh2_known_answers.npz")
+    for om, am, bd, fd in [(1.0, ka["floating_sphere_w1_added_mass"], ka["floating_sphere_w1_damping"],
+                            ka["floating_sphere_w1_diffraction"]),
+                           (2.0, ka["floating_sphere_w2_added_mass"], ka["floating_sphere_w2_damping"],
+                            ka["floating_sphere_w2_diffraction"])]:
+        res = solver.solve(cpa.RadiationProblem(body=body, omega=om))
+        assert abs(res.added_mass["Heave"] - float(am)) < atol
+        assert abs(res.radiation_damping["Heave"] - float(bd)) < atol
+        dres = solver.solve(cpa.DiffractionProblem(body=body, omega=om))
+        assert_allclose(dres.forces["Heave"], complex(fd), rtol=1e-3)
+
+
+def test_immersed_sphere_rankine_only():
+    """test_consistency_with_Nemoh_2.py:36-58: sphere in infinite fluid,
+    free_surface = inf (pure Rankine), heave and surge added masses."""
+    import capytaine_amd as cpa
+    from conftest import load_golden
+    ka = load_golden("nemoh2_known_answers.npz")
+    solver = cpa.BEMSolver(green_function=cpa.Delhommeau(tabulation_cache_dir=None, device=device()))
+    body = cpa.FloatingBody(cpa.mesh_sphere(radius=1.0, resolution=(10, 40)),
+                            {"Surge": cpa.TranslationDof((1, 0, 0)), "Heave": cpa.TranslationDof((0, 0, 1))})
+    atol = 1e-3 * 1000 * (4 / 3 * np.pi)
+    for dof, key in [("Heave", "immersed_sphere_heave_added_mass"), ("Surge", "immersed_sphere_surge_added_mass")]:
+        pb = cpa.RadiationProblem(body=body, radiating_dof=dof, free_surface=np.inf, water_depth=np.inf)
+        res = solver.solve(pb)
+        assert abs(res.added_mass[dof] - float(ka[key])) < atol, (dof, res.added_mass[dof])

@@ -1,0 +1,230 @@
+"""Pins the CPU oracle (oracle/delhommeau_oracle.c): against the golden
+vectors written by the reference's own Fortran core (tests/golden/
+make_golden.py), against the known answers of the reference test-suite, and
+against what the mathematics fixes (closed forms, brute-force quadrature,
+exactness of the Lagrange interpolation, index inversion)."""
+import ctypes
+
+import numpy as np
+import pytest
+from conftest import elem_ok, rel_err
+
+
+def test_grids_bit_exact(orc, core_golden):
+    for name, (nr, rmax, nz, zmin, method) in {"scaled": (676, 100.0, 372, -251.0, 1),
+                                               "legacy": (328, 100.0, 46, -16.0, 0),
+                                               "scaled_small": (100, 50.0, 100, -50.0, 1)}.items():
+        assert np.array_equal(orc.r_spacing(nr, rmax, method), core_golden[f"grid_{name}_r"])
+        assert np.array_equal(orc.z_spacing(nz, zmin, method), core_golden[f"grid_{name}_z"])
+
+
+def test_tables_bit_exact(default_table, legacy_table, core_golden):
+    # the fixtures assert the SHA-256 of the whole table; check the sample too
+    idx = core_golden["table_scaled_sample_idx"]
+    tab = default_table[2]
+    assert np.array_equal(tab[idx[:, 0], idx[:, 1], idx[:, 2]], core_golden["table_scaled_sample_val"])
+
+
+def test_numerical_integration_and_asymptotics(orc, core_golden):
+    for (r, z), ref, aref in zip(core_golden["numint_rz"], core_golden["numint_val"], core_golden["asym_val"]):
+        assert np.array_equal(orc.numerical_integration(r, z, 1001), ref)
+        np.testing.assert_allclose(orc.asymptotic(r, z), aref, rtol=1e-14, atol=1e-300)
+
+
+@pytest.mark.parametrize("gfs", [0, 1, 2])
+@pytest.mark.parametrize("k", [0.3, 2.0])
+def test_wave_part_golden(orc, core_golden, default_table, gfs, k):
+    g, ng = orc.wave_part(core_golden["wave_x0i"], core_golden["wave_x0j"], k, default_table, gfs=gfs)
+    gr, ngr = core_golden[f"wave_g_gfs{gfs}_k{k}"], core_golden[f"wave_ng_gfs{gfs}_k{k}"]
+    assert rel_err(g, gr) < 1e-14 and rel_err(ng, ngr) < 1e-14
+    assert elem_ok(g, gr, 1e-12, 1e-12) and elem_ok(ng, ngr, 1e-12, 1e-12)
+
+
+@pytest.mark.parametrize("kind", range(6))
+@pytest.mark.parametrize("deriv", [True, False])
+def test_rankine_golden(orc, core_golden, kind, deriv):
+    cg = core_golden
+    coefs = (-1.0, 0.0) if kind != 5 else (1.0, -4.0)
+    g, ng = orc.rankine(kind, cg["rk_M"], cg["rk_nodes"], cg["rk_centers"], cg["rk_normals"], cg["rk_areas"],
+                        cg["rk_radii"], deriv_first=deriv, coefs=coefs)
+    assert rel_err(g, cg[f"rk_g_{kind}_{int(deriv)}"]) < 1e-14
+    assert rel_err(ng, cg[f"rk_ng_{kind}_{int(deriv)}"]) < 1e-13
+
+
+def _case(bm, name):
+    keys = ["colloc", "dpn", "vertices", "faces", "centers", "normals", "areas", "radii", "quad_points",
+            "quad_weights"]
+    arr = {k: bm[f"{name}__{k}"] for k in keys}
+    opts = {k[len(name) + 6:]: bm[k] for k in bm.files if k.startswith(f"{name}__opt_")}
+    depth = float(opts.pop("depth"))
+    opts = {k: (v if v.ndim else v.item()) for k, v in opts.items()}
+    return arr, depth, opts
+
+
+BM_CASES = ["sphere_k1", "sphere_k01_hf", "sphere_k3_rp", "sphere_k1_direct", "sphere_k1_full", "sphere_k0",
+            "sphere_kinf", "barge_k06", "barge_lid_k06", "barge_fd_newer", "barge_fd_legacy", "sphere_gl2_k1"]
+
+
+@pytest.mark.parametrize("name", BM_CASES)
+def test_build_matrices_golden(orc, bm_golden, default_table, name):
+    arr, depth, opts = _case(bm_golden, name)
+    S, K = orc.build_matrices(**arr, depth=depth, table=default_table, **opts)
+    assert rel_err(S, bm_golden[f"{name}__S"]) < 1e-14
+    assert rel_err(K, bm_golden[f"{name}__K"]) < 1e-14
+    assert elem_ok(S, bm_golden[f"{name}__S"], 1e-10, 1e-10)
+
+
+def test_nemoh2_rankine_matrices(orc, nemoh_golden):
+    """pytest/test_consistency_with_Nemoh_2.py:61-99 (k = 0 and k = inf)."""
+    from capytaine_amd.meshes import Mesh
+    from oracle import mesh_arrays
+    mesh = Mesh(nemoh_golden["vertices"], nemoh_golden["faces"])
+    arr = mesh_arrays(mesh)
+    dummy = (np.zeros(3), np.zeros(3), np.zeros((3, 3, 5)))
+    S0, _ = orc.build_matrices(**arr, wavenumber=0.0, depth=np.inf, table=dummy, gf_singularities=1)
+    Sinf, _ = orc.build_matrices(**arr, wavenumber=np.inf, depth=np.inf, table=dummy, gf_singularities=0)
+    assert np.allclose(S0, nemoh_golden["S_k0"])
+    assert np.allclose(Sinf, nemoh_golden["S_kinf"])
+    # a plausible mistake (reflected term with the wrong sign) fails
+    assert not np.allclose(S0, nemoh_golden["S_kinf"])
+
+
+def test_wave_part_closed_forms(orc, default_table):
+    """Im G+ = 2 pi k e^z J0(r) and Im dG/dr = -2 pi k e^z J1(r)
+    (pytest/test_green_functions_fortran_core.py:37-53)."""
+    from scipy.special import j0, j1
+    rng = np.random.default_rng(1)
+    xi = rng.uniform(-2, 2, (300, 3))
+    xj = rng.uniform(-2, 2, (300, 3))
+    xi[:, 2] = -rng.uniform(0.05, 2, 300)
+    xj[:, 2] = -rng.uniform(0.05, 2, 300)
+    for k in (0.5, 2.0):
+        g, ng = orc.wave_part(xi, xj, k, default_table, gfs=1)
+        r = k * np.linalg.norm(xi[:, :2] - xj[:, :2], axis=1)
+        z = k * (xi[:, 2] + xj[:, 2])
+        # tolerance: quadratic interpolation error of the 676 x 372 table
+        np.testing.assert_allclose(g.imag, 2 * np.pi * k * np.exp(z) * j0(r), rtol=0, atol=2e-3 * k)
+        drdx = k * k * (xi[:, 0] - xj[:, 0]) / r
+        np.testing.assert_allclose(ng[:, 0].imag, k * drdx * (-2 * np.pi * np.exp(z) * j1(r)), rtol=0, atol=2e-3 * k ** 3)
+        # G- = G+ + 2k/r1 and both share the imaginary part
+        g_hf, _ = orc.wave_part(xi, xj, k, default_table, gfs=0)
+        # (I1 and I2 are interpolated separately: compare away from r1 = 0)
+        far = np.hypot(r, z) > 0.5
+        np.testing.assert_allclose((g.real + 2 * k / np.hypot(r, z))[far], g_hf.real[far], rtol=0, atol=2e-2 * k)
+        np.testing.assert_allclose(g_hf.imag, g.imag, rtol=1e-12)
+
+
+@pytest.mark.parametrize("gfs", [0, 1])
+def test_wave_part_gradient_and_symmetry(orc, default_table, gfs):
+    """Finite differences and x <-> xi symmetry
+    (test_green_functions_fortran_core.py:56-79, :101-114)."""
+    xi = np.array([[0.0, 0.0, -0.5]])
+    xj = np.array([[1.0, 1.0, -0.5]])
+    g, ng = orc.wave_part(xi, xj, 1.0, default_table, gfs=gfs)
+    eps = 1e-5
+    for c in range(3):
+        d = np.zeros((1, 3))
+        d[0, c] = eps
+        gp, _ = orc.wave_part(xi + d, xj, 1.0, default_table, gfs=gfs)
+        gm, _ = orc.wave_part(xi - d, xj, 1.0, default_table, gfs=gfs)
+        assert abs((gp - gm)[0] / (2 * eps) - ng[0, c]) < 2e-3 * max(1.0, abs(ng[0, c]))
+    g2, ng2 = orc.wave_part(xj, xi, 1.0, default_table, gfs=gfs)
+    assert abs(g2[0] - g[0]) < 1e-12
+    np.testing.assert_allclose(ng2[0, :2], -ng[0, :2], rtol=1e-12)
+    np.testing.assert_allclose(ng2[0, 2], ng[0, 2], rtol=1e-12)
+
+
+def test_spot_value(orc, default_table):
+    """test_green_functions_fortran_core.py:82-92."""
+    g, ng = orc.wave_part(np.array([[0.0, 0.0, -0.5]]), np.array([[1.0, 1.0, -0.5]]), 1.0, default_table, gfs=1)
+    assert abs(g[0] - (-2.04 + 1.29j)) < 0.1
+    np.testing.assert_allclose(ng[0], [0.25 + 0.88j, 0.25 + 0.88j, -0.88 + 1.29j], atol=0.1)
+
+
+def _brute_force_panel_integral(M, nodes, n_sub=200):
+    """Gauss-Legendre over the bilinear map of a planar quad: integral of
+    1/|M - x| and of its gradient with respect to M."""
+    t, w = np.polynomial.legendre.leggauss(n_sub)
+    u, v = np.meshgrid(t, t, indexing="ij")
+    W = np.outer(w, w)
+    a, b, c, d = nodes
+    P = ((1 - u)[..., None] * (1 - v)[..., None] * a + (1 + u)[..., None] * (1 - v)[..., None] * b
+         + (1 + u)[..., None] * (1 + v)[..., None] * c + (1 - u)[..., None] * (1 + v)[..., None] * d) / 4
+    dPu = (-(1 - v)[..., None] * a + (1 - v)[..., None] * b + (1 + v)[..., None] * c - (1 + v)[..., None] * d) / 4
+    dPv = (-(1 - u)[..., None] * a - (1 + u)[..., None] * b + (1 + u)[..., None] * c + (1 - u)[..., None] * d) / 4
+    J = np.linalg.norm(np.cross(dPu, dPv), axis=-1)
+    diff = M - P
+    r = np.linalg.norm(diff, axis=-1)
+    G = np.sum(W * J / r)
+    grad = -np.sum((W * J / r ** 3)[..., None] * diff, axis=(0, 1))
+    return G, grad
+
+
+def test_exact_rankine_vs_brute_force(orc):
+    rng = np.random.default_rng(5)
+    nodes = np.array([[0, 0, 0], [1.0, 0, 0], [1.2, 0.9, 0], [-0.1, 1.1, 0]])
+    a, b, c, d = nodes
+    n = np.cross(c - a, d - b)
+    n /= np.linalg.norm(n)
+    a1 = 0.5 * np.linalg.norm(np.cross(b - a, c - a))
+    a2 = 0.5 * np.linalg.norm(np.cross(c - a, d - a))
+    center = ((a + b + c) / 3 * a1 + (a + c + d) / 3 * a2) / (a1 + a2)
+    radius = np.linalg.norm(center - a)
+    for _ in range(10):
+        M = center + rng.normal(size=3) * 0.8
+        M[2] = rng.choice([-1, 1]) * rng.uniform(0.3, 1.5)
+        g, ng = orc.rankine(1, M[None], nodes[None], center[None], n[None], np.array([a1 + a2]),
+                            np.array([radius]))
+        G, grad = _brute_force_panel_integral(M, nodes)
+        assert abs(g[0] - G) < 1e-8 * abs(G)
+        np.testing.assert_allclose(ng[0], grad, rtol=1e-6, atol=1e-8 * np.linalg.norm(grad))
+        # one-point formula is A / r
+        g1, ng1 = orc.rankine(2, M[None], nodes[None], center[None], n[None], np.array([a1 + a2]),
+                              np.array([radius]))
+        ro = np.linalg.norm(center - M)
+        assert abs(g1[0] - (a1 + a2) / ro) < 1e-15 * g1[0]
+
+
+def test_lagrange_interpolation_exact_on_biquadratics(orc, default_table):
+    """pick_in_default_tabulation reproduces any biquadratic f(r) g(z) table
+    exactly: fails on a transposed stencil or a shifted index."""
+    r_range, z_range, _ = default_table
+    R, Z = np.meshgrid(r_range, z_range, indexing="ij")
+    coef = np.array([[0.3, -1.2, 0.7], [2.0, 0.5, -0.1], [-0.4, 0.25, 1.5], [1.0, 1.0, 1.0], [0.0, -2.0, 0.5]])
+    tab = np.stack([(c[0] + c[1] * R + c[2] * R * R) * (1 - 0.3 * Z + 0.01 * Z * Z) for c in coef], axis=2)
+    tab = np.asfortranarray(tab)
+    fn = orc.lib.orc_pick_in_default_tabulation
+    out = np.zeros(5)
+    rng = np.random.default_rng(2)
+    for _ in range(400):
+        r = 10 ** rng.uniform(-9, 1.9)
+        z = -10 ** rng.uniform(-9, 2.3)
+        fn(ctypes.c_double(r), ctypes.c_double(z), 1, r_range.ctypes.data_as(ctypes.c_void_p), len(r_range),
+           z_range.ctypes.data_as(ctypes.c_void_p), len(z_range), tab.ctypes.data_as(ctypes.c_void_p),
+           out.ctypes.data_as(ctypes.c_void_p))
+        expect = np.array([(c[0] + c[1] * r + c[2] * r * r) * (1 - 0.3 * z + 0.01 * z * z) for c in coef])
+        np.testing.assert_allclose(out, expect, rtol=1e-7, atol=1e-9 * np.abs(expect).max())
+
+
+def test_nearest_index_inversion(orc, default_table):
+    """test_green_functions_table_density.py:125-153: grid points map back to
+    their own index."""
+    r_range, z_range, _ = default_table
+    fr, fz = orc.lib.orc_nearest_r_index, orc.lib.orc_nearest_z_index
+    fr.argtypes = fz.argtypes = [ctypes.c_double, ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+    for i in range(1, len(r_range)):
+        assert fr(r_range[i], r_range.ctypes.data, len(r_range), 1) == i + 1
+    for j in range(len(z_range) - 1):
+        assert fz(z_range[j], z_range.ctypes.data, len(z_range), 1) == j + 1
+
+
+def test_early_dot_product_and_diagonal_structure(orc, bm_golden, default_table):
+    """K = n_I . (full gradient) and the 1/2 diagonal term
+    (test_green_functions_matrix_shape.py:29-97)."""
+    arr, depth, opts = _case(bm_golden, "sphere_k1")
+    S, K = orc.build_matrices(**arr, depth=depth, table=default_table, wavenumber=1.0)
+    _, Kf = orc.build_matrices(**arr, depth=depth, table=default_table, wavenumber=1.0, early_dot_product=False)
+    K2 = np.einsum("cij,ic->ij", Kf, arr["dpn"])
+    np.testing.assert_allclose(K2, K, rtol=1e-12, atol=1e-14)
+    _, Kn = orc.build_matrices(**arr, depth=depth, table=default_table, wavenumber=1.0, diagonal_term=False)
+    np.testing.assert_allclose(np.diag(K - Kn), 0.5, rtol=1e-13)
