@@ -35,7 +35,7 @@ SIGNATURES = {
 }
 
 _lib = None
-_lock = threading.Lock()
+_lock = threading.RLock()  # Context.get -> Context.__init__ -> load() re-enters it
 
 
 class CptError(RuntimeError):

@@ -333,7 +333,7 @@ int cpt_build_matrices(cpt_handle h, int nb_collocation_points, const double* co
         const double k = wavenumbers[b];
         fa[b].k = k;
         fa[b].gfs = gf_singularities[b];
-        fa[b].wave = (0.0 < k && std::isfinite(k)) ? 1 : 0;
+        fa[b].wave = (!rankine_only && 0.0 < k && std::isfinite(k)) ? 1 : 0;
         fa[b].nexp = ne;
         fa[b].pad = 0;
         fa[b].prony_lambda = prony_dev + (size_t)b * 2 * ne;

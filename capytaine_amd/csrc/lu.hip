@@ -392,7 +392,7 @@ __global__ void __launch_bounds__(256) zgemm_kernel(int M, int N, int K, double 
         for (int b = 0; b < 2; ++b)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int row = row0 + 16 * a + 4 * lk + v;
+                const int row = row0 + 16 * a + 4 * v + lk;
                 const int col = col0 + 16 * b + li;
                 if (row < M && col < N) {
                     cplx* c = C + row + (size_t)col * ldc;

@@ -45,6 +45,7 @@ def default_table(orc, core_golden):
     """The default (scaled_nemoh3, 676 x 372, 1001 points) table from the
     oracle, checked bit for bit against the reference table checksum."""
     import hashlib
+    print("[fixture] oracle default table (676 x 372 x 1001 points)", flush=True)
     r = orc.r_spacing(676, 100.0, 1)
     z = orc.z_spacing(372, -251.0, 1)
     tab = orc.tabulation(r, z, 1001)
@@ -56,6 +57,7 @@ def default_table(orc, core_golden):
 @pytest.fixture(scope="session")
 def legacy_table(orc, core_golden):
     import hashlib
+    print("[fixture] oracle legacy table", flush=True)
     r = orc.r_spacing(328, 100.0, 0)
     z = orc.z_spacing(46, -16.0, 0)
     tab = orc.tabulation(r, z, 251)

@@ -105,3 +105,23 @@ def test_no_gpu_fails_loudly():
     import capytaine_amd as cpa
     with pytest.raises(cpa.CptError):
         cpa.Delhommeau()
+
+
+def test_context_creation_does_not_deadlock():
+    """Context.get takes the module lock and Context.__init__ loads the
+    library under it: the lock must be re-entrant.  Without a GPU the
+    creation raises instead of blocking."""
+    import threading
+    from capytaine_amd import _lib
+    outcome = []
+
+    def target():
+        try:
+            _lib.Context.get("cuda:0")
+            outcome.append("created")
+        except Exception as e:
+            outcome.append(type(e).__name__)
+    t = threading.Thread(target=target, daemon=True)
+    t.start()
+    t.join(30)
+    assert outcome, "Context.get blocked"
