@@ -5,8 +5,14 @@ Workload (one step): rectangular barge 20 m x 8 m, draft 2 m, 0.25 m panels
 in 0.2-2.5 rad/s; 6 rigid-body radiation problems + diffraction at 8
 headings (0-315 deg) per frequency: 560 problems.  The step runs, on the
 device with the boundary conditions already in HBM: assembly of S and K for
-the 40 wavenumbers (one launch), 40 complex128 LU factorisations (batched),
-the 560 solves, the potentials S @ sources and the generalised forces.
+the 40 wavenumbers (one launch), the 40 factorisations (batched), the 560
+solves, the potentials S @ sources and the generalised forces.
+
+Two forms are timed in the same run: the plain FP64 form (complex128 LU on
+FP64 MFMA) and the mixed-precision batched form (complex64 LU on FP32 MFMA,
+refined with FP64 residuals against the FP64 matrix).  `value` is the mixed
+form's throughput; the FP64 throughput, the ratio and the largest relative
+difference of any coefficient between the two forms are reported beside it.
 
 Prints one JSON line (see the repository's bench contract).
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
@@ -24,6 +30,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "panel-pairs/s (S,K assembly) + BEM solves/s, mixed vs FP64 at 1e-6; p99 latency"
 MI355X_FP64_MFMA_TFLOPS = 78.6
+MI355X_FP32_MFMA_TFLOPS = 157.3
 MI355X_HBM_GBS = 8000.0
 
 
@@ -47,7 +54,7 @@ def problems_by_frequency(body, omegas, headings, rho, g):
     return out
 
 
-def cpu_baseline(mesh, body, pbs_by_freq, table, n_freq=2):
+def cpu_baseline(mesh, body, pbs_by_freq, table, n_freq=6):
     """The oracle (C restatement of the reference core, OpenMP on all host
     cores) + scipy LU, on a bounded sample: the first and last frequencies
     of the sweep, every problem of those frequencies."""
@@ -55,7 +62,7 @@ def cpu_baseline(mesh, body, pbs_by_freq, table, n_freq=2):
     from oracle import Oracle, mesh_arrays, solve_coefficients
     orc = Oracle()
     arr = mesh_arrays(mesh)
-    idx = [0, len(pbs_by_freq) - 1][:n_freq]
+    idx = list(np.linspace(0, len(pbs_by_freq) - 1, n_freq).round().astype(int))
     t0 = time.perf_counter()
     nprob = 0
     for i in idx:
@@ -85,6 +92,7 @@ def main():
     import torch
     import capytaine_amd as cpa
     from capytaine_amd.sweep import FrequencySweep, boundary_conditions
+    from capytaine_amd import _lib
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
@@ -95,70 +103,124 @@ def main():
     nrhs = len(pbs_by_freq[0])
     ks = [p[0].wavenumber for p in pbs_by_freq]
     gf = cpa.Delhommeau(device=dev)
-    sweep = FrequencySweep(gf, body, ks, nrhs)
     bc = boundary_conditions(pbs_by_freq, nrhs, n, dev)
     stream = torch.cuda.current_stream(dev)
+    omegas_arr = np.asarray(omegas)
 
-    for _ in range(args.warmup):
-        sweep.run(bc)
-    torch.cuda.synchronize()
+    def run(precision):
+        sweep = FrequencySweep(gf, body, ks, nrhs, precision=precision, max_iter=10, tol=1e-12)
+        for _ in range(args.warmup):
+            sweep.run(bc)
+        torch.cuda.synchronize()
+        phases = {"assembly": 0.0, "lu_factor": 0.0, "lu_solve": 0.0, "post": 0.0}
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(5 * args.steps)]
+        torch.cuda.synchronize()
+        _lib.profile_begin(sweep.ctx)
+        t0 = time.perf_counter()
+        for s in range(args.steps):
+            e = ev[5 * s:5 * s + 5]
+            e[0].record(stream)
+            sweep.assemble()
+            e[1].record(stream)
+            sweep.factor()
+            e[2].record(stream)
+            sweep.solve(bc)
+            e[3].record(stream)
+            sweep.post()
+            e[4].record(stream)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        kernels = _lib.profile_end(sweep.ctx)
+        for s in range(args.steps):
+            e = ev[5 * s:5 * s + 5]
+            for name, (a, b) in zip(phases, [(0, 1), (1, 2), (2, 3), (3, 4)]):
+                phases[name] += e[a].elapsed_time(e[b]) / 1e3
+        forces = sweep.forces(omegas_arr, rho)
+        out = dict(seconds=t1 - t0, phases={k: v / args.steps for k, v in phases.items()}, forces=forces,
+                   kernels=kernels,
+                   iterations=sweep.iterations, residual=sweep.residual,
+                   info_nonzero=int(np.count_nonzero(sweep.info.cpu().numpy())) if precision == "fp64" else 0)
+        del sweep
+        torch.cuda.empty_cache()
+        return out
 
-    phases = {"assembly": 0.0, "lu_factor": 0.0, "lu_solve": 0.0, "post": 0.0}
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(5 * args.steps)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for s in range(args.steps):
-        e = ev[5 * s:5 * s + 5]
-        e[0].record(stream)
-        sweep.assemble()
-        e[1].record(stream)
-        sweep.factor()
-        e[2].record(stream)
-        sweep.solve(bc)
-        e[3].record(stream)
-        sweep.post()
-        e[4].record(stream)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    for s in range(args.steps):
-        e = ev[5 * s:5 * s + 5]
-        for name, (a, b) in zip(phases, [(0, 1), (1, 2), (2, 3), (3, 4)]):
-            phases[name] += e[a].elapsed_time(e[b]) / 1e3
-    info = sweep.info.cpu().numpy()
-    ms_per_step = (t1 - t0) / args.steps * 1e3
-    nprob = len(ks) * nrhs
-    value = nprob * args.steps / (t1 - t0)
+    r64 = run("fp64")
+    rmx = run("mixed")
     nb = len(ks)
-    t_asm = phases["assembly"] / args.steps
-    t_lu = phases["lu_factor"] / args.steps
+    nprob = nb * nrhs
+    value64 = nprob * args.steps / r64["seconds"]
+    valuemx = nprob * args.steps / rmx["seconds"]
+    # per-coefficient agreement of the mixed form with the FP64 form
+    f64, fmx = r64["forces"], rmx["forces"]
+    scale = np.abs(f64).max(axis=2, keepdims=True)  # per problem
+    mixed_err = float(np.max(np.abs(fmx - f64) / scale))
+    t_asm = r64["phases"]["assembly"]
+    t_lu = r64["phases"]["lu_factor"]
     pairs_per_s = nb * n * n / t_asm
-    lu_flops = nb * 8.0 * n ** 3 / 3.0
-    lu_tflops = lu_flops / t_lu / 1e12
-    dominant = max(phases, key=phases.get)
-    roofline = {"bound": "mfma", "kernel": "lu_factor (panel + laswp + trsm + zgemm, batched)",
-                "achieved": lu_tflops, "peak": MI355X_FP64_MFMA_TFLOPS, "unit": "TFLOP/s",
-                "frac": lu_tflops / MI355X_FP64_MFMA_TFLOPS, "traffic": None,
-                "work_per_launch": f"{nb} x 8 n^3/3 flop, n = {n}"}
+    lu_tflops = nb * 8.0 * n ** 3 / 3.0 / t_lu / 1e12
+    # roofline of the dominant kernel of the reported (mixed) form, from the
+    # live HIP-event timing of every launch of it in the timed region
+    kern = rmx["kernels"]
+    dom = max(("zgemm", "cgemm", "skinny_z", "skinny_c"), key=lambda k: kern[k]["ms"])
+    kd = kern[dom]
+    names = {"zgemm": "gemm_tiled_kernel<double> (complex128 MFMA tile product)",
+             "cgemm": "gemm_tiled_kernel<float> (complex64 MFMA tile product)",
+             "skinny_z": "gemm_skinny_kernel<double> (complex128 product, <= 32 columns)",
+             "skinny_c": "gemm_skinny_kernel<float> (complex64 product, <= 32 columns)"}
+    if dom.startswith("skinny"):
+        achieved = kd["work"] / (kd["ms"] / 1e3) / 1e9
+        roofline = {"bound": "hbm", "kernel": names[dom], "achieved": achieved, "peak": MI355X_HBM_GBS,
+                    "unit": "GB/s", "frac": achieved / MI355X_HBM_GBS, "traffic": None,
+                    "work_per_launch": "bytes of A, B, C per product, summed over the batch"}
+    else:
+        achieved = kd["work"] / (kd["ms"] / 1e3) / 1e12
+        peak = MI355X_FP32_MFMA_TFLOPS if dom == "cgemm" else MI355X_FP64_MFMA_TFLOPS
+        roofline = {"bound": "mfma", "kernel": names[dom], "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                    "frac": achieved / peak, "traffic": None,
+                    "work_per_launch": "8 M N K flop per complex M x N x K product, summed over the batch"}
+    roofline.update(launches=kd["launches"], avg_launch_us=kd["ms"] / kd["launches"] * 1e3,
+                    work_per_launch_avg=kd["work"] / kd["launches"])
+
+    def kernel_table(r):
+        out = {}
+        for k, v in r["kernels"].items():
+            if not v["launches"]:
+                continue
+            rate = v["work"] / (v["ms"] / 1e3) if v["ms"] else 0.0
+            unit = {"assembly": ("gpairs_per_s", 1e9)}.get(k, ("gb_per_s", 1e9) if k.startswith("skinny")
+                                                          else ("tflops", 1e12))
+            out[k] = {"ms_per_step": v["ms"] / args.steps, "launches_per_step": v["launches"] / args.steps,
+                      unit[0]: rate / unit[1]}
+        return out
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": valuemx,
         "unit": "problems/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": rmx["seconds"] / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "c64 factor + f64 refinement (mixed); f64 reported beside",
         "data": "synthetic",
         "config": {"workload": "barge 20x8x2 m, 4352 panels, 40 frequencies 0.2-2.5 rad/s, 6 dof + 8 headings "
-                               "(560 problems), infinite depth, FP64 plain form",
-                   "panels": n, "frequencies": nb, "rhs_per_frequency": nrhs, "precision": "fp64"},
+                               "(560 problems), infinite depth; mixed-precision batched form vs FP64 plain form",
+                   "panels": n, "frequencies": nb, "rhs_per_frequency": nrhs},
+        "fp64_problems_per_s": value64,
+        "fp64_ms_per_step": r64["seconds"] / args.steps * 1e3,
+        "mixed_over_fp64": valuemx / value64,
+        "mixed_max_rel_err_vs_fp64": mixed_err,
+        "mixed_refinement_sweeps": rmx["iterations"],
+        "mixed_final_rel_residual": rmx["residual"],
         "assembly_panel_pairs_per_s": pairs_per_s,
-        "phase_ms_per_step": {k: v / args.steps * 1e3 for k, v in phases.items()},
-        "dominant_phase": dominant,
-        "lu_info_nonzero": int(np.count_nonzero(info)),
+        "phase_ms_per_step_fp64": {k: v * 1e3 for k, v in r64["phases"].items()},
+        "phase_ms_per_step_mixed": {k: v * 1e3 for k, v in rmx["phases"].items()},
+        "lu_info_nonzero": r64["info_nonzero"],
+        "fp64_lu_factor_tflops": lu_tflops,
+        "kernels_fp64": kernel_table(r64),
+        "kernels_mixed": kernel_table(rmx),
         "roofline": roofline,
     }
     if not args.no_cpu_baseline:

@@ -32,7 +32,14 @@ SIGNATURES = {
     "cpt_lu_factor": (_I, [_P, _P, _I, _L, _L, _P, _L, _P, _I, _P]),
     "cpt_lu_solve": (_I, [_P, _P, _I, _L, _L, _P, _L, _P, _I, _L, _L, _I, _P]),
     "cpt_zgemm": (_I, [_P, _I, _I, _I, _D, _P, _L, _L, _P, _L, _L, _P, _L, _L, _I, _I, _P]),
+    "cpt_lu_solve_mixed": (_I, [_P, _P, _I, _L, _L, _I, _P, _I, _L, _L, _P, _L, _L, _I, _D,
+                                ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double), _P]),
+    "cpt_profile_begin": (_I, [_P]),
+    "cpt_profile_end": (_I, [_P, _P, _P, _P]),
 }
+
+# Kernel classes of cpt_profile_end, in the order of the CPT_TIMED_* values.
+TIMED_KERNELS = ("zgemm", "cgemm", "panel_z", "panel_c", "assembly", "skinny_z", "skinny_c")
 
 _lib = None
 _lock = threading.RLock()  # Context.get -> Context.__init__ -> load() re-enters it
@@ -115,3 +122,17 @@ def default_device():
     import torch
     require_gpu()
     return torch.device("cuda", torch.cuda.current_device())
+
+
+def profile_begin(ctx):
+    """Start the live HIP-event timing of the hot kernels launched through ctx."""
+    check(load().cpt_profile_begin(ctx.handle), "cpt_profile_begin")
+
+
+def profile_end(ctx):
+    """Stop the timing; {class: {"ms", "work", "launches"}} summed since begin."""
+    n = len(TIMED_KERNELS)
+    ms, work, launches = np.zeros(n), np.zeros(n), np.zeros(n, dtype=np.int64)
+    check(load().cpt_profile_end(ctx.handle, ptr(ms), ptr(work), ptr(launches)), "cpt_profile_end")
+    return {k: {"ms": float(ms[i]), "work": float(work[i]), "launches": int(launches[i])}
+            for i, k in enumerate(TIMED_KERNELS)}

@@ -20,12 +20,58 @@ int launch_zgemm(int M, int N, int K, double alpha, const cplx* A, int64_t lda, 
                  int64_t ldb, int64_t sB, cplx* C, int64_t ldc, int64_t sC, int beta_zero, int batch,
                  hipStream_t stream);
 int launch_getrf(cplx* A, int N, int64_t lda, int64_t strideA, int* ipiv, int64_t strideP, int* info, int batch,
-                 hipStream_t stream);
+                 cplx* work, const LuStreams* aux, hipStream_t stream);
+size_t getrf_workspace_bytes(int N, int batch);
+size_t mixed_workspace_bytes(int n, int nrhs, int batch);
+int launch_mixed_solve(const cplx* K, int n, int64_t lda, int64_t strideK, int batch, const cplx* B, int nrhs,
+                       int64_t ldb, int64_t strideB, cplx* X, int64_t ldx, int64_t strideX, int max_iter, double tol,
+                       void* work, const LuStreams* aux, int* iterations, double* residual, hipStream_t stream);
 int launch_getrs(const cplx* LU, int N, int64_t lda, int64_t strideA, const int* ipiv, int64_t strideP, cplx* B,
                  int nrhs, int64_t ldb, int64_t strideB, int batch, hipStream_t stream);
 
 static thread_local std::string g_last_error;
 void set_error(const std::string& msg) { g_last_error = msg; }
+
+thread_local KernelTimer* active_timer = nullptr;
+
+int KernelTimer::start(hipStream_t s, int kind, double work)
+{
+    if (!on) return -1;
+    if (used == recs.size()) {
+        Rec r;
+        if (hipEventCreate(&r.a) != hipSuccess) return -1;
+        if (hipEventCreate(&r.b) != hipSuccess) {
+            (void)hipEventDestroy(r.a);
+            return -1;
+        }
+        recs.push_back(r);
+    }
+    Rec& r = recs[used];
+    r.kind = kind;
+    r.work = work;
+    if (hipEventRecord(r.a, s) != hipSuccess) return -1;
+    return (int)used++;
+}
+
+void KernelTimer::stop(int slot, hipStream_t s) { (void)hipEventRecord(recs[slot].b, s); }
+
+void KernelTimer::release()
+{
+    for (Rec& r : recs) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    recs.clear();
+    used = 0;
+    on = false;
+}
+
+// Installs a context's timer for the calling thread during one entry point.
+struct TimerScope {
+    KernelTimer* prev;
+    explicit TimerScope(KernelTimer* t) : prev(active_timer) { active_timer = (t && t->on) ? t : nullptr; }
+    ~TimerScope() { active_timer = prev; }
+};
 
 // Growable device buffer owned by a context.
 struct DevBuf {
@@ -61,7 +107,9 @@ struct cpt_context {
     bool has_table = false;
     // per-call workspace
     cpt::DevBuf faces, quad, freq, prony;
-    cpt::DevBuf lu_work;
+    cpt::DevBuf lu_work, mixed_work;
+    cpt::LuStreams lu_streams;
+    cpt::KernelTimer timer;
 };
 
 using namespace cpt;
@@ -129,6 +177,16 @@ __global__ void zero_int_kernel(int* __restrict__ p, int n)
     if (i < n) p[i] = 0;
 }
 
+// Side stream (default priority) and events of the LU look-ahead.
+int ensure_lu_streams(cpt_context* h)
+{
+    if (h->lu_streams.side) return 0;
+    if (hipStreamCreateWithFlags(&h->lu_streams.side, hipStreamNonBlocking) != hipSuccess) return -1;
+    if (hipEventCreateWithFlags(&h->lu_streams.ready, hipEventDisableTiming) != hipSuccess) return -1;
+    if (hipEventCreateWithFlags(&h->lu_streams.done, hipEventDisableTiming) != hipSuccess) return -1;
+    return 0;
+}
+
 Table table_of(const cpt_context* h)
 {
     Table T;
@@ -184,6 +242,11 @@ int cpt_destroy(cpt_handle h)
     h->freq.release();
     h->prony.release();
     h->lu_work.release();
+    h->mixed_work.release();
+    if (h->lu_streams.side) (void)hipStreamDestroy(h->lu_streams.side);
+    if (h->lu_streams.ready) (void)hipEventDestroy(h->lu_streams.ready);
+    if (h->lu_streams.done) (void)hipEventDestroy(h->lu_streams.done);
+    h->timer.release();
     delete h;
     return CPT_OK;
 }
@@ -309,6 +372,7 @@ int cpt_build_matrices(cpt_handle h, int nb_collocation_points, const double* co
         return CPT_ERR_STATE;
     }
     CPT_CHECK_HIP(hipSetDevice(h->device), "hipSetDevice");
+    TimerScope timer_scope(&h->timer);
 
     // Pack panels and quadrature points.
     const int ne = finite_depth ? std::max(nexp, 0) : 0;
@@ -391,6 +455,7 @@ int cpt_wave_part_infinite_depth(cpt_handle h, int np, const double* x0i, const 
     }
     if (np == 0) return CPT_OK;
     CPT_CHECK_HIP(hipSetDevice(h->device), "hipSetDevice");
+    TimerScope timer_scope(&h->timer);
     hipLaunchKernelGGL(wave_points_kernel, dim3((np + 255) / 256), dim3(256), 0, stream, table_of(h), np, x0i, x0j,
                        wavenumber, gf_singularities, (cplx*)g, (cplx*)nablag);
     CPT_CHECK_HIP(hipGetLastError(), "wave kernel launch");
@@ -414,9 +479,19 @@ int cpt_lu_factor(cpt_handle h, void* A, int n, int64_t lda, int64_t strideA, in
         return CPT_ERR_UNSUPPORTED;
     }
     CPT_CHECK_HIP(hipSetDevice(h->device), "hipSetDevice");
+    TimerScope timer_scope(&h->timer);
     hipLaunchKernelGGL(zero_int_kernel, dim3((batch + 255) / 256), dim3(256), 0, stream, info, batch);
     CPT_CHECK_HIP(hipGetLastError(), "zero info launch");
-    CPT_CHECK_HIP((hipError_t)launch_getrf((cplx*)A, n, lda, strideA, ipiv, strideP, info, batch, stream),
+    if (h->lu_work.reserve(getrf_workspace_bytes(n, batch))) {
+        set_error("cpt_lu_factor: workspace allocation failed");
+        return CPT_ERR_ALLOC;
+    }
+    if (ensure_lu_streams(h)) {
+        set_error("cpt_lu_factor: could not create the look-ahead stream");
+        return CPT_ERR_HIP;
+    }
+    CPT_CHECK_HIP((hipError_t)launch_getrf((cplx*)A, n, lda, strideA, ipiv, strideP, info, batch,
+                                           (cplx*)h->lu_work.p, &h->lu_streams, stream),
                   "getrf launch");
     return CPT_OK;
 }
@@ -435,6 +510,7 @@ int cpt_lu_solve(cpt_handle h, const void* LU, int n, int64_t lda, int64_t strid
         return CPT_ERR_ARG;
     }
     CPT_CHECK_HIP(hipSetDevice(h->device), "hipSetDevice");
+    TimerScope timer_scope(&h->timer);
     CPT_CHECK_HIP((hipError_t)launch_getrs((const cplx*)LU, n, lda, strideA, ipiv, strideP, (cplx*)B, nrhs, ldb,
                                            strideB, batch, stream),
                   "getrs launch");
@@ -455,10 +531,90 @@ int cpt_zgemm(cpt_handle h, int M, int N, int K, double alpha, const void* A, in
         return CPT_ERR_ARG;
     }
     CPT_CHECK_HIP(hipSetDevice(h->device), "hipSetDevice");
+    TimerScope timer_scope(&h->timer);
     CPT_CHECK_HIP((hipError_t)launch_zgemm(M, N, K, alpha, (const cplx*)A, lda, strideA, (const cplx*)B, ldb, strideB,
                                            (cplx*)C, ldc, strideC, beta_zero, batch, stream),
                   "zgemm launch");
     return CPT_OK;
 }
 
+int cpt_lu_solve_mixed(cpt_handle h, const void* K, int n, int64_t lda, int64_t strideK, int batch, const void* B,
+                       int nrhs, int64_t ldb, int64_t strideB, void* X, int64_t ldx, int64_t strideX, int max_iter,
+                       double tol, int* iterations, double* residual, hipStream_t stream)
+{
+    if (!h || n < 0 || nrhs < 0 || batch < 0 || lda < n || ldb < n || ldx < n || max_iter < 0 ||
+        (batch > 1 && (strideK < lda * n || strideB < ldb * nrhs || strideX < ldx * nrhs))) {
+        set_error("cpt_lu_solve_mixed: invalid sizes or strides");
+        return CPT_ERR_ARG;
+    }
+    if (!iterations || !residual) {
+        set_error("cpt_lu_solve_mixed: null output pointer");
+        return CPT_ERR_ARG;
+    }
+    *iterations = 0;
+    *residual = 0.0;
+    if (n == 0 || nrhs == 0 || batch == 0) return CPT_OK;
+    if (!K || !B || !X) {
+        set_error("cpt_lu_solve_mixed: null pointer argument");
+        return CPT_ERR_ARG;
+    }
+    if (n > 24576) {
+        set_error("cpt_lu_solve_mixed: n > 24576 is not supported");
+        return CPT_ERR_UNSUPPORTED;
+    }
+    CPT_CHECK_HIP(hipSetDevice(h->device), "hipSetDevice");
+    TimerScope timer_scope(&h->timer);
+    if (h->mixed_work.reserve(mixed_workspace_bytes(n, nrhs, batch))) {
+        set_error("cpt_lu_solve_mixed: workspace allocation failed");
+        return CPT_ERR_ALLOC;
+    }
+    if (ensure_lu_streams(h)) {
+        set_error("cpt_lu_solve_mixed: could not create the look-ahead stream");
+        return CPT_ERR_HIP;
+    }
+    CPT_CHECK_HIP((hipError_t)launch_mixed_solve((const cplx*)K, n, lda, strideK, batch, (const cplx*)B, nrhs, ldb,
+                                                 strideB, (cplx*)X, ldx, strideX, max_iter, tol, h->mixed_work.p,
+                                                 &h->lu_streams, iterations, residual, stream),
+                  "mixed solve");
+    return CPT_OK;
+}
+
 }  // extern "C"
+
+int cpt_profile_begin(cpt_handle h)
+{
+    if (!h) {
+        set_error("cpt_profile_begin: null handle");
+        return CPT_ERR_ARG;
+    }
+    h->timer.used = 0;
+    h->timer.on = true;
+    return CPT_OK;
+}
+
+int cpt_profile_end(cpt_handle h, double* ms, double* work, int64_t* launches)
+{
+    if (!h || !ms || !work || !launches) {
+        set_error("cpt_profile_end: null argument");
+        return CPT_ERR_ARG;
+    }
+    CPT_CHECK_HIP(hipSetDevice(h->device), "hipSetDevice");
+    for (int k = 0; k < CPT_TIMED_KERNELS; ++k) {
+        ms[k] = 0.0;
+        work[k] = 0.0;
+        launches[k] = 0;
+    }
+    KernelTimer& t = h->timer;
+    t.on = false;
+    for (size_t i = 0; i < t.used; ++i) {
+        KernelTimer::Rec& r = t.recs[i];
+        CPT_CHECK_HIP(hipEventSynchronize(r.b), "hipEventSynchronize");
+        float dt = 0.0f;
+        CPT_CHECK_HIP(hipEventElapsedTime(&dt, r.a, r.b), "hipEventElapsedTime");
+        ms[r.kind] += dt;
+        work[r.kind] += r.work;
+        launches[r.kind] += 1;
+    }
+    t.used = 0;
+    return CPT_OK;
+}

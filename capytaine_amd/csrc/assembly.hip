@@ -271,6 +271,7 @@ int launch_assembly(const AssemblyArgs& A, hipStream_t stream)
     const int gx = (A.n + 255) / 256;
     const int gy = (A.m + A.j_chunk - 1) / A.j_chunk;
     dim3 grid(gx, gy);
+    TimedLaunch timed(stream, TK_ASSEMBLY, (double)A.n * A.m * A.nb);
     if (A.finite_depth) hipLaunchKernelGGL(assemble_finite_depth_kernel, grid, block, 0, stream, A);
     else hipLaunchKernelGGL(assemble_infinite_depth_kernel, grid, block, 0, stream, A);
     return (int)hipGetLastError();

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <string>
+#include <vector>
 #include "green.hpp"
 
 namespace cpt {
@@ -39,6 +40,57 @@ struct AssemblyArgs {
 
 int launch_assembly(const AssemblyArgs& A, hipStream_t stream);
 
+// Side stream and events of the LU look-ahead (owned by a context).
+struct LuStreams {
+    hipStream_t side = nullptr;
+    hipEvent_t ready = nullptr, done = nullptr;
+};
+
 void set_error(const std::string& msg);
+
+// Live timing of selected kernels: start/stop HIP events recorded on the
+// stream each timed launch goes to, with the launch's algorithmic work
+// (flops, or panel pairs for the assembly).  Enabled per context; the entry
+// points install the context's timer for the calling thread.  Brackets may
+// nest: a panel bracket includes the tile products launched inside it, which
+// are also timed in their own class.
+enum TimedKernel {
+    TK_ZGEMM = 0,
+    TK_CGEMM = 1,
+    TK_PANEL_Z = 2,
+    TK_PANEL_C = 3,
+    TK_ASSEMBLY = 4,
+    TK_SKINNY_Z = 5,
+    TK_SKINNY_C = 6,
+    TK_COUNT = 7
+};
+struct KernelTimer {
+    struct Rec {
+        hipEvent_t a = nullptr, b = nullptr;
+        int kind = 0;
+        double work = 0.0;
+    };
+    std::vector<Rec> recs;
+    size_t used = 0;
+    bool on = false;
+    int start(hipStream_t s, int kind, double work);  // slot, or -1
+    void stop(int slot, hipStream_t s);
+    void release();
+};
+extern thread_local KernelTimer* active_timer;
+
+// Brackets one launch with the active timer's events (no-op when off).
+struct TimedLaunch {
+    int slot;
+    hipStream_t s;
+    TimedLaunch(hipStream_t stream, int kind, double work)
+        : slot(active_timer ? active_timer->start(stream, kind, work) : -1), s(stream)
+    {
+    }
+    ~TimedLaunch()
+    {
+        if (slot >= 0 && active_timer) active_timer->stop(slot, s);
+    }
+};
 
 }  // namespace cpt

@@ -1,93 +1,154 @@
-// Batched dense complex128 LU with partial pivoting and the matching
-// multi-right-hand-side solve, for gfx950.
+// Batched dense complex LU with partial pivoting, the matching multi-RHS
+// solve, complex GEMM, and the mixed-precision (complex64 factor, complex128
+// refinement) solve, for gfx950.
 //
 // Replaces LAPACK zgetrf / zgetrs as reached through scipy.linalg.lu_factor
 // and lu_solve (capytaine tools/block_circulant_matrices.py:361-385, called
-// from bem/engines.py:326-338).  Same conventions as LAPACK: column-major
-// storage, P A = L U with L unit lower triangular, pivots chosen by the
-// largest |Re|+|Im| (first index on ties), ipiv[i] = row swapped with row i
-// (0-based here), info = 1 + first zero pivot.
+// from bem/engines.py:326-338).  LAPACK conventions: column-major storage,
+// P A = L U with L unit lower triangular, pivots chosen by the largest
+// |Re|+|Im| (first index on ties), ipiv[i] = row swapped with row i (0-based
+// here), info = 1 + index of the first exactly-zero pivot.
 //
-// Right-looking blocked algorithm, one launch per step for the whole batch:
-//   1. panel: one work-group per matrix factors the NB-column panel; the
-//      panel is processed in register-resident sub-blocks of SB columns
-//      (left-looking inside the panel), with a block-wide pivot search;
+// Every kernel is templated on the real type R: R = double gives complex128
+// on v_mfma_f64_16x16x4f64, R = float gives complex64 on
+// v_mfma_f32_16x16x4f32.
+//
+// Right-looking blocked algorithm (block width 128, or 64 below 1024
+// rows), one launch per step for the whole batch:
+//   1. panel: recursive halving down to 16-column leaves factored by one
+//      work-group per matrix, the halves joined by trsm + MFMA GEMM;
 //   2. row interchanges applied to the columns outside the panel;
-//   3. U12 = L11^-1 A12 (unit lower triangular solve, L11 staged in LDS);
-//   4. A22 -= L21 U12 with an FP64-MFMA complex GEMM (v_mfma_f64_16x16x4).
+//   3. U12 = L11^-1 A12 in place (wavefront-per-column triangular solves);
+//   4. A22 -= L21 U12 with the MFMA complex GEMM, the next panel's columns
+//      first so that the next panel is factored while the rest is updated.
 #include <hip/hip_runtime.h>
 #include <climits>
+#include <cmath>
+#include <vector>
 #include "cpt_internal.hpp"
 
 namespace cpt {
 namespace {
 
 typedef double dvec4 __attribute__((ext_vector_type(4)));
+typedef float fvec4 __attribute__((ext_vector_type(4)));
 
-struct z2 {
-    double re, im;
+template <typename R>
+struct cx {
+    R re, im;
 };
 
-__device__ __forceinline__ z2 zmul(z2 a, z2 b) { return z2{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
-__device__ __forceinline__ z2 zsub(z2 a, z2 b) { return z2{a.re - b.re, a.im - b.im}; }
-__device__ __forceinline__ z2 zrcp(z2 a)
+template <typename R>
+__device__ __forceinline__ cx<R> zmul(cx<R> a, cx<R> b)
 {
-    // Smith's reciprocal, no overflow for |a| up to the double range.
+    return cx<R>{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+template <typename R>
+__device__ __forceinline__ cx<R> zsub(cx<R> a, cx<R> b)
+{
+    return cx<R>{a.re - b.re, a.im - b.im};
+}
+template <typename R>
+__device__ __forceinline__ cx<R> zrcp(cx<R> a)
+{
+    // Smith's reciprocal, no overflow for |a| up to the floating-point range.
     if (fabs(a.re) >= fabs(a.im)) {
-        const double r = a.im / a.re, d = a.re + a.im * r;
-        return z2{1.0 / d, -r / d};
+        const R r = a.im / a.re, d = a.re + a.im * r;
+        return cx<R>{R(1) / d, -r / d};
     } else {
-        const double r = a.re / a.im, d = a.re * r + a.im;
-        return z2{r / d, -1.0 / d};
+        const R r = a.re / a.im, d = a.re * r + a.im;
+        return cx<R>{r / d, R(-1) / d};
     }
 }
-__device__ __forceinline__ double cabs1(z2 a) { return fabs(a.re) + fabs(a.im); }
-
-__device__ __forceinline__ z2 ld(const cplx* p) { return z2{p->re, p->im}; }
-__device__ __forceinline__ void st(cplx* p, z2 v)
+template <typename R>
+__device__ __forceinline__ R cabs1(cx<R> a)
 {
-    p->re = v.re;
-    p->im = v.im;
+    return fabs(a.re) + fabs(a.im);
 }
 
-constexpr int NB = 64;  // panel width
+// MFMA 16 x 16 x 4 of each precision.  Operand lanes: A[i][k] at lane
+// i + 16 k, B[k][j] at lane j + 16 k.  Accumulator register v of lane l holds
+// D[row(l, v)][l % 16]: row 4 v + l/16 for FP64 (four groups of one row),
+// 4 (l/16) + v for FP32 (one group of four rows).
+template <typename R>
+struct Mfma;
+template <>
+struct Mfma<double> {
+    typedef dvec4 acc;
+    static __device__ __forceinline__ acc op(double a, double b, acc c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lane, int v) { return 4 * v + (lane >> 4); }
+};
+template <>
+struct Mfma<float> {
+    typedef fvec4 acc;
+    static __device__ __forceinline__ acc op(float a, float b, acc c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
+};
+
+constexpr int PB = 64;  // widest triangle the panel and trsm kernels hold (one wavefront)
 
 // ---------------------------------------------------------------------------
 // 1. Panel factorisation
+//
+// One work-group of NT threads per matrix.  Rows k0..N-1 of the panel are
+// distributed round-robin (row t = tid + NT q, q < RPT); the panel columns
+// are processed in sub-blocks of SB columns held in registers:
+//   (a) load the sub-block;
+//   (b) its rows above the diagonal block are U: solve with the unit lower
+//       triangle L11 of the columns already factored (L11 staged in LDS, one
+//       wavefront per column, the running solution held across the lanes);
+//   (c) left-looking update of the rows below with L(rows, k0:c0) U;
+//   (d) factor the SB columns: block-wide pivot search, row interchange and
+//       broadcast through LDS, scaling and rank-1 updates in registers;
+//   (e) store the sub-block, then apply its interchanges to the other
+//       columns [sw_lo, sw_hi) in global memory (once per sub-block).
 // ---------------------------------------------------------------------------
-template <int NT, int RPT, int SB>
-__global__ void __launch_bounds__(NT) panel_kernel(cplx* __restrict__ Abase, int64_t lda, int64_t strideA, int N,
-                                                   int k0, int kb, int* __restrict__ ipiv_base, int64_t strideP,
+template <typename R, int NT, int RPT, int SB>
+__global__ void __launch_bounds__(NT) panel_kernel(cx<R>* __restrict__ Abase, int64_t lda, int64_t strideA, int N,
+                                                   int k0, int kb, int sw_lo, int sw_hi,
+                                                   int* __restrict__ ipiv_base, int64_t strideP,
                                                    int* __restrict__ info_base)
 {
-    cplx* A = Abase + (size_t)blockIdx.x * strideA;
+    typedef cx<R> C;
+    C* A = Abase + (size_t)blockIdx.x * strideA;
     int* ipiv = ipiv_base + (size_t)blockIdx.x * strideP;
     const int M = N - k0;
     const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
     constexpr int NW = NT / 64;
+    const C zero{R(0), R(0)};
 
-    __shared__ z2 sU[NB * SB];
-    __shared__ z2 sRow[SB];
-    __shared__ z2 sSwap[2][SB];
-    __shared__ double sBest[NW];
+    __shared__ C sL[PB * PB];  // L11, row-major [i][j]
+    __shared__ C sU[PB * SB];
+    __shared__ C sRow[SB];
+    __shared__ C sSwap[2][SB];
+    __shared__ R sBest[NW];
     __shared__ int sIdx[NW];
-    __shared__ int sPiv;
+    __shared__ int sPiv[SB];
 
-    z2 v[RPT][SB];
+    C v[RPT][SB];
 
     for (int c0 = k0; c0 < k0 + kb; c0 += SB) {
         const int sbw = min(SB, k0 + kb - c0);
         const int t0 = c0 - k0;
-        // (a) load the sub-block
+        // (a)
 #pragma unroll
         for (int q = 0; q < RPT; ++q) {
             const int t = tid + NT * q;
 #pragma unroll
-            for (int s = 0; s < SB; ++s)
-                v[q][s] = (t < M && s < sbw) ? ld(A + (k0 + t) + (size_t)(c0 + s) * lda) : z2{0.0, 0.0};
+            for (int s = 0; s < SB; ++s) {
+                if (t < M && s < sbw) v[q][s] = A[(k0 + t) + (size_t)(c0 + s) * lda];
+                else v[q][s].re = v[q][s].im = R(0);
+            }
         }
         if (t0 > 0) {
-            // (b) U rows of the sub-block: solve with the unit lower L11 (t0 x t0)
+            // (b)
 #pragma unroll
             for (int q = 0; q < RPT; ++q) {
                 const int t = tid + NT * q;
@@ -95,18 +156,27 @@ __global__ void __launch_bounds__(NT) panel_kernel(cplx* __restrict__ Abase, int
 #pragma unroll
                     for (int s = 0; s < SB; ++s) sU[t * SB + s] = v[q][s];
             }
-            __syncthreads();
-            if (tid < sbw) {
-                const int s = tid;
-                for (int i = 1; i < t0; ++i) {
-                    z2 acc = sU[i * SB + s];
-                    for (int j = 0; j < i; ++j)
-                        acc = zsub(acc, zmul(ld(A + (k0 + i) + (size_t)(k0 + j) * lda), sU[j * SB + s]));
-                    sU[i * SB + s] = acc;
-                }
+            for (int e = tid; e < t0 * t0; e += NT) {
+                const int i = e % t0, j = e / t0;
+                sL[i * PB + j] = A[(k0 + i) + (size_t)(k0 + j) * lda];
             }
             __syncthreads();
-            // (c) left-looking update of the rows below
+            if (wave < sbw) {
+                const int s = wave;
+                C x = (lane < t0) ? sU[lane * SB + s] : zero;
+                for (int i = 1; i < t0; ++i) {
+                    C part = zero;
+                    if (lane < i) part = zmul(sL[i * PB + lane], x);
+                    for (int off = 32; off > 0; off >>= 1) {
+                        part.re += __shfl_xor(part.re, off);
+                        part.im += __shfl_xor(part.im, off);
+                    }
+                    if (lane == i) x = zsub(x, part);
+                }
+                if (lane < t0) sU[lane * SB + s] = x;
+            }
+            __syncthreads();
+            // (c)
 #pragma unroll
             for (int q = 0; q < RPT; ++q) {
                 const int t = tid + NT * q;
@@ -114,26 +184,28 @@ __global__ void __launch_bounds__(NT) panel_kernel(cplx* __restrict__ Abase, int
 #pragma unroll
                     for (int s = 0; s < SB; ++s) v[q][s] = sU[t * SB + s];
                 } else if (t < M) {
+                    const C* Lrow = A + (k0 + t) + (size_t)k0 * lda;
+#pragma unroll 8
                     for (int j = 0; j < t0; ++j) {
-                        const z2 l = ld(A + (k0 + t) + (size_t)(k0 + j) * lda);
+                        const C l = Lrow[(size_t)j * lda];
 #pragma unroll
                         for (int s = 0; s < SB; ++s) v[q][s] = zsub(v[q][s], zmul(l, sU[j * SB + s]));
                     }
                 }
             }
         }
-        // (d) factor the sub-block column by column
+        // (d)
 #pragma unroll
         for (int s = 0; s < SB; ++s) {
             if (s < sbw) {
                 const int td = t0 + s;
-                double best = -1.0;
+                R best = R(-1);
                 int bidx = INT_MAX;
 #pragma unroll
                 for (int q = 0; q < RPT; ++q) {
                     const int t = tid + NT * q;
                     if (t >= td && t < M) {
-                        const double a = cabs1(v[q][s]);
+                        const R a = cabs1(v[q][s]);
                         if (a > best) {
                             best = a;
                             bidx = t;
@@ -141,20 +213,20 @@ __global__ void __launch_bounds__(NT) panel_kernel(cplx* __restrict__ Abase, int
                     }
                 }
                 for (int off = 32; off > 0; off >>= 1) {
-                    const double ob = __shfl_xor(best, off);
+                    const R ob = __shfl_xor(best, off);
                     const int oi = __shfl_xor(bidx, off);
                     if (ob > best || (ob == best && oi < bidx)) {
                         best = ob;
                         bidx = oi;
                     }
                 }
-                if ((tid & 63) == 0) {
-                    sBest[tid >> 6] = best;
-                    sIdx[tid >> 6] = bidx;
+                if (lane == 0) {
+                    sBest[wave] = best;
+                    sIdx[wave] = bidx;
                 }
                 __syncthreads();
                 if (tid == 0) {
-                    double b = sBest[0];
+                    R b = sBest[0];
                     int bi = sIdx[0];
                     for (int w = 1; w < NW; ++w)
                         if (sBest[w] > b || (sBest[w] == b && sIdx[w] < bi)) {
@@ -162,15 +234,15 @@ __global__ void __launch_bounds__(NT) panel_kernel(cplx* __restrict__ Abase, int
                             bi = sIdx[w];
                         }
                     int p = bi;
-                    if (!(b > 0.0)) {  // zero column: no interchange, report singularity
+                    if (!(b > R(0))) {  // zero column: no interchange, report singularity
                         p = td;
                         if (info_base[blockIdx.x] == 0) info_base[blockIdx.x] = c0 + s + 1;
                     }
-                    sPiv = p;
+                    sPiv[s] = p;
                     ipiv[c0 + s] = k0 + p;
                 }
                 __syncthreads();
-                const int p = sPiv;
+                const int p = sPiv[s];
                 if (p != td) {
 #pragma unroll
                     for (int q = 0; q < RPT; ++q) {
@@ -181,17 +253,6 @@ __global__ void __launch_bounds__(NT) panel_kernel(cplx* __restrict__ Abase, int
                         if (t == p)
 #pragma unroll
                             for (int s2 = 0; s2 < SB; ++s2) sSwap[1][s2] = v[q][s2];
-                    }
-                    // interchange in the other columns of the panel (global memory)
-                    for (int c = tid; c < kb; c += NT) {
-                        const int col = k0 + c;
-                        if (col < c0 || col >= c0 + sbw) {
-                            cplx* a = A + (k0 + td) + (size_t)col * lda;
-                            cplx* b = A + (k0 + p) + (size_t)col * lda;
-                            const z2 x = ld(a), y = ld(b);
-                            st(a, y);
-                            st(b, x);
-                        }
                     }
                     __syncthreads();
 #pragma unroll
@@ -205,21 +266,20 @@ __global__ void __launch_bounds__(NT) panel_kernel(cplx* __restrict__ Abase, int
                             for (int s2 = 0; s2 < SB; ++s2) v[q][s2] = sSwap[0][s2];
                     }
                 }
-                // broadcast the pivot row
 #pragma unroll
                 for (int q = 0; q < RPT; ++q)
                     if (tid + NT * q == td)
 #pragma unroll
                         for (int s2 = 0; s2 < SB; ++s2) sRow[s2] = v[q][s2];
                 __syncthreads();
-                const z2 piv = sRow[s];
-                const bool nonzero = (piv.re != 0.0 || piv.im != 0.0);
-                const z2 r = nonzero ? zrcp(piv) : z2{1.0, 0.0};
+                const C piv = sRow[s];
+                const bool nonzero = (piv.re != R(0) || piv.im != R(0));
+                const C r = nonzero ? zrcp(piv) : C{R(1), R(0)};
 #pragma unroll
                 for (int q = 0; q < RPT; ++q) {
                     const int t = tid + NT * q;
                     if (t > td && t < M) {
-                        const z2 l = nonzero ? zmul(v[q][s], r) : v[q][s];
+                        const C l = nonzero ? zmul(v[q][s], r) : v[q][s];
                         v[q][s] = l;
 #pragma unroll
                         for (int s2 = s + 1; s2 < SB; ++s2) v[q][s2] = zsub(v[q][s2], zmul(l, sRow[s2]));
@@ -228,14 +288,27 @@ __global__ void __launch_bounds__(NT) panel_kernel(cplx* __restrict__ Abase, int
                 __syncthreads();
             }
         }
-        // (e) store the sub-block
+        // (e)
 #pragma unroll
         for (int q = 0; q < RPT; ++q) {
             const int t = tid + NT * q;
             if (t < M)
 #pragma unroll
                 for (int s = 0; s < SB; ++s)
-                    if (s < sbw) st(A + (k0 + t) + (size_t)(c0 + s) * lda, v[q][s]);
+                    if (s < sbw) A[(k0 + t) + (size_t)(c0 + s) * lda] = v[q][s];
+        }
+        for (int col = sw_lo + tid; col < sw_hi; col += NT) {
+            if (col < c0 || col >= c0 + sbw) {
+                C* a = A + k0 + (size_t)col * lda;
+                for (int s = 0; s < sbw; ++s) {
+                    const int td = t0 + s, p = sPiv[s];
+                    if (p != td) {
+                        const C x = a[td], y = a[p];
+                        a[td] = y;
+                        a[p] = x;
+                    }
+                }
+            }
         }
         __syncthreads();
     }
@@ -245,145 +318,156 @@ __global__ void __launch_bounds__(NT) panel_kernel(cplx* __restrict__ Abase, int
 // 2. Row interchanges ipiv[r0 .. r1) applied to the columns [0, c_lo) and
 //    [c_hi, ncols) of a column-major array (LAPACK zlaswp).
 // ---------------------------------------------------------------------------
-__global__ void laswp_kernel(cplx* __restrict__ Abase, int64_t lda, int64_t strideA, int ncols, int c_lo, int c_hi,
+template <typename R>
+__global__ void laswp_kernel(cx<R>* __restrict__ Abase, int64_t lda, int64_t strideA, int ncols, int c_lo, int c_hi,
                              const int* __restrict__ ipiv_base, int64_t strideP, int r0, int r1)
 {
     const int col = blockIdx.x * blockDim.x + threadIdx.x;
     if (col >= ncols || (col >= c_lo && col < c_hi)) return;
-    cplx* A = Abase + (size_t)blockIdx.y * strideA + (size_t)col * lda;
+    cx<R>* A = Abase + (size_t)blockIdx.y * strideA + (size_t)col * lda;
     const int* ipiv = ipiv_base + (size_t)blockIdx.y * strideP;
     for (int i = r0; i < r1; ++i) {
         const int p = ipiv[i];
         if (p != i) {
-            const z2 x = ld(A + i), y = ld(A + p);
-            st(A + i, y);
-            st(A + p, x);
+            const cx<R> x = A[i], y = A[p];
+            A[i] = y;
+            A[p] = x;
         }
     }
 }
 
 // ---------------------------------------------------------------------------
-// 3. Triangular solve with a kb x kb diagonal block T (kb <= NB), in place on
-//    the kb rows of B, one thread per column of B.
+// 3. Triangular solve with a kb x kb diagonal block T (kb <= PB = 64), in
+//    place on the kb rows of B.  One wavefront per column of B, lane i holds
+//    x_i; the triangle of T is staged in LDS (column-major, lane-consecutive
+//    reads) once per work-group and reused for TRSM_WAVES x TRSM_CPW columns,
+//    two columns in flight per wavefront.
 //    UPPER = false: T unit lower triangular; UPPER = true: T upper, non-unit.
 // ---------------------------------------------------------------------------
-template <bool UPPER>
-__global__ void __launch_bounds__(256) trsm_kernel(const cplx* __restrict__ Tbase, int64_t ldt, int64_t strideT,
-                                                   int kb, cplx* __restrict__ Bbase, int64_t ldb, int64_t strideB,
-                                                   int ncols)
+constexpr int TRSM_WAVES = 4, TRSM_CPW = 8;
+
+template <typename R, bool UPPER>
+__global__ void __launch_bounds__(64 * TRSM_WAVES) trsm_kernel(const cx<R>* __restrict__ Tbase, int64_t ldt,
+                                                               int64_t strideT, int kb, cx<R>* __restrict__ Bbase,
+                                                               int64_t ldb, int64_t strideB, int ncols)
 {
-    __shared__ z2 sT[NB * NB];
-    __shared__ z2 sD[NB];
-    const cplx* T = Tbase + (size_t)blockIdx.y * strideT;
+    typedef cx<R> C;
+    __shared__ C sT[PB * PB];
+    __shared__ C sD[PB];
+    const C* T = Tbase + (size_t)blockIdx.y * strideT;
     for (int e = threadIdx.x; e < kb * kb; e += blockDim.x) {
         const int i = e % kb, j = e / kb;
-        sT[i + NB * j] = ld(T + i + (size_t)j * ldt);
+        if (UPPER ? (i <= j) : (i > j)) sT[i + PB * j] = T[i + (size_t)j * ldt];
     }
     __syncthreads();
     if (UPPER) {
-        for (int i = threadIdx.x; i < kb; i += blockDim.x) sD[i] = zrcp(sT[i + NB * i]);
+        for (int i = threadIdx.x; i < kb; i += blockDim.x) sD[i] = zrcp(sT[i + PB * i]);
         __syncthreads();
     }
-    const int col = blockIdx.x * blockDim.x + threadIdx.x;
-    if (col >= ncols) return;
-    cplx* B = Bbase + (size_t)blockIdx.y * strideB + (size_t)col * ldb;
-    z2 x[NB];
-#pragma unroll
-    for (int i = 0; i < NB; ++i) x[i] = (i < kb) ? ld(B + i) : z2{0.0, 0.0};
-    if (!UPPER) {
-#pragma unroll
-        for (int j = 0; j < NB; ++j) {
-            if (j < kb) {
-                const z2 xj = x[j];
-#pragma unroll
-                for (int i = j + 1; i < NB; ++i)
-                    if (i < kb) x[i] = zsub(x[i], zmul(sT[i + NB * j], xj));
+    const int lane = threadIdx.x & 63;
+    const int cbase = (blockIdx.x * TRSM_WAVES + (threadIdx.x >> 6)) * TRSM_CPW;
+    const C zero{R(0), R(0)};
+    for (int c = 0; c < TRSM_CPW; c += 2) {
+        const int col0 = cbase + c;
+        if (col0 >= ncols) break;
+        const bool has1 = col0 + 1 < ncols;
+        C* B0 = Bbase + (size_t)blockIdx.y * strideB + (size_t)col0 * ldb;
+        C* B1 = B0 + ldb;
+        C x0 = (lane < kb) ? B0[lane] : zero;
+        C x1 = (has1 && lane < kb) ? B1[lane] : zero;
+        if (!UPPER) {
+            for (int j = 0; j < kb - 1; ++j) {
+                const C y0{__shfl(x0.re, j), __shfl(x0.im, j)};
+                const C y1{__shfl(x1.re, j), __shfl(x1.im, j)};
+                if (lane > j && lane < kb) {
+                    const C t = sT[lane + PB * j];
+                    x0 = zsub(x0, zmul(t, y0));
+                    x1 = zsub(x1, zmul(t, y1));
+                }
+            }
+        } else {
+            for (int j = kb - 1; j >= 0; --j) {
+                if (lane == j) {
+                    x0 = zmul(x0, sD[j]);
+                    x1 = zmul(x1, sD[j]);
+                }
+                const C y0{__shfl(x0.re, j), __shfl(x0.im, j)};
+                const C y1{__shfl(x1.re, j), __shfl(x1.im, j)};
+                if (lane < j) {
+                    const C t = sT[lane + PB * j];
+                    x0 = zsub(x0, zmul(t, y0));
+                    x1 = zsub(x1, zmul(t, y1));
+                }
             }
         }
-    } else {
-#pragma unroll
-        for (int j = NB - 1; j >= 0; --j) {
-            if (j < kb) {
-                x[j] = zmul(x[j], sD[j]);
-                const z2 xj = x[j];
-#pragma unroll
-                for (int i = 0; i < j; ++i) x[i] = zsub(x[i], zmul(sT[i + NB * j], xj));
-            }
+        if (lane < kb) {
+            B0[lane] = x0;
+            if (has1) B1[lane] = x1;
         }
     }
-#pragma unroll
-    for (int i = 0; i < NB; ++i)
-        if (i < kb) st(B + i, x[i]);
+}
+
+template <typename R, bool UPPER>
+int launch_trsm(const cx<R>* T, int64_t ldt, int64_t sT, int kb, cx<R>* B, int64_t ldb, int64_t sB, int ncols,
+                int batch, hipStream_t stream)
+{
+    constexpr int per_block = TRSM_WAVES * TRSM_CPW;
+    hipLaunchKernelGGL((trsm_kernel<R, UPPER>), dim3((ncols + per_block - 1) / per_block, batch),
+                       dim3(64 * TRSM_WAVES), 0, stream, T, ldt, sT, kb, B, ldb, sB, ncols);
+    return (int)hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
-// 4. C (M x N) += alpha * A (M x K) * B (K x N), complex128, column-major,
-//    batched by blockIdx.z.  Work-group tile 64 x 64, four wavefronts of
-//    32 x 32, each a 2 x 2 grid of 16 x 16 FP64 MFMA tiles; the complex
-//    product is four real MFMAs per tile pair (re*re - im*im, re*im + im*re).
-//    Operand fragments are read straight from global memory (L1/L2) in the
-//    MFMA lane layout: A[i][k] at lane (i + 16 k), B[k][j] at lane (j + 16 k);
-//    the accumulator holds D[4 (lane/16) + v][lane % 16], v = 0..3.
+// 4. C (M x N) = alpha * A (M x K) * B (K x N) + (beta_zero ? 0 : C), complex,
+//    column-major, batched by blockIdx.z.  Work-group tile 64 x 64, four
+//    wavefronts of 32 x 32, each a 2 x 2 grid of 16 x 16 MFMA tiles, operand
+//    fragments read straight from global memory; the complex product is four
+//    real MFMAs per tile pair (re*re - im*im, re*im + im*re).  Used for small
+//    M (forces, diagonal blocks).
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) zgemm_kernel(int M, int N, int K, double alpha, const cplx* __restrict__ Abase,
-                                                    int64_t lda, int64_t strideA, const cplx* __restrict__ Bbase,
-                                                    int64_t ldb, int64_t strideB, cplx* __restrict__ Cbase,
-                                                    int64_t ldc, int64_t strideC, int beta_zero)
+template <typename R>
+__global__ void __launch_bounds__(256) gemm_small_kernel(int M, int N, int K, R alpha, const cx<R>* __restrict__ Abase,
+                                                         int64_t lda, int64_t strideA, const cx<R>* __restrict__ Bbase,
+                                                         int64_t ldb, int64_t strideB, cx<R>* __restrict__ Cbase,
+                                                         int64_t ldc, int64_t strideC, int beta_zero)
 {
+    typedef typename Mfma<R>::acc acc;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int row0 = blockIdx.x * 64 + (wave & 1) * 32;
     const int col0 = blockIdx.y * 64 + (wave >> 1) * 32;
-    const cplx* A = Abase + (size_t)blockIdx.z * strideA;
-    const cplx* B = Bbase + (size_t)blockIdx.z * strideB;
-    cplx* C = Cbase + (size_t)blockIdx.z * strideC;
+    const cx<R>* A = Abase + (size_t)blockIdx.z * strideA;
+    const cx<R>* B = Bbase + (size_t)blockIdx.z * strideB;
+    cx<R>* C = Cbase + (size_t)blockIdx.z * strideC;
     const int li = lane & 15, lk = lane >> 4;
-
-    dvec4 cr[2][2], ci[2][2];
+    acc cr[2][2], ci[2][2];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b) {
-            cr[a][b] = dvec4{0.0, 0.0, 0.0, 0.0};
-            ci[a][b] = dvec4{0.0, 0.0, 0.0, 0.0};
+            cr[a][b] = acc{0, 0, 0, 0};
+            ci[a][b] = acc{0, 0, 0, 0};
         }
-    int arow[2], bcol[2];
-    bool aok[2], bok[2];
-#pragma unroll
-    for (int t = 0; t < 2; ++t) {
-        arow[t] = row0 + 16 * t + li;
-        bcol[t] = col0 + 16 * t + li;
-        aok[t] = arow[t] < M;
-        bok[t] = bcol[t] < N;
-    }
     for (int k = 0; k < K; k += 4) {
         const int kk = k + lk;
-        const bool kok = kk < K;
-        double ar[2], ai[2], br[2], bi[2];
+        R ar[2], ai[2], br[2], bi[2];
 #pragma unroll
         for (int t = 0; t < 2; ++t) {
-            if (aok[t] && kok) {
-                const cplx a = A[arow[t] + (size_t)kk * lda];
-                ar[t] = a.re;
-                ai[t] = a.im;
-            } else {
-                ar[t] = ai[t] = 0.0;
-            }
-            if (bok[t] && kok) {
-                const cplx b = B[kk + (size_t)bcol[t] * ldb];
-                br[t] = b.re;
-                bi[t] = b.im;
-            } else {
-                br[t] = bi[t] = 0.0;
-            }
+            const int row = row0 + 16 * t + li, col = col0 + 16 * t + li;
+            const cx<R> a = (row < M && kk < K) ? A[row + (size_t)kk * lda] : cx<R>{R(0), R(0)};
+            const cx<R> b = (col < N && kk < K) ? B[kk + (size_t)col * ldb] : cx<R>{R(0), R(0)};
+            ar[t] = a.re;
+            ai[t] = a.im;
+            br[t] = b.re;
+            bi[t] = b.im;
         }
 #pragma unroll
         for (int a = 0; a < 2; ++a)
 #pragma unroll
             for (int b = 0; b < 2; ++b) {
-                cr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], br[b], cr[a][b], 0, 0, 0);
-                cr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[a], bi[b], cr[a][b], 0, 0, 0);
-                ci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], bi[b], ci[a][b], 0, 0, 0);
-                ci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[a], br[b], ci[a][b], 0, 0, 0);
+                cr[a][b] = Mfma<R>::op(ar[a], br[b], cr[a][b]);
+                cr[a][b] = Mfma<R>::op(-ai[a], bi[b], cr[a][b]);
+                ci[a][b] = Mfma<R>::op(ar[a], bi[b], ci[a][b]);
+                ci[a][b] = Mfma<R>::op(ai[a], br[b], ci[a][b]);
             }
     }
 #pragma unroll
@@ -392,12 +476,12 @@ __global__ void __launch_bounds__(256) zgemm_kernel(int M, int N, int K, double 
         for (int b = 0; b < 2; ++b)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int row = row0 + 16 * a + 4 * v + lk;
+                const int row = row0 + 16 * a + Mfma<R>::row(lane, v);
                 const int col = col0 + 16 * b + li;
                 if (row < M && col < N) {
-                    cplx* c = C + row + (size_t)col * ldc;
-                    double re = beta_zero ? 0.0 : c->re;
-                    double im = beta_zero ? 0.0 : c->im;
+                    cx<R>* c = C + row + (size_t)col * ldc;
+                    R re = beta_zero ? R(0) : c->re;
+                    R im = beta_zero ? R(0) : c->im;
                     re += alpha * cr[a][b][v];
                     im += alpha * ci[a][b][v];
                     c->re = re;
@@ -406,109 +490,618 @@ __global__ void __launch_bounds__(256) zgemm_kernel(int M, int N, int K, double 
             }
 }
 
-template <int NT, int RPT, int SB>
-void launch_panel(cplx* A, int64_t lda, int64_t sA, int N, int k0, int kb, int* ipiv, int64_t sP, int* info,
-                  int batch, hipStream_t stream)
+// ---------------------------------------------------------------------------
+// 4c. Skinny product for few right-hand sides: C (M x N) = alpha A B +
+//     (beta_zero ? 0 : C) with N <= 2 SKN.  Work-group of four wavefronts on
+//     a 64-row x 16-column tile of C; the wavefronts split K in four, each
+//     streams its quarter of the 64-row slab of A (the next 16 columns in
+//     flight while the current ones feed the MFMAs), and the four partial
+//     tiles are summed through LDS.  The stream of A from HBM bounds it.
+// ---------------------------------------------------------------------------
+constexpr int SKM = 64, SKN = 16, SKK = 16;
+
+template <typename R>
+__global__ void __launch_bounds__(256) gemm_skinny_kernel(int M, int N, int K, R alpha,
+                                                          const cx<R>* __restrict__ Abase, int64_t lda,
+                                                          int64_t strideA, const cx<R>* __restrict__ Bbase,
+                                                          int64_t ldb, int64_t strideB, cx<R>* __restrict__ Cbase,
+                                                          int64_t ldc, int64_t strideC, int beta_zero)
 {
-    hipLaunchKernelGGL((panel_kernel<NT, RPT, SB>), dim3(batch), dim3(NT), 0, stream, A, lda, sA, N, k0, kb, ipiv,
-                       sP, info);
+    typedef cx<R> Cx;
+    typedef typename Mfma<R>::acc acc;
+    __shared__ Cx sC[4][SKM * SKN];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int row0 = blockIdx.x * SKM, col0 = blockIdx.y * SKN;
+    const Cx* A = Abase + (size_t)blockIdx.z * strideA;
+    const Cx* B = Bbase + (size_t)blockIdx.z * strideB;
+    Cx* C = Cbase + (size_t)blockIdx.z * strideC;
+    const int nchunks = (K + SKK - 1) / SKK;
+    const int per = (nchunks + 3) / 4;
+    const int kbeg = min(K, wave * per * SKK);
+    const int kend = min(K, kbeg + per * SKK);
+    const int bcol = col0 + li;
+    const bool b_ok = bcol < N;
+    acc cr[4], ci[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        cr[a] = acc{0, 0, 0, 0};
+        ci[a] = acc{0, 0, 0, 0};
+    }
+    Cx ra[4][4], rb[4];
+#define CPT_SKINNY_LOAD(kc_)                                                                  \
+    do {                                                                                      \
+        _Pragma("unroll") for (int ks = 0; ks < 4; ++ks)                                      \
+        {                                                                                     \
+            const int k = (kc_) + 4 * ks + lk;                                                \
+            _Pragma("unroll") for (int a = 0; a < 4; ++a)                                     \
+            {                                                                                 \
+                const int row = row0 + 16 * a + li;                                           \
+                if (k < kend && row < M) ra[ks][a] = A[row + (size_t)k * lda];                \
+                else ra[ks][a].re = ra[ks][a].im = R(0);                                      \
+            }                                                                                 \
+            if (k < kend && b_ok) rb[ks] = B[k + (size_t)bcol * ldb];                         \
+            else rb[ks].re = rb[ks].im = R(0);                                                \
+        }                                                                                     \
+    } while (0)
+
+    if (kbeg < kend) CPT_SKINNY_LOAD(kbeg);
+    for (int kc = kbeg; kc < kend; kc += SKK) {
+        Cx ca[4][4], cb[4];
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) {
+            cb[ks] = rb[ks];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) ca[ks][a] = ra[ks][a];
+        }
+        if (kc + SKK < kend) CPT_SKINNY_LOAD(kc + SKK);
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks)
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                cr[a] = Mfma<R>::op(ca[ks][a].re, cb[ks].re, cr[a]);
+                ci[a] = Mfma<R>::op(ca[ks][a].re, cb[ks].im, ci[a]);
+                cr[a] = Mfma<R>::op(-ca[ks][a].im, cb[ks].im, cr[a]);
+                ci[a] = Mfma<R>::op(ca[ks][a].im, cb[ks].re, ci[a]);
+            }
+    }
+#undef CPT_SKINNY_LOAD
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int r = 16 * a + Mfma<R>::row(lane, v);
+            sC[wave][r + SKM * li] = Cx{cr[a][v], ci[a][v]};
+        }
+    __syncthreads();
+    for (int e = threadIdx.x; e < SKM * SKN; e += 256) {
+        const int r = e % SKM, c = e / SKM;
+        const int row = row0 + r, col = col0 + c;
+        if (row < M && col < N) {
+            R re = (sC[0][e].re + sC[1][e].re) + (sC[2][e].re + sC[3][e].re);
+            R im = (sC[0][e].im + sC[1][e].im) + (sC[2][e].im + sC[3][e].im);
+            Cx* p = C + row + (size_t)col * ldc;
+            const R c_re = beta_zero ? R(0) : p->re;
+            const R c_im = beta_zero ? R(0) : p->im;
+            p->re = c_re + alpha * re;
+            p->im = c_im + alpha * im;
+        }
+    }
 }
 
-int panel_dispatch(cplx* A, int64_t lda, int64_t sA, int N, int k0, int kb, int* ipiv, int64_t sP, int* info,
-                   int batch, hipStream_t stream)
+// ---------------------------------------------------------------------------
+// 4b. Same product, LDS-tiled for large M: work-group tile 128 x 64, K staged
+//     in chunks of 16 through LDS (A as [k][row], B as [col][k] padded to 17
+//     so that both the stores and the MFMA fragment reads are free of bank
+//     conflicts), the next chunk prefetched into registers while the current
+//     one feeds the MFMAs.  Wavefront w computes rows 64 (w & 1) .. +64 and
+//     columns 32 (w >> 1) .. +32: a 4 x 2 grid of 16 x 16 tiles.
+// ---------------------------------------------------------------------------
+constexpr int TBM = 128, TBN = 64, TKC = 16;
+
+template <typename R>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+gemm_tiled_kernel(int M, int N, int K, R alpha, const cx<R>* __restrict__ Abase, int64_t lda, int64_t strideA,
+                  const cx<R>* __restrict__ Bbase, int64_t ldb, int64_t strideB, cx<R>* __restrict__ Cbase,
+                  int64_t ldc, int64_t strideC, int beta_zero)
+{
+    typedef cx<R> Cx;
+    typedef typename Mfma<R>::acc acc;
+    __shared__ Cx sA[TKC][TBM];
+    __shared__ Cx sB[TBN][TKC + 1];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int row0 = blockIdx.x * TBM, col0 = blockIdx.y * TBN;
+    const Cx* A = Abase + (size_t)blockIdx.z * strideA;
+    const Cx* B = Bbase + (size_t)blockIdx.z * strideB;
+    Cx* C = Cbase + (size_t)blockIdx.z * strideC;
+
+    Cx ra[8], rb[4];
+    const int a_row = row0 + (tid & 127);
+    const bool a_ok = a_row < M;
+    acc cr[4][2], ci[4][2];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            cr[a][b] = acc{0, 0, 0, 0};
+            ci[a][b] = acc{0, 0, 0, 0};
+        }
+
+#define CPT_LOAD_CHUNK(kc_)                                                                   \
+    do {                                                                                      \
+        _Pragma("unroll") for (int i = 0; i < 8; ++i)                                         \
+        {                                                                                     \
+            const int k = (kc_) + (tid >> 7) + 2 * i;                                         \
+            if (a_ok && k < K) ra[i] = A[a_row + (size_t)k * lda];                            \
+            else ra[i].re = ra[i].im = R(0);                                                  \
+        }                                                                                     \
+        _Pragma("unroll") for (int i = 0; i < 4; ++i)                                         \
+        {                                                                                     \
+            const int idx = tid + 256 * i;                                                    \
+            const int col = col0 + (idx >> 4), k = (kc_) + (idx & 15);                        \
+            if (col < N && k < K) rb[i] = B[k + (size_t)col * ldb];                           \
+            else rb[i].re = rb[i].im = R(0);                                                  \
+        }                                                                                     \
+    } while (0)
+
+    CPT_LOAD_CHUNK(0);
+    for (int kc = 0; kc < K; kc += TKC) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) sA[(tid >> 7) + 2 * i][tid & 127] = ra[i];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int idx = tid + 256 * i;
+            sB[idx >> 4][idx & 15] = rb[i];
+        }
+        __syncthreads();
+        if (kc + TKC < K) CPT_LOAD_CHUNK(kc + TKC);
+#pragma unroll
+        for (int ks = 0; ks < TKC; ks += 4) {
+            R ar[4], ai[4], br[2], bi[2];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                const Cx v = sA[ks + lk][wm * 64 + a * 16 + li];
+                ar[a] = v.re;
+                ai[a] = v.im;
+            }
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                const Cx v = sB[wn * 32 + b * 16 + li][ks + lk];
+                br[b] = v.re;
+                bi[b] = v.im;
+            }
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b) {
+                    cr[a][b] = Mfma<R>::op(ar[a], br[b], cr[a][b]);
+                    ci[a][b] = Mfma<R>::op(ar[a], bi[b], ci[a][b]);
+                }
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b) {
+                    cr[a][b] = Mfma<R>::op(-ai[a], bi[b], cr[a][b]);
+                    ci[a][b] = Mfma<R>::op(ai[a], br[b], ci[a][b]);
+                }
+        }
+        __syncthreads();
+    }
+#undef CPT_LOAD_CHUNK
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = row0 + wm * 64 + 16 * a + Mfma<R>::row(lane, v);
+                const int col = col0 + wn * 32 + 16 * b + li;
+                if (row < M && col < N) {
+                    Cx* c = C + row + (size_t)col * ldc;
+                    R re = beta_zero ? R(0) : c->re;
+                    R im = beta_zero ? R(0) : c->im;
+                    re += alpha * cr[a][b][v];
+                    im += alpha * ci[a][b][v];
+                    c->re = re;
+                    c->im = im;
+                }
+            }
+}
+
+template <typename R, int NT, int RPT, int SB>
+void launch_panel(cx<R>* A, int64_t lda, int64_t sA, int N, int k0, int kb, int sw_lo, int sw_hi, int* ipiv,
+                  int64_t sP, int* info, int batch, hipStream_t stream)
+{
+    hipLaunchKernelGGL((panel_kernel<R, NT, RPT, SB>), dim3(batch), dim3(NT), 0, stream, A, lda, sA, N, k0, kb, sw_lo,
+                       sw_hi, ipiv, sP, info);
+}
+
+// Factor columns [k0, k0 + kb) of rows [k0, N) with one work-group per
+// matrix; interchanges are applied to the columns [sw_lo, sw_hi).
+template <typename R>
+int panel_leaf(cx<R>* A, int64_t lda, int64_t sA, int N, int k0, int kb, int sw_lo, int sw_hi, int* ipiv, int64_t sP,
+               int* info, int batch, hipStream_t stream)
 {
     // 512 threads (two wavefronts per SIMD, up to 256 VGPRs each); the
     // register-resident sub-block holds RPT rows x SB columns per thread,
-    // RPT * SB <= 48 complex values.
+    // RPT * SB <= 32 complex values up to 8192 rows.
     const int M = N - k0;
-    if (M <= 512) launch_panel<512, 1, 8>(A, lda, sA, N, k0, kb, ipiv, sP, info, batch, stream);
-    else if (M <= 1024) launch_panel<512, 2, 8>(A, lda, sA, N, k0, kb, ipiv, sP, info, batch, stream);
-    else if (M <= 2048) launch_panel<512, 4, 8>(A, lda, sA, N, k0, kb, ipiv, sP, info, batch, stream);
-    else if (M <= 3072) launch_panel<512, 6, 8>(A, lda, sA, N, k0, kb, ipiv, sP, info, batch, stream);
-    else if (M <= 4096) launch_panel<512, 8, 4>(A, lda, sA, N, k0, kb, ipiv, sP, info, batch, stream);
-    else if (M <= 6144) launch_panel<512, 12, 4>(A, lda, sA, N, k0, kb, ipiv, sP, info, batch, stream);
-    else if (M <= 8192) launch_panel<512, 16, 2>(A, lda, sA, N, k0, kb, ipiv, sP, info, batch, stream);
-    else if (M <= 12288) launch_panel<512, 24, 2>(A, lda, sA, N, k0, kb, ipiv, sP, info, batch, stream);
-    else if (M <= 16384) launch_panel<512, 32, 1>(A, lda, sA, N, k0, kb, ipiv, sP, info, batch, stream);
-    else if (M <= 24576) launch_panel<512, 48, 1>(A, lda, sA, N, k0, kb, ipiv, sP, info, batch, stream);
+    if (M <= 512) launch_panel<R, 512, 1, 8>(A, lda, sA, N, k0, kb, sw_lo, sw_hi, ipiv, sP, info, batch, stream);
+    else if (M <= 1024) launch_panel<R, 512, 2, 8>(A, lda, sA, N, k0, kb, sw_lo, sw_hi, ipiv, sP, info, batch, stream);
+    else if (M <= 2048) launch_panel<R, 512, 4, 8>(A, lda, sA, N, k0, kb, sw_lo, sw_hi, ipiv, sP, info, batch, stream);
+    else if (M <= 4096) launch_panel<R, 512, 8, 4>(A, lda, sA, N, k0, kb, sw_lo, sw_hi, ipiv, sP, info, batch, stream);
+    else if (M <= 8192) launch_panel<R, 512, 16, 2>(A, lda, sA, N, k0, kb, sw_lo, sw_hi, ipiv, sP, info, batch, stream);
+    else if (M <= 16384) launch_panel<R, 512, 32, 1>(A, lda, sA, N, k0, kb, sw_lo, sw_hi, ipiv, sP, info, batch, stream);
+    else if (M <= 24576) launch_panel<R, 512, 48, 1>(A, lda, sA, N, k0, kb, sw_lo, sw_hi, ipiv, sP, info, batch, stream);
     else return (int)hipErrorInvalidValue;
     return (int)hipGetLastError();
 }
 
-}  // namespace
+template <typename R>
+int gemm(int M, int N, int K, R alpha, const cx<R>* A, int64_t lda, int64_t sA, const cx<R>* B, int64_t ldb,
+         int64_t sB, cx<R>* C, int64_t ldc, int64_t sC, int beta_zero, int batch, hipStream_t stream);
 
-int launch_zgemm(int M, int N, int K, double alpha, const cplx* A, int64_t lda, int64_t sA, const cplx* B,
-                 int64_t ldb, int64_t sB, cplx* C, int64_t ldc, int64_t sC, int beta_zero, int batch,
-                 hipStream_t stream)
+// Columns narrower than this are factored by one work-group per matrix.
+constexpr int PANEL_LEAF = 16;
+
+// Recursive factorisation of columns [c0, c0 + w) of rows [c0, N) inside the
+// panel [sw_lo, sw_hi): factor the left half, solve its unit-lower triangle
+// against the top of the right half (trsm), update the rest of the right
+// half with an MFMA product spread over the whole device, factor the right
+// half.  The one-work-group leaves stream only PANEL_LEAF columns.
+template <typename R>
+int panel_rec(cx<R>* A, int64_t lda, int64_t sA, int N, int c0, int w, int sw_lo, int sw_hi, int* ipiv, int64_t sP,
+              int* info, int batch, hipStream_t stream)
+{
+    const int M = N - c0;
+    if (w <= PANEL_LEAF || M <= 2 * PANEL_LEAF)
+        return panel_leaf<R>(A, lda, sA, N, c0, w, sw_lo, sw_hi, ipiv, sP, info, batch, stream);
+    const int h = w / 2;
+    int e;
+    if ((e = panel_rec<R>(A, lda, sA, N, c0, h, sw_lo, sw_hi, ipiv, sP, info, batch, stream))) return e;
+    cx<R>* A11 = A + c0 + (size_t)c0 * lda;
+    cx<R>* A12 = A + c0 + (size_t)(c0 + h) * lda;
+    cx<R>* A21 = A + (c0 + h) + (size_t)c0 * lda;
+    cx<R>* A22 = A + (c0 + h) + (size_t)(c0 + h) * lda;
+    if ((e = launch_trsm<R, false>(A11, lda, sA, h, A12, lda, sA, w - h, batch, stream))) return e;
+    if (M > h && (e = gemm<R>(M - h, w - h, h, R(-1), A21, lda, sA, A12, lda, sA, A22, lda, sA, 0, batch, stream)))
+        return e;
+    return panel_rec<R>(A, lda, sA, N, c0 + h, w - h, sw_lo, sw_hi, ipiv, sP, info, batch, stream);
+}
+
+// LU with partial pivoting of the panel columns [k0, k0 + kb), rows [k0, N);
+// interchanges applied within the panel columns only.
+template <typename R>
+int panel_dispatch(cx<R>* A, int64_t lda, int64_t sA, int N, int k0, int kb, int* ipiv, int64_t sP, int* info,
+                   int batch, hipStream_t stream)
+{
+    const int M = N - k0;
+    // complex LU of an M x kb panel: 4 (M kb^2 - kb^3 / 3) real flops
+    TimedLaunch timed(stream, sizeof(R) == 8 ? TK_PANEL_Z : TK_PANEL_C,
+                      4.0 * ((double)M * kb * kb - (double)kb * kb * kb / 3.0) * batch);
+    return panel_rec<R>(A, lda, sA, N, k0, kb, k0, k0 + kb, ipiv, sP, info, batch, stream);
+}
+
+template <typename R>
+int gemm(int M, int N, int K, R alpha, const cx<R>* A, int64_t lda, int64_t sA, const cx<R>* B, int64_t ldb,
+         int64_t sB, cx<R>* C, int64_t ldc, int64_t sC, int beta_zero, int batch, hipStream_t stream)
 {
     if (M <= 0 || N <= 0 || batch <= 0) return 0;
-    dim3 grid((M + 63) / 64, (N + 63) / 64, batch);
-    hipLaunchKernelGGL(zgemm_kernel, grid, dim3(256), 0, stream, M, N, K, alpha, A, lda, sA, B, ldb, sB, C, ldc, sC,
-                       beta_zero);
+    if (N <= 2 * SKN && M >= SKM && K >= 4 * SKK) {
+        dim3 grid((M + SKM - 1) / SKM, (N + SKN - 1) / SKN, batch);
+        TimedLaunch timed(stream, sizeof(R) == 8 ? TK_SKINNY_Z : TK_SKINNY_C,
+                          (double)sizeof(cx<R>) * ((double)M * K + (double)K * N + (beta_zero ? 1.0 : 2.0) * M * N) *
+                              batch);
+        hipLaunchKernelGGL(gemm_skinny_kernel<R>, grid, dim3(256), 0, stream, M, N, K, alpha, A, lda, sA, B, ldb, sB,
+                           C, ldc, sC, beta_zero);
+    } else if (M >= 96 && K >= 16) {
+        dim3 grid((M + TBM - 1) / TBM, (N + TBN - 1) / TBN, batch);
+        TimedLaunch timed(stream, sizeof(R) == 8 ? TK_ZGEMM : TK_CGEMM, 8.0 * M * N * K * batch);
+        hipLaunchKernelGGL(gemm_tiled_kernel<R>, grid, dim3(256), 0, stream, M, N, K, alpha, A, lda, sA, B, ldb, sB, C,
+                           ldc, sC, beta_zero);
+    } else {
+        dim3 grid((M + 63) / 64, (N + 63) / 64, batch);
+        hipLaunchKernelGGL(gemm_small_kernel<R>, grid, dim3(256), 0, stream, M, N, K, alpha, A, lda, sA, B, ldb, sB, C,
+                           ldc, sC, beta_zero);
+    }
     return (int)hipGetLastError();
 }
 
-int launch_getrf(cplx* A, int N, int64_t lda, int64_t strideA, int* ipiv, int64_t strideP, int* info, int batch,
-                 hipStream_t stream)
+// L^-1 B in place for a unit lower kb x kb block L, kb <= 2 PB: one or two
+// wavefront-wide triangles with the product between them.
+template <typename R>
+int trsm_lower(const cx<R>* L, int64_t lda, int64_t sA, int kb, cx<R>* B, int64_t ldb, int64_t sB, int ncols,
+               int batch, hipStream_t stream)
+{
+    const int k1 = min(kb, PB);
+    int e;
+    if ((e = launch_trsm<R, false>(L, lda, sA, k1, B, ldb, sB, ncols, batch, stream))) return e;
+    if (kb == k1) return 0;
+    if ((e = gemm<R>(kb - k1, ncols, k1, R(-1), L + k1, lda, sA, B, ldb, sB, B + k1, ldb, sB, 0, batch, stream)))
+        return e;
+    return launch_trsm<R, false>(L + k1 + (size_t)k1 * lda, lda, sA, kb - k1, B + k1, ldb, sB, ncols, batch, stream);
+}
+
+// Block width of the right-looking factorisation: 128 columns halve the
+// trailing-matrix traffic per flop against 64 where the matrices are large.
+inline int getrf_block(int N) { return N >= 1024 ? 2 * PB : PB; }
+
+template <typename R>
+size_t getrf_ws_elems(int, int)
+{
+    return 0;
+}
+
+// Blocked right-looking LU with partial pivoting (LAPACK zgetrf semantics),
+// all matrices of the batch at once.  Per block step k0 of width kb:
+// interchanges of the panel applied to the other columns, U12 = L11^-1 A12
+// in place, A22 -= A21 U12 on MFMA.  With a side stream, the next panel's
+// columns are updated first and factored on the side stream while the rest
+// of A22 is updated (look-ahead of depth one).
+template <typename R>
+int getrf(cx<R>* A, int N, int64_t lda, int64_t strideA, int* ipiv, int64_t strideP, int* info, int batch,
+          cx<R>*, const LuStreams* aux, hipStream_t stream)
 {
     if (N <= 0 || batch <= 0) return 0;
     int e;
-    for (int k0 = 0; k0 < N; k0 += NB) {
-        const int kb = min(NB, N - k0);
-        if ((e = panel_dispatch(A, lda, strideA, N, k0, kb, ipiv, strideP, info, batch, stream))) return e;
-        const dim3 gs((N + 255) / 256, batch);
-        hipLaunchKernelGGL(laswp_kernel, gs, dim3(256), 0, stream, A, lda, strideA, N, k0, k0 + kb, ipiv, strideP, k0,
-                           k0 + kb);
+    const int nb = getrf_block(N);
+    const bool ahead = aux && aux->side;
+    if ((e = panel_dispatch<R>(A, lda, strideA, N, 0, min(nb, N), ipiv, strideP, info, batch, stream))) return e;
+    for (int k0 = 0; k0 < N; k0 += nb) {
+        const int kb = min(nb, N - k0);
+        // panel k0 is factored (on `stream`, or on the side stream joined below)
+        hipLaunchKernelGGL(laswp_kernel<R>, dim3((N + 255) / 256, batch), dim3(256), 0, stream, A, lda, strideA, N, k0,
+                           k0 + kb, ipiv, strideP, k0, k0 + kb);
         if ((e = (int)hipGetLastError())) return e;
         const int rest = N - k0 - kb;
-        if (rest > 0) {
-            cplx* A11 = A + k0 + (size_t)k0 * lda;
-            cplx* A12 = A + k0 + (size_t)(k0 + kb) * lda;
-            cplx* A21 = A + (k0 + kb) + (size_t)k0 * lda;
-            cplx* A22 = A + (k0 + kb) + (size_t)(k0 + kb) * lda;
-            hipLaunchKernelGGL(trsm_kernel<false>, dim3((rest + 255) / 256, batch), dim3(256), 0, stream, A11, lda,
-                               strideA, kb, A12, lda, strideA, rest);
-            if ((e = (int)hipGetLastError())) return e;
-            if ((e = launch_zgemm(rest, rest, kb, -1.0, A21, lda, strideA, A12, lda, strideA, A22, lda, strideA, 0,
-                                  batch, stream)))
+        if (rest <= 0) break;
+        cx<R>* A11 = A + k0 + (size_t)k0 * lda;
+        cx<R>* A12 = A + k0 + (size_t)(k0 + kb) * lda;
+        cx<R>* A21 = A + (k0 + kb) + (size_t)k0 * lda;
+        cx<R>* A22 = A + (k0 + kb) + (size_t)(k0 + kb) * lda;
+        if ((e = trsm_lower<R>(A11, lda, strideA, kb, A12, lda, strideA, rest, batch, stream))) return e;
+        const int kn = min(nb, rest);  // width of the next panel
+        // trailing update of the next panel's columns first ...
+        if ((e = gemm<R>(rest, kn, kb, R(-1), A21, lda, strideA, A12, lda, strideA, A22, lda, strideA, 0, batch,
+                         stream)))
+            return e;
+        if (ahead) {
+            // ... factor the next panel on the side stream while the rest of
+            // the trailing matrix is updated on `stream` (disjoint columns)
+            if (hipEventRecord(aux->ready, stream) != hipSuccess) return (int)hipErrorUnknown;
+            if (hipStreamWaitEvent(aux->side, aux->ready, 0) != hipSuccess) return (int)hipErrorUnknown;
+            if ((e = panel_dispatch<R>(A, lda, strideA, N, k0 + kb, kn, ipiv, strideP, info, batch, aux->side)))
                 return e;
+            if (hipEventRecord(aux->done, aux->side) != hipSuccess) return (int)hipErrorUnknown;
+        }
+        if (rest > kn && (e = gemm<R>(rest, rest - kn, kb, R(-1), A21, lda, strideA, A12 + (size_t)kn * lda, lda,
+                                      strideA, A22 + (size_t)kn * lda, lda, strideA, 0, batch, stream)))
+            return e;
+        if (ahead) {
+            if (hipStreamWaitEvent(stream, aux->done, 0) != hipSuccess) return (int)hipErrorUnknown;
+        } else if ((e = panel_dispatch<R>(A, lda, strideA, N, k0 + kb, kn, ipiv, strideP, info, batch, stream))) {
+            return e;
         }
     }
     return 0;
 }
 
-int launch_getrs(const cplx* LU, int N, int64_t lda, int64_t strideA, const int* ipiv, int64_t strideP, cplx* B,
-                 int nrhs, int64_t ldb, int64_t strideB, int batch, hipStream_t stream)
+template <typename R>
+int getrs(const cx<R>* LU, int N, int64_t lda, int64_t strideA, const int* ipiv, int64_t strideP, cx<R>* B, int nrhs,
+          int64_t ldb, int64_t strideB, int batch, hipStream_t stream)
 {
     if (N <= 0 || nrhs <= 0 || batch <= 0) return 0;
     int e;
-    hipLaunchKernelGGL(laswp_kernel, dim3((nrhs + 255) / 256, batch), dim3(256), 0, stream, B, ldb, strideB, nrhs, 0,
+    hipLaunchKernelGGL(laswp_kernel<R>, dim3((nrhs + 255) / 256, batch), dim3(256), 0, stream, B, ldb, strideB, nrhs, 0,
                        0, ipiv, strideP, 0, N);
     if ((e = (int)hipGetLastError())) return e;
     // forward substitution with the unit lower factor
-    for (int k0 = 0; k0 < N; k0 += NB) {
-        const int kb = min(NB, N - k0);
-        hipLaunchKernelGGL(trsm_kernel<false>, dim3((nrhs + 255) / 256, batch), dim3(256), 0, stream,
-                           LU + k0 + (size_t)k0 * lda, lda, strideA, kb, B + k0, ldb, strideB, nrhs);
-        if ((e = (int)hipGetLastError())) return e;
+    for (int k0 = 0; k0 < N; k0 += PB) {
+        const int kb = min(PB, N - k0);
+        if ((e = launch_trsm<R, false>(LU + k0 + (size_t)k0 * lda, lda, strideA, kb, B + k0, ldb, strideB, nrhs,
+                                       batch, stream)))
+            return e;
         const int rest = N - k0 - kb;
-        if (rest > 0 &&
-            (e = launch_zgemm(rest, nrhs, kb, -1.0, LU + (k0 + kb) + (size_t)k0 * lda, lda, strideA, B + k0, ldb,
-                              strideB, B + k0 + kb, ldb, strideB, 0, batch, stream)))
+        if (rest > 0 && (e = gemm<R>(rest, nrhs, kb, R(-1), LU + (k0 + kb) + (size_t)k0 * lda, lda, strideA, B + k0,
+                                     ldb, strideB, B + k0 + kb, ldb, strideB, 0, batch, stream)))
             return e;
     }
     // backward substitution with the upper factor
-    const int nblk = (N + NB - 1) / NB;
+    const int nblk = (N + PB - 1) / PB;
     for (int blk = nblk - 1; blk >= 0; --blk) {
-        const int k0 = blk * NB;
-        const int kb = min(NB, N - k0);
-        hipLaunchKernelGGL(trsm_kernel<true>, dim3((nrhs + 255) / 256, batch), dim3(256), 0, stream,
-                           LU + k0 + (size_t)k0 * lda, lda, strideA, kb, B + k0, ldb, strideB, nrhs);
-        if ((e = (int)hipGetLastError())) return e;
-        if (k0 > 0 && (e = launch_zgemm(k0, nrhs, kb, -1.0, LU + (size_t)k0 * lda, lda, strideA, B + k0, ldb, strideB,
-                                        B, ldb, strideB, 0, batch, stream)))
+        const int k0 = blk * PB;
+        const int kb = min(PB, N - k0);
+        if ((e = launch_trsm<R, true>(LU + k0 + (size_t)k0 * lda, lda, strideA, kb, B + k0, ldb, strideB, nrhs,
+                                      batch, stream)))
+            return e;
+        if (k0 > 0 && (e = gemm<R>(k0, nrhs, kb, R(-1), LU + (size_t)k0 * lda, lda, strideA, B + k0, ldb, strideB, B,
+                                   ldb, strideB, 0, batch, stream)))
             return e;
     }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Mixed precision: complex128 <-> complex64 conversions, x += d, and the
+// per-column squared norms used by the refinement's stopping test.
+// ---------------------------------------------------------------------------
+__global__ void z2c_kernel(const cx<double>* __restrict__ src, int64_t lds, int64_t strideS, cx<float>* __restrict__ dst,
+                           int64_t ldd, int64_t strideD, int rows, int cols)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x, c = blockIdx.y;
+    if (r >= rows) return;
+    const cx<double> v = src[(size_t)blockIdx.z * strideS + r + (size_t)c * lds];
+    dst[(size_t)blockIdx.z * strideD + r + (size_t)c * ldd] = cx<float>{(float)v.re, (float)v.im};
+}
+
+__global__ void c2z_add_kernel(const cx<float>* __restrict__ src, int64_t lds, int64_t strideS,
+                               cx<double>* __restrict__ dst, int64_t ldd, int64_t strideD, int rows, int cols,
+                               int accumulate)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x, c = blockIdx.y;
+    if (r >= rows) return;
+    const cx<float> v = src[(size_t)blockIdx.z * strideS + r + (size_t)c * lds];
+    cx<double>* d = dst + (size_t)blockIdx.z * strideD + r + (size_t)c * ldd;
+    if (accumulate) {
+        d->re += (double)v.re;
+        d->im += (double)v.im;
+    } else {
+        d->re = (double)v.re;
+        d->im = (double)v.im;
+    }
+}
+
+__global__ void copy_z_kernel(const cx<double>* __restrict__ src, int64_t lds, int64_t strideS,
+                              cx<double>* __restrict__ dst, int64_t ldd, int64_t strideD, int rows)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x, c = blockIdx.y;
+    if (r >= rows) return;
+    dst[(size_t)blockIdx.z * strideD + r + (size_t)c * ldd] = src[(size_t)blockIdx.z * strideS + r + (size_t)c * lds];
+}
+
+__global__ void zero_ints_kernel(int* __restrict__ p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// out[b * cols + c] = sum_r |a(r, c)|^2 for one column per work-group.
+__global__ void colnorm2_kernel(const cx<double>* __restrict__ a, int64_t lda, int64_t strideA, int rows, int cols,
+                                double* __restrict__ out)
+{
+    __shared__ double part[256];
+    const int c = blockIdx.x, b = blockIdx.y;
+    const cx<double>* col = a + (size_t)b * strideA + (size_t)c * lda;
+    double s = 0.0;
+    for (int r = threadIdx.x; r < rows; r += blockDim.x) s += col[r].re * col[r].re + col[r].im * col[r].im;
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[(size_t)b * cols + c] = part[0];
+}
+
+}  // namespace
+
+// --------------------------- entry points ---------------------------------
+int launch_zgemm(int M, int N, int K, double alpha, const cplx* A, int64_t lda, int64_t sA, const cplx* B,
+                 int64_t ldb, int64_t sB, cplx* C, int64_t ldc, int64_t sC, int beta_zero, int batch,
+                 hipStream_t stream)
+{
+    return gemm<double>(M, N, K, alpha, (const cx<double>*)A, lda, sA, (const cx<double>*)B, ldb, sB, (cx<double>*)C,
+                        ldc, sC, beta_zero, batch, stream);
+}
+
+size_t getrf_workspace_bytes(int N, int batch) { return sizeof(cplx) * getrf_ws_elems<double>(N, batch); }
+
+int launch_getrf(cplx* A, int N, int64_t lda, int64_t strideA, int* ipiv, int64_t strideP, int* info, int batch,
+                 cplx* work, const LuStreams* aux, hipStream_t stream)
+{
+    return getrf<double>((cx<double>*)A, N, lda, strideA, ipiv, strideP, info, batch, (cx<double>*)work, aux, stream);
+}
+
+int launch_getrs(const cplx* LU, int N, int64_t lda, int64_t strideA, const int* ipiv, int64_t strideP, cplx* B,
+                 int nrhs, int64_t ldb, int64_t strideB, int batch, hipStream_t stream)
+{
+    return getrs<double>((const cx<double>*)LU, N, lda, strideA, ipiv, strideP, (cx<double>*)B, nrhs, ldb, strideB,
+                         batch, stream);
+}
+
+// Workspace of the mixed solve, in bytes: complex64 factors (batch n^2),
+// pivots and info, FP64 residuals (batch n nrhs), complex64 corrections,
+// the complex64 getrf workspace, and the norm buffers.
+size_t mixed_workspace_bytes(int n, int nrhs, int batch)
+{
+    const size_t b = (size_t)batch;
+    size_t bytes = 0;
+    bytes += sizeof(cx<float>) * b * n * n;                           // K32
+    bytes += sizeof(int) * (b * n + b);                               // ipiv, info
+    bytes = (bytes + 255) & ~(size_t)255;
+    bytes += sizeof(cx<double>) * b * n * nrhs;                       // residual
+    bytes += sizeof(cx<float>) * b * n * nrhs;                        // correction
+    bytes = (bytes + 255) & ~(size_t)255;
+    bytes += sizeof(cx<float>) * getrf_ws_elems<float>(n, batch);     // getrf workspace
+    bytes += 2 * sizeof(double) * b * nrhs;                           // norms
+    return bytes + 1024;
+}
+
+// Mixed-precision solve of K X = B (complex128 K, B, X): K is rounded to
+// complex64 and factorised on FP32 MFMA; X is refined with FP64 residuals
+// r = B - K X (FP64 MFMA GEMM against the unrounded K) and complex64
+// corrections until max_j ||r_j|| / ||b_j|| <= tol or max_iter sweeps.
+// Synchronises the stream once per sweep to read the residual norms.
+int launch_mixed_solve(const cplx* K, int n, int64_t lda, int64_t strideK, int batch, const cplx* B, int nrhs,
+                       int64_t ldb, int64_t strideB, cplx* X, int64_t ldx, int64_t strideX, int max_iter, double tol,
+                       void* work, const LuStreams* aux, int* iterations, double* residual, hipStream_t stream)
+{
+    typedef cx<float> c64;
+    typedef cx<double> c128;
+    const size_t b = (size_t)batch;
+    char* p = (char*)work;
+    c64* K32 = (c64*)p;
+    p += sizeof(c64) * b * n * n;
+    int* ipiv = (int*)p;
+    p += sizeof(int) * b * n;
+    int* info = (int*)p;
+    p += sizeof(int) * b;
+    p = (char*)(((uintptr_t)p + 255) & ~(uintptr_t)255);
+    c128* Rz = (c128*)p;
+    p += sizeof(c128) * b * n * nrhs;
+    c64* D = (c64*)p;
+    p += sizeof(c64) * b * n * nrhs;
+    p = (char*)(((uintptr_t)p + 255) & ~(uintptr_t)255);
+    c64* gws = (c64*)p;
+    p += sizeof(c64) * getrf_ws_elems<float>(n, batch);
+    double* nb2 = (double*)p;
+    double* nr2 = nb2 + b * nrhs;
+    const int64_t sK32 = (int64_t)n * n, sRD = (int64_t)n * nrhs;
+    int e;
+    const dim3 blk(256);
+    const dim3 gK((n + 255) / 256, n, batch), gB((n + 255) / 256, nrhs, batch);
+    hipLaunchKernelGGL(z2c_kernel, gK, blk, 0, stream, (const c128*)K, lda, strideK, K32, (int64_t)n, sK32, n, n);
+    hipLaunchKernelGGL(zero_ints_kernel, dim3((batch + 255) / 256), blk, 0, stream, info, batch);
+    if ((e = getrf<float>(K32, n, n, sK32, ipiv, n, info, batch, gws, aux, stream))) return e;
+    // x0 = LU32 \ b
+    hipLaunchKernelGGL(z2c_kernel, gB, blk, 0, stream, (const c128*)B, ldb, strideB, D, (int64_t)n, sRD, n, nrhs);
+    if ((e = getrs<float>(K32, n, n, sK32, ipiv, n, D, nrhs, n, sRD, batch, stream))) return e;
+    hipLaunchKernelGGL(c2z_add_kernel, gB, blk, 0, stream, D, (int64_t)n, sRD, (c128*)X, ldx, strideX, n, nrhs, 0);
+    hipLaunchKernelGGL(colnorm2_kernel, dim3(nrhs, batch), blk, 0, stream, (const c128*)B, ldb, strideB, n, nrhs, nb2);
+    if ((e = (int)hipGetLastError())) return e;
+    std::vector<double> hb(b * nrhs), hr(b * nrhs);
+    if (hipMemcpyAsync(hb.data(), nb2, sizeof(double) * b * nrhs, hipMemcpyDeviceToHost, stream) != hipSuccess)
+        return (int)hipErrorUnknown;
+    double worst = INFINITY;
+    int it = 0;
+    for (;;) {
+        // r = b - K x  (FP64)
+        hipLaunchKernelGGL(copy_z_kernel, gB, blk, 0, stream, (const c128*)B, ldb, strideB, Rz, (int64_t)n, sRD, n);
+        if ((e = gemm<double>(n, nrhs, n, -1.0, (const c128*)K, lda, strideK, (const c128*)X, ldx, strideX, Rz, n, sRD,
+                              0, batch, stream)))
+            return e;
+        hipLaunchKernelGGL(colnorm2_kernel, dim3(nrhs, batch), blk, 0, stream, Rz, (int64_t)n, sRD, n, nrhs, nr2);
+        if (hipMemcpyAsync(hr.data(), nr2, sizeof(double) * b * nrhs, hipMemcpyDeviceToHost, stream) != hipSuccess)
+            return (int)hipErrorUnknown;
+        if (hipStreamSynchronize(stream) != hipSuccess) return (int)hipErrorUnknown;
+        worst = 0.0;
+        for (size_t i = 0; i < b * nrhs; ++i) {
+            const double rel = hb[i] > 0 ? std::sqrt(hr[i] / hb[i]) : std::sqrt(hr[i]);
+            worst = std::isnan(rel) ? INFINITY : std::max(worst, rel);
+        }
+        if (worst <= tol || it >= max_iter) break;
+        // x += LU32 \ r
+        hipLaunchKernelGGL(z2c_kernel, gB, blk, 0, stream, Rz, (int64_t)n, sRD, D, (int64_t)n, sRD, n, nrhs);
+        if ((e = getrs<float>(K32, n, n, sK32, ipiv, n, D, nrhs, n, sRD, batch, stream))) return e;
+        hipLaunchKernelGGL(c2z_add_kernel, gB, blk, 0, stream, D, (int64_t)n, sRD, (c128*)X, ldx, strideX, n, nrhs, 1);
+        if ((e = (int)hipGetLastError())) return e;
+        ++it;
+    }
+    *iterations = it;
+    *residual = worst;
     return 0;
 }
 

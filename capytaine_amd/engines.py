@@ -146,6 +146,44 @@ class LUDecomposedDeviceMatrix:
         return x
 
 
+class MixedPrecisionSolver:
+    """Complex64 LU on FP32 MFMA with FP64 iterative refinement against the
+    FP64 matrix (cpt_lu_solve_mixed).  Keeps A; factorises at every solve
+    call (the factor lives in the library workspace)."""
+
+    def __init__(self, A: DeviceMatrix, max_iter=10, tol=1e-12):
+        self.A = A
+        self.max_iter = max_iter
+        self.tol = tol
+        self.iterations = None
+        self.residual = None
+
+    shape = property(lambda self: self.A.shape)
+    dtype = property(lambda self: self.A.dtype)
+
+    def solve_device(self, b):
+        import ctypes
+        import torch
+        nb, nrhs, n = b.shape
+        x = torch.empty_like(b)
+        it, res = ctypes.c_int(), ctypes.c_double()
+        ctx = _lib.Context.get(b.device)
+        check(_lib.load().cpt_lu_solve_mixed(ctx.handle, ptr(self.A.data), n, n, n * n, nb, ptr(b), nrhs, n, n * nrhs,
+                                             ptr(x), n, n * nrhs, self.max_iter, self.tol, ctypes.byref(it),
+                                             ctypes.byref(res), stream_handle(b.device)), "cpt_lu_solve_mixed")
+        self.iterations, self.residual = it.value, res.value
+        return x
+
+    def solve(self, b):
+        import torch
+        b = np.asarray(b)
+        vec = b.ndim == 1
+        bb = b.reshape(b.shape[0], -1)
+        bd = torch.as_tensor(np.ascontiguousarray(bb.T), dtype=torch.complex128).to(self.A.device)[None]
+        x = self.solve_device(bd).cpu().numpy()[0].T
+        return x[:, 0] if vec else x
+
+
 def has_been_lu_decomposed(A):
     return isinstance(A, LUDecomposedDeviceMatrix)
 
@@ -174,7 +212,8 @@ class DefaultMatrixEngine:
     ----------
     green_function: Delhommeau (GPU), default: Delhommeau()
     linear_solver: "lu_decomposition" (default), "lu_decomposition_with_overwrite",
-        or a callable (A, b) -> x that receives numpy arrays.
+        "mixed_precision" (complex64 LU + FP64 refinement), or a callable
+        (A, b) -> x that receives numpy arrays.
     check_nan: scan the matrices for NaN after assembly (engines.py:255).
     """
 
@@ -255,6 +294,14 @@ class DefaultMatrixEngine:
             else:
                 luA = A
             return luA.solve(b)
+        if self._linear_solver == "mixed_precision":
+            if not isinstance(A, MixedPrecisionSolver):
+                solver = MixedPrecisionSolver(A)
+                if self.last_computed_matrices is not None and A is self.last_computed_matrices[1]:
+                    self.last_computed_matrices = (self.last_computed_matrices[0], solver)
+            else:
+                solver = A
+            return solver.solve(b)
         raise NotImplementedError(f"Unknown `linear_solver` in DefaultMatrixEngine: {self._linear_solver}")
 
     def compute_ram_estimation(self, problem):

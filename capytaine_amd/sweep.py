@@ -21,7 +21,8 @@ from ._lib import check, ptr, stream_handle
 
 
 class FrequencySweep:
-    def __init__(self, green_function, body, wavenumbers, nrhs, *, free_surface=0.0, water_depth=np.inf):
+    def __init__(self, green_function, body, wavenumbers, nrhs, *, free_surface=0.0, water_depth=np.inf,
+                 precision="fp64", max_iter=10, tol=1e-12):
         import torch
         self.gf = green_function
         self.body = body
@@ -33,6 +34,13 @@ class FrequencySweep:
         self.nrhs = nrhs
         self.free_surface = free_surface
         self.water_depth = water_depth
+        if precision not in ("fp64", "mixed"):
+            raise ValueError(f"precision must be 'fp64' or 'mixed', got {precision!r}")
+        self.precision = precision
+        self.max_iter = max_iter
+        self.tol = tol
+        self.iterations = 0
+        self.residual = 0.0
         n, nb = self.n, self.nb
         cplx = torch.complex128
         self.S = torch.empty((nb, n, n), dtype=cplx, device=self.dev)
@@ -59,6 +67,10 @@ class FrequencySweep:
                                    diagonal_term_in_double_layer=True, out=(self.S, self.K))
 
     def factor(self):
+        """FP64: LU of K in place.  Mixed: nothing here (the complex64 factor
+        is computed inside the mixed solve, K is kept for the residuals)."""
+        if self.precision == "mixed":
+            return
         n = self.n
         check(_lib.load().cpt_lu_factor(self.ctx.handle, ptr(self.K), n, n, n * n, ptr(self.ipiv), n,
                                         ptr(self.info), self.nb, self.stream), "cpt_lu_factor")
@@ -66,8 +78,16 @@ class FrequencySweep:
     def solve(self, bc):
         """bc: device tensor (nb, nrhs, n) of boundary conditions (not modified)."""
         n = self.n
-        self.X.copy_(bc)
         lib = _lib.load()
+        if self.precision == "mixed":
+            import ctypes
+            it, res = ctypes.c_int(), ctypes.c_double()
+            check(lib.cpt_lu_solve_mixed(self.ctx.handle, ptr(self.K), n, n, n * n, self.nb, ptr(bc), self.nrhs, n,
+                                         n * self.nrhs, ptr(self.X), n, n * self.nrhs, self.max_iter, self.tol,
+                                         ctypes.byref(it), ctypes.byref(res), self.stream), "cpt_lu_solve_mixed")
+            self.iterations, self.residual = it.value, res.value
+            return
+        self.X.copy_(bc)
         check(lib.cpt_lu_solve(self.ctx.handle, ptr(self.K), n, n, n * n, ptr(self.ipiv), n, ptr(self.X),
                                self.nrhs, n, n * self.nrhs, self.nb, self.stream), "cpt_lu_solve")
 

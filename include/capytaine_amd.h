@@ -133,12 +133,49 @@ int cpt_lu_factor(cpt_handle h, void* A, int n, int64_t lda, int64_t strideA, in
 int cpt_lu_solve(cpt_handle h, const void* LU, int n, int64_t lda, int64_t strideA, const int* ipiv,
                  int64_t strideP, void* B, int nrhs, int64_t ldb, int64_t strideB, int batch, hipStream_t stream);
 
+/* Mixed-precision solve of K X = B for `batch` complex128 systems with nrhs
+ * right-hand sides each: K is rounded to complex64 and factorised with FP32
+ * MFMA trailing updates; X is then refined with FP64 residuals r = B - K X
+ * (against the unrounded complex128 K) and complex64 corrections until
+ * max_j ||r_j||_2 / ||b_j||_2 <= tol or max_iter refinement sweeps.  K and B
+ * are not modified.  Outputs the sweeps used and the final relative
+ * residual.  Synchronises `stream` once per sweep.
+ * Replaces, with the north-star's mixed-precision scheme, the factor + solve
+ * of DefaultMatrixEngine.linear_solver (src/capytaine/bem/engines.py:326-338,
+ * LAPACK zgetrf/zgetrs through block_circulant_matrices.py:374-385). */
+int cpt_lu_solve_mixed(cpt_handle h, const void* K, int n, int64_t lda, int64_t strideK, int batch, const void* B,
+                       int nrhs, int64_t ldb, int64_t strideB, void* X, int64_t ldx, int64_t strideX, int max_iter,
+                       double tol, int* iterations, double* residual, hipStream_t stream);
+
 /* C = alpha*A*B + (beta_zero ? 0 : C), complex128, column-major, batched.
  * Replaces the products `S @ sources` of src/capytaine/bem/solver.py:197-198
  * (numpy matmul), done here for all right-hand sides at once. */
 int cpt_zgemm(cpt_handle h, int M, int N, int K, double alpha, const void* A, int64_t lda, int64_t strideA,
               const void* B, int64_t ldb, int64_t strideB, void* C, int64_t ldc, int64_t strideC, int beta_zero,
               int batch, hipStream_t stream);
+
+/* Live timing of the hot kernels, for measurement only (no counterpart in
+ * the reference).  Between cpt_profile_begin and cpt_profile_end every
+ * launch of the kernel classes below made through `h` is bracketed by HIP
+ * events recorded on the stream it is launched on.  cpt_profile_end waits
+ * for those events and writes, per class, the summed kernel time in ms, the
+ * summed algorithmic work (flops for the product and panel classes, 8 M N K
+ * per complex product; bytes of A, B and C moved for the skinny products;
+ * collocation-point x panel x wavenumber pairs for the assembly) and the
+ * number of launches: arrays of CPT_TIMED_KERNELS.  A panel bracket spans
+ * the products launched inside it, which are also counted in their class. */
+enum {
+    CPT_TIMED_ZGEMM = 0,     /* complex128 MFMA tile product (LU update) */
+    CPT_TIMED_CGEMM = 1,     /* complex64 MFMA tile product (mixed precision) */
+    CPT_TIMED_PANEL_Z = 2,   /* complex128 panel factorisation */
+    CPT_TIMED_PANEL_C = 3,   /* complex64 panel factorisation */
+    CPT_TIMED_ASSEMBLY = 4,  /* S/K assembly */
+    CPT_TIMED_SKINNY_Z = 5,  /* complex128 product with <= 32 columns (solves, residuals, S X) */
+    CPT_TIMED_SKINNY_C = 6,  /* complex64 product with <= 32 columns */
+    CPT_TIMED_KERNELS = 7
+};
+int cpt_profile_begin(cpt_handle h);
+int cpt_profile_end(cpt_handle h, double* ms, double* work, int64_t* launches);
 
 #ifdef __cplusplus
 }

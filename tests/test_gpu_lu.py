@@ -45,7 +45,7 @@ def _random_system(rng, n, r, diag=2.0):
     return A, B
 
 
-@pytest.mark.parametrize("n", [1, 7, 64, 65, 200, 517, 1100, 2300])
+@pytest.mark.parametrize("n", [1, 7, 64, 65, 200, 517, 1024, 1100, 2300])
 def test_lu_solve_random(n):
     from scipy.linalg import lu_factor, lu_solve
     print(f"[gpu] LU n={n}", flush=True)
@@ -100,8 +100,10 @@ def test_lu_batched():
         assert np.linalg.norm(X[b] - Xr) / np.linalg.norm(Xr) < 1e-10
 
 
-@pytest.mark.parametrize("shape", [(5, 3, 4), (64, 64, 64), (130, 70, 33), (1000, 14, 1000), (6, 14, 4352)])
+@pytest.mark.parametrize("shape", [(5, 3, 4), (64, 64, 64), (130, 70, 33), (1000, 14, 1000), (6, 14, 4352),
+                                   (64, 1, 64), (517, 30, 333), (2100, 16, 70), (4352, 14, 4352)])
 def test_zgemm(shape):
+    """Covers the small, LDS-tiled and skinny (N <= 32) product kernels."""
     import torch
     from capytaine_amd import _lib
     from capytaine_amd._lib import check, ptr, stream_handle
@@ -134,3 +136,92 @@ def test_lu_on_bem_matrix(default_table):
     assert eng.last_computed_matrices[1] is not K  # LU cached in place of K
     xr = lu_solve(lu_factor(Kh), b)
     assert np.linalg.norm(x - xr) / np.linalg.norm(xr) < 1e-12
+
+
+def _mixed(As, Bs, max_iter=10, tol=1e-12):
+    import ctypes
+    import torch
+    from capytaine_amd import _lib
+    from capytaine_amd._lib import check, ptr, stream_handle
+    nb, n, _ = As.shape
+    r = Bs.shape[2]
+    A = _dev(np.swapaxes(As, 1, 2))
+    B = _dev(np.swapaxes(Bs, 1, 2))
+    X = torch.empty_like(B)
+    it, res = ctypes.c_int(), ctypes.c_double()
+    ctx = _lib.Context.get(device())
+    check(_lib.load().cpt_lu_solve_mixed(ctx.handle, ptr(A), n, n, n * n, nb, ptr(B), r, n, n * r, ptr(X), n, n * r,
+                                         max_iter, tol, ctypes.byref(it), ctypes.byref(res),
+                                         stream_handle(device())), "mixed")
+    torch.cuda.synchronize()
+    # K and B are left untouched
+    assert np.array_equal(np.swapaxes(A.cpu().numpy(), 1, 2), As)
+    return np.swapaxes(X.cpu().numpy(), 1, 2), it.value, res.value
+
+
+@pytest.mark.parametrize("n,nb", [(7, 1), (300, 3), (1100, 2), (2300, 1)])
+def test_mixed_precision_solve(n, nb):
+    """complex64 factor (FP32 MFMA) + FP64 refinement reaches FP64 accuracy;
+    without refinement the error is at float32 level."""
+    print(f"[gpu] mixed solve n={n} batch={nb}", flush=True)
+    rng = np.random.default_rng(100 + n)
+    As = np.empty((nb, n, n), dtype=complex)
+    Bs = np.empty((nb, n, 6), dtype=complex)
+    for b in range(nb):
+        As[b], Bs[b] = _random_system(rng, n, 6)
+    X0, it0, res0 = _mixed(As, Bs, max_iter=0)
+    X, it, res = _mixed(As, Bs)
+    assert it <= 6 and res <= 1e-12
+    for b in range(nb):
+        Xr = np.linalg.solve(As[b], Bs[b])
+        e0 = np.linalg.norm(X0[b] - Xr) / np.linalg.norm(Xr)
+        e = np.linalg.norm(X[b] - Xr) / np.linalg.norm(Xr)
+        assert 1e-9 < e0 < 1e-4, e0          # float32 factor alone
+        assert e < 1e-12, e                  # refined
+
+
+def test_mixed_precision_on_bem_matrix(default_table):
+    """Sphere (450 panels): mixed engine gives the FP64 engine's sources."""
+    from capytaine_amd import DefaultMatrixEngine, mesh_sphere
+    from gpu_helpers import ref_table_gf
+    mesh = mesh_sphere(radius=1.0, resolution=(30, 30)).immersed_part()
+    gf = ref_table_gf(default_table)
+    params = dict(free_surface=0.0, water_depth=np.inf, wavenumber=2.0, adjoint_double_layer=True,
+                  diagonal_term_in_double_layer=True)
+    b = -1j * mesh.faces_normals[:, 2].astype(complex)
+    e64 = DefaultMatrixEngine(green_function=gf)
+    _, K = e64.build_matrices(mesh, mesh, **params)
+    x64 = e64.linear_solver(K, b)
+    emx = DefaultMatrixEngine(green_function=gf, linear_solver="mixed_precision")
+    _, K = emx.build_matrices(mesh, mesh, **params)
+    xmx = emx.linear_solver(K, b)
+    assert np.linalg.norm(xmx - x64) / np.linalg.norm(x64) < 1e-11
+
+
+def test_kernel_timing():
+    """cpt_profile_begin/end count and time every tile-product and panel
+    launch of one factorisation, with the flop count of the blocked LU."""
+    import torch
+    from capytaine_amd import _lib
+    print("[gpu] kernel timing", flush=True)
+    rng = np.random.default_rng(3)
+    n = 300
+    A, B = _random_system(rng, n, 4)
+    ctx = _lib.Context.get(device())
+    _lib.profile_begin(ctx)
+    X, info, _, _ = _factor_solve(A[None], B[None])
+    prof = _lib.profile_end(ctx)
+    assert info[0] == 0
+    panels = prof["panel_z"]
+    assert panels["launches"] == (n + 63) // 64
+    # panel flops: sum over panels of 4 (M kb^2 - kb^3/3)
+    expect = sum(4.0 * ((n - k0) * min(64, n - k0) ** 2 - min(64, n - k0) ** 3 / 3.0) for k0 in range(0, n, 64))
+    assert abs(panels["work"] - expect) < 1e-9 * expect
+    assert panels["ms"] > 0 and prof["zgemm"]["launches"] > 0 and prof["zgemm"]["ms"] > 0
+    assert prof["cgemm"]["launches"] == 0 and prof["assembly"]["launches"] == 0
+    # timing off again: nothing recorded
+    _factor_solve(A[None], B[None])
+    _lib.profile_begin(ctx)
+    prof2 = _lib.profile_end(ctx)
+    assert all(v["launches"] == 0 for v in prof2.values())
+    torch.cuda.synchronize()
