@@ -83,6 +83,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the autotuner (a stored tuning of this size class is still applied)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the frequency sweep shards by "
@@ -107,7 +109,21 @@ def main():
     stream = torch.cuda.current_stream(dev)
     omegas_arr = np.asarray(omegas)
 
+    from capytaine_amd import autotune
+    ctx = _lib.Context.get(dev)
+    tuning = {}
+    if not args.no_tune:
+        def log(msg):
+            print(msg, file=sys.stderr, flush=True)
+        for prec in ("fp64", "mixed"):
+            tuning[prec] = autotune.tune_lu(n, len(ks), prec, device=dev, nrhs=nrhs, log=log)
+        tuning["assembly"] = autotune.tune_assembly(gf, mesh, ks, log=log)
+        torch.cuda.empty_cache()
+
     def run(precision):
+        applied = autotune.apply(ctx, n, len(ks), precision, nb_wavenumbers=len(ks))
+        if applied:
+            print(f"[bench] {precision}: tuning {applied}", file=sys.stderr, flush=True)
         sweep = FrequencySweep(gf, body, ks, nrhs, precision=precision, max_iter=10, tol=1e-12)
         for _ in range(args.warmup):
             sweep.run(bc)
@@ -222,6 +238,7 @@ def main():
         "kernels_fp64": kernel_table(r64),
         "kernels_mixed": kernel_table(rmx),
         "roofline": roofline,
+        "tuning": {k: {kk: vv for kk, vv in v.items() if kk != "timings_ms"} for k, v in tuning.items()},
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(mesh, body, pbs_by_freq,

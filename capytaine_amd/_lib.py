@@ -36,7 +36,12 @@ SIGNATURES = {
                                 ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double), _P]),
     "cpt_profile_begin": (_I, [_P]),
     "cpt_profile_end": (_I, [_P, _P, _P, _P]),
+    "cpt_set_tuning": (_I, [_P, _I, _I]),
+    "cpt_get_tuning": (_I, [_P, _I, ctypes.POINTER(ctypes.c_int)]),
 }
+
+# Tuning parameters of cpt_set_tuning, in the order of the CPT_TUNE_* values.
+TUNING_PARAMS = ("lu_block", "panel_leaf", "lookahead", "assembly_chunk")
 
 # Kernel classes of cpt_profile_end, in the order of the CPT_TIMED_* values.
 TIMED_KERNELS = ("zgemm", "cgemm", "panel_z", "panel_c", "assembly", "skinny_z", "skinny_c")
@@ -136,3 +141,20 @@ def profile_end(ctx):
     check(load().cpt_profile_end(ctx.handle, ptr(ms), ptr(work), ptr(launches)), "cpt_profile_end")
     return {k: {"ms": float(ms[i]), "work": float(work[i]), "launches": int(launches[i])}
             for i, k in enumerate(TIMED_KERNELS)}
+
+
+def set_tuning(ctx, **params):
+    """Set tuning parameters of a context (names of TUNING_PARAMS)."""
+    lib = load()
+    for name, value in params.items():
+        check(lib.cpt_set_tuning(ctx.handle, TUNING_PARAMS.index(name), int(value)), f"cpt_set_tuning({name})")
+
+
+def get_tuning(ctx):
+    lib = load()
+    out = {}
+    for i, name in enumerate(TUNING_PARAMS):
+        v = ctypes.c_int()
+        check(lib.cpt_get_tuning(ctx.handle, i, ctypes.byref(v)), "cpt_get_tuning")
+        out[name] = v.value
+    return out

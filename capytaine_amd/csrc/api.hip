@@ -20,12 +20,12 @@ int launch_zgemm(int M, int N, int K, double alpha, const cplx* A, int64_t lda, 
                  int64_t ldb, int64_t sB, cplx* C, int64_t ldc, int64_t sC, int beta_zero, int batch,
                  hipStream_t stream);
 int launch_getrf(cplx* A, int N, int64_t lda, int64_t strideA, int* ipiv, int64_t strideP, int* info, int batch,
-                 cplx* work, const LuStreams* aux, hipStream_t stream);
+                 cplx* work, const LuSetup* aux, hipStream_t stream);
 size_t getrf_workspace_bytes(int N, int batch);
 size_t mixed_workspace_bytes(int n, int nrhs, int batch);
 int launch_mixed_solve(const cplx* K, int n, int64_t lda, int64_t strideK, int batch, const cplx* B, int nrhs,
                        int64_t ldb, int64_t strideB, cplx* X, int64_t ldx, int64_t strideX, int max_iter, double tol,
-                       void* work, const LuStreams* aux, int* iterations, double* residual, hipStream_t stream);
+                       void* work, const LuSetup* aux, int* iterations, double* residual, hipStream_t stream);
 int launch_getrs(const cplx* LU, int N, int64_t lda, int64_t strideA, const int* ipiv, int64_t strideP, cplx* B,
                  int nrhs, int64_t ldb, int64_t strideB, int batch, hipStream_t stream);
 
@@ -108,7 +108,8 @@ struct cpt_context {
     // per-call workspace
     cpt::DevBuf faces, quad, freq, prony;
     cpt::DevBuf lu_work, mixed_work;
-    cpt::LuStreams lu_streams;
+    cpt::LuSetup lu_streams;
+    int asm_chunk = 0;  // panels per assembly work-group, 0 = by size
     cpt::KernelTimer timer;
 };
 
@@ -429,7 +430,7 @@ int cpt_build_matrices(cpt_handle h, int nb_collocation_points, const double* co
     // Enough work-groups to fill 256 CUs several times over.
     const int gx = (n + 255) / 256;
     int chunk = (int)std::max<int64_t>(1, ((int64_t)gx * m) / 4096);
-    A.j_chunk = std::min(chunk, 64);
+    A.j_chunk = h->asm_chunk ? h->asm_chunk : std::min(chunk, 64);
     A.freq = (const FreqArgs*)h->freq.p;
     A.S = (cplx*)S;
     A.ldS = ldS;
@@ -617,4 +618,53 @@ int cpt_profile_end(cpt_handle h, double* ms, double* work, int64_t* launches)
     }
     t.used = 0;
     return CPT_OK;
+}
+
+int cpt_set_tuning(cpt_handle h, int param, int value)
+{
+    if (!h) {
+        set_error("cpt_set_tuning: null handle");
+        return CPT_ERR_ARG;
+    }
+    switch (param) {
+    case CPT_TUNE_LU_BLOCK:
+        if (value != 0 && value != 64 && value != 128) break;
+        h->lu_streams.block = value;
+        return CPT_OK;
+    case CPT_TUNE_PANEL_LEAF:
+        if (value < 1 || value > 64) break;
+        h->lu_streams.leaf = value;
+        return CPT_OK;
+    case CPT_TUNE_LOOKAHEAD:
+        if (value != 0 && value != 1) break;
+        h->lu_streams.lookahead = value;
+        return CPT_OK;
+    case CPT_TUNE_ASSEMBLY_CHUNK:
+        if (value < 0 || value > 1024) break;
+        h->asm_chunk = value;
+        return CPT_OK;
+    default:
+        set_error("cpt_set_tuning: unknown parameter " + std::to_string(param));
+        return CPT_ERR_ARG;
+    }
+    set_error("cpt_set_tuning: value " + std::to_string(value) + " out of range for parameter " +
+              std::to_string(param));
+    return CPT_ERR_ARG;
+}
+
+int cpt_get_tuning(cpt_handle h, int param, int* value)
+{
+    if (!h || !value) {
+        set_error("cpt_get_tuning: null argument");
+        return CPT_ERR_ARG;
+    }
+    switch (param) {
+    case CPT_TUNE_LU_BLOCK: *value = h->lu_streams.block; return CPT_OK;
+    case CPT_TUNE_PANEL_LEAF: *value = h->lu_streams.leaf; return CPT_OK;
+    case CPT_TUNE_LOOKAHEAD: *value = h->lu_streams.lookahead; return CPT_OK;
+    case CPT_TUNE_ASSEMBLY_CHUNK: *value = h->asm_chunk; return CPT_OK;
+    default:
+        set_error("cpt_get_tuning: unknown parameter " + std::to_string(param));
+        return CPT_ERR_ARG;
+    }
 }

@@ -40,10 +40,14 @@ struct AssemblyArgs {
 
 int launch_assembly(const AssemblyArgs& A, hipStream_t stream);
 
-// Side stream and events of the LU look-ahead (owned by a context).
-struct LuStreams {
+// Tuning of the blocked LU and the side stream / events of its look-ahead
+// (owned by a context).
+struct LuSetup {
     hipStream_t side = nullptr;
     hipEvent_t ready = nullptr, done = nullptr;
+    int block = 0;      // block width: 64 or 128, 0 = by size (128 from 1024 rows)
+    int leaf = 16;      // panel columns factored by one work-group per matrix
+    int lookahead = 1;  // factor the next panel on the side stream
 };
 
 void set_error(const std::string& msg);

@@ -744,24 +744,21 @@ template <typename R>
 int gemm(int M, int N, int K, R alpha, const cx<R>* A, int64_t lda, int64_t sA, const cx<R>* B, int64_t ldb,
          int64_t sB, cx<R>* C, int64_t ldc, int64_t sC, int beta_zero, int batch, hipStream_t stream);
 
-// Columns narrower than this are factored by one work-group per matrix.
-constexpr int PANEL_LEAF = 16;
-
 // Recursive factorisation of columns [c0, c0 + w) of rows [c0, N) inside the
 // panel [sw_lo, sw_hi): factor the left half, solve its unit-lower triangle
 // against the top of the right half (trsm), update the rest of the right
 // half with an MFMA product spread over the whole device, factor the right
-// half.  The one-work-group leaves stream only PANEL_LEAF columns.
+// half.  The one-work-group leaves stream only `leaf` columns.
 template <typename R>
 int panel_rec(cx<R>* A, int64_t lda, int64_t sA, int N, int c0, int w, int sw_lo, int sw_hi, int* ipiv, int64_t sP,
-              int* info, int batch, hipStream_t stream)
+              int* info, int batch, int leaf, hipStream_t stream)
 {
     const int M = N - c0;
-    if (w <= PANEL_LEAF || M <= 2 * PANEL_LEAF)
+    if (w <= leaf || M <= 2 * leaf)
         return panel_leaf<R>(A, lda, sA, N, c0, w, sw_lo, sw_hi, ipiv, sP, info, batch, stream);
     const int h = w / 2;
     int e;
-    if ((e = panel_rec<R>(A, lda, sA, N, c0, h, sw_lo, sw_hi, ipiv, sP, info, batch, stream))) return e;
+    if ((e = panel_rec<R>(A, lda, sA, N, c0, h, sw_lo, sw_hi, ipiv, sP, info, batch, leaf, stream))) return e;
     cx<R>* A11 = A + c0 + (size_t)c0 * lda;
     cx<R>* A12 = A + c0 + (size_t)(c0 + h) * lda;
     cx<R>* A21 = A + (c0 + h) + (size_t)c0 * lda;
@@ -769,20 +766,20 @@ int panel_rec(cx<R>* A, int64_t lda, int64_t sA, int N, int c0, int w, int sw_lo
     if ((e = launch_trsm<R, false>(A11, lda, sA, h, A12, lda, sA, w - h, batch, stream))) return e;
     if (M > h && (e = gemm<R>(M - h, w - h, h, R(-1), A21, lda, sA, A12, lda, sA, A22, lda, sA, 0, batch, stream)))
         return e;
-    return panel_rec<R>(A, lda, sA, N, c0 + h, w - h, sw_lo, sw_hi, ipiv, sP, info, batch, stream);
+    return panel_rec<R>(A, lda, sA, N, c0 + h, w - h, sw_lo, sw_hi, ipiv, sP, info, batch, leaf, stream);
 }
 
 // LU with partial pivoting of the panel columns [k0, k0 + kb), rows [k0, N);
 // interchanges applied within the panel columns only.
 template <typename R>
 int panel_dispatch(cx<R>* A, int64_t lda, int64_t sA, int N, int k0, int kb, int* ipiv, int64_t sP, int* info,
-                   int batch, hipStream_t stream)
+                   int batch, int leaf, hipStream_t stream)
 {
     const int M = N - k0;
     // complex LU of an M x kb panel: 4 (M kb^2 - kb^3 / 3) real flops
     TimedLaunch timed(stream, sizeof(R) == 8 ? TK_PANEL_Z : TK_PANEL_C,
                       4.0 * ((double)M * kb * kb - (double)kb * kb * kb / 3.0) * batch);
-    return panel_rec<R>(A, lda, sA, N, k0, kb, k0, k0 + kb, ipiv, sP, info, batch, stream);
+    return panel_rec<R>(A, lda, sA, N, k0, kb, k0, k0 + kb, ipiv, sP, info, batch, leaf, stream);
 }
 
 template <typename R>
@@ -827,7 +824,7 @@ int trsm_lower(const cx<R>* L, int64_t lda, int64_t sA, int kb, cx<R>* B, int64_
 
 // Block width of the right-looking factorisation: 128 columns halve the
 // trailing-matrix traffic per flop against 64 where the matrices are large.
-inline int getrf_block(int N) { return N >= 1024 ? 2 * PB : PB; }
+inline int getrf_block(int N, int tuned) { return tuned ? tuned : (N >= 1024 ? 2 * PB : PB); }
 
 template <typename R>
 size_t getrf_ws_elems(int, int)
@@ -843,13 +840,14 @@ size_t getrf_ws_elems(int, int)
 // of A22 is updated (look-ahead of depth one).
 template <typename R>
 int getrf(cx<R>* A, int N, int64_t lda, int64_t strideA, int* ipiv, int64_t strideP, int* info, int batch,
-          cx<R>*, const LuStreams* aux, hipStream_t stream)
+          cx<R>*, const LuSetup* aux, hipStream_t stream)
 {
     if (N <= 0 || batch <= 0) return 0;
     int e;
-    const int nb = getrf_block(N);
-    const bool ahead = aux && aux->side;
-    if ((e = panel_dispatch<R>(A, lda, strideA, N, 0, min(nb, N), ipiv, strideP, info, batch, stream))) return e;
+    const int nb = getrf_block(N, aux ? aux->block : 0);
+    const int leaf = aux ? aux->leaf : 16;
+    const bool ahead = aux && aux->side && aux->lookahead;
+    if ((e = panel_dispatch<R>(A, lda, strideA, N, 0, min(nb, N), ipiv, strideP, info, batch, leaf, stream))) return e;
     for (int k0 = 0; k0 < N; k0 += nb) {
         const int kb = min(nb, N - k0);
         // panel k0 is factored (on `stream`, or on the side stream joined below)
@@ -873,7 +871,7 @@ int getrf(cx<R>* A, int N, int64_t lda, int64_t strideA, int* ipiv, int64_t stri
             // the trailing matrix is updated on `stream` (disjoint columns)
             if (hipEventRecord(aux->ready, stream) != hipSuccess) return (int)hipErrorUnknown;
             if (hipStreamWaitEvent(aux->side, aux->ready, 0) != hipSuccess) return (int)hipErrorUnknown;
-            if ((e = panel_dispatch<R>(A, lda, strideA, N, k0 + kb, kn, ipiv, strideP, info, batch, aux->side)))
+            if ((e = panel_dispatch<R>(A, lda, strideA, N, k0 + kb, kn, ipiv, strideP, info, batch, leaf, aux->side)))
                 return e;
             if (hipEventRecord(aux->done, aux->side) != hipSuccess) return (int)hipErrorUnknown;
         }
@@ -882,7 +880,8 @@ int getrf(cx<R>* A, int N, int64_t lda, int64_t strideA, int* ipiv, int64_t stri
             return e;
         if (ahead) {
             if (hipStreamWaitEvent(stream, aux->done, 0) != hipSuccess) return (int)hipErrorUnknown;
-        } else if ((e = panel_dispatch<R>(A, lda, strideA, N, k0 + kb, kn, ipiv, strideP, info, batch, stream))) {
+        } else if ((e = panel_dispatch<R>(A, lda, strideA, N, k0 + kb, kn, ipiv, strideP, info, batch, leaf,
+                                          stream))) {
             return e;
         }
     }
@@ -1000,7 +999,7 @@ int launch_zgemm(int M, int N, int K, double alpha, const cplx* A, int64_t lda, 
 size_t getrf_workspace_bytes(int N, int batch) { return sizeof(cplx) * getrf_ws_elems<double>(N, batch); }
 
 int launch_getrf(cplx* A, int N, int64_t lda, int64_t strideA, int* ipiv, int64_t strideP, int* info, int batch,
-                 cplx* work, const LuStreams* aux, hipStream_t stream)
+                 cplx* work, const LuSetup* aux, hipStream_t stream)
 {
     return getrf<double>((cx<double>*)A, N, lda, strideA, ipiv, strideP, info, batch, (cx<double>*)work, aux, stream);
 }
@@ -1037,7 +1036,7 @@ size_t mixed_workspace_bytes(int n, int nrhs, int batch)
 // Synchronises the stream once per sweep to read the residual norms.
 int launch_mixed_solve(const cplx* K, int n, int64_t lda, int64_t strideK, int batch, const cplx* B, int nrhs,
                        int64_t ldb, int64_t strideB, cplx* X, int64_t ldx, int64_t strideX, int max_iter, double tol,
-                       void* work, const LuStreams* aux, int* iterations, double* residual, hipStream_t stream)
+                       void* work, const LuSetup* aux, int* iterations, double* residual, hipStream_t stream)
 {
     typedef cx<float> c64;
     typedef cx<double> c128;

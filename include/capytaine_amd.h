@@ -177,6 +177,19 @@ enum {
 int cpt_profile_begin(cpt_handle h);
 int cpt_profile_end(cpt_handle h, double* ms, double* work, int64_t* launches);
 
+/* Tuning parameters of a context, set by the autotuner (no counterpart in
+ * the reference, whose LAPACK and OpenMP loops have none).  Results depend
+ * on them only through rounding: every value factors with LAPACK's partial
+ * pivoting rule. */
+enum {
+    CPT_TUNE_LU_BLOCK = 0,       /* block width of the LU: 64, 128, or 0 = 128 from 1024 rows */
+    CPT_TUNE_PANEL_LEAF = 1,     /* panel columns factored by one work-group per matrix, 1..64 */
+    CPT_TUNE_LOOKAHEAD = 2,      /* 1: factor the next panel on a side stream, 0: in order */
+    CPT_TUNE_ASSEMBLY_CHUNK = 3  /* panels per assembly work-group, 0 = by size, else 1..1024 */
+};
+int cpt_set_tuning(cpt_handle h, int param, int value);
+int cpt_get_tuning(cpt_handle h, int param, int* value);
+
 #ifdef __cplusplus
 }
 #endif

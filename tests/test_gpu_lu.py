@@ -225,3 +225,38 @@ def test_kernel_timing():
     prof2 = _lib.profile_end(ctx)
     assert all(v["launches"] == 0 for v in prof2.values())
     torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("block,leaf,lookahead", [(64, 8, 0), (64, 32, 1), (128, 16, 1), (128, 64, 0)])
+def test_lu_tunings_agree(block, leaf, lookahead):
+    """Every tuning of the blocked LU solves to LAPACK's accuracy."""
+    from scipy.linalg import lu_factor, lu_solve
+    from capytaine_amd import _lib
+    print(f"[gpu] LU tuning block={block} leaf={leaf} lookahead={lookahead}", flush=True)
+    ctx = _lib.Context.get(device())
+    saved = _lib.get_tuning(ctx)
+    try:
+        _lib.set_tuning(ctx, lu_block=block, panel_leaf=leaf, lookahead=lookahead)
+        assert _lib.get_tuning(ctx)["lu_block"] == block
+        rng = np.random.default_rng(block + leaf)
+        n = 517
+        As, Bs = zip(*[_random_system(rng, n, 5) for _ in range(3)])
+        As, Bs = np.array(As), np.array(Bs)
+        X, info, _, _ = _factor_solve(As, Bs)
+        assert not info.any()
+        for b in range(3):
+            Xr = lu_solve(lu_factor(As[b]), Bs[b])
+            assert np.linalg.norm(X[b] - Xr) / np.linalg.norm(Xr) < 1e-10
+    finally:
+        _lib.set_tuning(ctx, **saved)
+
+
+def test_tuning_rejects_bad_values():
+    from capytaine_amd import _lib
+    ctx = _lib.Context.get(device())
+    for name, bad in [("lu_block", 96), ("panel_leaf", 0), ("panel_leaf", 65), ("lookahead", 2),
+                      ("assembly_chunk", -1)]:
+        with pytest.raises(_lib.CptError):
+            _lib.set_tuning(ctx, **{name: bad})
+    with pytest.raises(_lib.CptError):
+        _lib.check(_lib.load().cpt_set_tuning(ctx.handle, 99, 0), "cpt_set_tuning")
