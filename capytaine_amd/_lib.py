@@ -36,6 +36,10 @@ SIGNATURES = {
                                 ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double), _P]),
     "cpt_profile_begin": (_I, [_P]),
     "cpt_profile_end": (_I, [_P, _P, _P, _P]),
+    "cpt_build_matrices_batched": (_I, [_P, _I, _P, _I, _I, _D, _P]),
+    "cpt_lu_factor_grouped": (_I, [_P, _I, _P, _P]),
+    "cpt_lu_solve_grouped": (_I, [_P, _I, _P, _P]),
+    "cpt_lu_solve_mixed_grouped": (_I, [_P, _I, _P, _I, _D, _P, _P, _P]),
     "cpt_set_tuning": (_I, [_P, _I, _I]),
     "cpt_get_tuning": (_I, [_P, _I, ctypes.POINTER(ctypes.c_int)]),
 }
@@ -45,6 +49,24 @@ TUNING_PARAMS = ("lu_block", "panel_leaf", "lookahead", "assembly_chunk")
 
 # Kernel classes of cpt_profile_end, in the order of the CPT_TIMED_* values.
 TIMED_KERNELS = ("zgemm", "cgemm", "panel_z", "panel_c", "assembly", "skinny_z", "skinny_c")
+
+
+
+class CptBody(ctypes.Structure):
+    """cpt_body of include/capytaine_amd.h."""
+    _fields_ = [("nb_faces", ctypes.c_int), ("nb_vertices", ctypes.c_int), ("nb_quad_points", ctypes.c_int),
+                ("nb_wavenumbers", ctypes.c_int), ("vertices", _P), ("faces", _P), ("centers", _P),
+                ("normals", _P), ("areas", _P), ("radiuses", _P), ("quad_points", _P), ("quad_weights", _P),
+                ("wavenumbers", _P), ("gf_singularities", _P), ("S", _P), ("ldS", _L), ("strideS", _L),
+                ("K", _P), ("ldK", _L), ("strideK", _L)]
+
+
+class CptLuGroup(ctypes.Structure):
+    """cpt_lu_group of include/capytaine_amd.h."""
+    _fields_ = [("n", ctypes.c_int), ("batch", ctypes.c_int), ("A", _P), ("lda", _L), ("strideA", _L),
+                ("ipiv", _P), ("strideP", _L), ("info", _P), ("B", _P), ("nrhs", ctypes.c_int), ("ldb", _L),
+                ("strideB", _L), ("X", _P), ("ldx", _L), ("strideX", _L)]
+
 
 _lib = None
 _lock = threading.RLock()  # Context.get -> Context.__init__ -> load() re-enters it

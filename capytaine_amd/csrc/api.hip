@@ -3,6 +3,7 @@
 // reference interface each entry point replaces are in
 // include/capytaine_amd.h.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -23,6 +24,11 @@ int launch_getrf(cplx* A, int N, int64_t lda, int64_t strideA, int* ipiv, int64_
                  cplx* work, const LuSetup* aux, hipStream_t stream);
 size_t getrf_workspace_bytes(int N, int batch);
 size_t mixed_workspace_bytes(int n, int nrhs, int batch);
+size_t mixed_grouped_bytes(const MixedGroup* g, int ng);
+int launch_mixed_grouped(const MixedGroup* g, int ng, void* work, const LuSetup* tune, int max_iter, double tol,
+                         int* iterations, double* residual);
+int launch_getrf_lanes(const LuGroupArgs* g, int ng, const LuSetup* tune);
+int launch_getrs_lanes(const LuGroupArgs* g, int ng);
 int launch_mixed_solve(const cplx* K, int n, int64_t lda, int64_t strideK, int batch, const cplx* B, int nrhs,
                        int64_t ldb, int64_t strideB, cplx* X, int64_t ldx, int64_t strideX, int max_iter, double tol,
                        void* work, const LuSetup* aux, int* iterations, double* residual, hipStream_t stream);
@@ -107,9 +113,15 @@ struct cpt_context {
     bool has_table = false;
     // per-call workspace
     cpt::DevBuf faces, quad, freq, prony;
+    cpt::DevBuf bfaces, bquad, bfreq, bargs, btiles;  // multi-body assembly
     cpt::DevBuf lu_work, mixed_work;
     cpt::LuSetup lu_streams;
     int asm_chunk = 0;  // panels per assembly work-group, 0 = by size
+    // lanes of the grouped LU: default-priority streams, a fork event and
+    // one join event per lane
+    std::vector<hipStream_t> lanes;
+    std::vector<hipEvent_t> lane_done;
+    hipEvent_t fork = nullptr;
     cpt::KernelTimer timer;
 };
 
@@ -188,6 +200,83 @@ int ensure_lu_streams(cpt_context* h)
     return 0;
 }
 
+constexpr int NUM_LANES = 4;
+
+int ensure_lanes(cpt_context* h)
+{
+    if (!h->fork && hipEventCreateWithFlags(&h->fork, hipEventDisableTiming) != hipSuccess) return -1;
+    while ((int)h->lanes.size() < NUM_LANES) {
+        hipStream_t s;
+        hipEvent_t ev;
+        if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return -1;
+        if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+            (void)hipStreamDestroy(s);
+            return -1;
+        }
+        h->lanes.push_back(s);
+        h->lane_done.push_back(ev);
+    }
+    return 0;
+}
+
+// Lanes wait for the work already on `caller`.
+int fork_lanes(cpt_context* h, hipStream_t caller)
+{
+    if (hipEventRecord(h->fork, caller) != hipSuccess) return -1;
+    for (hipStream_t l : h->lanes)
+        if (hipStreamWaitEvent(l, h->fork, 0) != hipSuccess) return -1;
+    return 0;
+}
+
+// `caller` waits for the work enqueued on every lane.
+int join_lanes(cpt_context* h, hipStream_t caller)
+{
+    for (size_t i = 0; i < h->lanes.size(); ++i) {
+        if (hipEventRecord(h->lane_done[i], h->lanes[i]) != hipSuccess) return -1;
+        if (hipStreamWaitEvent(caller, h->lane_done[i], 0) != hipSuccess) return -1;
+    }
+    return 0;
+}
+
+// Longest-processing-time assignment of groups (cost n^3 batch) to lanes.
+std::vector<hipStream_t> assign_lanes(const cpt_context* h, const cpt_lu_group* g, int ng)
+{
+    std::vector<int> order(ng);
+    for (int i = 0; i < ng; ++i) order[i] = i;
+    auto cost = [&](int i) { return (double)g[i].n * g[i].n * g[i].n * g[i].batch; };
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost(a) > cost(b); });
+    std::vector<double> load(h->lanes.size(), 0.0);
+    std::vector<hipStream_t> out(ng);
+    for (int i : order) {
+        const size_t l = std::min_element(load.begin(), load.end()) - load.begin();
+        out[i] = h->lanes[l];
+        load[l] += cost(i);
+    }
+    return out;
+}
+
+bool valid_groups(const cpt_lu_group* g, int ng, bool need_factor, bool need_rhs, bool need_x, const char* who)
+{
+    for (int i = 0; i < ng; ++i) {
+        const cpt_lu_group& G = g[i];
+        bool ok = G.n >= 0 && G.batch >= 0 && G.lda >= G.n && (G.batch <= 1 || G.strideA >= G.lda * G.n);
+        if (ok && G.n > 0 && G.batch > 0) {
+            ok = ok && G.A;
+            if (need_factor) ok = ok && G.ipiv && G.info && (G.batch <= 1 || G.strideP >= G.n);
+            if (need_rhs)
+                ok = ok && G.nrhs >= 0 && G.ldb >= G.n && (G.nrhs == 0 || G.B) &&
+                     (G.batch <= 1 || G.strideB >= G.ldb * G.nrhs);
+            if (need_x) ok = ok && G.ldx >= G.n && (G.nrhs == 0 || G.X) && (G.batch <= 1 || G.strideX >= G.ldx * G.nrhs);
+            if (G.n > 24576) ok = false;
+        }
+        if (!ok) {
+            set_error(std::string(who) + ": invalid sizes, strides or null pointers in group " + std::to_string(i));
+            return false;
+        }
+    }
+    return true;
+}
+
 Table table_of(const cpt_context* h)
 {
     Table T;
@@ -240,6 +329,11 @@ int cpt_destroy(cpt_handle h)
     h->table.release();
     h->faces.release();
     h->quad.release();
+    h->bfaces.release();
+    h->bquad.release();
+    h->bfreq.release();
+    h->bargs.release();
+    h->btiles.release();
     h->freq.release();
     h->prony.release();
     h->lu_work.release();
@@ -248,6 +342,9 @@ int cpt_destroy(cpt_handle h)
     if (h->lu_streams.ready) (void)hipEventDestroy(h->lu_streams.ready);
     if (h->lu_streams.done) (void)hipEventDestroy(h->lu_streams.done);
     h->timer.release();
+    for (hipStream_t l : h->lanes) (void)hipStreamDestroy(l);
+    for (hipEvent_t ev : h->lane_done) (void)hipEventDestroy(ev);
+    if (h->fork) (void)hipEventDestroy(h->fork);
     delete h;
     return CPT_OK;
 }
@@ -440,6 +537,165 @@ int cpt_build_matrices(cpt_handle h, int nb_collocation_points, const double* co
     A.strideK = strideK;
     int e = launch_assembly(A, stream);
     CPT_CHECK_HIP((hipError_t)e, "assembly kernel launch");
+    return CPT_OK;
+}
+
+int cpt_build_matrices_batched(cpt_handle h, int nb_bodies, const cpt_body* bodies, int adjoint_double_layer,
+                               int diagonal_term_in_double_layer, double free_surface, hipStream_t stream)
+{
+    if (!h || nb_bodies < 0 || (nb_bodies > 0 && !bodies)) {
+        set_error("cpt_build_matrices_batched: invalid arguments");
+        return CPT_ERR_ARG;
+    }
+    if (std::isinf(free_surface)) {
+        set_error("cpt_build_matrices_batched: free_surface=inf is not supported in the batched form");
+        return CPT_ERR_UNSUPPORTED;
+    }
+    size_t tot_m = 0, tot_q = 0, tot_nb = 0;
+    bool need_table = false;
+    for (int i = 0; i < nb_bodies; ++i) {
+        const cpt_body& B = bodies[i];
+        const int m = B.nb_faces, nb = B.nb_wavenumbers;
+        if (m < 0 || B.nb_vertices < 0 || B.nb_quad_points < 1 || B.nb_quad_points > 16 || nb < 1 || B.ldS < m ||
+            B.ldK < m || (nb > 1 && (B.strideS < B.ldS * m || B.strideK < B.ldK * m))) {
+            set_error("cpt_build_matrices_batched: invalid sizes or strides of body " + std::to_string(i));
+            return CPT_ERR_ARG;
+        }
+        if (m > 0 && (!B.vertices || !B.faces || !B.centers || !B.normals || !B.areas || !B.radiuses ||
+                      !B.quad_points || !B.quad_weights || !B.wavenumbers || !B.gf_singularities || !B.S || !B.K)) {
+            set_error("cpt_build_matrices_batched: null pointer in body " + std::to_string(i));
+            return CPT_ERR_ARG;
+        }
+        for (int b = 0; b < nb; ++b) {
+            const double k = B.wavenumbers[b];
+            const int g = B.gf_singularities[b];
+            if (g != CPT_HIGH_FREQ && g != CPT_LOW_FREQ && g != CPT_LOW_FREQ_WITH_RANKINE_PART) {
+                set_error("cpt_build_matrices_batched: invalid gf_singularities value");
+                return CPT_ERR_ARG;
+            }
+            if (std::isnan(k) || k < 0) {
+                set_error("cpt_build_matrices_batched: invalid wavenumber");
+                return CPT_ERR_ARG;
+            }
+            if (k > 0 && std::isfinite(k)) need_table = true;
+        }
+        tot_m += m;
+        tot_q += (size_t)m * B.nb_quad_points;
+        tot_nb += nb;
+    }
+    if (tot_m == 0) return CPT_OK;
+    if (need_table && !h->has_table) {
+        set_error("cpt_build_matrices_batched: no tabulation set (call cpt_set_tabulation first)");
+        return CPT_ERR_STATE;
+    }
+    CPT_CHECK_HIP(hipSetDevice(h->device), "hipSetDevice");
+    TimerScope timer_scope(&h->timer);
+    if (h->bfaces.reserve(sizeof(Face) * tot_m) || h->bquad.reserve(sizeof(double) * 4 * tot_q) ||
+        h->bfreq.reserve(sizeof(FreqArgs) * tot_nb) || h->bargs.reserve(sizeof(AssemblyArgs) * nb_bodies)) {
+        set_error("cpt_build_matrices_batched: device allocation failed");
+        return CPT_ERR_ALLOC;
+    }
+    // Panels per tile: the same for every body, sized for ~8 tiles per
+    // resident work-group over the whole work-list.
+    size_t rowblock_panels = 0;
+    for (int i = 0; i < nb_bodies; ++i)
+        rowblock_panels += (size_t)((bodies[i].nb_faces + 255) / 256) * bodies[i].nb_faces;
+    const int chunk = h->asm_chunk ? h->asm_chunk
+                                   : (int)std::min<size_t>(64, std::max<size_t>(1, rowblock_panels / 8192));
+    std::vector<FreqArgs> fa(tot_nb);
+    std::vector<AssemblyArgs> args(nb_bodies);
+    std::vector<AsmTile> tiles;
+    std::vector<double> cost;
+    Face* faces_d = (Face*)h->bfaces.p;
+    double* quad_d = (double*)h->bquad.p;
+    const FreqArgs* freq_d = (const FreqArgs*)h->bfreq.p;
+    size_t om = 0, oq = 0, ob = 0;
+    double pairs = 0.0;
+    const Table T = table_of(h);
+    for (int i = 0; i < nb_bodies; ++i) {
+        const cpt_body& B = bodies[i];
+        const int m = B.nb_faces, nq = B.nb_quad_points, nb = B.nb_wavenumbers;
+        if (m > 0) {
+            hipLaunchKernelGGL(pack_faces_kernel, dim3((m + 255) / 256), dim3(256), 0, stream, B.vertices,
+                               B.nb_vertices, B.faces, B.centers, B.normals, B.areas, B.radiuses, B.quad_points,
+                               B.quad_weights, m, nq, faces_d + om, quad_d + 4 * oq);
+            CPT_CHECK_HIP(hipGetLastError(), "pack kernel launch");
+        }
+        for (int b = 0; b < nb; ++b) {
+            const double k = B.wavenumbers[b];
+            FreqArgs& f = fa[ob + b];
+            f.k = k;
+            f.gfs = B.gf_singularities[b];
+            f.wave = (0.0 < k && std::isfinite(k)) ? 1 : 0;
+            f.nexp = 0;
+            f.pad = 0;
+            f.prony_lambda = nullptr;
+            f.prony_a = nullptr;
+        }
+        AssemblyArgs& A = args[i];
+        std::memset(&A, 0, sizeof(A));
+        A.colloc = B.centers;
+        A.dpn = B.normals;
+        A.n = m;
+        A.m = m;
+        A.nq = nq;
+        A.nk = 1;
+        A.faces = faces_d + om;
+        A.quad = quad_d + 4 * oq;
+        A.T = T;
+        A.depth = INFINITY;
+        A.adjoint = adjoint_double_layer ? 1 : 0;
+        A.diag = diagonal_term_in_double_layer ? 1 : 0;
+        A.n_diag = m;
+        A.free_surface = free_surface;
+        A.nb = nb;
+        A.j_chunk = chunk;
+        A.freq = freq_d + ob;
+        A.S = (cplx*)B.S;
+        A.ldS = B.ldS;
+        A.strideS = B.strideS;
+        A.K = (cplx*)B.K;
+        A.ldK = B.ldK;
+        A.strideK = B.strideK;
+        for (int ib = 0; ib < (m + 255) / 256; ++ib)
+            for (int jc = 0; jc * chunk < m; ++jc) {
+                tiles.push_back(AsmTile{i, ib, jc, 0});
+                cost.push_back((double)nb * (std::min(m, (jc + 1) * chunk) - jc * chunk));
+            }
+        pairs += (double)m * m * nb;
+        om += m;
+        oq += (size_t)m * nq;
+        ob += nb;
+    }
+    // Most expensive tiles first so that the grid stride ends evenly.
+    std::vector<int> order(tiles.size());
+    for (size_t t = 0; t < order.size(); ++t) order[t] = (int)t;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+    std::vector<AsmTile> sorted(tiles.size());
+    for (size_t t = 0; t < order.size(); ++t) sorted[t] = tiles[order[t]];
+    if (h->btiles.reserve(sizeof(AsmTile) * sorted.size())) {
+        set_error("cpt_build_matrices_batched: device allocation failed");
+        return CPT_ERR_ALLOC;
+    }
+    CPT_CHECK_HIP(hipMemcpyAsync(h->bfreq.p, fa.data(), sizeof(FreqArgs) * tot_nb, hipMemcpyHostToDevice, stream),
+                  "copy frequency table");
+    CPT_CHECK_HIP(hipMemcpyAsync(h->bargs.p, args.data(), sizeof(AssemblyArgs) * nb_bodies, hipMemcpyHostToDevice,
+                                 stream),
+                  "copy body table");
+    CPT_CHECK_HIP(hipMemcpyAsync(h->btiles.p, sorted.data(), sizeof(AsmTile) * sorted.size(), hipMemcpyHostToDevice,
+                                 stream),
+                  "copy work-list");
+    // Persistent grid: two work-groups per CU (the kernel's occupancy) on
+    // every CU, or fewer when the work-list is short.
+    int ncu = 256;
+    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->device);
+    const int grid = (int)std::min<size_t>(sorted.size(), (size_t)ncu * 2);
+    const int e = launch_assembly_worklist((const AssemblyArgs*)h->bargs.p, (const AsmTile*)h->btiles.p,
+                                           (int)sorted.size(), grid, pairs, stream);
+    CPT_CHECK_HIP((hipError_t)e, "batched assembly kernel launch");
+    // The host tables above are copied from pageable memory: wait for the
+    // copies before they go out of scope.
+    CPT_CHECK_HIP(hipStreamSynchronize(stream), "batched assembly upload");
     return CPT_OK;
 }
 
@@ -667,4 +923,106 @@ int cpt_get_tuning(cpt_handle h, int param, int* value)
         set_error("cpt_get_tuning: unknown parameter " + std::to_string(param));
         return CPT_ERR_ARG;
     }
+}
+
+int cpt_lu_factor_grouped(cpt_handle h, int ngroups, const cpt_lu_group* groups, hipStream_t stream)
+{
+    if (!h || ngroups < 0 || (ngroups > 0 && !groups)) {
+        set_error("cpt_lu_factor_grouped: invalid arguments");
+        return CPT_ERR_ARG;
+    }
+    if (!valid_groups(groups, ngroups, true, false, false, "cpt_lu_factor_grouped")) return CPT_ERR_ARG;
+    if (ngroups == 0) return CPT_OK;
+    CPT_CHECK_HIP(hipSetDevice(h->device), "hipSetDevice");
+    TimerScope timer_scope(&h->timer);
+    if (ensure_lanes(h) || fork_lanes(h, stream)) {
+        set_error("cpt_lu_factor_grouped: could not set up the lane streams");
+        return CPT_ERR_HIP;
+    }
+    const std::vector<hipStream_t> lane = assign_lanes(h, groups, ngroups);
+    std::vector<LuGroupArgs> g(ngroups);
+    for (int i = 0; i < ngroups; ++i) {
+        const cpt_lu_group& G = groups[i];
+        g[i] = LuGroupArgs{(cplx*)G.A, G.n, G.batch, G.lda, G.strideA, G.ipiv, G.strideP, G.info, nullptr, 0, 0, 0,
+                           lane[i]};
+    }
+    LuSetup tune = h->lu_streams;
+    tune.side = nullptr;  // no look-ahead inside a lane: the lanes overlap instead
+    tune.lookahead = 0;
+    const int e = launch_getrf_lanes(g.data(), ngroups, &tune);
+    if (join_lanes(h, stream)) {
+        set_error("cpt_lu_factor_grouped: could not join the lane streams");
+        return CPT_ERR_HIP;
+    }
+    CPT_CHECK_HIP((hipError_t)e, "grouped getrf launch");
+    return CPT_OK;
+}
+
+int cpt_lu_solve_grouped(cpt_handle h, int ngroups, const cpt_lu_group* groups, hipStream_t stream)
+{
+    if (!h || ngroups < 0 || (ngroups > 0 && !groups)) {
+        set_error("cpt_lu_solve_grouped: invalid arguments");
+        return CPT_ERR_ARG;
+    }
+    if (!valid_groups(groups, ngroups, true, true, false, "cpt_lu_solve_grouped")) return CPT_ERR_ARG;
+    if (ngroups == 0) return CPT_OK;
+    CPT_CHECK_HIP(hipSetDevice(h->device), "hipSetDevice");
+    TimerScope timer_scope(&h->timer);
+    if (ensure_lanes(h) || fork_lanes(h, stream)) {
+        set_error("cpt_lu_solve_grouped: could not set up the lane streams");
+        return CPT_ERR_HIP;
+    }
+    const std::vector<hipStream_t> lane = assign_lanes(h, groups, ngroups);
+    std::vector<LuGroupArgs> g(ngroups);
+    for (int i = 0; i < ngroups; ++i) {
+        const cpt_lu_group& G = groups[i];
+        g[i] = LuGroupArgs{(cplx*)G.A, G.n, G.batch, G.lda, G.strideA, G.ipiv, G.strideP, G.info, (cplx*)G.B,
+                           G.nrhs, G.ldb, G.strideB, lane[i]};
+    }
+    const int e = launch_getrs_lanes(g.data(), ngroups);
+    if (join_lanes(h, stream)) {
+        set_error("cpt_lu_solve_grouped: could not join the lane streams");
+        return CPT_ERR_HIP;
+    }
+    CPT_CHECK_HIP((hipError_t)e, "grouped getrs launch");
+    return CPT_OK;
+}
+
+int cpt_lu_solve_mixed_grouped(cpt_handle h, int ngroups, const cpt_lu_group* groups, int max_iter, double tol,
+                               int* iterations, double* residual, hipStream_t stream)
+{
+    if (!h || ngroups < 0 || (ngroups > 0 && (!groups || !iterations || !residual)) || max_iter < 0 ||
+        !(tol >= 0)) {
+        set_error("cpt_lu_solve_mixed_grouped: invalid arguments");
+        return CPT_ERR_ARG;
+    }
+    if (!valid_groups(groups, ngroups, false, true, true, "cpt_lu_solve_mixed_grouped")) return CPT_ERR_ARG;
+    if (ngroups == 0) return CPT_OK;
+    CPT_CHECK_HIP(hipSetDevice(h->device), "hipSetDevice");
+    TimerScope timer_scope(&h->timer);
+    if (ensure_lanes(h) || fork_lanes(h, stream)) {
+        set_error("cpt_lu_solve_mixed_grouped: could not set up the lane streams");
+        return CPT_ERR_HIP;
+    }
+    const std::vector<hipStream_t> lane = assign_lanes(h, groups, ngroups);
+    std::vector<MixedGroup> g(ngroups);
+    for (int i = 0; i < ngroups; ++i) {
+        const cpt_lu_group& G = groups[i];
+        g[i] = MixedGroup{(const cplx*)G.A, G.n, G.lda, G.strideA, G.batch, (const cplx*)G.B, G.nrhs, G.ldb,
+                          G.strideB, (cplx*)G.X, G.ldx, G.strideX, lane[i]};
+    }
+    if (h->mixed_work.reserve(mixed_grouped_bytes(g.data(), ngroups))) {
+        set_error("cpt_lu_solve_mixed_grouped: workspace allocation failed");
+        return CPT_ERR_ALLOC;
+    }
+    LuSetup tune = h->lu_streams;
+    tune.side = nullptr;
+    tune.lookahead = 0;
+    const int e = launch_mixed_grouped(g.data(), ngroups, h->mixed_work.p, &tune, max_iter, tol, iterations, residual);
+    if (join_lanes(h, stream)) {
+        set_error("cpt_lu_solve_mixed_grouped: could not join the lane streams");
+        return CPT_ERR_HIP;
+    }
+    CPT_CHECK_HIP((hipError_t)e, "grouped mixed solve");
+    return CPT_OK;
 }

@@ -108,23 +108,40 @@ __device__ __forceinline__ void wave_panel_finite_depth(const Table& T, double k
     }
 }
 
+// One tile: collocation points iblk * 256 + threadIdx.x against panels
+// jc * j_chunk .. + j_chunk, for every wavenumber of A.
+template <bool FINITE_DEPTH>
+__device__ __forceinline__ void assemble_tile(const AssemblyArgs& A, int iblk, int jc);
+
 // Infinite depth: two wavefronts per SIMD (about 220 VGPRs, no spills of the
 // main path); finite depth carries more live state and runs at one.
-template <bool FINITE_DEPTH>
-__device__ __forceinline__ void assemble_body(const AssemblyArgs& A);
-
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 assemble_infinite_depth_kernel(AssemblyArgs A)
 {
-    assemble_body<false>(A);
+    assemble_tile<false>(A, blockIdx.x, blockIdx.y);
 }
 
-__global__ void __launch_bounds__(256) assemble_finite_depth_kernel(AssemblyArgs A) { assemble_body<true>(A); }
+__global__ void __launch_bounds__(256) assemble_finite_depth_kernel(AssemblyArgs A)
+{
+    assemble_tile<true>(A, blockIdx.x, blockIdx.y);
+}
+
+// Persistent form over many bodies: each work-group walks the work-list of
+// (body, row block, panel chunk) tiles with a grid stride.  The tiles are
+// ordered by decreasing cost on the host so the stride balances the load.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+assemble_worklist_kernel(const AssemblyArgs* __restrict__ bodies, const AsmTile* __restrict__ tiles, int ntiles)
+{
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const AsmTile tl = tiles[t];
+        assemble_tile<false>(bodies[tl.body], tl.iblk, tl.jc);
+    }
+}
 
 template <bool FINITE_DEPTH>
-__device__ __forceinline__ void assemble_body(const AssemblyArgs& A)
+__device__ __forceinline__ void assemble_tile(const AssemblyArgs& A, int iblk, int jc)
 {
-    const int I = blockIdx.x * blockDim.x + threadIdx.x;
+    const int I = iblk * blockDim.x + threadIdx.x;
     const bool active = I < A.n;
     const int Ic = active ? I : 0;
     const double x0 = A.colloc[3 * (size_t)Ic], x1 = A.colloc[3 * (size_t)Ic + 1], x2 = A.colloc[3 * (size_t)Ic + 2];
@@ -135,7 +152,7 @@ __device__ __forceinline__ void assemble_body(const AssemblyArgs& A)
         nz = A.dpn[3 * (size_t)Ic + 2];
     }
     const bool deriv_first = A.adjoint != 0;
-    const int j_begin = blockIdx.y * A.j_chunk;
+    const int j_begin = jc * A.j_chunk;
     const int j_end = min(A.m, j_begin + A.j_chunk);
 
     for (int J = j_begin; J < j_end; ++J) {
@@ -274,6 +291,15 @@ int launch_assembly(const AssemblyArgs& A, hipStream_t stream)
     TimedLaunch timed(stream, TK_ASSEMBLY, (double)A.n * A.m * A.nb);
     if (A.finite_depth) hipLaunchKernelGGL(assemble_finite_depth_kernel, grid, block, 0, stream, A);
     else hipLaunchKernelGGL(assemble_infinite_depth_kernel, grid, block, 0, stream, A);
+    return (int)hipGetLastError();
+}
+
+int launch_assembly_worklist(const AssemblyArgs* bodies, const AsmTile* tiles, int ntiles, int grid, double pairs,
+                             hipStream_t stream)
+{
+    if (ntiles <= 0) return 0;
+    TimedLaunch timed(stream, TK_ASSEMBLY, pairs);
+    hipLaunchKernelGGL(assemble_worklist_kernel, dim3(grid), dim3(256), 0, stream, bodies, tiles, ntiles);
     return (int)hipGetLastError();
 }
 

@@ -40,6 +40,13 @@ struct AssemblyArgs {
 
 int launch_assembly(const AssemblyArgs& A, hipStream_t stream);
 
+// Work-list entry of the persistent multi-body assembly.
+struct AsmTile {
+    int body, iblk, jc, pad;
+};
+int launch_assembly_worklist(const AssemblyArgs* bodies, const AsmTile* tiles, int ntiles, int grid, double pairs,
+                             hipStream_t stream);
+
 // Tuning of the blocked LU and the side stream / events of its look-ahead
 // (owned by a context).
 struct LuSetup {
@@ -48,6 +55,33 @@ struct LuSetup {
     int block = 0;      // block width: 64 or 128, 0 = by size (128 from 1024 rows)
     int leaf = 16;      // panel columns factored by one work-group per matrix
     int lookahead = 1;  // factor the next panel on the side stream
+};
+
+// One group of the grouped (variable-size) LU: `batch` matrices of order n,
+// run on stream s.
+struct MixedGroup {
+    const cplx* K;
+    int n;
+    int64_t lda, strideK;
+    int batch;
+    const cplx* B;
+    int nrhs;
+    int64_t ldb, strideB;
+    cplx* X;
+    int64_t ldx, strideX;
+    hipStream_t s;
+};
+struct LuGroupArgs {
+    cplx* A;
+    int n, batch;
+    int64_t lda, strideA;
+    int* ipiv;
+    int64_t strideP;
+    int* info;
+    cplx* B;
+    int nrhs;
+    int64_t ldb, strideB;
+    hipStream_t s;
 };
 
 void set_error(const std::string& msg);

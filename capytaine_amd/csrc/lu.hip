@@ -1014,93 +1014,196 @@ int launch_getrs(const cplx* LU, int N, int64_t lda, int64_t strideA, const int*
 // Workspace of the mixed solve, in bytes: complex64 factors (batch n^2),
 // pivots and info, FP64 residuals (batch n nrhs), complex64 corrections,
 // the complex64 getrf workspace, and the norm buffers.
-size_t mixed_workspace_bytes(int n, int nrhs, int batch)
+// Workspace of one group of the mixed solve: complex64 copy of K, pivots,
+// info, FP64 residuals, complex64 corrections (256-byte aligned parts).
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+size_t mixed_group_bytes(int n, int nrhs, int batch)
 {
     const size_t b = (size_t)batch;
-    size_t bytes = 0;
-    bytes += sizeof(cx<float>) * b * n * n;                           // K32
-    bytes += sizeof(int) * (b * n + b);                               // ipiv, info
-    bytes = (bytes + 255) & ~(size_t)255;
-    bytes += sizeof(cx<double>) * b * n * nrhs;                       // residual
-    bytes += sizeof(cx<float>) * b * n * nrhs;                        // correction
-    bytes = (bytes + 255) & ~(size_t)255;
-    bytes += sizeof(cx<float>) * getrf_ws_elems<float>(n, batch);     // getrf workspace
-    bytes += 2 * sizeof(double) * b * nrhs;                           // norms
-    return bytes + 1024;
+    size_t bytes = align256(sizeof(cx<float>) * b * n * n);
+    bytes += align256(sizeof(int) * (b * n + b));
+    bytes += align256(sizeof(cx<double>) * b * n * nrhs);
+    bytes += align256(sizeof(cx<float>) * b * n * nrhs);
+    return bytes;
 }
 
-// Mixed-precision solve of K X = B (complex128 K, B, X): K is rounded to
+size_t mixed_workspace_bytes(int n, int nrhs, int batch)
+{
+    return mixed_group_bytes(n, nrhs, batch) + align256(2 * sizeof(double) * (size_t)batch * nrhs) + 256;
+}
+
+size_t mixed_grouped_bytes(const MixedGroup* g, int ng)
+{
+    size_t bytes = 256, norms = 0;
+    for (int i = 0; i < ng; ++i) {
+        bytes += mixed_group_bytes(g[i].n, g[i].nrhs, g[i].batch);
+        norms += (size_t)g[i].batch * g[i].nrhs;
+    }
+    return bytes + align256(2 * sizeof(double) * norms);
+}
+
+// Mixed-precision solve of K X = B for groups of complex128 systems (one
+// order per group), each group on its own stream: K is rounded to
 // complex64 and factorised on FP32 MFMA; X is refined with FP64 residuals
-// r = B - K X (FP64 MFMA GEMM against the unrounded K) and complex64
-// corrections until max_j ||r_j|| / ||b_j|| <= tol or max_iter sweeps.
-// Synchronises the stream once per sweep to read the residual norms.
+// r = B - K X (FP64 MFMA product against the unrounded K) and complex64
+// corrections until max_j ||r_j|| / ||b_j|| <= tol or max_iter sweeps, per
+// group.  The host waits for all streams once per sweep to read the
+// residual norms of every group at once.
+int launch_mixed_grouped(const MixedGroup* g, int ng, void* work, const LuSetup* tune, int max_iter, double tol,
+                         int* iterations, double* residual)
+{
+    typedef cx<float> c64;
+    typedef cx<double> c128;
+    struct State {
+        c64* K32;
+        int* ipiv;
+        int* info;
+        c128* Rz;
+        c64* D;
+        double* nb2;
+        double* nr2;
+        size_t norm_off;
+        bool active;
+    };
+    std::vector<State> st(ng);
+    char* p = (char*)(((uintptr_t)work + 255) & ~(uintptr_t)255);
+    size_t nnorm = 0;
+    for (int i = 0; i < ng; ++i) {
+        const size_t b = (size_t)g[i].batch, n = (size_t)g[i].n, r = (size_t)g[i].nrhs;
+        st[i].K32 = (c64*)p;
+        p += align256(sizeof(c64) * b * n * n);
+        st[i].ipiv = (int*)p;
+        st[i].info = st[i].ipiv + b * n;
+        p += align256(sizeof(int) * (b * n + b));
+        st[i].Rz = (c128*)p;
+        p += align256(sizeof(c128) * b * n * r);
+        st[i].D = (c64*)p;
+        p += align256(sizeof(c64) * b * n * r);
+        st[i].norm_off = nnorm;
+        nnorm += b * r;
+    }
+    double* nb2_all = (double*)p;
+    double* nr2_all = nb2_all + nnorm;
+    std::vector<double> hb(nnorm), hr(nnorm);
+    int e;
+    const dim3 blk(256);
+    for (int i = 0; i < ng; ++i) {
+        const MixedGroup& G = g[i];
+        State& S = st[i];
+        S.nb2 = nb2_all + S.norm_off;
+        S.nr2 = nr2_all + S.norm_off;
+        S.active = G.n > 0 && G.batch > 0 && G.nrhs > 0;
+        iterations[i] = 0;
+        residual[i] = 0.0;
+        if (!S.active) continue;
+        const int n = G.n, r = G.nrhs, batch = G.batch;
+        const int64_t sK32 = (int64_t)n * n, sRD = (int64_t)n * r;
+        const dim3 gK((n + 255) / 256, n, batch), gB((n + 255) / 256, r, batch);
+        hipLaunchKernelGGL(z2c_kernel, gK, blk, 0, G.s, (const c128*)G.K, G.lda, G.strideK, S.K32, (int64_t)n, sK32,
+                           n, n);
+        hipLaunchKernelGGL(zero_ints_kernel, dim3((batch + 255) / 256), blk, 0, G.s, S.info, batch);
+        if ((e = getrf<float>(S.K32, n, n, sK32, S.ipiv, n, S.info, batch, nullptr, tune, G.s))) return e;
+        // x0 = LU32 \ b
+        hipLaunchKernelGGL(z2c_kernel, gB, blk, 0, G.s, (const c128*)G.B, G.ldb, G.strideB, S.D, (int64_t)n, sRD, n,
+                           r);
+        if ((e = getrs<float>(S.K32, n, n, sK32, S.ipiv, n, S.D, r, n, sRD, batch, G.s))) return e;
+        hipLaunchKernelGGL(c2z_add_kernel, gB, blk, 0, G.s, S.D, (int64_t)n, sRD, (c128*)G.X, G.ldx, G.strideX, n, r,
+                           0);
+        hipLaunchKernelGGL(colnorm2_kernel, dim3(r, batch), blk, 0, G.s, (const c128*)G.B, G.ldb, G.strideB, n, r,
+                           S.nb2);
+        if ((e = (int)hipGetLastError())) return e;
+    }
+    bool have_b = false;
+    for (;;) {
+        bool any = false;
+        for (int i = 0; i < ng; ++i) {
+            if (!st[i].active) continue;
+            any = true;
+            const MixedGroup& G = g[i];
+            State& S = st[i];
+            const int n = G.n, r = G.nrhs, batch = G.batch;
+            const int64_t sRD = (int64_t)n * r;
+            const dim3 gB((n + 255) / 256, r, batch);
+            // r = b - K x  (FP64)
+            hipLaunchKernelGGL(copy_z_kernel, gB, blk, 0, G.s, (const c128*)G.B, G.ldb, G.strideB, S.Rz, (int64_t)n,
+                               sRD, n);
+            if ((e = gemm<double>(n, r, n, -1.0, (const c128*)G.K, G.lda, G.strideK, (const c128*)G.X, G.ldx,
+                                  G.strideX, S.Rz, n, sRD, 0, batch, G.s)))
+                return e;
+            hipLaunchKernelGGL(colnorm2_kernel, dim3(r, batch), blk, 0, G.s, S.Rz, (int64_t)n, sRD, n, r, S.nr2);
+            if ((e = (int)hipGetLastError())) return e;
+        }
+        if (!any) break;
+        for (int i = 0; i < ng; ++i)
+            if (st[i].active && hipStreamSynchronize(g[i].s) != hipSuccess) return (int)hipErrorUnknown;
+        if (!have_b) {
+            if (hipMemcpy(hb.data(), nb2_all, sizeof(double) * nnorm, hipMemcpyDeviceToHost) != hipSuccess)
+                return (int)hipErrorUnknown;
+            have_b = true;
+        }
+        if (hipMemcpy(hr.data(), nr2_all, sizeof(double) * nnorm, hipMemcpyDeviceToHost) != hipSuccess)
+            return (int)hipErrorUnknown;
+        for (int i = 0; i < ng; ++i) {
+            State& S = st[i];
+            if (!S.active) continue;
+            const MixedGroup& G = g[i];
+            double worst = 0.0;
+            for (size_t j = S.norm_off; j < S.norm_off + (size_t)G.batch * G.nrhs; ++j) {
+                const double rel = hb[j] > 0 ? std::sqrt(hr[j] / hb[j]) : std::sqrt(hr[j]);
+                worst = std::isnan(rel) ? INFINITY : std::max(worst, rel);
+            }
+            residual[i] = worst;
+            if (worst <= tol || iterations[i] >= max_iter) {
+                S.active = false;
+                continue;
+            }
+            // x += LU32 \ r
+            const int n = G.n, r = G.nrhs, batch = G.batch;
+            const int64_t sK32 = (int64_t)n * n, sRD = (int64_t)n * r;
+            const dim3 gB((n + 255) / 256, r, batch);
+            hipLaunchKernelGGL(z2c_kernel, gB, blk, 0, G.s, S.Rz, (int64_t)n, sRD, S.D, (int64_t)n, sRD, n, r);
+            if ((e = getrs<float>(S.K32, n, n, sK32, S.ipiv, n, S.D, r, n, sRD, batch, G.s))) return e;
+            hipLaunchKernelGGL(c2z_add_kernel, gB, blk, 0, G.s, S.D, (int64_t)n, sRD, (c128*)G.X, G.ldx, G.strideX,
+                               n, r, 1);
+            if ((e = (int)hipGetLastError())) return e;
+            ++iterations[i];
+        }
+    }
+    return 0;
+}
+
 int launch_mixed_solve(const cplx* K, int n, int64_t lda, int64_t strideK, int batch, const cplx* B, int nrhs,
                        int64_t ldb, int64_t strideB, cplx* X, int64_t ldx, int64_t strideX, int max_iter, double tol,
                        void* work, const LuSetup* aux, int* iterations, double* residual, hipStream_t stream)
 {
-    typedef cx<float> c64;
-    typedef cx<double> c128;
-    const size_t b = (size_t)batch;
-    char* p = (char*)work;
-    c64* K32 = (c64*)p;
-    p += sizeof(c64) * b * n * n;
-    int* ipiv = (int*)p;
-    p += sizeof(int) * b * n;
-    int* info = (int*)p;
-    p += sizeof(int) * b;
-    p = (char*)(((uintptr_t)p + 255) & ~(uintptr_t)255);
-    c128* Rz = (c128*)p;
-    p += sizeof(c128) * b * n * nrhs;
-    c64* D = (c64*)p;
-    p += sizeof(c64) * b * n * nrhs;
-    p = (char*)(((uintptr_t)p + 255) & ~(uintptr_t)255);
-    c64* gws = (c64*)p;
-    p += sizeof(c64) * getrf_ws_elems<float>(n, batch);
-    double* nb2 = (double*)p;
-    double* nr2 = nb2 + b * nrhs;
-    const int64_t sK32 = (int64_t)n * n, sRD = (int64_t)n * nrhs;
+    MixedGroup g{K, n, lda, strideK, batch, B, nrhs, ldb, strideB, X, ldx, strideX, stream};
+    return launch_mixed_grouped(&g, 1, work, aux, max_iter, tol, iterations, residual);
+}
+
+int launch_getrf_lanes(const LuGroupArgs* g, int ng, const LuSetup* tune)
+{
     int e;
-    const dim3 blk(256);
-    const dim3 gK((n + 255) / 256, n, batch), gB((n + 255) / 256, nrhs, batch);
-    hipLaunchKernelGGL(z2c_kernel, gK, blk, 0, stream, (const c128*)K, lda, strideK, K32, (int64_t)n, sK32, n, n);
-    hipLaunchKernelGGL(zero_ints_kernel, dim3((batch + 255) / 256), blk, 0, stream, info, batch);
-    if ((e = getrf<float>(K32, n, n, sK32, ipiv, n, info, batch, gws, aux, stream))) return e;
-    // x0 = LU32 \ b
-    hipLaunchKernelGGL(z2c_kernel, gB, blk, 0, stream, (const c128*)B, ldb, strideB, D, (int64_t)n, sRD, n, nrhs);
-    if ((e = getrs<float>(K32, n, n, sK32, ipiv, n, D, nrhs, n, sRD, batch, stream))) return e;
-    hipLaunchKernelGGL(c2z_add_kernel, gB, blk, 0, stream, D, (int64_t)n, sRD, (c128*)X, ldx, strideX, n, nrhs, 0);
-    hipLaunchKernelGGL(colnorm2_kernel, dim3(nrhs, batch), blk, 0, stream, (const c128*)B, ldb, strideB, n, nrhs, nb2);
-    if ((e = (int)hipGetLastError())) return e;
-    std::vector<double> hb(b * nrhs), hr(b * nrhs);
-    if (hipMemcpyAsync(hb.data(), nb2, sizeof(double) * b * nrhs, hipMemcpyDeviceToHost, stream) != hipSuccess)
-        return (int)hipErrorUnknown;
-    double worst = INFINITY;
-    int it = 0;
-    for (;;) {
-        // r = b - K x  (FP64)
-        hipLaunchKernelGGL(copy_z_kernel, gB, blk, 0, stream, (const c128*)B, ldb, strideB, Rz, (int64_t)n, sRD, n);
-        if ((e = gemm<double>(n, nrhs, n, -1.0, (const c128*)K, lda, strideK, (const c128*)X, ldx, strideX, Rz, n, sRD,
-                              0, batch, stream)))
+    for (int i = 0; i < ng; ++i) {
+        if (g[i].n <= 0 || g[i].batch <= 0) continue;
+        hipLaunchKernelGGL(zero_ints_kernel, dim3((g[i].batch + 255) / 256), dim3(256), 0, g[i].s, g[i].info,
+                           g[i].batch);
+        if ((e = getrf<double>((cx<double>*)g[i].A, g[i].n, g[i].lda, g[i].strideA, g[i].ipiv, g[i].strideP,
+                               g[i].info, g[i].batch, nullptr, tune, g[i].s)))
             return e;
-        hipLaunchKernelGGL(colnorm2_kernel, dim3(nrhs, batch), blk, 0, stream, Rz, (int64_t)n, sRD, n, nrhs, nr2);
-        if (hipMemcpyAsync(hr.data(), nr2, sizeof(double) * b * nrhs, hipMemcpyDeviceToHost, stream) != hipSuccess)
-            return (int)hipErrorUnknown;
-        if (hipStreamSynchronize(stream) != hipSuccess) return (int)hipErrorUnknown;
-        worst = 0.0;
-        for (size_t i = 0; i < b * nrhs; ++i) {
-            const double rel = hb[i] > 0 ? std::sqrt(hr[i] / hb[i]) : std::sqrt(hr[i]);
-            worst = std::isnan(rel) ? INFINITY : std::max(worst, rel);
-        }
-        if (worst <= tol || it >= max_iter) break;
-        // x += LU32 \ r
-        hipLaunchKernelGGL(z2c_kernel, gB, blk, 0, stream, Rz, (int64_t)n, sRD, D, (int64_t)n, sRD, n, nrhs);
-        if ((e = getrs<float>(K32, n, n, sK32, ipiv, n, D, nrhs, n, sRD, batch, stream))) return e;
-        hipLaunchKernelGGL(c2z_add_kernel, gB, blk, 0, stream, D, (int64_t)n, sRD, (c128*)X, ldx, strideX, n, nrhs, 1);
-        if ((e = (int)hipGetLastError())) return e;
-        ++it;
     }
-    *iterations = it;
-    *residual = worst;
+    return 0;
+}
+
+int launch_getrs_lanes(const LuGroupArgs* g, int ng)
+{
+    int e;
+    for (int i = 0; i < ng; ++i) {
+        if (g[i].n <= 0 || g[i].batch <= 0 || g[i].nrhs <= 0) continue;
+        if ((e = getrs<double>((const cx<double>*)g[i].A, g[i].n, g[i].lda, g[i].strideA, g[i].ipiv, g[i].strideP,
+                               (cx<double>*)g[i].B, g[i].nrhs, g[i].ldb, g[i].strideB, g[i].batch, g[i].s)))
+            return e;
+    }
     return 0;
 }
 

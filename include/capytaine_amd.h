@@ -105,6 +105,37 @@ int cpt_build_matrices(cpt_handle h, int nb_collocation_points, const double* co
                        int diagonal_term_in_double_layer, double free_surface, void* S, int64_t ldS,
                        int64_t strideS, void* K, int64_t ldK, int64_t strideK, hipStream_t stream);
 
+/* One body of cpt_build_matrices_batched: the arrays cpt_build_matrices
+ * takes, for a mesh against itself (collocation points and dot-product
+ * normals are the panel centres and normals, as BEMSolver uses them). */
+typedef struct {
+    int nb_faces, nb_vertices, nb_quad_points, nb_wavenumbers;
+    const double* vertices;       /* (nv,3) device */
+    const int32_t* faces;         /* (m,4) device, 0-based */
+    const double* centers;        /* (m,3) device */
+    const double* normals;        /* (m,3) device */
+    const double* areas;          /* (m) device */
+    const double* radiuses;       /* (m) device */
+    const double* quad_points;    /* (m,nq,3) device */
+    const double* quad_weights;   /* (m,nq) device */
+    const double* wavenumbers;    /* HOST (nb) */
+    const int* gf_singularities;  /* HOST (nb), after the overrides of delhommeau.py:466-490 */
+    void* S;                      /* complex128, S[i + ldS*j] + b*strideS, OVERWRITTEN */
+    int64_t ldS, strideS;
+    void* K;                      /* complex128, K[i + ldK*j] + b*strideK (early dot product), OVERWRITTEN */
+    int64_t ldK, strideK;
+} cpt_body;
+
+/* S and K of many bodies, each at its own list of wavenumbers, in ONE
+ * persistent launch walking a (body, row block, panel chunk) work-list.
+ * Infinite depth and early dot product only; other options as in
+ * cpt_build_matrices.  Equal, entry by entry, to one cpt_build_matrices call
+ * per body.  Replaces the loop of BEMSolver.solve_all over bodies and
+ * frequencies (src/capytaine/bem/solver.py:268-372) down to
+ * matrices.build_matrices (libs/libDelhommeau/src/matrices.f90:28-325). */
+int cpt_build_matrices_batched(cpt_handle h, int nb_bodies, const cpt_body* bodies, int adjoint_double_layer,
+                               int diagonal_term_in_double_layer, double free_surface, hipStream_t stream);
+
 /* Wave part of the infinite-depth Green function at np point pairs.
  * Replaces interface.vectorized_wave_part_infinite_depth
  * (libs/libDelhommeau/src/interface.f90:29-67) and
@@ -146,6 +177,37 @@ int cpt_lu_solve(cpt_handle h, const void* LU, int n, int64_t lda, int64_t strid
 int cpt_lu_solve_mixed(cpt_handle h, const void* K, int n, int64_t lda, int64_t strideK, int batch, const void* B,
                        int nrhs, int64_t ldb, int64_t strideB, void* X, int64_t ldx, int64_t strideX, int max_iter,
                        double tol, int* iterations, double* residual, hipStream_t stream);
+
+/* One group of the grouped (variable-size) LU: `batch` matrices of order n
+ * with the strides of cpt_lu_factor / cpt_lu_solve / cpt_lu_solve_mixed.
+ * Fields a call does not use may be zero. */
+typedef struct {
+    int n, batch;
+    void* A;            /* complex128: factored in place (factor), factors (solve), K (mixed, read) */
+    int64_t lda, strideA;
+    int* ipiv;          /* factor, solve */
+    int64_t strideP;
+    int* info;          /* factor: (batch) */
+    void* B;            /* solve: right-hand sides, overwritten by X; mixed: read */
+    int nrhs;
+    int64_t ldb, strideB;
+    void* X;            /* mixed: solution */
+    int64_t ldx, strideX;
+} cpt_lu_group;
+
+/* Variable-size batched LU: each group is one cpt_lu_factor /
+ * cpt_lu_solve / cpt_lu_solve_mixed batch; the groups run concurrently on
+ * the context's lane streams (longest first), ordered after the work
+ * already on `stream`, which waits for all of them.  The mixed form
+ * synchronises once per refinement sweep for all groups together and
+ * writes iterations[g] and residual[g] per group.  Replaces the per-problem
+ * LAPACK calls of DefaultMatrixEngine.linear_solver
+ * (src/capytaine/bem/engines.py:300-346) over the bodies and frequencies of
+ * BEMSolver.solve_all (src/capytaine/bem/solver.py:268-372). */
+int cpt_lu_factor_grouped(cpt_handle h, int ngroups, const cpt_lu_group* groups, hipStream_t stream);
+int cpt_lu_solve_grouped(cpt_handle h, int ngroups, const cpt_lu_group* groups, hipStream_t stream);
+int cpt_lu_solve_mixed_grouped(cpt_handle h, int ngroups, const cpt_lu_group* groups, int max_iter, double tol,
+                               int* iterations, double* residual, hipStream_t stream);
 
 /* C = alpha*A*B + (beta_zero ? 0 : C), complex128, column-major, batched.
  * Replaces the products `S @ sources` of src/capytaine/bem/solver.py:197-198

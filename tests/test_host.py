@@ -125,3 +125,36 @@ def test_context_creation_does_not_deadlock():
     t.start()
     t.join(30)
     assert outcome, "Context.get blocked"
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """The ctypes mirrors of cpt_body and cpt_lu_group have the C layout of
+    include/capytaine_amd.h (offsets and sizes from gcc on the header)."""
+    import ctypes
+    import shutil
+    import subprocess
+    from capytaine_amd._lib import CptBody, CptLuGroup
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "capytaine_amd.h"', 'int main(void) {']
+    for cname, cls in (("cpt_body", CptBody), ("cpt_lu_group", CptLuGroup)):
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in cls._fields_:
+            lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(root, "include"), "-I", "/opt/rocm/include",
+                    "-D__HIP_PLATFORM_AMD__", str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {}
+    for line in out:
+        if line:
+            c, f, v = line.split()
+            got[(c, f)] = int(v)
+    for cname, cls in (("cpt_body", CptBody), ("cpt_lu_group", CptLuGroup)):
+        assert got[(cname, "size")] == ctypes.sizeof(cls)
+        for fname, _ in cls._fields_:
+            assert got[(cname, fname)] == getattr(cls, fname).offset, (cname, fname)
