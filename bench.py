@@ -77,12 +77,80 @@ def cpu_baseline(mesh, body, pbs_by_freq, table, n_freq=6):
                        f"{dt:.1f} s")
 
 
+def fleet_benchmark(gf, dev, rho=1025.0, g=9.81, seed=0, budget_gb=110.0, log=None):
+    """64 half-immersed ellipsoid hulls (panel counts log-uniform in
+    300-6000), 16 frequencies, 6 dof + 4 headings: one BatchedSweep per
+    memory chunk (persistent multi-body assembly + grouped LU), mixed and
+    FP64.  Inputs (boundary conditions) are resident before timing."""
+    import torch
+    from capytaine_amd import workloads
+    from capytaine_amd.batch import BatchedSweep, chunks_by_memory, footprint_bytes
+    from capytaine_amd.sweep import boundary_conditions
+    from capytaine_amd import _lib
+    # library defaults (block width by matrix order) for the mix of sizes
+    _lib.set_tuning(_lib.Context.get(dev), lu_block=0, panel_leaf=16, lookahead=1, assembly_chunk=0)
+    bodies, omegas, headings = workloads.hull_fleet(seed)
+    ks = omegas ** 2 / g
+    nrhs = 6 + len(headings)
+    ns = [b.mesh.nb_faces for b in bodies]
+    bcs = []
+    for b in bodies:
+        pbs = workloads.problems_by_frequency(b, omegas, headings, rho=rho, g=g)
+        bcs.append(boundary_conditions(pbs, nrhs, b.mesh.nb_faces, dev))
+    out = {"workload": f"64 ellipsoid hulls (seed {seed}), panels {min(ns)}-{max(ns)} log-uniform, 16 frequencies "
+                       f"0.3-3.0 rad/s, 6 dof + 4 headings, infinite depth",
+           "problems": len(bodies) * len(ks) * nrhs, "panel_pairs": int(sum(len(ks) * n * n for n in ns))}
+    forces = {}
+    for precision in ("fp64", "mixed"):
+        sizes = [footprint_bytes(n, len(ks), nrhs, precision) for n in ns]
+        chunks = chunks_by_memory(sizes, budget_gb * 1e9)
+        t_total, t_asm, sweeps, res = 0.0, 0.0, 0, 0.0
+        forces[precision] = [None] * len(bodies)
+        for ch in chunks:
+            bs = BatchedSweep(gf, [bodies[i] for i in ch], ks, nrhs, precision=precision)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            t0 = time.perf_counter()
+            e0.record()
+            bs.assemble()
+            e1.record()
+            bs.solve([bcs[i] for i in ch])
+            bs.post()
+            torch.cuda.synchronize()
+            t_total += time.perf_counter() - t0
+            t_asm += e0.elapsed_time(e1) / 1e3
+            if precision == "mixed":
+                sweeps = max(sweeps, int(bs.iterations.max()))
+                res = max(res, float(bs.residual.max()))
+            for j, f in zip(ch, bs.forces([omegas] * len(ch), rho)):
+                forces[precision][j] = f
+            del bs
+            torch.cuda.empty_cache()
+        out[f"{precision}_problems_per_s"] = out["problems"] / t_total
+        out[f"{precision}_seconds"] = t_total
+        out[f"{precision}_assembly_panel_pairs_per_s"] = out["panel_pairs"] / t_asm
+        out[f"{precision}_launches"] = len(chunks)
+        if precision == "mixed":
+            out["mixed_refinement_sweeps_max"] = sweeps
+            out["mixed_final_rel_residual_max"] = res
+        if log:
+            log(f"[fleet] {precision}: {t_total:.2f} s in {len(chunks)} chunk(s)")
+    err = 0.0
+    for f64, fmx in zip(forces["fp64"], forces["mixed"]):
+        scale = np.abs(f64).max(axis=2, keepdims=True)
+        err = max(err, float(np.max(np.abs(fmx - f64) / scale)))
+    out["mixed_over_fp64"] = out["mixed_problems_per_s"] / out["fp64_problems_per_s"]
+    out["mixed_max_rel_err_vs_fp64"] = err
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fleet", action="store_true", help="skip the 64-hull batched measurement")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner (a stored tuning of this size class is still applied)")
     args = ap.parse_args()
@@ -240,6 +308,8 @@ def main():
         "roofline": roofline,
         "tuning": {k: {kk: vv for kk, vv in v.items() if kk != "timings_ms"} for k, v in tuning.items()},
     }
+    if not args.no_fleet:
+        out["fleet"] = fleet_benchmark(gf, dev, log=lambda m: print(m, file=sys.stderr, flush=True))
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(mesh, body, pbs_by_freq,
                                            (gf.tabulated_r_range, gf.tabulated_z_range, gf.tabulated_integrals))
