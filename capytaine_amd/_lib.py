@@ -122,6 +122,13 @@ class Context:
         self.handle = h
         self.table_key = None
 
+    def close(self):
+        """Destroy a context made with Context(device_index) (not one of the
+        shared per-device contexts of Context.get)."""
+        if self.handle is not None and self._contexts.get(self.device_index) is not self:
+            check(load().cpt_destroy(self.handle), "cpt_destroy")
+            self.handle = None
+
     @classmethod
     def get(cls, device):
         import torch

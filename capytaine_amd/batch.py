@@ -58,7 +58,7 @@ class BatchedSweep:
     """
 
     def __init__(self, green_function, bodies, wavenumbers, nrhs, *, precision="mixed", max_iter=10, tol=1e-12,
-                 free_surface=0.0):
+                 free_surface=0.0, context=None):
         import torch
         if precision not in ("fp64", "mixed"):
             raise ValueError(f"precision must be 'fp64' or 'mixed', got {precision!r}")
@@ -91,16 +91,18 @@ class BatchedSweep:
             W = -(b.dof_normal_matrix() * mesh.faces_areas[None, :]).T
             self.W.append(torch.as_tensor(np.ascontiguousarray(W), dtype=cplx).to(dev))
             self.G.append(torch.empty((len(k), r, W.shape[1]), dtype=cplx, device=dev))
-        self.ctx = _lib.Context.get(dev)
+        # a private context (own workspace) lets several sweeps run at once
+        # on different streams; the work goes to the current stream
+        self.ctx = self.gf._context(context)  # with the table uploaded
         self.stream = stream_handle(dev)
         self.iterations = np.zeros(nbody, dtype=np.int32)
         self.residual = np.zeros(nbody)
-        self.gf._context()  # table upload
         self._gfs = [np.array([_SING[self.gf._gf_singularities_for(np.inf, float(kk), m)] for kk in k],
                               dtype=np.int32) for k, m in zip(self.ks, self.meshes)]
         self._bodies_c = (CptBody * nbody)()
+        self._mesh_arrays = [mesh.device_arrays(dev) for mesh in self.meshes]  # kept alive with the sweep
         for i, (mesh, k) in enumerate(zip(self.meshes, self.ks)):
-            md = mesh.device_arrays(dev)
+            md = self._mesh_arrays[i]
             n = self.n[i]
             self._bodies_c[i] = CptBody(
                 nb_faces=n, nb_vertices=mesh.nb_vertices, nb_quad_points=md["nq"], nb_wavenumbers=len(k),

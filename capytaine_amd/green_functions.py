@@ -318,9 +318,10 @@ class Delhommeau(AbstractGreenFunction):
         return (self.tabulation_nb_integration_points, self.tabulation_grid_shape_index,
                 self.tabulated_r_range, self.tabulated_z_range, self.tabulated_integrals)
 
-    def _context(self):
-        """Library context of the device with this Green function's table loaded."""
-        ctx = _lib.Context.get(self.device)
+    def _context(self, ctx=None):
+        """Library context (the device's shared one by default) with this
+        Green function's table loaded."""
+        ctx = _lib.Context.get(self.device) if ctx is None else ctx
         key = id(self._table_device[2])
         if ctx.table_key != key:
             r_d, z_d, tab_d = self._table_device

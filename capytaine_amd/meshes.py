@@ -18,10 +18,15 @@ The generators rebuild the reference's predefined meshes
 the same panel layout, plus an ellipsoid hull generator for batched and
 serving workloads (the reference has no ellipsoid mesher).
 """
+import threading
+
 import numpy as np
 
 __all__ = ["Mesh", "mesh_sphere", "mesh_parallelepiped", "mesh_rectangle", "mesh_ellipsoid",
            "mesh_vertical_cylinder", "mesh_horizontal_rectangle_lid"]
+
+
+_device_cache_lock = threading.Lock()
 
 
 def _cross(a, b):
@@ -184,6 +189,11 @@ class Mesh:
     def device_arrays(self, device):
         import torch
         key = str(device)
+        with _device_cache_lock:
+            return self._device_arrays_locked(device, key)
+
+    def _device_arrays_locked(self, device, key):
+        import torch
         if key not in self._device_cache:
             qp, qw = self.quadrature_points
             as_t = lambda a, dt=torch.float64: torch.as_tensor(np.ascontiguousarray(a), dtype=dt).to(device)

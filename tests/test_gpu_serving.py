@@ -1,0 +1,68 @@
+"""Serving loop: a request's result is the same alone and in a mix of
+concurrent requests, batches overlap on the device, and the coefficients
+match the one-body FP64 sweep.
+
+Tolerances: alone vs in the mix within 1e-12 relative (the per-body
+arithmetic does not depend on the batch); mixed precision vs the FP64 sweep
+within 1e-10 relative (refinement to a 1e-12 residual)."""
+import numpy as np
+import pytest
+
+from gpu_helpers import device, ref_table_gf
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def test_result_alone_equals_result_in_mix(default_table):
+    from capytaine_amd import workloads
+    from capytaine_amd.serving import BEMServer
+    from capytaine_amd.sweep import FrequencySweep, boundary_conditions
+    print("[gpu] serving: alone vs mix", flush=True)
+    gf = ref_table_gf(default_table)
+    target = workloads.hull(4.0, 2.0, 1.2, 380, "target")
+    om_t, hd_t = np.array([0.4, 0.9, 1.7]), np.array([0.0, np.pi / 3])
+    rng = np.random.default_rng(11)
+    others = [workloads.serving_request(rng, n_range=(120, 700)) for _ in range(6)]
+    with BEMServer(gf, slots=2, max_batch_jobs=2, rho=1025.0) as srv:
+        alone = srv.submit(target, om_t, hd_t).result(timeout=120)
+        futs = [srv.submit(b, om, hd) for b, om, hd in others[:3]]
+        fut_t = srv.submit(target, om_t, hd_t)
+        futs += [srv.submit(b, om, hd) for b, om, hd in others[3:]]
+        mixed = fut_t.result(timeout=120)
+        for f in futs:
+            f.result(timeout=120)
+        stats = srv.stats()
+    print(f"[gpu] serving stats {stats}", flush=True)
+    assert stats["completed"] == 8 and stats["max_inflight_batches"] >= 2
+    for key in ("added_mass", "radiation_damping", "diffraction_force", "excitation_force"):
+        assert _rel(mixed[key], alone[key]) <= 1e-12, key
+    # against the one-body FP64 sweep
+    ks = om_t ** 2 / 9.81
+    pbs = workloads.problems_by_frequency(target, om_t, hd_t, rho=1025.0)
+    sw = FrequencySweep(gf, target, ks, len(pbs[0]), precision="fp64")
+    sw.run(boundary_conditions(pbs, len(pbs[0]), target.mesh.nb_faces, device()))
+    ref = sw.forces(om_t, 1025.0)
+    assert _rel(alone["forces"], ref) <= 1e-10
+    assert alone["final_rel_residual"] <= 1e-12
+
+
+def test_request_split_by_memory(default_table):
+    """A request larger than the job size runs as frequency chunks; the
+    result equals the unsplit one."""
+    from capytaine_amd import workloads
+    from capytaine_amd.batch import footprint_bytes
+    from capytaine_amd.serving import BEMServer
+    print("[gpu] serving: frequency chunks", flush=True)
+    gf = ref_table_gf(default_table)
+    body = workloads.hull(5.0, 2.0, 1.0, 300, "split")
+    om, hd = np.linspace(0.3, 2.0, 5), np.array([0.0])
+    with BEMServer(gf, slots=2, rho=1025.0) as srv:
+        whole = srv.submit(body, om, hd).result(timeout=120)
+    per_freq = footprint_bytes(body.mesh.nb_faces, 1, 7, "mixed")
+    with BEMServer(gf, slots=2, rho=1025.0, max_job_bytes=2 * per_freq) as srv:
+        split = srv.submit(body, om, hd).result(timeout=120)
+    assert _rel(split["forces"], whole["forces"]) <= 1e-12
