@@ -124,8 +124,8 @@ def fleet_benchmark(gf, dev, rho=1025.0, g=9.81, seed=0, budget_gb=110.0, log=No
                 res = max(res, float(bs.residual.max()))
             for j, f in zip(ch, bs.forces([omegas] * len(ch), rho)):
                 forces[precision][j] = f
-            del bs
-            torch.cuda.empty_cache()
+            del bs  # its blocks are reused by the next chunk (same stream)
+        torch.cuda.empty_cache()
         out[f"{precision}_problems_per_s"] = out["problems"] / t_total
         out[f"{precision}_seconds"] = t_total
         out[f"{precision}_assembly_panel_pairs_per_s"] = out["panel_pairs"] / t_asm
@@ -144,6 +144,73 @@ def fleet_benchmark(gf, dev, rho=1025.0, g=9.81, seed=0, budget_gb=110.0, log=No
     return out
 
 
+def serving_benchmark(gf, dev, seed=1, requests=24, loads=(0.5, 0.8, 1.0, 1.2), log=None):
+    """Poisson request stream against the continuous-batching server:
+    saturation throughput from a burst of `requests` requests, then
+    latency percentiles at arrival rates of `loads` x saturation."""
+    import threading
+    from capytaine_amd import workloads
+    from capytaine_amd.serving import BEMServer
+
+    def make(rng, count):
+        return [workloads.serving_request(rng) for _ in range(count)]
+
+    def problems(reqs):
+        return sum(len(om) * (len(b.dofs) + len(hd)) for b, om, hd in reqs)
+
+    rng = np.random.default_rng(seed)
+    out = {"workload": f"hull requests (seed {seed}): 500-12000 panels log-uniform, 1-32 frequencies in 0.2-3.0 "
+                       "rad/s, random dof subset, 0-8 headings; mixed precision; Poisson arrivals"}
+    with BEMServer(gf, rho=1025.0) as srv:
+        # warm: first use of the slot contexts
+        b, om, hd = workloads.serving_request(np.random.default_rng(seed + 1000), n_range=(500, 800))
+        srv.submit(b, om[:1], hd).result()
+        burst = make(rng, requests)
+        t0 = time.perf_counter()
+        futs = [srv.submit(*r) for r in burst]
+        for f in futs:
+            f.result()
+        span = time.perf_counter() - t0
+        sat = requests / span
+        out["saturation_requests_per_s"] = sat
+        out["saturation_solves_per_s"] = problems(burst) / span
+        if log:
+            log(f"[serving] saturation {sat:.2f} requests/s ({span:.2f} s for {requests})")
+        out["loads"] = []
+        for load in loads:
+            reqs = make(rng, requests)
+            gaps = rng.exponential(1.0 / (load * sat), size=requests)
+            arrivals = np.cumsum(gaps) - gaps[0]
+            n_before = len(srv.latencies)
+            futs = []
+
+            def client():
+                start = time.perf_counter()
+                for r, t in zip(reqs, arrivals):
+                    delay = start + t - time.perf_counter()
+                    if delay > 0:
+                        time.sleep(delay)
+                    futs.append(srv.submit(*r))
+
+            t0 = time.perf_counter()
+            th = threading.Thread(target=client)
+            th.start()
+            th.join()
+            for f in futs:
+                f.result()
+            span = time.perf_counter() - t0
+            lat = np.array(srv.latencies[n_before:])
+            row = {"load": load, "requests": requests, "p50_s": float(np.percentile(lat, 50)),
+                   "p99_s": float(np.percentile(lat, 99)), "requests_per_s": requests / span,
+                   "solves_per_s": problems(reqs) / span}
+            out["loads"].append(row)
+            if log:
+                log(f"[serving] load {load}: p50 {row['p50_s']:.3f} s p99 {row['p99_s']:.3f} s, "
+                    f"{row['solves_per_s']:.0f} solves/s")
+        out["max_inflight_batches"] = srv.max_inflight_batches
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -151,6 +218,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fleet", action="store_true", help="skip the 64-hull batched measurement")
+    ap.add_argument("--no-serving", action="store_true", help="skip the request-stream latency measurement")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner (a stored tuning of this size class is still applied)")
     args = ap.parse_args()
@@ -310,6 +378,9 @@ def main():
     }
     if not args.no_fleet:
         out["fleet"] = fleet_benchmark(gf, dev, log=lambda m: print(m, file=sys.stderr, flush=True))
+    if not args.no_serving:
+        out["serving"] = serving_benchmark(gf, dev, log=lambda m: print(m, file=sys.stderr, flush=True))
+        out["p99_latency_s_at_saturation"] = next(r["p99_s"] for r in out["serving"]["loads"] if r["load"] == 1.0)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(mesh, body, pbs_by_freq,
                                            (gf.tabulated_r_range, gf.tabulated_z_range, gf.tabulated_integrals))

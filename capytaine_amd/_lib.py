@@ -40,6 +40,9 @@ SIGNATURES = {
     "cpt_lu_factor_grouped": (_I, [_P, _I, _P, _P]),
     "cpt_lu_solve_grouped": (_I, [_P, _I, _P, _P]),
     "cpt_lu_solve_mixed_grouped": (_I, [_P, _I, _P, _I, _D, _P, _P, _P]),
+    "cpt_release_workspace": (_I, [_P]),
+    "cpt_set_workspace": (_I, [_P, _P, ctypes.c_size_t]),
+    "cpt_mixed_workspace_size": (_I, [_P, _I, _P, ctypes.POINTER(ctypes.c_size_t)]),
     "cpt_set_tuning": (_I, [_P, _I, _I]),
     "cpt_get_tuning": (_I, [_P, _I, ctypes.POINTER(ctypes.c_int)]),
 }
@@ -72,8 +75,27 @@ _lib = None
 _lock = threading.RLock()  # Context.get -> Context.__init__ -> load() re-enters it
 
 
+CPT_ERR_ALLOC = -3
+
+
 class CptError(RuntimeError):
     """Error reported by libcapytaine_amd."""
+
+    def __init__(self, msg, status=None):
+        super().__init__(msg)
+        self.status = status
+
+
+def call_with_memory_retry(fn, what):
+    """fn() -> status; on a device allocation failure, return torch's cached
+    device memory to the driver and try once more."""
+    status = fn()
+    if status == CPT_ERR_ALLOC:
+        import torch
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+        status = fn()
+    check(status, what)
 
 
 def load():
@@ -95,7 +117,7 @@ def load():
 def check(status, what):
     if status != 0:
         msg = load().cpt_last_error().decode(errors="replace")
-        raise CptError(f"{what} failed with status {status}: {msg}")
+        raise CptError(f"{what} failed with status {status}: {msg}", status)
 
 
 def ptr(t):

@@ -21,7 +21,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import CptBody, CptLuGroup, check, ptr, stream_handle
+from ._lib import CptBody, CptLuGroup, call_with_memory_retry, check, ptr, stream_handle
 from .green_functions import _SING
 
 
@@ -123,9 +123,9 @@ class BatchedSweep:
         return int(sum(len(k) * r for k, r in zip(self.ks, self.nrhs)))
 
     def assemble(self):
-        check(_lib.load().cpt_build_matrices_batched(self.ctx.handle, len(self.bodies), self._bodies_c, 1, 1,
-                                                     float(self.free_surface), self.stream),
-              "cpt_build_matrices_batched")
+        call_with_memory_retry(lambda: _lib.load().cpt_build_matrices_batched(
+            self.ctx.handle, len(self.bodies), self._bodies_c, 1, 1, float(self.free_surface), self.stream),
+            "cpt_build_matrices_batched")
 
     def _groups(self, bcs):
         g = (CptLuGroup * len(self.bodies))()
@@ -150,14 +150,25 @@ class BatchedSweep:
             for x, bc in zip(self.X, bcs):
                 x.copy_(bc)
             g = self._groups(bcs)
-            check(lib.cpt_lu_factor_grouped(self.ctx.handle, nbody, g, self.stream), "cpt_lu_factor_grouped")
+            call_with_memory_retry(lambda: lib.cpt_lu_factor_grouped(self.ctx.handle, nbody, g, self.stream),
+                                   "cpt_lu_factor_grouped")
             check(lib.cpt_lu_solve_grouped(self.ctx.handle, nbody, g, self.stream), "cpt_lu_solve_grouped")
             return
+        import torch
         g = self._groups(bcs)
         its = np.zeros(nbody, dtype=np.int32)
         res = np.zeros(nbody)
-        check(lib.cpt_lu_solve_mixed_grouped(self.ctx.handle, nbody, g, self.max_iter, self.tol, ptr(its), ptr(res),
-                                             self.stream), "cpt_lu_solve_mixed_grouped")
+        # the complex64 factors live in a torch buffer on this stream
+        nbytes = ctypes.c_size_t()
+        check(lib.cpt_mixed_workspace_size(self.ctx.handle, nbody, g, ctypes.byref(nbytes)), "workspace size")
+        work = torch.empty(nbytes.value, dtype=torch.uint8, device=self.dev)
+        check(lib.cpt_set_workspace(self.ctx.handle, work.data_ptr(), nbytes.value), "cpt_set_workspace")
+        try:
+            call_with_memory_retry(lambda: lib.cpt_lu_solve_mixed_grouped(
+                self.ctx.handle, nbody, g, self.max_iter, self.tol, ptr(its), ptr(res), self.stream),
+                "cpt_lu_solve_mixed_grouped")
+        finally:
+            check(lib.cpt_set_workspace(self.ctx.handle, None, 0), "cpt_set_workspace")
         self.iterations, self.residual = its, res
 
     def post(self):

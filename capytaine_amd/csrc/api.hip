@@ -117,6 +117,8 @@ struct cpt_context {
     cpt::DevBuf lu_work, mixed_work;
     cpt::LuSetup lu_streams;
     int asm_chunk = 0;  // panels per assembly work-group, 0 = by size
+    void* ext_work = nullptr;  // caller-owned workspace of the mixed solves
+    size_t ext_bytes = 0;
     // lanes of the grouped LU: default-priority streams, a fork event and
     // one join event per lane
     std::vector<hipStream_t> lanes;
@@ -275,6 +277,15 @@ bool valid_groups(const cpt_lu_group* g, int ng, bool need_factor, bool need_rhs
         }
     }
     return true;
+}
+
+// Workspace of a mixed solve: the caller's (cpt_set_workspace) when large
+// enough, else the context's own, grown as needed.  nullptr on failure.
+void* mixed_workspace(cpt_context* h, size_t bytes)
+{
+    if (h->ext_work && h->ext_bytes >= bytes) return h->ext_work;
+    if (h->mixed_work.reserve(bytes)) return nullptr;
+    return h->mixed_work.p;
 }
 
 Table table_of(const cpt_context* h)
@@ -821,7 +832,8 @@ int cpt_lu_solve_mixed(cpt_handle h, const void* K, int n, int64_t lda, int64_t 
     }
     CPT_CHECK_HIP(hipSetDevice(h->device), "hipSetDevice");
     TimerScope timer_scope(&h->timer);
-    if (h->mixed_work.reserve(mixed_workspace_bytes(n, nrhs, batch))) {
+    void* work = mixed_workspace(h, mixed_workspace_bytes(n, nrhs, batch));
+    if (!work) {
         set_error("cpt_lu_solve_mixed: workspace allocation failed");
         return CPT_ERR_ALLOC;
     }
@@ -830,7 +842,7 @@ int cpt_lu_solve_mixed(cpt_handle h, const void* K, int n, int64_t lda, int64_t 
         return CPT_ERR_HIP;
     }
     CPT_CHECK_HIP((hipError_t)launch_mixed_solve((const cplx*)K, n, lda, strideK, batch, (const cplx*)B, nrhs, ldb,
-                                                 strideB, (cplx*)X, ldx, strideX, max_iter, tol, h->mixed_work.p,
+                                                 strideB, (cplx*)X, ldx, strideX, max_iter, tol, work,
                                                  &h->lu_streams, iterations, residual, stream),
                   "mixed solve");
     return CPT_OK;
@@ -1011,18 +1023,66 @@ int cpt_lu_solve_mixed_grouped(cpt_handle h, int ngroups, const cpt_lu_group* gr
         g[i] = MixedGroup{(const cplx*)G.A, G.n, G.lda, G.strideA, G.batch, (const cplx*)G.B, G.nrhs, G.ldb,
                           G.strideB, (cplx*)G.X, G.ldx, G.strideX, lane[i]};
     }
-    if (h->mixed_work.reserve(mixed_grouped_bytes(g.data(), ngroups))) {
+    void* work = mixed_workspace(h, mixed_grouped_bytes(g.data(), ngroups));
+    if (!work) {
         set_error("cpt_lu_solve_mixed_grouped: workspace allocation failed");
         return CPT_ERR_ALLOC;
     }
     LuSetup tune = h->lu_streams;
     tune.side = nullptr;
     tune.lookahead = 0;
-    const int e = launch_mixed_grouped(g.data(), ngroups, h->mixed_work.p, &tune, max_iter, tol, iterations, residual);
+    const int e = launch_mixed_grouped(g.data(), ngroups, work, &tune, max_iter, tol, iterations, residual);
     if (join_lanes(h, stream)) {
         set_error("cpt_lu_solve_mixed_grouped: could not join the lane streams");
         return CPT_ERR_HIP;
     }
     CPT_CHECK_HIP((hipError_t)e, "grouped mixed solve");
+    return CPT_OK;
+}
+
+int cpt_release_workspace(cpt_handle h)
+{
+    if (!h) {
+        set_error("cpt_release_workspace: null handle");
+        return CPT_ERR_ARG;
+    }
+    CPT_CHECK_HIP(hipSetDevice(h->device), "hipSetDevice");
+    CPT_CHECK_HIP(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    h->faces.release();
+    h->quad.release();
+    h->freq.release();
+    h->prony.release();
+    h->bfaces.release();
+    h->bquad.release();
+    h->bfreq.release();
+    h->bargs.release();
+    h->btiles.release();
+    h->lu_work.release();
+    h->mixed_work.release();
+    return CPT_OK;
+}
+
+int cpt_set_workspace(cpt_handle h, void* ptr, size_t bytes)
+{
+    if (!h || (!ptr && bytes)) {
+        set_error("cpt_set_workspace: invalid arguments");
+        return CPT_ERR_ARG;
+    }
+    h->ext_work = ptr;
+    h->ext_bytes = ptr ? bytes : 0;
+    return CPT_OK;
+}
+
+int cpt_mixed_workspace_size(cpt_handle h, int ngroups, const cpt_lu_group* groups, size_t* bytes)
+{
+    if (!h || ngroups < 0 || (ngroups > 0 && !groups) || !bytes) {
+        set_error("cpt_mixed_workspace_size: invalid arguments");
+        return CPT_ERR_ARG;
+    }
+    std::vector<MixedGroup> g(ngroups);
+    for (int i = 0; i < ngroups; ++i)
+        g[i] = MixedGroup{nullptr, groups[i].n, 0, 0, groups[i].batch, nullptr, groups[i].nrhs, 0, 0, nullptr, 0, 0,
+                          nullptr};
+    *bytes = mixed_grouped_bytes(g.data(), ngroups);
     return CPT_OK;
 }

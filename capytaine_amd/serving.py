@@ -147,7 +147,7 @@ class BEMServer:
         self.gf = green_function
         self.dev = green_function.device
         if memory_budget is None:
-            memory_budget = int(0.6 * torch.cuda.get_device_properties(self.dev).total_memory)
+            memory_budget = int(0.5 * torch.cuda.get_device_properties(self.dev).total_memory)
         self.memory_budget = int(memory_budget)
         self.max_job_bytes = int(max_job_bytes or self.memory_budget // 2)
         self.max_batch_jobs = max_batch_jobs

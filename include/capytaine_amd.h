@@ -239,6 +239,21 @@ enum {
 int cpt_profile_begin(cpt_handle h);
 int cpt_profile_end(cpt_handle h, double* ms, double* work, int64_t* launches);
 
+/* Caller-owned device workspace for cpt_lu_solve_mixed and
+ * cpt_lu_solve_mixed_grouped (the complex64 factors, pivots, residuals and
+ * corrections), so that an allocator outside the library (torch's) holds
+ * all large buffers.  Used by the following mixed solves of `h` while set
+ * and at least as large as they need (cpt_mixed_workspace_size: the same
+ * for a single batch, as one group); ptr = NULL unsets.  The buffer must
+ * stay allocated until the mixed solve's work on its streams is done. */
+int cpt_set_workspace(cpt_handle h, void* ptr, size_t bytes);
+int cpt_mixed_workspace_size(cpt_handle h, int ngroups, const cpt_lu_group* groups, size_t* bytes);
+
+/* Free the device workspace a context keeps between calls (it grows to the
+ * largest call and is reused otherwise); the tabulation stays.  Waits for
+ * the device.  No counterpart in the reference (host memory there). */
+int cpt_release_workspace(cpt_handle h);
+
 /* Tuning parameters of a context, set by the autotuner (no counterpart in
  * the reference, whose LAPACK and OpenMP loops have none).  Results depend
  * on them only through rounding: every value factors with LAPACK's partial
