@@ -77,6 +77,26 @@ def cpu_baseline(mesh, body, pbs_by_freq, table, n_freq=6):
                        f"{dt:.1f} s")
 
 
+def assembly_error(gf, mesh, ks, _lib):
+    """Largest |S_mixed - S_fp64| / max |S_fp64| (and the same for K) over
+    the given wavenumbers: the FP32 far field against the FP64 assembly."""
+    import torch
+    ctx = gf._context()
+    out = {}
+    for fp32 in (0, 1):
+        _lib.check(_lib.load().cpt_set_assembly_precision(ctx.handle, fp32), "assembly precision")
+        out[fp32] = gf.evaluate_on_device(mesh, mesh, wavenumber=list(ks))
+    _lib.check(_lib.load().cpt_set_assembly_precision(ctx.handle, 0), "assembly precision")
+    torch.cuda.synchronize()
+    err = {}
+    for i, name in enumerate(("S", "K")):
+        a64, a32 = out[0][i], out[1][i]
+        err[name] = max(float((a32[b] - a64[b]).abs().max() / a64[b].abs().max()) for b in range(len(ks)))
+    del out
+    torch.cuda.empty_cache()
+    return err
+
+
 def fleet_benchmark(gf, dev, rho=1025.0, g=9.81, seed=0, budget_gb=110.0, log=None):
     """64 half-immersed ellipsoid hulls (panel counts log-uniform in
     300-6000), 16 frequencies, 6 dof + 4 headings: one BatchedSweep per
@@ -256,11 +276,12 @@ def main():
         tuning["assembly"] = autotune.tune_assembly(gf, mesh, ks, log=log)
         torch.cuda.empty_cache()
 
-    def run(precision):
+    def run(precision, far_field_fp32=False):
         applied = autotune.apply(ctx, n, len(ks), precision, nb_wavenumbers=len(ks))
         if applied:
             print(f"[bench] {precision}: tuning {applied}", file=sys.stderr, flush=True)
-        sweep = FrequencySweep(gf, body, ks, nrhs, precision=precision, max_iter=10, tol=1e-12)
+        sweep = FrequencySweep(gf, body, ks, nrhs, precision=precision, max_iter=10, tol=1e-12,
+                               far_field_fp32=far_field_fp32)
         for _ in range(args.warmup):
             sweep.run(bc)
         torch.cuda.synchronize()
@@ -298,6 +319,8 @@ def main():
 
     r64 = run("fp64")
     rmx = run("mixed")
+    rff = run("mixed", far_field_fp32=True)
+    asm_err = assembly_error(gf, mesh, [ks[0], ks[len(ks) // 2], ks[-1]], _lib)
     nb = len(ks)
     nprob = nb * nrhs
     value64 = nprob * args.steps / r64["seconds"]
@@ -306,6 +329,7 @@ def main():
     f64, fmx = r64["forces"], rmx["forces"]
     scale = np.abs(f64).max(axis=2, keepdims=True)  # per problem
     mixed_err = float(np.max(np.abs(fmx - f64) / scale))
+    ff_err = float(np.max(np.abs(rff["forces"] - f64) / scale))
     t_asm = r64["phases"]["assembly"]
     t_lu = r64["phases"]["lu_factor"]
     pairs_per_s = nb * n * n / t_asm
@@ -364,6 +388,13 @@ def main():
         "fp64_ms_per_step": r64["seconds"] / args.steps * 1e3,
         "mixed_over_fp64": valuemx / value64,
         "mixed_max_rel_err_vs_fp64": mixed_err,
+        "mixed_fp32_far_field": {
+            "problems_per_s": nprob * args.steps / rff["seconds"],
+            "phase_ms_per_step": {k: v * 1e3 for k, v in rff["phases"].items()},
+            "assembly_panel_pairs_per_s": nb * n * n / rff["phases"]["assembly"],
+            "max_rel_err_vs_fp64": ff_err,
+            "assembly_max_rel_err": {"S": asm_err["S"], "K": asm_err["K"],
+                                     "wavenumbers": "first, middle and last of the sweep"}},
         "mixed_refinement_sweeps": rmx["iterations"],
         "mixed_final_rel_residual": rmx["residual"],
         "assembly_panel_pairs_per_s": pairs_per_s,

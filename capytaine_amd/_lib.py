@@ -41,6 +41,7 @@ SIGNATURES = {
     "cpt_lu_solve_grouped": (_I, [_P, _I, _P, _P]),
     "cpt_lu_solve_mixed_grouped": (_I, [_P, _I, _P, _I, _D, _P, _P, _P]),
     "cpt_release_workspace": (_I, [_P]),
+    "cpt_set_assembly_precision": (_I, [_P, _I]),
     "cpt_set_workspace": (_I, [_P, _P, ctypes.c_size_t]),
     "cpt_mixed_workspace_size": (_I, [_P, _I, _P, ctypes.POINTER(ctypes.c_size_t)]),
     "cpt_set_tuning": (_I, [_P, _I, _I]),
@@ -142,7 +143,7 @@ class Context:
         h = ctypes.c_void_p()
         check(load().cpt_create(device_index, ctypes.byref(h)), "cpt_create")
         self.handle = h
-        self.table_key = None
+        self.table_key = None  # the device table tensor last uploaded
 
     def close(self):
         """Destroy a context made with Context(device_index) (not one of the

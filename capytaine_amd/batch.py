@@ -6,7 +6,7 @@ problems (solver.py:268-372 -> _solve :145-215 per problem), on the device:
 
   1. one persistent assembly launch over every (body, wavenumber) pair
      (cpt_build_matrices_batched: a work-list of (body, row block, panel
-     chunk) tiles, most expensive first);
+     chunk) tiles, most expensive first), the far field optionally in FP32;
   2. a variable-size batched LU: one group per body (its matrices share the
      body's order), the groups running concurrently on the library's lane
      streams -- complex128 (cpt_lu_factor_grouped + cpt_lu_solve_grouped) or
@@ -58,7 +58,7 @@ class BatchedSweep:
     """
 
     def __init__(self, green_function, bodies, wavenumbers, nrhs, *, precision="mixed", max_iter=10, tol=1e-12,
-                 free_surface=0.0, context=None):
+                 free_surface=0.0, context=None, far_field_fp32=False):
         import torch
         if precision not in ("fp64", "mixed"):
             raise ValueError(f"precision must be 'fp64' or 'mixed', got {precision!r}")
@@ -73,6 +73,8 @@ class BatchedSweep:
         if len(self.ks) != nbody or len(self.nrhs) != nbody:
             raise ValueError("one wavenumber list and one nrhs per body")
         self.precision = precision
+        # FP32 far field on request (mixed form only)
+        self.far_field_fp32 = bool(far_field_fp32) and precision == "mixed"
         self.max_iter, self.tol = max_iter, tol
         self.free_surface = free_surface
         self.meshes = [b.mesh_including_lid for b in self.bodies]
@@ -123,9 +125,14 @@ class BatchedSweep:
         return int(sum(len(k) * r for k, r in zip(self.ks, self.nrhs)))
 
     def assemble(self):
-        call_with_memory_retry(lambda: _lib.load().cpt_build_matrices_batched(
-            self.ctx.handle, len(self.bodies), self._bodies_c, 1, 1, float(self.free_surface), self.stream),
-            "cpt_build_matrices_batched")
+        lib = _lib.load()
+        check(lib.cpt_set_assembly_precision(self.ctx.handle, int(self.far_field_fp32)), "assembly precision")
+        try:
+            call_with_memory_retry(lambda: lib.cpt_build_matrices_batched(
+                self.ctx.handle, len(self.bodies), self._bodies_c, 1, 1, float(self.free_surface), self.stream),
+                "cpt_build_matrices_batched")
+        finally:
+            check(lib.cpt_set_assembly_precision(self.ctx.handle, 0), "assembly precision")
 
     def _groups(self, bcs):
         g = (CptLuGroup * len(self.bodies))()

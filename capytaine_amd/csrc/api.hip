@@ -107,6 +107,8 @@ struct cpt_context {
     int device = 0;
     // tabulation
     cpt::DevBuf r_range, z_range, table;
+    cpt::DevBuf r_range_f, z_range_f, table_f;  // float copies (FP32 far field)
+    int far_fp32 = 0;                           // mixed-precision assembly
     int nr = 0, nz = 0, grid = 1, npts = 1001;
     double rmax = 0, zmax_abs = 0, dz = 0;
     int index_of_1 = 81;
@@ -184,6 +186,12 @@ __global__ void wave_points_kernel(Table T, int np, const double* __restrict__ x
                x0i[3 * (size_t)p + 2] + x0j[3 * (size_t)p + 2], G, nG);
     g[p] = G;
     for (int c = 0; c < 3; ++c) ng[3 * (size_t)p + c] = nG[c];
+}
+
+__global__ void to_float_kernel(const double* __restrict__ src, float* __restrict__ dst, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = (float)src[i];
 }
 
 __global__ void zero_int_kernel(int* __restrict__ p, int n)
@@ -302,6 +310,9 @@ Table table_of(const cpt_context* h)
     T.zmax_abs = h->zmax_abs;
     T.dz = h->dz;
     T.index_of_1 = h->index_of_1;
+    T.tab_f = (const float*)h->table_f.p;
+    T.r_range_f = (const float*)h->r_range_f.p;
+    T.z_range_f = (const float*)h->z_range_f.p;
     return T;
 }
 
@@ -338,6 +349,9 @@ int cpt_destroy(cpt_handle h)
     h->r_range.release();
     h->z_range.release();
     h->table.release();
+    h->r_range_f.release();
+    h->z_range_f.release();
+    h->table_f.release();
     h->faces.release();
     h->quad.release();
     h->bfaces.release();
@@ -400,6 +414,18 @@ int cpt_set_tabulation(cpt_handle h, const double* r_range, int nr, const double
                   "copy z_range");
     int e = launch_table_relayout(tab, nr, nz, (double*)h->table.p, stream);
     CPT_CHECK_HIP((hipError_t)e, "table relayout launch");
+    if (h->r_range_f.reserve(sizeof(float) * nr) || h->z_range_f.reserve(sizeof(float) * nz) ||
+        h->table_f.reserve(sizeof(float) * 5 * (size_t)nr * nz)) {
+        set_error("cpt_set_tabulation: device allocation failed");
+        return CPT_ERR_ALLOC;
+    }
+    hipLaunchKernelGGL(to_float_kernel, dim3((nr + 255) / 256), dim3(256), 0, stream, (const double*)h->r_range.p,
+                       (float*)h->r_range_f.p, (size_t)nr);
+    hipLaunchKernelGGL(to_float_kernel, dim3((nz + 255) / 256), dim3(256), 0, stream, (const double*)h->z_range.p,
+                       (float*)h->z_range_f.p, (size_t)nz);
+    hipLaunchKernelGGL(to_float_kernel, dim3((5 * (size_t)nr * nz + 255) / 256), dim3(256), 0, stream,
+                       (const double*)h->table.p, (float*)h->table_f.p, 5 * (size_t)nr * nz);
+    CPT_CHECK_HIP(hipGetLastError(), "table conversion launch");
     double last[2];
     CPT_CHECK_HIP(hipMemcpyAsync(&last[0], (const double*)r_range + (nr - 1), sizeof(double),
                                  hipMemcpyDeviceToHost, stream),
@@ -539,6 +565,7 @@ int cpt_build_matrices(cpt_handle h, int nb_collocation_points, const double* co
     const int gx = (n + 255) / 256;
     int chunk = (int)std::max<int64_t>(1, ((int64_t)gx * m) / 4096);
     A.j_chunk = h->asm_chunk ? h->asm_chunk : std::min(chunk, 64);
+    A.far_fp32 = (h->far_fp32 && !finite_depth && !rankine_only) ? 1 : 0;
     A.freq = (const FreqArgs*)h->freq.p;
     A.S = (cplx*)S;
     A.ldS = ldS;
@@ -661,6 +688,7 @@ int cpt_build_matrices_batched(cpt_handle h, int nb_bodies, const cpt_body* bodi
         A.free_surface = free_surface;
         A.nb = nb;
         A.j_chunk = chunk;
+        A.far_fp32 = h->far_fp32;
         A.freq = freq_d + ob;
         A.S = (cplx*)B.S;
         A.ldS = B.ldS;
@@ -702,7 +730,7 @@ int cpt_build_matrices_batched(cpt_handle h, int nb_bodies, const cpt_body* bodi
     (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->device);
     const int grid = (int)std::min<size_t>(sorted.size(), (size_t)ncu * 2);
     const int e = launch_assembly_worklist((const AssemblyArgs*)h->bargs.p, (const AsmTile*)h->btiles.p,
-                                           (int)sorted.size(), grid, pairs, stream);
+                                           (int)sorted.size(), grid, pairs, h->far_fp32, stream);
     CPT_CHECK_HIP((hipError_t)e, "batched assembly kernel launch");
     // The host tables above are copied from pageable memory: wait for the
     // copies before they go out of scope.
@@ -1084,5 +1112,15 @@ int cpt_mixed_workspace_size(cpt_handle h, int ngroups, const cpt_lu_group* grou
         g[i] = MixedGroup{nullptr, groups[i].n, 0, 0, groups[i].batch, nullptr, groups[i].nrhs, 0, 0, nullptr, 0, 0,
                           nullptr};
     *bytes = mixed_grouped_bytes(g.data(), ngroups);
+    return CPT_OK;
+}
+
+int cpt_set_assembly_precision(cpt_handle h, int far_field_fp32)
+{
+    if (!h || (far_field_fp32 != 0 && far_field_fp32 != 1)) {
+        set_error("cpt_set_assembly_precision: invalid arguments");
+        return CPT_ERR_ARG;
+    }
+    h->far_fp32 = far_field_fp32;
     return CPT_OK;
 }

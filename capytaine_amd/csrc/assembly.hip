@@ -12,6 +12,7 @@
 //     computed once per pair and reused for every wavenumber of the batch,
 //     with the same order of additions as the reference for each entry.
 #include "green.hpp"
+#include "green_f32.hpp"
 #include "cpt_internal.hpp"
 
 namespace cpt {
@@ -108,9 +109,93 @@ __device__ __forceinline__ void wave_panel_finite_depth(const Table& T, double k
     }
 }
 
+// Far pair (r0 > 7 R, infinite depth) of the mixed-precision assembly: the
+// one-point Rankine and reflected-Rankine terms and the wave term in FP32,
+// summed per wavenumber in the order of the FP64 path and stored in
+// complex128.  Coordinate differences are taken in FP64 before rounding.
+__device__ __forceinline__ void far_pair_f32(const AssemblyArgs& A, int I, int J, bool active, double x0, double x1,
+                                             double x2, double cx, double cy, double cz, double area,
+                                             const double* qp, double nx, double ny, double nz, bool deriv_first)
+{
+    const float af = (float)area;
+    float gd, gdx, gdy, gdz, gr, grx, gry, grz;
+    one_point_rankine_f(0.0f, 0.0f, 0.0f, (float)(cx - x0), (float)(cy - x1), (float)(cz - x2), af, gd, gdx, gdy,
+                        gdz);
+    // image point (x, y, -z): Green_Rankine.f90:202-338 with a = -1, b = 0
+    one_point_rankine_f(0.0f, 0.0f, 0.0f, (float)(cx - x0), (float)(cy - x1), (float)(cz + x2), af, gr, grx, gry,
+                        grz);
+    grz = -grz;
+    if (!deriv_first) {
+        gdx = -gdx;
+        gdy = -gdy;
+        gdz = -gdz;
+        grx = -grx;
+        gry = -gry;
+    }
+    for (int b = 0; b < A.nb; ++b) {
+        const FreqArgs& P = A.freq[b];
+        const float sgn = (P.gfs == HIGH_FREQ) ? -1.0f : 1.0f;
+        float Gre = gd + sgn * gr, Gim = 0.0f;
+        float n0r = gdx + sgn * grx, n1r = gdy + sgn * gry, n2r = gdz + sgn * grz;
+        float n0i = 0.0f, n1i = 0.0f, n2i = 0.0f;
+        if (P.gfs == LOW_FREQ_WITH_RANKINE_PART) n2r += 2 * (float)P.k * gr;
+        if (P.wave) {
+            float wGr = 0.0f, wGi = 0.0f, w0r = 0.0f, w0i = 0.0f, w1r = 0.0f, w1i = 0.0f, w2r = 0.0f, w2i = 0.0f;
+            for (int q = 0; q < A.nq; ++q) {
+                const double qx = qp[4 * q], qy = qp[4 * q + 1], qz = qp[4 * q + 2];
+                const float w = (float)qp[4 * q + 3];
+                cplxf g, ng[3];
+                wave_point_f(A.T, P.k, P.gfs, x0 - qx, x1 - qy, x2 + qz, g, ng);
+                wGr += g.re * w;
+                wGi += g.im * w;
+                w0r += ng[0].re * w;
+                w0i += ng[0].im * w;
+                w1r += ng[1].re * w;
+                w1i += ng[1].im * w;
+                w2r += ng[2].re * w;
+                w2i += ng[2].im * w;
+            }
+            if (!deriv_first) {
+                w0r = -w0r;
+                w0i = -w0i;
+                w1r = -w1r;
+                w1i = -w1i;
+            }
+            Gre += wGr;
+            Gim += wGi;
+            n0r += w0r;
+            n0i += w0i;
+            n1r += w1r;
+            n1i += w1i;
+            n2r += w2r;
+            n2i += w2i;
+        }
+        if (active) {
+            const size_t ij = (size_t)I + (size_t)A.ldS * J;
+            cplx* S = A.S + (size_t)b * A.strideS;
+            S[ij].re = M1_4PI * (double)Gre;
+            S[ij].im = M1_4PI * (double)Gim;
+            cplx* K = A.K + (size_t)b * A.strideK;
+            if (A.nk == 1) {
+                const size_t kij = (size_t)I + (size_t)A.ldK * J;
+                K[kij].re = M1_4PI * (nx * (double)n0r + ny * (double)n1r + nz * (double)n2r);
+                K[kij].im = M1_4PI * (nx * (double)n0i + ny * (double)n1i + nz * (double)n2i);
+            } else {
+                const size_t kij = 3 * ((size_t)I + (size_t)A.ldK * J);
+                K[kij + 0].re = M1_4PI * (double)n0r;
+                K[kij + 0].im = M1_4PI * (double)n0i;
+                K[kij + 1].re = M1_4PI * (double)n1r;
+                K[kij + 1].im = M1_4PI * (double)n1i;
+                K[kij + 2].re = M1_4PI * (double)n2r;
+                K[kij + 2].im = M1_4PI * (double)n2i;
+            }
+        }
+    }
+}
+
 // One tile: collocation points iblk * 256 + threadIdx.x against panels
 // jc * j_chunk .. + j_chunk, for every wavenumber of A.
-template <bool FINITE_DEPTH>
+template <bool FINITE_DEPTH, bool FAR32 = false>
 __device__ __forceinline__ void assemble_tile(const AssemblyArgs& A, int iblk, int jc);
 
 // Infinite depth: two wavefronts per SIMD (about 220 VGPRs, no spills of the
@@ -126,19 +211,27 @@ __global__ void __launch_bounds__(256) assemble_finite_depth_kernel(AssemblyArgs
     assemble_tile<true>(A, blockIdx.x, blockIdx.y);
 }
 
+// Mixed-precision form: far pairs in FP32.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+assemble_infinite_depth_mixed_kernel(AssemblyArgs A)
+{
+    assemble_tile<false, true>(A, blockIdx.x, blockIdx.y);
+}
+
 // Persistent form over many bodies: each work-group walks the work-list of
 // (body, row block, panel chunk) tiles with a grid stride.  The tiles are
 // ordered by decreasing cost on the host so the stride balances the load.
+template <bool FAR32>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 assemble_worklist_kernel(const AssemblyArgs* __restrict__ bodies, const AsmTile* __restrict__ tiles, int ntiles)
 {
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const AsmTile tl = tiles[t];
-        assemble_tile<false>(bodies[tl.body], tl.iblk, tl.jc);
+        assemble_tile<false, FAR32>(bodies[tl.body], tl.iblk, tl.jc);
     }
 }
 
-template <bool FINITE_DEPTH>
+template <bool FINITE_DEPTH, bool FAR32>
 __device__ __forceinline__ void assemble_tile(const AssemblyArgs& A, int iblk, int jc)
 {
     const int I = iblk * blockDim.x + threadIdx.x;
@@ -163,6 +256,16 @@ __device__ __forceinline__ void assemble_tile(const AssemblyArgs& A, int iblk, i
             nx = A.dpn[3 * (size_t)J];
             ny = A.dpn[3 * (size_t)J + 1];
             nz = A.dpn[3 * (size_t)J + 2];
+        }
+        if (FAR32) {
+            // the reference's near-field tests (Green_Rankine.f90:50 and
+            // its reflected form), with the FP64 path's expressions
+            const double r0 = norm3(x0 - cx, x1 - cy, x2 - cz);
+            const double r1 = sqrt((x0 - cx) * (x0 - cx) + (x1 - cy) * (x1 - cy) + (-x2 - cz) * (-x2 - cz));
+            if (r0 > 7 * radius && r1 > 7 * radius) {
+                far_pair_f32(A, I, J, active, x0, x1, x2, cx, cy, cz, area, qp, nx, ny, nz, deriv_first);
+                continue;
+            }
         }
         // Rankine term (matrices.f90:122-134)
         double gd, gdx, gdy, gdz;
@@ -290,16 +393,20 @@ int launch_assembly(const AssemblyArgs& A, hipStream_t stream)
     dim3 grid(gx, gy);
     TimedLaunch timed(stream, TK_ASSEMBLY, (double)A.n * A.m * A.nb);
     if (A.finite_depth) hipLaunchKernelGGL(assemble_finite_depth_kernel, grid, block, 0, stream, A);
+    else if (A.far_fp32) hipLaunchKernelGGL(assemble_infinite_depth_mixed_kernel, grid, block, 0, stream, A);
     else hipLaunchKernelGGL(assemble_infinite_depth_kernel, grid, block, 0, stream, A);
     return (int)hipGetLastError();
 }
 
 int launch_assembly_worklist(const AssemblyArgs* bodies, const AsmTile* tiles, int ntiles, int grid, double pairs,
-                             hipStream_t stream)
+                             int far_fp32, hipStream_t stream)
 {
     if (ntiles <= 0) return 0;
     TimedLaunch timed(stream, TK_ASSEMBLY, pairs);
-    hipLaunchKernelGGL(assemble_worklist_kernel, dim3(grid), dim3(256), 0, stream, bodies, tiles, ntiles);
+    if (far_fp32)
+        hipLaunchKernelGGL(assemble_worklist_kernel<true>, dim3(grid), dim3(256), 0, stream, bodies, tiles, ntiles);
+    else
+        hipLaunchKernelGGL(assemble_worklist_kernel<false>, dim3(grid), dim3(256), 0, stream, bodies, tiles, ntiles);
     return (int)hipGetLastError();
 }
 

@@ -31,6 +31,7 @@ struct AssemblyArgs {
     int diag, n_diag;
     double free_surface;
     int nb, j_chunk;
+    int far_fp32, pad_;    // FP32 far field (infinite depth, mixed form)
     const FreqArgs* freq;  // device, nb
     cplx* S;
     int64_t ldS, strideS;
@@ -45,7 +46,7 @@ struct AsmTile {
     int body, iblk, jc, pad;
 };
 int launch_assembly_worklist(const AssemblyArgs* bodies, const AsmTile* tiles, int ntiles, int grid, double pairs,
-                             hipStream_t stream);
+                             int far_fp32, hipStream_t stream);
 
 // Tuning of the blocked LU and the side stream / events of its look-ahead
 // (owned by a context).

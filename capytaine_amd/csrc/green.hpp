@@ -47,6 +47,10 @@ struct Table {
     double zmax_abs;        // |z_range[nz-1]|
     double dz;              // (log10(|z_range[nz-1]|) + 10) / nz  (scaled grid)
     int index_of_1;         // change-of-slope index of the scaled r grid
+    // float copies (same layout) for the FP32 far field of the mixed form
+    const float* tab_f;
+    const float* r_range_f;
+    const float* z_range_f;
 };
 
 struct cplx { double re, im; };

@@ -322,8 +322,10 @@ class Delhommeau(AbstractGreenFunction):
         """Library context (the device's shared one by default) with this
         Green function's table loaded."""
         ctx = _lib.Context.get(self.device) if ctx is None else ctx
-        key = id(self._table_device[2])
-        if ctx.table_key != key:
+        # the context holds the uploaded table tensor itself: an identity
+        # test on a live object cannot be fooled by a reused id()
+        key = self._table_device[2]
+        if ctx.table_key is not key:
             r_d, z_d, tab_d = self._table_device
             check(_lib.load().cpt_set_tabulation(ctx.handle, ptr(r_d), r_d.numel(), ptr(z_d), z_d.numel(),
                                                  ptr(tab_d), self.tabulation_grid_shape_index,

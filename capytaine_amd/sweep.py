@@ -22,7 +22,7 @@ from ._lib import check, ptr, stream_handle
 
 class FrequencySweep:
     def __init__(self, green_function, body, wavenumbers, nrhs, *, free_surface=0.0, water_depth=np.inf,
-                 precision="fp64", max_iter=10, tol=1e-12):
+                 precision="fp64", max_iter=10, tol=1e-12, far_field_fp32=False):
         import torch
         self.gf = green_function
         self.body = body
@@ -37,6 +37,9 @@ class FrequencySweep:
         if precision not in ("fp64", "mixed"):
             raise ValueError(f"precision must be 'fp64' or 'mixed', got {precision!r}")
         self.precision = precision
+        # FP32 far field on request (mixed form): coefficients then move by
+        # ~1e-6 relative; the default keeps the FP64 assembly
+        self.far_field_fp32 = bool(far_field_fp32) and precision == "mixed"
         self.max_iter = max_iter
         self.tol = tol
         self.iterations = 0
@@ -62,9 +65,14 @@ class FrequencySweep:
 
     # individual phases, so that callers can time them
     def assemble(self):
-        self.gf.evaluate_on_device(self.mesh, self.mesh, free_surface=self.free_surface,
-                                   water_depth=self.water_depth, wavenumber=self.ks, adjoint_double_layer=True,
-                                   diagonal_term_in_double_layer=True, out=(self.S, self.K))
+        lib = _lib.load()
+        check(lib.cpt_set_assembly_precision(self.ctx.handle, int(self.far_field_fp32)), "assembly precision")
+        try:
+            self.gf.evaluate_on_device(self.mesh, self.mesh, free_surface=self.free_surface,
+                                       water_depth=self.water_depth, wavenumber=self.ks, adjoint_double_layer=True,
+                                       diagonal_term_in_double_layer=True, out=(self.S, self.K))
+        finally:
+            check(lib.cpt_set_assembly_precision(self.ctx.handle, 0), "assembly precision")
 
     def factor(self):
         """FP64: LU of K in place.  Mixed: nothing here (the complex64 factor

@@ -239,6 +239,21 @@ enum {
 int cpt_profile_begin(cpt_handle h);
 int cpt_profile_end(cpt_handle h, double* ms, double* work, int64_t* launches);
 
+/* Precision of the following assemblies of `h` (cpt_build_matrices,
+ * cpt_build_matrices_batched), for the mixed-precision form:
+ * far_field_fp32 = 1 evaluates the panel pairs outside the near-field radius
+ * (r0 > 7 R, the reference's switch to the one-point Rankine formula,
+ * Green_Rankine.f90:50) in FP32 -- one-point Rankine and reflected Rankine,
+ * and the infinite-depth wave term interpolated in a float copy of the
+ * table with the FP64 path's branch and stencil decisions; near pairs, the
+ * diagonal term and the free-surface self term stay FP64, and S and K are
+ * stored in complex128.  Entries then differ from the FP64 assembly by
+ * FP32 rounding (~1e-7 of max |S|, |K|).  Infinite depth only (ignored in
+ * finite depth).
+ * 0 (default) is the plain FP64 assembly.  No reference counterpart: the
+ * reference is FP64 (or FP32 throughout when built in single precision). */
+int cpt_set_assembly_precision(cpt_handle h, int far_field_fp32);
+
 /* Caller-owned device workspace for cpt_lu_solve_mixed and
  * cpt_lu_solve_mixed_grouped (the complex64 factors, pivots, residuals and
  * corrections), so that an allocator outside the library (torch's) holds

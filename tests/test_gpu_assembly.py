@@ -187,3 +187,31 @@ def test_points_as_collocation(gf_ref, orc, default_table):
     arr = dict(mesh_arrays(mesh), colloc=pts, dpn=np.zeros((37, 3)))
     So, _ = orc.build_matrices(**arr, wavenumber=0.7, depth=np.inf, table=default_table, diagonal_term=False)
     assert rel_err(S, So) < RTOL_NORM
+
+
+def test_table_switch_between_green_functions(default_table, legacy_table):
+    """Two Green functions with different tables share the device context:
+    each evaluation runs on its own table, including after the other one
+    was freed (the context tracks the uploaded tensor, not its id())."""
+    import gc
+    import torch
+    import capytaine_amd as cpa
+    print("[gpu] table switch", flush=True)
+    mesh = cpa.mesh_sphere(radius=1.0, resolution=(6, 12)).immersed_part()
+
+    def legacy_gf():
+        gf = cpa.Delhommeau(tabulation_nr=328, tabulation_nz=46, tabulation_grid_shape="legacy",
+                            tabulation_nb_integration_points=251, tabulation_cache_dir=None, device=device())
+        gf._set_tabulation(legacy_table[0], legacy_table[1], np.asfortranarray(legacy_table[2]))
+        return gf
+
+    ref_legacy = [a.cpu() for a in legacy_gf().evaluate_on_device(mesh, mesh, wavenumber=0.5)]
+    ref_default = [a.cpu() for a in ref_table_gf(default_table).evaluate_on_device(mesh, mesh, wavenumber=0.5)]
+    assert not torch.equal(ref_legacy[0], ref_default[0])
+    for _ in range(4):
+        for make, ref in ((legacy_gf, ref_legacy), (lambda: ref_table_gf(default_table), ref_default)):
+            gf = make()
+            S, K = gf.evaluate_on_device(mesh, mesh, wavenumber=0.5)
+            assert torch.equal(S.cpu(), ref[0]) and torch.equal(K.cpu(), ref[1])
+            del gf, S, K
+            gc.collect()

@@ -2,9 +2,10 @@
 assembly + variable-size grouped LU): equal to the one-body path.
 
 Tolerances: S and K bit-equal to cpt_build_matrices per body (the same
-per-pair arithmetic); FP64 forces within 1e-12 relative of the one-body
-FP64 sweep; mixed-precision forces within 1e-10 of FP64 (refinement to a
-1e-12 residual); grouped LU solutions within 1e-10 of scipy."""
+per-pair arithmetic); forces within 1e-12 relative of the one-body sweep of
+the same form; mixed-precision forces (FP32 far field, complex64 factors,
+refinement to a 1e-12 residual) within 1e-6 of FP64; grouped LU solutions
+within 1e-10 of scipy."""
 import numpy as np
 import pytest
 
@@ -35,19 +36,27 @@ def _bcs(bodies, omegas, headings, rho=1025.0):
     return out, nrhs
 
 
-def test_batched_assembly_equals_single(default_table):
+@pytest.mark.parametrize("far_field_fp32", [False, True])
+def test_batched_assembly_equals_single(default_table, far_field_fp32):
     import torch
+    from capytaine_amd import _lib
     from capytaine_amd.batch import BatchedSweep
-    print("[gpu] batched assembly vs one body at a time", flush=True)
+    print(f"[gpu] batched assembly vs one body at a time (FP32 far field: {far_field_fp32})", flush=True)
     gf = ref_table_gf(default_table)
     bodies, omegas, headings = _fleet()
     ks = [om ** 2 / 9.81 for om in omegas]
-    bs = BatchedSweep(gf, bodies, ks, 1, precision="fp64")
+    bs = BatchedSweep(gf, bodies, ks, 1, precision="mixed" if far_field_fp32 else "fp64",
+                      far_field_fp32=far_field_fp32)
     bs.assemble()
     torch.cuda.synchronize()
+    ctx = gf._context()
     for i, b in enumerate(bodies):
-        S, K = gf.evaluate_on_device(b.mesh, b.mesh, wavenumber=ks[i])
-        torch.cuda.synchronize()
+        _lib.check(_lib.load().cpt_set_assembly_precision(ctx.handle, int(far_field_fp32)), "precision")
+        try:
+            S, K = gf.evaluate_on_device(b.mesh, b.mesh, wavenumber=ks[i])
+            torch.cuda.synchronize()
+        finally:
+            _lib.check(_lib.load().cpt_set_assembly_precision(ctx.handle, 0), "precision")
         assert torch.equal(S, bs.S[i]) and torch.equal(K, bs.K[i]), b.name
 
 
@@ -67,12 +76,18 @@ def test_batched_forces_match_sweep(default_table, precision):
     if precision == "mixed":
         assert np.all(bs.residual <= 1e-12) and np.all(bs.iterations >= 1)
     for i, b in enumerate(bodies):
-        sw = FrequencySweep(gf, b, ks[i], nrhs[i], precision="fp64")
+        # the same form one body at a time: same arithmetic
+        sw = FrequencySweep(gf, b, ks[i], nrhs[i], precision=precision)
         sw.run(bcs[i])
         ref = sw.forces(omegas[i], 1025.0)
         scale = np.abs(ref).max()
-        tol = 1e-12 if precision == "fp64" else 1e-10
-        assert np.abs(got[i] - ref).max() <= tol * scale, (b.name, np.abs(got[i] - ref).max() / scale)
+        assert np.abs(got[i] - ref).max() <= 1e-12 * scale, (b.name, np.abs(got[i] - ref).max() / scale)
+        if precision == "mixed":
+            # FP32 far field + complex64 factors against the FP64 form
+            sw = FrequencySweep(gf, b, ks[i], nrhs[i], precision="fp64")
+            sw.run(bcs[i])
+            ref = sw.forces(omegas[i], 1025.0)
+            assert np.abs(got[i] - ref).max() <= 1e-6 * np.abs(ref).max(), b.name
     torch.cuda.synchronize()
 
 

@@ -3,8 +3,9 @@ concurrent requests, batches overlap on the device, and the coefficients
 match the one-body FP64 sweep.
 
 Tolerances: alone vs in the mix within 1e-12 relative (the per-body
-arithmetic does not depend on the batch); mixed precision vs the FP64 sweep
-within 1e-10 relative (refinement to a 1e-12 residual)."""
+arithmetic does not depend on the batch); the served (mixed) result within
+1e-12 of the one-body mixed sweep and within 1e-6 of the FP64 sweep (FP32
+far field, complex64 factors refined to a 1e-12 residual)."""
 import numpy as np
 import pytest
 
@@ -40,13 +41,14 @@ def test_result_alone_equals_result_in_mix(default_table):
     assert stats["completed"] == 8 and stats["max_inflight_batches"] >= 2
     for key in ("added_mass", "radiation_damping", "diffraction_force", "excitation_force"):
         assert _rel(mixed[key], alone[key]) <= 1e-12, key
-    # against the one-body FP64 sweep
+    # against the one-body sweeps: the same (mixed) form, then FP64
     ks = om_t ** 2 / 9.81
     pbs = workloads.problems_by_frequency(target, om_t, hd_t, rho=1025.0)
-    sw = FrequencySweep(gf, target, ks, len(pbs[0]), precision="fp64")
-    sw.run(boundary_conditions(pbs, len(pbs[0]), target.mesh.nb_faces, device()))
-    ref = sw.forces(om_t, 1025.0)
-    assert _rel(alone["forces"], ref) <= 1e-10
+    bc = boundary_conditions(pbs, len(pbs[0]), target.mesh.nb_faces, device())
+    for prec, tol in (("mixed", 1e-12), ("fp64", 1e-6)):
+        sw = FrequencySweep(gf, target, ks, len(pbs[0]), precision=prec)
+        sw.run(bc)
+        assert _rel(alone["forces"], sw.forces(om_t, 1025.0)) <= tol, prec
     assert alone["final_rel_residual"] <= 1e-12
 
 
