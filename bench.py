@@ -15,7 +15,8 @@ form's throughput; the FP64 throughput, the ratio and the largest relative
 difference of any coefficient between the two forms are reported beside it.
 
 Prints one JSON line (see the repository's bench contract).
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline] [--no-fleet] [--no-serving]
+    python bench.py --workload cylinders     # 20000-panel finite-depth array instead
 """
 import argparse
 import json
@@ -231,6 +232,74 @@ def serving_benchmark(gf, dev, seed=1, requests=24, loads=(0.5, 0.8, 1.0, 1.2), 
     return out
 
 
+def cylinders_benchmark(args, rho=1025.0, g=9.81, chunk=4):
+    """5 x 5 truncated cylinders (20000 panels), depth 40 m, 20 frequencies,
+    150 radiation dof + 8 headings: the finite-depth assembly and the
+    N = 20000 factorisation, FP64 form and mixed form (complex64 factors
+    refined against the complex128 K), `chunk` frequencies at a time
+    (S + K are 12.8 GB per frequency).  One timed pass over the 20
+    frequencies per form, after a warm pass over one chunk."""
+    import torch
+    import capytaine_amd as cpa
+    from capytaine_amd import workloads
+    from capytaine_amd.sweep import FrequencySweep, boundary_conditions
+    dev = torch.device("cuda", 0)
+    body, omegas, headings, depth = workloads.cylinder_array()
+    n = body.mesh.nb_faces
+    gf = cpa.Delhommeau(device=dev)
+    pbs = workloads.problems_by_frequency(body, omegas, headings, rho=rho, g=g, water_depth=depth)
+    nrhs = len(pbs[0])
+    ks = np.array([p[0].wavenumber for p in pbs])
+    out = {"metric": METRIC, "unit": "problems/s", "n_gpus": 1, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "data": "synthetic",
+           "config": {"workload": "5 x 5 truncated vertical cylinders, radius 2 m, draft 4 m, 12 m apart, "
+                                  "20000 panels, depth 40 m, 20 frequencies 0.3-1.8 rad/s, 150 dof + 8 headings",
+                      "panels": n, "frequencies": len(ks), "rhs_per_frequency": nrhs, "frequency_chunk": chunk}}
+    forces = {}
+    for precision in ("fp64", "mixed"):
+        t_asm = t_all = 0.0
+        forces[precision] = []
+        for warm in (True, False):
+            for c0 in range(0, len(ks), chunk):
+                sl = slice(c0, min(c0 + chunk, len(ks)))
+                bc = boundary_conditions(pbs[sl], nrhs, n, dev)
+                sw = FrequencySweep(gf, body, ks[sl], nrhs, water_depth=depth, precision=precision)
+                torch.cuda.synchronize()
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                t0 = time.perf_counter()
+                e0.record()
+                sw.assemble()
+                e1.record()
+                sw.factor()
+                sw.solve(bc)
+                sw.post()
+                torch.cuda.synchronize()
+                if not warm:
+                    t_all += time.perf_counter() - t0
+                    t_asm += e0.elapsed_time(e1) / 1e3
+                    forces[precision].append(sw.forces(omegas[sl], rho))
+                del sw, bc
+                print(f"[cylinders] {precision} {'warm' if warm else 'timed'} frequencies {sl.start}-{sl.stop - 1}",
+                      file=sys.stderr, flush=True)
+                if warm:
+                    break
+        torch.cuda.empty_cache()
+        nprob = len(ks) * nrhs
+        out[f"{precision}_problems_per_s"] = nprob / t_all
+        out[f"{precision}_seconds"] = t_all
+        out[f"{precision}_assembly_panel_pairs_per_s"] = len(ks) * n * n / t_asm
+        out[f"{precision}_lu_tflops"] = len(ks) * 8.0 * n ** 3 / 3.0 / (t_all - t_asm) / 1e12
+    f64 = np.concatenate(forces["fp64"])
+    fmx = np.concatenate(forces["mixed"])
+    scale = np.abs(f64).max(axis=2, keepdims=True)
+    out["value"] = out["mixed_problems_per_s"]
+    out["mixed_over_fp64"] = out["mixed_problems_per_s"] / out["fp64_problems_per_s"]
+    out["mixed_max_rel_err_vs_fp64"] = float(np.max(np.abs(fmx - f64) / scale))
+    out["dtype"] = "c64 factor + f64 refinement (mixed); f64 reported beside"
+    out["steps"], out["warmup"] = 1, 1
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -241,11 +310,17 @@ def main():
     ap.add_argument("--no-serving", action="store_true", help="skip the request-stream latency measurement")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner (a stored tuning of this size class is still applied)")
+    ap.add_argument("--workload", choices=("barge", "cylinders"), default="barge",
+                    help="barge: the headline line (plus fleet and serving); cylinders: the 20000-panel "
+                         "finite-depth array")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the frequency sweep shards by "
               "frequency across GPUs by construction but this build measures a single GPU.", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "cylinders":
+        cylinders_benchmark(args)
+        return
 
     import torch
     import capytaine_amd as cpa
