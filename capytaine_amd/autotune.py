@@ -24,8 +24,9 @@ from . import _lib
 from ._lib import check, ptr, stream_handle
 
 LU_BLOCKS = (64, 128)
-PANEL_LEAVES = (8, 16, 32)
-LOOKAHEAD = (0, 1)
+PANEL_LEAVES = (8, 16)
+LOOKAHEAD = (0, 1, 2)        # in order, side stream, CUs reserved for the panels
+PANEL_CUS = (16, 32)         # reserved CUs tried with LOOKAHEAD = 2
 ASSEMBLY_CHUNKS = (0, 4, 8, 16, 32, 64)
 
 _lock = threading.Lock()
@@ -100,9 +101,9 @@ def _time(fn, device, reps):
 
 
 def tune_lu(n, batch, precision="fp64", device=None, reps=2, nrhs=14, candidates=None, store=True, log=None):
-    """Time every (lu_block, panel_leaf, lookahead) candidate on `batch`
-    synthetic n x n matrices and return the fastest as a dict, with the
-    timings of all candidates under "timings_ms"."""
+    """Time every (lu_block, panel_leaf, lookahead, panel_cus) candidate on
+    `batch` synthetic n x n matrices and return the fastest as a dict, with
+    the timings of all candidates under "timings_ms"."""
     import torch
     device = torch.device("cuda", 0) if device is None else torch.device(device)
     if precision not in ("fp64", "mixed"):
@@ -131,21 +132,24 @@ def tune_lu(n, batch, precision="fp64", device=None, reps=2, nrhs=14, candidates
                   "cpt_lu_solve_mixed")
 
     timings = {}
-    cands = candidates or list(itertools.product(LU_BLOCKS, PANEL_LEAVES, LOOKAHEAD))
+    if candidates is None:
+        candidates = [(blk, leaf, la, cus) for blk, leaf, la in itertools.product(LU_BLOCKS, PANEL_LEAVES, LOOKAHEAD)
+                      for cus in (PANEL_CUS if la == 2 else PANEL_CUS[:1])]
     try:
-        for blk, leaf, la in cands:
+        for blk, leaf, la, cus in candidates:
             if leaf > blk // 2:
                 continue
-            _lib.set_tuning(ctx, lu_block=blk, panel_leaf=leaf, lookahead=la)
+            _lib.set_tuning(ctx, lu_block=blk, panel_leaf=leaf, lookahead=la, panel_cus=cus)
             t = _time(run, device, reps)
-            timings[f"{blk}/{leaf}/{la}"] = t
+            timings[f"{blk}/{leaf}/{la}/{cus}"] = t
             if log:
-                log(f"[tune] n={n} batch={batch} {precision} block={blk} leaf={leaf} lookahead={la}: {t:.2f} ms")
+                log(f"[tune] n={n} batch={batch} {precision} block={blk} leaf={leaf} lookahead={la} "
+                    f"panel_cus={cus}: {t:.2f} ms")
     finally:
         _lib.set_tuning(ctx, **saved)
     best_key = min(timings, key=timings.get)
-    blk, leaf, la = (int(x) for x in best_key.split("/"))
-    best = {"lu_block": blk, "panel_leaf": leaf, "lookahead": la, "ms": timings[best_key],
+    blk, leaf, la, cus = (int(x) for x in best_key.split("/"))
+    best = {"lu_block": blk, "panel_leaf": leaf, "lookahead": la, "panel_cus": cus, "ms": timings[best_key],
             "timings_ms": timings}
     if store:
         _store(_key(device, "lu", precision, n, batch), best)
@@ -194,7 +198,8 @@ def apply(ctx, n, batch, precision="fp64", nb_wavenumbers=None):
     applied = {}
     best = lookup(device, "lu", precision, n, batch)
     if best:
-        applied.update(lu_block=best["lu_block"], panel_leaf=best["panel_leaf"], lookahead=best["lookahead"])
+        applied.update(lu_block=best["lu_block"], panel_leaf=best["panel_leaf"], lookahead=best["lookahead"],
+                       panel_cus=best.get("panel_cus", 16))
     best = lookup(device, "assembly", "fp64", n, nb_wavenumbers or batch)
     if best:
         applied["assembly_chunk"] = best["assembly_chunk"]

@@ -119,6 +119,8 @@ struct cpt_context {
     cpt::DevBuf lu_work, mixed_work;
     cpt::LuSetup lu_streams;
     int asm_chunk = 0;  // panels per assembly work-group, 0 = by size
+    int part_cus = 0;   // CUs of the current partitioned look-ahead streams
+    bool part_failed = false;  // the device refused CU-masked streams
     void* ext_work = nullptr;  // caller-owned workspace of the mixed solves
     size_t ext_bytes = 0;
     // lanes of the grouped LU: default-priority streams, a fork event and
@@ -201,8 +203,51 @@ __global__ void zero_int_kernel(int* __restrict__ p, int n)
 }
 
 // Side stream (default priority) and events of the LU look-ahead.
+// Streams of the CU-partitioned look-ahead: panel_cus CUs spread evenly
+// over the device for the panels, the other CUs for the rest.  Recreated
+// when panel_cus changes.
+int ensure_partition(cpt_context* h)
+{
+    LuSetup& L = h->lu_streams;
+    if (L.lookahead != 2 || h->part_failed) return 0;
+    int ncu = 0;
+    hipError_t err = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->device);
+    if (err != hipSuccess || ncu < 8) {
+        set_error(std::string("CU count query failed: ") + hipGetErrorString(err));
+        return -1;
+    }
+    const int want = std::max(1, std::min(L.panel_cus, ncu / 2));
+    if (L.part_main && h->part_cus == want) return 0;
+    if (L.part_main) (void)hipStreamDestroy(L.part_main);
+    if (L.part_side) (void)hipStreamDestroy(L.part_side);
+    L.part_main = L.part_side = nullptr;
+    const int words = (ncu + 31) / 32;
+    std::vector<uint32_t> side(words, 0u), rest(words, 0u);
+    for (int c = 0; c < ncu; ++c) {
+        const bool panel = (c * want) / ncu != ((c + 1) * want) / ncu;  // want CUs, evenly spaced
+        (panel ? side : rest)[c / 32] |= 1u << (c % 32);
+    }
+    if ((err = hipExtStreamCreateWithCUMask(&L.part_side, words, side.data())) != hipSuccess ||
+        (err = hipExtStreamCreateWithCUMask(&L.part_main, words, rest.data())) != hipSuccess) {
+        set_error(std::string("hipExtStreamCreateWithCUMask (") + std::to_string(ncu) + " CUs, " +
+                  std::to_string(want) + " for the panels): " + hipGetErrorString(err));
+        if (L.part_side) (void)hipStreamDestroy(L.part_side);
+        L.part_side = nullptr;
+        (void)hipGetLastError();  // do not leave the failure for the next launch check
+        h->part_failed = true;
+        return -1;
+    }
+    if (!L.enter && hipEventCreateWithFlags(&L.enter, hipEventDisableTiming) != hipSuccess) return -1;
+    if (!L.leave && hipEventCreateWithFlags(&L.leave, hipEventDisableTiming) != hipSuccess) return -1;
+    h->part_cus = want;
+    return 0;
+}
+
 int ensure_lu_streams(cpt_context* h)
 {
+    // A CU-masked stream needs a hardware queue of its own; when the device
+    // has none left (GPU_MAX_HW_QUEUES), mode 2 runs as mode 1.
+    if (ensure_partition(h)) h->lu_streams.part_main = nullptr;
     if (h->lu_streams.side) return 0;
     if (hipStreamCreateWithFlags(&h->lu_streams.side, hipStreamNonBlocking) != hipSuccess) return -1;
     if (hipEventCreateWithFlags(&h->lu_streams.ready, hipEventDisableTiming) != hipSuccess) return -1;
@@ -366,6 +411,10 @@ int cpt_destroy(cpt_handle h)
     if (h->lu_streams.side) (void)hipStreamDestroy(h->lu_streams.side);
     if (h->lu_streams.ready) (void)hipEventDestroy(h->lu_streams.ready);
     if (h->lu_streams.done) (void)hipEventDestroy(h->lu_streams.done);
+    if (h->lu_streams.part_main) (void)hipStreamDestroy(h->lu_streams.part_main);
+    if (h->lu_streams.part_side) (void)hipStreamDestroy(h->lu_streams.part_side);
+    if (h->lu_streams.enter) (void)hipEventDestroy(h->lu_streams.enter);
+    if (h->lu_streams.leave) (void)hipEventDestroy(h->lu_streams.leave);
     h->timer.release();
     for (hipStream_t l : h->lanes) (void)hipStreamDestroy(l);
     for (hipEvent_t ev : h->lane_done) (void)hipEventDestroy(ev);
@@ -783,7 +832,7 @@ int cpt_lu_factor(cpt_handle h, void* A, int n, int64_t lda, int64_t strideA, in
         return CPT_ERR_ALLOC;
     }
     if (ensure_lu_streams(h)) {
-        set_error("cpt_lu_factor: could not create the look-ahead stream");
+        set_error("cpt_lu_factor: could not create the look-ahead streams: " + std::string(cpt_last_error()));
         return CPT_ERR_HIP;
     }
     CPT_CHECK_HIP((hipError_t)launch_getrf((cplx*)A, n, lda, strideA, ipiv, strideP, info, batch,
@@ -866,7 +915,7 @@ int cpt_lu_solve_mixed(cpt_handle h, const void* K, int n, int64_t lda, int64_t 
         return CPT_ERR_ALLOC;
     }
     if (ensure_lu_streams(h)) {
-        set_error("cpt_lu_solve_mixed: could not create the look-ahead stream");
+        set_error("cpt_lu_solve_mixed: could not create the look-ahead streams: " + std::string(cpt_last_error()));
         return CPT_ERR_HIP;
     }
     CPT_CHECK_HIP((hipError_t)launch_mixed_solve((const cplx*)K, n, lda, strideK, batch, (const cplx*)B, nrhs, ldb,
@@ -932,8 +981,12 @@ int cpt_set_tuning(cpt_handle h, int param, int value)
         h->lu_streams.leaf = value;
         return CPT_OK;
     case CPT_TUNE_LOOKAHEAD:
-        if (value != 0 && value != 1) break;
+        if (value < 0 || value > 2) break;
         h->lu_streams.lookahead = value;
+        return CPT_OK;
+    case CPT_TUNE_PANEL_CUS:
+        if (value < 1 || value > 128) break;
+        h->lu_streams.panel_cus = value;
         return CPT_OK;
     case CPT_TUNE_ASSEMBLY_CHUNK:
         if (value < 0 || value > 1024) break;
@@ -958,6 +1011,7 @@ int cpt_get_tuning(cpt_handle h, int param, int* value)
     case CPT_TUNE_LU_BLOCK: *value = h->lu_streams.block; return CPT_OK;
     case CPT_TUNE_PANEL_LEAF: *value = h->lu_streams.leaf; return CPT_OK;
     case CPT_TUNE_LOOKAHEAD: *value = h->lu_streams.lookahead; return CPT_OK;
+    case CPT_TUNE_PANEL_CUS: *value = h->lu_streams.panel_cus; return CPT_OK;
     case CPT_TUNE_ASSEMBLY_CHUNK: *value = h->asm_chunk; return CPT_OK;
     default:
         set_error("cpt_get_tuning: unknown parameter " + std::to_string(param));

@@ -55,7 +55,12 @@ struct LuSetup {
     hipEvent_t ready = nullptr, done = nullptr;
     int block = 0;      // block width: 64 or 128, 0 = by size (128 from 1024 rows)
     int leaf = 16;      // panel columns factored by one work-group per matrix
-    int lookahead = 1;  // factor the next panel on the side stream
+    int lookahead = 1;  // 1: next panel on the side stream; 2: on CUs reserved for it
+    // CU-partitioned look-ahead: panels on `part_side` (panel_cus CUs), the
+    // rest of the factorisation on `part_main` (the other CUs)
+    int panel_cus = 16;
+    hipStream_t part_main = nullptr, part_side = nullptr;
+    hipEvent_t enter = nullptr, leave = nullptr;
 };
 
 // One group of the grouped (variable-size) LU: `batch` matrices of order n,

@@ -839,10 +839,33 @@ size_t getrf_ws_elems(int, int)
 // columns are updated first and factored on the side stream while the rest
 // of A22 is updated (look-ahead of depth one).
 template <typename R>
+int getrf_steps(cx<R>* A, int N, int64_t lda, int64_t strideA, int* ipiv, int64_t strideP, int* info, int batch,
+                const LuSetup* aux, hipStream_t stream);
+
+template <typename R>
 int getrf(cx<R>* A, int N, int64_t lda, int64_t strideA, int* ipiv, int64_t strideP, int* info, int batch,
           cx<R>*, const LuSetup* aux, hipStream_t stream)
 {
     if (N <= 0 || batch <= 0) return 0;
+    if (!aux || aux->lookahead != 2 || !aux->part_main) return getrf_steps<R>(A, N, lda, strideA, ipiv, strideP, info,
+                                                                             batch, aux, stream);
+    // CU-partitioned look-ahead: the factorisation runs on the two masked
+    // streams, ordered after the caller's stream, which then waits for it
+    LuSetup part = *aux;
+    part.side = aux->part_side;
+    if (hipEventRecord(aux->enter, stream) != hipSuccess) return (int)hipErrorUnknown;
+    if (hipStreamWaitEvent(aux->part_main, aux->enter, 0) != hipSuccess) return (int)hipErrorUnknown;
+    if (hipStreamWaitEvent(aux->part_side, aux->enter, 0) != hipSuccess) return (int)hipErrorUnknown;
+    const int e = getrf_steps<R>(A, N, lda, strideA, ipiv, strideP, info, batch, &part, aux->part_main);
+    if (hipEventRecord(aux->leave, aux->part_main) != hipSuccess) return (int)hipErrorUnknown;
+    if (hipStreamWaitEvent(stream, aux->leave, 0) != hipSuccess) return (int)hipErrorUnknown;
+    return e;
+}
+
+template <typename R>
+int getrf_steps(cx<R>* A, int N, int64_t lda, int64_t strideA, int* ipiv, int64_t strideP, int* info, int batch,
+                const LuSetup* aux, hipStream_t stream)
+{
     int e;
     const int nb = getrf_block(N, aux ? aux->block : 0);
     const int leaf = aux ? aux->leaf : 16;

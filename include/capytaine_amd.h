@@ -276,8 +276,11 @@ int cpt_release_workspace(cpt_handle h);
 enum {
     CPT_TUNE_LU_BLOCK = 0,       /* block width of the LU: 64, 128, or 0 = 128 from 1024 rows */
     CPT_TUNE_PANEL_LEAF = 1,     /* panel columns factored by one work-group per matrix, 1..64 */
-    CPT_TUNE_LOOKAHEAD = 2,      /* 1: factor the next panel on a side stream, 0: in order */
-    CPT_TUNE_ASSEMBLY_CHUNK = 3  /* panels per assembly work-group, 0 = by size, else 1..1024 */
+    CPT_TUNE_LOOKAHEAD = 2,      /* 0: in order; 1: next panel on a side stream; 2: next panel on
+                                    CUs reserved for it (streams with CU masks; runs as 1 when the
+                                    device has no hardware queue left for them) */
+    CPT_TUNE_ASSEMBLY_CHUNK = 3, /* panels per assembly work-group, 0 = by size, else 1..1024 */
+    CPT_TUNE_PANEL_CUS = 4       /* CUs reserved for the panels when CPT_TUNE_LOOKAHEAD = 2, 1..128 */
 };
 int cpt_set_tuning(cpt_handle h, int param, int value);
 int cpt_get_tuning(cpt_handle h, int param, int* value);

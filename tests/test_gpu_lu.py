@@ -227,7 +227,8 @@ def test_kernel_timing():
     torch.cuda.synchronize()
 
 
-@pytest.mark.parametrize("block,leaf,lookahead", [(64, 8, 0), (64, 32, 1), (128, 16, 1), (128, 64, 0)])
+@pytest.mark.parametrize("block,leaf,lookahead", [(64, 8, 0), (64, 32, 1), (128, 16, 1), (128, 64, 0),
+                                                  (128, 16, 2), (64, 8, 2)])
 def test_lu_tunings_agree(block, leaf, lookahead):
     """Every tuning of the blocked LU solves to LAPACK's accuracy."""
     from scipy.linalg import lu_factor, lu_solve
@@ -244,9 +245,12 @@ def test_lu_tunings_agree(block, leaf, lookahead):
         As, Bs = np.array(As), np.array(Bs)
         X, info, _, _ = _factor_solve(As, Bs)
         assert not info.any()
+        Xm, _, res = _mixed(As, Bs)
+        assert res <= 1e-12
         for b in range(3):
             Xr = lu_solve(lu_factor(As[b]), Bs[b])
             assert np.linalg.norm(X[b] - Xr) / np.linalg.norm(Xr) < 1e-10
+            assert np.linalg.norm(Xm[b] - Xr) / np.linalg.norm(Xr) < 1e-10
     finally:
         _lib.set_tuning(ctx, **saved)
 
@@ -254,8 +258,8 @@ def test_lu_tunings_agree(block, leaf, lookahead):
 def test_tuning_rejects_bad_values():
     from capytaine_amd import _lib
     ctx = _lib.Context.get(device())
-    for name, bad in [("lu_block", 96), ("panel_leaf", 0), ("panel_leaf", 65), ("lookahead", 2),
-                      ("assembly_chunk", -1)]:
+    for name, bad in [("lu_block", 96), ("panel_leaf", 0), ("panel_leaf", 65), ("lookahead", 3),
+                      ("assembly_chunk", -1), ("panel_cus", 0), ("panel_cus", 129)]:
         with pytest.raises(_lib.CptError):
             _lib.set_tuning(ctx, **{name: bad})
     with pytest.raises(_lib.CptError):
