@@ -338,13 +338,25 @@ __global__ void laswp_kernel(cx<R>* __restrict__ Abase, int64_t lda, int64_t str
 
 // ---------------------------------------------------------------------------
 // 3. Triangular solve with a kb x kb diagonal block T (kb <= PB = 64), in
-//    place on the kb rows of B.  One wavefront per column of B, lane i holds
-//    x_i; the triangle of T is staged in LDS (column-major, lane-consecutive
-//    reads) once per work-group and reused for TRSM_WAVES x TRSM_CPW columns,
-//    two columns in flight per wavefront.
+//    place on the kb rows of B.  One wavefront per group of columns of B,
+//    lane i holds x_i of TRSM_ILP columns at once; x_j is broadcast with
+//    v_readlane (j is wave-uniform), T(i, j) comes from LDS, where the
+//    triangle is staged once per work-group (loads batched four per thread)
+//    and reused for TRSM_WAVES x TRSM_CPW columns.
 //    UPPER = false: T unit lower triangular; UPPER = true: T upper, non-unit.
 // ---------------------------------------------------------------------------
-constexpr int TRSM_WAVES = 4, TRSM_CPW = 8;
+constexpr int TRSM_WAVES = 4, TRSM_CPW = 8, TRSM_ILP = 4;
+
+__device__ __forceinline__ double lane_bcast(double v, int j)
+{
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), j);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), j);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ float lane_bcast(float v, int j)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+}
 
 template <typename R, bool UPPER>
 __global__ void __launch_bounds__(64 * TRSM_WAVES) trsm_kernel(const cx<R>* __restrict__ Tbase, int64_t ldt,
@@ -354,10 +366,22 @@ __global__ void __launch_bounds__(64 * TRSM_WAVES) trsm_kernel(const cx<R>* __re
     typedef cx<R> C;
     __shared__ C sT[PB * PB];
     __shared__ C sD[PB];
+    const C zero{R(0), R(0)};
     const C* T = Tbase + (size_t)blockIdx.y * strideT;
-    for (int e = threadIdx.x; e < kb * kb; e += blockDim.x) {
-        const int i = e % kb, j = e / kb;
-        if (UPPER ? (i <= j) : (i > j)) sT[i + PB * j] = T[i + (size_t)j * ldt];
+    const int tot = kb * kb;
+    for (int e0 = threadIdx.x; e0 < tot; e0 += 4 * 64 * TRSM_WAVES) {
+        C v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int e = e0 + u * 64 * TRSM_WAVES;
+            const int i = e % kb, j = e / kb;
+            v[u] = (e < tot && (UPPER ? (i <= j) : (i > j))) ? T[i + (size_t)j * ldt] : zero;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int e = e0 + u * 64 * TRSM_WAVES;
+            if (e < tot) sT[(e % kb) + PB * (e / kb)] = v[u];
+        }
     }
     __syncthreads();
     if (UPPER) {
@@ -366,43 +390,43 @@ __global__ void __launch_bounds__(64 * TRSM_WAVES) trsm_kernel(const cx<R>* __re
     }
     const int lane = threadIdx.x & 63;
     const int cbase = (blockIdx.x * TRSM_WAVES + (threadIdx.x >> 6)) * TRSM_CPW;
-    const C zero{R(0), R(0)};
-    for (int c = 0; c < TRSM_CPW; c += 2) {
+    C* Bm = Bbase + (size_t)blockIdx.y * strideB;
+    for (int c = 0; c < TRSM_CPW; c += TRSM_ILP) {
         const int col0 = cbase + c;
         if (col0 >= ncols) break;
-        const bool has1 = col0 + 1 < ncols;
-        C* B0 = Bbase + (size_t)blockIdx.y * strideB + (size_t)col0 * ldb;
-        C* B1 = B0 + ldb;
-        C x0 = (lane < kb) ? B0[lane] : zero;
-        C x1 = (has1 && lane < kb) ? B1[lane] : zero;
+        C x[TRSM_ILP];
+#pragma unroll
+        for (int q = 0; q < TRSM_ILP; ++q)
+            x[q] = (col0 + q < ncols && lane < kb) ? Bm[lane + (size_t)(col0 + q) * ldb] : zero;
         if (!UPPER) {
             for (int j = 0; j < kb - 1; ++j) {
-                const C y0{__shfl(x0.re, j), __shfl(x0.im, j)};
-                const C y1{__shfl(x1.re, j), __shfl(x1.im, j)};
-                if (lane > j && lane < kb) {
-                    const C t = sT[lane + PB * j];
-                    x0 = zsub(x0, zmul(t, y0));
-                    x1 = zsub(x1, zmul(t, y1));
+                const C t = sT[lane + PB * j];
+                const bool below = lane > j && lane < kb;
+#pragma unroll
+                for (int q = 0; q < TRSM_ILP; ++q) {
+                    const C y{lane_bcast(x[q].re, j), lane_bcast(x[q].im, j)};
+                    if (below) x[q] = zsub(x[q], zmul(t, y));
                 }
             }
         } else {
             for (int j = kb - 1; j >= 0; --j) {
-                if (lane == j) {
-                    x0 = zmul(x0, sD[j]);
-                    x1 = zmul(x1, sD[j]);
-                }
-                const C y0{__shfl(x0.re, j), __shfl(x0.im, j)};
-                const C y1{__shfl(x1.re, j), __shfl(x1.im, j)};
-                if (lane < j) {
-                    const C t = sT[lane + PB * j];
-                    x0 = zsub(x0, zmul(t, y0));
-                    x1 = zsub(x1, zmul(t, y1));
+                const C d = sD[j];
+#pragma unroll
+                for (int q = 0; q < TRSM_ILP; ++q)
+                    if (lane == j) x[q] = zmul(x[q], d);
+                const C t = sT[lane + PB * j];
+                const bool above = lane < j;
+#pragma unroll
+                for (int q = 0; q < TRSM_ILP; ++q) {
+                    const C y{lane_bcast(x[q].re, j), lane_bcast(x[q].im, j)};
+                    if (above) x[q] = zsub(x[q], zmul(t, y));
                 }
             }
         }
         if (lane < kb) {
-            B0[lane] = x0;
-            if (has1) B1[lane] = x1;
+#pragma unroll
+            for (int q = 0; q < TRSM_ILP; ++q)
+                if (col0 + q < ncols) Bm[lane + (size_t)(col0 + q) * ldb] = x[q];
         }
     }
 }
