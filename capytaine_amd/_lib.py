@@ -49,7 +49,7 @@ SIGNATURES = {
 }
 
 # Tuning parameters of cpt_set_tuning, in the order of the CPT_TUNE_* values.
-TUNING_PARAMS = ("lu_block", "panel_leaf", "lookahead", "assembly_chunk", "panel_cus")
+TUNING_PARAMS = ("lu_block", "panel_leaf", "lookahead", "assembly_chunk", "panel_cus", "gemm_tiles")
 
 # Kernel classes of cpt_profile_end, in the order of the CPT_TIMED_* values.
 TIMED_KERNELS = ("zgemm", "cgemm", "panel_z", "panel_c", "assembly", "skinny_z", "skinny_c")

@@ -27,6 +27,7 @@ LU_BLOCKS = (64, 128)
 PANEL_LEAVES = (8, 16)
 LOOKAHEAD = (0, 1, 2)        # in order, side stream, CUs reserved for the panels
 PANEL_CUS = (16, 32)         # reserved CUs tried with LOOKAHEAD = 2
+GEMM_TILES = (0, 1)          # complex64 product tiles: 128 x 64, 128 x 128
 ASSEMBLY_CHUNKS = (0, 4, 8, 16, 32, 64)
 
 _lock = threading.Lock()
@@ -133,24 +134,26 @@ def tune_lu(n, batch, precision="fp64", device=None, reps=2, nrhs=14, candidates
 
     timings = {}
     if candidates is None:
-        candidates = [(blk, leaf, la, cus) for blk, leaf, la in itertools.product(LU_BLOCKS, PANEL_LEAVES, LOOKAHEAD)
+        tiles = GEMM_TILES if precision == "mixed" else (0,)
+        candidates = [(blk, leaf, la, cus, tl)
+                      for blk, leaf, la, tl in itertools.product(LU_BLOCKS, PANEL_LEAVES, LOOKAHEAD, tiles)
                       for cus in (PANEL_CUS if la == 2 else PANEL_CUS[:1])]
     try:
-        for blk, leaf, la, cus in candidates:
+        for blk, leaf, la, cus, tl in candidates:
             if leaf > blk // 2:
                 continue
-            _lib.set_tuning(ctx, lu_block=blk, panel_leaf=leaf, lookahead=la, panel_cus=cus)
+            _lib.set_tuning(ctx, lu_block=blk, panel_leaf=leaf, lookahead=la, panel_cus=cus, gemm_tiles=tl)
             t = _time(run, device, reps)
-            timings[f"{blk}/{leaf}/{la}/{cus}"] = t
+            timings[f"{blk}/{leaf}/{la}/{cus}/{tl}"] = t
             if log:
                 log(f"[tune] n={n} batch={batch} {precision} block={blk} leaf={leaf} lookahead={la} "
-                    f"panel_cus={cus}: {t:.2f} ms")
+                    f"panel_cus={cus} gemm_tiles={tl}: {t:.2f} ms")
     finally:
         _lib.set_tuning(ctx, **saved)
     best_key = min(timings, key=timings.get)
-    blk, leaf, la, cus = (int(x) for x in best_key.split("/"))
-    best = {"lu_block": blk, "panel_leaf": leaf, "lookahead": la, "panel_cus": cus, "ms": timings[best_key],
-            "timings_ms": timings}
+    blk, leaf, la, cus, tl = (int(x) for x in best_key.split("/"))
+    best = {"lu_block": blk, "panel_leaf": leaf, "lookahead": la, "panel_cus": cus, "gemm_tiles": tl,
+            "ms": timings[best_key], "timings_ms": timings}
     if store:
         _store(_key(device, "lu", precision, n, batch), best)
     return best
@@ -199,7 +202,7 @@ def apply(ctx, n, batch, precision="fp64", nb_wavenumbers=None):
     best = lookup(device, "lu", precision, n, batch)
     if best:
         applied.update(lu_block=best["lu_block"], panel_leaf=best["panel_leaf"], lookahead=best["lookahead"],
-                       panel_cus=best.get("panel_cus", 16))
+                       panel_cus=best.get("panel_cus", 16), gemm_tiles=best.get("gemm_tiles", 1))
     best = lookup(device, "assembly", "fp64", n, nb_wavenumbers or batch)
     if best:
         applied["assembly_chunk"] = best["assembly_chunk"]

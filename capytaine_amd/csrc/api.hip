@@ -39,6 +39,7 @@ static thread_local std::string g_last_error;
 void set_error(const std::string& msg) { g_last_error = msg; }
 
 thread_local KernelTimer* active_timer = nullptr;
+thread_local int active_gemm_tiles = 0;
 
 int KernelTimer::start(hipStream_t s, int kind, double work)
 {
@@ -72,11 +73,21 @@ void KernelTimer::release()
     on = false;
 }
 
-// Installs a context's timer for the calling thread during one entry point.
-struct TimerScope {
+// Installs a context's timer and product tiling for the calling thread
+// during one entry point.
+struct CallScope {
     KernelTimer* prev;
-    explicit TimerScope(KernelTimer* t) : prev(active_timer) { active_timer = (t && t->on) ? t : nullptr; }
-    ~TimerScope() { active_timer = prev; }
+    int prev_tiles;
+    CallScope(KernelTimer* t, int tiles) : prev(active_timer), prev_tiles(active_gemm_tiles)
+    {
+        active_timer = (t && t->on) ? t : nullptr;
+        active_gemm_tiles = tiles;
+    }
+    ~CallScope()
+    {
+        active_timer = prev;
+        active_gemm_tiles = prev_tiles;
+    }
 };
 
 // Growable device buffer owned by a context.
@@ -120,6 +131,7 @@ struct cpt_context {
     cpt::LuSetup lu_streams;
     int asm_chunk = 0;  // panels per assembly work-group, 0 = by size
     int part_cus = 0;   // CUs of the current partitioned look-ahead streams
+    int gemm_tiles = 0; // 1: complex64 products on 128 x 128 tiles, 0: 128 x 64
     bool part_failed = false;  // the device refused CU-masked streams
     void* ext_work = nullptr;  // caller-owned workspace of the mixed solves
     size_t ext_bytes = 0;
@@ -556,7 +568,7 @@ int cpt_build_matrices(cpt_handle h, int nb_collocation_points, const double* co
         return CPT_ERR_STATE;
     }
     CPT_CHECK_HIP(hipSetDevice(h->device), "hipSetDevice");
-    TimerScope timer_scope(&h->timer);
+    CallScope call_scope(&h->timer, h->gemm_tiles);
 
     // Pack panels and quadrature points.
     const int ne = finite_depth ? std::max(nexp, 0) : 0;
@@ -676,7 +688,7 @@ int cpt_build_matrices_batched(cpt_handle h, int nb_bodies, const cpt_body* bodi
         return CPT_ERR_STATE;
     }
     CPT_CHECK_HIP(hipSetDevice(h->device), "hipSetDevice");
-    TimerScope timer_scope(&h->timer);
+    CallScope call_scope(&h->timer, h->gemm_tiles);
     if (h->bfaces.reserve(sizeof(Face) * tot_m) || h->bquad.reserve(sizeof(double) * 4 * tot_q) ||
         h->bfreq.reserve(sizeof(FreqArgs) * tot_nb) || h->bargs.reserve(sizeof(AssemblyArgs) * nb_bodies)) {
         set_error("cpt_build_matrices_batched: device allocation failed");
@@ -800,7 +812,7 @@ int cpt_wave_part_infinite_depth(cpt_handle h, int np, const double* x0i, const 
     }
     if (np == 0) return CPT_OK;
     CPT_CHECK_HIP(hipSetDevice(h->device), "hipSetDevice");
-    TimerScope timer_scope(&h->timer);
+    CallScope call_scope(&h->timer, h->gemm_tiles);
     hipLaunchKernelGGL(wave_points_kernel, dim3((np + 255) / 256), dim3(256), 0, stream, table_of(h), np, x0i, x0j,
                        wavenumber, gf_singularities, (cplx*)g, (cplx*)nablag);
     CPT_CHECK_HIP(hipGetLastError(), "wave kernel launch");
@@ -824,7 +836,7 @@ int cpt_lu_factor(cpt_handle h, void* A, int n, int64_t lda, int64_t strideA, in
         return CPT_ERR_UNSUPPORTED;
     }
     CPT_CHECK_HIP(hipSetDevice(h->device), "hipSetDevice");
-    TimerScope timer_scope(&h->timer);
+    CallScope call_scope(&h->timer, h->gemm_tiles);
     hipLaunchKernelGGL(zero_int_kernel, dim3((batch + 255) / 256), dim3(256), 0, stream, info, batch);
     CPT_CHECK_HIP(hipGetLastError(), "zero info launch");
     if (h->lu_work.reserve(getrf_workspace_bytes(n, batch))) {
@@ -855,7 +867,7 @@ int cpt_lu_solve(cpt_handle h, const void* LU, int n, int64_t lda, int64_t strid
         return CPT_ERR_ARG;
     }
     CPT_CHECK_HIP(hipSetDevice(h->device), "hipSetDevice");
-    TimerScope timer_scope(&h->timer);
+    CallScope call_scope(&h->timer, h->gemm_tiles);
     CPT_CHECK_HIP((hipError_t)launch_getrs((const cplx*)LU, n, lda, strideA, ipiv, strideP, (cplx*)B, nrhs, ldb,
                                            strideB, batch, stream),
                   "getrs launch");
@@ -876,7 +888,7 @@ int cpt_zgemm(cpt_handle h, int M, int N, int K, double alpha, const void* A, in
         return CPT_ERR_ARG;
     }
     CPT_CHECK_HIP(hipSetDevice(h->device), "hipSetDevice");
-    TimerScope timer_scope(&h->timer);
+    CallScope call_scope(&h->timer, h->gemm_tiles);
     CPT_CHECK_HIP((hipError_t)launch_zgemm(M, N, K, alpha, (const cplx*)A, lda, strideA, (const cplx*)B, ldb, strideB,
                                            (cplx*)C, ldc, strideC, beta_zero, batch, stream),
                   "zgemm launch");
@@ -908,7 +920,7 @@ int cpt_lu_solve_mixed(cpt_handle h, const void* K, int n, int64_t lda, int64_t 
         return CPT_ERR_UNSUPPORTED;
     }
     CPT_CHECK_HIP(hipSetDevice(h->device), "hipSetDevice");
-    TimerScope timer_scope(&h->timer);
+    CallScope call_scope(&h->timer, h->gemm_tiles);
     void* work = mixed_workspace(h, mixed_workspace_bytes(n, nrhs, batch));
     if (!work) {
         set_error("cpt_lu_solve_mixed: workspace allocation failed");
@@ -988,6 +1000,10 @@ int cpt_set_tuning(cpt_handle h, int param, int value)
         if (value < 1 || value > 128) break;
         h->lu_streams.panel_cus = value;
         return CPT_OK;
+    case CPT_TUNE_GEMM_TILES:
+        if (value < 0 || value > 1) break;
+        h->gemm_tiles = value;
+        return CPT_OK;
     case CPT_TUNE_ASSEMBLY_CHUNK:
         if (value < 0 || value > 1024) break;
         h->asm_chunk = value;
@@ -1012,6 +1028,7 @@ int cpt_get_tuning(cpt_handle h, int param, int* value)
     case CPT_TUNE_PANEL_LEAF: *value = h->lu_streams.leaf; return CPT_OK;
     case CPT_TUNE_LOOKAHEAD: *value = h->lu_streams.lookahead; return CPT_OK;
     case CPT_TUNE_PANEL_CUS: *value = h->lu_streams.panel_cus; return CPT_OK;
+    case CPT_TUNE_GEMM_TILES: *value = h->gemm_tiles; return CPT_OK;
     case CPT_TUNE_ASSEMBLY_CHUNK: *value = h->asm_chunk; return CPT_OK;
     default:
         set_error("cpt_get_tuning: unknown parameter " + std::to_string(param));
@@ -1028,7 +1045,7 @@ int cpt_lu_factor_grouped(cpt_handle h, int ngroups, const cpt_lu_group* groups,
     if (!valid_groups(groups, ngroups, true, false, false, "cpt_lu_factor_grouped")) return CPT_ERR_ARG;
     if (ngroups == 0) return CPT_OK;
     CPT_CHECK_HIP(hipSetDevice(h->device), "hipSetDevice");
-    TimerScope timer_scope(&h->timer);
+    CallScope call_scope(&h->timer, h->gemm_tiles);
     if (ensure_lanes(h) || fork_lanes(h, stream)) {
         set_error("cpt_lu_factor_grouped: could not set up the lane streams");
         return CPT_ERR_HIP;
@@ -1061,7 +1078,7 @@ int cpt_lu_solve_grouped(cpt_handle h, int ngroups, const cpt_lu_group* groups, 
     if (!valid_groups(groups, ngroups, true, true, false, "cpt_lu_solve_grouped")) return CPT_ERR_ARG;
     if (ngroups == 0) return CPT_OK;
     CPT_CHECK_HIP(hipSetDevice(h->device), "hipSetDevice");
-    TimerScope timer_scope(&h->timer);
+    CallScope call_scope(&h->timer, h->gemm_tiles);
     if (ensure_lanes(h) || fork_lanes(h, stream)) {
         set_error("cpt_lu_solve_grouped: could not set up the lane streams");
         return CPT_ERR_HIP;
@@ -1093,7 +1110,7 @@ int cpt_lu_solve_mixed_grouped(cpt_handle h, int ngroups, const cpt_lu_group* gr
     if (!valid_groups(groups, ngroups, false, true, true, "cpt_lu_solve_mixed_grouped")) return CPT_ERR_ARG;
     if (ngroups == 0) return CPT_OK;
     CPT_CHECK_HIP(hipSetDevice(h->device), "hipSetDevice");
-    TimerScope timer_scope(&h->timer);
+    CallScope call_scope(&h->timer, h->gemm_tiles);
     if (ensure_lanes(h) || fork_lanes(h, stream)) {
         set_error("cpt_lu_solve_mixed_grouped: could not set up the lane streams");
         return CPT_ERR_HIP;

@@ -122,6 +122,9 @@ struct KernelTimer {
     void release();
 };
 extern thread_local KernelTimer* active_timer;
+// Tile width of the tiled complex64 product for the calling thread's current
+// entry point (1: 128 x 128 tiles, 0: 128 x 64).
+extern thread_local int active_gemm_tiles;
 
 // Brackets one launch with the active timer's events (no-op when off).
 struct TimedLaunch {

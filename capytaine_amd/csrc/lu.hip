@@ -614,16 +614,19 @@ __global__ void __launch_bounds__(256) gemm_skinny_kernel(int M, int N, int K, R
 }
 
 // ---------------------------------------------------------------------------
-// 4b. Same product, LDS-tiled for large M: work-group tile 128 x 64, K staged
-//     in chunks of 16 through LDS (A as [k][row], B as [col][k] padded to 17
-//     so that both the stores and the MFMA fragment reads are free of bank
-//     conflicts), the next chunk prefetched into registers while the current
-//     one feeds the MFMAs.  Wavefront w computes rows 64 (w & 1) .. +64 and
-//     columns 32 (w >> 1) .. +32: a 4 x 2 grid of 16 x 16 tiles.
+// 4b. Same product, LDS-tiled for large M: work-group tile 128 x TBN (TBN =
+//     64, or 128 for complex64), K staged in chunks of 16 through LDS (A as [k][row], B as
+//     [col][k] padded to 17 so that both the stores and the MFMA fragment
+//     reads are free of bank conflicts), the next chunk prefetched into
+//     registers while the current one feeds the MFMAs.  Wavefront w computes
+//     rows 64 (w & 1) .. +64 and columns (TBN/2) (w >> 1) .. +TBN/2: a
+//     4 x TBN/32 grid of 16 x 16 tiles.  The wider tile doubles the MFMA work
+//     per barrier and per A fragment (used for complex64, whose MFMA takes
+//     half the cycles of complex128's).
 // ---------------------------------------------------------------------------
-constexpr int TBM = 128, TBN = 64, TKC = 16;
+constexpr int TBM = 128, TKC = 16;
 
-template <typename R>
+template <typename R, int TBN>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 gemm_tiled_kernel(int M, int N, int K, R alpha, const cx<R>* __restrict__ Abase, int64_t lda, int64_t strideA,
                   const cx<R>* __restrict__ Bbase, int64_t ldb, int64_t strideB, cx<R>* __restrict__ Cbase,
@@ -631,6 +634,8 @@ gemm_tiled_kernel(int M, int N, int K, R alpha, const cx<R>* __restrict__ Abase,
 {
     typedef cx<R> Cx;
     typedef typename Mfma<R>::acc acc;
+    constexpr int FB = TBN / 32;        // 16-column tiles per wavefront
+    constexpr int NB_LOAD = TBN / 16;   // B values per thread per chunk
     __shared__ Cx sA[TKC][TBM];
     __shared__ Cx sB[TBN][TKC + 1];
     const int tid = threadIdx.x;
@@ -642,14 +647,14 @@ gemm_tiled_kernel(int M, int N, int K, R alpha, const cx<R>* __restrict__ Abase,
     const Cx* B = Bbase + (size_t)blockIdx.z * strideB;
     Cx* C = Cbase + (size_t)blockIdx.z * strideC;
 
-    Cx ra[8], rb[4];
+    Cx ra[8], rb[NB_LOAD];
     const int a_row = row0 + (tid & 127);
     const bool a_ok = a_row < M;
-    acc cr[4][2], ci[4][2];
+    acc cr[4][FB], ci[4][FB];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) {
+        for (int b = 0; b < FB; ++b) {
             cr[a][b] = acc{0, 0, 0, 0};
             ci[a][b] = acc{0, 0, 0, 0};
         }
@@ -662,7 +667,7 @@ gemm_tiled_kernel(int M, int N, int K, R alpha, const cx<R>* __restrict__ Abase,
             if (a_ok && k < K) ra[i] = A[a_row + (size_t)k * lda];                            \
             else ra[i].re = ra[i].im = R(0);                                                  \
         }                                                                                     \
-        _Pragma("unroll") for (int i = 0; i < 4; ++i)                                         \
+        _Pragma("unroll") for (int i = 0; i < NB_LOAD; ++i)                                   \
         {                                                                                     \
             const int idx = tid + 256 * i;                                                    \
             const int col = col0 + (idx >> 4), k = (kc_) + (idx & 15);                        \
@@ -676,7 +681,7 @@ gemm_tiled_kernel(int M, int N, int K, R alpha, const cx<R>* __restrict__ Abase,
 #pragma unroll
         for (int i = 0; i < 8; ++i) sA[(tid >> 7) + 2 * i][tid & 127] = ra[i];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
+        for (int i = 0; i < NB_LOAD; ++i) {
             const int idx = tid + 256 * i;
             sB[idx >> 4][idx & 15] = rb[i];
         }
@@ -684,7 +689,7 @@ gemm_tiled_kernel(int M, int N, int K, R alpha, const cx<R>* __restrict__ Abase,
         if (kc + TKC < K) CPT_LOAD_CHUNK(kc + TKC);
 #pragma unroll
         for (int ks = 0; ks < TKC; ks += 4) {
-            R ar[4], ai[4], br[2], bi[2];
+            R ar[4], ai[4], br[FB], bi[FB];
 #pragma unroll
             for (int a = 0; a < 4; ++a) {
                 const Cx v = sA[ks + lk][wm * 64 + a * 16 + li];
@@ -692,22 +697,22 @@ gemm_tiled_kernel(int M, int N, int K, R alpha, const cx<R>* __restrict__ Abase,
                 ai[a] = v.im;
             }
 #pragma unroll
-            for (int b = 0; b < 2; ++b) {
-                const Cx v = sB[wn * 32 + b * 16 + li][ks + lk];
+            for (int b = 0; b < FB; ++b) {
+                const Cx v = sB[wn * (TBN / 2) + b * 16 + li][ks + lk];
                 br[b] = v.re;
                 bi[b] = v.im;
             }
 #pragma unroll
             for (int a = 0; a < 4; ++a)
 #pragma unroll
-                for (int b = 0; b < 2; ++b) {
+                for (int b = 0; b < FB; ++b) {
                     cr[a][b] = Mfma<R>::op(ar[a], br[b], cr[a][b]);
                     ci[a][b] = Mfma<R>::op(ar[a], bi[b], ci[a][b]);
                 }
 #pragma unroll
             for (int a = 0; a < 4; ++a)
 #pragma unroll
-                for (int b = 0; b < 2; ++b) {
+                for (int b = 0; b < FB; ++b) {
                     cr[a][b] = Mfma<R>::op(-ai[a], bi[b], cr[a][b]);
                     ci[a][b] = Mfma<R>::op(ai[a], br[b], ci[a][b]);
                 }
@@ -718,11 +723,11 @@ gemm_tiled_kernel(int M, int N, int K, R alpha, const cx<R>* __restrict__ Abase,
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
+        for (int b = 0; b < FB; ++b)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int row = row0 + wm * 64 + 16 * a + Mfma<R>::row(lane, v);
-                const int col = col0 + wn * 32 + 16 * b + li;
+                const int col = col0 + wn * (TBN / 2) + 16 * b + li;
                 if (row < M && col < N) {
                     Cx* c = C + row + (size_t)col * ldc;
                     R re = beta_zero ? R(0) : c->re;
@@ -734,6 +739,7 @@ gemm_tiled_kernel(int M, int N, int K, R alpha, const cx<R>* __restrict__ Abase,
                 }
             }
 }
+
 
 template <typename R, int NT, int RPT, int SB>
 void launch_panel(cx<R>* A, int64_t lda, int64_t sA, int N, int k0, int kb, int sw_lo, int sw_hi, int* ipiv,
@@ -819,10 +825,22 @@ int gemm(int M, int N, int K, R alpha, const cx<R>* A, int64_t lda, int64_t sA, 
         hipLaunchKernelGGL(gemm_skinny_kernel<R>, grid, dim3(256), 0, stream, M, N, K, alpha, A, lda, sA, B, ldb, sB,
                            C, ldc, sC, beta_zero);
     } else if (M >= 96 && K >= 16) {
-        dim3 grid((M + TBM - 1) / TBM, (N + TBN - 1) / TBN, batch);
         TimedLaunch timed(stream, sizeof(R) == 8 ? TK_ZGEMM : TK_CGEMM, 8.0 * M * N * K * batch);
-        hipLaunchKernelGGL(gemm_tiled_kernel<R>, grid, dim3(256), 0, stream, M, N, K, alpha, A, lda, sA, B, ldb, sB, C,
-                           ldc, sC, beta_zero);
+        // complex64: 128 x 128 tiles for wide products when tuned so
+        // (complex128 would need more than the 512 registers of a lane)
+        if constexpr (sizeof(R) == 4) {
+            if (active_gemm_tiles && N > 64) {
+                dim3 grid((M + TBM - 1) / TBM, (N + 127) / 128, batch);
+                hipLaunchKernelGGL((gemm_tiled_kernel<R, 128>), grid, dim3(256), 0, stream, M, N, K, alpha, A, lda,
+                                   sA, B, ldb, sB, C, ldc, sC, beta_zero);
+                return (int)hipGetLastError();
+            }
+        }
+        {
+            dim3 grid((M + TBM - 1) / TBM, (N + 63) / 64, batch);
+            hipLaunchKernelGGL((gemm_tiled_kernel<R, 64>), grid, dim3(256), 0, stream, M, N, K, alpha, A, lda, sA, B,
+                               ldb, sB, C, ldc, sC, beta_zero);
+        }
     } else {
         dim3 grid((M + 63) / 64, (N + 63) / 64, batch);
         hipLaunchKernelGGL(gemm_small_kernel<R>, grid, dim3(256), 0, stream, M, N, K, alpha, A, lda, sA, B, ldb, sB, C,

@@ -227,17 +227,17 @@ def test_kernel_timing():
     torch.cuda.synchronize()
 
 
-@pytest.mark.parametrize("block,leaf,lookahead", [(64, 8, 0), (64, 32, 1), (128, 16, 1), (128, 64, 0),
-                                                  (128, 16, 2), (64, 8, 2)])
-def test_lu_tunings_agree(block, leaf, lookahead):
+@pytest.mark.parametrize("block,leaf,lookahead,tiles", [(64, 8, 0, 1), (64, 32, 1, 0), (128, 16, 1, 1),
+                                                        (128, 64, 0, 0), (128, 16, 2, 1), (64, 8, 2, 0)])
+def test_lu_tunings_agree(block, leaf, lookahead, tiles):
     """Every tuning of the blocked LU solves to LAPACK's accuracy."""
     from scipy.linalg import lu_factor, lu_solve
     from capytaine_amd import _lib
-    print(f"[gpu] LU tuning block={block} leaf={leaf} lookahead={lookahead}", flush=True)
+    print(f"[gpu] LU tuning block={block} leaf={leaf} lookahead={lookahead} tiles={tiles}", flush=True)
     ctx = _lib.Context.get(device())
     saved = _lib.get_tuning(ctx)
     try:
-        _lib.set_tuning(ctx, lu_block=block, panel_leaf=leaf, lookahead=lookahead)
+        _lib.set_tuning(ctx, lu_block=block, panel_leaf=leaf, lookahead=lookahead, gemm_tiles=tiles)
         assert _lib.get_tuning(ctx)["lu_block"] == block
         rng = np.random.default_rng(block + leaf)
         n = 517
@@ -259,7 +259,7 @@ def test_tuning_rejects_bad_values():
     from capytaine_amd import _lib
     ctx = _lib.Context.get(device())
     for name, bad in [("lu_block", 96), ("panel_leaf", 0), ("panel_leaf", 65), ("lookahead", 3),
-                      ("assembly_chunk", -1), ("panel_cus", 0), ("panel_cus", 129)]:
+                      ("assembly_chunk", -1), ("panel_cus", 0), ("panel_cus", 129), ("gemm_tiles", 2)]:
         with pytest.raises(_lib.CptError):
             _lib.set_tuning(ctx, **{name: bad})
     with pytest.raises(_lib.CptError):
