@@ -78,6 +78,19 @@ def cpu_baseline(mesh, body, pbs_by_freq, table, n_freq=6):
                        f"{dt:.1f} s")
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (a line "traffic: <kernel> <bytes>"), or None."""
+    path = os.path.join(ROOT, "profiles", "c2_mixed_hbm_traffic_counters.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.split()
+        if len(parts) == 3 and parts[0] == "traffic:" and parts[1] == kernel:
+            return float(parts[2])
+    return None
+
+
 def assembly_error(gf, mesh, ks, _lib):
     """Largest |S_mixed - S_fp64| / max |S_fp64| (and the same for K) over
     the given wavenumbers: the FP32 far field against the FP64 assembly."""
@@ -232,6 +245,24 @@ def serving_benchmark(gf, dev, seed=1, requests=24, loads=(0.5, 0.8, 1.0, 1.2), 
     return out
 
 
+def copy_calibration(nbytes=1 << 30, reps=4):
+    """Element-wise copy of `nbytes` (float32, y = 1 * x) `reps` times: the
+    HBM counters of this kernel (FETCH_SIZE, WRITE_SIZE) against the known
+    bytes read and written give their scale factor."""
+    import torch
+    dev = torch.device("cuda", 0)
+    x = torch.ones(nbytes // 4, dtype=torch.float32, device=dev)
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        torch.mul(x, 1.0, out=y)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    print(json.dumps({"workload": "copy", "bytes_read_per_launch": nbytes, "bytes_written_per_launch": nbytes,
+                      "launches": reps, "GB_per_s": 2 * nbytes / dt / 1e9}), flush=True)
+
+
 def cylinders_benchmark(args, rho=1025.0, g=9.81, chunk=4):
     """5 x 5 truncated cylinders (20000 panels), depth 40 m, 20 frequencies,
     150 radiation dof + 8 headings: the finite-depth assembly and the
@@ -310,9 +341,12 @@ def main():
     ap.add_argument("--no-serving", action="store_true", help="skip the request-stream latency measurement")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner (a stored tuning of this size class is still applied)")
-    ap.add_argument("--workload", choices=("barge", "cylinders"), default="barge",
+    ap.add_argument("--workload", choices=("barge", "cylinders", "copy"), default="barge",
                     help="barge: the headline line (plus fleet and serving); cylinders: the 20000-panel "
-                         "finite-depth array")
+                         "finite-depth array; copy: a 1 GiB device copy, the calibration of the HBM "
+                         "byte counters")
+    ap.add_argument("--only", choices=("fp64", "mixed"), default=None,
+                    help="run one form of the barge sweep only (for counter passes); prints no JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the frequency sweep shards by "
@@ -320,6 +354,9 @@ def main():
         sys.exit(2)
     if args.workload == "cylinders":
         cylinders_benchmark(args)
+        return
+    if args.workload == "copy":
+        copy_calibration()
         return
 
     import torch
@@ -392,6 +429,11 @@ def main():
         torch.cuda.empty_cache()
         return out
 
+    if args.only:
+        # one form, for profiler counter passes: same kernels and shapes
+        r = run(args.only)
+        print(f"[bench] {args.only}: {r['seconds'] / args.steps * 1e3:.1f} ms per step", file=sys.stderr, flush=True)
+        return
     r64 = run("fp64")
     rmx = run("mixed")
     rff = run("mixed", far_field_fp32=True)
@@ -430,7 +472,14 @@ def main():
                     "frac": achieved / peak, "traffic": None,
                     "work_per_launch": "8 M N K flop per complex M x N x K product, summed over the batch"}
     roofline.update(launches=kd["launches"], avg_launch_us=kd["ms"] / kd["launches"] * 1e3,
-                    work_per_launch_avg=kd["work"] / kd["launches"])
+                    work_per_launch_avg=kd["work"] / kd["launches"],
+                    algorithmic_bytes_per_launch=kd["bytes"] / kd["launches"])
+    traffic = measured_traffic(names[dom].split(" ")[0])
+    if traffic:
+        roofline["traffic"] = traffic
+        roofline["traffic_over_algorithmic"] = traffic / roofline["algorithmic_bytes_per_launch"]
+        roofline["traffic_source"] = "profiles/c2_mixed_hbm_traffic_counters.md (rocprofv3 FETCH_SIZE, " \
+                                     "WRITE_SIZE, scaled by a copy kernel)"
 
     def kernel_table(r):
         out = {}

@@ -35,7 +35,7 @@ SIGNATURES = {
     "cpt_lu_solve_mixed": (_I, [_P, _P, _I, _L, _L, _I, _P, _I, _L, _L, _P, _L, _L, _I, _D,
                                 ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double), _P]),
     "cpt_profile_begin": (_I, [_P]),
-    "cpt_profile_end": (_I, [_P, _P, _P, _P]),
+    "cpt_profile_end": (_I, [_P, _P, _P, _P, _P]),
     "cpt_build_matrices_batched": (_I, [_P, _I, _P, _I, _I, _D, _P]),
     "cpt_lu_factor_grouped": (_I, [_P, _I, _P, _P]),
     "cpt_lu_solve_grouped": (_I, [_P, _I, _P, _P]),
@@ -187,11 +187,11 @@ def profile_begin(ctx):
 
 
 def profile_end(ctx):
-    """Stop the timing; {class: {"ms", "work", "launches"}} summed since begin."""
+    """Stop the timing; {class: {"ms", "work", "launches", "bytes"}} summed since begin."""
     n = len(TIMED_KERNELS)
-    ms, work, launches = np.zeros(n), np.zeros(n), np.zeros(n, dtype=np.int64)
-    check(load().cpt_profile_end(ctx.handle, ptr(ms), ptr(work), ptr(launches)), "cpt_profile_end")
-    return {k: {"ms": float(ms[i]), "work": float(work[i]), "launches": int(launches[i])}
+    ms, work, launches, nbytes = np.zeros(n), np.zeros(n), np.zeros(n, dtype=np.int64), np.zeros(n)
+    check(load().cpt_profile_end(ctx.handle, ptr(ms), ptr(work), ptr(launches), ptr(nbytes)), "cpt_profile_end")
+    return {k: {"ms": float(ms[i]), "work": float(work[i]), "launches": int(launches[i]), "bytes": float(nbytes[i])}
             for i, k in enumerate(TIMED_KERNELS)}
 
 

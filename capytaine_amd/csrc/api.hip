@@ -41,7 +41,7 @@ void set_error(const std::string& msg) { g_last_error = msg; }
 thread_local KernelTimer* active_timer = nullptr;
 thread_local int active_gemm_tiles = 0;
 
-int KernelTimer::start(hipStream_t s, int kind, double work)
+int KernelTimer::start(hipStream_t s, int kind, double work, double bytes)
 {
     if (!on) return -1;
     if (used == recs.size()) {
@@ -56,6 +56,7 @@ int KernelTimer::start(hipStream_t s, int kind, double work)
     Rec& r = recs[used];
     r.kind = kind;
     r.work = work;
+    r.bytes = bytes;
     if (hipEventRecord(r.a, s) != hipSuccess) return -1;
     return (int)used++;
 }
@@ -950,7 +951,7 @@ int cpt_profile_begin(cpt_handle h)
     return CPT_OK;
 }
 
-int cpt_profile_end(cpt_handle h, double* ms, double* work, int64_t* launches)
+int cpt_profile_end(cpt_handle h, double* ms, double* work, int64_t* launches, double* bytes)
 {
     if (!h || !ms || !work || !launches) {
         set_error("cpt_profile_end: null argument");
@@ -961,6 +962,7 @@ int cpt_profile_end(cpt_handle h, double* ms, double* work, int64_t* launches)
         ms[k] = 0.0;
         work[k] = 0.0;
         launches[k] = 0;
+        if (bytes) bytes[k] = 0.0;
     }
     KernelTimer& t = h->timer;
     t.on = false;
@@ -972,6 +974,7 @@ int cpt_profile_end(cpt_handle h, double* ms, double* work, int64_t* launches)
         ms[r.kind] += dt;
         work[r.kind] += r.work;
         launches[r.kind] += 1;
+        if (bytes) bytes[r.kind] += r.bytes;
     }
     t.used = 0;
     return CPT_OK;

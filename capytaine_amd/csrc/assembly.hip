@@ -391,7 +391,8 @@ int launch_assembly(const AssemblyArgs& A, hipStream_t stream)
     const int gx = (A.n + 255) / 256;
     const int gy = (A.m + A.j_chunk - 1) / A.j_chunk;
     dim3 grid(gx, gy);
-    TimedLaunch timed(stream, TK_ASSEMBLY, (double)A.n * A.m * A.nb);
+    // stores of S and K: 32 bytes per pair and wavenumber
+    TimedLaunch timed(stream, TK_ASSEMBLY, (double)A.n * A.m * A.nb, 32.0 * A.n * A.m * A.nb);
     if (A.finite_depth) hipLaunchKernelGGL(assemble_finite_depth_kernel, grid, block, 0, stream, A);
     else if (A.far_fp32) hipLaunchKernelGGL(assemble_infinite_depth_mixed_kernel, grid, block, 0, stream, A);
     else hipLaunchKernelGGL(assemble_infinite_depth_kernel, grid, block, 0, stream, A);
@@ -402,7 +403,7 @@ int launch_assembly_worklist(const AssemblyArgs* bodies, const AsmTile* tiles, i
                              int far_fp32, hipStream_t stream)
 {
     if (ntiles <= 0) return 0;
-    TimedLaunch timed(stream, TK_ASSEMBLY, pairs);
+    TimedLaunch timed(stream, TK_ASSEMBLY, pairs, 32.0 * pairs);
     if (far_fp32)
         hipLaunchKernelGGL(assemble_worklist_kernel<true>, dim3(grid), dim3(256), 0, stream, bodies, tiles, ntiles);
     else

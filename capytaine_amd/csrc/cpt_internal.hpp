@@ -113,11 +113,12 @@ struct KernelTimer {
         hipEvent_t a = nullptr, b = nullptr;
         int kind = 0;
         double work = 0.0;
+        double bytes = 0.0;  // algorithmic bytes moved (operands read, results written)
     };
     std::vector<Rec> recs;
     size_t used = 0;
     bool on = false;
-    int start(hipStream_t s, int kind, double work);  // slot, or -1
+    int start(hipStream_t s, int kind, double work, double bytes);  // slot, or -1
     void stop(int slot, hipStream_t s);
     void release();
 };
@@ -130,8 +131,8 @@ extern thread_local int active_gemm_tiles;
 struct TimedLaunch {
     int slot;
     hipStream_t s;
-    TimedLaunch(hipStream_t stream, int kind, double work)
-        : slot(active_timer ? active_timer->start(stream, kind, work) : -1), s(stream)
+    TimedLaunch(hipStream_t stream, int kind, double work, double bytes = 0.0)
+        : slot(active_timer ? active_timer->start(stream, kind, work, bytes) : -1), s(stream)
     {
     }
     ~TimedLaunch()

@@ -819,13 +819,15 @@ int gemm(int M, int N, int K, R alpha, const cx<R>* A, int64_t lda, int64_t sA, 
     if (M <= 0 || N <= 0 || batch <= 0) return 0;
     if (N <= 2 * SKN && M >= SKM && K >= 4 * SKK) {
         dim3 grid((M + SKM - 1) / SKM, (N + SKN - 1) / SKN, batch);
-        TimedLaunch timed(stream, sizeof(R) == 8 ? TK_SKINNY_Z : TK_SKINNY_C,
-                          (double)sizeof(cx<R>) * ((double)M * K + (double)K * N + (beta_zero ? 1.0 : 2.0) * M * N) *
-                              batch);
+        const double nbytes =
+            (double)sizeof(cx<R>) * ((double)M * K + (double)K * N + (beta_zero ? 1.0 : 2.0) * M * N) * batch;
+        TimedLaunch timed(stream, sizeof(R) == 8 ? TK_SKINNY_Z : TK_SKINNY_C, nbytes, nbytes);
         hipLaunchKernelGGL(gemm_skinny_kernel<R>, grid, dim3(256), 0, stream, M, N, K, alpha, A, lda, sA, B, ldb, sB,
                            C, ldc, sC, beta_zero);
     } else if (M >= 96 && K >= 16) {
-        TimedLaunch timed(stream, sizeof(R) == 8 ? TK_ZGEMM : TK_CGEMM, 8.0 * M * N * K * batch);
+        TimedLaunch timed(stream, sizeof(R) == 8 ? TK_ZGEMM : TK_CGEMM, 8.0 * M * N * K * batch,
+                          (double)sizeof(cx<R>) * ((double)M * K + (double)K * N + (beta_zero ? 1.0 : 2.0) * M * N) *
+                              batch);
         // complex64: 128 x 128 tiles for wide products when tuned so
         // (complex128 would need more than the 512 registers of a lane)
         if constexpr (sizeof(R) == 4) {

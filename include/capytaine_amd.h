@@ -224,8 +224,10 @@ int cpt_zgemm(cpt_handle h, int M, int N, int K, double alpha, const void* A, in
  * summed algorithmic work (flops for the product and panel classes, 8 M N K
  * per complex product; bytes of A, B and C moved for the skinny products;
  * collocation-point x panel x wavenumber pairs for the assembly) and the
- * number of launches: arrays of CPT_TIMED_KERNELS.  A panel bracket spans
- * the products launched inside it, which are also counted in their class. */
+ * number of launches, and, when `bytes` is not NULL, the algorithmic bytes
+ * the launches read and write (operands once, results once): arrays of
+ * CPT_TIMED_KERNELS.  A panel bracket spans the products launched inside
+ * it, which are also counted in their class. */
 enum {
     CPT_TIMED_ZGEMM = 0,     /* complex128 MFMA tile product (LU update) */
     CPT_TIMED_CGEMM = 1,     /* complex64 MFMA tile product (mixed precision) */
@@ -237,7 +239,7 @@ enum {
     CPT_TIMED_KERNELS = 7
 };
 int cpt_profile_begin(cpt_handle h);
-int cpt_profile_end(cpt_handle h, double* ms, double* work, int64_t* launches);
+int cpt_profile_end(cpt_handle h, double* ms, double* work, int64_t* launches, double* bytes);
 
 /* Precision of the following assemblies of `h` (cpt_build_matrices,
  * cpt_build_matrices_batched), for the mixed-precision form:
