@@ -23,7 +23,7 @@ import numpy as np
 from . import _lib
 from ._lib import check, ptr, stream_handle
 
-LU_BLOCKS = (64, 128)
+LU_BLOCKS = (128, 256)      # 64 loses at every measured size above 1000 rows
 PANEL_LEAVES = (8, 16)
 LOOKAHEAD = (0, 1, 2)        # in order, side stream, CUs reserved for the panels
 PANEL_CUS = (16, 32)         # reserved CUs tried with LOOKAHEAD = 2

@@ -53,7 +53,7 @@ int launch_assembly_worklist(const AssemblyArgs* bodies, const AsmTile* tiles, i
 struct LuSetup {
     hipStream_t side = nullptr;
     hipEvent_t ready = nullptr, done = nullptr;
-    int block = 0;      // block width: 64 or 128, 0 = by size (128 from 1024 rows)
+    int block = 0;      // block width: 64, 128 or 256, 0 = by size (256 from 2048 rows, 128 from 1024)
     int leaf = 16;      // panel columns factored by one work-group per matrix
     int lookahead = 1;  // 1: next panel on the side stream; 2: on CUs reserved for it
     // CU-partitioned look-ahead: panels on `part_side` (panel_cus CUs), the

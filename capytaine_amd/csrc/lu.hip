@@ -13,8 +13,8 @@
 // on v_mfma_f64_16x16x4f64, R = float gives complex64 on
 // v_mfma_f32_16x16x4f32.
 //
-// Right-looking blocked algorithm (block width 128, or 64 below 1024
-// rows), one launch per step for the whole batch:
+// Right-looking blocked algorithm (block width 256 from 2048 rows, 128 from
+// 1024, else 64; tunable), one launch per step for the whole batch:
 //   1. panel: recursive halving down to 16-column leaves factored by one
 //      work-group per matrix, the halves joined by trsm + MFMA GEMM;
 //   2. row interchanges applied to the columns outside the panel;
@@ -126,10 +126,9 @@ __global__ void __launch_bounds__(NT) panel_kernel(cx<R>* __restrict__ Abase, in
 
     __shared__ C sL[PB * PB];  // L11, row-major [i][j]
     __shared__ C sU[PB * SB];
-    __shared__ C sRow[SB];
-    __shared__ C sSwap[2][SB];
-    __shared__ R sBest[NW];
-    __shared__ int sIdx[NW];
+    __shared__ C sSwap[2][2][SB];
+    __shared__ R sBest[2][NW];
+    __shared__ int sIdx[2][NW];
     __shared__ int sPiv[SB];
 
     C v[RPT][SB];
@@ -194,11 +193,14 @@ __global__ void __launch_bounds__(NT) panel_kernel(cx<R>* __restrict__ Abase, in
                 }
             }
         }
-        // (d)
+        // (d) two barriers per column: the per-wave maxima, then the two
+        // interchanged rows, each in its own half of a double buffer so the
+        // next column can start while a slow wave still reads this one
 #pragma unroll
         for (int s = 0; s < SB; ++s) {
             if (s < sbw) {
                 const int td = t0 + s;
+                const int buf = s & 1;
                 R best = R(-1);
                 int bidx = INT_MAX;
 #pragma unroll
@@ -221,58 +223,52 @@ __global__ void __launch_bounds__(NT) panel_kernel(cx<R>* __restrict__ Abase, in
                     }
                 }
                 if (lane == 0) {
-                    sBest[wave] = best;
-                    sIdx[wave] = bidx;
+                    sBest[buf][wave] = best;
+                    sIdx[buf][wave] = bidx;
                 }
                 __syncthreads();
+                // every thread combines the wave maxima (same order, same result)
+                R b = sBest[buf][0];
+                int p = sIdx[buf][0];
+#pragma unroll
+                for (int w = 1; w < NW; ++w) {
+                    const R bw = sBest[buf][w];
+                    const int iw = sIdx[buf][w];
+                    if (bw > b || (bw == b && iw < p)) {
+                        b = bw;
+                        p = iw;
+                    }
+                }
+                if (!(b > R(0))) p = td;  // zero column: no interchange, report singularity
                 if (tid == 0) {
-                    R b = sBest[0];
-                    int bi = sIdx[0];
-                    for (int w = 1; w < NW; ++w)
-                        if (sBest[w] > b || (sBest[w] == b && sIdx[w] < bi)) {
-                            b = sBest[w];
-                            bi = sIdx[w];
-                        }
-                    int p = bi;
-                    if (!(b > R(0))) {  // zero column: no interchange, report singularity
-                        p = td;
-                        if (info_base[blockIdx.x] == 0) info_base[blockIdx.x] = c0 + s + 1;
-                    }
-                    sPiv[s] = p;
+                    if (!(b > R(0)) && info_base[blockIdx.x] == 0) info_base[blockIdx.x] = c0 + s + 1;
                     ipiv[c0 + s] = k0 + p;
+                    sPiv[s] = p;
+                }
+                // rows td and p to LDS (slot 0: row td, slot 1: row p)
+#pragma unroll
+                for (int q = 0; q < RPT; ++q) {
+                    const int t = tid + NT * q;
+                    if (t == td)
+#pragma unroll
+                        for (int s2 = 0; s2 < SB; ++s2) sSwap[buf][0][s2] = v[q][s2];
+                    if (t == p)
+#pragma unroll
+                        for (int s2 = 0; s2 < SB; ++s2) sSwap[buf][1][s2] = v[q][s2];
                 }
                 __syncthreads();
-                const int p = sPiv[s];
-                if (p != td) {
 #pragma unroll
-                    for (int q = 0; q < RPT; ++q) {
-                        const int t = tid + NT * q;
-                        if (t == td)
+                for (int q = 0; q < RPT; ++q) {
+                    const int t = tid + NT * q;
+                    if (p != td && t == td)
 #pragma unroll
-                            for (int s2 = 0; s2 < SB; ++s2) sSwap[0][s2] = v[q][s2];
-                        if (t == p)
+                        for (int s2 = 0; s2 < SB; ++s2) v[q][s2] = sSwap[buf][1][s2];
+                    if (p != td && t == p)
 #pragma unroll
-                            for (int s2 = 0; s2 < SB; ++s2) sSwap[1][s2] = v[q][s2];
-                    }
-                    __syncthreads();
-#pragma unroll
-                    for (int q = 0; q < RPT; ++q) {
-                        const int t = tid + NT * q;
-                        if (t == td)
-#pragma unroll
-                            for (int s2 = 0; s2 < SB; ++s2) v[q][s2] = sSwap[1][s2];
-                        if (t == p)
-#pragma unroll
-                            for (int s2 = 0; s2 < SB; ++s2) v[q][s2] = sSwap[0][s2];
-                    }
+                        for (int s2 = 0; s2 < SB; ++s2) v[q][s2] = sSwap[buf][0][s2];
                 }
-#pragma unroll
-                for (int q = 0; q < RPT; ++q)
-                    if (tid + NT * q == td)
-#pragma unroll
-                        for (int s2 = 0; s2 < SB; ++s2) sRow[s2] = v[q][s2];
-                __syncthreads();
-                const C piv = sRow[s];
+                // the pivot row is the new row td: slot 1
+                const C piv = sSwap[buf][1][s];
                 const bool nonzero = (piv.re != R(0) || piv.im != R(0));
                 const C r = nonzero ? zrcp(piv) : C{R(1), R(0)};
 #pragma unroll
@@ -282,12 +278,12 @@ __global__ void __launch_bounds__(NT) panel_kernel(cx<R>* __restrict__ Abase, in
                         const C l = nonzero ? zmul(v[q][s], r) : v[q][s];
                         v[q][s] = l;
 #pragma unroll
-                        for (int s2 = s + 1; s2 < SB; ++s2) v[q][s2] = zsub(v[q][s2], zmul(l, sRow[s2]));
+                        for (int s2 = s + 1; s2 < SB; ++s2) v[q][s2] = zsub(v[q][s2], zmul(l, sSwap[buf][1][s2]));
                     }
                 }
-                __syncthreads();
             }
         }
+        __syncthreads();  // sPiv of the sub-block complete
         // (e)
 #pragma unroll
         for (int q = 0; q < RPT; ++q) {
@@ -297,18 +293,42 @@ __global__ void __launch_bounds__(NT) panel_kernel(cx<R>* __restrict__ Abase, in
                 for (int s = 0; s < SB; ++s)
                     if (s < sbw) A[(k0 + t) + (size_t)(c0 + s) * lda] = v[q][s];
         }
-        for (int col = sw_lo + tid; col < sw_hi; col += NT) {
-            if (col < c0 || col >= c0 + sbw) {
-                C* a = A + k0 + (size_t)col * lda;
-                for (int s = 0; s < sbw; ++s) {
-                    const int td = t0 + s, p = sPiv[s];
-                    if (p != td) {
-                        const C x = a[td], y = a[p];
-                        a[td] = y;
-                        a[p] = x;
+        // the sub-block's interchanges on the other columns [sw_lo, sw_hi)
+        // (at most 2 PB = 128 of them): one thread per (column, involved
+        // row) pair finds, walking the interchanges backwards, which
+        // original row ends in that row, loads it, and after a barrier
+        // stores it -- one round trip for all the interchanges
+        {
+            constexpr int PPT = (2 * SB * 2 * PB + NT - 1) / NT;  // pairs per thread
+            const int nslot = 2 * sbw;
+            const int npairs = (sw_hi - sw_lo) * nslot;
+            C val[PPT];
+            int dst[PPT];
+            C* colp[PPT];
+#pragma unroll
+            for (int u = 0; u < PPT; ++u) {
+                const int pair = tid + NT * u;
+                colp[u] = nullptr;
+                dst[u] = 0;
+                if (pair < npairs) {
+                    const int col = sw_lo + pair / nslot, j = pair % nslot;
+                    if (col < c0 || col >= c0 + sbw) {
+                        const int r = (j & 1) ? sPiv[j >> 1] : t0 + (j >> 1);
+                        int x = r;
+                        for (int s = sbw - 1; s >= 0; --s) {
+                            if (x == t0 + s) x = sPiv[s];
+                            else if (x == sPiv[s]) x = t0 + s;
+                        }
+                        colp[u] = A + k0 + (size_t)col * lda;
+                        dst[u] = r;
+                        val[u] = colp[u][x];
                     }
                 }
             }
+            __syncthreads();
+#pragma unroll
+            for (int u = 0; u < PPT; ++u)
+                if (colp[u]) colp[u][dst[u]] = val[u];
         }
         __syncthreads();
     }
@@ -774,6 +794,10 @@ template <typename R>
 int gemm(int M, int N, int K, R alpha, const cx<R>* A, int64_t lda, int64_t sA, const cx<R>* B, int64_t ldb,
          int64_t sB, cx<R>* C, int64_t ldc, int64_t sC, int beta_zero, int batch, hipStream_t stream);
 
+template <typename R>
+int trsm_lower(const cx<R>* L, int64_t lda, int64_t sA, int kb, cx<R>* B, int64_t ldb, int64_t sB, int ncols,
+               int batch, hipStream_t stream);
+
 // Recursive factorisation of columns [c0, c0 + w) of rows [c0, N) inside the
 // panel [sw_lo, sw_hi): factor the left half, solve its unit-lower triangle
 // against the top of the right half (trsm), update the rest of the right
@@ -784,7 +808,8 @@ int panel_rec(cx<R>* A, int64_t lda, int64_t sA, int N, int c0, int w, int sw_lo
               int* info, int batch, int leaf, hipStream_t stream)
 {
     const int M = N - c0;
-    if (w <= leaf || M <= 2 * leaf)
+    // the leaf kernel holds at most PB columns (its LDS triangle)
+    if (w <= PB && (w <= leaf || M <= 2 * leaf))
         return panel_leaf<R>(A, lda, sA, N, c0, w, sw_lo, sw_hi, ipiv, sP, info, batch, stream);
     const int h = w / 2;
     int e;
@@ -793,7 +818,7 @@ int panel_rec(cx<R>* A, int64_t lda, int64_t sA, int N, int c0, int w, int sw_lo
     cx<R>* A12 = A + c0 + (size_t)(c0 + h) * lda;
     cx<R>* A21 = A + (c0 + h) + (size_t)c0 * lda;
     cx<R>* A22 = A + (c0 + h) + (size_t)(c0 + h) * lda;
-    if ((e = launch_trsm<R, false>(A11, lda, sA, h, A12, lda, sA, w - h, batch, stream))) return e;
+    if ((e = trsm_lower<R>(A11, lda, sA, h, A12, lda, sA, w - h, batch, stream))) return e;
     if (M > h && (e = gemm<R>(M - h, w - h, h, R(-1), A21, lda, sA, A12, lda, sA, A22, lda, sA, 0, batch, stream)))
         return e;
     return panel_rec<R>(A, lda, sA, N, c0 + h, w - h, sw_lo, sw_hi, ipiv, sP, info, batch, leaf, stream);
@@ -851,24 +876,31 @@ int gemm(int M, int N, int K, R alpha, const cx<R>* A, int64_t lda, int64_t sA, 
     return (int)hipGetLastError();
 }
 
-// L^-1 B in place for a unit lower kb x kb block L, kb <= 2 PB: one or two
-// wavefront-wide triangles with the product between them.
+// L^-1 B in place for a unit lower kb x kb block L (any kb): blocked
+// forward substitution over PB-row diagonal blocks, each solved by the
+// wavefront-wide triangle kernel, the blocks below updated by the product.
 template <typename R>
 int trsm_lower(const cx<R>* L, int64_t lda, int64_t sA, int kb, cx<R>* B, int64_t ldb, int64_t sB, int ncols,
                int batch, hipStream_t stream)
 {
-    const int k1 = min(kb, PB);
     int e;
-    if ((e = launch_trsm<R, false>(L, lda, sA, k1, B, ldb, sB, ncols, batch, stream))) return e;
-    if (kb == k1) return 0;
-    if ((e = gemm<R>(kb - k1, ncols, k1, R(-1), L + k1, lda, sA, B, ldb, sB, B + k1, ldb, sB, 0, batch, stream)))
-        return e;
-    return launch_trsm<R, false>(L + k1 + (size_t)k1 * lda, lda, sA, kb - k1, B + k1, ldb, sB, ncols, batch, stream);
+    for (int j0 = 0; j0 < kb; j0 += PB) {
+        const int jb = min(PB, kb - j0);
+        if ((e = launch_trsm<R, false>(L + j0 + (size_t)j0 * lda, lda, sA, jb, B + j0, ldb, sB, ncols, batch,
+                                       stream)))
+            return e;
+        const int rest = kb - j0 - jb;
+        if (rest > 0 && (e = gemm<R>(rest, ncols, jb, R(-1), L + (j0 + jb) + (size_t)j0 * lda, lda, sA, B + j0, ldb,
+                                     sB, B + j0 + jb, ldb, sB, 0, batch, stream)))
+            return e;
+    }
+    return 0;
 }
 
 // Block width of the right-looking factorisation: 128 columns halve the
-// trailing-matrix traffic per flop against 64 where the matrices are large.
-inline int getrf_block(int N, int tuned) { return tuned ? tuned : (N >= 1024 ? 2 * PB : PB); }
+// trailing-matrix traffic per flop against 64 where the matrices are large
+// (256 halves it again, at the price of wider panels: a tuning choice).
+inline int getrf_block(int N, int tuned) { return tuned ? tuned : (N >= 2048 ? 4 * PB : N >= 1024 ? 2 * PB : PB); }
 
 template <typename R>
 size_t getrf_ws_elems(int, int)

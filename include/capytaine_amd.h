@@ -276,7 +276,8 @@ int cpt_release_workspace(cpt_handle h);
  * on them only through rounding: every value factors with LAPACK's partial
  * pivoting rule. */
 enum {
-    CPT_TUNE_LU_BLOCK = 0,       /* block width of the LU: 64, 128, or 0 = 128 from 1024 rows */
+    CPT_TUNE_LU_BLOCK = 0,       /* block width of the LU: 64, 128, 256, or 0 = by size (256 from
+                                    2048 rows, 128 from 1024, else 64) */
     CPT_TUNE_PANEL_LEAF = 1,     /* panel columns factored by one work-group per matrix, 1..64 */
     CPT_TUNE_LOOKAHEAD = 2,      /* 0: in order; 1: next panel on a side stream; 2: next panel on
                                     CUs reserved for it (streams with CU masks; runs as 1 when the

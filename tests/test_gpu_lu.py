@@ -228,7 +228,8 @@ def test_kernel_timing():
 
 
 @pytest.mark.parametrize("block,leaf,lookahead,tiles", [(64, 8, 0, 1), (64, 32, 1, 0), (128, 16, 1, 1),
-                                                        (128, 64, 0, 0), (128, 16, 2, 1), (64, 8, 2, 0)])
+                                                        (128, 64, 0, 0), (128, 16, 2, 1), (64, 8, 2, 0),
+                                                        (256, 16, 0, 0), (256, 8, 1, 0)])
 def test_lu_tunings_agree(block, leaf, lookahead, tiles):
     """Every tuning of the blocked LU solves to LAPACK's accuracy."""
     from scipy.linalg import lu_factor, lu_solve
@@ -258,9 +259,31 @@ def test_lu_tunings_agree(block, leaf, lookahead, tiles):
 def test_tuning_rejects_bad_values():
     from capytaine_amd import _lib
     ctx = _lib.Context.get(device())
-    for name, bad in [("lu_block", 96), ("panel_leaf", 0), ("panel_leaf", 65), ("lookahead", 3),
+    for name, bad in [("lu_block", 96), ("lu_block", 512), ("panel_leaf", 0), ("panel_leaf", 65), ("lookahead", 3),
                       ("assembly_chunk", -1), ("panel_cus", 0), ("panel_cus", 129), ("gemm_tiles", 2)]:
         with pytest.raises(_lib.CptError):
             _lib.set_tuning(ctx, **{name: bad})
     with pytest.raises(_lib.CptError):
         _lib.check(_lib.load().cpt_set_tuning(ctx.handle, 99, 0), "cpt_set_tuning")
+
+
+@pytest.mark.parametrize("n,block,leaf", [(640, 128, 64), (768, 256, 64), (300, 256, 32)])
+def test_lu_last_panel_wider_than_leaf_kernel(n, block, leaf):
+    """A last panel of 128 or more rows with a 64-column leaf setting is
+    split before it reaches the one-work-group kernel (at most 64 columns):
+    regression for the bound of the recursive panel."""
+    from scipy.linalg import lu_factor, lu_solve
+    from capytaine_amd import _lib
+    print(f"[gpu] LU last panel n={n} block={block} leaf={leaf}", flush=True)
+    ctx = _lib.Context.get(device())
+    saved = _lib.get_tuning(ctx)
+    try:
+        _lib.set_tuning(ctx, lu_block=block, panel_leaf=leaf, lookahead=0)
+        rng = np.random.default_rng(n)
+        A, B = _random_system(rng, n, 3)
+        X, info, _, _ = _factor_solve(A[None], B[None])
+        assert info[0] == 0
+        Xr = lu_solve(lu_factor(A), B)
+        assert np.linalg.norm(X[0] - Xr) / np.linalg.norm(Xr) < 1e-10
+    finally:
+        _lib.set_tuning(ctx, **saved)
