@@ -17,6 +17,11 @@ difference of any coefficient between the two forms are reported beside it.
 Prints one JSON line (see the repository's bench contract).
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline] [--no-fleet] [--no-serving]
     python bench.py --workload cylinders     # 20000-panel finite-depth array instead
+    python bench.py --workload serving       # the request-stream measurement alone
+
+The request-stream measurement runs in a child process (started after the
+headline and fleet legs have released their device memory), so that a
+failure there is reported in the line instead of ending the run.
 """
 import argparse
 import json
@@ -245,6 +250,22 @@ def serving_benchmark(gf, dev, seed=1, requests=24, loads=(0.5, 0.8, 1.0, 1.2), 
     return out
 
 
+def serving_child(timeout_s=900):
+    """serving_benchmark in a child process (same interpreter, this file
+    with --workload serving): its JSON line, or the exit status when the
+    child ends without one.  Its progress lines go to this process's stderr."""
+    import subprocess
+    cmd = [sys.executable, "-u", os.path.abspath(__file__), "--workload", "serving"]
+    try:
+        p = subprocess.run(cmd, stdout=subprocess.PIPE, text=True, timeout=timeout_s)
+    except subprocess.TimeoutExpired:
+        return {"error": f"request-stream measurement exceeded {timeout_s} s"}
+    lines = [ln for ln in p.stdout.splitlines() if ln.startswith("{")]
+    if p.returncode != 0 or not lines:
+        return {"error": f"request-stream measurement exited with status {p.returncode}"}
+    return json.loads(lines[-1])
+
+
 def copy_calibration(nbytes=1 << 30, reps=4):
     """Element-wise copy of `nbytes` (float32, y = 1 * x) `reps` times: the
     HBM counters of this kernel (FETCH_SIZE, WRITE_SIZE) against the known
@@ -341,10 +362,10 @@ def main():
     ap.add_argument("--no-serving", action="store_true", help="skip the request-stream latency measurement")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner (a stored tuning of this size class is still applied)")
-    ap.add_argument("--workload", choices=("barge", "cylinders", "copy"), default="barge",
+    ap.add_argument("--workload", choices=("barge", "cylinders", "copy", "serving"), default="barge",
                     help="barge: the headline line (plus fleet and serving); cylinders: the 20000-panel "
                          "finite-depth array; copy: a 1 GiB device copy, the calibration of the HBM "
-                         "byte counters")
+                         "byte counters; serving: the request-stream measurement alone")
     ap.add_argument("--only", choices=("fp64", "mixed"), default=None,
                     help="run one form of the barge sweep only (for counter passes); prints no JSON line")
     args = ap.parse_args()
@@ -357,6 +378,14 @@ def main():
         return
     if args.workload == "copy":
         copy_calibration()
+        return
+    if args.workload == "serving":
+        import torch
+        import capytaine_amd as cpa
+        dev = torch.device("cuda", 0)
+        torch.cuda.set_device(dev)
+        res = serving_benchmark(cpa.Delhommeau(device=dev), dev, log=lambda m: print(m, file=sys.stderr, flush=True))
+        print(json.dumps(res), flush=True)
         return
 
     import torch
@@ -534,8 +563,12 @@ def main():
     if not args.no_fleet:
         out["fleet"] = fleet_benchmark(gf, dev, log=lambda m: print(m, file=sys.stderr, flush=True))
     if not args.no_serving:
-        out["serving"] = serving_benchmark(gf, dev, log=lambda m: print(m, file=sys.stderr, flush=True))
-        out["p99_latency_s_at_saturation"] = next(r["p99_s"] for r in out["serving"]["loads"] if r["load"] == 1.0)
+        _lib.check(_lib.load().cpt_release_workspace(ctx.handle), "cpt_release_workspace")
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+        out["serving"] = serving_child()
+        loads = out["serving"].get("loads", [])
+        out["p99_latency_s_at_saturation"] = next((r["p99_s"] for r in loads if r["load"] == 1.0), None)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(mesh, body, pbs_by_freq,
                                            (gf.tabulated_r_range, gf.tabulated_z_range, gf.tabulated_integrals))

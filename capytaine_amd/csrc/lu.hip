@@ -293,42 +293,46 @@ __global__ void __launch_bounds__(NT) panel_kernel(cx<R>* __restrict__ Abase, in
                 for (int s = 0; s < SB; ++s)
                     if (s < sbw) A[(k0 + t) + (size_t)(c0 + s) * lda] = v[q][s];
         }
-        // the sub-block's interchanges on the other columns [sw_lo, sw_hi)
-        // (at most 2 PB = 128 of them): one thread per (column, involved
-        // row) pair finds, walking the interchanges backwards, which
-        // original row ends in that row, loads it, and after a barrier
-        // stores it -- one round trip for all the interchanges
+        // the sub-block's interchanges on the other columns [sw_lo, sw_hi):
+        // one thread per (column, involved row) pair finds, walking the
+        // interchanges backwards, which original row ends in that row, loads
+        // it, and after a barrier stores it -- one round trip per chunk of
+        // whole columns (columns are independent, so chunks need no order)
         {
-            constexpr int PPT = (2 * SB * 2 * PB + NT - 1) / NT;  // pairs per thread
+            constexpr int PPT = 4;  // pairs per thread and chunk
             const int nslot = 2 * sbw;
-            const int npairs = (sw_hi - sw_lo) * nslot;
-            C val[PPT];
-            int dst[PPT];
-            C* colp[PPT];
+            const int ncol = sw_hi - sw_lo;
+            const int cols_per_chunk = (NT * PPT) / nslot;
+            for (int cb = 0; cb < ncol; cb += cols_per_chunk) {
+                const int npairs = min(cols_per_chunk, ncol - cb) * nslot;
+                C val[PPT];
+                int dst[PPT];
+                C* colp[PPT];
 #pragma unroll
-            for (int u = 0; u < PPT; ++u) {
-                const int pair = tid + NT * u;
-                colp[u] = nullptr;
-                dst[u] = 0;
-                if (pair < npairs) {
-                    const int col = sw_lo + pair / nslot, j = pair % nslot;
-                    if (col < c0 || col >= c0 + sbw) {
-                        const int r = (j & 1) ? sPiv[j >> 1] : t0 + (j >> 1);
-                        int x = r;
-                        for (int s = sbw - 1; s >= 0; --s) {
-                            if (x == t0 + s) x = sPiv[s];
-                            else if (x == sPiv[s]) x = t0 + s;
+                for (int u = 0; u < PPT; ++u) {
+                    const int pair = tid + NT * u;
+                    colp[u] = nullptr;
+                    dst[u] = 0;
+                    if (pair < npairs) {
+                        const int col = sw_lo + cb + pair / nslot, j = pair % nslot;
+                        if (col < c0 || col >= c0 + sbw) {
+                            const int r = (j & 1) ? sPiv[j >> 1] : t0 + (j >> 1);
+                            int x = r;
+                            for (int s = sbw - 1; s >= 0; --s) {
+                                if (x == t0 + s) x = sPiv[s];
+                                else if (x == sPiv[s]) x = t0 + s;
+                            }
+                            colp[u] = A + k0 + (size_t)col * lda;
+                            dst[u] = r;
+                            val[u] = colp[u][x];
                         }
-                        colp[u] = A + k0 + (size_t)col * lda;
-                        dst[u] = r;
-                        val[u] = colp[u][x];
                     }
                 }
-            }
-            __syncthreads();
+                __syncthreads();
 #pragma unroll
-            for (int u = 0; u < PPT; ++u)
-                if (colp[u]) colp[u][dst[u]] = val[u];
+                for (int u = 0; u < PPT; ++u)
+                    if (colp[u]) colp[u][dst[u]] = val[u];
+            }
         }
         __syncthreads();
     }

@@ -256,6 +256,40 @@ def test_lu_tunings_agree(block, leaf, lookahead, tiles):
         _lib.set_tuning(ctx, **saved)
 
 
+@pytest.mark.parametrize("block,leaf", [(256, 16), (256, 8), (128, 8), (64, 16)])
+def test_lu_pivoting_every_block_width(block, leaf):
+    """Matrices with no diagonal weight (a row interchange at nearly every
+    column, rows from every part of the panel): the interchanges of each
+    leaf reach all the panel's other columns, 256 of them at block width
+    256; the pivot sequence equals LAPACK's.  Regression for a panel
+    kernel that applied a leaf's interchanges to its first 128 columns
+    only."""
+    from scipy.linalg import lu_factor, lu_solve
+    from capytaine_amd import _lib
+    print(f"[gpu] LU pivoting block={block} leaf={leaf}", flush=True)
+    ctx = _lib.Context.get(device())
+    saved = _lib.get_tuning(ctx)
+    try:
+        _lib.set_tuning(ctx, lu_block=block, panel_leaf=leaf, lookahead=0)
+        rng = np.random.default_rng(block * 7 + leaf)
+        n = 700
+        As, Bs = zip(*[_random_system(rng, n, 4, diag=0.0) for _ in range(2)])
+        As, Bs = np.array(As), np.array(Bs)
+        X, info, ipiv, _ = _factor_solve(As, Bs)
+        assert not info.any()
+        Xm, _, res = _mixed(As, Bs)
+        assert res <= 1e-12
+        for b in range(2):
+            lu, piv = lu_factor(As[b])
+            assert np.mean(piv != np.arange(n)) > 0.9  # the case pivots
+            assert np.mean(ipiv[b] == piv) > 0.99
+            Xr = lu_solve((lu, piv), Bs[b])
+            assert np.linalg.norm(X[b] - Xr) / np.linalg.norm(Xr) < 1e-10
+            assert np.linalg.norm(Xm[b] - Xr) / np.linalg.norm(Xr) < 1e-10
+    finally:
+        _lib.set_tuning(ctx, **saved)
+
+
 def test_tuning_rejects_bad_values():
     from capytaine_amd import _lib
     ctx = _lib.Context.get(device())
