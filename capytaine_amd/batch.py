@@ -159,7 +159,8 @@ class BatchedSweep:
             g = self._groups(bcs)
             call_with_memory_retry(lambda: lib.cpt_lu_factor_grouped(self.ctx.handle, nbody, g, self.stream),
                                    "cpt_lu_factor_grouped")
-            check(lib.cpt_lu_solve_grouped(self.ctx.handle, nbody, g, self.stream), "cpt_lu_solve_grouped")
+            call_with_memory_retry(lambda: lib.cpt_lu_solve_grouped(self.ctx.handle, nbody, g, self.stream),
+                                   "cpt_lu_solve_grouped")
             return
         import torch
         g = self._groups(bcs)

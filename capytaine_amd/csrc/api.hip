@@ -28,12 +28,13 @@ size_t mixed_grouped_bytes(const MixedGroup* g, int ng);
 int launch_mixed_grouped(const MixedGroup* g, int ng, void* work, const LuSetup* tune, int max_iter, double tol,
                          int* iterations, double* residual);
 int launch_getrf_lanes(const LuGroupArgs* g, int ng, const LuSetup* tune);
-int launch_getrs_lanes(const LuGroupArgs* g, int ng);
+int launch_getrs_lanes(const LuGroupArgs* g, int ng, void* work);
+size_t getrs_workspace_bytes(int N, int batch);
 int launch_mixed_solve(const cplx* K, int n, int64_t lda, int64_t strideK, int batch, const cplx* B, int nrhs,
                        int64_t ldb, int64_t strideB, cplx* X, int64_t ldx, int64_t strideX, int max_iter, double tol,
                        void* work, const LuSetup* aux, int* iterations, double* residual, hipStream_t stream);
 int launch_getrs(const cplx* LU, int N, int64_t lda, int64_t strideA, const int* ipiv, int64_t strideP, cplx* B,
-                 int nrhs, int64_t ldb, int64_t strideB, int batch, hipStream_t stream);
+                 int nrhs, int64_t ldb, int64_t strideB, int batch, cplx* work, hipStream_t stream);
 
 static thread_local std::string g_last_error;
 void set_error(const std::string& msg) { g_last_error = msg; }
@@ -128,7 +129,7 @@ struct cpt_context {
     // per-call workspace
     cpt::DevBuf faces, quad, freq, prony;
     cpt::DevBuf bfaces, bquad, bfreq, bargs, btiles;  // multi-body assembly
-    cpt::DevBuf lu_work, mixed_work;
+    cpt::DevBuf lu_work, mixed_work, solve_work;  // solve_work: inverted diagonal blocks of the solves
     cpt::LuSetup lu_streams;
     int asm_chunk = 0;  // panels per assembly work-group, 0 = by size
     int part_cus = 0;   // CUs of the current partitioned look-ahead streams
@@ -421,6 +422,7 @@ int cpt_destroy(cpt_handle h)
     h->prony.release();
     h->lu_work.release();
     h->mixed_work.release();
+    h->solve_work.release();
     if (h->lu_streams.side) (void)hipStreamDestroy(h->lu_streams.side);
     if (h->lu_streams.ready) (void)hipEventDestroy(h->lu_streams.ready);
     if (h->lu_streams.done) (void)hipEventDestroy(h->lu_streams.done);
@@ -869,8 +871,12 @@ int cpt_lu_solve(cpt_handle h, const void* LU, int n, int64_t lda, int64_t strid
     }
     CPT_CHECK_HIP(hipSetDevice(h->device), "hipSetDevice");
     CallScope call_scope(&h->timer, h->gemm_tiles);
+    if (h->solve_work.reserve(getrs_workspace_bytes(n, batch))) {
+        set_error("cpt_lu_solve: workspace allocation failed");
+        return CPT_ERR_ALLOC;
+    }
     CPT_CHECK_HIP((hipError_t)launch_getrs((const cplx*)LU, n, lda, strideA, ipiv, strideP, (cplx*)B, nrhs, ldb,
-                                           strideB, batch, stream),
+                                           strideB, batch, (cplx*)h->solve_work.p, stream),
                   "getrs launch");
     return CPT_OK;
 }
@@ -1082,6 +1088,14 @@ int cpt_lu_solve_grouped(cpt_handle h, int ngroups, const cpt_lu_group* groups, 
     if (ngroups == 0) return CPT_OK;
     CPT_CHECK_HIP(hipSetDevice(h->device), "hipSetDevice");
     CallScope call_scope(&h->timer, h->gemm_tiles);
+    size_t ws = 0;
+    for (int i = 0; i < ngroups; ++i)
+        if (groups[i].n > 0 && groups[i].batch > 0 && groups[i].nrhs > 0)
+            ws += getrs_workspace_bytes(groups[i].n, groups[i].batch);
+    if (h->solve_work.reserve(ws)) {
+        set_error("cpt_lu_solve_grouped: workspace allocation failed");
+        return CPT_ERR_ALLOC;
+    }
     if (ensure_lanes(h) || fork_lanes(h, stream)) {
         set_error("cpt_lu_solve_grouped: could not set up the lane streams");
         return CPT_ERR_HIP;
@@ -1093,7 +1107,7 @@ int cpt_lu_solve_grouped(cpt_handle h, int ngroups, const cpt_lu_group* groups, 
         g[i] = LuGroupArgs{(cplx*)G.A, G.n, G.batch, G.lda, G.strideA, G.ipiv, G.strideP, G.info, (cplx*)G.B,
                            G.nrhs, G.ldb, G.strideB, lane[i]};
     }
-    const int e = launch_getrs_lanes(g.data(), ngroups);
+    const int e = launch_getrs_lanes(g.data(), ngroups, h->solve_work.p);
     if (join_lanes(h, stream)) {
         set_error("cpt_lu_solve_grouped: could not join the lane streams");
         return CPT_ERR_HIP;
@@ -1161,6 +1175,7 @@ int cpt_release_workspace(cpt_handle h)
     h->btiles.release();
     h->lu_work.release();
     h->mixed_work.release();
+    h->solve_work.release();
     return CPT_OK;
 }
 

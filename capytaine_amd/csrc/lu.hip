@@ -60,6 +60,16 @@ __device__ __forceinline__ cx<R> zrcp(cx<R> a)
         return cx<R>{r / d, R(-1) / d};
     }
 }
+// p[i] where ok, else zero, as a load on every lane (index 0 where !ok)
+// and a select of values: a select of addresses would turn the access into
+// a flat load between the buffer and a zero in private memory.
+template <typename R>
+__device__ __forceinline__ cx<R> ld_or_zero(const cx<R>* p, size_t i, bool ok)
+{
+    const cx<R> v = p[ok ? i : 0];
+    return cx<R>{ok ? v.re : R(0), ok ? v.im : R(0)};
+}
+
 template <typename R>
 __device__ __forceinline__ R cabs1(cx<R> a)
 {
@@ -162,7 +172,7 @@ __global__ void __launch_bounds__(NT) panel_kernel(cx<R>* __restrict__ Abase, in
             __syncthreads();
             if (wave < sbw) {
                 const int s = wave;
-                C x = (lane < t0) ? sU[lane * SB + s] : zero;
+                C x = ld_or_zero(sU, lane * SB + s, lane < t0);
                 for (int i = 1; i < t0; ++i) {
                     C part = zero;
                     if (lane < i) part = zmul(sL[i * PB + lane], x);
@@ -382,6 +392,69 @@ __device__ __forceinline__ float lane_bcast(float v, int j)
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
 }
 
+// The kb x kb triangle of T (column-major, leading dimension ldt) into LDS
+// (sT[i + PB j]), loads batched four per thread; for UPPER the diagonal is
+// replaced by its reciprocals.  Ends with a barrier.
+template <typename R, bool UPPER, int NT>
+__device__ __forceinline__ void stage_triangle(const cx<R>* __restrict__ T, int64_t ldt, int kb, cx<R>* sT)
+{
+    typedef cx<R> C;
+    const C zero{R(0), R(0)};
+    const int tot = kb * kb;
+    for (int e0 = threadIdx.x; e0 < tot; e0 += 4 * NT) {
+        C v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int e = e0 + u * NT;
+            const int i = e % kb, j = e / kb;
+            v[u] = ld_or_zero(T, i + (size_t)j * ldt, e < tot && (UPPER ? (i <= j) : (i > j)));
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int e = e0 + u * NT;
+            if (e < tot) sT[(e % kb) + PB * (e / kb)] = v[u];
+        }
+    }
+    __syncthreads();
+    if (UPPER) {
+        for (int i = threadIdx.x; i < kb; i += NT) sT[i + PB * i] = zrcp(sT[i + PB * i]);
+        __syncthreads();
+    }
+}
+
+// x <- T^-1 x for TRSM_ILP columns held by one wavefront (lane i holds row
+// i; rows >= kb are zero and stay zero); T from stage_triangle.
+template <typename R, bool UPPER>
+__device__ __forceinline__ void solve_triangle(const cx<R>* sT, int kb, int lane, cx<R> (&x)[TRSM_ILP])
+{
+    typedef cx<R> C;
+    if (!UPPER) {
+        for (int j = 0; j < kb - 1; ++j) {
+            const C t = sT[lane + PB * j];
+            const bool below = lane > j && lane < kb;
+#pragma unroll
+            for (int q = 0; q < TRSM_ILP; ++q) {
+                const C y{lane_bcast(x[q].re, j), lane_bcast(x[q].im, j)};
+                if (below) x[q] = zsub(x[q], zmul(t, y));
+            }
+        }
+    } else {
+        for (int j = kb - 1; j >= 0; --j) {
+            const C d = sT[j + PB * j];  // reciprocal of the diagonal
+#pragma unroll
+            for (int q = 0; q < TRSM_ILP; ++q)
+                if (lane == j) x[q] = zmul(x[q], d);
+            const C t = sT[lane + PB * j];
+            const bool above = lane < j;
+#pragma unroll
+            for (int q = 0; q < TRSM_ILP; ++q) {
+                const C y{lane_bcast(x[q].re, j), lane_bcast(x[q].im, j)};
+                if (above) x[q] = zsub(x[q], zmul(t, y));
+            }
+        }
+    }
+}
+
 template <typename R, bool UPPER>
 __global__ void __launch_bounds__(64 * TRSM_WAVES) trsm_kernel(const cx<R>* __restrict__ Tbase, int64_t ldt,
                                                                int64_t strideT, int kb, cx<R>* __restrict__ Bbase,
@@ -389,29 +462,8 @@ __global__ void __launch_bounds__(64 * TRSM_WAVES) trsm_kernel(const cx<R>* __re
 {
     typedef cx<R> C;
     __shared__ C sT[PB * PB];
-    __shared__ C sD[PB];
     const C zero{R(0), R(0)};
-    const C* T = Tbase + (size_t)blockIdx.y * strideT;
-    const int tot = kb * kb;
-    for (int e0 = threadIdx.x; e0 < tot; e0 += 4 * 64 * TRSM_WAVES) {
-        C v[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int e = e0 + u * 64 * TRSM_WAVES;
-            const int i = e % kb, j = e / kb;
-            v[u] = (e < tot && (UPPER ? (i <= j) : (i > j))) ? T[i + (size_t)j * ldt] : zero;
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int e = e0 + u * 64 * TRSM_WAVES;
-            if (e < tot) sT[(e % kb) + PB * (e / kb)] = v[u];
-        }
-    }
-    __syncthreads();
-    if (UPPER) {
-        for (int i = threadIdx.x; i < kb; i += blockDim.x) sD[i] = zrcp(sT[i + PB * i]);
-        __syncthreads();
-    }
+    stage_triangle<R, UPPER, 64 * TRSM_WAVES>(Tbase + (size_t)blockIdx.y * strideT, ldt, kb, sT);
     const int lane = threadIdx.x & 63;
     const int cbase = (blockIdx.x * TRSM_WAVES + (threadIdx.x >> 6)) * TRSM_CPW;
     C* Bm = Bbase + (size_t)blockIdx.y * strideB;
@@ -421,37 +473,207 @@ __global__ void __launch_bounds__(64 * TRSM_WAVES) trsm_kernel(const cx<R>* __re
         C x[TRSM_ILP];
 #pragma unroll
         for (int q = 0; q < TRSM_ILP; ++q)
-            x[q] = (col0 + q < ncols && lane < kb) ? Bm[lane + (size_t)(col0 + q) * ldb] : zero;
-        if (!UPPER) {
-            for (int j = 0; j < kb - 1; ++j) {
-                const C t = sT[lane + PB * j];
-                const bool below = lane > j && lane < kb;
-#pragma unroll
-                for (int q = 0; q < TRSM_ILP; ++q) {
-                    const C y{lane_bcast(x[q].re, j), lane_bcast(x[q].im, j)};
-                    if (below) x[q] = zsub(x[q], zmul(t, y));
-                }
-            }
-        } else {
-            for (int j = kb - 1; j >= 0; --j) {
-                const C d = sD[j];
-#pragma unroll
-                for (int q = 0; q < TRSM_ILP; ++q)
-                    if (lane == j) x[q] = zmul(x[q], d);
-                const C t = sT[lane + PB * j];
-                const bool above = lane < j;
-#pragma unroll
-                for (int q = 0; q < TRSM_ILP; ++q) {
-                    const C y{lane_bcast(x[q].re, j), lane_bcast(x[q].im, j)};
-                    if (above) x[q] = zsub(x[q], zmul(t, y));
-                }
-            }
-        }
+            x[q] = ld_or_zero(Bm, lane + (size_t)(col0 + q) * ldb, col0 + q < ncols && lane < kb);
+        solve_triangle<R, UPPER>(sT, kb, lane, x);
         if (lane < kb) {
 #pragma unroll
             for (int q = 0; q < TRSM_ILP; ++q)
                 if (col0 + q < ncols) Bm[lane + (size_t)(col0 + q) * ldb] = x[q];
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 3b. Inverses of the PB x PB diagonal blocks of an LU factor, for the
+//     substitution below: inv(L_kk) (unit lower, its strict lower part kept)
+//     and inv(U_kk) (upper, diagonal included) share one PB x PB block of D
+//     the way L and U share the factor, so D may be the factor itself (dstep
+//     = 1: block k at D(k0, k0)) or a strip of PB columns (dstep = 0: block k
+//     at D(k0, 0)).  One work-group per (block, matrix); the columns of the
+//     identity are solved TRSM_ILP at a time by the wavefront triangle solve.
+//     In place, each triangle is staged in LDS before any of its entries is
+//     overwritten, and the two triangles do not overlap.
+// ---------------------------------------------------------------------------
+template <typename R, bool UPPER>
+__device__ __forceinline__ void invert_triangle(const cx<R>* sT, int kb, int lane, int wave, cx<R>* D, int64_t ldd)
+{
+    typedef cx<R> C;
+    for (int g = 0; g < PB / (4 * TRSM_ILP); ++g) {
+        const int col0 = (4 * g + wave) * TRSM_ILP;
+        C x[TRSM_ILP];
+#pragma unroll
+        for (int q = 0; q < TRSM_ILP; ++q) x[q] = C{R(lane == col0 + q ? 1 : 0), R(0)};
+        solve_triangle<R, UPPER>(sT, kb, lane, x);
+#pragma unroll
+        for (int q = 0; q < TRSM_ILP; ++q) {
+            const int col = col0 + q;
+            if (col < kb && lane < kb && (UPPER ? lane <= col : lane > col)) D[lane + (size_t)col * ldd] = x[q];
+        }
+    }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) invert_diag_kernel(const cx<R>* __restrict__ LUbase, int64_t lda,
+                                                          int64_t strideA, int N, cx<R>* Dbase, int64_t ldd,
+                                                          int64_t strideD, int dstep)
+{
+    typedef cx<R> C;
+    __shared__ C sT[PB * PB];
+    const int k0 = blockIdx.x * PB, kb = min(PB, N - k0);
+    const C* T = LUbase + (size_t)blockIdx.y * strideA + k0 + (size_t)k0 * lda;
+    C* D = Dbase + (size_t)blockIdx.y * strideD + k0 + (size_t)(dstep * k0) * ldd;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    stage_triangle<R, false, 256>(T, lda, kb, sT);
+    invert_triangle<R, false>(sT, kb, lane, wave, D, ldd);
+    __syncthreads();  // every wavefront is done with L before U replaces it
+    stage_triangle<R, true, 256>(T, lda, kb, sT);
+    invert_triangle<R, true>(sT, kb, lane, wave, D, ldd);
+}
+
+// ---------------------------------------------------------------------------
+// 3c. Substitution of the solve (getrs) for a few right-hand sides, one
+//     launch per diagonal block: every work-group forms x_k = inv(T_kk) b_k
+//     for its right-hand sides (SUB_RC at a time) with MFMA -- wavefront w
+//     the rows 16 w .. +16 -- and, with UPDATE, subtracts T(rows, k) x_k
+//     from its slab of sub_slab rows of [row_lo, row_hi) on MFMA: the rows
+//     below the block for L, above it for U.  The block's own rows of B are
+//     not written by an UPDATE launch, so every work-group reads the same
+//     b_k; the last launch (UPDATE = false, block = blockIdx.x) writes x_k
+//     for every block in place.
+// ---------------------------------------------------------------------------
+constexpr int SUB_RC = 16;
+// 16-row tiles per wavefront of an UPDATE launch (all of their operands in
+// registers at once) and rows per work-group
+template <typename R>
+constexpr int sub_tiles() { return sizeof(R) == 8 ? 1 : 2; }
+template <typename R>
+constexpr int sub_slab() { return 64 * sub_tiles<R>(); }
+
+// x_k(rows 16 wave .., cols c0 + (lane & 15)) = inv(T_kk) b_k, accumulator
+// layout (row Mfma<R>::row(lane, v) of the tile); inverse entries outside
+// the triangle read as zero, the unit diagonal of inv(L) as one.
+template <typename R, bool UPPER>
+__device__ __forceinline__ void diag_product(const cx<R>* Dk, int64_t ldd, const cx<R>* Bk, int64_t ldb, int kb,
+                                             int c0, int nrhs, int wave, int lane, typename Mfma<R>::acc& xr,
+                                             typename Mfma<R>::acc& xi)
+{
+    typedef cx<R> C;
+    const int li = lane & 15, lk = lane >> 4;
+    const int i = 16 * wave + li, col = c0 + li;
+    xr = typename Mfma<R>::acc{0, 0, 0, 0};
+    xi = typename Mfma<R>::acc{0, 0, 0, 0};
+    // all loads of a batch in flight at once: the whole block for complex64,
+    // half of it for complex128 (register budget)
+    constexpr int DS = sizeof(R) == 8 ? PB / 8 : PB / 4;
+#pragma unroll 1
+    for (int s0 = 0; s0 < PB / 4; s0 += DS) {
+        C a[DS], b[DS];
+#pragma unroll
+        for (int s = 0; s < DS; ++s) {
+            const int k = 4 * (s0 + s) + lk;
+            const bool in = i < kb && k < kb;
+            a[s] = ld_or_zero(Dk, i + (size_t)k * ldd, in && (UPPER ? i <= k : i > k));
+            if (!UPPER && in && i == k) a[s] = C{R(1), R(0)};
+            b[s] = ld_or_zero(Bk, k + (size_t)col * ldb, k < kb && col < nrhs);
+        }
+#pragma unroll
+        for (int s = 0; s < DS; ++s) {
+            xr = Mfma<R>::op(a[s].re, b[s].re, xr);
+            xi = Mfma<R>::op(a[s].re, b[s].im, xi);
+            xr = Mfma<R>::op(-a[s].im, b[s].im, xr);
+            xi = Mfma<R>::op(a[s].im, b[s].re, xi);
+        }
+    }
+}
+
+template <typename R, bool UPPER, bool UPDATE>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+substitution_kernel(const cx<R>* __restrict__ LUbase, int64_t lda, int64_t strideA, int N, int kblk,
+                    const cx<R>* __restrict__ Dbase, int64_t ldd, int64_t strideD, int dstep, cx<R>* __restrict__ Bbase,
+                    int64_t ldb, int64_t strideB, int nrhs, int row_lo, int row_hi)
+{
+    typedef cx<R> C;
+    typedef typename Mfma<R>::acc acc;
+    __shared__ C sX[PB][SUB_RC];  // x_k(k, col)
+    const int k0 = (UPDATE ? kblk : (int)blockIdx.x) * PB;
+    const int kb = min(PB, N - k0);
+    const C* LU = LUbase + (size_t)blockIdx.y * strideA;
+    const C* Dk = Dbase + (size_t)blockIdx.y * strideD + k0 + (size_t)(dstep * k0) * ldd;
+    C* B = Bbase + (size_t)blockIdx.y * strideB;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    // UPDATE: wavefront w takes the rows wrow .. +16 TW of the slab as TW
+    // 16 x 16 tiles; all of its T(rows, k) operands (lane l: row
+    // 16 t + (l & 15), k 4 s + (l >> 4)) are loaded before x_k is formed, so
+    // that their latency overlaps that of the diagonal product
+    constexpr int TW = sub_tiles<R>();
+    const int wrow = row_lo + blockIdx.x * sub_slab<R>() + 16 * TW * wave;
+    const C* Tk = LU + (size_t)k0 * lda;
+    C ta[UPDATE ? PB / 4 : 1][TW];
+    if (UPDATE) {
+#pragma unroll
+        for (int s = 0; s < PB / 4; ++s)
+#pragma unroll
+            for (int t = 0; t < TW; ++t) {
+                const int k = 4 * s + lk, row = wrow + 16 * t + li;
+                ta[s][t] = ld_or_zero(Tk, row + (size_t)k * lda, k < kb && row < row_hi);
+            }
+    }
+    for (int c0 = 0; c0 < nrhs; c0 += SUB_RC) {
+        acc xr, xi;
+        diag_product<R, UPPER>(Dk, ldd, B + k0, ldb, kb, c0, nrhs, wave, lane, xr, xi);
+        if (!UPDATE) {
+            // every wavefront has consumed b_k (its MFMAs read it) before any
+            // row of x_k replaces it
+            __syncthreads();
+            if (c0 + li < nrhs) {
+                C* Bc = B + k0 + (size_t)(c0 + li) * ldb;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int r = 16 * wave + Mfma<R>::row(lane, v);
+                    if (r < kb) Bc[r] = C{xr[v], xi[v]};
+                }
+            }
+            continue;
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) sX[16 * wave + Mfma<R>::row(lane, v)][li] = C{xr[v], xi[v]};
+        __syncthreads();
+        // B(rows, cols) -= T(rows, k) x_k on MFMA (x_k(k 4 s + (l >> 4),
+        // col l & 15) from LDS)
+        acc cr[TW], ci[TW];
+#pragma unroll
+        for (int t = 0; t < TW; ++t) {
+            cr[t] = acc{0, 0, 0, 0};
+            ci[t] = acc{0, 0, 0, 0};
+        }
+#pragma unroll
+        for (int s = 0; s < PB / 4; ++s) {
+            const int k = 4 * s + lk;
+            const C xb = ld_or_zero(&sX[0][0], (size_t)k * SUB_RC + li, k < kb);
+#pragma unroll
+            for (int t = 0; t < TW; ++t) {
+                const C a = ta[s][t];
+                cr[t] = Mfma<R>::op(a.re, xb.re, cr[t]);
+                ci[t] = Mfma<R>::op(a.re, xb.im, ci[t]);
+                cr[t] = Mfma<R>::op(-a.im, xb.im, cr[t]);
+                ci[t] = Mfma<R>::op(a.im, xb.re, ci[t]);
+            }
+        }
+        if (c0 + li < nrhs) {
+            C* Bc = B + (size_t)(c0 + li) * ldb;
+#pragma unroll
+            for (int t = 0; t < TW; ++t)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int row = wrow + 16 * t + Mfma<R>::row(lane, v);
+                    if (row < row_hi) {
+                        Bc[row].re -= cr[t][v];
+                        Bc[row].im -= ci[t][v];
+                    }
+                }
+        }
+        __syncthreads();  // sX is rewritten by the next group of columns
     }
 }
 
@@ -991,39 +1213,51 @@ int getrf_steps(cx<R>* A, int N, int64_t lda, int64_t strideA, int* ipiv, int64_
     return 0;
 }
 
+// Inverted diagonal blocks of the factor LU into D (dstep = 1: D is the
+// factor itself, its diagonal blocks overwritten; 0: a strip of PB columns
+// per matrix, leading dimension ldd >= N).
 template <typename R>
-int getrs(const cx<R>* LU, int N, int64_t lda, int64_t strideA, const int* ipiv, int64_t strideP, cx<R>* B, int nrhs,
-          int64_t ldb, int64_t strideB, int batch, hipStream_t stream)
+int invert_diag(const cx<R>* LU, int N, int64_t lda, int64_t strideA, cx<R>* D, int64_t ldd, int64_t strideD,
+                int dstep, int batch, hipStream_t stream)
+{
+    if (N <= 0 || batch <= 0) return 0;
+    hipLaunchKernelGGL(invert_diag_kernel<R>, dim3((N + PB - 1) / PB, batch), dim3(256), 0, stream, LU, lda, strideA,
+                       N, D, ldd, strideD, dstep);
+    return (int)hipGetLastError();
+}
+
+// Solve with the factor (LAPACK zgetrs): interchanges of B, then the
+// substitutions with the inverted diagonal blocks D (from invert_diag).
+template <typename R>
+int getrs(const cx<R>* LU, int N, int64_t lda, int64_t strideA, const int* ipiv, int64_t strideP, const cx<R>* D,
+          int64_t ldd, int64_t strideD, int dstep, cx<R>* B, int nrhs, int64_t ldb, int64_t strideB, int batch,
+          hipStream_t stream)
 {
     if (N <= 0 || nrhs <= 0 || batch <= 0) return 0;
-    int e;
     hipLaunchKernelGGL(laswp_kernel<R>, dim3((nrhs + 255) / 256, batch), dim3(256), 0, stream, B, ldb, strideB, nrhs, 0,
                        0, ipiv, strideP, 0, N);
-    if ((e = (int)hipGetLastError())) return e;
-    // forward substitution with the unit lower factor
-    for (int k0 = 0; k0 < N; k0 += PB) {
-        const int kb = min(PB, N - k0);
-        if ((e = launch_trsm<R, false>(LU + k0 + (size_t)k0 * lda, lda, strideA, kb, B + k0, ldb, strideB, nrhs,
-                                       batch, stream)))
-            return e;
-        const int rest = N - k0 - kb;
-        if (rest > 0 && (e = gemm<R>(rest, nrhs, kb, R(-1), LU + (k0 + kb) + (size_t)k0 * lda, lda, strideA, B + k0,
-                                     ldb, strideB, B + k0 + kb, ldb, strideB, 0, batch, stream)))
-            return e;
-    }
-    // backward substitution with the upper factor
+    // forward substitution with the unit lower factor: the rows below each
+    // diagonal block, then every diagonal block in place
     const int nblk = (N + PB - 1) / PB;
-    for (int blk = nblk - 1; blk >= 0; --blk) {
-        const int k0 = blk * PB;
-        const int kb = min(PB, N - k0);
-        if ((e = launch_trsm<R, true>(LU + k0 + (size_t)k0 * lda, lda, strideA, kb, B + k0, ldb, strideB, nrhs,
-                                      batch, stream)))
-            return e;
-        if (k0 > 0 && (e = gemm<R>(k0, nrhs, kb, R(-1), LU + (size_t)k0 * lda, lda, strideA, B + k0, ldb, strideB, B,
-                                   ldb, strideB, 0, batch, stream)))
-            return e;
+    for (int k = 0; k + 1 < nblk; ++k) {
+        const int lo = (k + 1) * PB;
+        hipLaunchKernelGGL((substitution_kernel<R, false, true>), dim3((N - lo + sub_slab<R>() - 1) / sub_slab<R>(), batch),
+                           dim3(256), 0, stream, LU, lda, strideA, N, k, D, ldd, strideD, dstep, B, ldb, strideB, nrhs,
+                           lo, N);
     }
-    return 0;
+    hipLaunchKernelGGL((substitution_kernel<R, false, false>), dim3(nblk, batch), dim3(256), 0, stream, LU, lda,
+                       strideA, N, 0, D, ldd, strideD, dstep, B, ldb, strideB, nrhs, 0, 0);
+    // backward substitution with the upper factor: the rows above each
+    // diagonal block, then every diagonal block in place
+    for (int k = nblk - 1; k >= 1; --k) {
+        const int hi = k * PB;
+        hipLaunchKernelGGL((substitution_kernel<R, true, true>), dim3((hi + sub_slab<R>() - 1) / sub_slab<R>(), batch),
+                           dim3(256), 0, stream, LU, lda, strideA, N, k, D, ldd, strideD, dstep, B, ldb, strideB, nrhs,
+                           0, hi);
+    }
+    hipLaunchKernelGGL((substitution_kernel<R, true, false>), dim3(nblk, batch), dim3(256), 0, stream, LU, lda,
+                       strideA, N, 0, D, ldd, strideD, dstep, B, ldb, strideB, nrhs, 0, 0);
+    return (int)hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
@@ -1107,11 +1341,20 @@ int launch_getrf(cplx* A, int N, int64_t lda, int64_t strideA, int* ipiv, int64_
     return getrf<double>((cx<double>*)A, N, lda, strideA, ipiv, strideP, info, batch, (cx<double>*)work, aux, stream);
 }
 
+// Workspace of the solve: the inverted diagonal blocks, a strip of N x PB
+// per matrix.
+size_t getrs_workspace_bytes(int N, int batch) { return sizeof(cplx) * (size_t)N * PB * (size_t)batch; }
+
 int launch_getrs(const cplx* LU, int N, int64_t lda, int64_t strideA, const int* ipiv, int64_t strideP, cplx* B,
-                 int nrhs, int64_t ldb, int64_t strideB, int batch, hipStream_t stream)
+                 int nrhs, int64_t ldb, int64_t strideB, int batch, cplx* work, hipStream_t stream)
 {
-    return getrs<double>((const cx<double>*)LU, N, lda, strideA, ipiv, strideP, (cx<double>*)B, nrhs, ldb, strideB,
-                         batch, stream);
+    if (N <= 0 || nrhs <= 0 || batch <= 0) return 0;
+    int e;
+    cx<double>* D = (cx<double>*)work;
+    const int64_t sD = (int64_t)N * PB;
+    if ((e = invert_diag<double>((const cx<double>*)LU, N, lda, strideA, D, N, sD, 0, batch, stream))) return e;
+    return getrs<double>((const cx<double>*)LU, N, lda, strideA, ipiv, strideP, D, N, sD, 0, (cx<double>*)B, nrhs, ldb,
+                         strideB, batch, stream);
 }
 
 // Workspace of the mixed solve, in bytes: complex64 factors (batch n^2),
@@ -1207,10 +1450,13 @@ int launch_mixed_grouped(const MixedGroup* g, int ng, void* work, const LuSetup*
                            n, n);
         hipLaunchKernelGGL(zero_ints_kernel, dim3((batch + 255) / 256), blk, 0, G.s, S.info, batch);
         if ((e = getrf<float>(S.K32, n, n, sK32, S.ipiv, n, S.info, batch, nullptr, tune, G.s))) return e;
+        // the complex64 factor is internal: its diagonal blocks are replaced
+        // by their inverses for the substitutions
+        if ((e = invert_diag<float>(S.K32, n, n, sK32, S.K32, n, sK32, 1, batch, G.s))) return e;
         // x0 = LU32 \ b
         hipLaunchKernelGGL(z2c_kernel, gB, blk, 0, G.s, (const c128*)G.B, G.ldb, G.strideB, S.D, (int64_t)n, sRD, n,
                            r);
-        if ((e = getrs<float>(S.K32, n, n, sK32, S.ipiv, n, S.D, r, n, sRD, batch, G.s))) return e;
+        if ((e = getrs<float>(S.K32, n, n, sK32, S.ipiv, n, S.K32, n, sK32, 1, S.D, r, n, sRD, batch, G.s))) return e;
         hipLaunchKernelGGL(c2z_add_kernel, gB, blk, 0, G.s, S.D, (int64_t)n, sRD, (c128*)G.X, G.ldx, G.strideX, n, r,
                            0);
         hipLaunchKernelGGL(colnorm2_kernel, dim3(r, batch), blk, 0, G.s, (const c128*)G.B, G.ldb, G.strideB, n, r,
@@ -1266,7 +1512,7 @@ int launch_mixed_grouped(const MixedGroup* g, int ng, void* work, const LuSetup*
             const int64_t sK32 = (int64_t)n * n, sRD = (int64_t)n * r;
             const dim3 gB((n + 255) / 256, r, batch);
             hipLaunchKernelGGL(z2c_kernel, gB, blk, 0, G.s, S.Rz, (int64_t)n, sRD, S.D, (int64_t)n, sRD, n, r);
-            if ((e = getrs<float>(S.K32, n, n, sK32, S.ipiv, n, S.D, r, n, sRD, batch, G.s))) return e;
+            if ((e = getrs<float>(S.K32, n, n, sK32, S.ipiv, n, S.K32, n, sK32, 1, S.D, r, n, sRD, batch, G.s))) return e;
             hipLaunchKernelGGL(c2z_add_kernel, gB, blk, 0, G.s, S.D, (int64_t)n, sRD, (c128*)G.X, G.ldx, G.strideX,
                                n, r, 1);
             if ((e = (int)hipGetLastError())) return e;
@@ -1298,14 +1544,18 @@ int launch_getrf_lanes(const LuGroupArgs* g, int ng, const LuSetup* tune)
     return 0;
 }
 
-int launch_getrs_lanes(const LuGroupArgs* g, int ng)
+// One strip of inverted diagonal blocks per group, back to back in `work`
+// (getrs_workspace_bytes of each group, in group order).
+int launch_getrs_lanes(const LuGroupArgs* g, int ng, void* work)
 {
     int e;
+    char* p = (char*)work;
     for (int i = 0; i < ng; ++i) {
         if (g[i].n <= 0 || g[i].batch <= 0 || g[i].nrhs <= 0) continue;
-        if ((e = getrs<double>((const cx<double>*)g[i].A, g[i].n, g[i].lda, g[i].strideA, g[i].ipiv, g[i].strideP,
-                               (cx<double>*)g[i].B, g[i].nrhs, g[i].ldb, g[i].strideB, g[i].batch, g[i].s)))
+        if ((e = launch_getrs((const cplx*)g[i].A, g[i].n, g[i].lda, g[i].strideA, g[i].ipiv, g[i].strideP,
+                              (cplx*)g[i].B, g[i].nrhs, g[i].ldb, g[i].strideB, g[i].batch, (cplx*)p, g[i].s)))
             return e;
+        p += getrs_workspace_bytes(g[i].n, g[i].batch);
     }
     return 0;
 }

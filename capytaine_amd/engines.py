@@ -123,8 +123,9 @@ class LUDecomposedDeviceMatrix:
         """Solve in place for a device tensor b of shape (nb, nrhs, n)."""
         nb, nrhs, n = b.shape
         ctx = _lib.Context.get(b.device)
-        check(_lib.load().cpt_lu_solve(ctx.handle, ptr(self.lu.data), n, n, n * n, ptr(self.ipiv), n, ptr(b),
-                                       nrhs, n, n * nrhs, nb, stream_handle(b.device)), "cpt_lu_solve")
+        _lib.call_with_memory_retry(lambda: _lib.load().cpt_lu_solve(
+            ctx.handle, ptr(self.lu.data), n, n, n * n, ptr(self.ipiv), n, ptr(b), nrhs, n, n * nrhs, nb,
+            stream_handle(b.device)), "cpt_lu_solve")
         return b
 
     def solve(self, b):

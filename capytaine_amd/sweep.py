@@ -96,8 +96,9 @@ class FrequencySweep:
             self.iterations, self.residual = it.value, res.value
             return
         self.X.copy_(bc)
-        check(lib.cpt_lu_solve(self.ctx.handle, ptr(self.K), n, n, n * n, ptr(self.ipiv), n, ptr(self.X),
-                               self.nrhs, n, n * self.nrhs, self.nb, self.stream), "cpt_lu_solve")
+        _lib.call_with_memory_retry(lambda: lib.cpt_lu_solve(
+            self.ctx.handle, ptr(self.K), n, n, n * n, ptr(self.ipiv), n, ptr(self.X), self.nrhs, n, n * self.nrhs,
+            self.nb, self.stream), "cpt_lu_solve")
 
     def post(self):
         n, r, nd = self.n, self.nrhs, len(self.dofs)
