@@ -27,7 +27,7 @@ size_t mixed_workspace_bytes(int n, int nrhs, int batch);
 size_t mixed_grouped_bytes(const MixedGroup* g, int ng);
 int launch_mixed_grouped(const MixedGroup* g, int ng, void* work, const LuSetup* tune, int max_iter, double tol,
                          int* iterations, double* residual);
-int launch_getrf_lanes(const LuGroupArgs* g, int ng, const LuSetup* tune);
+int launch_getrf_lanes(const LuGroupArgs* g, int ng, const LuSetup* tune, void* work);
 int launch_getrs_lanes(const LuGroupArgs* g, int ng, void* work);
 size_t getrs_workspace_bytes(int N, int batch);
 int launch_mixed_solve(const cplx* K, int n, int64_t lda, int64_t strideK, int batch, const cplx* B, int nrhs,
@@ -1055,6 +1055,13 @@ int cpt_lu_factor_grouped(cpt_handle h, int ngroups, const cpt_lu_group* groups,
     if (ngroups == 0) return CPT_OK;
     CPT_CHECK_HIP(hipSetDevice(h->device), "hipSetDevice");
     CallScope call_scope(&h->timer, h->gemm_tiles);
+    size_t ws = 0;
+    for (int i = 0; i < ngroups; ++i)
+        if (groups[i].n > 0 && groups[i].batch > 0) ws += getrf_workspace_bytes(groups[i].n, groups[i].batch);
+    if (h->lu_work.reserve(ws)) {
+        set_error("cpt_lu_factor_grouped: workspace allocation failed");
+        return CPT_ERR_ALLOC;
+    }
     if (ensure_lanes(h) || fork_lanes(h, stream)) {
         set_error("cpt_lu_factor_grouped: could not set up the lane streams");
         return CPT_ERR_HIP;
@@ -1069,7 +1076,7 @@ int cpt_lu_factor_grouped(cpt_handle h, int ngroups, const cpt_lu_group* groups,
     LuSetup tune = h->lu_streams;
     tune.side = nullptr;  // no look-ahead inside a lane: the lanes overlap instead
     tune.lookahead = 0;
-    const int e = launch_getrf_lanes(g.data(), ngroups, &tune);
+    const int e = launch_getrf_lanes(g.data(), ngroups, &tune, h->lu_work.p);
     if (join_lanes(h, stream)) {
         set_error("cpt_lu_factor_grouped: could not join the lane streams");
         return CPT_ERR_HIP;

@@ -512,7 +512,7 @@ __device__ __forceinline__ void invert_triangle(const cx<R>* sT, int kb, int lan
     }
 }
 
-template <typename R>
+template <typename R, bool WITH_U>
 __global__ void __launch_bounds__(256) invert_diag_kernel(const cx<R>* __restrict__ LUbase, int64_t lda,
                                                           int64_t strideA, int N, cx<R>* Dbase, int64_t ldd,
                                                           int64_t strideD, int dstep)
@@ -525,9 +525,92 @@ __global__ void __launch_bounds__(256) invert_diag_kernel(const cx<R>* __restric
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     stage_triangle<R, false, 256>(T, lda, kb, sT);
     invert_triangle<R, false>(sT, kb, lane, wave, D, ldd);
+    if (!WITH_U) return;
     __syncthreads();  // every wavefront is done with L before U replaces it
     stage_triangle<R, true, 256>(T, lda, kb, sT);
     invert_triangle<R, true>(sT, kb, lane, wave, D, ldd);
+}
+
+// ---------------------------------------------------------------------------
+// 3d. B_j <- inv(L_jj) B_j in place for a wide B (kb <= PB rows, inverse
+//     from invert_diag_kernel): work-group of four wavefronts over
+//     APPLY_COLS columns of one matrix, 16 columns at a time.  Each group of
+//     columns is read from memory once, coalesced (thread t: row t % 64),
+//     into an LDS buffer while the previous group is multiplied
+//     (double-buffered, one barrier per group); wavefront w keeps the
+//     inverse's rows 16 w .. +16 as MFMA operands and writes those rows of
+//     the result, in place: a group is already in LDS when it is written.
+// ---------------------------------------------------------------------------
+constexpr int APPLY_COLS = 128;
+
+template <typename R>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+apply_inverse_kernel(const cx<R>* __restrict__ Dbase, int64_t ldd, int64_t strideD, int kb, cx<R>* Bbase, int64_t ldb,
+                     int64_t strideB, int ncols)
+{
+    typedef cx<R> C;
+    typedef typename Mfma<R>::acc acc;
+    __shared__ C sB[2][PB][17];  // [buffer][row k][column], padded against bank conflicts
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const C* D = Dbase + (size_t)blockIdx.y * strideD;
+    C* B = Bbase + (size_t)blockIdx.y * strideB;
+    const int i = 16 * wave + li;
+    C a[PB / 4];
+#pragma unroll
+    for (int s = 0; s < PB / 4; ++s) {
+        const int k = 4 * s + lk;
+        const bool in = i < kb && k < kb;
+        a[s] = ld_or_zero(D, i + (size_t)k * ldd, in && i > k);
+        if (in && i == k) a[s] = C{R(1), R(0)};
+    }
+    const int cbeg = blockIdx.x * APPLY_COLS, cend = min(ncols, cbeg + APPLY_COLS);
+    // thread t stages row t % 64 of the columns t / 64 + 4 u, u < 4
+    const int sr = threadIdx.x & 63, sc = threadIdx.x >> 6;
+    C g[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int col = cbeg + sc + 4 * u;
+        g[u] = ld_or_zero(B, sr + (size_t)col * ldb, sr < kb && col < cend);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) sB[0][sr][sc + 4 * u] = g[u];
+    __syncthreads();
+    int buf = 0;
+    for (int c0 = cbeg; c0 < cend; c0 += 16) {
+        const bool more = c0 + 16 < cend;
+        if (more) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int col = c0 + 16 + sc + 4 * u;
+                g[u] = ld_or_zero(B, sr + (size_t)col * ldb, sr < kb && col < cend);
+            }
+        }
+        acc xr{0, 0, 0, 0}, xi{0, 0, 0, 0};
+#pragma unroll
+        for (int s = 0; s < PB / 4; ++s) {
+            const C bv = sB[buf][4 * s + lk][li];
+            xr = Mfma<R>::op(a[s].re, bv.re, xr);
+            xi = Mfma<R>::op(a[s].re, bv.im, xi);
+            xr = Mfma<R>::op(-a[s].im, bv.im, xr);
+            xi = Mfma<R>::op(a[s].im, bv.re, xi);
+        }
+        const int col = c0 + li;
+        if (col < cend) {
+            C* Bc = B + (size_t)col * ldb;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int r = 16 * wave + Mfma<R>::row(lane, v);
+                if (r < kb) Bc[r] = C{xr[v], xi[v]};
+            }
+        }
+        if (more) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) sB[buf ^ 1][sr][sc + 4 * u] = g[u];
+        }
+        __syncthreads();  // the next group is staged; this buffer is free again
+        buf ^= 1;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1020,9 +1103,26 @@ template <typename R>
 int gemm(int M, int N, int K, R alpha, const cx<R>* A, int64_t lda, int64_t sA, const cx<R>* B, int64_t ldb,
          int64_t sB, cx<R>* C, int64_t ldc, int64_t sC, int beta_zero, int batch, hipStream_t stream);
 
+// Inverted diagonal blocks of the factor LU into D (dstep = 1: D is the
+// factor itself, its diagonal blocks overwritten; 0: a strip of PB columns
+// per matrix, leading dimension ldd >= N).
+template <typename R>
+int invert_diag(const cx<R>* LU, int N, int64_t lda, int64_t strideA, cx<R>* D, int64_t ldd, int64_t strideD,
+                int dstep, int batch, hipStream_t stream, bool with_u = true)
+{
+    if (N <= 0 || batch <= 0) return 0;
+    if (with_u)
+        hipLaunchKernelGGL((invert_diag_kernel<R, true>), dim3((N + PB - 1) / PB, batch), dim3(256), 0, stream, LU,
+                           lda, strideA, N, D, ldd, strideD, dstep);
+    else
+        hipLaunchKernelGGL((invert_diag_kernel<R, false>), dim3((N + PB - 1) / PB, batch), dim3(256), 0, stream, LU,
+                           lda, strideA, N, D, ldd, strideD, dstep);
+    return (int)hipGetLastError();
+}
+
 template <typename R>
 int trsm_lower(const cx<R>* L, int64_t lda, int64_t sA, int kb, cx<R>* B, int64_t ldb, int64_t sB, int ncols,
-               int batch, hipStream_t stream);
+               int batch, hipStream_t stream, cx<R>* inv = nullptr);
 
 // Recursive factorisation of columns [c0, c0 + w) of rows [c0, N) inside the
 // panel [sw_lo, sw_hi): factor the left half, solve its unit-lower triangle
@@ -1103,18 +1203,28 @@ int gemm(int M, int N, int K, R alpha, const cx<R>* A, int64_t lda, int64_t sA, 
 }
 
 // L^-1 B in place for a unit lower kb x kb block L (any kb): blocked
-// forward substitution over PB-row diagonal blocks, each solved by the
-// wavefront-wide triangle kernel, the blocks below updated by the product.
+// forward substitution over PB-row diagonal blocks, the blocks below updated
+// by the product.  Each diagonal block is solved by the wavefront-wide
+// triangle kernel, or, given `inv` (room for kb x PB per matrix), by the
+// MFMA product with its inverse, all of L's diagonal blocks inverted first
+// in one launch.
 template <typename R>
 int trsm_lower(const cx<R>* L, int64_t lda, int64_t sA, int kb, cx<R>* B, int64_t ldb, int64_t sB, int ncols,
-               int batch, hipStream_t stream)
+               int batch, hipStream_t stream, cx<R>* inv)
 {
     int e;
+    const int64_t sInv = (int64_t)kb * PB;
+    if (inv && (e = invert_diag<R>(L, kb, lda, sA, inv, kb, sInv, 0, batch, stream, false))) return e;
     for (int j0 = 0; j0 < kb; j0 += PB) {
         const int jb = min(PB, kb - j0);
-        if ((e = launch_trsm<R, false>(L + j0 + (size_t)j0 * lda, lda, sA, jb, B + j0, ldb, sB, ncols, batch,
-                                       stream)))
+        if (inv) {
+            hipLaunchKernelGGL(apply_inverse_kernel<R>, dim3((ncols + APPLY_COLS - 1) / APPLY_COLS, batch), dim3(256),
+                               0, stream, inv + j0, (int64_t)kb, sInv, jb, B + j0, ldb, sB, ncols);
+            if ((e = (int)hipGetLastError())) return e;
+        } else if ((e = launch_trsm<R, false>(L + j0 + (size_t)j0 * lda, lda, sA, jb, B + j0, ldb, sB, ncols, batch,
+                                              stream))) {
             return e;
+        }
         const int rest = kb - j0 - jb;
         if (rest > 0 && (e = gemm<R>(rest, ncols, jb, R(-1), L + (j0 + jb) + (size_t)j0 * lda, lda, sA, B + j0, ldb,
                                      sB, B + j0 + jb, ldb, sB, 0, batch, stream)))
@@ -1128,10 +1238,12 @@ int trsm_lower(const cx<R>* L, int64_t lda, int64_t sA, int kb, cx<R>* B, int64_
 // (256 halves it again, at the price of wider panels: a tuning choice).
 inline int getrf_block(int N, int tuned) { return tuned ? tuned : (N >= 2048 ? 4 * PB : N >= 1024 ? 2 * PB : PB); }
 
+// Workspace of the factorisation (elements): the inverted diagonal blocks
+// of L11 for U12 = L11^-1 A12, kb x PB per matrix for the widest block.
 template <typename R>
-size_t getrf_ws_elems(int, int)
+size_t getrf_ws_elems(int, int batch)
 {
-    return 0;
+    return (size_t)4 * PB * PB * (size_t)batch;
 }
 
 // Blocked right-looking LU with partial pivoting (LAPACK zgetrf semantics),
@@ -1142,15 +1254,17 @@ size_t getrf_ws_elems(int, int)
 // of A22 is updated (look-ahead of depth one).
 template <typename R>
 int getrf_steps(cx<R>* A, int N, int64_t lda, int64_t strideA, int* ipiv, int64_t strideP, int* info, int batch,
-                const LuSetup* aux, hipStream_t stream);
+                cx<R>* work, const LuSetup* aux, hipStream_t stream);
 
+// `work`: getrf_ws_elems elements, or null (then U12 is solved by the
+// wavefront triangle kernel instead of the inverted diagonal blocks)
 template <typename R>
 int getrf(cx<R>* A, int N, int64_t lda, int64_t strideA, int* ipiv, int64_t strideP, int* info, int batch,
-          cx<R>*, const LuSetup* aux, hipStream_t stream)
+          cx<R>* work, const LuSetup* aux, hipStream_t stream)
 {
     if (N <= 0 || batch <= 0) return 0;
     if (!aux || aux->lookahead != 2 || !aux->part_main) return getrf_steps<R>(A, N, lda, strideA, ipiv, strideP, info,
-                                                                             batch, aux, stream);
+                                                                             batch, work, aux, stream);
     // CU-partitioned look-ahead: the factorisation runs on the two masked
     // streams, ordered after the caller's stream, which then waits for it
     LuSetup part = *aux;
@@ -1158,7 +1272,7 @@ int getrf(cx<R>* A, int N, int64_t lda, int64_t strideA, int* ipiv, int64_t stri
     if (hipEventRecord(aux->enter, stream) != hipSuccess) return (int)hipErrorUnknown;
     if (hipStreamWaitEvent(aux->part_main, aux->enter, 0) != hipSuccess) return (int)hipErrorUnknown;
     if (hipStreamWaitEvent(aux->part_side, aux->enter, 0) != hipSuccess) return (int)hipErrorUnknown;
-    const int e = getrf_steps<R>(A, N, lda, strideA, ipiv, strideP, info, batch, &part, aux->part_main);
+    const int e = getrf_steps<R>(A, N, lda, strideA, ipiv, strideP, info, batch, work, &part, aux->part_main);
     if (hipEventRecord(aux->leave, aux->part_main) != hipSuccess) return (int)hipErrorUnknown;
     if (hipStreamWaitEvent(stream, aux->leave, 0) != hipSuccess) return (int)hipErrorUnknown;
     return e;
@@ -1166,7 +1280,7 @@ int getrf(cx<R>* A, int N, int64_t lda, int64_t strideA, int* ipiv, int64_t stri
 
 template <typename R>
 int getrf_steps(cx<R>* A, int N, int64_t lda, int64_t strideA, int* ipiv, int64_t strideP, int* info, int batch,
-                const LuSetup* aux, hipStream_t stream)
+                cx<R>* work, const LuSetup* aux, hipStream_t stream)
 {
     int e;
     const int nb = getrf_block(N, aux ? aux->block : 0);
@@ -1185,7 +1299,7 @@ int getrf_steps(cx<R>* A, int N, int64_t lda, int64_t strideA, int* ipiv, int64_
         cx<R>* A12 = A + k0 + (size_t)(k0 + kb) * lda;
         cx<R>* A21 = A + (k0 + kb) + (size_t)k0 * lda;
         cx<R>* A22 = A + (k0 + kb) + (size_t)(k0 + kb) * lda;
-        if ((e = trsm_lower<R>(A11, lda, strideA, kb, A12, lda, strideA, rest, batch, stream))) return e;
+        if ((e = trsm_lower<R>(A11, lda, strideA, kb, A12, lda, strideA, rest, batch, stream, work))) return e;
         const int kn = min(nb, rest);  // width of the next panel
         // trailing update of the next panel's columns first ...
         if ((e = gemm<R>(rest, kn, kb, R(-1), A21, lda, strideA, A12, lda, strideA, A22, lda, strideA, 0, batch,
@@ -1211,19 +1325,6 @@ int getrf_steps(cx<R>* A, int N, int64_t lda, int64_t strideA, int* ipiv, int64_
         }
     }
     return 0;
-}
-
-// Inverted diagonal blocks of the factor LU into D (dstep = 1: D is the
-// factor itself, its diagonal blocks overwritten; 0: a strip of PB columns
-// per matrix, leading dimension ldd >= N).
-template <typename R>
-int invert_diag(const cx<R>* LU, int N, int64_t lda, int64_t strideA, cx<R>* D, int64_t ldd, int64_t strideD,
-                int dstep, int batch, hipStream_t stream)
-{
-    if (N <= 0 || batch <= 0) return 0;
-    hipLaunchKernelGGL(invert_diag_kernel<R>, dim3((N + PB - 1) / PB, batch), dim3(256), 0, stream, LU, lda, strideA,
-                       N, D, ldd, strideD, dstep);
-    return (int)hipGetLastError();
 }
 
 // Solve with the factor (LAPACK zgetrs): interchanges of B, then the
@@ -1371,6 +1472,7 @@ size_t mixed_group_bytes(int n, int nrhs, int batch)
     bytes += align256(sizeof(int) * (b * n + b));
     bytes += align256(sizeof(cx<double>) * b * n * nrhs);
     bytes += align256(sizeof(cx<float>) * b * n * nrhs);
+    bytes += align256(sizeof(cx<float>) * getrf_ws_elems<float>(n, batch));
     return bytes;
 }
 
@@ -1407,6 +1509,7 @@ int launch_mixed_grouped(const MixedGroup* g, int ng, void* work, const LuSetup*
         int* info;
         c128* Rz;
         c64* D;
+        c64* W;  // getrf workspace
         double* nb2;
         double* nr2;
         size_t norm_off;
@@ -1426,6 +1529,8 @@ int launch_mixed_grouped(const MixedGroup* g, int ng, void* work, const LuSetup*
         p += align256(sizeof(c128) * b * n * r);
         st[i].D = (c64*)p;
         p += align256(sizeof(c64) * b * n * r);
+        st[i].W = (c64*)p;
+        p += align256(sizeof(c64) * getrf_ws_elems<float>((int)n, (int)b));
         st[i].norm_off = nnorm;
         nnorm += b * r;
     }
@@ -1449,7 +1554,7 @@ int launch_mixed_grouped(const MixedGroup* g, int ng, void* work, const LuSetup*
         hipLaunchKernelGGL(z2c_kernel, gK, blk, 0, G.s, (const c128*)G.K, G.lda, G.strideK, S.K32, (int64_t)n, sK32,
                            n, n);
         hipLaunchKernelGGL(zero_ints_kernel, dim3((batch + 255) / 256), blk, 0, G.s, S.info, batch);
-        if ((e = getrf<float>(S.K32, n, n, sK32, S.ipiv, n, S.info, batch, nullptr, tune, G.s))) return e;
+        if ((e = getrf<float>(S.K32, n, n, sK32, S.ipiv, n, S.info, batch, S.W, tune, G.s))) return e;
         // the complex64 factor is internal: its diagonal blocks are replaced
         // by their inverses for the substitutions
         if ((e = invert_diag<float>(S.K32, n, n, sK32, S.K32, n, sK32, 1, batch, G.s))) return e;
@@ -1530,16 +1635,20 @@ int launch_mixed_solve(const cplx* K, int n, int64_t lda, int64_t strideK, int b
     return launch_mixed_grouped(&g, 1, work, aux, max_iter, tol, iterations, residual);
 }
 
-int launch_getrf_lanes(const LuGroupArgs* g, int ng, const LuSetup* tune)
+// One factorisation workspace per group, back to back in `work`
+// (getrf_workspace_bytes of each group, in group order).
+int launch_getrf_lanes(const LuGroupArgs* g, int ng, const LuSetup* tune, void* work)
 {
     int e;
+    char* p = (char*)work;
     for (int i = 0; i < ng; ++i) {
         if (g[i].n <= 0 || g[i].batch <= 0) continue;
         hipLaunchKernelGGL(zero_ints_kernel, dim3((g[i].batch + 255) / 256), dim3(256), 0, g[i].s, g[i].info,
                            g[i].batch);
         if ((e = getrf<double>((cx<double>*)g[i].A, g[i].n, g[i].lda, g[i].strideA, g[i].ipiv, g[i].strideP,
-                               g[i].info, g[i].batch, nullptr, tune, g[i].s)))
+                               g[i].info, g[i].batch, (cx<double>*)p, tune, g[i].s)))
             return e;
+        p += getrf_workspace_bytes(g[i].n, g[i].batch);
     }
     return 0;
 }
