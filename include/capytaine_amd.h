@@ -157,7 +157,10 @@ int cpt_lu_factor(cpt_handle h, void* A, int n, int64_t lda, int64_t strideA, in
                   int* info, int batch, hipStream_t stream);
 
 /* Solve A X = B for nrhs right-hand sides at once with the factors of
- * cpt_lu_factor; B (n x nrhs, column-major, ldb) is overwritten by X.
+ * cpt_lu_factor; B (n x nrhs, column-major, ldb) is overwritten by X.  The
+ * inverses of the factors' 64 x 64 diagonal blocks are formed in the
+ * context's workspace (n x 64 complex128 per matrix; CPT_ERR_ALLOC if it
+ * cannot grow); calls on one context share it, so order them on one stream.
  * Replaces scipy.linalg.lu_solve (LAPACK zgetrs),
  * src/capytaine/tools/block_circulant_matrices.py:380-385, which the
  * reference calls once per right-hand side (engines.py:338). */
