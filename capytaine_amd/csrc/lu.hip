@@ -119,6 +119,75 @@ constexpr int PB = 64;  // widest triangle the panel and trsm kernels hold (one 
 //   (e) store the sub-block, then apply its interchanges to the other
 //       columns [sw_lo, sw_hi) in global memory (once per sub-block).
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ double lane_bcast(double v, int j)
+{
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), j);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), j);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ float lane_bcast(float v, int j)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+}
+
+// Value of lane j (wave-uniform j) in every lane, through a scalar register.
+// DPP moves within rows of 16 lanes: xor 1, xor 2 (quad permutations),
+// then the mirrors of 8 and of 16 lanes.
+constexpr int DPP_XOR1 = 0xB1, DPP_XOR2 = 0x4E, DPP_HALF_MIRROR = 0x141, DPP_MIRROR = 0x140;
+
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int v)
+{
+    return __builtin_amdgcn_update_dpp(v, v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_r(float v)
+{
+    return __int_as_float(dpp_i<CTRL>(__float_as_int(v)));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_r(double v)
+{
+    const int lo = dpp_i<CTRL>(__double2loint(v)), hi = dpp_i<CTRL>(__double2hiint(v));
+    return __hiloint2double(hi, lo);
+}
+
+// (best, idx) <- the largest best of the wavefront, the smallest idx among
+// equal values (LAPACK's first index on ties); result uniform.  Lanes
+// without a candidate hold best = -1, idx = INT_MAX.
+template <typename R, int CTRL>
+__device__ __forceinline__ void argmax_step(R& best, int& idx)
+{
+    const R ob = dpp_r<CTRL>(best);
+    const int oi = dpp_i<CTRL>(idx);
+    if (ob > best || (ob == best && oi < idx)) {
+        best = ob;
+        idx = oi;
+    }
+}
+template <typename R>
+__device__ __forceinline__ void wave_argmax(R& best, int& idx)
+{
+    argmax_step<R, DPP_XOR1>(best, idx);
+    argmax_step<R, DPP_XOR2>(best, idx);
+    argmax_step<R, DPP_HALF_MIRROR>(best, idx);
+    argmax_step<R, DPP_MIRROR>(best, idx);
+    // every lane of a row of 16 holds the row's result: combine the four rows
+    R b = lane_bcast(best, 0);
+    int i = __builtin_amdgcn_readlane(idx, 0);
+#pragma unroll
+    for (int r = 16; r < 64; r += 16) {
+        const R rb = lane_bcast(best, r);
+        const int ri = __builtin_amdgcn_readlane(idx, r);
+        if (rb > b || (rb == b && ri < i)) {
+            b = rb;
+            i = ri;
+        }
+    }
+    best = b;
+    idx = i;
+}
+
 template <typename R, int NT, int RPT, int SB>
 __global__ void __launch_bounds__(NT) panel_kernel(cx<R>* __restrict__ Abase, int64_t lda, int64_t strideA, int N,
                                                    int k0, int kb, int sw_lo, int sw_hi,
@@ -134,7 +203,7 @@ __global__ void __launch_bounds__(NT) panel_kernel(cx<R>* __restrict__ Abase, in
     constexpr int NW = NT / 64;
     const C zero{R(0), R(0)};
 
-    __shared__ C sL[PB * PB];  // L11, row-major [i][j]
+    __shared__ C sL[PB * PB];  // L11, column-major (i + PB j)
     __shared__ C sU[PB * SB];
     __shared__ C sSwap[2][2][SB];
     __shared__ R sBest[2][NW];
@@ -167,20 +236,16 @@ __global__ void __launch_bounds__(NT) panel_kernel(cx<R>* __restrict__ Abase, in
             }
             for (int e = tid; e < t0 * t0; e += NT) {
                 const int i = e % t0, j = e / t0;
-                sL[i * PB + j] = A[(k0 + i) + (size_t)(k0 + j) * lda];
+                sL[i + PB * j] = A[(k0 + i) + (size_t)(k0 + j) * lda];
             }
             __syncthreads();
             if (wave < sbw) {
+                // column sweep: x_j broadcast from lane j, lanes below update
                 const int s = wave;
                 C x = ld_or_zero(sU, lane * SB + s, lane < t0);
-                for (int i = 1; i < t0; ++i) {
-                    C part = zero;
-                    if (lane < i) part = zmul(sL[i * PB + lane], x);
-                    for (int off = 32; off > 0; off >>= 1) {
-                        part.re += __shfl_xor(part.re, off);
-                        part.im += __shfl_xor(part.im, off);
-                    }
-                    if (lane == i) x = zsub(x, part);
+                for (int j = 0; j < t0 - 1; ++j) {
+                    const C xj{lane_bcast(x.re, j), lane_bcast(x.im, j)};
+                    if (lane > j && lane < t0) x = zsub(x, zmul(sL[lane + PB * j], xj));
                 }
                 if (lane < t0) sU[lane * SB + s] = x;
             }
@@ -224,14 +289,7 @@ __global__ void __launch_bounds__(NT) panel_kernel(cx<R>* __restrict__ Abase, in
                         }
                     }
                 }
-                for (int off = 32; off > 0; off >>= 1) {
-                    const R ob = __shfl_xor(best, off);
-                    const int oi = __shfl_xor(bidx, off);
-                    if (ob > best || (ob == best && oi < bidx)) {
-                        best = ob;
-                        bidx = oi;
-                    }
-                }
+                wave_argmax(best, bidx);
                 if (lane == 0) {
                     sBest[buf][wave] = best;
                     sIdx[buf][wave] = bidx;
@@ -380,17 +438,6 @@ __global__ void laswp_kernel(cx<R>* __restrict__ Abase, int64_t lda, int64_t str
 //    UPPER = false: T unit lower triangular; UPPER = true: T upper, non-unit.
 // ---------------------------------------------------------------------------
 constexpr int TRSM_WAVES = 4, TRSM_CPW = 8, TRSM_ILP = 4;
-
-__device__ __forceinline__ double lane_bcast(double v, int j)
-{
-    const int lo = __builtin_amdgcn_readlane(__double2loint(v), j);
-    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), j);
-    return __hiloint2double(hi, lo);
-}
-__device__ __forceinline__ float lane_bcast(float v, int j)
-{
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
-}
 
 // The kb x kb triangle of T (column-major, leading dimension ldt) into LDS
 // (sT[i + PB j]), loads batched four per thread; for UPPER the diagonal is
