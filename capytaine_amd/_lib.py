@@ -202,6 +202,14 @@ def set_tuning(ctx, **params):
         check(lib.cpt_set_tuning(ctx.handle, TUNING_PARAMS.index(name), int(value)), f"cpt_set_tuning({name})")
 
 
+def lookahead_active(ctx):
+    """Look-ahead mode the context's last factorisation ran (mode 2 runs as 1
+    when the device has no hardware queue for its CU-masked streams)."""
+    v = ctypes.c_int()
+    check(load().cpt_get_tuning(ctx.handle, 6, ctypes.byref(v)), "cpt_get_tuning")
+    return v.value
+
+
 def get_tuning(ctx):
     lib = load()
     out = {}

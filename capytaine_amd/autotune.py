@@ -146,8 +146,9 @@ def tune_lu(n, batch, precision="fp64", device=None, reps=2, nrhs=14, candidates
             t = _time(run, device, reps)
             timings[f"{blk}/{leaf}/{la}/{cus}/{tl}"] = t
             if log:
-                log(f"[tune] n={n} batch={batch} {precision} block={blk} leaf={leaf} lookahead={la} "
-                    f"panel_cus={cus} gemm_tiles={tl}: {t:.2f} ms")
+                ran = _lib.lookahead_active(ctx)
+                log(f"[tune] n={n} batch={batch} {precision} block={blk} leaf={leaf} lookahead={la}"
+                    f"{'' if ran == la else f' (ran as {ran})'} panel_cus={cus} gemm_tiles={tl}: {t:.2f} ms")
     finally:
         _lib.set_tuning(ctx, **saved)
     best_key = min(timings, key=timings.get)

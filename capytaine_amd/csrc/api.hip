@@ -1038,6 +1038,12 @@ int cpt_get_tuning(cpt_handle h, int param, int* value)
     case CPT_TUNE_LOOKAHEAD: *value = h->lu_streams.lookahead; return CPT_OK;
     case CPT_TUNE_PANEL_CUS: *value = h->lu_streams.panel_cus; return CPT_OK;
     case CPT_TUNE_GEMM_TILES: *value = h->gemm_tiles; return CPT_OK;
+    case CPT_TUNE_LOOKAHEAD_ACTIVE:
+        // mode 2 runs as mode 1 when its CU-masked streams could not be made
+        *value = (h->lu_streams.lookahead == 2 && (h->part_failed || !h->lu_streams.part_main))
+                     ? (h->part_failed ? 1 : 2)
+                     : h->lu_streams.lookahead;
+        return CPT_OK;
     case CPT_TUNE_ASSEMBLY_CHUNK: *value = h->asm_chunk; return CPT_OK;
     default:
         set_error("cpt_get_tuning: unknown parameter " + std::to_string(param));
