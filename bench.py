@@ -183,7 +183,7 @@ def fleet_benchmark(gf, dev, rho=1025.0, g=9.81, seed=0, budget_gb=110.0, log=No
     return out
 
 
-def serving_benchmark(gf, dev, seed=1, requests=24, loads=(0.5, 0.8, 1.0, 1.2), log=None):
+def serving_benchmark(gf, dev, seed=1, requests=40, loads=(0.5, 0.8, 1.0, 1.2), log=None):
     """Poisson request stream against the continuous-batching server:
     saturation throughput from a burst of `requests` requests, then
     latency percentiles at arrival rates of `loads` x saturation."""

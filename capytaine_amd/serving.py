@@ -8,15 +8,17 @@ radiation damping, diffraction and Froude-Krylov forces.
 
 Design
   * Requests become jobs of at most `max_job_bytes` of device memory
-    (frequency chunks of a request whose matrices do not fit at once).
+    (frequency chunks; by default an eighth of the budget, so that a large
+    request is served in chunks that take turns with other requests).
   * Admission is by device-memory footprint (S, K, the complex64 copy and
     the right-hand sides: about N^2 x 40 bytes per frequency in mixed
     precision) against `memory_budget`; a job waits until in-flight work has
-    released enough memory.  Jobs start in arrival order (no overtaking of a
-    waiting job, so large requests are not starved).
-  * Jobs are batched by size class of their panel count: the oldest waiting
-    job opens a batch, which takes the waiting jobs of the same class in
-    arrival order while memory and `max_batch_jobs` allow.
+    released enough memory.  Service order is round robin over requests (a
+    job's rank is its index among its request's chunks), then arrival; the
+    first job in that order is never overtaken, so nothing is starved.
+  * Jobs are batched by size class of their panel count: the first waiting
+    job in service order opens a batch, which takes the waiting jobs of the
+    same class in that order while memory and `max_batch_jobs` allow.
   * A batch runs on one of `slots` workers, each with its own HIP stream and
     its own library context (workspace): one persistent assembly launch for
     all its bodies and frequencies, one grouped LU, the forces.  Batches on
@@ -40,7 +42,7 @@ import numpy as np
 
 from . import _lib
 from .batch import BatchedSweep, footprint_bytes
-from .problems import DiffractionProblem, froude_krylov_force, wavenumber_from_omega
+from .problems import wavenumber_from_omega
 
 SIZE_CLASSES = (1000, 2500, 6000)
 
@@ -72,16 +74,33 @@ class _Request:
         self.residual = 0.0
         self.precision = precision
 
+    def _incident(self, idx):
+        """Deep-water Airy wave at the panel centres for the frequencies idx
+        and every heading, (F, H, n) each: exp(k z) exp(i k (x cos b +
+        y sin b)) and the normal factor cos b n_x + sin b n_y - i n_z, the
+        terms of DiffractionProblem.boundary_condition and of the
+        Froude-Krylov pressure (problems.airy_waves_velocity / _potential,
+        airy_waves.py:19-94, no forward speed) for all problems at once."""
+        mesh = self.body.mesh
+        x, y, z = np.asarray(mesh.faces_centers).T
+        nx, ny, nz = np.asarray(mesh.faces_normals).T
+        k = self.ks[idx][:, None, None]
+        cb = np.cos(self.headings)[None, :, None]
+        sb = np.sin(self.headings)[None, :, None]
+        wave = np.exp(k * z) * np.exp(1j * k * (x * cb + y * sb))
+        return wave, cb * nx + sb * ny - 1j * nz
+
     def boundary_conditions(self, idx):
         """(len(idx), nrhs, n) host array: radiation then diffraction."""
+        idx = np.asarray(idx)
+        om = self.omegas[idx]
         dn = self.body.dof_normal_matrix()  # (ndof, n)
+        nd = len(self.dofs)
         out = np.empty((len(idx), self.nrhs, self.n), dtype=complex)
-        for a, i in enumerate(idx):
-            om = self.omegas[i]
-            out[a, :len(self.dofs)] = -1j * om * dn
-            for h, beta in enumerate(self.headings):
-                pb = DiffractionProblem(body=self.body, omega=om, wave_direction=beta, rho=self.rho, g=self.g)
-                out[a, len(self.dofs) + h] = pb.boundary_condition
+        out[:, :nd] = -1j * om[:, None, None] * dn[None]
+        if len(self.headings):
+            wave, normal = self._incident(idx)
+            out[:, nd:] = -(self.g * self.ks[idx] / om)[:, None, None] * wave * normal
         return out
 
     def result(self):
@@ -94,11 +113,11 @@ class _Request:
         damping = np.imag(rad) / om[:, None, None]
         diffraction = F[:, nd:, :]
         fk = np.zeros_like(diffraction)
-        for i, w in enumerate(om):
-            for h, beta in enumerate(self.headings):
-                pb = DiffractionProblem(body=self.body, omega=w, wave_direction=beta, rho=self.rho, g=self.g)
-                f = froude_krylov_force(pb)
-                fk[i, h] = [f[d] for d in self.dofs]
+        if len(self.headings):
+            # pressure rho g exp(kz) exp(i k w); force on dof d: -sum p (d . n) A
+            wave, _ = self._incident(np.arange(len(om)))
+            dna = self.body.dof_normal_matrix() * np.asarray(self.body.mesh.faces_areas)[None]
+            fk[:] = -(self.rho * self.g) * (wave @ dna.T)
         return {"omegas": om, "dofs": self.dofs, "headings": self.headings, "added_mass": added_mass,
                 "radiation_damping": damping, "diffraction_force": diffraction, "froude_krylov_force": fk,
                 "excitation_force": diffraction + fk, "forces": F, "refinement_sweeps": self.iterations,
@@ -107,24 +126,33 @@ class _Request:
 
 
 class _Job:
-    def __init__(self, req, idx, nbytes):
-        self.req, self.idx, self.nbytes = req, np.asarray(idx), nbytes
+    def __init__(self, req, idx, nbytes, rank=0):
+        self.req, self.idx, self.nbytes, self.rank = req, np.asarray(idx), nbytes, rank
         self.cls = size_class(req.n)
 
 
+def _order(job, pos):
+    """Service order: round robin over requests (the rank of a job is its
+    index among its request's frequency chunks), then arrival."""
+    return (getattr(job, "rank", 0), pos)
+
+
 def select_batch(waiting, used, budget, inflight, max_jobs):
-    """Admission and batching rule: the oldest waiting job opens the batch
-    if its bytes fit beside the `used` in-flight bytes (or nothing is in
-    flight: a job larger than the free memory runs alone); later jobs of the
-    same size class join in arrival order while they fit, up to `max_jobs`.
+    """Admission and batching rule: the first waiting job in service order
+    (the lowest chunk rank, then the oldest) opens the batch if its bytes fit
+    beside the `used` in-flight bytes (or nothing is in flight: a job larger
+    than the free memory runs alone); other jobs of the same size class join
+    in service order while they fit, up to `max_jobs`.  A new request's
+    first chunk thus goes ahead of the remaining chunks of a large one.
     Returns (jobs, bytes) or None (the head waits: no overtaking)."""
     if not waiting:
         return None
-    head = waiting[0]
+    ordered = [job for _, job in sorted(((_order(j, i), j) for i, j in enumerate(waiting)), key=lambda t: t[0])]
+    head = ordered[0]
     if used + head.nbytes > budget and inflight > 0:
         return None
     batch, nbytes = [head], head.nbytes
-    for job in list(waiting)[1:]:
+    for job in ordered[1:]:
         if len(batch) >= max_jobs:
             break
         if job.cls == head.cls and used + nbytes + job.nbytes <= budget:
@@ -149,7 +177,16 @@ class BEMServer:
         if memory_budget is None:
             memory_budget = int(0.5 * torch.cuda.get_device_properties(self.dev).total_memory)
         self.memory_budget = int(memory_budget)
-        self.max_job_bytes = int(max_job_bytes or self.memory_budget // 2)
+        # torch keeps the blocks of finished batches cached; batches of other
+        # sizes cannot reuse them all, so the cache would grow until the
+        # device is full and the runtime could not place a kernel's scratch
+        # or the library's workspaces.  Above this many reserved bytes the
+        # cache is returned to the driver after a batch.
+        total = torch.cuda.get_device_properties(self.dev).total_memory
+        self.cache_limit = self.memory_budget + max(0, total - self.memory_budget) // 2
+        # jobs of at most an eighth of the budget: a large request is served
+        # in frequency chunks that take turns with other requests
+        self.max_job_bytes = int(max_job_bytes or self.memory_budget // 8)
         self.max_batch_jobs = max_batch_jobs
         self.precision = precision
         self.rho, self.g = rho, g
@@ -186,8 +223,8 @@ class BEMServer:
         with self._cv:
             if self._closed:
                 raise RuntimeError("server is closed")
-            for c in chunks:
-                self._waiting.append(_Job(req, c, per_freq * len(c)))
+            for rank, c in enumerate(chunks):
+                self._waiting.append(_Job(req, c, per_freq * len(c), rank))
             self._cv.notify_all()
         return req.future
 
@@ -258,6 +295,8 @@ class BEMServer:
                 G = [g.cpu().numpy() for g in sweep.G]  # synchronises this slot's stream
                 its, res = sweep.iterations, sweep.residual
                 del sweep, bcs
+                if torch.cuda.memory_reserved(self.dev) > self.cache_limit:
+                    torch.cuda.empty_cache()
             for b, job in enumerate(batch):
                 om = job.req.omegas[job.idx]
                 job.req.forces[job.idx] = 1j * om[:, None, None] * job.req.rho * G[b]
