@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -224,10 +225,20 @@ int ensure_partition(cpt_context* h)
 {
     LuSetup& L = h->lu_streams;
     if (L.lookahead != 2 || h->part_failed) return 0;
+    // CU-masked streams only on request: on MI355X under ROCm 7.2 they are
+    // refused ("out of memory"), and creating one from a partial mask has
+    // crashed a process inside the runtime
+    const char* opt = std::getenv("CAPYTAINE_AMD_CU_MASK");
+    if (!opt || std::strcmp(opt, "1") != 0) {
+        set_error("CU-masked streams are off (CAPYTAINE_AMD_CU_MASK=1 enables them)");
+        h->part_failed = true;
+        return -1;
+    }
     int ncu = 0;
     hipError_t err = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->device);
     if (err != hipSuccess || ncu < 8) {
         set_error(std::string("CU count query failed: ") + hipGetErrorString(err));
+        h->part_failed = true;
         return -1;
     }
     const int want = std::max(1, std::min(L.panel_cus, ncu / 2));
@@ -251,8 +262,12 @@ int ensure_partition(cpt_context* h)
         h->part_failed = true;
         return -1;
     }
-    if (!L.enter && hipEventCreateWithFlags(&L.enter, hipEventDisableTiming) != hipSuccess) return -1;
-    if (!L.leave && hipEventCreateWithFlags(&L.leave, hipEventDisableTiming) != hipSuccess) return -1;
+    if ((!L.enter && hipEventCreateWithFlags(&L.enter, hipEventDisableTiming) != hipSuccess) ||
+        (!L.leave && hipEventCreateWithFlags(&L.leave, hipEventDisableTiming) != hipSuccess)) {
+        set_error("look-ahead events could not be created");
+        h->part_failed = true;
+        return -1;
+    }
     h->part_cus = want;
     return 0;
 }
@@ -261,7 +276,11 @@ int ensure_lu_streams(cpt_context* h)
 {
     // A CU-masked stream needs a hardware queue of its own; when the device
     // has none left (GPU_MAX_HW_QUEUES), mode 2 runs as mode 1.
-    if (ensure_partition(h)) h->lu_streams.part_main = nullptr;
+    if (ensure_partition(h)) {
+        h->lu_streams.part_main = nullptr;
+        // once per context: say why mode 2 runs as mode 1
+        std::fprintf(stderr, "capytaine_amd: look-ahead mode 2 runs as mode 1: %s\n", cpt_last_error());
+    }
     if (h->lu_streams.side) return 0;
     if (hipStreamCreateWithFlags(&h->lu_streams.side, hipStreamNonBlocking) != hipSuccess) return -1;
     if (hipEventCreateWithFlags(&h->lu_streams.ready, hipEventDisableTiming) != hipSuccess) return -1;

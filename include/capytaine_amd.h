@@ -283,8 +283,9 @@ enum {
                                     2048 rows, 128 from 1024, else 64) */
     CPT_TUNE_PANEL_LEAF = 1,     /* panel columns factored by one work-group per matrix, 1..64 */
     CPT_TUNE_LOOKAHEAD = 2,      /* 0: in order; 1: next panel on a side stream; 2: next panel on
-                                    CUs reserved for it (streams with CU masks; runs as 1 when the
-                                    device has no hardware queue left for them) */
+                                    CUs reserved for it (streams with CU masks, tried only when the
+                                    environment sets CAPYTAINE_AMD_CU_MASK=1; runs as 1 otherwise
+                                    or when the runtime refuses them) */
     CPT_TUNE_ASSEMBLY_CHUNK = 3, /* panels per assembly work-group, 0 = by size, else 1..1024 */
     CPT_TUNE_PANEL_CUS = 4,      /* CUs reserved for the panels when CPT_TUNE_LOOKAHEAD = 2, 1..128 */
     CPT_TUNE_GEMM_TILES = 5,     /* 1: complex64 tiled products on 128 x 128 tiles, 0: 128 x 64
