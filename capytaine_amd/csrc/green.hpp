@@ -75,11 +75,18 @@ __device__ __forceinline__ int f_int(double x)
 // Rankine part (Green_Rankine.f90)
 // ---------------------------------------------------------------------------
 
+// Integral of 1/r over a panel and its gradient.
+struct RankineInt {
+    double G, gx, gy, gz;
+};
+
 // exact_integral_of_Rankine, Green_Rankine.f90:68-157.  Gradient with respect
 // to the first variable (M); callers flip signs for the other convention.
-__device__ __noinline__ void exact_rankine(double mx, double my, double mz, const Face* __restrict__ Fp,
-                                           double& G, double& gx, double& gy, double& gz)
+// Out of line (register pressure of the caller); the result is returned by
+// value, in registers, rather than through pointers into private memory.
+__device__ __noinline__ RankineInt exact_rankine(double mx, double my, double mz, const Face* __restrict__ Fp)
 {
+    double G, gx, gy, gz;
     const Face& F = *Fp;
     const double nx = F.n[0], ny = F.n[1], nz = F.n[2];
     const double GZ = (mx - F.c[0]) * nx + (my - F.c[1]) * ny + (mz - F.c[2]) * nz;
@@ -132,6 +139,17 @@ __device__ __noinline__ void exact_rankine(double mx, double my, double mz, cons
             gz = gz + ALDEN * az - c2 * nz + c3 * lz - c4 * (c5 * az * DNT - tz * ANT);
         }
     }
+    return RankineInt{G, gx, gy, gz};
+}
+
+__device__ __forceinline__ void exact_rankine(double mx, double my, double mz, const Face* __restrict__ Fp, double& G,
+                                              double& gx, double& gy, double& gz)
+{
+    const RankineInt r = exact_rankine(mx, my, mz, Fp);
+    G = r.G;
+    gx = r.gx;
+    gy = r.gy;
+    gz = r.gz;
 }
 
 // one_point_integral_of_Rankine, Green_Rankine.f90:161-198 (first-variable gradient).
