@@ -64,6 +64,23 @@ def test_lu_solve_random(n):
     assert np.mean(ipiv[0] == piv) > 0.99
 
 
+@pytest.mark.parametrize("n,r", [(517, 40), (130, 17), (64, 33), (1, 20), (200, 16)])
+def test_lu_solve_many_rhs(n, r):
+    """More right-hand sides than one substitution pass holds (16): every
+    group of columns, the last one partial, against scipy; FP64 and mixed."""
+    from scipy.linalg import lu_factor, lu_solve
+    print(f"[gpu] LU solve n={n} nrhs={r}", flush=True)
+    rng = np.random.default_rng(n * 100 + r)
+    A, B = _random_system(rng, n, r)
+    X, info, _, _ = _factor_solve(A[None], B[None])
+    assert info[0] == 0
+    Xr = lu_solve(lu_factor(A), B)
+    assert np.linalg.norm(X[0] - Xr) / np.linalg.norm(Xr) < 1e-10
+    Xm, _, res = _mixed(A[None], B[None])
+    assert res <= 1e-12
+    assert np.linalg.norm(Xm[0] - Xr) / np.linalg.norm(Xr) < 1e-10
+
+
 def test_lu_requires_pivoting():
     """Zero leading diagonal: fails without row interchanges."""
     rng = np.random.default_rng(0)
