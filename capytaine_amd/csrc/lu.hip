@@ -114,8 +114,9 @@ constexpr int PB = 64;  // widest triangle the panel and trsm kernels hold (one 
 //       triangle L11 of the columns already factored (L11 staged in LDS, one
 //       wavefront per column, the running solution held across the lanes);
 //   (c) left-looking update of the rows below with L(rows, k0:c0) U;
-//   (d) factor the SB columns: block-wide pivot search, row interchange and
-//       broadcast through LDS, scaling and rank-1 updates in registers;
+//   (d) factor the SB columns: block-wide pivot search with the candidate
+//       rows broadcast through LDS (one barrier per column), row
+//       interchange, scaling and rank-1 updates in registers;
 //   (e) store the sub-block, then apply its interchanges to the other
 //       columns [sw_lo, sw_hi) in global memory (once per sub-block).
 // ---------------------------------------------------------------------------
@@ -205,7 +206,8 @@ __global__ void __launch_bounds__(NT) panel_kernel(cx<R>* __restrict__ Abase, in
 
     __shared__ C sL[PB * PB];  // L11, column-major (i + PB j)
     __shared__ C sU[PB * SB];
-    __shared__ C sSwap[2][2][SB];
+    __shared__ C sCand[2][NW][SB];  // per column parity: each wavefront's candidate row
+    __shared__ C sDiag[2][SB];      // and the diagonal row
     __shared__ R sBest[2][NW];
     __shared__ int sIdx[2][NW];
     __shared__ int sPiv[SB];
@@ -268,9 +270,13 @@ __global__ void __launch_bounds__(NT) panel_kernel(cx<R>* __restrict__ Abase, in
                 }
             }
         }
-        // (d) two barriers per column: the per-wave maxima, then the two
-        // interchanged rows, each in its own half of a double buffer so the
-        // next column can start while a slow wave still reads this one
+        // (d) one barrier per column: every wavefront writes its best
+        // candidate (|Re|+|Im|, row, and that row's SB values) and the owner
+        // of the diagonal row writes that row, each into the half of a
+        // double buffer of this column's parity (a wavefront can only write
+        // a half again after the next column's barrier, which every reader
+        // of this column has passed); after the barrier every thread picks
+        // the same winner and reads the pivot row directly
 #pragma unroll
         for (int s = 0; s < SB; ++s) {
             if (s < sbw) {
@@ -294,10 +300,20 @@ __global__ void __launch_bounds__(NT) panel_kernel(cx<R>* __restrict__ Abase, in
                     sBest[buf][wave] = best;
                     sIdx[buf][wave] = bidx;
                 }
+#pragma unroll
+                for (int q = 0; q < RPT; ++q) {
+                    const int t = tid + NT * q;
+                    if (t == bidx)
+#pragma unroll
+                        for (int s2 = 0; s2 < SB; ++s2) sCand[buf][wave][s2] = v[q][s2];
+                    if (t == td)
+#pragma unroll
+                        for (int s2 = 0; s2 < SB; ++s2) sDiag[buf][s2] = v[q][s2];
+                }
                 __syncthreads();
                 // every thread combines the wave maxima (same order, same result)
                 R b = sBest[buf][0];
-                int p = sIdx[buf][0];
+                int p = sIdx[buf][0], wb = 0;
 #pragma unroll
                 for (int w = 1; w < NW; ++w) {
                     const R bw = sBest[buf][w];
@@ -305,38 +321,29 @@ __global__ void __launch_bounds__(NT) panel_kernel(cx<R>* __restrict__ Abase, in
                     if (bw > b || (bw == b && iw < p)) {
                         b = bw;
                         p = iw;
+                        wb = w;
                     }
                 }
-                if (!(b > R(0))) p = td;  // zero column: no interchange, report singularity
+                const bool pivoted = b > R(0);
+                if (!pivoted) p = td;  // zero column: no interchange, report singularity
                 if (tid == 0) {
-                    if (!(b > R(0)) && info_base[blockIdx.x] == 0) info_base[blockIdx.x] = c0 + s + 1;
+                    if (!pivoted && info_base[blockIdx.x] == 0) info_base[blockIdx.x] = c0 + s + 1;
                     ipiv[c0 + s] = k0 + p;
                     sPiv[s] = p;
                 }
-                // rows td and p to LDS (slot 0: row td, slot 1: row p)
-#pragma unroll
-                for (int q = 0; q < RPT; ++q) {
-                    const int t = tid + NT * q;
-                    if (t == td)
-#pragma unroll
-                        for (int s2 = 0; s2 < SB; ++s2) sSwap[buf][0][s2] = v[q][s2];
-                    if (t == p)
-#pragma unroll
-                        for (int s2 = 0; s2 < SB; ++s2) sSwap[buf][1][s2] = v[q][s2];
-                }
-                __syncthreads();
+                // the pivot row becomes row td, the old row td goes to row p
+                const C* prow = pivoted ? sCand[buf][wb] : sDiag[buf];
 #pragma unroll
                 for (int q = 0; q < RPT; ++q) {
                     const int t = tid + NT * q;
                     if (p != td && t == td)
 #pragma unroll
-                        for (int s2 = 0; s2 < SB; ++s2) v[q][s2] = sSwap[buf][1][s2];
+                        for (int s2 = 0; s2 < SB; ++s2) v[q][s2] = prow[s2];
                     if (p != td && t == p)
 #pragma unroll
-                        for (int s2 = 0; s2 < SB; ++s2) v[q][s2] = sSwap[buf][0][s2];
+                        for (int s2 = 0; s2 < SB; ++s2) v[q][s2] = sDiag[buf][s2];
                 }
-                // the pivot row is the new row td: slot 1
-                const C piv = sSwap[buf][1][s];
+                const C piv = prow[s];
                 const bool nonzero = (piv.re != R(0) || piv.im != R(0));
                 const C r = nonzero ? zrcp(piv) : C{R(1), R(0)};
 #pragma unroll
@@ -346,7 +353,7 @@ __global__ void __launch_bounds__(NT) panel_kernel(cx<R>* __restrict__ Abase, in
                         const C l = nonzero ? zmul(v[q][s], r) : v[q][s];
                         v[q][s] = l;
 #pragma unroll
-                        for (int s2 = s + 1; s2 < SB; ++s2) v[q][s2] = zsub(v[q][s2], zmul(l, sSwap[buf][1][s2]));
+                        for (int s2 = s + 1; s2 < SB; ++s2) v[q][s2] = zsub(v[q][s2], zmul(l, prow[s2]));
                     }
                 }
             }
