@@ -15,12 +15,17 @@
 //
 // Right-looking blocked algorithm (block width 256 from 2048 rows, 128 from
 // 1024, else 64; tunable), one launch per step for the whole batch:
-//   1. panel: recursive halving down to 16-column leaves factored by one
-//      work-group per matrix, the halves joined by trsm + MFMA GEMM;
+//   1. panel: recursive halving down to 8- or 16-column leaves factored by
+//      one work-group per matrix, the halves joined by trsm + MFMA GEMM;
 //   2. row interchanges applied to the columns outside the panel;
-//   3. U12 = L11^-1 A12 in place (wavefront-per-column triangular solves);
+//   3. U12 = L11^-1 A12 in place: L11's 64 x 64 diagonal blocks inverted in
+//      one launch, each block row multiplied by its inverse on MFMA, the
+//      rows below updated by the MFMA GEMM;
 //   4. A22 -= L21 U12 with the MFMA complex GEMM, the next panel's columns
-//      first so that the next panel is factored while the rest is updated.
+//      first so that, with a side stream, the next panel can be factored
+//      while the rest is updated.
+// The solve inverts the factor's 64 x 64 diagonal blocks once and runs one
+// substitution launch per block and direction (MFMA products only).
 #include <hip/hip_runtime.h>
 #include <climits>
 #include <cmath>
@@ -436,13 +441,14 @@ __global__ void laswp_kernel(cx<R>* __restrict__ Abase, int64_t lda, int64_t str
 }
 
 // ---------------------------------------------------------------------------
-// 3. Triangular solve with a kb x kb diagonal block T (kb <= PB = 64), in
-//    place on the kb rows of B.  One wavefront per group of columns of B,
-//    lane i holds x_i of TRSM_ILP columns at once; x_j is broadcast with
-//    v_readlane (j is wave-uniform), T(i, j) comes from LDS, where the
-//    triangle is staged once per work-group (loads batched four per thread)
-//    and reused for TRSM_WAVES x TRSM_CPW columns.
-//    UPPER = false: T unit lower triangular; UPPER = true: T upper, non-unit.
+// 3. Triangular solves with a kb x kb diagonal block T (kb <= PB = 64).
+//    stage_triangle / solve_triangle: lane i holds x_i of TRSM_ILP columns
+//    at once; x_j is broadcast with v_readlane (j is wave-uniform), T(i, j)
+//    comes from LDS, where the triangle is staged once per work-group (loads
+//    batched four per thread).  UPPER = false: T unit lower triangular;
+//    UPPER = true: T upper, non-unit (the inversion of 3b).  trsm_kernel:
+//    L^-1 B in place on the kb rows of B, one wavefront per group of
+//    columns, the triangle reused for TRSM_WAVES x TRSM_CPW columns.
 // ---------------------------------------------------------------------------
 constexpr int TRSM_WAVES = 4, TRSM_CPW = 8, TRSM_ILP = 4;
 
@@ -509,7 +515,7 @@ __device__ __forceinline__ void solve_triangle(const cx<R>* sT, int kb, int lane
     }
 }
 
-template <typename R, bool UPPER>
+template <typename R>
 __global__ void __launch_bounds__(64 * TRSM_WAVES) trsm_kernel(const cx<R>* __restrict__ Tbase, int64_t ldt,
                                                                int64_t strideT, int kb, cx<R>* __restrict__ Bbase,
                                                                int64_t ldb, int64_t strideB, int ncols)
@@ -517,7 +523,7 @@ __global__ void __launch_bounds__(64 * TRSM_WAVES) trsm_kernel(const cx<R>* __re
     typedef cx<R> C;
     __shared__ C sT[PB * PB];
     const C zero{R(0), R(0)};
-    stage_triangle<R, UPPER, 64 * TRSM_WAVES>(Tbase + (size_t)blockIdx.y * strideT, ldt, kb, sT);
+    stage_triangle<R, false, 64 * TRSM_WAVES>(Tbase + (size_t)blockIdx.y * strideT, ldt, kb, sT);
     const int lane = threadIdx.x & 63;
     const int cbase = (blockIdx.x * TRSM_WAVES + (threadIdx.x >> 6)) * TRSM_CPW;
     C* Bm = Bbase + (size_t)blockIdx.y * strideB;
@@ -528,7 +534,7 @@ __global__ void __launch_bounds__(64 * TRSM_WAVES) trsm_kernel(const cx<R>* __re
 #pragma unroll
         for (int q = 0; q < TRSM_ILP; ++q)
             x[q] = ld_or_zero(Bm, lane + (size_t)(col0 + q) * ldb, col0 + q < ncols && lane < kb);
-        solve_triangle<R, UPPER>(sT, kb, lane, x);
+        solve_triangle<R, false>(sT, kb, lane, x);
         if (lane < kb) {
 #pragma unroll
             for (int q = 0; q < TRSM_ILP; ++q)
@@ -814,12 +820,12 @@ substitution_kernel(const cx<R>* __restrict__ LUbase, int64_t lda, int64_t strid
     }
 }
 
-template <typename R, bool UPPER>
+template <typename R>
 int launch_trsm(const cx<R>* T, int64_t ldt, int64_t sT, int kb, cx<R>* B, int64_t ldb, int64_t sB, int ncols,
                 int batch, hipStream_t stream)
 {
     constexpr int per_block = TRSM_WAVES * TRSM_CPW;
-    hipLaunchKernelGGL((trsm_kernel<R, UPPER>), dim3((ncols + per_block - 1) / per_block, batch),
+    hipLaunchKernelGGL((trsm_kernel<R>), dim3((ncols + per_block - 1) / per_block, batch),
                        dim3(64 * TRSM_WAVES), 0, stream, T, ldt, sT, kb, B, ldb, sB, ncols);
     return (int)hipGetLastError();
 }
@@ -1275,7 +1281,7 @@ int trsm_lower(const cx<R>* L, int64_t lda, int64_t sA, int kb, cx<R>* B, int64_
             hipLaunchKernelGGL(apply_inverse_kernel<R>, dim3((ncols + APPLY_COLS - 1) / APPLY_COLS, batch), dim3(256),
                                0, stream, inv + j0, (int64_t)kb, sInv, jb, B + j0, ldb, sB, ncols);
             if ((e = (int)hipGetLastError())) return e;
-        } else if ((e = launch_trsm<R, false>(L + j0 + (size_t)j0 * lda, lda, sA, jb, B + j0, ldb, sB, ncols, batch,
+        } else if ((e = launch_trsm<R>(L + j0 + (size_t)j0 * lda, lda, sA, jb, B + j0, ldb, sB, ncols, batch,
                                               stream))) {
             return e;
         }
