@@ -16,8 +16,10 @@ difference of any coefficient between the two forms are reported beside it.
 
 Prints one JSON line (see the repository's bench contract).
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline] [--no-fleet] [--no-serving]
+                    [--no-post]
     python bench.py --workload cylinders     # 20000-panel finite-depth array instead
     python bench.py --workload serving       # the request-stream measurement alone
+    python bench.py --workload post          # the post-processing maps alone
 
 The request-stream measurement runs in a child process (started after the
 headline and fleet legs have released their device memory), so that a
@@ -266,6 +268,78 @@ def serving_child(timeout_s=900):
     return json.loads(lines[-1])
 
 
+def post_processing_benchmark(gf, dev, mesh, body, headings, steps=3, rho=1025.0, g=9.81, omega=1.0,
+                              cpu_points=16384, log=None):
+    """Free-surface elevation map and velocity section of one frequency's
+    problems (6 dof + 8 headings) around the barge, through BEMSolver's
+    post-processing engine (points as collocation, products on the device).
+
+    Elevation: 0.2 m grid over 80 m x 40 m at z = 0 minus the barge
+    footprint, all 14 source vectors at once.  Velocity: 0.4 m grid on the
+    vertical plane y = 6 m, x in [-40, 40], z in [-20, -0.2].  Times are per
+    map, host points in and host values out; the CPU sample is the oracle's
+    S and K for `cpu_points` points of the map (OpenMP on the host cores)."""
+    import torch
+    import capytaine_amd as cpa
+    from capytaine_amd import _lib
+    solver = cpa.BEMSolver(engine=cpa.DefaultMatrixEngine(green_function=gf))
+    pbs = [cpa.RadiationProblem(body=body, radiating_dof=d, omega=omega, rho=rho, g=g) for d in body.dofs]
+    pbs += [cpa.DiffractionProblem(body=body, omega=omega, wave_direction=b, rho=rho, g=g) for b in headings]
+    results = solver.solve_all(pbs, keep_details=True)
+    src = np.stack([r.sources for r in results], axis=1)
+    gp = dict(free_surface=0.0, water_depth=np.inf, wavenumber=pbs[0].wavenumber)
+    x, y = np.meshgrid(np.arange(-40.0, 40.0 + 1e-9, 0.2), np.arange(-20.0, 20.0 + 1e-9, 0.2), indexing="ij")
+    keep = (np.abs(x) > 10.05) | (np.abs(y) > 4.05)
+    fs_pts = np.stack([x[keep], y[keep], np.zeros(keep.sum())], axis=1)
+    xs, zs = np.meshgrid(np.arange(-40.0, 40.0 + 1e-9, 0.4), np.arange(-20.0, -0.2 + 1e-9, 0.4), indexing="ij")
+    vel_pts = np.stack([xs.ravel(), np.full(xs.size, 6.0), zs.ravel()], axis=1)
+    n = mesh.nb_faces
+    ctx = _lib.Context.get(dev)
+    out = {}
+    for name, pts, gradient in (("elevation_map", fs_pts, False), ("velocity_section", vel_pts, True)):
+        solver.engine.field_at_points(pts, mesh, src, gradient=gradient, **gp)  # warm-up
+        torch.cuda.synchronize()
+        _lib.profile_begin(ctx)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            val = solver.engine.field_at_points(pts, mesh, src, gradient=gradient, **gp)
+        dt = (time.perf_counter() - t0) / steps
+        kern = _lib.profile_end(ctx)
+        if not gradient:
+            val = 1j * omega / g * val  # compute_free_surface_elevation
+        asm = kern["assembly"]
+        prod = {k: v for k, v in kern.items() if k in ("zgemm", "skinny_z") and v["launches"]}
+        out[name] = {"points": len(pts), "panels": n, "source_vectors": src.shape[1], "ms_per_map": dt * 1e3,
+                     "points_per_s": len(pts) / dt, "panel_pairs_per_s": len(pts) * n / dt,
+                     "assembly_ms_per_map": asm["ms"] / steps,
+                     "assembly_panel_pairs_per_s": asm["work"] / (asm["ms"] / 1e3) if asm["ms"] else None,
+                     "product_ms_per_map": sum(v["ms"] for v in prod.values()) / steps,
+                     "finite": bool(np.all(np.isfinite(val)))}
+        if log:
+            log(f"[post] {name}: {len(pts)} points x {n} panels, {dt * 1e3:.1f} ms per map")
+    out["elevation_map"]["grid"] = "0.2 m over [-40, 40] x [-20, 20] m at z = 0, barge footprint removed"
+    out["velocity_section"]["grid"] = "0.4 m over x in [-40, 40], z in [-20, -0.2] m, y = 6 m"
+    # CPU: the oracle's S, K at a bounded sample of the map's points
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import Oracle, mesh_arrays
+    orc = Oracle()
+    sample = fs_pts[np.linspace(0, len(fs_pts) - 1, cpu_points).round().astype(int)]
+    arr = dict(mesh_arrays(mesh), colloc=sample, dpn=np.zeros_like(sample))
+    table = (gf.tabulated_r_range, gf.tabulated_z_range, gf.tabulated_integrals)
+    t0 = time.perf_counter()
+    S, _ = orc.build_matrices(**arr, wavenumber=gp["wavenumber"], depth=np.inf, table=table)
+    S @ src
+    dt = time.perf_counter() - t0
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    out["cpu_baseline"] = {"panel_pairs_per_s": cpu_points * n / dt, "cores": threads, "kind": "port",
+                           "sample": f"{cpu_points} points of the elevation map x {n} panels, oracle S and K "
+                                     f"+ product, {dt:.1f} s",
+                           "ms_per_elevation_map_extrapolated": dt / cpu_points * len(fs_pts) * 1e3}
+    del results, solver
+    torch.cuda.empty_cache()
+    return out
+
+
 def copy_calibration(nbytes=1 << 30, reps=4):
     """Element-wise copy of `nbytes` (float32, y = 1 * x) `reps` times: the
     HBM counters of this kernel (FETCH_SIZE, WRITE_SIZE) against the known
@@ -360,9 +434,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fleet", action="store_true", help="skip the 64-hull batched measurement")
     ap.add_argument("--no-serving", action="store_true", help="skip the request-stream latency measurement")
+    ap.add_argument("--no-post", action="store_true", help="skip the post-processing (elevation, velocity) maps")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner (a stored tuning of this size class is still applied)")
-    ap.add_argument("--workload", choices=("barge", "cylinders", "copy", "serving"), default="barge",
+    ap.add_argument("--workload", choices=("barge", "cylinders", "copy", "serving", "post"), default="barge",
                     help="barge: the headline line (plus fleet and serving); cylinders: the 20000-panel "
                          "finite-depth array; copy: a 1 GiB device copy, the calibration of the HBM "
                          "byte counters; serving: the request-stream measurement alone")
@@ -385,6 +460,17 @@ def main():
         dev = torch.device("cuda", 0)
         torch.cuda.set_device(dev)
         res = serving_benchmark(cpa.Delhommeau(device=dev), dev, log=lambda m: print(m, file=sys.stderr, flush=True))
+        print(json.dumps(res), flush=True)
+        return
+
+    if args.workload == "post":
+        import torch
+        import capytaine_amd as cpa
+        dev = torch.device("cuda", 0)
+        torch.cuda.set_device(dev)
+        mesh, body, _, headings = barge_workload()
+        res = post_processing_benchmark(cpa.Delhommeau(device=dev), dev, mesh, body, headings, steps=args.steps,
+                                        log=lambda m: print(m, file=sys.stderr, flush=True))
         print(json.dumps(res), flush=True)
         return
 
@@ -560,6 +646,9 @@ def main():
         "roofline": roofline,
         "tuning": {k: {kk: vv for kk, vv in v.items() if kk != "timings_ms"} for k, v in tuning.items()},
     }
+    if not args.no_post:
+        out["post_processing"] = post_processing_benchmark(gf, dev, mesh, body, headings, rho=rho, g=g,
+                                                           log=lambda m: print(m, file=sys.stderr, flush=True))
     if not args.no_fleet:
         out["fleet"] = fleet_benchmark(gf, dev, log=lambda m: print(m, file=sys.stderr, flush=True))
     if not args.no_serving:

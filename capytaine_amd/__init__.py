@@ -13,7 +13,8 @@ from .meshes import (Mesh, mesh_sphere, mesh_parallelepiped, mesh_rectangle, mes
 from .green_functions import Delhommeau, XieDelhommeau, GreenFunctionEvaluationError
 from .engines import DefaultMatrixEngine, BasicMatrixEngine, DeviceMatrix, LUDecomposedDeviceMatrix
 from .problems import (FloatingBody, RadiationProblem, DiffractionProblem, rigid_body_dofs,
-                       TranslationDof, RotationDof, froude_krylov_force, airy_waves_velocity)
+                       TranslationDof, RotationDof, froude_krylov_force, airy_waves_velocity,
+                       airy_waves_potential, airy_waves_pressure, airy_waves_free_surface_elevation)
 from .solver import BEMSolver, problems_for_body
 from ._lib import CptError
 

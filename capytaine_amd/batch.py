@@ -90,7 +90,7 @@ class BatchedSweep:
             self.info = [torch.empty((len(k),), dtype=torch.int32, device=dev) for k in self.ks]
         self.W, self.G = [], []
         for b, mesh, k, r in zip(self.bodies, self.meshes, self.ks, self.nrhs):
-            W = -(b.dof_normal_matrix() * mesh.faces_areas[None, :]).T
+            W = -(b.dof_normal_matrix(including_lid=True) * mesh.faces_areas[None, :]).T
             self.W.append(torch.as_tensor(np.ascontiguousarray(W), dtype=cplx).to(dev))
             self.G.append(torch.empty((len(k), r, W.shape[1]), dtype=cplx, device=dev))
         # a private context (own workspace) lets several sweeps run at once

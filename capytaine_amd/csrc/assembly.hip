@@ -262,7 +262,8 @@ __device__ __forceinline__ void assemble_tile(const AssemblyArgs& A, int iblk, i
             // its reflected form), with the FP64 path's expressions
             const double r0 = norm3(x0 - cx, x1 - cy, x2 - cz);
             const double r1 = sqrt((x0 - cx) * (x0 - cx) + (x1 - cy) * (x1 - cy) + (-x2 - cz) * (-x2 - cz));
-            if (r0 > 7 * radius && r1 > 7 * radius) {
+            if (far_direct(x0 - cx, x1 - cy, x2 - cz, r0, radius) &&
+                far_image(x0 - cx, x1 - cy, -x2 - cz, r1, radius)) {
                 far_pair_f32(A, I, J, active, x0, x1, x2, cx, cy, cz, area, qp, nx, ny, nz, deriv_first);
                 continue;
             }

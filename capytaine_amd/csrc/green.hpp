@@ -57,6 +57,67 @@ struct cplx { double re, im; };
 
 __device__ __forceinline__ double norm3(double x, double y, double z) { return sqrt(x * x + y * y + z * z); }
 
+// NORM2 as gfortran evaluates it: a scaled sum of squares, every operation
+// rounded on its own (no contraction), scale * sqrt(sum) at the end.
+__device__ __noinline__ double norm2_rounded(double x, double y, double z)
+{
+#pragma clang fp contract(off)
+    const double v[3] = {x, y, z};
+    double scale = 0.0, ssq = 1.0;
+    for (int i = 0; i < 3; ++i) {
+        if (v[i] != 0.0) {
+            const double a = fabs(v[i]);
+            if (scale < a) {
+                const double t = scale / a;
+                ssq = 1.0 + ssq * t * t;
+                scale = a;
+            } else {
+                const double t = a / scale;
+                ssq += t * t;
+            }
+        }
+    }
+    return scale * sqrt(ssq);
+}
+
+// sqrt(a^2 + b^2) correctly rounded (a, b of moderate size): the sum of
+// squares as a double-double (FMA residuals), then one Newton correction of
+// its rounded square root.
+__device__ __noinline__ double hypot_rounded(double a, double b)
+{
+#pragma clang fp contract(off)
+    const double p = a * a, pe = fma(a, a, -p);
+    const double q = b * b, qe = fma(b, b, -q);
+    const double s = p + q, t = s - p;
+    const double lo = ((p - (s - t)) + (q - t)) + pe + qe;
+    const double r = sqrt(s);
+    if (r == 0.0) return 0.0;
+    return r + (fma(-r, r, s) + lo) / (2.0 * r);
+}
+
+// The near-field tests r0 > 7 R and r1 > 7 R (Green_Rankine.f90:48-50 and
+// :224-227).  On regular meshes 7 R often equals a centre-to-centre distance
+// exactly, and the last bit of the reference's distance then picks the
+// branch (the exact and one-point integrals differ by ~1e-5 there).  The
+// fast distance r is within a few ulps of the reference's; inside that
+// window the test is taken again on the reference's expression: NORM2 of
+// the difference for the direct term, HYPOT of the horizontal NORM2 and the
+// vertical distance for the image (HYPOT correctly rounded; a C library's
+// hypot can be one ulp off in rare cases).
+__device__ __forceinline__ bool far_direct(double dx, double dy, double dz, double r, double radius)
+{
+    const double thr = 7 * radius;
+    if (fabs(r - thr) > 16 * 2.220446049250313e-16 * thr) return r > thr;
+    return norm2_rounded(dx, dy, dz) > thr;
+}
+
+__device__ __forceinline__ bool far_image(double dx, double dy, double dz, double r, double radius)
+{
+    const double thr = 7 * radius;
+    if (fabs(r - thr) > 16 * 2.220446049250313e-16 * thr) return r > thr;
+    return hypot_rounded(norm2_rounded(dx, dy, 0.0), dz) > thr;
+}
+
 // Fortran NINT (half away from zero) and INT (truncation); values outside
 // the int range give INT_MIN like the reference's x86 conversion, which the
 // callers then clamp.
@@ -179,7 +240,7 @@ __device__ __forceinline__ void rankine(double mx, double my, double mz, const F
                                         double& G, double& gx, double& gy, double& gz)
 {
     const double r0 = norm3(mx - cx, my - cy, mz - cz);
-    if (r0 > 7 * radius) one_point_rankine(mx, my, mz, cx, cy, cz, area, G, gx, gy, gz);
+    if (far_direct(mx - cx, my - cy, mz - cz, r0, radius)) one_point_rankine(mx, my, mz, cx, cy, cz, area, G, gx, gy, gz);
     else exact_rankine(mx, my, mz, Fp, G, gx, gy, gz);
     if (!deriv_first) {
         gx = -gx;
@@ -202,7 +263,7 @@ __device__ __forceinline__ void reflected_rankine(double mx, double my, double m
     if (!one_point) {
         const double dx = mx - cx, dy = my - cy;
         const double r1 = sqrt(dx * dx + dy * dy + (rz - cz) * (rz - cz));
-        one_point = r1 > 7 * radius;
+        one_point = far_image(dx, dy, rz - cz, r1, radius);
     }
     if (one_point) one_point_rankine(mx, my, rz, cx, cy, cz, area, G, gx, gy, gz);
     else exact_rankine(mx, my, rz, Fp, G, gx, gy, gz);

@@ -5,7 +5,9 @@ Drop-in for ``capytaine.bem.engines.DefaultMatrixEngine``
 last-computed cache keyed by (mesh1, mesh2, gf_params) (:258-298),
 ``linear_solver(A, b)`` with the LU factorisation cached in place of K
 (:300-346), ``build_S_matrix`` / ``build_fullK_matrix`` (:142-176),
-``compute_ram_estimation`` (:348-398) and the NaN check (:400-405).  The
+``compute_ram_estimation`` (:348-398) and the NaN check (:400-405), plus
+``field_at_points``: the post-processing products S @ sources and
+grad G @ sources at points, evaluated chunk by chunk on the device.  The
 matrices it returns are :class:`DeviceMatrix` objects: they expose
 ``shape``, ``dtype``, ``@`` and ``__array__`` (a Fortran-ordered numpy copy),
 which is all the solver needs from them.
@@ -251,6 +253,77 @@ class DefaultMatrixEngine:
         _, fullK = self.green_function.evaluate(mesh1, mesh2.merged(), **gf_params)
         check_if_nan_in_matrix([fullK])
         return fullK
+
+    def field_at_points(self, points, mesh, sources, *, gradient=False, max_chunk_bytes=None, **gf_params):
+        """The products build_S_matrix(points, mesh) @ sources (gradient
+        False) or build_fullK_matrix(points, mesh)[c] @ sources for c = 0, 1, 2
+        (gradient True) of the post-processing (solver.py:446-479), formed on
+        the device: the points are evaluated in chunks of at most
+        max_chunk_bytes of S and K (default: a quarter of the free device
+        memory, at most 8 GiB), each chunk's product taken at once by
+        cpt_zgemm, so the matrices never reach host memory.
+
+        points: (N, 3) array, or a Mesh whose face centres are the points;
+            with gradient, a Mesh also brings the diagonal term of
+            build_fullK_matrix (matrices.f90:329-361): half (one on the free
+            surface) its face normal for the faces i < min(N, faces) of
+            `mesh`, times the source of face i.
+        sources: (faces,) or (faces, r) complex.
+        Returns (N,) or (N, r) for the potential, (N, 3) or (N, r, 3) for
+        the gradient, as numpy arrays.
+        """
+        import torch
+        gf = self.green_function
+        mesh = mesh.merged()
+        normals = None
+        if hasattr(points, "faces_centers"):
+            normals = points.faces_normals if gradient else None
+            points = points.faces_centers
+        points = np.ascontiguousarray(points, dtype=np.float64)
+        src = np.asarray(sources, dtype=np.complex128)
+        single = src.ndim == 1
+        src = src.reshape(src.shape[0], -1)
+        n, m, r = points.shape[0], mesh.nb_faces, src.shape[1]
+        if src.shape[0] != m:
+            raise ValueError(f"{src.shape[0]} sources for a mesh of {m} faces")
+        dev = gf.device
+        nk = 3 if gradient else 1
+        per_point = 16 * m * (1 + nk)
+        if max_chunk_bytes is None:
+            free, _ = torch.cuda.mem_get_info(dev)
+            max_chunk_bytes = min(8 << 30, free // 4)
+        chunk = int(max(1, min(n, max_chunk_bytes // per_point)))
+        chunk = -(-n // -(-n // chunk))  # chunks of equal size: the product's grid is one row slab per chunk
+        S_buf = torch.empty(m * chunk, dtype=torch.complex128, device=dev)
+        K_buf = torch.empty(m * chunk * nk, dtype=torch.complex128, device=dev)
+        X = torch.as_tensor(np.ascontiguousarray(src.T), dtype=torch.complex128).to(dev)[None]  # (1, r, m)
+        P = torch.as_tensor(points).to(dev)  # one upload: a pageable copy per chunk would wait for the device
+        out = torch.empty((r, n, nk), dtype=torch.complex128, device=dev)
+        for p0 in range(0, n, chunk):
+            nc = min(chunk, n - p0)
+            S = S_buf[:m * nc].view(1, m, nc)
+            K = K_buf[:m * nc * nk].view((1, m, nc) if nk == 1 else (1, m, nc, 3))
+            gf.evaluate_on_device(P[p0:p0 + nc], mesh, adjoint_double_layer=True, early_dot_product=not gradient,
+                                  diagonal_term_in_double_layer=False, out=(S, K), **gf_params)
+            A = K.view(1, m, nc * 3) if gradient else S
+            out[:, p0:p0 + nc, :] = DeviceMatrix(A).matmul_device(X)[0].view(r, nc, nk)
+        if self.check_nan:
+            # a NaN entry of the matrix reaches every product it enters (NaN
+            # times any source is NaN): one scan of the (r, N) result instead
+            # of the (faces, N) matrix
+            check_if_nan_in_matrix([out])
+        if normals is not None:
+            nd = min(n, m)
+            fs = gf_params.get("free_surface", 0.0)
+            on_fs = np.abs(mesh.faces_centers[:nd, 2] - fs) < float(np.float32(1e-8))
+            coef = np.where(on_fs, 1.0, 0.5)[:, None] * normals[:nd]  # (nd, 3)
+            out[:, :nd, :] += (torch.as_tensor(coef, device=dev)[None] * X[0, :, :nd, None])
+        host = torch.empty(out.shape, dtype=out.dtype, pin_memory=True)
+        host.copy_(out)
+        res = host.numpy()
+        res = res[..., 0] if not gradient else res
+        res = np.moveaxis(res, 0, 1)  # (N, r) or (N, r, 3)
+        return res[:, 0] if single else res
 
     def _build(self, mesh1, mesh2, *, diagonal_term_in_double_layer=True, **gf_params):
         gf_params.setdefault("early_dot_product", True)

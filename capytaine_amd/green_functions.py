@@ -391,16 +391,31 @@ class Delhommeau(AbstractGreenFunction):
         if free_surface == np.inf and water_depth != np.inf:
             raise ValueError("When setting free_surface=inf, the water depth should also be infinite "
                              f"(got water_depth={water_depth})")
-        pts, dpn = self._get_colocation_points_and_normals(mesh1, mesh2, adjoint_double_layer)
-        n, m = pts.shape[0], mesh2.nb_faces
         dev = self.device
         md = mesh2.device_arrays(dev)
-        if mesh1 is mesh2:
-            pts_d = md["centers"]
-            dpn_d = md["normals"]
+        m = mesh2.nb_faces
+        if isinstance(mesh1, torch.Tensor):
+            # points already in device memory, (n, 3) float64: collocation
+            # points of the post-processing, with zero normals (only S and the
+            # full gradient are meaningful then, abstract_green_function.py:46-53)
+            if not (mesh1.dtype == torch.float64 and mesh1.dim() == 2 and mesh1.shape[1] == 3
+                    and mesh1.is_contiguous() and mesh1.device == torch.device(dev)):
+                raise ValueError("device points must be a contiguous (n, 3) float64 tensor on the Green "
+                                 "function's device")
+            if not adjoint_double_layer:
+                raise ValueError("device points need adjoint_double_layer=True")
+            n = mesh1.shape[0]
+            pts_d = mesh1
+            dpn_d = torch.zeros_like(mesh1)
         else:
-            pts_d = torch.as_tensor(np.ascontiguousarray(pts), dtype=torch.float64).to(dev)
-            dpn_d = torch.as_tensor(np.ascontiguousarray(dpn), dtype=torch.float64).to(dev)
+            pts, dpn = self._get_colocation_points_and_normals(mesh1, mesh2, adjoint_double_layer)
+            n = pts.shape[0]
+            if mesh1 is mesh2:
+                pts_d = md["centers"]
+                dpn_d = md["normals"]
+            else:
+                pts_d = torch.as_tensor(np.ascontiguousarray(pts), dtype=torch.float64).to(dev)
+                dpn_d = torch.as_tensor(np.ascontiguousarray(dpn), dtype=torch.float64).to(dev)
         gfs = np.array([_SING[self._gf_singularities_for(water_depth, float(k), mesh2)] for k in ks],
                        dtype=np.int32)
         if water_depth == np.inf or free_surface == np.inf:

@@ -20,6 +20,8 @@ serving workloads (the reference has no ellipsoid mesher).
 """
 import threading
 
+import logging
+
 import numpy as np
 
 __all__ = ["Mesh", "mesh_sphere", "mesh_parallelepiped", "mesh_rectangle", "mesh_ellipsoid",
@@ -27,6 +29,9 @@ __all__ = ["Mesh", "mesh_sphere", "mesh_parallelepiped", "mesh_rectangle", "mesh
 
 
 _device_cache_lock = threading.Lock()
+
+
+LOG = logging.getLogger(__name__)
 
 
 def _cross(a, b):
@@ -159,11 +164,24 @@ class Mesh:
     def merged(self):
         return self
 
+    def with_normal_vector_going_down(self):
+        """meshes.py:767-798: a horizontal mesh whose normals all point up is
+        returned with its faces reversed (normals down); any other mesh is
+        returned as is.  Triangles keep their repeated vertex last."""
+        if not np.allclose(self.faces_normals[:, 2], np.ones(self.nb_faces)):
+            return self
+        LOG.warning("Inverting the direction of the normal vectors of %s to be downward.", self)
+        quad = self.faces[:, 2] != self.faces[:, 3]
+        faces = np.where(quad[:, None], self.faces[:, ::-1], self.faces[:, [2, 1, 0, 0]])
+        return Mesh(self.vertices, faces, quadrature_method=self.quadrature_method, name=self.name)
+
     def join(self, other, name=None):
         """Union of two meshes (faces of self first)."""
         faces = np.concatenate([self.faces, other.faces + self.nb_vertices])
+        same = self.quadrature_method == other.quadrature_method  # else dropped, as meshes.py:620-624
         return Mesh(np.concatenate([self.vertices, other.vertices]), faces,
-                    quadrature_method=self.quadrature_method, name=name or f"{self.name}+{other.name}")
+                    quadrature_method=self.quadrature_method if same else None,
+                    name=name or f"{self.name}+{other.name}")
 
     def __add__(self, other):
         return self.join(other)

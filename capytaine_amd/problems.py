@@ -54,20 +54,43 @@ def rigid_body_dofs(only=None, rotation_center=None):
 
 
 class FloatingBody:
-    """A mesh with degrees of freedom (bodies/bodies.py:85-250, no lid)."""
+    """A mesh with degrees of freedom and an optional lid (bodies/bodies.py:
+    85-250).
 
-    def __init__(self, mesh, dofs=None, name=None):
+    lid_mesh: panels of the interior free surface (or just below it) that
+    remove the irregular frequencies; a lid whose normals all point up is
+    turned over (bodies.py:91-99).  The BEM problem is set on
+    ``mesh_including_lid`` (hull faces first); ``hull_mask`` selects the
+    hull's faces, on which the boundary conditions and the forces live."""
+
+    def __init__(self, mesh, dofs=None, name=None, *, lid_mesh=None):
         self.mesh = mesh
         self.dofs = {} if dofs is None else dict(dofs)
         self.name = name or mesh.name
+        if lid_mesh is not None and lid_mesh.nb_faces == 0:
+            LOG.warning("Lid mesh %s provided for body initialization is empty. The lid mesh is ignored.", lid_mesh)
+            lid_mesh = None
+        self.lid_mesh = None if lid_mesh is None else lid_mesh.with_normal_vector_going_down()
+        if self.lid_mesh is None:
+            self.mesh_including_lid = mesh
+        else:
+            self.mesh_including_lid = mesh.join(self.lid_mesh, name=f"{mesh.name}+{self.lid_mesh.name}")
+        self.hull_mask = np.arange(self.mesh_including_lid.nb_faces) < mesh.nb_faces
 
-    @property
-    def mesh_including_lid(self):
-        return self.mesh
-
-    @property
-    def hull_mask(self):
-        return np.ones(self.mesh.nb_faces, dtype=bool)
+    def first_irregular_frequency_estimate(self, *, g=9.81):
+        """bodies.py:566-601: lowest irregular frequency of a box of the
+        body's horizontal spans and draft (the lid's depth when there is a
+        lid; infinite for a lid on the free surface), for a body of one
+        connected waterplane."""
+        if self.lid_mesh is None:
+            draft = abs(self.mesh.vertices[:, 2].min())
+        else:
+            draft = abs(self.lid_mesh.vertices[:, 2].min())
+            if draft < 1e-6:
+                return np.inf
+        v = self.mesh.vertices
+        p = np.hypot(1 / (v[:, 0].max() - v[:, 0].min()), 1 / (v[:, 1].max() - v[:, 1].min()))
+        return float(np.sqrt(np.pi * g * p / np.tanh(np.pi * draft * p)))
 
     def add_all_rigid_body_dofs(self, rotation_center=None):
         self.dofs.update(rigid_body_dofs(rotation_center=rotation_center))
@@ -77,10 +100,17 @@ class FloatingBody:
         dof = self.dofs[dof_name]
         return dof.evaluate_motion(self.mesh) if hasattr(dof, "evaluate_motion") else np.asarray(dof)
 
-    def dof_normal_matrix(self):
-        """(ndof, m) matrix of dof . n on each face, dofs in insertion order."""
+    def dof_normal_matrix(self, including_lid=False):
+        """(ndof, faces) matrix of dof . n on each face of the hull, dofs in
+        insertion order; with including_lid, over mesh_including_lid with
+        zero columns on the lid."""
         n = self.mesh.faces_normals
-        return np.array([np.sum(self.dof_motion(d) * n, axis=1) for d in self.dofs])
+        dn = np.array([np.sum(self.dof_motion(d) * n, axis=1) for d in self.dofs]).reshape(len(self.dofs), -1)
+        if not including_lid or self.lid_mesh is None:
+            return dn
+        out = np.zeros((len(self.dofs), self.mesh_including_lid.nb_faces))
+        out[:, self.hull_mask] = dn
+        return out
 
     def integrate_pressure(self, pressure):
         """bodies.py:238-250: force on each dof, F = -sum p (dof . n) A."""
@@ -136,35 +166,60 @@ class LinearPotentialFlowProblem:
         return (id(self.body), self.water_depth, self.omega, self.rho, self.g, self.free_surface)
 
 
+def _airy_depth_terms(k, h, z):
+    """cosh(k(z+h))/cosh(kh) and sinh(k(z+h))/cosh(kh), or exp(kz) for both
+    from kh >= 20 on (airy_waves.py:40-46)."""
+    if 0 <= k * h < 20:
+        return np.cosh(k * (z + h)) / np.cosh(k * h), np.sinh(k * (z + h)) / np.cosh(k * h)
+    return np.exp(k * z), np.exp(k * z)
+
+
 def airy_waves_velocity(points, pb):
-    """airy_waves.py:56-94."""
-    x, y, z = np.asarray(points).T
+    """airy_waves.py:56-94: velocity of the undisturbed incident wave,
+    shape (*output_shape, 3); NaN at zero or infinite wavenumber."""
+    from .lists_of_points import _normalize_points
+    points, output_shape = _normalize_points(points)
+    if float(pb.wavenumber) in (0.0, np.inf):
+        return np.nan * np.ones((*output_shape, 3))
+    x, y, z = points.T
     k, h, beta = pb.wavenumber, pb.water_depth, pb.encounter_wave_direction
     wbar = x * np.cos(beta) + y * np.sin(beta)
-    if 0 <= k * h < 20:
-        cih = np.cosh(k * (z + h)) / np.cosh(k * h)
-        sih = np.sinh(k * (z + h)) / np.cosh(k * h)
-    else:
-        cih = np.exp(k * z)
-        sih = np.exp(k * z)
+    cih, sih = _airy_depth_terms(k, h, z)
     v = pb.g * k / pb.omega * np.exp(1j * k * wbar) * \
         np.array([np.cos(pb.wave_direction) * cih, np.sin(pb.wave_direction) * cih, -1j * sih])
-    return v.T
+    return v.T.reshape((*output_shape, 3))
 
 
 def airy_waves_potential(points, pb):
-    """airy_waves.py:19-53."""
-    x, y, z = np.asarray(points).T
+    """airy_waves.py:19-53: potential of the undisturbed incident wave,
+    shape output_shape; NaN at zero or infinite wavenumber."""
+    from .lists_of_points import _normalize_points
+    points, output_shape = _normalize_points(points)
+    if float(pb.wavenumber) in (0.0, np.inf):
+        return np.nan * np.ones(output_shape)
+    x, y, z = points.T
     k, h, beta = pb.wavenumber, pb.water_depth, pb.encounter_wave_direction
     wbar = x * np.cos(beta) + y * np.sin(beta)
-    cih = np.cosh(k * (z + h)) / np.cosh(k * h) if 0 <= k * h < 20 else np.exp(k * z)
-    return -1j * pb.g / pb.omega * cih * np.exp(1j * k * wbar)
+    cih, _ = _airy_depth_terms(k, h, z)
+    return (-1j * pb.g / pb.omega * cih * np.exp(1j * k * wbar)).reshape(output_shape)
+
+
+def airy_waves_pressure(points, pb):
+    """airy_waves.py:97-98."""
+    return 1j * float(pb.omega) * pb.rho * airy_waves_potential(points, pb)
+
+
+def airy_waves_free_surface_elevation(points, pb):
+    """airy_waves.py:105-122: elevation of the undisturbed incident wave;
+    points of two coordinates are taken at z = 0."""
+    from .lists_of_points import _normalize_free_surface_points
+    points, output_shape = _normalize_free_surface_points(points)
+    return 1j * pb.omega / pb.g * airy_waves_potential(points, pb).reshape(output_shape)
 
 
 def froude_krylov_force(pb):
     """airy_waves.py:97-102."""
-    pressure = 1j * pb.omega * pb.rho * airy_waves_potential(pb.body.mesh.faces_centers, pb)
-    return pb.body.integrate_pressure(pressure)
+    return pb.body.integrate_pressure(airy_waves_pressure(pb.body.mesh.faces_centers, pb))
 
 
 class RadiationProblem(LinearPotentialFlowProblem):
@@ -176,7 +231,9 @@ class RadiationProblem(LinearPotentialFlowProblem):
         if self.radiating_dof not in body.dofs:
             raise ValueError(f"Unknown radiating dof {self.radiating_dof!r}")
         displacement = np.sum(body.dof_motion(self.radiating_dof) * body.mesh.faces_normals, axis=1)
-        self.boundary_condition = -1j * self.encounter_omega * displacement.astype(np.complex128)
+        # problems_and_results.py:436-447: zero on the lid
+        self.boundary_condition = np.zeros(body.mesh_including_lid.nb_faces, dtype=np.complex128)
+        self.boundary_condition[body.hull_mask] = -1j * self.encounter_omega * displacement
 
     def make_results_container(self, forces=None, sources=None, potential=None, pressure=None):
         return RadiationResult(self, forces, sources, potential, pressure)
@@ -193,8 +250,10 @@ class RadiationProblem(LinearPotentialFlowProblem):
 class DiffractionProblem(LinearPotentialFlowProblem):
     def __init__(self, *, body=None, **kwargs):
         super().__init__(body=body, **kwargs)
-        self.boundary_condition = -(airy_waves_velocity(body.mesh.faces_centers, self)
-                                    * body.mesh.faces_normals).sum(axis=1)
+        # problems_and_results.py:373-381: zero on the lid
+        self.boundary_condition = np.zeros(body.mesh_including_lid.nb_faces, dtype=np.complex128)
+        self.boundary_condition[body.hull_mask] = -(airy_waves_velocity(body.mesh.faces_centers, self)
+                                                    * body.mesh.faces_normals).sum(axis=1)
 
     def make_results_container(self, forces=None, sources=None, potential=None, pressure=None):
         return DiffractionResult(self, forces, sources, potential, pressure)
@@ -218,7 +277,13 @@ class LinearPotentialFlowResult:
         self.omega = problem.omega
         self.encounter_omega = problem.encounter_omega
         self.wavenumber = problem.wavenumber
+        self.encounter_wavenumber = problem.encounter_wavenumber
         self.body = problem.body
+        self.free_surface = problem.free_surface
+        self.water_depth = problem.water_depth
+        self.forward_speed = problem.forward_speed
+        self.rho = problem.rho
+        self.g = problem.g
 
 
 class RadiationResult(LinearPotentialFlowResult):

@@ -62,7 +62,7 @@ class _Request:
         self.headings = np.asarray(headings, dtype=float)
         self.dofs = list(body.dofs)
         self.ks = np.array([wavenumber_from_omega(om, g, np.inf) for om in self.omegas])
-        self.n = body.mesh.nb_faces
+        self.n = body.mesh_including_lid.nb_faces
         self.nrhs = len(self.dofs) + len(self.headings)
         self.rho, self.g = rho, g
         self.future = concurrent.futures.Future()
@@ -94,13 +94,14 @@ class _Request:
         """(len(idx), nrhs, n) host array: radiation then diffraction."""
         idx = np.asarray(idx)
         om = self.omegas[idx]
-        dn = self.body.dof_normal_matrix()  # (ndof, n)
+        dn = self.body.dof_normal_matrix(including_lid=True)  # (ndof, n), zero on a lid
         nd = len(self.dofs)
         out = np.empty((len(idx), self.nrhs, self.n), dtype=complex)
         out[:, :nd] = -1j * om[:, None, None] * dn[None]
         if len(self.headings):
             wave, normal = self._incident(idx)
-            out[:, nd:] = -(self.g * self.ks[idx] / om)[:, None, None] * wave * normal
+            out[:, nd:] = 0.0
+            out[:, nd:, self.body.hull_mask] = -(self.g * self.ks[idx] / om)[:, None, None] * wave * normal
         return out
 
     def result(self):

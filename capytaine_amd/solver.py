@@ -8,7 +8,10 @@ boundary condition, potential = S @ sources, pressure, forces), and
 frequency, rho and g (problems_and_results.py:266-275) -- with one assembly,
 one factorisation and one multi-right-hand-side solve.  ``solve_batched``
 goes further: every frequency of a body in one assembly launch, one batched
-LU and one batched solve.
+LU and one batched solve.  ``compute_potential``, ``compute_velocity``,
+``compute_pressure`` and ``compute_free_surface_elevation`` (:426-561)
+evaluate a solved problem at points with the same assembly kernel, points
+as collocation, and take the sum over the panels on the device.
 """
 import logging
 import time
@@ -19,6 +22,7 @@ import numpy as np
 from . import _lib
 from ._lib import check, ptr, stream_handle
 from .engines import DefaultMatrixEngine, DeviceMatrix, LUDecomposedDeviceMatrix, check_if_nan_in_matrix
+from .lists_of_points import _normalize_points, _normalize_free_surface_points
 from .problems import DiffractionProblem, RadiationProblem
 
 LOG = logging.getLogger(__name__)
@@ -168,7 +172,7 @@ class BEMSolver:
         chunk = max(1, min(len(omega_list), max_batch_bytes // per_freq))
         # W[i, d] = -(dof_d . n_i) A_i  (integrate_pressure, bodies.py:238-250)
         dofs = list(body.dofs)
-        W = -(body.dof_normal_matrix() * mesh.faces_areas[None, :]).T  # (n, ndof)
+        W = -(body.dof_normal_matrix(including_lid=True) * mesh.faces_areas[None, :]).T  # (n, ndof)
         W_d = torch.as_tensor(np.ascontiguousarray(W), dtype=torch.complex128).to(dev)
         ndof = len(dofs)
         results = [None] * len(problems)
@@ -217,6 +221,79 @@ class BEMSolver:
                         results[j] = pb.make_results_container(forces)
             del S, K, lu, X, Phi
         return results
+
+
+    # ------------------------------------------------------------------
+    # Post-processing of a solved problem (solver.py:426-561)
+    @staticmethod
+    def _check_sources(result):
+        if result.sources is None:
+            raise Exception(f"The values of the sources of {result} cannot been found.\n"
+                            "They probably have not been stored by the solver because the option keep_details=True "
+                            "have not been set or the direct method has been used.\n"
+                            "Please re-run the resolution with the indirect method and keep_details=True.")
+        if float(result.encounter_omega) in (0.0, np.inf):
+            raise NotImplementedError("Post-processing at zero or infinite frequency is not supported.")
+
+    def _field(self, points, result, gradient):
+        """Sum over the panels of the sources times S (gradient False) or
+        times the gradient of the Green function (gradient True), at the
+        points: (N,) or (N, 3)."""
+        gf_params = dict(free_surface=result.free_surface, water_depth=result.water_depth,
+                         wavenumber=result.encounter_wavenumber)
+        mesh = result.body.mesh_including_lid
+        if hasattr(self.engine, "field_at_points"):
+            return self.engine.field_at_points(points, mesh, result.sources, gradient=gradient, **gf_params)
+        if gradient:  # an engine of the reference's interface
+            gradG = self.engine.build_fullK_matrix(points, mesh, **gf_params)
+            return np.stack([gradG[c] @ result.sources for c in range(3)], axis=-1)
+        return self.engine.build_S_matrix(points, mesh, **gf_params) @ result.sources
+
+    def compute_potential(self, points, result):
+        """Potential at points (a point, a list of points, a meshgrid or a
+        mesh) of a problem solved with keep_details (solver.py:426-456)."""
+        points, output_shape = _normalize_points(points)
+        self._check_sources(result)
+        with self.timer("Post-processing potential"):
+            potential = self._field(points, result, gradient=False)
+        return potential.reshape(output_shape)
+
+    def _compute_potential_gradient(self, points, result):
+        """solver.py:458-479; a mesh keeps its normals for the diagonal term."""
+        points, output_shape = _normalize_points(points, keep_mesh=True)
+        self._check_sources(result)
+        with self.timer("Post-processing velocity"):
+            velocities = self._field(points, result, gradient=True)
+        return velocities.reshape((*output_shape, 3))
+
+    def compute_velocity(self, points, result):
+        """Velocity at points, shape (*output_shape, 3) (solver.py:481-503)."""
+        nabla_phi = self._compute_potential_gradient(points, result)
+        if result.forward_speed != 0.0:
+            nabla_phi[..., 0] -= result.forward_speed
+        return nabla_phi
+
+    def compute_pressure(self, points, result):
+        """Pressure i omega rho phi at points (solver.py:505-530)."""
+        if result.forward_speed != 0:
+            pressure = 1j * result.encounter_omega * result.rho * self.compute_potential(points, result)
+            nabla_phi = self._compute_potential_gradient(points, result)
+            pressure += result.rho * result.forward_speed * nabla_phi[..., 0]
+        else:
+            pressure = 1j * result.omega * result.rho * self.compute_potential(points, result)
+        return pressure
+
+    def compute_free_surface_elevation(self, points, result):
+        """Free surface elevation i omega / g phi at points given by x, y (z = 0)
+        or x, y, z (solver.py:533-561)."""
+        points, output_shape = _normalize_free_surface_points(points)
+        if result.forward_speed != 0:
+            fs_elevation = -1 / result.g * (-1j * result.encounter_omega) * self.compute_potential(points, result)
+            nabla_phi = self._compute_potential_gradient(points, result)
+            fs_elevation += -1 / result.g * result.forward_speed * nabla_phi[..., 0]
+        else:
+            fs_elevation = -1 / result.g * (-1j * result.omega) * self.compute_potential(points, result)
+        return fs_elevation.reshape(output_shape)
 
 
 def problems_for_body(body, omegas, *, headings=(), rho=1000.0, g=9.81, water_depth=np.inf, dofs=None):

@@ -54,7 +54,7 @@ class FrequencySweep:
         self.info = torch.empty((nb,), dtype=torch.int32, device=self.dev)
         dofs = list(body.dofs)
         self.dofs = dofs
-        W = -(body.dof_normal_matrix() * self.mesh.faces_areas[None, :]).T  # (n, ndof)
+        W = -(body.dof_normal_matrix(including_lid=True) * self.mesh.faces_areas[None, :]).T  # (n, ndof)
         self.W = torch.as_tensor(np.ascontiguousarray(W), dtype=cplx).to(self.dev)
         self.G = torch.empty((nb, nrhs, len(dofs)), dtype=cplx, device=self.dev)
         self.ctx = _lib.Context.get(self.dev)

@@ -30,6 +30,11 @@ def bm_golden():
 
 
 @pytest.fixture(scope="session")
+def pp_golden():
+    return load_golden("post_processing.npz")
+
+
+@pytest.fixture(scope="session")
 def nemoh_golden():
     return load_golden("nemoh2_known_answers.npz")
 

@@ -4,7 +4,8 @@ oracle/_ref/libdelhommeau_ref.so) plus the known-answer data of the
 reference's test-suite.  Run in a container where the reference tree was
 available to build oracle/_ref:
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py                   # every fixture
+    python tests/golden/make_golden.py post_processing   # post_processing.npz only
 
 The fixtures hold inputs and expected outputs only.
 """
@@ -183,7 +184,68 @@ def main():
     np.savez_compressed(os.path.join(HERE, "nemoh2_known_answers.npz"), **nemoh)
     for f in ("delhommeau_core.npz", "build_matrices.npz", "nemoh2_known_answers.npz"):
         print(f, os.path.getsize(os.path.join(HERE, f)))
+    post_processing(R)
+
+
+def post_processing(R):
+    """S and the full gradient K (early_dot_product=False) with points that
+    are not the mesh's face centres as collocation, as the post-processing
+    evaluates them (abstract_green_function.py:42-54, engines.py:142-176):
+    points in the fluid with zero normals, and the faces of another mesh with
+    its normals and the diagonal term of build_fullK_matrix."""
+    rng = np.random.default_rng(20261204)
+    r = R.r_spacing(676, 100.0, 1)
+    z = R.z_spacing(372, -251.0, 1)
+    T = (r, z, R.tabulation(r, z, 1001))
+    sphere = mesh_sphere(radius=1.0, resolution=(8, 12)).immersed_part()
+    barge = mesh_parallelepiped(size=(4.0, 2.0, 1.0), center=(0, 0, -0.5), resolution=(8, 4, 2),
+                                missing_sides={"top"})
+    cylinder = mesh_vertical_cylinder(radius=2.0, length=5.0, resolution=(2, 8, 10)).immersed_part()
+
+    def fluid_points(n, half_width, depth, keep):
+        pts = []
+        while len(pts) < n:
+            p = rng.uniform([-half_width, -half_width, -depth], [half_width, half_width, -0.02])
+            if keep(p):
+                pts.append(p)
+        return np.array(pts)
+
+    far = fluid_points(30, 2.5, 2.5, lambda p: np.linalg.norm(p) > 1.05)
+    near = rng.normal(size=(6, 3))
+    near[:, 2] = -np.abs(near[:, 2])
+    near = near / np.linalg.norm(near, axis=1)[:, None] * rng.uniform(1.01, 1.1, (6, 1))  # near field
+    surface = np.array([[1.5, 0.0, 0.0], [0.0, -2.0, 0.0], [-1.2, 1.3, 0.0]])  # on the free surface
+    sphere_points = np.concatenate([far, near, surface])
+    barge_points = fluid_points(25, 3.0, 2.9, lambda p: np.any(np.abs(p - (0, 0, -0.5)) > (2.05, 1.05, 0.55)))
+    cases = {
+        "points_sphere_k07": (sphere, dict(colloc=sphere_points, dpn=np.zeros_like(sphere_points)),
+                              dict(wavenumber=0.7, early_dot_product=False)),
+        "cylinder_faces_sphere_k07": (sphere, dict(colloc=cylinder.faces_centers, dpn=cylinder.faces_normals),
+                                      dict(wavenumber=0.7, early_dot_product=False)),
+        "points_barge_fd": (barge, dict(colloc=barge_points, dpn=np.zeros_like(barge_points)),
+                            dict(wavenumber=0.8, depth=3.0, early_dot_product=False,
+                                 prony=np.array([[-0.6, -1.5, -4.0, -9.0], [0.4, -0.3, 0.2, 0.05]]))),
+    }
+    mats = {}
+    for name, (mesh, colloc, opts) in cases.items():
+        o = dict(opts)
+        depth = o.pop("depth", np.inf)
+        arr = dict(mesh_arrays(mesh), **colloc)
+        S, K = R.build_matrices(**arr, depth=depth, table=T, free_surface=0.0, **o)
+        for key, val in arr.items():
+            mats[f"{name}__{key}"] = np.asarray(val)
+        for key, val in o.items():
+            mats[f"{name}__opt_{key}"] = np.asarray(val)
+        mats[f"{name}__opt_depth"] = np.asarray(depth)
+        mats[f"{name}__S"] = S
+        mats[f"{name}__K"] = K
+    f = os.path.join(HERE, "post_processing.npz")
+    np.savez_compressed(f, **mats)
+    print(f, os.path.getsize(f))
 
 
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["post_processing"]:
+        post_processing(Reference())
+    else:
+        main()

@@ -102,6 +102,17 @@ def test_build_matrices_golden(gf_ref, bm_golden, name):
     assert elem_ok(K[0], Kg, 1e-9, 1e-8)
 
 
+@pytest.mark.parametrize("name", ["points_sphere_k07", "cylinder_faces_sphere_k07", "points_barge_fd"])
+def test_post_processing_matrices_golden(gf_ref, pp_golden, name):
+    """Points (and another mesh's faces) as collocation, full gradient."""
+    print(f"[gpu] post-processing golden case {name}", flush=True)
+    arr, k, kw = _case(pp_golden, name)
+    S, K = build_raw(gf_ref, arr, k, **kw)
+    Sg, Kg = pp_golden[f"{name}__S"], pp_golden[f"{name}__K"]
+    assert rel_err(S[0], Sg) < RTOL_NORM and rel_err(K[0], Kg) < RTOL_NORM
+    assert elem_ok(S[0], Sg, 1e-9, 1e-8) and elem_ok(K[0], Kg, 1e-9, 1e-8)
+
+
 def test_sphere_450_vs_oracle(gf_ref, orc, default_table):
     """Sphere R = 1, 30 x 30 panels clipped at z = 0 (450 panels), k = 1/g."""
     from capytaine_amd import mesh_sphere

@@ -74,6 +74,29 @@ def test_build_matrices_golden(orc, bm_golden, default_table, name):
     assert elem_ok(S, bm_golden[f"{name}__S"], 1e-10, 1e-10)
 
 
+PP_CASES = ["points_sphere_k07", "cylinder_faces_sphere_k07", "points_barge_fd"]
+
+
+@pytest.mark.parametrize("name", PP_CASES)
+def test_post_processing_matrices_golden(orc, pp_golden, default_table, name):
+    """Points that are not the mesh's face centres as collocation, with the
+    full gradient (early_dot_product=False): points in the fluid with zero
+    normals (no diagonal term), and the faces of another mesh whose diagonal
+    entries get half their own normal (matrices.f90:329-361)."""
+    arr, depth, opts = _case(pp_golden, name)
+    S, K = orc.build_matrices(**arr, depth=depth, table=default_table, **opts)
+    assert K.shape == (3,) + S.shape
+    assert rel_err(S, pp_golden[f"{name}__S"]) < 1e-14
+    assert rel_err(K, pp_golden[f"{name}__K"]) < 1e-14
+    assert elem_ok(K, pp_golden[f"{name}__K"], 1e-10, 1e-10)
+    if name.startswith("cylinder"):
+        # the diagonal term is there: without it the first min(n, m) diagonal entries differ by n_i / 2
+        _, K0 = orc.build_matrices(**arr, depth=depth, table=default_table, diagonal_term=False, **opts)
+        nd = min(S.shape)
+        d = np.array([K[:, i, i] - K0[:, i, i] for i in range(nd)])
+        assert np.allclose(d, 0.5 * arr["dpn"][:nd], atol=1e-14)
+
+
 def test_nemoh2_rankine_matrices(orc, nemoh_golden):
     """pytest/test_consistency_with_Nemoh_2.py:61-99 (k = 0 and k = inf)."""
     from capytaine_amd.meshes import Mesh
