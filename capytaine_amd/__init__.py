@@ -9,7 +9,7 @@ ctypes.  The Python API mirrors capytaine's: ``Delhommeau``,
 generators.
 """
 from .meshes import (Mesh, mesh_sphere, mesh_parallelepiped, mesh_rectangle, mesh_ellipsoid,
-                     mesh_vertical_cylinder)
+                     mesh_vertical_cylinder, ReflectionSymmetricMesh, RotationSymmetricMesh)
 from .green_functions import Delhommeau, XieDelhommeau, GreenFunctionEvaluationError
 from .engines import DefaultMatrixEngine, BasicMatrixEngine, DeviceMatrix, LUDecomposedDeviceMatrix
 from .problems import (FloatingBody, RadiationProblem, DiffractionProblem, rigid_body_dofs,

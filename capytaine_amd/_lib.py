@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(_HERE, "libcapytaine_amd.so")
 CPT_LEGACY_GRID, CPT_SCALED_NEMOH3_GRID = 0, 1
 CPT_LEGACY_FINITE_DEPTH, CPT_NEWER_FINITE_DEPTH = 0, 1
 CPT_HIGH_FREQ, CPT_LOW_FREQ, CPT_LOW_FREQ_WITH_RANKINE_PART = 0, 1, 2
+CPT_BLOCKS_CYCLIC, CPT_BLOCKS_XOR = 0, 1
 
 # Every symbol of include/capytaine_amd.h with its argument types.
 _P, _I, _L, _D = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_double
@@ -46,6 +47,7 @@ SIGNATURES = {
     "cpt_mixed_workspace_size": (_I, [_P, _I, _P, ctypes.POINTER(ctypes.c_size_t)]),
     "cpt_set_tuning": (_I, [_P, _I, _I]),
     "cpt_get_tuning": (_I, [_P, _I, ctypes.POINTER(ctypes.c_int)]),
+    "cpt_block_transform": (_I, [_P, _I, _I, _I, _I, _I, _I, _P, _L, _L, _L, _P, _L, _L, _L, _P]),
 }
 
 # Tuning parameters of cpt_set_tuning, in the order of the CPT_TUNE_* values.

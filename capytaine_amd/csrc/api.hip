@@ -36,6 +36,9 @@ int launch_mixed_solve(const cplx* K, int n, int64_t lda, int64_t strideK, int b
                        void* work, const LuSetup* aux, int* iterations, double* residual, hipStream_t stream);
 int launch_getrs(const cplx* LU, int N, int64_t lda, int64_t strideA, const int* ipiv, int64_t strideP, cplx* B,
                  int nrhs, int64_t ldb, int64_t strideB, int batch, cplx* work, hipStream_t stream);
+int launch_block_transform(int xor_kind, int n, int inverse, int R, int C, int nbatch, const cplx* in, int64_t sbi,
+                           int64_t sj, int64_t ldi, cplx* out, int64_t sbo, int64_t sk, int64_t ldo, hipStream_t stream);
+int block_transform_max_blocks();
 
 static thread_local std::string g_last_error;
 void set_error(const std::string& msg) { g_last_error = msg; }
@@ -918,6 +921,41 @@ int cpt_zgemm(cpt_handle h, int M, int N, int K, double alpha, const void* A, in
     CPT_CHECK_HIP((hipError_t)launch_zgemm(M, N, K, alpha, (const cplx*)A, lda, strideA, (const cplx*)B, ldb, strideB,
                                            (cplx*)C, ldc, strideC, beta_zero, batch, stream),
                   "zgemm launch");
+    return CPT_OK;
+}
+
+int cpt_block_transform(cpt_handle h, int kind, int n, int inverse, int rows, int cols, int batch, const void* in,
+                        int64_t in_batch_stride, int64_t in_block_stride, int64_t ldi, void* out,
+                        int64_t out_batch_stride, int64_t out_block_stride, int64_t ldo, hipStream_t stream)
+{
+    if (!h || (kind != CPT_BLOCKS_CYCLIC && kind != CPT_BLOCKS_XOR) || n < 1 || rows < 0 || cols < 0 || batch < 0 ||
+        ldi < rows || ldo < rows || in_block_stride < 0 || out_block_stride < 0 || in_batch_stride < 0 ||
+        out_batch_stride < 0) {
+        set_error("cpt_block_transform: invalid arguments");
+        return CPT_ERR_ARG;
+    }
+    if (n > block_transform_max_blocks()) {
+        set_error("cpt_block_transform: at most " + std::to_string(block_transform_max_blocks()) + " blocks");
+        return CPT_ERR_UNSUPPORTED;
+    }
+    if (kind == CPT_BLOCKS_XOR && (n & (n - 1)) != 0) {
+        set_error("cpt_block_transform: the reflection kind needs a power-of-two number of blocks");
+        return CPT_ERR_ARG;
+    }
+    if (rows == 0 || cols == 0 || batch == 0) return CPT_OK;
+    if (!in || !out) {
+        set_error("cpt_block_transform: null pointer argument");
+        return CPT_ERR_ARG;
+    }
+    if (in == out) {
+        set_error("cpt_block_transform: input and output must be distinct buffers");
+        return CPT_ERR_ARG;
+    }
+    CPT_CHECK_HIP(hipSetDevice(h->device), "hipSetDevice");
+    CPT_CHECK_HIP((hipError_t)launch_block_transform(kind == CPT_BLOCKS_XOR, n, inverse ? 1 : 0, rows, cols, batch,
+                                                     (const cplx*)in, in_batch_stride, in_block_stride, ldi,
+                                                     (cplx*)out, out_batch_stride, out_block_stride, ldo, stream),
+                  "block transform launch");
     return CPT_OK;
 }
 

@@ -188,10 +188,14 @@ class MixedPrecisionSolver:
 
 
 def has_been_lu_decomposed(A):
-    return isinstance(A, LUDecomposedDeviceMatrix)
+    from .symmetry import LUDecomposedSymmetricMatrix
+    return isinstance(A, (LUDecomposedDeviceMatrix, LUDecomposedSymmetricMatrix))
 
 
 def lu_decompose(A, overwrite_a=False):
+    from .symmetry import BlockSymmetricDeviceMatrix, LUDecomposedSymmetricMatrix
+    if isinstance(A, BlockSymmetricDeviceMatrix):
+        return LUDecomposedSymmetricMatrix(A, overwrite_a=overwrite_a)
     if not isinstance(A, DeviceMatrix):
         raise TypeError("The device engine factorises DeviceMatrix objects")
     return LUDecomposedDeviceMatrix(A, overwrite_a=overwrite_a)
@@ -200,7 +204,7 @@ def lu_decompose(A, overwrite_a=False):
 def check_if_nan_in_matrix(matrices):
     import torch
     for m in matrices:
-        data = m.data if isinstance(m, DeviceMatrix) else m
+        data = m.data if isinstance(m, DeviceMatrix) else getattr(m, "cols", m)
         bad = bool(torch.isnan(torch.view_as_real(data)).any().item()) if hasattr(data, "device") else \
             bool(np.any(np.isnan(data)))
         if bad:
@@ -332,6 +336,18 @@ class DefaultMatrixEngine:
                                                 **gf_params)
             check_if_nan_in_matrix([S, K])
             return S, K
+        from .symmetry import BlockSymmetricDeviceMatrix, symmetry_of
+        sym = symmetry_of(mesh1) if mesh1 is mesh2 else None
+        if sym is not None:
+            # engines.py:178-247: the first block column, all collocation
+            # points against the reference block
+            S, K = self.green_function.evaluate_on_device(mesh1.merged(), sym.block_mesh,
+                                                          diagonal_term_in_double_layer=diagonal_term_in_double_layer,
+                                                          **gf_params)
+            S, K = BlockSymmetricDeviceMatrix(S, sym), BlockSymmetricDeviceMatrix(K, sym)
+            if self.check_nan:
+                check_if_nan_in_matrix([S, K])
+            return S, K
         S, K = self.green_function.evaluate_on_device(mesh1, mesh2,
                                                       diagonal_term_in_double_layer=diagonal_term_in_double_layer,
                                                       **gf_params)
@@ -369,6 +385,9 @@ class DefaultMatrixEngine:
                 luA = A
             return luA.solve(b)
         if self._linear_solver == "mixed_precision":
+            if hasattr(A, "sym"):
+                raise NotImplementedError("The mixed-precision solver takes plain meshes; "
+                                          "symmetric meshes use the FP64 block-diagonal LU.")
             if not isinstance(A, MixedPrecisionSolver):
                 solver = MixedPrecisionSolver(A)
                 if self.last_computed_matrices is not None and A is self.last_computed_matrices[1]:

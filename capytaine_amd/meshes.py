@@ -175,8 +175,31 @@ class Mesh:
         faces = np.where(quad[:, None], self.faces[:, ::-1], self.faces[:, [2, 1, 0, 0]])
         return Mesh(self.vertices, faces, quadrature_method=self.quadrature_method, name=self.name)
 
+    def mirrored(self, plane, name=None):
+        """Mirror image across the plane xOz (y -> -y) or yOz (x -> -x), faces
+        reversed to keep the normals outward (meshes.py:554-573)."""
+        v = self.vertices.copy()
+        if plane == "xOz":
+            v[:, 1] *= -1
+        elif plane == "yOz":
+            v[:, 0] *= -1
+        else:
+            raise ValueError(f"Unsupported value for plane: {plane}")
+        quad = self.faces[:, 2] != self.faces[:, 3]
+        faces = np.where(quad[:, None], self.faces[:, ::-1], self.faces[:, [2, 1, 0, 0]])
+        return Mesh(v, faces, quadrature_method=self.quadrature_method, name=name or f"mirrored_{self.name}")
+
+    def rotated_z(self, angle, name=None):
+        """Rotation by `angle` about Oz (meshes.py:541-552)."""
+        c, s = np.cos(angle), np.sin(angle)
+        R = np.array([[c, -s, 0.0], [s, c, 0.0], [0.0, 0.0, 1.0]])
+        return Mesh(self.vertices @ R.T, self.faces, quadrature_method=self.quadrature_method,
+                    name=name or self.name)
+
     def join(self, other, name=None):
         """Union of two meshes (faces of self first)."""
+        if isinstance(other, SymmetricMesh):
+            other = other.merged()
         faces = np.concatenate([self.faces, other.faces + self.nb_vertices])
         same = self.quadrature_method == other.quadrature_method  # else dropped, as meshes.py:620-624
         return Mesh(np.concatenate([self.vertices, other.vertices]), faces,
@@ -246,6 +269,185 @@ def _gauss_legendre_2(vertices, faces):
 
 
 # ----------------------------------------------------------------------
+# symmetric meshes (meshes/symmetric_meshes.py)
+# ----------------------------------------------------------------------
+class SymmetricMesh:
+    """A mesh made of copies of a reference block under a group of
+    isometries.  Its faces are those of ``merged()`` (copies in order, the
+    reference block first); the solver evaluates only the reference block's
+    column of S and K and block-diagonalises the rest (see
+    capytaine_amd.symmetry)."""
+
+    _merged = None
+
+    def merged(self):
+        if self._merged is None:
+            self._merged = self._build_merged()
+        return self._merged
+
+    # geometry of the merged mesh
+    nb_faces = property(lambda self: self.merged().nb_faces)
+    nb_vertices = property(lambda self: self.merged().nb_vertices)
+    vertices = property(lambda self: self.merged().vertices)
+    faces = property(lambda self: self.merged().faces)
+    faces_normals = property(lambda self: self.merged().faces_normals)
+    faces_centers = property(lambda self: self.merged().faces_centers)
+    faces_areas = property(lambda self: self.merged().faces_areas)
+    faces_radiuses = property(lambda self: self.merged().faces_radiuses)
+    quadrature_points = property(lambda self: self.merged().quadrature_points)
+    quadrature_method = property(lambda self: self.merged().quadrature_method)
+
+    def device_arrays(self, device):
+        return self.merged().device_arrays(device)
+
+    def __eq__(self, other):
+        return self is other
+
+    def __hash__(self):
+        return id(self)
+
+    def join(self, other, name=None):
+        """Symmetric when both meshes have the same symmetry (the blocks are
+        joined), otherwise the merged meshes are joined
+        (symmetric_meshes.py:182-232)."""
+        joined = self._join_symmetric(other, name)
+        if joined is not None:
+            return joined
+        LOG.warning("Joining meshes of different symmetries: the symmetry is discarded.")
+        return self.merged().join(other, name=name)
+
+    def __add__(self, other):
+        return self.join(other)
+
+
+def join_with_mask(hull, lid, name=None):
+    """hull.join(lid) and the mask of the hull's faces in it: hull faces
+    first for plain meshes; block by block for symmetric joins, whose
+    reference block holds the hull's block then the lid's
+    (meshes.py:575-650, return_masks=True)."""
+    joined = hull.join(lid, name=name)
+    if isinstance(joined, ReflectionSymmetricMesh):
+        _, half_mask = join_with_mask(hull.half, lid.half)
+        return joined, np.concatenate([half_mask, half_mask])
+    if isinstance(joined, RotationSymmetricMesh):
+        _, wedge_mask = join_with_mask(hull.wedge, lid.wedge)
+        return joined, np.tile(wedge_mask, joined.n)
+    return joined, np.arange(joined.nb_faces) < hull.nb_faces
+
+
+class ReflectionSymmetricMesh(SymmetricMesh):
+    """`half` and its mirror image across `plane` ("xOz": y -> -y, "yOz":
+    x -> -x), symmetric_meshes.py:30-311.  `half` may itself be a
+    ReflectionSymmetricMesh across the other plane."""
+
+    def __init__(self, half, plane, name=None):
+        if plane not in ("xOz", "yOz"):
+            raise ValueError(f"Plane must be 'xOz' or 'yOz', got '{plane}'")
+        self.half = half
+        self.plane = plane
+        self.other_half = half.mirrored(plane)
+        self.name = name or f"symmetric_{half.name}"
+
+    def _build_merged(self):
+        a, b = self.half, self.other_half
+        a = a.merged() if isinstance(a, SymmetricMesh) else a
+        b = b.merged() if isinstance(b, SymmetricMesh) else b
+        return a.join(b, name=self.name)
+
+    def mirrored(self, plane, name=None):
+        return ReflectionSymmetricMesh(self.half.mirrored(plane), self.plane, name=name)
+
+    def translated(self, shift, name=None):
+        shift = np.asarray(shift, dtype=float)
+        if (self.plane == "xOz" and abs(shift[1]) < 1e-12) or (self.plane == "yOz" and abs(shift[0]) < 1e-12):
+            return ReflectionSymmetricMesh(self.half.translated(shift), self.plane, name=name or self.name)
+        return self.merged().translated(shift, name=name)
+
+    def immersed_part(self, free_surface=0.0, tol=1e-8):
+        return ReflectionSymmetricMesh(self.half.immersed_part(free_surface, tol), self.plane, name=self.name)
+
+    def with_normal_vector_going_down(self):
+        half = self.half.with_normal_vector_going_down()
+        return self if half is self.half else ReflectionSymmetricMesh(half, self.plane, name=self.name)
+
+    def with_quadrature(self, quadrature_method):
+        return ReflectionSymmetricMesh(self.half.with_quadrature(quadrature_method), self.plane, name=self.name)
+
+    def _join_symmetric(self, other, name):
+        if isinstance(other, ReflectionSymmetricMesh) and other.plane == self.plane:
+            half = self.half.join(other.half)
+            if isinstance(half, SymmetricMesh) == isinstance(self.half, SymmetricMesh):
+                return ReflectionSymmetricMesh(half, self.plane, name=name or f"{self.name}+{other.name}")
+        return None
+
+    def __repr__(self):
+        return f"ReflectionSymmetricMesh(half={self.half!r}, plane={self.plane!r})"
+
+
+class RotationSymmetricMesh(SymmetricMesh):
+    """`wedge` and its rotations by 2 pi i / n about Oz, i = 1..n-1
+    (symmetric_meshes.py:314-423, axis 'z+')."""
+
+    def __init__(self, wedge, n, name=None):
+        if n < 2:
+            raise ValueError(f"Rotation order must be >= 2, got: {n}")
+        self.wedge = wedge
+        self.n = int(n)
+        self.all_wedges = [wedge] + [wedge.rotated_z(2 * i * np.pi / n) for i in range(1, n)]
+        self.name = name or f"rotation_{wedge.name}"
+
+    @classmethod
+    def from_profile_points(cls, points, n, name=None):
+        """The surface swept by the profile `points` (sorted by increasing z)
+        rotated n times about Oz (symmetric_meshes.py:386-423): the wedge has
+        one quadrilateral per profile segment."""
+        c, s = np.cos(2 * np.pi / n), np.sin(2 * np.pi / n)
+        R = np.array([[c, -s, 0.0], [s, c, 0.0], [0.0, 0.0, 1.0]])
+        points = np.asarray(sorted(list(np.asarray(points, dtype=float)), key=lambda p: p[2]))
+        m = len(points)
+        vertices = np.concatenate([points, points @ R.T])
+        faces = np.array([(i, i + m, i + m + 1, i + 1) for i in range(m - 1)])
+        return cls(Mesh(vertices, faces, name=f"wedge_of_{name}" if name else None), n, name=name)
+
+    def _build_merged(self):
+        merged = self.all_wedges[0]
+        for w in self.all_wedges[1:]:
+            merged = merged.join(w)
+        merged.name = self.name
+        return merged
+
+    def rotated_z(self, angle, name=None):
+        return RotationSymmetricMesh(self.wedge.rotated_z(angle), self.n, name=name or self.name)
+
+    def mirrored(self, plane, name=None):
+        return self.merged().mirrored(plane, name=name)
+
+    def translated(self, shift, name=None):
+        shift = np.asarray(shift, dtype=float)
+        if abs(shift[0]) < 1e-12 and abs(shift[1]) < 1e-12:
+            return RotationSymmetricMesh(self.wedge.translated(shift), self.n, name=name or self.name)
+        return self.merged().translated(shift, name=name)
+
+    def immersed_part(self, free_surface=0.0, tol=1e-8):
+        return RotationSymmetricMesh(self.wedge.immersed_part(free_surface, tol), self.n, name=self.name)
+
+    def with_normal_vector_going_down(self):
+        wedge = self.wedge.with_normal_vector_going_down()
+        return self if wedge is self.wedge else RotationSymmetricMesh(wedge, self.n, name=self.name)
+
+    def with_quadrature(self, quadrature_method):
+        return RotationSymmetricMesh(self.wedge.with_quadrature(quadrature_method), self.n, name=self.name)
+
+    def _join_symmetric(self, other, name):
+        if isinstance(other, RotationSymmetricMesh) and other.n == self.n:
+            return RotationSymmetricMesh(self.wedge.join(other.wedge), self.n, name=name or f"{self.name}+{other.name}")
+        return None
+
+    def __repr__(self):
+        return f"RotationSymmetricMesh(wedge={self.wedge!r}, n={self.n})"
+
+
+# ----------------------------------------------------------------------
 # generators
 # ----------------------------------------------------------------------
 def _revolution(profile, n, name, scale=(1.0, 1.0, 1.0)):
@@ -286,14 +488,26 @@ def _revolution(profile, n, name, scale=(1.0, 1.0, 1.0)):
     return Mesh(verts, faces, name=name)
 
 
-def mesh_sphere(*, radius=1.0, center=(0.0, 0.0, 0.0), resolution=(10, 10), name=None):
+def _axial(mesh, n, name):
+    """The RotationSymmetricMesh whose wedge is the first of the n meridian
+    strips of a _revolution mesh."""
+    nf = mesh.nb_faces // n
+    wedge = Mesh(mesh.vertices, mesh.faces[:nf], quadrature_method=mesh.quadrature_method, name=f"wedge_of_{name}")
+    return RotationSymmetricMesh(wedge, n, name=name)
+
+
+def mesh_sphere(*, radius=1.0, center=(0.0, 0.0, 0.0), resolution=(10, 10), axial_symmetry=False, name=None):
     """Sphere, as predefined/spheres.py:26-66 (theta = linspace(0, pi, ntheta+1)
-    on a meridian, nphi rotations)."""
+    on a meridian, nphi rotations); axial_symmetry gives the
+    RotationSymmetricMesh of one meridian strip."""
     ntheta, nphi = resolution
     theta = np.linspace(0.0, np.pi, ntheta + 1)
     profile = radius * np.stack([np.sin(theta), np.zeros_like(theta), -np.cos(theta)], axis=1)
     profile[0, 0] = profile[-1, 0] = 0.0
-    return _revolution(profile, nphi, name or "sphere").translated(center)
+    mesh = _revolution(profile, nphi, name or "sphere")
+    if axial_symmetry:
+        mesh = _axial(mesh, nphi, name or "sphere")
+    return mesh.translated(center)
 
 
 def mesh_ellipsoid(*, semi_axes=(2.0, 1.0, 1.0), center=(0.0, 0.0, 0.0), resolution=(10, 10), name=None):
@@ -336,12 +550,28 @@ def mesh_rectangle(*, size=(5.0, 5.0), center=(0.0, 0.0, 0.0), resolution=(5, 5)
 
 
 def mesh_parallelepiped(size=(1.0, 1.0, 1.0), center=(0.0, 0.0, 0.0), resolution=(4, 4, 4), missing_sides=(),
-                        name=None):
+                        reflection_symmetry=False, name=None):
     """Six rectangles (predefined/rectangles.py:103-220), outward normals,
-    in the order left, right, front, back, top, bottom."""
+    in the order left, right, front, back, top, bottom.  With
+    reflection_symmetry, the quarter x < 0, y < 0 mirrored across yOz, then
+    across xOz (rectangles.py:157-175); the centre must be on Oz and the
+    resolution even in x and y."""
     width, thickness, height = size
     nw, nth, nh = resolution
     missing = set(missing_sides)
+    if reflection_symmetry:
+        if nw % 2 or nth % 2:
+            raise ValueError("To define a parallelepiped mesh with reflection symmetry, "
+                             "it should have an even number of panels in both direction.")
+        if abs(center[0]) > 1e-12 or abs(center[1]) > 1e-12:
+            raise ValueError("To define a rectangle mesh with reflection symmetry, "
+                             "its center should be on the Oz axis (that is x=0 and y=0).")
+        name = name or "parallelepiped"
+        quarter = mesh_parallelepiped(size=(width / 2, thickness / 2, height), resolution=(nw // 2, nth // 2, nh),
+                                      center=(-width / 4, -thickness / 4, center[2]),
+                                      missing_sides=missing | {"right", "back"}, name=f"quarter_of_{name}")
+        half = ReflectionSymmetricMesh(quarter, "yOz", name=f"half_of_{name}")
+        return ReflectionSymmetricMesh(half, "xOz", name=name)
     sides = []
     if "left" not in missing:
         sides.append(mesh_rectangle(size=(thickness, height), resolution=(nth, nh), center=(-width / 2, 0, 0),
@@ -368,11 +598,13 @@ def mesh_parallelepiped(size=(1.0, 1.0, 1.0), center=(0.0, 0.0, 0.0), resolution
     return mesh
 
 
-def mesh_vertical_cylinder(*, radius=1.0, length=1.0, center=(0.0, 0.0, 0.0), resolution=(2, 8, 10), name=None):
+def mesh_vertical_cylinder(*, radius=1.0, length=1.0, center=(0.0, 0.0, 0.0), resolution=(2, 8, 10),
+                           axial_symmetry=False, name=None):
     """Vertical cylinder open at the top (its top sits at center_z + length/2),
     closed by a disk at the bottom: resolution = (nr rings on the bottom disk,
     ntheta panels around, nz panels along the side), as predefined/
-    cylinders.py:108-180 with the top disk removed."""
+    cylinders.py:108-180 with the top disk removed; axial_symmetry gives the
+    RotationSymmetricMesh of one of the ntheta slices."""
     nr, ntheta, nz = resolution
     z_top, z_bot = length / 2, -length / 2
     side_profile = [(radius, 0.0, z) for z in np.linspace(z_bot, z_top, nz + 1)]
@@ -380,6 +612,8 @@ def mesh_vertical_cylinder(*, radius=1.0, length=1.0, center=(0.0, 0.0, 0.0), re
     # bottom disk from the axis outwards, then the side upwards
     profile = np.array(disk_profile + side_profile[1:])
     mesh = _revolution(profile, ntheta, name or "cylinder")
+    if axial_symmetry:
+        mesh = _axial(mesh, ntheta, name or "cylinder")
     return mesh.translated(center)
 
 

@@ -73,9 +73,11 @@ class FloatingBody:
         self.lid_mesh = None if lid_mesh is None else lid_mesh.with_normal_vector_going_down()
         if self.lid_mesh is None:
             self.mesh_including_lid = mesh
+            self.hull_mask = np.ones(mesh.nb_faces, dtype=bool)
         else:
-            self.mesh_including_lid = mesh.join(self.lid_mesh, name=f"{mesh.name}+{self.lid_mesh.name}")
-        self.hull_mask = np.arange(self.mesh_including_lid.nb_faces) < mesh.nb_faces
+            from .meshes import join_with_mask
+            self.mesh_including_lid, self.hull_mask = join_with_mask(mesh, self.lid_mesh,
+                                                                     name=f"{mesh.name}+{self.lid_mesh.name}")
 
     def first_irregular_frequency_estimate(self, *, g=9.81):
         """bodies.py:566-601: lowest irregular frequency of a box of the

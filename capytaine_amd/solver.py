@@ -23,6 +23,7 @@ from . import _lib
 from ._lib import check, ptr, stream_handle
 from .engines import DefaultMatrixEngine, DeviceMatrix, LUDecomposedDeviceMatrix, check_if_nan_in_matrix
 from .lists_of_points import _normalize_points, _normalize_free_surface_points
+from .symmetry import BlockSymmetricDeviceMatrix, LUDecomposedSymmetricMatrix, symmetry_of
 from .problems import DiffractionProblem, RadiationProblem
 
 LOG = logging.getLogger(__name__)
@@ -168,7 +169,8 @@ class BEMSolver:
         if max_batch_bytes is None:
             free, _ = torch.cuda.mem_get_info(dev)
             max_batch_bytes = int(0.6 * free)
-        per_freq = 2 * n * n * 16
+        sym = symmetry_of(mesh)
+        per_freq = 2 * n * n * 16 // (sym.n if sym else 1)
         chunk = max(1, min(len(omega_list), max_batch_bytes // per_freq))
         # W[i, d] = -(dof_d . n_i) A_i  (integrate_pressure, bodies.py:238-250)
         dofs = list(body.dofs)
@@ -183,11 +185,15 @@ class BEMSolver:
             ks = [problems[omegas[om][0]].wavenumber for om in oms]
             pb0 = problems[omegas[oms[0]][0]]
             with self.timer("Green function"):
-                S, K = gf.evaluate_on_device(mesh, mesh, free_surface=pb0.free_surface,
-                                             water_depth=pb0.water_depth, wavenumber=ks,
-                                             adjoint_double_layer=True, diagonal_term_in_double_layer=True)
+                # a symmetric mesh: the first block column only (engines.py:178-247)
+                S, K = gf.evaluate_on_device(mesh.merged() if sym else mesh, sym.block_mesh if sym else mesh,
+                                             free_surface=pb0.free_surface, water_depth=pb0.water_depth,
+                                             wavenumber=ks, adjoint_double_layer=True,
+                                             diagonal_term_in_double_layer=True)
+                S, K = (BlockSymmetricDeviceMatrix(S, sym), BlockSymmetricDeviceMatrix(K, sym)) if sym else \
+                    (DeviceMatrix(S), DeviceMatrix(K))
                 if self.engine.check_nan:
-                    check_if_nan_in_matrix([DeviceMatrix(S), DeviceMatrix(K)])
+                    check_if_nan_in_matrix([S, K])
             nb = len(oms)
             R = max(len(omegas[om]) for om in oms)
             bc = np.zeros((nb, R, n), dtype=np.complex128)
@@ -195,11 +201,11 @@ class BEMSolver:
                 for r, j in enumerate(omegas[om]):
                     bc[b, r] = problems[j].boundary_condition
             with self.timer("Linear solver"):
-                lu = LUDecomposedDeviceMatrix(DeviceMatrix(K), overwrite_a=True)
+                lu = LUDecomposedSymmetricMatrix(K) if sym else LUDecomposedDeviceMatrix(K, overwrite_a=True)
                 X = torch.as_tensor(bc).to(dev)
                 lu.solve_device(X)
             with self.timer("Matrix-vector product"):
-                Phi = DeviceMatrix(S).matmul_device(X)
+                Phi = S.matmul_device(X)
                 # G (ndof x R) = W^T (ndof x n) Phi (n x R), per frequency
                 G = torch.empty((nb, R, ndof), dtype=torch.complex128, device=dev)
                 check(lib.cpt_zgemm(ctx.handle, ndof, R, n, 1.0, ptr(W_d), ndof, 0, ptr(Phi), n, n * R, ptr(G),

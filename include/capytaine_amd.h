@@ -219,6 +219,24 @@ int cpt_zgemm(cpt_handle h, int M, int N, int K, double alpha, const void* A, in
               const void* B, int64_t ldb, int64_t strideB, void* C, int64_t ldc, int64_t strideC, int beta_zero,
               int batch, hipStream_t stream);
 
+/* Group Fourier transform over the n blocks of a symmetric mesh's
+ * matrices or vectors: out_k = s * sum_j chi(k, j) in_j, with
+ * chi(k, j) = exp(-2 pi i j k / n) for rotations (kind CPT_BLOCKS_CYCLIC;
+ * exp(+...) and s = 1/n when `inverse`) or (-1)^popcount(j & k) for one or
+ * two reflection planes (kind CPT_BLOCKS_XOR, n = 2 or 4; s = 1/n when
+ * `inverse`).  Element (r, c) of block j of batch item b is
+ * in[b*in_batch_stride + j*in_block_stride + c*ldi + r], r < rows, c < cols;
+ * of output block k: out[b*out_batch_stride + k*out_block_stride + c*ldo + r].
+ * complex128, 1 <= n <= 128; `in` and `out` must not overlap.
+ * Replaces BlockCirculantMatrix / NestedBlockCirculantMatrix
+ * .block_diagonalize and the FFTs of .solve
+ * (src/capytaine/tools/block_circulant_matrices.py:131-160, :243-254),
+ * called from engines.py:178-247 (_build_matrices_with_symmetries). */
+enum { CPT_BLOCKS_CYCLIC = 0, CPT_BLOCKS_XOR = 1 };
+int cpt_block_transform(cpt_handle h, int kind, int n, int inverse, int rows, int cols, int batch, const void* in,
+                        int64_t in_batch_stride, int64_t in_block_stride, int64_t ldi, void* out,
+                        int64_t out_batch_stride, int64_t out_block_stride, int64_t ldo, hipStream_t stream);
+
 /* Live timing of the hot kernels, for measurement only (no counterpart in
  * the reference).  Between cpt_profile_begin and cpt_profile_end every
  * launch of the kernel classes below made through `h` is bracketed by HIP

@@ -1,0 +1,169 @@
+"""Symmetric meshes on the device (engines.py:178-247,
+block_circulant_matrices.py): the block transform against numpy, the
+block-symmetric S and K against the matrices of the merged mesh, and the
+solve against the oracle pipeline restating the reference's symmetric
+algorithm (oracle S, K of all collocation points against the reference
+block, the full matrix rebuilt from its block column, scipy LU).
+
+Tolerances: transforms 1e-14 (4e-16 n for n blocks); solve against the
+oracle 1e-10 (FP64 path); symmetric against merged mesh 1e-12 (matrices) and
+1e-10 (forces) on a sphere, 1e-6 on a box, whose
+regular grid puts panel pairs exactly at the near-field distance, where
+the mirrored copy's rounding can pick the other branch."""
+import numpy as np
+import pytest
+from conftest import rel_err
+
+from gpu_helpers import device, ref_table_gf
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def solver(default_table):
+    import capytaine_amd as cpa
+    print("[gpu] symmetry: solver with the reference table", flush=True)
+    return cpa.BEMSolver(engine=cpa.DefaultMatrixEngine(green_function=ref_table_gf(default_table)))
+
+
+def _meshes():
+    import capytaine_amd as cpa
+    box = dict(size=(2.0, 1.0, 1.0), center=(0.0, 0.0, -0.5), resolution=(8, 4, 4), missing_sides={"top"})
+    half_box = cpa.mesh_parallelepiped(size=(2.0, 0.5, 1.0), center=(0.0, -0.25, -0.5), resolution=(8, 2, 4),
+                                       missing_sides={"top", "back"})
+    return {
+        "box_two_planes": cpa.mesh_parallelepiped(**box, reflection_symmetry=True),
+        "box_one_plane": cpa.ReflectionSymmetricMesh(half_box, "xOz"),
+        "sphere_rotation": cpa.mesh_sphere(radius=1.0, resolution=(12, 16), axial_symmetry=True).immersed_part(),
+    }
+
+
+@pytest.mark.parametrize("kind,n", [("xor", 2), ("xor", 4), ("cyclic", 3), ("cyclic", 16), ("cyclic", 72)])
+def test_block_transform_vs_numpy(kind, n):
+    import torch
+    from capytaine_amd.symmetry import Symmetry, BlockSymmetricDeviceMatrix, transform_vectors
+    print(f"[gpu] symmetry: block transform {kind} n={n}", flush=True)
+    rng = np.random.default_rng(n)
+    w, nb, r = 5, 2, 3
+    N = n * w
+
+    class _Block:
+        nb_faces = w
+    sym = Symmetry(kind, n, _Block())
+    cols = rng.normal(size=(nb, w, N)) + 1j * rng.normal(size=(nb, w, N))
+    A = BlockSymmetricDeviceMatrix(torch.as_tensor(cols).to(device()), sym)
+    D = A.diagonal_blocks().cpu().numpy().reshape(nb, n, w, w)  # [b, k, col, row]
+    blocks = cols.reshape(nb, w, n, w)  # [b, col, j, row]
+    chi = sym.characters()
+    ref = np.einsum("kj,bcjr->bkcr", chi, blocks)
+    tol = max(1e-14, 4e-16 * n)  # sums of n terms
+    assert rel_err(D, ref) < tol
+    x = rng.normal(size=(nb, r, N)) + 1j * rng.normal(size=(nb, r, N))
+    xd = torch.as_tensor(x).to(device())
+    xh = transform_vectors(sym, xd)
+    ref = np.einsum("kj,bqjr->bkqr", chi, x.reshape(nb, r, n, w)).reshape(nb * n, r, w)
+    assert rel_err(xh.cpu().numpy(), ref) < tol
+    back = transform_vectors(sym, xh, inverse=True).cpu().numpy()
+    assert rel_err(back, x) < tol
+    # the product through the transforms equals the dense block matrix's
+    y = A.matmul_device(xd).cpu().numpy()
+    dense = np.asarray(A)
+    assert rel_err(y, np.einsum("bij,bqj->bqi", dense, x)) < 10 * tol
+
+
+@pytest.mark.parametrize("name", ["box_two_planes", "box_one_plane", "sphere_rotation"])
+def test_symmetric_matrices_vs_merged_mesh(solver, name):
+    mesh = _meshes()[name]
+    gp = dict(free_surface=0.0, water_depth=np.inf, wavenumber=1.3)
+    S, K = solver.engine.build_matrices(mesh, mesh, **gp)
+    assert hasattr(S, "sym"), "the symmetric path was not taken"
+    merged = mesh.merged()
+    Sm, Km = solver.engine.build_matrices(merged, merged, **gp)
+    tol = 1e-12 if name.startswith("sphere") else 1e-6
+    assert rel_err(np.asarray(S), np.asarray(Sm)) < tol and rel_err(np.asarray(K), np.asarray(Km)) < tol
+
+
+def _oracle_forces(orc, table, mesh, body, problems):
+    """The reference's symmetric algorithm restated with the oracle: S and K
+    of every collocation point against the reference block, the full matrix
+    rebuilt from that block column, scipy LU."""
+    from oracle import mesh_arrays
+    from scipy.linalg import lu_factor, lu_solve
+    from capytaine_amd.symmetry import symmetry_of
+    sym = symmetry_of(mesh)
+    merged = mesh.merged()
+    arr = dict(mesh_arrays(sym.block_mesh), colloc=merged.faces_centers, dpn=merged.faces_normals)
+    Sc, Kc = orc.build_matrices(**arr, wavenumber=problems[0].wavenumber, depth=np.inf, table=table)
+    n, w = sym.n, sym.w
+
+    def full(cols):
+        out = np.empty((n * w, n * w), dtype=complex)
+        for i in range(n):
+            for j in range(n):
+                a = sym.block_index(i, j)
+                out[i * w:(i + 1) * w, j * w:(j + 1) * w] = cols[a * w:(a + 1) * w, :]
+        return out
+    S, K = full(Sc), full(Kc)
+    lu = lu_factor(K)
+    return [body.integrate_pressure(1j * pb.omega * pb.rho * (S @ lu_solve(lu, pb.boundary_condition)))
+            for pb in problems]
+
+
+@pytest.mark.parametrize("name", ["box_two_planes", "box_one_plane", "sphere_rotation"])
+def test_symmetric_solve_vs_oracle(solver, orc, default_table, name):
+    import capytaine_amd as cpa
+    print(f"[gpu] symmetry: solve vs oracle, {name}", flush=True)
+    mesh = _meshes()[name]
+    body = cpa.FloatingBody(mesh, cpa.rigid_body_dofs(rotation_center=(0.1, 0.0, -0.2)))
+    pbs = [cpa.RadiationProblem(body=body, radiating_dof=d, omega=1.4) for d in ("Surge", "Heave", "Yaw")]
+    pbs.append(cpa.DiffractionProblem(body=body, omega=1.4, wave_direction=0.6))
+    refs = _oracle_forces(orc, default_table, mesh, body, pbs)
+    batched = solver.solve_all(pbs)
+    for pb, ref, res in zip(pbs, refs, batched):
+        scale = max(abs(v) for v in ref.values())
+        single = solver.solve(pb)
+        for dof in ref:
+            assert abs(res.forces[dof] - ref[dof]) <= 1e-10 * scale, (pb, dof)
+            assert abs(single.forces[dof] - ref[dof]) <= 1e-10 * scale, (pb, dof)
+
+
+@pytest.mark.parametrize("name", ["box_two_planes", "box_one_plane", "sphere_rotation"])
+def test_symmetric_solve_vs_merged_mesh(solver, name):
+    """pytest/test_bem_irregular_frequencies_removal.py:233-253 compares the
+    symmetric and merged forms with pytest.approx (1e-6)."""
+    import capytaine_amd as cpa
+    mesh = _meshes()[name]
+    out = []
+    for m in (mesh, mesh.merged()):
+        body = cpa.FloatingBody(m, cpa.rigid_body_dofs())
+        pbs = [cpa.RadiationProblem(body=body, radiating_dof="Heave", omega=w) for w in (0.7, 2.1)]
+        pbs += [cpa.DiffractionProblem(body=body, omega=1.1, wave_direction=0.3)]
+        out.append([r.forces for r in solver.solve_all(pbs)])
+    tol = 1e-10 if name.startswith("sphere") else 1e-6
+    for fs, fm in zip(*out):
+        scale = max(abs(v) for v in fm.values())
+        for dof in fm:
+            assert abs(fs[dof] - fm[dof]) <= tol * scale, dof
+
+
+def test_symmetric_body_with_symmetric_lid_and_post_processing(solver):
+    """A reflection-symmetric hull with a reflection-symmetric lid stays
+    symmetric; its solve equals the merged form's and the post-processing
+    on it runs on the merged mesh."""
+    import capytaine_amd as cpa
+    from capytaine_amd.symmetry import symmetry_of
+    hull = cpa.mesh_parallelepiped(size=(2.0, 1.0, 1.0), center=(0.0, 0.0, -0.5), resolution=(8, 4, 4),
+                                   missing_sides={"top"}, reflection_symmetry=True)
+    lid_q = cpa.mesh_rectangle(size=(1.0, 0.5), resolution=(4, 2), center=(-0.5, -0.25, 0.0), normal=(0, 0, -1))
+    lid = cpa.ReflectionSymmetricMesh(cpa.ReflectionSymmetricMesh(lid_q, "yOz"), "xOz")
+    body = cpa.FloatingBody(hull, cpa.rigid_body_dofs(), lid_mesh=lid)
+    assert symmetry_of(body.mesh_including_lid) is not None
+    plain = cpa.FloatingBody(hull.merged(), cpa.rigid_body_dofs(), lid_mesh=lid.merged())
+    assert body.hull_mask.sum() == hull.nb_faces
+    res = [solver.solve(cpa.DiffractionProblem(body=b, omega=1.2, wave_direction=0.2), keep_details=True)
+           for b in (body, plain)]
+    scale = max(abs(v) for v in res[1].forces.values())
+    for dof in res[1].forces:
+        assert abs(res[0].forces[dof] - res[1].forces[dof]) <= 1e-6 * scale
+    eta = [solver.compute_free_surface_elevation(np.array([[3.0, 2.0], [-2.5, 1.0]]), r) for r in res]
+    assert np.abs(eta[0] - eta[1]).max() <= 1e-6 * np.abs(eta[1]).max()
