@@ -16,10 +16,11 @@ difference of any coefficient between the two forms are reported beside it.
 
 Prints one JSON line (see the repository's bench contract).
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline] [--no-fleet] [--no-serving]
-                    [--no-post]
+                    [--no-post] [--no-symmetric]
     python bench.py --workload cylinders     # 20000-panel finite-depth array instead
     python bench.py --workload serving       # the request-stream measurement alone
     python bench.py --workload post          # the post-processing maps alone
+    python bench.py --workload symmetric     # the barge as two reflection planes alone
 
 The request-stream measurement runs in a child process (started after the
 headline and fleet legs have released their device memory), so that a
@@ -340,6 +341,76 @@ def post_processing_benchmark(gf, dev, mesh, body, headings, steps=3, rho=1025.0
     return out
 
 
+def symmetric_benchmark(gf, dev, steps=3, warmup=1, rho=1025.0, g=9.81, ref_forces=None, log=None):
+    """The barge workload with both reflection planes (four blocks of 1088
+    panels): per step, the first block column of S and K for the 40
+    wavenumbers (one launch), the block transforms, one batched FP64 LU of
+    the 160 transformed blocks, the 560 solves, S @ sources and the forces,
+    inputs resident in HBM.  `ref_forces` (the plain FP64 form's) gives the
+    largest relative difference of the coefficients."""
+    import torch
+    import capytaine_amd as cpa
+    from capytaine_amd import _lib
+    from capytaine_amd.sweep import boundary_conditions
+    from capytaine_amd.symmetry import BlockSymmetricDeviceMatrix, LUDecomposedSymmetricMatrix, symmetry_of
+    mesh = cpa.mesh_parallelepiped(size=(20, 8, 2), center=(0, 0, -1), resolution=(80, 32, 8), missing_sides={"top"},
+                                   reflection_symmetry=True)
+    body = cpa.FloatingBody(mesh, name="barge").add_all_rigid_body_dofs(rotation_center=(0, 0, 0))
+    _, _, omegas, headings = barge_workload()
+    pbs_by_freq = problems_by_frequency(body, omegas, headings, rho, g)
+    sym = symmetry_of(mesh)
+    n = mesh.nb_faces
+    nrhs = len(pbs_by_freq[0])
+    ks = [p[0].wavenumber for p in pbs_by_freq]
+    nb = len(ks)
+    bc = boundary_conditions(pbs_by_freq, nrhs, n, dev)
+    W = -(body.dof_normal_matrix() * mesh.faces_areas[None, :]).T
+    W_d = torch.as_tensor(np.ascontiguousarray(W), dtype=torch.complex128).to(dev)
+    ndof = W.shape[1]
+    ctx = _lib.Context.get(dev)
+    merged = mesh.merged()
+    G = torch.empty((nb, nrhs, ndof), dtype=torch.complex128, device=dev)
+
+    def step():
+        S, K = gf.evaluate_on_device(merged, sym.block_mesh, wavenumber=ks, adjoint_double_layer=True,
+                                     diagonal_term_in_double_layer=True)
+        lu = LUDecomposedSymmetricMatrix(BlockSymmetricDeviceMatrix(K, sym))
+        X = bc.clone()
+        lu.solve_device(X)
+        Phi = BlockSymmetricDeviceMatrix(S, sym).matmul_device(X)
+        _lib.check(_lib.load().cpt_zgemm(ctx.handle, ndof, nrhs, n, 1.0, _lib.ptr(W_d), ndof, 0, _lib.ptr(Phi), n,
+                                         n * nrhs, _lib.ptr(G), ndof, ndof * nrhs, 1, nb, _lib.stream_handle(dev)),
+                   "cpt_zgemm")
+        return lu
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    _lib.profile_begin(ctx)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        lu = step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    kern = _lib.profile_end(ctx)
+    lu.check_info()
+    forces = 1j * np.asarray(omegas)[:, None, None] * rho * G.cpu().numpy()
+    out = {"workload": "barge 20x8x2 m, 4352 panels as two reflection planes (4 blocks of 1088), 40 frequencies, "
+                       "560 problems, FP64",
+           "problems_per_s": nb * nrhs / dt, "ms_per_step": dt * 1e3,
+           "assembly_ms_per_step": kern["assembly"]["ms"] / steps,
+           "assembly_panel_pairs_per_s": kern["assembly"]["work"] / (kern["assembly"]["ms"] / 1e3),
+           "lu_blocks": nb * sym.n, "block_order": sym.w}
+    if ref_forces is not None:
+        scale = np.abs(ref_forces).max(axis=2, keepdims=True)
+        out["max_rel_diff_vs_plain_fp64"] = float(np.max(np.abs(forces - ref_forces) / scale))
+    if log:
+        log(f"[symmetric] {dt * 1e3:.1f} ms per step, {nb * nrhs / dt:.0f} problems/s")
+    del lu
+    torch.cuda.empty_cache()
+    return out
+
+
 def copy_calibration(nbytes=1 << 30, reps=4):
     """Element-wise copy of `nbytes` (float32, y = 1 * x) `reps` times: the
     HBM counters of this kernel (FETCH_SIZE, WRITE_SIZE) against the known
@@ -435,9 +506,11 @@ def main():
     ap.add_argument("--no-fleet", action="store_true", help="skip the 64-hull batched measurement")
     ap.add_argument("--no-serving", action="store_true", help="skip the request-stream latency measurement")
     ap.add_argument("--no-post", action="store_true", help="skip the post-processing (elevation, velocity) maps")
+    ap.add_argument("--no-symmetric", action="store_true", help="skip the reflection-symmetric barge measurement")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner (a stored tuning of this size class is still applied)")
-    ap.add_argument("--workload", choices=("barge", "cylinders", "copy", "serving", "post"), default="barge",
+    ap.add_argument("--workload", choices=("barge", "cylinders", "copy", "serving", "post", "symmetric"),
+                    default="barge",
                     help="barge: the headline line (plus fleet and serving); cylinders: the 20000-panel "
                          "finite-depth array; copy: a 1 GiB device copy, the calibration of the HBM "
                          "byte counters; serving: the request-stream measurement alone")
@@ -463,6 +536,15 @@ def main():
         print(json.dumps(res), flush=True)
         return
 
+    if args.workload == "symmetric":
+        import torch
+        import capytaine_amd as cpa
+        dev = torch.device("cuda", 0)
+        torch.cuda.set_device(dev)
+        res = symmetric_benchmark(cpa.Delhommeau(device=dev), dev, steps=args.steps, warmup=args.warmup,
+                                  log=lambda m: print(m, file=sys.stderr, flush=True))
+        print(json.dumps(res), flush=True)
+        return
     if args.workload == "post":
         import torch
         import capytaine_amd as cpa
@@ -646,6 +728,10 @@ def main():
         "roofline": roofline,
         "tuning": {k: {kk: vv for kk, vv in v.items() if kk != "timings_ms"} for k, v in tuning.items()},
     }
+    if not args.no_symmetric:
+        out["symmetric"] = symmetric_benchmark(gf, dev, steps=args.steps, warmup=args.warmup, rho=rho, g=g,
+                                               ref_forces=r64["forces"],
+                                               log=lambda m: print(m, file=sys.stderr, flush=True))
     if not args.no_post:
         out["post_processing"] = post_processing_benchmark(gf, dev, mesh, body, headings, rho=rho, g=g,
                                                            log=lambda m: print(m, file=sys.stderr, flush=True))
