@@ -8,7 +8,7 @@ given by x and y only; z is then 0.
 """
 import numpy as np
 
-from .meshes import Mesh
+from .meshes import Mesh, SymmetricMesh
 from .problems import FloatingBody
 
 
@@ -17,7 +17,7 @@ def _normalize_points(points, keep_mesh=False):
     itself when keep_mesh (the gradient needs the face normals of a mesh)."""
     if isinstance(points, FloatingBody):
         points = points.mesh
-    if isinstance(points, Mesh):
+    if isinstance(points, (Mesh, SymmetricMesh)):
         return (points if keep_mesh else points.faces_centers), (points.nb_faces,)
     points = np.asarray(points)
     if points.ndim == 1:  # one point
@@ -31,7 +31,7 @@ def _normalize_points(points, keep_mesh=False):
 
 def _normalize_free_surface_points(points, keep_mesh=False):
     """As _normalize_points; (N, 2) points get z = 0."""
-    if keep_mesh and isinstance(points, (FloatingBody, Mesh)):
+    if keep_mesh and isinstance(points, (FloatingBody, Mesh, SymmetricMesh)):
         return _normalize_points(points, keep_mesh=True)
     points, output_shape = _normalize_points(points, keep_mesh)
     if points.ndim == 2 and points.shape[1] == 2:

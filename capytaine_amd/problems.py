@@ -27,6 +27,10 @@ class TranslationDof:
     def evaluate_motion(self, mesh):
         return np.tile(self.direction, (mesh.nb_faces, 1))
 
+    def evaluate_gradient_of_motion(self, mesh):
+        """(faces, 3, 3) Jacobian of the motion, zero (dofs.py:71-72)."""
+        return np.zeros((mesh.nb_faces, 3, 3))
+
 
 class RotationDof:
     def __init__(self, rotation_center, direction):
@@ -35,6 +39,12 @@ class RotationDof:
 
     def evaluate_motion(self, mesh):
         return np.cross(self.direction, mesh.faces_centers - self.rotation_center)
+
+    def evaluate_gradient_of_motion(self, mesh):
+        """(faces, 3, 3) Jacobian d(motion_i)/d(x_j) = (direction x e_j)_i
+        (dofs.py:92-96)."""
+        grad = np.cross(self.direction, np.eye(3)).T
+        return np.tile(grad, (mesh.nb_faces, 1, 1))
 
 
 def rigid_body_dofs(only=None, rotation_center=None):
@@ -136,14 +146,14 @@ def wavenumber_from_omega(omega, g, water_depth):
 
 class LinearPotentialFlowProblem:
     def __init__(self, *, body=None, free_surface=0.0, water_depth=np.inf, omega=None, wavenumber=None,
-                 rho=1000.0, g=9.81, wave_direction=0.0):
+                 rho=1000.0, g=9.81, wave_direction=0.0, forward_speed=0.0):
         self.body = body
         self.free_surface = float(free_surface)
         self.water_depth = float(water_depth)
         self.rho = float(rho)
         self.g = float(g)
         self.wave_direction = float(wave_direction)
-        self.forward_speed = 0.0
+        self.forward_speed = float(forward_speed)
         if omega is not None and wavenumber is not None:
             raise ValueError("Provide omega or wavenumber, not both")
         if wavenumber is not None:
@@ -155,9 +165,21 @@ class LinearPotentialFlowProblem:
                 self.wavenumber = self.omega
             else:
                 self.wavenumber = float(wavenumber_from_omega(self.omega, self.g, self.water_depth))
-        self.encounter_omega = self.omega
-        self.encounter_wavenumber = self.wavenumber
-        self.encounter_wave_direction = self.wave_direction
+        if self.omega in (0.0, np.inf) and self.forward_speed != 0.0:
+            raise NotImplementedError(f"omega={self.omega} is only implemented without forward speed "
+                                      f"(provided forward speed: {self.forward_speed}).")
+        if self.forward_speed != 0.0:
+            # problems_and_results.py:96-105: Doppler-shifted frequency; a
+            # body faster than the waves meets them from the other side
+            dopplered = self.omega - self.wavenumber * self.forward_speed * np.cos(self.wave_direction)
+            self.encounter_omega = float(abs(dopplered))
+            self.encounter_wavenumber = (self.encounter_omega if self.encounter_omega == 0.0 else
+                                         float(wavenumber_from_omega(self.encounter_omega, self.g, self.water_depth)))
+            self.encounter_wave_direction = self.wave_direction + (0.0 if dopplered >= 0.0 else np.pi)
+        else:
+            self.encounter_omega = self.omega
+            self.encounter_wavenumber = self.wavenumber
+            self.encounter_wave_direction = self.wave_direction
 
     @property
     def influenced_dofs(self):
@@ -165,7 +187,7 @@ class LinearPotentialFlowProblem:
 
     def _group_key(self):
         """problems_and_results.py:266-275 grouping key."""
-        return (id(self.body), self.water_depth, self.omega, self.rho, self.g, self.free_surface)
+        return (id(self.body), self.water_depth, self.omega, self.rho, self.g, self.free_surface, self.forward_speed)
 
 
 def _airy_depth_terms(k, h, z):
@@ -236,6 +258,17 @@ class RadiationProblem(LinearPotentialFlowProblem):
         # problems_and_results.py:436-447: zero on the lid
         self.boundary_condition = np.zeros(body.mesh_including_lid.nb_faces, dtype=np.complex128)
         self.boundary_condition[body.hull_mask] = -1j * self.encounter_omega * displacement
+        if self.forward_speed != 0.0:
+            # the m-terms (problems_and_results.py:451-464): -U (d motion / dx) . n
+            dof = body.dofs[self.radiating_dof]
+            if not hasattr(dof, "evaluate_gradient_of_motion"):
+                raise NotImplementedError(
+                    "Radiation problem with forward speed is currently only implemented for rigid bodies.\n"
+                    "Only radiating dofs instantiating a TranslationDof or a RotationDof are supported.\n"
+                    f"Got instead `radiating_dof={self.radiating_dof}`")
+            ddofdx = dof.evaluate_gradient_of_motion(body.mesh)[:, :, 0]
+            self.boundary_condition[body.hull_mask] += -self.forward_speed * np.sum(ddofdx * body.mesh.faces_normals,
+                                                                                     axis=1)
 
     def make_results_container(self, forces=None, sources=None, potential=None, pressure=None):
         return RadiationResult(self, forces, sources, potential, pressure)
@@ -278,6 +311,7 @@ class LinearPotentialFlowResult:
         self.pressure = pressure
         self.omega = problem.omega
         self.encounter_omega = problem.encounter_omega
+        self.encounter_wave_direction = problem.encounter_wave_direction
         self.wavenumber = problem.wavenumber
         self.encounter_wavenumber = problem.encounter_wavenumber
         self.body = problem.body
@@ -301,7 +335,7 @@ class RadiationResult(LinearPotentialFlowResult):
     @property
     def radiation_damping(self):
         w = self.encounter_omega
-        if float(w) in {0.0, np.inf}:
+        if float(w) in {0.0, np.inf} and self.forward_speed == 0.0:
             return {dof: 0.0 for dof in self.forces}
         return {dof: float(np.imag(f) / w) for dof, f in self.forces.items()}
 

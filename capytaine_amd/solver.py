@@ -76,10 +76,13 @@ class BEMSolver:
     def _solve(self, problem, method=None, keep_details=True):
         if isinstance(problem, DiffractionProblem) and float(problem.encounter_omega) in {0.0, np.inf}:
             raise ValueError("Diffraction problems at zero or infinite frequency are not defined")
-        omega, wavenumber = problem.omega, problem.wavenumber
+        # solver.py:163-167: with forward speed, the encounter frequency
+        omega, wavenumber = problem.encounter_omega, problem.encounter_wavenumber
         gf_params = dict(free_surface=problem.free_surface, water_depth=problem.water_depth, wavenumber=wavenumber)
         mesh = problem.body.mesh_including_lid
         method = method if method is not None else self.method
+        if method == "direct" and problem.forward_speed != 0.0:
+            raise NotImplementedError("Direct solver is not able to solve problems with forward speed.")
         if method == "direct":
             with self.timer("Green function"):
                 S, D = self.engine.build_matrices(mesh, mesh, **gf_params, adjoint_double_layer=False,
@@ -98,6 +101,11 @@ class BEMSolver:
             with self.timer("Matrix-vector product"):
                 potential = S @ sources
         pressure = 1j * omega * problem.rho * potential
+        if problem.forward_speed != 0.0:
+            # solver.py:200-204: + rho U d(phi)/dx, the gradient on the panels
+            # of the mesh itself (with the diagonal term of build_fullK_matrix)
+            nabla_phi = self._compute_potential_gradient(mesh, problem.make_results_container(sources=sources))
+            pressure = pressure + problem.rho * problem.forward_speed * nabla_phi[:, 0]
         forces = problem.body.integrate_pressure(pressure[problem.body.hull_mask])
         if not keep_details:
             return problem.make_results_container(forces)
@@ -128,6 +136,11 @@ class BEMSolver:
         results = [None] * len(problems)
         groups = OrderedDict()
         for i, pb in enumerate(problems):
+            if pb.forward_speed != 0.0:
+                # one problem at a time: its matrices depend on the encounter
+                # frequency, and its pressure on the full gradient on the hull
+                results[i] = self._solve_and_catch_errors(pb, keep_details=keep_details)
+                continue
             groups.setdefault((id(pb.body), pb.water_depth, pb.rho, pb.g, pb.free_surface), []).append(i)
         for key, idx in groups.items():
             sub = [problems[i] for i in idx]

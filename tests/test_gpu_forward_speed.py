@@ -1,0 +1,125 @@
+"""Forward speed on the device (solver.py:163-204): matrices at the encounter
+wavenumber, the m-terms in the radiation conditions, and the pressure
+i omega_e rho phi + rho U d(phi)/dx with the full gradient on the hull.
+
+Parity: against the oracle pipeline (oracle S and K at the encounter
+wavenumber, scipy LU, oracle full gradient with the diagonal term) at 1e-10;
+the pressure reconstructed by compute_pressure integrates to the forces
+(pytest/test_forward_speed.py:97-99).  Validation: Malenica (1995)'s
+bottom-mounted cylinder in finite depth, Froude numbers -0.05, 0 and 0.05,
+the digitised values of the reference's test (test_forward_speed.py:
+160-222) within its tolerances (5 % excitation, 10 % added mass and
+damping)."""
+import numpy as np
+import pytest
+
+from gpu_helpers import ref_table_gf
+
+pytestmark = pytest.mark.gpu
+
+RHO, G = 1025.0, 9.81
+
+
+@pytest.fixture(scope="module")
+def solver(default_table):
+    import capytaine_amd as cpa
+    print("[gpu] forward speed: solver with the reference table", flush=True)
+    return cpa.BEMSolver(engine=cpa.DefaultMatrixEngine(green_function=ref_table_gf(default_table)))
+
+
+def _oracle_forces(orc, table, pb):
+    from oracle import mesh_arrays
+    from scipy.linalg import lu_factor, lu_solve
+    mesh = pb.body.mesh_including_lid
+    arr = mesh_arrays(mesh)
+    S, K = orc.build_matrices(**arr, wavenumber=pb.encounter_wavenumber, depth=np.inf, table=table)
+    sigma = lu_solve(lu_factor(K), pb.boundary_condition)
+    _, gradK = orc.build_matrices(**arr, wavenumber=pb.encounter_wavenumber, depth=np.inf, table=table,
+                                  early_dot_product=False)
+    pressure = 1j * pb.encounter_omega * pb.rho * (S @ sigma) + pb.rho * pb.forward_speed * (gradK[0] @ sigma)
+    return pb.body.integrate_pressure(pressure[pb.body.hull_mask])
+
+
+@pytest.mark.parametrize("speed", [1.0, -0.6])
+def test_forward_speed_vs_oracle(solver, orc, default_table, speed):
+    import capytaine_amd as cpa
+    print(f"[gpu] forward speed: vs oracle, U = {speed}", flush=True)
+    mesh = cpa.mesh_sphere(radius=1.0, resolution=(12, 16)).immersed_part()
+    body = cpa.FloatingBody(mesh, cpa.rigid_body_dofs(rotation_center=(0.0, 0.0, -0.2)))
+    pbs = [cpa.RadiationProblem(body=body, omega=1.5, forward_speed=speed, wave_direction=0.3, radiating_dof=d)
+           for d in ("Surge", "Pitch", "Yaw")]
+    pbs.append(cpa.DiffractionProblem(body=body, omega=1.5, forward_speed=speed, wave_direction=0.3))
+    for pb, res in zip(pbs, solver.solve_all(pbs)):
+        ref = _oracle_forces(orc, default_table, pb)
+        scale = max(abs(v) for v in ref.values())
+        for dof in ref:
+            assert abs(res.forces[dof] - ref[dof]) <= 1e-10 * scale, (pb, dof)
+
+
+def test_pressure_reconstruction(solver):
+    import capytaine_amd as cpa
+    mesh = cpa.mesh_sphere(radius=1.0, resolution=(12, 16)).immersed_part()
+    body = cpa.FloatingBody(mesh, cpa.rigid_body_dofs(only=["Surge", "Heave", "Pitch"]))
+    res = solver.solve(cpa.DiffractionProblem(body=body, omega=2.0, forward_speed=1.0, wave_direction=0.0),
+                       keep_details=True)
+    f = body.integrate_pressure(solver.compute_pressure(body.mesh, res))
+    scale = max(abs(v) for v in res.forces.values())
+    for dof in res.forces:
+        assert abs(f[dof] - res.forces[dof]) <= 1e-9 * scale
+    # z = -1.5, not -1: (0, 0, -1) is a vertex of the sphere
+    pts = np.meshgrid(np.linspace(-10.0, 10.0, 3), np.linspace(-10.0, 10.0, 3), np.linspace(-10.0, -1.5, 2))
+    assert solver.compute_velocity(pts, res).shape == (3, 3, 2, 3)
+    fse = solver.compute_free_surface_elevation(np.meshgrid(np.linspace(-10, 10, 3), np.linspace(-10, 10, 3)), res)
+    assert fse.shape == (3, 3) and np.all(np.isfinite(fse))
+
+
+def _malenica_body():
+    """Vertical cylinder R = 1 standing on the sea bottom in depth 1, open at
+    both ends (the reference clips a closed one at the bottom and the free
+    surface), 20 panels around, 10 along."""
+    import capytaine_amd as cpa
+    from capytaine_amd.meshes import _revolution
+    profile = np.array([(1.0, 0.0, z) for z in np.linspace(-1.0, 0.0, 11)])
+    mesh = _revolution(profile, 20, "cylinder")
+    return cpa.FloatingBody(mesh, cpa.rigid_body_dofs(only=["Surge", "Heave", "Pitch"], rotation_center=(0, 0, -0.5)),
+                            name="body")
+
+
+# (wavenumber, normalised value, Froude number), Malenica 1995 figure 1, as digitised
+# in pytest/test_forward_speed.py:160-222
+EXCITATION = [(0.269, 1.662, 0.05), (0.615, 3.192, 0.05), (1.086, 3.168, 0.05), (1.568, 2.225, 0.05),
+              (1.955, 1.608, 0.05), (0.201, 1.281, 0.0), (0.506, 2.928, 0.0), (1.308, 2.733, 0.0),
+              (1.840, 1.890, 0.0), (0.353, 2.263, -0.05), (0.568, 3.224, -0.05), (1.171, 2.963, -0.05),
+              (1.977, 1.893, -0.05)]
+ADDED_MASS = [(0.620, 3.302, 0.05), (0.855, 2.634, 0.05), (1.303, 1.407, 0.05), (0.510, 3.433, 0.0),
+              (0.916, 2.217, 0.0), (1.495, 0.916, 0.0), (0.602, 3.172, -0.05), (1.069, 1.518, -0.05)]
+DAMPING = [(0.456, 0.965, 0.05), (0.727, 1.917, 0.05), (1.129, 2.284, 0.05), (0.434, 0.978, 0.0),
+           (0.674, 1.884, 0.0), (1.125, 2.218, 0.0), (0.408, 0.971, -0.05), (0.642, 1.906, -0.05),
+           (1.042, 2.234, -0.05)]
+
+
+def test_malenica_cylinder(solver):
+    import capytaine_amd as cpa
+    print("[gpu] forward speed: Malenica cylinder (30 problems, depth 1)", flush=True)
+    body = _malenica_body()
+    failures = []
+    for k, ref, fr in EXCITATION:
+        pb = cpa.DiffractionProblem(body=body, wavenumber=k, forward_speed=fr * np.sqrt(G), wave_direction=0.0,
+                                    water_depth=1.0, rho=RHO)
+        f = solver.solve(pb).forces["Surge"] + cpa.froude_krylov_force(pb)["Surge"]
+        val = abs(f) / (G * RHO)
+        if abs(val - ref) > 5e-2 * ref:
+            failures.append(("excitation", k, fr, val, ref))
+    for k, ref, fr in ADDED_MASS:
+        pb = cpa.RadiationProblem(body=body, wavenumber=k, forward_speed=fr * np.sqrt(G), wave_direction=0.0,
+                                  water_depth=1.0, radiating_dof="Surge", rho=RHO)
+        val = abs(solver.solve(pb).added_mass["Surge"]) / RHO
+        if abs(val - ref) > 1e-1 * ref:
+            failures.append(("added mass", k, fr, val, ref))
+    for k, ref, fr in DAMPING:
+        pb = cpa.RadiationProblem(body=body, wavenumber=k, forward_speed=fr * np.sqrt(G), wave_direction=0.0,
+                                  water_depth=1.0, radiating_dof="Surge", rho=RHO)
+        val = abs(solver.solve(pb).radiation_damping["Surge"]) / (RHO * pb.encounter_omega)
+        if abs(val - ref) > 1e-1 * ref:
+            failures.append(("damping", k, fr, val, ref))
+    assert not failures, failures
