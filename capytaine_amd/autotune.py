@@ -31,7 +31,9 @@ PANEL_LEAVES = (8, 16)
 # from a partial mask; it remains available through cpt_set_tuning.
 LOOKAHEAD = (0, 1)
 PANEL_CUS = (16, 32)         # reserved CUs tried with LOOKAHEAD = 2
-GEMM_TILES = (0, 1)          # complex64 product tiles: 128 x 64, 128 x 128
+# complex64 product tiles: 128 x 64, 128 x 128, and the three-product (Gauss)
+# forms 128 x 128 (one wavefront per SIMD) and 128 x 64
+GEMM_TILES = (0, 1, 2, 3)
 ASSEMBLY_CHUNKS = (0, 4, 8, 16, 32, 64)
 
 _lock = threading.Lock()

@@ -1015,11 +1015,14 @@ __global__ void __launch_bounds__(256) gemm_skinny_kernel(int M, int N, int K, R
 // ---------------------------------------------------------------------------
 constexpr int TBM = 128, TKC = 16;
 
-template <typename R, int TBN>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
-gemm_tiled_kernel(int M, int N, int K, R alpha, const cx<R>* __restrict__ Abase, int64_t lda, int64_t strideA,
-                  const cx<R>* __restrict__ Bbase, int64_t ldb, int64_t strideB, cx<R>* __restrict__ Cbase,
-                  int64_t ldc, int64_t strideC, int beta_zero)
+// GAUSS: three real products per complex one, P1 = Ar Br, P2 = Ai Bi,
+// P3 = (Ar + Ai)(Br + Bi), Re = P1 - P2, Im = P3 - P1 - P2 (6 M N K real
+// MFMA flops for the 8 M N K of the product; one more accumulator set).
+template <typename R, int TBN, bool GAUSS>
+__device__ __forceinline__ void gemm_tiled_body(int M, int N, int K, R alpha, const cx<R>* __restrict__ Abase,
+                                                int64_t lda, int64_t strideA, const cx<R>* __restrict__ Bbase,
+                                                int64_t ldb, int64_t strideB, cx<R>* __restrict__ Cbase, int64_t ldc,
+                                                int64_t strideC, int beta_zero)
 {
     typedef cx<R> Cx;
     typedef typename Mfma<R>::acc acc;
@@ -1039,13 +1042,14 @@ gemm_tiled_kernel(int M, int N, int K, R alpha, const cx<R>* __restrict__ Abase,
     Cx ra[8], rb[NB_LOAD];
     const int a_row = row0 + (tid & 127);
     const bool a_ok = a_row < M;
-    acc cr[4][FB], ci[4][FB];
+    acc cr[4][FB], ci[4][FB], cs[GAUSS ? 4 : 1][GAUSS ? FB : 1];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < FB; ++b) {
             cr[a][b] = acc{0, 0, 0, 0};
             ci[a][b] = acc{0, 0, 0, 0};
+            if (GAUSS) cs[GAUSS ? a : 0][GAUSS ? b : 0] = acc{0, 0, 0, 0};
         }
 
 #define CPT_LOAD_CHUNK(kc_)                                                                   \
@@ -1091,24 +1095,49 @@ gemm_tiled_kernel(int M, int N, int K, R alpha, const cx<R>* __restrict__ Abase,
                 br[b] = v.re;
                 bi[b] = v.im;
             }
+            if (GAUSS) {
+                // cr: P1, ci: P2, cs: P3
 #pragma unroll
-            for (int a = 0; a < 4; ++a)
+                for (int a = 0; a < 4; ++a)
 #pragma unroll
-                for (int b = 0; b < FB; ++b) {
-                    cr[a][b] = Mfma<R>::op(ar[a], br[b], cr[a][b]);
-                    ci[a][b] = Mfma<R>::op(ar[a], bi[b], ci[a][b]);
-                }
+                    for (int b = 0; b < FB; ++b) {
+                        cr[a][b] = Mfma<R>::op(ar[a], br[b], cr[a][b]);
+                        ci[a][b] = Mfma<R>::op(ai[a], bi[b], ci[a][b]);
+                        cs[GAUSS ? a : 0][GAUSS ? b : 0] =
+                            Mfma<R>::op(ar[a] + ai[a], br[b] + bi[b], cs[GAUSS ? a : 0][GAUSS ? b : 0]);
+                    }
+            } else {
 #pragma unroll
-            for (int a = 0; a < 4; ++a)
+                for (int a = 0; a < 4; ++a)
 #pragma unroll
-                for (int b = 0; b < FB; ++b) {
-                    cr[a][b] = Mfma<R>::op(-ai[a], bi[b], cr[a][b]);
-                    ci[a][b] = Mfma<R>::op(ai[a], br[b], ci[a][b]);
-                }
+                    for (int b = 0; b < FB; ++b) {
+                        cr[a][b] = Mfma<R>::op(ar[a], br[b], cr[a][b]);
+                        ci[a][b] = Mfma<R>::op(ar[a], bi[b], ci[a][b]);
+                    }
+#pragma unroll
+                for (int a = 0; a < 4; ++a)
+#pragma unroll
+                    for (int b = 0; b < FB; ++b) {
+                        cr[a][b] = Mfma<R>::op(-ai[a], bi[b], cr[a][b]);
+                        ci[a][b] = Mfma<R>::op(ai[a], br[b], ci[a][b]);
+                    }
+            }
         }
         __syncthreads();
     }
 #undef CPT_LOAD_CHUNK
+    if (GAUSS) {
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < FB; ++b)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const R p1 = cr[a][b][v], p2 = ci[a][b][v];
+                    cr[a][b][v] = p1 - p2;
+                    ci[a][b][v] = cs[GAUSS ? a : 0][GAUSS ? b : 0][v] - p1 - p2;
+                }
+    }
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
@@ -1127,6 +1156,28 @@ gemm_tiled_kernel(int M, int N, int K, R alpha, const cx<R>* __restrict__ Abase,
                     c->im = im;
                 }
             }
+}
+
+template <typename R, int TBN, bool GAUSS = false>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+gemm_tiled_kernel(int M, int N, int K, R alpha, const cx<R>* __restrict__ Abase, int64_t lda, int64_t strideA,
+                  const cx<R>* __restrict__ Bbase, int64_t ldb, int64_t strideB, cx<R>* __restrict__ Cbase,
+                  int64_t ldc, int64_t strideC, int beta_zero)
+{
+    gemm_tiled_body<R, TBN, GAUSS>(M, N, K, alpha, Abase, lda, strideA, Bbase, ldb, strideB, Cbase, ldc, strideC,
+                                   beta_zero);
+}
+
+// The Gauss form with one wavefront per SIMD: its three accumulator sets fit
+// the registers of a lane without spilling where two wavefronts would not.
+template <typename R, int TBN>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
+gemm_tiled_gauss_w1_kernel(int M, int N, int K, R alpha, const cx<R>* __restrict__ Abase, int64_t lda,
+                           int64_t strideA, const cx<R>* __restrict__ Bbase, int64_t ldb, int64_t strideB,
+                           cx<R>* __restrict__ Cbase, int64_t ldc, int64_t strideC, int beta_zero)
+{
+    gemm_tiled_body<R, TBN, true>(M, N, K, alpha, Abase, lda, strideA, Bbase, ldb, strideB, Cbase, ldc, strideC,
+                                  beta_zero);
 }
 
 
@@ -1242,10 +1293,29 @@ int gemm(int M, int N, int K, R alpha, const cx<R>* A, int64_t lda, int64_t sA, 
         // complex64: 128 x 128 tiles for wide products when tuned so
         // (complex128 would need more than the 512 registers of a lane)
         if constexpr (sizeof(R) == 4) {
-            if (active_gemm_tiles && N > 64) {
+            if (active_gemm_tiles == 1 && N > 64) {
                 dim3 grid((M + TBM - 1) / TBM, (N + 127) / 128, batch);
                 hipLaunchKernelGGL((gemm_tiled_kernel<R, 128>), grid, dim3(256), 0, stream, M, N, K, alpha, A, lda,
                                    sA, B, ldb, sB, C, ldc, sC, beta_zero);
+                return (int)hipGetLastError();
+            }
+            if (active_gemm_tiles == 2 && N > 64) {
+                dim3 grid((M + TBM - 1) / TBM, (N + 127) / 128, batch);
+                hipLaunchKernelGGL((gemm_tiled_gauss_w1_kernel<R, 128>), grid, dim3(256), 0, stream, M, N, K,
+                                   alpha, A, lda, sA, B, ldb, sB, C, ldc, sC, beta_zero);
+                return (int)hipGetLastError();
+            }
+            if (active_gemm_tiles == 3) {
+                dim3 grid((M + TBM - 1) / TBM, (N + 63) / 64, batch);
+                hipLaunchKernelGGL((gemm_tiled_kernel<R, 64, true>), grid, dim3(256), 0, stream, M, N, K, alpha, A,
+                                   lda, sA, B, ldb, sB, C, ldc, sC, beta_zero);
+                return (int)hipGetLastError();
+            }
+        } else {
+            if (active_gemm_tiles == 3) {
+                dim3 grid((M + TBM - 1) / TBM, (N + 63) / 64, batch);
+                hipLaunchKernelGGL((gemm_tiled_gauss_w1_kernel<R, 64>), grid, dim3(256), 0, stream, M, N, K, alpha,
+                                   A, lda, sA, B, ldb, sB, C, ldc, sC, beta_zero);
                 return (int)hipGetLastError();
             }
         }

@@ -246,9 +246,11 @@ def test_kernel_timing():
 
 @pytest.mark.parametrize("block,leaf,lookahead,tiles", [(64, 8, 0, 1), (64, 32, 1, 0), (128, 16, 1, 1),
                                                         (128, 64, 0, 0), (128, 16, 2, 1), (64, 8, 2, 0),
-                                                        (256, 16, 0, 0), (256, 8, 1, 0)])
+                                                        (256, 16, 0, 0), (256, 8, 1, 0), (256, 16, 1, 2),
+                                                        (128, 16, 1, 3), (256, 8, 0, 3)])
 def test_lu_tunings_agree(block, leaf, lookahead, tiles):
-    """Every tuning of the blocked LU solves to LAPACK's accuracy."""
+    """Every tuning of the blocked LU solves to LAPACK's accuracy, the
+    three-product (Gauss) tile products included (tiles 2 and 3)."""
     from scipy.linalg import lu_factor, lu_solve
     from capytaine_amd import _lib
     print(f"[gpu] LU tuning block={block} leaf={leaf} lookahead={lookahead} tiles={tiles}", flush=True)
@@ -313,7 +315,7 @@ def test_tuning_rejects_bad_values():
     from capytaine_amd import _lib
     ctx = _lib.Context.get(device())
     for name, bad in [("lu_block", 96), ("lu_block", 512), ("panel_leaf", 0), ("panel_leaf", 65), ("lookahead", 3),
-                      ("assembly_chunk", -1), ("panel_cus", 0), ("panel_cus", 129), ("gemm_tiles", 2)]:
+                      ("assembly_chunk", -1), ("panel_cus", 0), ("panel_cus", 129), ("gemm_tiles", 4)]:
         with pytest.raises(_lib.CptError):
             _lib.set_tuning(ctx, **{name: bad})
     with pytest.raises(_lib.CptError):
