@@ -508,8 +508,8 @@ def main():
     ap.add_argument("--no-post", action="store_true", help="skip the post-processing (elevation, velocity) maps")
     ap.add_argument("--no-symmetric", action="store_true", help="skip the reflection-symmetric barge measurement")
     ap.add_argument("--gemm-tiles", type=int, default=None,
-                    help="force the tile product variant (cpt_set_tuning gemm_tiles, 0-3) after tuning, "
-                         "for counter passes of one variant")
+                    help="force the complex64 tile product variant of the mixed form (cpt_set_tuning "
+                         "gemm_tiles, 0-3) after tuning, for profiler passes of one variant")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner (a stored tuning of this size class is still applied)")
     ap.add_argument("--workload", choices=("barge", "cylinders", "copy", "serving", "post", "symmetric"),
@@ -590,7 +590,7 @@ def main():
 
     def run(precision, far_field_fp32=False):
         applied = autotune.apply(ctx, n, len(ks), precision, nb_wavenumbers=len(ks))
-        if args.gemm_tiles is not None:
+        if args.gemm_tiles is not None and precision == "mixed":
             _lib.set_tuning(ctx, gemm_tiles=args.gemm_tiles)
             applied = dict(applied or {}, gemm_tiles=args.gemm_tiles)
         if applied:
