@@ -344,15 +344,16 @@ def post_processing_benchmark(gf, dev, mesh, body, headings, steps=3, rho=1025.0
 def symmetric_benchmark(gf, dev, steps=3, warmup=1, rho=1025.0, g=9.81, ref_forces=None, log=None):
     """The barge workload with both reflection planes (four blocks of 1088
     panels): per step, the first block column of S and K for the 40
-    wavenumbers (one launch), the block transforms, one batched FP64 LU of
-    the 160 transformed blocks, the 560 solves, S @ sources and the forces,
-    inputs resident in HBM.  `ref_forces` (the plain FP64 form's) gives the
+    wavenumbers (one launch), the block transforms, one batched LU of the 160
+    transformed blocks (FP64; then complex64 with FP64 refinement), the 560
+    solves, S @ sources and the forces, inputs resident in HBM.  `ref_forces` (the plain FP64 form's) gives the
     largest relative difference of the coefficients."""
     import torch
     import capytaine_amd as cpa
     from capytaine_amd import _lib
     from capytaine_amd.sweep import boundary_conditions
-    from capytaine_amd.symmetry import BlockSymmetricDeviceMatrix, LUDecomposedSymmetricMatrix, symmetry_of
+    from capytaine_amd.symmetry import (BlockSymmetricDeviceMatrix, LUDecomposedSymmetricMatrix,
+                                        MixedSymmetricSolver, symmetry_of)
     mesh = cpa.mesh_parallelepiped(size=(20, 8, 2), center=(0, 0, -1), resolution=(80, 32, 8), missing_sides={"top"},
                                    reflection_symmetry=True)
     body = cpa.FloatingBody(mesh, name="barge").add_all_rigid_body_dofs(rotation_center=(0, 0, 0))
@@ -371,42 +372,56 @@ def symmetric_benchmark(gf, dev, steps=3, warmup=1, rho=1025.0, g=9.81, ref_forc
     merged = mesh.merged()
     G = torch.empty((nb, nrhs, ndof), dtype=torch.complex128, device=dev)
 
-    def step():
+    def step(mixed):
         S, K = gf.evaluate_on_device(merged, sym.block_mesh, wavenumber=ks, adjoint_double_layer=True,
                                      diagonal_term_in_double_layer=True)
-        lu = LUDecomposedSymmetricMatrix(BlockSymmetricDeviceMatrix(K, sym))
-        X = bc.clone()
-        lu.solve_device(X)
+        if mixed:
+            lu = MixedSymmetricSolver(BlockSymmetricDeviceMatrix(K, sym))
+            X = lu.solve_device(bc)
+        else:
+            lu = LUDecomposedSymmetricMatrix(BlockSymmetricDeviceMatrix(K, sym))
+            X = bc.clone()
+            lu.solve_device(X)
         Phi = BlockSymmetricDeviceMatrix(S, sym).matmul_device(X)
         _lib.check(_lib.load().cpt_zgemm(ctx.handle, ndof, nrhs, n, 1.0, _lib.ptr(W_d), ndof, 0, _lib.ptr(Phi), n,
                                          n * nrhs, _lib.ptr(G), ndof, ndof * nrhs, 1, nb, _lib.stream_handle(dev)),
                    "cpt_zgemm")
         return lu
 
-    for _ in range(warmup):
-        step()
-    torch.cuda.synchronize()
-    _lib.profile_begin(ctx)
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        lu = step()
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / steps
-    kern = _lib.profile_end(ctx)
-    lu.check_info()
-    forces = 1j * np.asarray(omegas)[:, None, None] * rho * G.cpu().numpy()
     out = {"workload": "barge 20x8x2 m, 4352 panels as two reflection planes (4 blocks of 1088), 40 frequencies, "
-                       "560 problems, FP64",
-           "problems_per_s": nb * nrhs / dt, "ms_per_step": dt * 1e3,
-           "assembly_ms_per_step": kern["assembly"]["ms"] / steps,
-           "assembly_panel_pairs_per_s": kern["assembly"]["work"] / (kern["assembly"]["ms"] / 1e3),
+                       "560 problems; FP64 block-diagonal LU, and complex64 LU of the blocks with FP64 refinement",
            "lu_blocks": nb * sym.n, "block_order": sym.w}
+    forces = {}
+    for mixed in (False, True):
+        for _ in range(warmup):
+            step(mixed)
+        torch.cuda.synchronize()
+        _lib.profile_begin(ctx)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            lu = step(mixed)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        kern = _lib.profile_end(ctx)
+        forces[mixed] = 1j * np.asarray(omegas)[:, None, None] * rho * G.cpu().numpy()
+        tag = "mixed_" if mixed else ""
+        out[tag + "problems_per_s"] = nb * nrhs / dt
+        out[tag + "ms_per_step"] = dt * 1e3
+        if mixed:
+            out["mixed_refinement_sweeps"], out["mixed_final_rel_residual"] = lu.iterations, lu.residual
+        else:
+            lu.check_info()
+            out["assembly_ms_per_step"] = kern["assembly"]["ms"] / steps
+            out["assembly_panel_pairs_per_s"] = kern["assembly"]["work"] / (kern["assembly"]["ms"] / 1e3)
+        if log:
+            log(f"[symmetric] {'mixed' if mixed else 'fp64'}: {dt * 1e3:.1f} ms per step, "
+                f"{nb * nrhs / dt:.0f} problems/s")
+        del lu
+    scale = np.abs(forces[False]).max(axis=2, keepdims=True)
+    out["mixed_max_rel_err_vs_fp64"] = float(np.max(np.abs(forces[True] - forces[False]) / scale))
     if ref_forces is not None:
         scale = np.abs(ref_forces).max(axis=2, keepdims=True)
-        out["max_rel_diff_vs_plain_fp64"] = float(np.max(np.abs(forces - ref_forces) / scale))
-    if log:
-        log(f"[symmetric] {dt * 1e3:.1f} ms per step, {nb * nrhs / dt:.0f} problems/s")
-    del lu
+        out["max_rel_diff_vs_plain_fp64"] = float(np.max(np.abs(forces[False] - ref_forces) / scale))
     torch.cuda.empty_cache()
     return out
 

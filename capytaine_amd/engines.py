@@ -385,11 +385,10 @@ class DefaultMatrixEngine:
                 luA = A
             return luA.solve(b)
         if self._linear_solver == "mixed_precision":
-            if hasattr(A, "sym"):
-                raise NotImplementedError("The mixed-precision solver takes plain meshes; "
-                                          "symmetric meshes use the FP64 block-diagonal LU.")
-            if not isinstance(A, MixedPrecisionSolver):
-                solver = MixedPrecisionSolver(A)
+            from .symmetry import BlockSymmetricDeviceMatrix, MixedSymmetricSolver
+            if not isinstance(A, (MixedPrecisionSolver, MixedSymmetricSolver)):
+                solver = MixedSymmetricSolver(A) if isinstance(A, BlockSymmetricDeviceMatrix) else \
+                    MixedPrecisionSolver(A)
                 if self.last_computed_matrices is not None and A is self.last_computed_matrices[1]:
                     self.last_computed_matrices = (self.last_computed_matrices[0], solver)
             else:

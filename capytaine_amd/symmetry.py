@@ -205,3 +205,34 @@ class LUDecomposedSymmetricMatrix:
             x = x[0]
             return x[:, 0] if vec else x
         return x
+
+
+class MixedSymmetricSolver:
+    """The mixed-precision solve of a BlockSymmetricDeviceMatrix: right-hand
+    sides transformed, the n transformed blocks solved by the complex64 LU
+    with FP64 refinement against the complex128 blocks
+    (cpt_lu_solve_mixed), solutions transformed back."""
+
+    def __init__(self, A, max_iter=10, tol=1e-12):
+        from .engines import DeviceMatrix, MixedPrecisionSolver
+        self.sym = A.sym
+        self.shape, self.dtype, self.device, self.batch = A.shape, A.dtype, A.device, A.batch
+        self.inner = MixedPrecisionSolver(DeviceMatrix(A.diagonal_blocks()), max_iter=max_iter, tol=tol)
+
+    iterations = property(lambda self: self.inner.iterations)
+    residual = property(lambda self: self.inner.residual)
+
+    def solve_device(self, b):
+        """x for b (nb, r, N) on the device (b is left unchanged)."""
+        xh = self.inner.solve_device(transform_vectors(self.sym, b))
+        return transform_vectors(self.sym, xh, inverse=True)
+
+    def solve(self, b):
+        import torch
+        b = np.asarray(b)
+        vec = b.ndim == 1
+        bb = b.reshape(b.shape[0], -1)
+        bd = torch.as_tensor(np.ascontiguousarray(bb.T), dtype=torch.complex128).to(self.device)[None]
+        x = self.solve_device(bd).cpu().numpy()[0].T
+        return x[:, 0] if vec else x
+

@@ -167,3 +167,24 @@ def test_symmetric_body_with_symmetric_lid_and_post_processing(solver):
         assert abs(res[0].forces[dof] - res[1].forces[dof]) <= 1e-6 * scale
     eta = [solver.compute_free_surface_elevation(np.array([[3.0, 2.0], [-2.5, 1.0]]), r) for r in res]
     assert np.abs(eta[0] - eta[1]).max() <= 1e-6 * np.abs(eta[1]).max()
+
+
+def test_symmetric_mixed_precision_solve(default_table):
+    """The mixed-precision solver on the transformed blocks (complex64 LU,
+    FP64 refinement) gives the FP64 block-diagonal solve's forces."""
+    import capytaine_amd as cpa
+    print("[gpu] symmetry: mixed precision", flush=True)
+    gf = ref_table_gf(default_table)
+    mesh = _meshes()["box_two_planes"]
+    body = cpa.FloatingBody(mesh, cpa.rigid_body_dofs())
+    pbs = [cpa.RadiationProblem(body=body, radiating_dof="Pitch", omega=1.7),
+           cpa.DiffractionProblem(body=body, omega=1.7, wave_direction=0.4)]
+    out = []
+    for ls in ("lu_decomposition", "mixed_precision"):
+        solver = cpa.BEMSolver(engine=cpa.DefaultMatrixEngine(green_function=gf, linear_solver=ls))
+        out.append([solver.solve(pb).forces for pb in pbs])
+    for f64, fmx in zip(*out):
+        scale = max(abs(v) for v in f64.values())
+        for dof in f64:
+            assert abs(fmx[dof] - f64[dof]) <= 1e-10 * scale, dof
+
