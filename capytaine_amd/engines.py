@@ -336,15 +336,13 @@ class DefaultMatrixEngine:
                                                 **gf_params)
             check_if_nan_in_matrix([S, K])
             return S, K
-        from .symmetry import BlockSymmetricDeviceMatrix, symmetry_of
+        from .symmetry import evaluate_symmetric, symmetry_of
         sym = symmetry_of(mesh1) if mesh1 is mesh2 else None
         if sym is not None:
             # engines.py:178-247: the first block column, all collocation
             # points against the reference block
-            S, K = self.green_function.evaluate_on_device(mesh1.merged(), sym.block_mesh,
-                                                          diagonal_term_in_double_layer=diagonal_term_in_double_layer,
-                                                          **gf_params)
-            S, K = BlockSymmetricDeviceMatrix(S, sym), BlockSymmetricDeviceMatrix(K, sym)
+            S, K = evaluate_symmetric(self.green_function, mesh1, sym,
+                                      diagonal_term_in_double_layer=diagonal_term_in_double_layer, **gf_params)
             if self.check_nan:
                 check_if_nan_in_matrix([S, K])
             return S, K

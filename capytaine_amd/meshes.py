@@ -357,6 +357,10 @@ class ReflectionSymmetricMesh(SymmetricMesh):
     def mirrored(self, plane, name=None):
         return ReflectionSymmetricMesh(self.half.mirrored(plane), self.plane, name=name)
 
+    def rotated_z(self, angle, name=None):
+        """The merged mesh rotated (symmetric_meshes.py:171-172)."""
+        return self.merged().rotated_z(angle, name=name)
+
     def translated(self, shift, name=None):
         shift = np.asarray(shift, dtype=float)
         if (self.plane == "xOz" and abs(shift[1]) < 1e-12) or (self.plane == "yOz" and abs(shift[0]) < 1e-12):
@@ -410,8 +414,9 @@ class RotationSymmetricMesh(SymmetricMesh):
         return cls(Mesh(vertices, faces, name=f"wedge_of_{name}" if name else None), n, name=name)
 
     def _build_merged(self):
-        merged = self.all_wedges[0]
-        for w in self.all_wedges[1:]:
+        wedges = [w.merged() if isinstance(w, SymmetricMesh) else w for w in self.all_wedges]
+        merged = wedges[0]
+        for w in wedges[1:]:
             merged = merged.join(w)
         merged.name = self.name
         return merged

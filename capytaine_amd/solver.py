@@ -23,7 +23,7 @@ from . import _lib
 from ._lib import check, ptr, stream_handle
 from .engines import DefaultMatrixEngine, DeviceMatrix, LUDecomposedDeviceMatrix, check_if_nan_in_matrix
 from .lists_of_points import _normalize_points, _normalize_free_surface_points
-from .symmetry import BlockSymmetricDeviceMatrix, LUDecomposedSymmetricMatrix, symmetry_of
+from .symmetry import LUDecomposedSymmetricMatrix, evaluate_symmetric, symmetry_of
 from .problems import DiffractionProblem, RadiationProblem
 
 LOG = logging.getLogger(__name__)
@@ -198,13 +198,12 @@ class BEMSolver:
             ks = [problems[omegas[om][0]].wavenumber for om in oms]
             pb0 = problems[omegas[oms[0]][0]]
             with self.timer("Green function"):
-                # a symmetric mesh: the first block column only (engines.py:178-247)
-                S, K = gf.evaluate_on_device(mesh.merged() if sym else mesh, sym.block_mesh if sym else mesh,
-                                             free_surface=pb0.free_surface, water_depth=pb0.water_depth,
-                                             wavenumber=ks, adjoint_double_layer=True,
-                                             diagonal_term_in_double_layer=True)
-                S, K = (BlockSymmetricDeviceMatrix(S, sym), BlockSymmetricDeviceMatrix(K, sym)) if sym else \
-                    (DeviceMatrix(S), DeviceMatrix(K))
+                gp = dict(free_surface=pb0.free_surface, water_depth=pb0.water_depth, wavenumber=ks,
+                          adjoint_double_layer=True, diagonal_term_in_double_layer=True)
+                if sym:  # the first block column only (engines.py:178-247)
+                    S, K = evaluate_symmetric(gf, mesh, sym, **gp)
+                else:
+                    S, K = (DeviceMatrix(t) for t in gf.evaluate_on_device(mesh, mesh, **gp))
                 if self.engine.check_nan:
                     check_if_nan_in_matrix([S, K])
             nb = len(oms)

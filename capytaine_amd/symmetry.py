@@ -9,15 +9,17 @@ for the symmetries whose group is abelian:
 * two reflection planes (a ``ReflectionSymmetricMesh`` of one across the
   other plane): four blocks, A_(i xor j), bit 0 the inner plane;
 * n rotations about Oz (``RotationSymmetricMesh`` of a plain wedge), n <= 128:
-  block (i, j) = A_((i - j) mod n), block circulant.
+  block (i, j) = A_((i - j) mod n), block circulant;
+* n rotations of a wedge symmetric across a vertical plane (dihedral): the
+  half-wedge's column is evaluated and the wedge's rebuilt from it, then the
+  n wedges are block circulant (NestedBlockCirculantMatrix).
 
 The assembly evaluates the first block column only (all N = n w collocation
 points against the w faces of the reference block: a factor n fewer pairs).
 The group's Fourier transform (``cpt_block_transform``) turns it into n
 independent w x w matrices, factorised by one batched LU (n^2 fewer flops than
 the N x N factorisation).  Right-hand sides are transformed, solved and
-transformed back.  Dihedral symmetries (rotations of a reflection-symmetric
-wedge) and other nestings are evaluated on the merged mesh.
+transformed back.  Other nestings are evaluated on the merged mesh.
 """
 import numpy as np
 
@@ -28,10 +30,18 @@ from .meshes import ReflectionSymmetricMesh, RotationSymmetricMesh, SymmetricMes
 
 class Symmetry:
     """kind "xor" (reflections) or "cyclic" (rotations), n blocks, the
-    reference block's mesh."""
+    reference block's mesh.  For a dihedral mesh (n rotations of a wedge
+    symmetric across a vertical plane), `half` is the wedge's half: the
+    assembly evaluates against it and `macro_column` rebuilds the wedge's
+    column, after which the n wedges are cyclic blocks."""
 
-    def __init__(self, kind, n, block_mesh):
-        self.kind, self.n, self.block_mesh = kind, int(n), block_mesh
+    def __init__(self, kind, n, block_mesh, half=None):
+        self.kind, self.n, self.block_mesh, self.half = kind, int(n), block_mesh, half
+
+    @property
+    def evaluated_mesh(self):
+        """The faces the assembly evaluates all collocation points against."""
+        return self.half if self.half is not None else self.block_mesh
 
     @property
     def w(self):
@@ -65,9 +75,45 @@ def symmetry_of(mesh):
         if isinstance(h, ReflectionSymmetricMesh) and h.plane != mesh.plane and not isinstance(h.half, SymmetricMesh):
             return Symmetry("xor", 4, h.half)
         return None
-    if isinstance(mesh, RotationSymmetricMesh) and not isinstance(mesh.wedge, SymmetricMesh) and mesh.n <= 128:
-        return Symmetry("cyclic", mesh.n, mesh.wedge)
+    if isinstance(mesh, RotationSymmetricMesh) and mesh.n <= 128:
+        wd = mesh.wedge
+        if not isinstance(wd, SymmetricMesh):
+            return Symmetry("cyclic", mesh.n, wd)
+        if isinstance(wd, ReflectionSymmetricMesh) and not isinstance(wd.half, SymmetricMesh):
+            return Symmetry("cyclic", mesh.n, wd.merged(), half=wd.half)
     return None
+
+
+def macro_column(sym, cols):
+    """Dihedral mesh: from cols (nb, h, N), all collocation points against
+    the wedge's half (N = 2 n h), the column of the n wedge blocks (nb, 2h, N).
+
+    Block p = 2 i + s of the merged faces is R^i M^s applied to the half
+    (R the rotation by 2 pi / n, M the wedge's mirror), and G is invariant
+    under both, so block (p, q) of the full matrix is the evaluated block of
+    T_q^-1 T_p.  For the wedge's second half (q = 1, T = M) that is
+    M R^i M^s = R^-i M^(1-s): block 2 ((n - i) mod n) + 1 - s.  This is
+    NestedBlockCirculantMatrix.to_BlockCirculantMatrix
+    (block_circulant_matrices.py:222-249)."""
+    import torch
+    n = sym.n
+    nb, h, N = cols.shape
+    p = np.arange(2 * n)
+    perm = torch.as_tensor(2 * ((n - p // 2) % n) + 1 - p % 2, device=cols.device)
+    out = torch.empty((nb, 2 * h, N), dtype=cols.dtype, device=cols.device)
+    out[:, :h, :] = cols
+    out[:, h:, :] = cols.view(nb, h, 2 * n, h)[:, :, perm, :].reshape(nb, h, N)
+    return out
+
+
+def evaluate_symmetric(gf, mesh, sym, **gf_params):
+    """S and K of a symmetric mesh as BlockSymmetricDeviceMatrix: the first
+    block column (engines.py:178-247), for a dihedral mesh rebuilt from the
+    half-wedge's."""
+    S, K = gf.evaluate_on_device(mesh.merged(), sym.evaluated_mesh, **gf_params)
+    if sym.half is not None:
+        S, K = macro_column(sym, S), macro_column(sym, K)
+    return BlockSymmetricDeviceMatrix(S, sym), BlockSymmetricDeviceMatrix(K, sym)
 
 
 def _transform(sym, inp, out, R, C, NB, sbi, sj, ldi, sbo, sk, ldo, inverse):

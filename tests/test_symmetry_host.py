@@ -9,6 +9,7 @@ import pytest
 import capytaine_amd as cpa
 from capytaine_amd.meshes import join_with_mask
 from capytaine_amd.symmetry import Symmetry, symmetry_of
+from mesh_helpers import dihedral_sphere
 
 
 def _by_center(mesh):
@@ -93,3 +94,42 @@ def test_symmetric_lid_join_mask():
                                  omega=1.0).boundary_condition
     assert np.all(bc[~body.hull_mask] == 0)
     np.testing.assert_array_equal(bc[body.hull_mask], plain)
+
+
+def test_dihedral_mesh_and_detection():
+    mesh = dihedral_sphere(6, 8)
+    s = symmetry_of(mesh)
+    assert s.kind == "cyclic" and s.n == 6 and s.half is mesh.wedge.half and s.w == 2 * mesh.wedge.half.nb_faces
+    # the merged faces are those of a 12-meridian sphere, normals outward
+    c, nrm = mesh.faces_centers, mesh.faces_normals
+    assert mesh.nb_faces == 12 * 8
+    assert np.all(np.einsum("ij,ij->i", c, nrm) > 0)
+    ang = np.sort(np.round(np.arctan2(c[:, 1], c[:, 0]) % (2 * np.pi), 9))
+    np.testing.assert_allclose(np.unique(ang), (np.arange(12) + 0.5) * np.pi / 6, atol=1e-9)
+
+
+def test_dihedral_macro_column_is_the_group_rule():
+    """The macro column equals the first two block columns of the matrix
+    whose block (p, q) is the evaluated block of T_q^-1 T_p (p = 2 i + s:
+    R^i M^s), built here directly from that rule."""
+    import torch
+    from capytaine_amd.symmetry import Symmetry, macro_column
+    n, h, nb = 5, 3, 2
+    N = 2 * n * h
+    rng = np.random.default_rng(1)
+    cols = rng.normal(size=(nb, h, N)) + 1j * rng.normal(size=(nb, h, N))
+
+    class _Half:
+        nb_faces = h
+    sym = Symmetry("cyclic", n, None, half=_Half())
+    mac = macro_column(sym, torch.as_tensor(cols)).numpy()   # (nb, 2h, N)
+
+    def rel(p, q):  # index of T_q^-1 T_p in the first block column
+        (i, s), (j, t) = divmod(p, 2), divmod(q, 2)
+        return 2 * ((i - j) % n) + s if t == 0 else 2 * ((j - i) % n) + 1 - s
+    blocks = cols.transpose(0, 2, 1).reshape(nb, 2 * n, h, h)  # [b, p, row, col]
+    for q in (0, 1):
+        for p in range(2 * n):
+            want = blocks[:, rel(p, q)]
+            got = mac[:, q * h:(q + 1) * h, p * h:(p + 1) * h].transpose(0, 2, 1)
+            np.testing.assert_array_equal(got, want)
