@@ -136,12 +136,12 @@ class BEMSolver:
         results = [None] * len(problems)
         groups = OrderedDict()
         for i, pb in enumerate(problems):
-            if pb.forward_speed != 0.0:
-                # one problem at a time: its matrices depend on the encounter
-                # frequency, and its pressure on the full gradient on the hull
+            if pb.forward_speed != 0.0 and symmetry_of(pb.body.mesh_including_lid) is not None:
+                # symmetric mesh with forward speed: one problem at a time
                 results[i] = self._solve_and_catch_errors(pb, keep_details=keep_details)
                 continue
-            groups.setdefault((id(pb.body), pb.water_depth, pb.rho, pb.g, pb.free_surface), []).append(i)
+            groups.setdefault((id(pb.body), pb.water_depth, pb.rho, pb.g, pb.free_surface, pb.forward_speed),
+                              []).append(i)
         for key, idx in groups.items():
             sub = [problems[i] for i in idx]
             if batch_frequencies:
@@ -150,7 +150,7 @@ class BEMSolver:
                 res = []
                 by_omega = OrderedDict()
                 for j, pb in enumerate(sub):
-                    by_omega.setdefault(pb.omega, []).append(j)
+                    by_omega.setdefault(pb.encounter_omega, []).append(j)
                 res = [None] * len(sub)
                 for om, js in by_omega.items():
                     out = self.solve_batched([sub[j] for j in js], keep_details=keep_details,
@@ -162,28 +162,36 @@ class BEMSolver:
         return results
 
     def solve_batched(self, problems, *, keep_details=False, max_batch_bytes=None, return_device=False):
-        """All problems share body, depth, rho, g, free surface; frequencies may
-        differ.  Per chunk of frequencies: one assembly launch for all of
-        them, one batched LU of the K matrices (overwritten), one batched
-        solve for all right-hand sides, one batched S @ sources, and the
-        forces as a (dof x panels) product on the device."""
+        """All problems share body, depth, rho, g, free surface and forward
+        speed; frequencies may differ.  Per chunk of encounter frequencies: one
+        assembly launch for all of them, one batched LU of the K matrices
+        (overwritten), one batched solve for all right-hand sides, one batched
+        S @ sources, and the forces as a (dof x panels) product on the device.
+        With forward speed, one more assembly launch gives the full gradient
+        on the panels for the pressure term rho U d(phi)/dx
+        (solver.py:200-204)."""
         import torch
         body = problems[0].body
         mesh = body.mesh_including_lid
         gf = self.engine.green_function
         dev = gf.device
         n = mesh.nb_faces
+        speed = problems[0].forward_speed
         omegas = OrderedDict()
         for j, pb in enumerate(problems):
-            if isinstance(pb, DiffractionProblem) and float(pb.omega) in {0.0, np.inf}:
+            if isinstance(pb, DiffractionProblem) and float(pb.encounter_omega) in {0.0, np.inf}:
                 raise ValueError("Diffraction problems at zero or infinite frequency are not defined")
-            omegas.setdefault(pb.omega, []).append(j)
+            if pb.forward_speed != speed:
+                raise ValueError("solve_batched takes problems of one forward speed")
+            omegas.setdefault(pb.encounter_omega, []).append(j)
         omega_list = list(omegas)
         if max_batch_bytes is None:
             free, _ = torch.cuda.mem_get_info(dev)
             max_batch_bytes = int(0.6 * free)
         sym = symmetry_of(mesh)
-        per_freq = 2 * n * n * 16 // (sym.n if sym else 1)
+        if sym is not None and speed != 0.0:
+            raise NotImplementedError("solve_batched: forward speed on a symmetric mesh (solve them one by one)")
+        per_freq = (5 if speed != 0.0 else 2) * n * n * 16 // (sym.n if sym else 1)
         chunk = max(1, min(len(omega_list), max_batch_bytes // per_freq))
         # W[i, d] = -(dof_d . n_i) A_i  (integrate_pressure, bodies.py:238-250)
         dofs = list(body.dofs)
@@ -195,7 +203,7 @@ class BEMSolver:
         lib = _lib.load()
         for c0 in range(0, len(omega_list), chunk):
             oms = omega_list[c0:c0 + chunk]
-            ks = [problems[omegas[om][0]].wavenumber for om in oms]
+            ks = [problems[omegas[om][0]].encounter_wavenumber for om in oms]
             pb0 = problems[omegas[oms[0]][0]]
             with self.timer("Green function"):
                 gp = dict(free_surface=pb0.free_surface, water_depth=pb0.water_depth, wavenumber=ks,
@@ -222,6 +230,20 @@ class BEMSolver:
                 G = torch.empty((nb, R, ndof), dtype=torch.complex128, device=dev)
                 check(lib.cpt_zgemm(ctx.handle, ndof, R, n, 1.0, ptr(W_d), ndof, 0, ptr(Phi), n, n * R, ptr(G),
                                     ndof, ndof * R, 1, nb, stream_handle(dev)), "cpt_zgemm")
+            Dx_h = None
+            if speed != 0.0:
+                with self.timer("Post-processing velocity"):
+                    # d(phi)/dx on the panels: the x row of the full gradient
+                    # (build_fullK_matrix(mesh, mesh), diagonal term included)
+                    S = K = None
+                    _, K3 = gf.evaluate_on_device(mesh, mesh, **dict(gp, early_dot_product=False))
+                    D3 = DeviceMatrix(K3.view(nb, n, 3 * n)).matmul_device(X)  # (nb, R, 3n)
+                    del K3
+                    Dx = D3.view(nb, R, n, 3)[..., 0].contiguous()
+                    G2 = torch.empty_like(G)
+                    check(lib.cpt_zgemm(ctx.handle, ndof, R, n, 1.0, ptr(W_d), ndof, 0, ptr(Dx), n, n * R, ptr(G2),
+                                        ndof, ndof * R, 1, nb, stream_handle(dev)), "cpt_zgemm")
+                    Dx_h, G2_h = Dx.cpu().numpy(), G2.cpu().numpy()
             lu.check_info()
             G_h = G.cpu().numpy()
             if keep_details:
@@ -229,15 +251,20 @@ class BEMSolver:
             for b, om in enumerate(oms):
                 for r, j in enumerate(omegas[om]):
                     pb = problems[j]
-                    f = 1j * pb.omega * pb.rho * G_h[b, r]
+                    w = pb.encounter_omega
+                    f = 1j * w * pb.rho * G_h[b, r]
+                    if Dx_h is not None:
+                        f = f + pb.rho * speed * G2_h[b, r]
                     forces = {d: f[q] for q, d in enumerate(dofs)}
                     if keep_details:
                         pot = Phi_h[b, r]
-                        results[j] = pb.make_results_container(forces, X_h[b, r], pot,
-                                                               1j * pb.omega * pb.rho * pot)
+                        pressure = 1j * w * pb.rho * pot
+                        if Dx_h is not None:
+                            pressure = pressure + pb.rho * speed * Dx_h[b, r]
+                        results[j] = pb.make_results_container(forces, X_h[b, r], pot, pressure)
                     else:
                         results[j] = pb.make_results_container(forces)
-            del S, K, lu, X, Phi
+            S = K = lu = X = Phi = None  # free this chunk's matrices before the next one's
         return results
 
 

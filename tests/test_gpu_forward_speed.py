@@ -4,6 +4,7 @@ i omega_e rho phi + rho U d(phi)/dx with the full gradient on the hull.
 
 Parity: against the oracle pipeline (oracle S and K at the encounter
 wavenumber, scipy LU, oracle full gradient with the diagonal term) at 1e-10;
+batched (grouped by encounter frequency) equal to one by one at 1e-10;
 the pressure reconstructed by compute_pressure integrates to the forces
 (pytest/test_forward_speed.py:97-99).  Validation: Malenica (1995)'s
 bottom-mounted cylinder in finite depth, Froude numbers -0.05, 0 and 0.05,
@@ -38,6 +39,27 @@ def _oracle_forces(orc, table, pb):
                                   early_dot_product=False)
     pressure = 1j * pb.encounter_omega * pb.rho * (S @ sigma) + pb.rho * pb.forward_speed * (gradK[0] @ sigma)
     return pb.body.integrate_pressure(pressure[pb.body.hull_mask])
+
+
+def test_batched_forward_speed_equals_one_by_one(solver):
+    """solve_all groups forward-speed problems by encounter frequency (one
+    assembly, LU and gradient launch per chunk); the forces and pressures
+    equal the single-problem solves'."""
+    import capytaine_amd as cpa
+    print("[gpu] forward speed: batched", flush=True)
+    mesh = cpa.mesh_sphere(radius=1.0, resolution=(12, 16)).immersed_part()
+    body = cpa.FloatingBody(mesh, cpa.rigid_body_dofs(rotation_center=(0.0, 0.0, -0.2)))
+    pbs = [cpa.RadiationProblem(body=body, omega=w, forward_speed=0.8, wave_direction=b, radiating_dof=d)
+           for w in (1.2, 1.9) for b in (0.0, 2.0) for d in ("Surge", "Pitch")]
+    pbs += [cpa.DiffractionProblem(body=body, omega=w, forward_speed=0.8, wave_direction=b)
+            for w in (1.2, 1.9) for b in (0.0, 2.0)]
+    batched = solver.solve_all(pbs, keep_details=True)
+    for pb, res in zip(pbs, batched):
+        single = solver.solve(pb, keep_details=True)
+        scale = max(abs(v) for v in single.forces.values())
+        for dof in single.forces:
+            assert abs(res.forces[dof] - single.forces[dof]) <= 1e-10 * scale, (pb, dof)
+        assert np.abs(res.pressure - single.pressure).max() <= 1e-10 * np.abs(single.pressure).max()
 
 
 @pytest.mark.parametrize("speed", [1.0, -0.6])
