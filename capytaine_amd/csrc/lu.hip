@@ -1311,14 +1311,9 @@ int gemm(int M, int N, int K, R alpha, const cx<R>* A, int64_t lda, int64_t sA, 
                                    lda, sA, B, ldb, sB, C, ldc, sC, beta_zero);
                 return (int)hipGetLastError();
             }
-        } else {
-            if (active_gemm_tiles == 3) {
-                dim3 grid((M + TBM - 1) / TBM, (N + 63) / 64, batch);
-                hipLaunchKernelGGL((gemm_tiled_gauss_w1_kernel<R, 64>), grid, dim3(256), 0, stream, M, N, K, alpha,
-                                   A, lda, sA, B, ldb, sB, C, ldc, sC, beta_zero);
-                return (int)hipGetLastError();
-            }
         }
+        // complex128 keeps the four-product form: its Gauss form needs one
+        // wavefront per SIMD to hold three accumulator sets, and loses
         {
             dim3 grid((M + TBM - 1) / TBM, (N + 63) / 64, batch);
             hipLaunchKernelGGL((gemm_tiled_kernel<R, 64>), grid, dim3(256), 0, stream, M, N, K, alpha, A, lda, sA, B,

@@ -306,10 +306,11 @@ enum {
                                     or when the runtime refuses them) */
     CPT_TUNE_ASSEMBLY_CHUNK = 3, /* panels per assembly work-group, 0 = by size, else 1..1024 */
     CPT_TUNE_PANEL_CUS = 4,      /* CUs reserved for the panels when CPT_TUNE_LOOKAHEAD = 2, 1..128 */
-    CPT_TUNE_GEMM_TILES = 5,     /* tiled products: 0: 128 x 64 tiles; 1: complex64 on 128 x 128;
-                                    2: complex64 on 128 x 128, three real products per complex
-                                    one (Gauss), one wavefront per SIMD; 3: the Gauss form on
-                                    128 x 64 (complex128: one wavefront per SIMD); default 0 */
+    CPT_TUNE_GEMM_TILES = 5,     /* complex64 tiled products: 0: 128 x 64 tiles; 1: 128 x 128;
+                                    2: 128 x 128, three real products per complex one (Gauss),
+                                    one wavefront per SIMD; 3: the Gauss form on 128 x 64.
+                                    complex128 products always use 128 x 64, four products;
+                                    default 0 */
     CPT_TUNE_LOOKAHEAD_ACTIVE = 6 /* read only (cpt_get_tuning): the look-ahead mode the last
                                      factorisation ran, 1 where mode 2 found no hardware queue */
 };

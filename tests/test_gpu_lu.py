@@ -250,7 +250,8 @@ def test_kernel_timing():
                                                         (128, 16, 1, 3), (256, 8, 0, 3)])
 def test_lu_tunings_agree(block, leaf, lookahead, tiles):
     """Every tuning of the blocked LU solves to LAPACK's accuracy, the
-    three-product (Gauss) tile products included (tiles 2 and 3)."""
+    complex64 three-product (Gauss) tile products included (tiles 2 and 3;
+    the FP64 factorisation keeps the four-product form under any setting)."""
     from scipy.linalg import lu_factor, lu_solve
     from capytaine_amd import _lib
     print(f"[gpu] LU tuning block={block} leaf={leaf} lookahead={lookahead} tiles={tiles}", flush=True)
