@@ -47,6 +47,41 @@ class RotationDof:
         return np.tile(grad, (mesh.nb_faces, 1, 1))
 
 
+class DofOnSubmesh:
+    """`dof` on the faces of `mask`, zero elsewhere (dofs.py:99-128): the
+    dofs of a body inside a joined multibody mesh."""
+
+    def __init__(self, dof, mask):
+        self.dof = dof
+        self.mask = np.asarray(mask, dtype=bool)
+
+    def evaluate_motion(self, mesh):
+        out = np.zeros((mesh.nb_faces, 3))
+        out[self.mask] = self.dof.evaluate_motion(mesh)[self.mask]
+        return out
+
+    def evaluate_gradient_of_motion(self, mesh):
+        out = np.zeros((mesh.nb_faces, 3, 3))
+        out[self.mask] = self.dof.evaluate_gradient_of_motion(mesh)[self.mask]
+        return out
+
+    def translated(self, shift):
+        return DofOnSubmesh(_translated_dof(self.dof, shift), self.mask)
+
+
+def _translated_dof(dof, shift):
+    if isinstance(dof, RotationDof):
+        return RotationDof(dof.rotation_center + np.asarray(shift, float), dof.direction)
+    if hasattr(dof, "translated"):
+        return dof.translated(shift)
+    return dof
+
+
+def normalize_name(name):
+    """dofs.py:175-176."""
+    return name[0].upper() + name[1:].lower()
+
+
 def rigid_body_dofs(only=None, rotation_center=None):
     """dofs.py:131-172."""
     rc = np.zeros(3) if rotation_center is None else np.asarray(rotation_center, float)
@@ -73,10 +108,12 @@ class FloatingBody:
     ``mesh_including_lid`` (hull faces first); ``hull_mask`` selects the
     hull's faces, on which the boundary conditions and the forces live."""
 
-    def __init__(self, mesh, dofs=None, name=None, *, lid_mesh=None):
+    def __init__(self, mesh, dofs=None, name=None, *, lid_mesh=None, center_of_mass=None, mass=None):
         self.mesh = mesh
         self.dofs = {} if dofs is None else dict(dofs)
         self.name = name or mesh.name
+        self.center_of_mass = None if center_of_mass is None else np.asarray(center_of_mass, dtype=float)
+        self.mass = mass
         if lid_mesh is not None and lid_mesh.nb_faces == 0:
             LOG.warning("Lid mesh %s provided for body initialization is empty. The lid mesh is ignored.", lid_mesh)
             lid_mesh = None
@@ -104,9 +141,106 @@ class FloatingBody:
         p = np.hypot(1 / (v[:, 0].max() - v[:, 0].min()), 1 / (v[:, 1].max() - v[:, 1].min()))
         return float(np.sqrt(np.pi * g * p / np.tanh(np.pi * draft * p)))
 
+    @property
+    def nb_dofs(self):
+        return len(self.dofs)
+
+    def add_translation_dof(self, direction=None, name=None):
+        """bodies.py:176-194: a translation; the direction of Surge, Sway or
+        Heave is inferred from the name when not given."""
+        if name is None:
+            name = f"dof_{self.nb_dofs}_translation"
+        if direction is None and normalize_name(name) in {"Surge", "Sway", "Heave"}:
+            self.dofs[name] = rigid_body_dofs()[normalize_name(name)]
+        else:
+            self.dofs[name] = TranslationDof(direction)
+        return self
+
+    def add_rotation_dof(self, rotation_center=None, direction=None, name=None):
+        """bodies.py:196-227: a rotation about an axis through
+        `rotation_center` (default: the body's centre of mass if given, else
+        the origin); Roll, Pitch or Yaw infer the direction from the name."""
+        if name is None:
+            name = f"dof_{self.nb_dofs}_rotation"
+        if rotation_center is None and self.center_of_mass is not None:
+            rotation_center = self.center_of_mass
+        if direction is None and normalize_name(name) in {"Roll", "Pitch", "Yaw"}:
+            self.dofs[name] = rigid_body_dofs(rotation_center=rotation_center)[normalize_name(name)]
+        else:
+            self.dofs[name] = RotationDof(rotation_center, direction)
+        return self
+
     def add_all_rigid_body_dofs(self, rotation_center=None):
         self.dofs.update(rigid_body_dofs(rotation_center=rotation_center))
         return self
+
+    def copy(self, name=None):
+        return FloatingBody(self.mesh, dict(self.dofs), name=name or self.name, lid_mesh=self.lid_mesh,
+                            center_of_mass=self.center_of_mass, mass=self.mass)
+
+    def translated(self, shift, name=None):
+        """bodies.py:389-400: mesh, lid, rotation centres and centre of mass
+        moved by `shift`."""
+        shift = np.asarray(shift, dtype=float)
+        dofs = {k: _translated_dof(v, shift) for k, v in self.dofs.items()}
+        com = None if self.center_of_mass is None else np.asarray(self.center_of_mass) + shift
+        return FloatingBody(self.mesh.translated(shift), dofs, name=name or self.name,
+                            lid_mesh=None if self.lid_mesh is None else self.lid_mesh.translated(shift),
+                            center_of_mass=com, mass=self.mass)
+
+    def translated_x(self, dx, name=None):
+        return self.translated((dx, 0.0, 0.0), name=name)
+
+    def translated_y(self, dy, name=None):
+        return self.translated((0.0, dy, 0.0), name=name)
+
+    def translated_z(self, dz, name=None):
+        return self.translated((0.0, 0.0, dz), name=name)
+
+    def immersed_part(self, free_surface=0.0, name=None):
+        """The body with the faces of its mesh below the free surface; rigid
+        dofs are kept (their motion is evaluated on the new mesh)."""
+        dofs = {k: v for k, v in self.dofs.items() if hasattr(v, "evaluate_motion")}
+        return FloatingBody(self.mesh.immersed_part(free_surface), dofs, name=name or self.name,
+                            lid_mesh=self.lid_mesh, center_of_mass=self.center_of_mass, mass=self.mass)
+
+    def join_bodies(self, *others, name=None):
+        """The bodies as one (multibodies.py:34-131): meshes joined, each
+        body's dofs restricted to its faces and renamed body__dof, lids
+        joined.  Names must be distinct."""
+        from .meshes import SymmetricMesh
+        bodies = [self, *others]
+        if len({b.name for b in bodies}) < len(bodies):
+            raise ValueError(f"In Multibody, all component bodies must have a distinct name.\n"
+                             f"Got: {[b.name for b in bodies]}")
+        meshes = [b.mesh.merged() if isinstance(b.mesh, SymmetricMesh) else b.mesh for b in bodies]
+        mesh = meshes[0]
+        for m in meshes[1:]:
+            mesh = mesh.join(m)
+        sizes = np.cumsum([0] + [m.nb_faces for m in meshes])
+        dofs = {}
+        for b, lo, hi in zip(bodies, sizes[:-1], sizes[1:]):
+            mask = np.zeros(mesh.nb_faces, dtype=bool)
+            mask[lo:hi] = True
+            for k, dof in b.dofs.items():
+                new_name = k if "__" in k else f"{b.name}__{k}"
+                if hasattr(dof, "evaluate_motion"):
+                    dofs[new_name] = DofOnSubmesh(dof, mask)
+                else:
+                    arr = np.zeros((mesh.nb_faces, 3))
+                    arr[mask] = np.asarray(dof)
+                    dofs[new_name] = arr
+        lids = [b.lid_mesh for b in bodies if b.lid_mesh is not None]
+        lid = None
+        if lids:
+            lid = lids[0]
+            for m in lids[1:]:
+                lid = lid.join(m)
+        mesh.name = name or "+".join(b.name for b in bodies)
+        return FloatingBody(mesh, dofs, name=name or "+".join(b.name for b in bodies), lid_mesh=lid)
+
+    def __add__(self, other):
+        return self.join_bodies(other)
 
     def dof_motion(self, dof_name):
         dof = self.dofs[dof_name]
@@ -146,7 +280,8 @@ def wavenumber_from_omega(omega, g, water_depth):
 
 class LinearPotentialFlowProblem:
     def __init__(self, *, body=None, free_surface=0.0, water_depth=np.inf, omega=None, wavenumber=None,
-                 rho=1000.0, g=9.81, wave_direction=0.0, forward_speed=0.0):
+                 freq=None, period=None, wavelength=None, rho=1000.0, g=9.81, wave_direction=0.0,
+                 forward_speed=0.0):
         self.body = body
         self.free_surface = float(free_surface)
         self.water_depth = float(water_depth)
@@ -154,11 +289,25 @@ class LinearPotentialFlowProblem:
         self.g = float(g)
         self.wave_direction = float(wave_direction)
         self.forward_speed = float(forward_speed)
-        if omega is not None and wavenumber is not None:
-            raise ValueError("Provide omega or wavenumber, not both")
+        given = {k: v for k, v in dict(omega=omega, freq=freq, period=period, wavenumber=wavenumber,
+                                           wavelength=wavelength).items() if v is not None}
+        if len(given) > 1:
+            raise ValueError("Settings a problem requires at most one of the following: omega (angular frequency) "
+                             "OR freq (in Hz) OR period OR wavenumber OR wavelength.\n"
+                             f"Received {len(given)} of them: {given}")
+        # problems_and_results.py:115-177: the other forms of the frequency
+        if freq is not None:
+            omega = 2 * np.pi * float(freq)
+        elif period is not None:
+            omega = 2 * np.pi / float(period) if float(period) not in (0.0, np.inf) else \
+                (np.inf if float(period) == 0.0 else 0.0)
+        elif wavelength is not None:
+            wavenumber = 2 * np.pi / float(wavelength) if float(wavelength) not in (0.0, np.inf) else \
+                (np.inf if float(wavelength) == 0.0 else 0.0)
         if wavenumber is not None:
             self.wavenumber = float(wavenumber)
-            self.omega = float(np.sqrt(self.g * self.wavenumber * np.tanh(self.wavenumber * self.water_depth)))
+            self.omega = float(np.sqrt(self.g * self.wavenumber * np.tanh(self.wavenumber * self.water_depth))) \
+                if self.wavenumber not in (0.0, np.inf) else self.wavenumber
         else:
             self.omega = float(_default_parameters["omega"] if omega is None else omega)
             if self.omega in (0.0, np.inf):
@@ -184,6 +333,10 @@ class LinearPotentialFlowProblem:
     @property
     def influenced_dofs(self):
         return self.body.dofs
+
+    freq = property(lambda self: self.omega / (2 * np.pi))
+    period = property(lambda self: 2 * np.pi / self.omega if self.omega != 0.0 else np.inf)
+    wavelength = property(lambda self: 2 * np.pi / self.wavenumber if self.wavenumber != 0.0 else np.inf)
 
     def _group_key(self):
         """problems_and_results.py:266-275 grouping key."""
@@ -306,6 +459,7 @@ class LinearPotentialFlowResult:
     def __init__(self, problem, forces=None, sources=None, potential=None, pressure=None):
         self.problem = problem
         self.forces = forces if forces is not None else {}
+        self.force = self.forces  # alias of the reference (problems_and_results.py:529-531)
         self.sources = sources
         self.potential = potential
         self.pressure = pressure
@@ -326,6 +480,9 @@ class RadiationResult(LinearPotentialFlowResult):
     @property
     def radiating_dof(self):
         return self.problem.radiating_dof
+
+    added_masses = property(lambda self: self.added_mass)
+    radiation_dampings = property(lambda self: self.radiation_damping)
 
     @property
     def added_mass(self):

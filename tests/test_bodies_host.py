@@ -1,0 +1,84 @@
+"""Bodies, dofs and problem parameters on the host (bodies/bodies.py:
+176-400, bodies/multibodies.py:34-131, bodies/dofs.py,
+problems_and_results.py:115-177).  No GPU."""
+import numpy as np
+import pytest
+
+import capytaine_amd as cpa
+from capytaine_amd.problems import DofOnSubmesh, normalize_name
+
+
+@pytest.fixture
+def sphere():
+    return cpa.mesh_sphere(radius=1.0, resolution=(6, 8)).immersed_part()
+
+
+def test_dof_helpers(sphere):
+    body = cpa.FloatingBody(sphere, name="s")
+    body.add_translation_dof(name="Heave")
+    body.add_translation_dof(direction=(1.0, 1.0, 0.0), name="diag")
+    body.add_rotation_dof(rotation_center=(0, 0, -0.5), name="Pitch")
+    body.add_rotation_dof(rotation_center=(0, 0, 0), direction=(0, 0, 1))
+    assert list(body.dofs) == ["Heave", "diag", "Pitch", "dof_3_rotation"] and body.nb_dofs == 4
+    np.testing.assert_array_equal(body.dof_motion("Heave"), np.tile((0, 0, 1.0), (sphere.nb_faces, 1)))
+    c = sphere.faces_centers
+    np.testing.assert_allclose(body.dof_motion("Pitch"), np.cross((0, 1.0, 0), c - (0, 0, -0.5)))
+    assert normalize_name("heave") == "Heave"
+
+
+def test_frequency_forms(sphere):
+    body = cpa.FloatingBody(sphere, cpa.rigid_body_dofs(only=["Heave"]))
+    ref = cpa.RadiationProblem(body=body, omega=1.3)
+    for kw in (dict(freq=1.3 / (2 * np.pi)), dict(period=2 * np.pi / 1.3), dict(wavenumber=1.3 ** 2 / 9.81),
+               dict(wavelength=2 * np.pi * 9.81 / 1.3 ** 2)):
+        pb = cpa.RadiationProblem(body=body, **kw)
+        assert pb.omega == pytest.approx(1.3) and pb.wavenumber == pytest.approx(ref.wavenumber)
+    assert ref.period == pytest.approx(2 * np.pi / 1.3) and ref.freq == pytest.approx(1.3 / (2 * np.pi))
+    assert ref.wavelength == pytest.approx(2 * np.pi / ref.wavenumber)
+    fd = cpa.RadiationProblem(body=body, wavelength=3.0, water_depth=2.0)
+    assert fd.wavenumber == pytest.approx(2 * np.pi / 3.0)
+    assert fd.omega ** 2 == pytest.approx(9.81 * fd.wavenumber * np.tanh(fd.wavenumber * 2.0))
+    with pytest.raises(ValueError):
+        cpa.RadiationProblem(body=body, omega=1.0, period=2.0)
+
+
+def test_translated_body(sphere):
+    body = cpa.FloatingBody(sphere, cpa.rigid_body_dofs(rotation_center=(0, 0, -0.3)), name="s")
+    moved = body.translated_x(5.0, name="t")
+    np.testing.assert_allclose(moved.mesh.faces_centers, sphere.faces_centers + (5, 0, 0))
+    # motion is the same rigid motion about the moved centre
+    np.testing.assert_allclose(moved.dof_motion("Yaw"), body.dof_motion("Yaw"), atol=1e-14)
+
+
+def test_join_bodies(sphere):
+    """multibodies.py:34-131: joined mesh, dofs body__dof restricted to the
+    body's faces; the boundary conditions of a radiation problem of the
+    joined body equal the single body's on its faces and vanish elsewhere."""
+    a = cpa.FloatingBody(sphere, cpa.rigid_body_dofs(only=["Heave", "Pitch"]), name="a")
+    b = a.translated_x(6.0, name="b")
+    ab = a + b
+    n = sphere.nb_faces
+    assert ab.mesh.nb_faces == 2 * n
+    assert list(ab.dofs) == ["a__Heave", "a__Pitch", "b__Heave", "b__Pitch"]
+    assert isinstance(ab.dofs["b__Pitch"], DofOnSubmesh)
+    pb = cpa.RadiationProblem(body=ab, omega=1.0, radiating_dof="b__Pitch")
+    ref = cpa.RadiationProblem(body=b, omega=1.0, radiating_dof="Pitch")
+    np.testing.assert_array_equal(pb.boundary_condition[:n], 0)
+    np.testing.assert_allclose(pb.boundary_condition[n:], ref.boundary_condition)
+    # forward-speed m-terms follow the dof onto its faces
+    pbu = cpa.RadiationProblem(body=ab, omega=1.5, forward_speed=1.0, radiating_dof="a__Pitch")
+    refu = cpa.RadiationProblem(body=a, omega=1.5, forward_speed=1.0, radiating_dof="Pitch")
+    np.testing.assert_allclose(pbu.boundary_condition[:n], refu.boundary_condition)
+    with pytest.raises(ValueError):
+        a + a.translated_x(3.0)  # same name
+    # associativity of the names
+    c = a.translated_y(6.0, name="c")
+    assert list((ab + c).dofs)[-1] == "c__Pitch" and "a__Heave" in (ab + c).dofs
+
+
+def test_result_aliases(sphere):
+    body = cpa.FloatingBody(sphere, cpa.rigid_body_dofs(only=["Heave"]))
+    pb = cpa.RadiationProblem(body=body, omega=1.0)
+    res = pb.make_results_container({"Heave": 2.0 + 3.0j})
+    assert res.force is res.forces
+    assert res.added_masses == res.added_mass and res.radiation_dampings == res.radiation_damping

@@ -109,3 +109,27 @@ def test_immersed_sphere_rankine_only():
         pb = cpa.RadiationProblem(body=body, radiating_dof=dof, free_surface=np.inf, water_depth=np.inf)
         res = solver.solve(pb)
         assert abs(res.added_mass[dof] - float(ka[key])) < atol, (dof, res.added_mass[dof])
+
+
+def test_multibody_with_very_distant_bodies(default_table):
+    """Two bodies 50 m apart interact little: the pitch force of one on
+    itself is within 1 % of the lone body's, with forward speed
+    (pytest/test_forward_speed.py:225-232); the joined body's batched solve
+    equals its one-by-one solves."""
+    import capytaine_amd as cpa
+    print("[gpu] multibody", flush=True)
+    mesh = cpa.mesh_vertical_cylinder(radius=1.0, length=1.0, center=(0.0, 0.0, -0.5), resolution=(2, 16, 6))
+    body = cpa.FloatingBody(mesh, cpa.rigid_body_dofs(only=["Surge", "Heave", "Pitch"], rotation_center=(0, 0, -0.5)),
+                            name="body")
+    two = body + body.translated_x(50.0, name="other_body")
+    solver = cpa.BEMSolver(engine=cpa.DefaultMatrixEngine(green_function=ref_table_gf(default_table)))
+    ref = solver.solve(cpa.RadiationProblem(body=body, omega=2.0, forward_speed=1.0, radiating_dof="Pitch"))
+    pb = cpa.RadiationProblem(body=two, omega=2.0, forward_speed=1.0, radiating_dof="body__Pitch")
+    res = solver.solve(pb)
+    assert np.isclose(res.forces["body__Pitch"], ref.force["Pitch"], rtol=1e-2)
+    pbs = [cpa.RadiationProblem(body=two, omega=w, radiating_dof=d) for w in (0.8, 1.6) for d in two.dofs]
+    for p, r in zip(pbs, solver.solve_all(pbs)):
+        s = solver.solve(p)
+        scale = max(abs(v) for v in s.forces.values())
+        for dof in s.forces:
+            assert abs(r.forces[dof] - s.forces[dof]) <= 1e-10 * scale
