@@ -21,9 +21,10 @@ import numpy as np
 
 from . import _lib
 from ._lib import check, ptr, stream_handle
-from .engines import DefaultMatrixEngine, DeviceMatrix, LUDecomposedDeviceMatrix, check_if_nan_in_matrix
+from .engines import (DefaultMatrixEngine, DeviceMatrix, LUDecomposedDeviceMatrix, MixedPrecisionSolver,
+                      check_if_nan_in_matrix)
 from .lists_of_points import _normalize_points, _normalize_free_surface_points
-from .symmetry import LUDecomposedSymmetricMatrix, evaluate_symmetric, symmetry_of
+from .symmetry import LUDecomposedSymmetricMatrix, MixedSymmetricSolver, evaluate_symmetric, symmetry_of
 from .problems import DiffractionProblem, RadiationProblem
 
 LOG = logging.getLogger(__name__)
@@ -69,6 +70,14 @@ class BEMSolver:
 
     def __str__(self):
         return f"BEMSolver(engine={self.engine}, method={self.method})"
+
+    def reset_timer(self):
+        """solver.py:113-114."""
+        self.timer = Timer()
+
+    def timer_summary(self):
+        """Total wall time and count per step (solver.py:116-126)."""
+        return {step: {"total_s": t, "count": self.timer.counts[step]} for step, t in self.timer.totals.items()}
 
     __repr__ = __str__
 
@@ -124,12 +133,15 @@ class BEMSolver:
 
     # ------------------------------------------------------------------
     def solve_all(self, problems, *, method=None, keep_details=False, batch_frequencies=True,
-                  max_batch_bytes=None):
+                  max_batch_bytes=None, n_jobs=1, n_threads=None, progress_bar=None):
         """Solve several problems.  Problems sharing a body, depth, rho, g and
         free surface are solved together: with batch_frequencies, all their
         frequencies at once (`solve_batched`); otherwise one frequency at a
         time with all its right-hand sides at once.  Results come back in
-        the order of `problems`."""
+        the order of `problems`.  The engine's linear_solver
+        ("mixed_precision" or not) chooses the batched factorisation.
+        n_jobs, n_threads and progress_bar are accepted for the reference's
+        signature (solver.py:268) and unused: the device batches instead."""
         method = method if method is not None else self.method
         if method != "indirect":
             return [self._solve_and_catch_errors(pb, method=method, keep_details=keep_details) for pb in problems]
@@ -221,9 +233,14 @@ class BEMSolver:
                 for r, j in enumerate(omegas[om]):
                     bc[b, r] = problems[j].boundary_condition
             with self.timer("Linear solver"):
-                lu = LUDecomposedSymmetricMatrix(K) if sym else LUDecomposedDeviceMatrix(K, overwrite_a=True)
                 X = torch.as_tensor(bc).to(dev)
-                lu.solve_device(X)
+                if getattr(self.engine, "_linear_solver", None) == "mixed_precision":
+                    # complex64 LU with FP64 refinement, all frequencies at once
+                    lu = MixedSymmetricSolver(K) if sym else MixedPrecisionSolver(K)
+                    X = lu.solve_device(X)
+                else:
+                    lu = LUDecomposedSymmetricMatrix(K) if sym else LUDecomposedDeviceMatrix(K, overwrite_a=True)
+                    lu.solve_device(X)
             with self.timer("Matrix-vector product"):
                 Phi = S.matmul_device(X)
                 # G (ndof x R) = W^T (ndof x n) Phi (n x R), per frequency
@@ -244,7 +261,8 @@ class BEMSolver:
                     check(lib.cpt_zgemm(ctx.handle, ndof, R, n, 1.0, ptr(W_d), ndof, 0, ptr(Dx), n, n * R, ptr(G2),
                                         ndof, ndof * R, 1, nb, stream_handle(dev)), "cpt_zgemm")
                     Dx_h, G2_h = Dx.cpu().numpy(), G2.cpu().numpy()
-            lu.check_info()
+            if hasattr(lu, "check_info"):
+                lu.check_info()
             G_h = G.cpu().numpy()
             if keep_details:
                 X_h, Phi_h = X.cpu().numpy(), Phi.cpu().numpy()

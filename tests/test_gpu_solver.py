@@ -133,3 +133,26 @@ def test_multibody_with_very_distant_bodies(default_table):
         scale = max(abs(v) for v in s.forces.values())
         for dof in s.forces:
             assert abs(r.forces[dof] - s.forces[dof]) <= 1e-10 * scale
+
+
+def test_solve_all_with_mixed_precision_engine(default_table):
+    """An engine with linear_solver="mixed_precision" makes solve_all batch
+    the complex64 LU with FP64 refinement; coefficients equal the FP64
+    engine's to 1e-10 (plain and symmetric meshes)."""
+    import capytaine_amd as cpa
+    print("[gpu] solve_all with the mixed-precision engine", flush=True)
+    gf = ref_table_gf(default_table)
+    for mesh in (cpa.mesh_sphere(radius=1.0, resolution=(12, 16)).immersed_part(),
+                 cpa.mesh_parallelepiped(size=(2.0, 1.0, 1.0), center=(0.0, 0.0, -0.5), resolution=(8, 4, 4),
+                                         missing_sides={"top"}, reflection_symmetry=True)):
+        body = cpa.FloatingBody(mesh, cpa.rigid_body_dofs())
+        pbs = cpa.problems_for_body(body, [0.7, 1.4, 2.3], headings=[0.0, 1.0])
+        out = []
+        for ls in ("lu_decomposition", "mixed_precision"):
+            solver = cpa.BEMSolver(engine=cpa.DefaultMatrixEngine(green_function=gf, linear_solver=ls))
+            out.append(solver.solve_all(pbs))
+        for r64, rmx in zip(*out):
+            scale = max(abs(v) for v in r64.forces.values())
+            for dof in r64.forces:
+                assert abs(rmx.forces[dof] - r64.forces[dof]) <= 1e-10 * scale
+    assert "Linear solver" in solver.timer_summary()
