@@ -145,6 +145,14 @@ class FloatingBody:
     def nb_dofs(self):
         return len(self.dofs)
 
+    @property
+    def minimal_computable_wavelength(self):
+        """bodies.py:558-563: eight times the largest panel radius."""
+        r = self.mesh.faces_radiuses.max()
+        if self.lid_mesh is not None:
+            r = max(r, self.lid_mesh.faces_radiuses.max())
+        return 8 * r
+
     def add_translation_dof(self, direction=None, name=None):
         """bodies.py:176-194: a translation; the direction of Surge, Sway or
         Heave is inferred from the name when not given."""

@@ -26,6 +26,7 @@ from .engines import (DefaultMatrixEngine, DeviceMatrix, LUDecomposedDeviceMatri
 from .lists_of_points import _normalize_points, _normalize_free_surface_points
 from .symmetry import LUDecomposedSymmetricMatrix, MixedSymmetricSolver, evaluate_symmetric, symmetry_of
 from .problems import DiffractionProblem, RadiationProblem
+from .problems_checks import check_problems
 
 LOG = logging.getLogger(__name__)
 
@@ -120,7 +121,10 @@ class BEMSolver:
             return problem.make_results_container(forces)
         return problem.make_results_container(forces, sources, potential, pressure)
 
-    def solve(self, problem, method=None, keep_details=True):
+    def solve(self, problem, method=None, keep_details=True, n_threads=None, _check_wavelength=True):
+        """solver.py:217-254 (n_threads accepted and unused)."""
+        if _check_wavelength:
+            check_problems([problem])
         with self.timer("Total solve function"):
             return self._solve(problem, method=method, keep_details=keep_details)
 
@@ -143,14 +147,16 @@ class BEMSolver:
         n_jobs, n_threads and progress_bar are accepted for the reference's
         signature (solver.py:268) and unused: the device batches instead."""
         method = method if method is not None else self.method
+        check_problems(problems)
         if method != "indirect":
-            return [self._solve_and_catch_errors(pb, method=method, keep_details=keep_details) for pb in problems]
+            return [self._solve_and_catch_errors(pb, method=method, keep_details=keep_details,
+                                                 _check_wavelength=False) for pb in problems]
         results = [None] * len(problems)
         groups = OrderedDict()
         for i, pb in enumerate(problems):
             if pb.forward_speed != 0.0 and symmetry_of(pb.body.mesh_including_lid) is not None:
                 # symmetric mesh with forward speed: one problem at a time
-                results[i] = self._solve_and_catch_errors(pb, keep_details=keep_details)
+                results[i] = self._solve_and_catch_errors(pb, keep_details=keep_details, _check_wavelength=False)
                 continue
             groups.setdefault((id(pb.body), pb.water_depth, pb.rho, pb.g, pb.free_surface, pb.forward_speed),
                               []).append(i)

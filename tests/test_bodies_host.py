@@ -82,3 +82,27 @@ def test_result_aliases(sphere):
     res = pb.make_results_container({"Heave": 2.0 + 3.0j})
     assert res.force is res.forces
     assert res.added_masses == res.added_mass and res.radiation_dampings == res.radiation_damping
+
+
+def test_problem_checks_warn(sphere, caplog):
+    """bem/problems_checks.py: coarse mesh, needlessly finite depth,
+    irregular frequencies."""
+    import logging
+    from capytaine_amd.problems_checks import (_check_wavelength_and_irregular_frequencies,
+                                               _check_wavelength_and_mesh_resolution,
+                                               _check_wavelength_and_water_depth)
+    body = cpa.FloatingBody(sphere, cpa.rigid_body_dofs(only=["Heave"]))
+    short = cpa.RadiationProblem(body=body, wavelength=0.5 * body.minimal_computable_wavelength)
+    fine = cpa.RadiationProblem(body=body, wavelength=2.0 * body.minimal_computable_wavelength)
+    with caplog.at_level(logging.WARNING):
+        assert _check_wavelength_and_mesh_resolution([short, fine]) == [short]
+    assert "Mesh resolution" in caplog.text
+    deep = cpa.RadiationProblem(body=body, wavelength=2.0, water_depth=50.0)
+    assert _check_wavelength_and_water_depth([deep, fine]) == [deep]
+    w_irr = body.first_irregular_frequency_estimate()
+    above = cpa.RadiationProblem(body=body, omega=1.1 * w_irr)
+    below = cpa.RadiationProblem(body=body, omega=0.5 * w_irr)
+    caplog.clear()
+    with caplog.at_level(logging.WARNING):
+        assert _check_wavelength_and_irregular_frequencies([above, below]) == [above]
+    assert "lid" in caplog.text
