@@ -154,10 +154,6 @@ class BEMSolver:
         results = [None] * len(problems)
         groups = OrderedDict()
         for i, pb in enumerate(problems):
-            if pb.forward_speed != 0.0 and symmetry_of(pb.body.mesh_including_lid) is not None:
-                # symmetric mesh with forward speed: one problem at a time
-                results[i] = self._solve_and_catch_errors(pb, keep_details=keep_details, _check_wavelength=False)
-                continue
             groups.setdefault((id(pb.body), pb.water_depth, pb.rho, pb.g, pb.free_surface, pb.forward_speed),
                               []).append(i)
         for key, idx in groups.items():
@@ -207,9 +203,7 @@ class BEMSolver:
             free, _ = torch.cuda.mem_get_info(dev)
             max_batch_bytes = int(0.6 * free)
         sym = symmetry_of(mesh)
-        if sym is not None and speed != 0.0:
-            raise NotImplementedError("solve_batched: forward speed on a symmetric mesh (solve them one by one)")
-        per_freq = (5 if speed != 0.0 else 2) * n * n * 16 // (sym.n if sym else 1)
+        per_freq = (3 * n * n * 16 if speed != 0.0 else 0) + 2 * n * n * 16 // (sym.n if sym else 1)
         chunk = max(1, min(len(omega_list), max_batch_bytes // per_freq))
         # W[i, d] = -(dof_d . n_i) A_i  (integrate_pressure, bodies.py:238-250)
         dofs = list(body.dofs)
@@ -257,9 +251,11 @@ class BEMSolver:
             if speed != 0.0:
                 with self.timer("Post-processing velocity"):
                     # d(phi)/dx on the panels: the x row of the full gradient
-                    # (build_fullK_matrix(mesh, mesh), diagonal term included)
+                    # (build_fullK_matrix(mesh, mesh), diagonal term included);
+                    # a symmetric mesh's on its merged faces, in the same order
                     S = K = None
-                    _, K3 = gf.evaluate_on_device(mesh, mesh, **dict(gp, early_dot_product=False))
+                    gm = mesh.merged() if sym else mesh
+                    _, K3 = gf.evaluate_on_device(gm, gm, **dict(gp, early_dot_product=False))
                     D3 = DeviceMatrix(K3.view(nb, n, 3 * n)).matmul_device(X)  # (nb, R, 3n)
                     del K3
                     Dx = D3.view(nb, R, n, 3)[..., 0].contiguous()
