@@ -108,6 +108,16 @@ struct Mfma<float> {
 
 constexpr int PB = 64;  // widest triangle the panel and trsm kernels hold (one wavefront)
 
+// The panel kernel's sub-block update loads L for groups of panel_qg rows
+// and panel_jc columns per round trip: at most `cap` complex values per
+// thread in flight; panel_jc divides SB and panel_qg divides RPT.
+constexpr int panel_qg(int rpt, int cap) { return rpt < cap ? rpt : cap; }
+constexpr int panel_jc(int rpt, int sb, int cap)
+{
+    const int jc = rpt >= cap ? 1 : cap / rpt;
+    return jc < sb ? jc : sb;
+}
+
 // ---------------------------------------------------------------------------
 // 1. Panel factorisation
 //
@@ -257,20 +267,45 @@ __global__ void __launch_bounds__(NT) panel_kernel(cx<R>* __restrict__ Abase, in
                 if (lane < t0) sU[lane * SB + s] = x;
             }
             __syncthreads();
-            // (c)
+            // (c) the rows above t0 take the solved block; the rows below
+            // subtract L's columns [k0, k0 + t0) times it, JC columns of QG
+            // rows loaded at once (one round trip per group of rows and JC
+            // columns; t0 is a multiple of SB), j ascending as in a
+            // column-by-column update
+            constexpr int CAP = sizeof(R) == 4 ? 16 : 8;
+            constexpr int QG = panel_qg(RPT, CAP);
+            constexpr int JC = panel_jc(RPT, SB, CAP);
 #pragma unroll
             for (int q = 0; q < RPT; ++q) {
                 const int t = tid + NT * q;
-                if (t < t0) {
+                if (t < t0)
 #pragma unroll
                     for (int s = 0; s < SB; ++s) v[q][s] = sU[t * SB + s];
-                } else if (t < M) {
-                    const C* Lrow = A + (k0 + t) + (size_t)k0 * lda;
-#pragma unroll 8
-                    for (int j = 0; j < t0; ++j) {
-                        const C l = Lrow[(size_t)j * lda];
+            }
 #pragma unroll
-                        for (int s = 0; s < SB; ++s) v[q][s] = zsub(v[q][s], zmul(l, sU[j * SB + s]));
+            for (int g = 0; g < RPT; g += QG) {
+                for (int j0 = 0; j0 < t0; j0 += JC) {
+                    C l[QG][JC];
+#pragma unroll
+                    for (int qq = 0; qq < QG; ++qq) {
+                        const int t = tid + NT * (g + qq);
+#pragma unroll
+                        for (int u = 0; u < JC; ++u) l[qq][u] = zero;
+                        if (t >= t0 && t < M) {
+                            const C* Lrow = A + (k0 + t) + (size_t)(k0 + j0) * lda;
+#pragma unroll
+                            for (int u = 0; u < JC; ++u) l[qq][u] = Lrow[(size_t)u * lda];
+                        }
+                    }
+#pragma unroll
+                    for (int qq = 0; qq < QG; ++qq) {
+                        const int t = tid + NT * (g + qq);
+                        if (t >= t0 && t < M)
+#pragma unroll
+                            for (int u = 0; u < JC; ++u)
+#pragma unroll
+                                for (int s = 0; s < SB; ++s)
+                                    v[g + qq][s] = zsub(v[g + qq][s], zmul(l[qq][u], sU[(j0 + u) * SB + s]));
                     }
                 }
             }
