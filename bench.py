@@ -525,6 +525,9 @@ def main():
     ap.add_argument("--gemm-tiles", type=int, default=None,
                     help="force the complex64 tile product variant of the mixed form (cpt_set_tuning "
                          "gemm_tiles, 0-3) after tuning, for profiler passes of one variant")
+    ap.add_argument("--panel-threads", type=int, default=None, choices=(256, 512),
+                    help="force the panel leaf work-group size of both forms (cpt_set_tuning "
+                         "panel_threads) after tuning, for profiler passes of one variant")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner (a stored tuning of this size class is still applied)")
     ap.add_argument("--workload", choices=("barge", "cylinders", "copy", "serving", "post", "symmetric"),
@@ -608,6 +611,9 @@ def main():
         if args.gemm_tiles is not None and precision == "mixed":
             _lib.set_tuning(ctx, gemm_tiles=args.gemm_tiles)
             applied = dict(applied or {}, gemm_tiles=args.gemm_tiles)
+        if args.panel_threads is not None:
+            _lib.set_tuning(ctx, panel_threads=args.panel_threads)
+            applied = dict(applied or {}, panel_threads=args.panel_threads)
         if applied:
             print(f"[bench] {precision}: tuning {applied}", file=sys.stderr, flush=True)
         sweep = FrequencySweep(gf, body, ks, nrhs, precision=precision, max_iter=10, tol=1e-12,

@@ -51,7 +51,8 @@ SIGNATURES = {
 }
 
 # Tuning parameters of cpt_set_tuning, in the order of the CPT_TUNE_* values.
-TUNING_PARAMS = ("lu_block", "panel_leaf", "lookahead", "assembly_chunk", "panel_cus", "gemm_tiles")
+TUNING_PARAMS = ("lu_block", "panel_leaf", "lookahead", "assembly_chunk", "panel_cus", "gemm_tiles",
+                 "lookahead_active", "panel_threads")
 
 # Kernel classes of cpt_profile_end, in the order of the CPT_TIMED_* values.
 TIMED_KERNELS = ("zgemm", "cgemm", "panel_z", "panel_c", "assembly", "skinny_z", "skinny_c")
@@ -208,7 +209,8 @@ def lookahead_active(ctx):
     """Look-ahead mode the context's last factorisation ran (mode 2 runs as 1
     when the device has no hardware queue for its CU-masked streams)."""
     v = ctypes.c_int()
-    check(load().cpt_get_tuning(ctx.handle, 6, ctypes.byref(v)), "cpt_get_tuning")
+    check(load().cpt_get_tuning(ctx.handle, TUNING_PARAMS.index("lookahead_active"), ctypes.byref(v)),
+          "cpt_get_tuning")
     return v.value
 
 
@@ -216,6 +218,8 @@ def get_tuning(ctx):
     lib = load()
     out = {}
     for i, name in enumerate(TUNING_PARAMS):
+        if name == "lookahead_active":  # read only
+            continue
         v = ctypes.c_int()
         check(lib.cpt_get_tuning(ctx.handle, i, ctypes.byref(v)), "cpt_get_tuning")
         out[name] = v.value

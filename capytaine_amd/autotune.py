@@ -34,6 +34,13 @@ PANEL_CUS = (16, 32)         # reserved CUs tried with LOOKAHEAD = 2
 # complex64 product tiles: 128 x 64, 128 x 128, and the three-product (Gauss)
 # forms 128 x 128 (one wavefront per SIMD) and 128 x 64
 GEMM_TILES = (0, 1, 2, 3)
+# threads of a panel leaf's work-group timed with look-ahead 1 (the only
+# mode where a leaf runs beside the trailing update).  256 (one wavefront
+# per SIMD) fits on a CU beside one work-group of the tile product, but
+# measured slower on MI355X at 4352 rows x 40 (FP64 228 vs 204 ms; mixed
+# 127.7 vs 127.0 ms with the Gauss tiles), so only 512 is timed by default;
+# pass candidates with a sixth element 256 to time it
+PANEL_THREADS = (512,)
 ASSEMBLY_CHUNKS = (0, 4, 8, 16, 32, 64)
 
 _lock = threading.Lock()
@@ -108,9 +115,10 @@ def _time(fn, device, reps):
 
 
 def tune_lu(n, batch, precision="fp64", device=None, reps=2, nrhs=14, candidates=None, store=True, log=None):
-    """Time every (lu_block, panel_leaf, lookahead, panel_cus) candidate on
-    `batch` synthetic n x n matrices and return the fastest as a dict, with
-    the timings of all candidates under "timings_ms"."""
+    """Time every (lu_block, panel_leaf, lookahead, panel_cus, gemm_tiles,
+    panel_threads) candidate on `batch` synthetic n x n matrices and return
+    the fastest as a dict, with the timings of all candidates under
+    "timings_ms"."""
     import torch
     device = torch.device("cuda", 0) if device is None else torch.device(device)
     if precision not in ("fp64", "mixed"):
@@ -141,26 +149,31 @@ def tune_lu(n, batch, precision="fp64", device=None, reps=2, nrhs=14, candidates
     timings = {}
     if candidates is None:
         tiles = GEMM_TILES if precision == "mixed" else (0,)
-        candidates = [(blk, leaf, la, cus, tl)
+        candidates = [(blk, leaf, la, cus, tl, nt)
                       for blk, leaf, la, tl in itertools.product(LU_BLOCKS, PANEL_LEAVES, LOOKAHEAD, tiles)
-                      for cus in (PANEL_CUS if la == 2 else PANEL_CUS[:1])]
+                      for cus in (PANEL_CUS if la == 2 else PANEL_CUS[:1])
+                      for nt in (PANEL_THREADS if la == 1 else PANEL_THREADS[:1])]
     try:
-        for blk, leaf, la, cus, tl in candidates:
+        for cand in candidates:
+            blk, leaf, la, cus, tl = cand[:5]
+            nt = cand[5] if len(cand) > 5 else 512
             if leaf > blk // 2:
                 continue
-            _lib.set_tuning(ctx, lu_block=blk, panel_leaf=leaf, lookahead=la, panel_cus=cus, gemm_tiles=tl)
+            _lib.set_tuning(ctx, lu_block=blk, panel_leaf=leaf, lookahead=la, panel_cus=cus, gemm_tiles=tl,
+                            panel_threads=nt)
             t = _time(run, device, reps)
-            timings[f"{blk}/{leaf}/{la}/{cus}/{tl}"] = t
+            timings[f"{blk}/{leaf}/{la}/{cus}/{tl}/{nt}"] = t
             if log:
                 ran = _lib.lookahead_active(ctx)
                 log(f"[tune] n={n} batch={batch} {precision} block={blk} leaf={leaf} lookahead={la}"
-                    f"{'' if ran == la else f' (ran as {ran})'} panel_cus={cus} gemm_tiles={tl}: {t:.2f} ms")
+                    f"{'' if ran == la else f' (ran as {ran})'} panel_cus={cus} gemm_tiles={tl}"
+                    f" panel_threads={nt}: {t:.2f} ms")
     finally:
         _lib.set_tuning(ctx, **saved)
     best_key = min(timings, key=timings.get)
-    blk, leaf, la, cus, tl = (int(x) for x in best_key.split("/"))
+    blk, leaf, la, cus, tl, nt = (int(x) for x in best_key.split("/"))
     best = {"lu_block": blk, "panel_leaf": leaf, "lookahead": la, "panel_cus": cus, "gemm_tiles": tl,
-            "ms": timings[best_key], "timings_ms": timings}
+            "panel_threads": nt, "ms": timings[best_key], "timings_ms": timings}
     if store:
         _store(_key(device, "lu", precision, n, batch), best)
     return best
@@ -209,7 +222,8 @@ def apply(ctx, n, batch, precision="fp64", nb_wavenumbers=None):
     best = lookup(device, "lu", precision, n, batch)
     if best:
         applied.update(lu_block=best["lu_block"], panel_leaf=best["panel_leaf"], lookahead=best["lookahead"],
-                       panel_cus=best.get("panel_cus", 16), gemm_tiles=best.get("gemm_tiles", 1))
+                       panel_cus=best.get("panel_cus", 16), gemm_tiles=best.get("gemm_tiles", 1),
+                       panel_threads=best.get("panel_threads", 512))
     best = lookup(device, "assembly", "fp64", n, nb_wavenumbers or batch)
     if best:
         applied["assembly_chunk"] = best["assembly_chunk"]

@@ -1074,6 +1074,10 @@ int cpt_set_tuning(cpt_handle h, int param, int value)
         if (value < 0 || value > 1024) break;
         h->asm_chunk = value;
         return CPT_OK;
+    case CPT_TUNE_PANEL_THREADS:
+        if (value != 256 && value != 512) break;
+        h->lu_streams.panel_threads = value;
+        return CPT_OK;
     default:
         set_error("cpt_set_tuning: unknown parameter " + std::to_string(param));
         return CPT_ERR_ARG;
@@ -1102,6 +1106,7 @@ int cpt_get_tuning(cpt_handle h, int param, int* value)
                      : h->lu_streams.lookahead;
         return CPT_OK;
     case CPT_TUNE_ASSEMBLY_CHUNK: *value = h->asm_chunk; return CPT_OK;
+    case CPT_TUNE_PANEL_THREADS: *value = h->lu_streams.panel_threads; return CPT_OK;
     default:
         set_error("cpt_get_tuning: unknown parameter " + std::to_string(param));
         return CPT_ERR_ARG;

@@ -55,6 +55,7 @@ struct LuSetup {
     hipEvent_t ready = nullptr, done = nullptr;
     int block = 0;      // block width: 64, 128 or 256, 0 = by size (256 from 2048 rows, 128 from 1024)
     int leaf = 16;      // panel columns factored by one work-group per matrix
+    int panel_threads = 512;  // threads of a panel leaf's work-group: 512 or 256
     int lookahead = 1;  // 1: next panel on the side stream; 2: on CUs reserved for it
     // CU-partitioned look-ahead: panels on `part_side` (panel_cus CUs), the
     // rest of the factorisation on `part_main` (the other CUs)

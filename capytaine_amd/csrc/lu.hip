@@ -256,9 +256,9 @@ __global__ void __launch_bounds__(NT) panel_kernel(cx<R>* __restrict__ Abase, in
                 sL[i + PB * j] = A[(k0 + i) + (size_t)(k0 + j) * lda];
             }
             __syncthreads();
-            if (wave < sbw) {
-                // column sweep: x_j broadcast from lane j, lanes below update
-                const int s = wave;
+            // column sweep, one wavefront per column (NW may be below SB):
+            // x_j broadcast from lane j, lanes below update
+            for (int s = wave; s < sbw; s += NW) {
                 C x = ld_or_zero(sU, lane * SB + s, lane < t0);
                 for (int j = 0; j < t0 - 1; ++j) {
                     const C xj{lane_bcast(x.re, j), lane_bcast(x.im, j)};
@@ -1228,12 +1228,25 @@ void launch_panel(cx<R>* A, int64_t lda, int64_t sA, int N, int k0, int kb, int 
 // matrix; interchanges are applied to the columns [sw_lo, sw_hi).
 template <typename R>
 int panel_leaf(cx<R>* A, int64_t lda, int64_t sA, int N, int k0, int kb, int sw_lo, int sw_hi, int* ipiv, int64_t sP,
-               int* info, int batch, hipStream_t stream)
+               int* info, int batch, int nt, hipStream_t stream)
 {
     // 512 threads (two wavefronts per SIMD, up to 256 VGPRs each); the
     // register-resident sub-block holds RPT rows x SB columns per thread,
-    // RPT * SB <= 32 complex values up to 8192 rows.
+    // RPT * SB <= 32 complex values up to 8192 rows.  256 threads (one
+    // wavefront per SIMD) up to 8192 rows: a work-group fits on a CU beside
+    // one work-group of the tile product, so a leaf factored while a large
+    // product runs starts when one of the product's work-groups ends
+    // rather than when a whole CU has drained.
     const int M = N - k0;
+    if (nt == 256 && M <= 8192) {
+        if (M <= 256) launch_panel<R, 256, 1, 8>(A, lda, sA, N, k0, kb, sw_lo, sw_hi, ipiv, sP, info, batch, stream);
+        else if (M <= 512) launch_panel<R, 256, 2, 8>(A, lda, sA, N, k0, kb, sw_lo, sw_hi, ipiv, sP, info, batch, stream);
+        else if (M <= 1024) launch_panel<R, 256, 4, 8>(A, lda, sA, N, k0, kb, sw_lo, sw_hi, ipiv, sP, info, batch, stream);
+        else if (M <= 2048) launch_panel<R, 256, 8, 4>(A, lda, sA, N, k0, kb, sw_lo, sw_hi, ipiv, sP, info, batch, stream);
+        else if (M <= 4096) launch_panel<R, 256, 16, 2>(A, lda, sA, N, k0, kb, sw_lo, sw_hi, ipiv, sP, info, batch, stream);
+        else launch_panel<R, 256, 32, 1>(A, lda, sA, N, k0, kb, sw_lo, sw_hi, ipiv, sP, info, batch, stream);
+        return (int)hipGetLastError();
+    }
     if (M <= 512) launch_panel<R, 512, 1, 8>(A, lda, sA, N, k0, kb, sw_lo, sw_hi, ipiv, sP, info, batch, stream);
     else if (M <= 1024) launch_panel<R, 512, 2, 8>(A, lda, sA, N, k0, kb, sw_lo, sw_hi, ipiv, sP, info, batch, stream);
     else if (M <= 2048) launch_panel<R, 512, 4, 8>(A, lda, sA, N, k0, kb, sw_lo, sw_hi, ipiv, sP, info, batch, stream);
@@ -1277,15 +1290,15 @@ int trsm_lower(const cx<R>* L, int64_t lda, int64_t sA, int kb, cx<R>* B, int64_
 // half.  The one-work-group leaves stream only `leaf` columns.
 template <typename R>
 int panel_rec(cx<R>* A, int64_t lda, int64_t sA, int N, int c0, int w, int sw_lo, int sw_hi, int* ipiv, int64_t sP,
-              int* info, int batch, int leaf, hipStream_t stream)
+              int* info, int batch, int leaf, int nt, hipStream_t stream)
 {
     const int M = N - c0;
     // the leaf kernel holds at most PB columns (its LDS triangle)
     if (w <= PB && (w <= leaf || M <= 2 * leaf))
-        return panel_leaf<R>(A, lda, sA, N, c0, w, sw_lo, sw_hi, ipiv, sP, info, batch, stream);
+        return panel_leaf<R>(A, lda, sA, N, c0, w, sw_lo, sw_hi, ipiv, sP, info, batch, nt, stream);
     const int h = w / 2;
     int e;
-    if ((e = panel_rec<R>(A, lda, sA, N, c0, h, sw_lo, sw_hi, ipiv, sP, info, batch, leaf, stream))) return e;
+    if ((e = panel_rec<R>(A, lda, sA, N, c0, h, sw_lo, sw_hi, ipiv, sP, info, batch, leaf, nt, stream))) return e;
     cx<R>* A11 = A + c0 + (size_t)c0 * lda;
     cx<R>* A12 = A + c0 + (size_t)(c0 + h) * lda;
     cx<R>* A21 = A + (c0 + h) + (size_t)c0 * lda;
@@ -1293,20 +1306,20 @@ int panel_rec(cx<R>* A, int64_t lda, int64_t sA, int N, int c0, int w, int sw_lo
     if ((e = trsm_lower<R>(A11, lda, sA, h, A12, lda, sA, w - h, batch, stream))) return e;
     if (M > h && (e = gemm<R>(M - h, w - h, h, R(-1), A21, lda, sA, A12, lda, sA, A22, lda, sA, 0, batch, stream)))
         return e;
-    return panel_rec<R>(A, lda, sA, N, c0 + h, w - h, sw_lo, sw_hi, ipiv, sP, info, batch, leaf, stream);
+    return panel_rec<R>(A, lda, sA, N, c0 + h, w - h, sw_lo, sw_hi, ipiv, sP, info, batch, leaf, nt, stream);
 }
 
 // LU with partial pivoting of the panel columns [k0, k0 + kb), rows [k0, N);
 // interchanges applied within the panel columns only.
 template <typename R>
 int panel_dispatch(cx<R>* A, int64_t lda, int64_t sA, int N, int k0, int kb, int* ipiv, int64_t sP, int* info,
-                   int batch, int leaf, hipStream_t stream)
+                   int batch, int leaf, int nt, hipStream_t stream)
 {
     const int M = N - k0;
     // complex LU of an M x kb panel: 4 (M kb^2 - kb^3 / 3) real flops
     TimedLaunch timed(stream, sizeof(R) == 8 ? TK_PANEL_Z : TK_PANEL_C,
                       4.0 * ((double)M * kb * kb - (double)kb * kb * kb / 3.0) * batch);
-    return panel_rec<R>(A, lda, sA, N, k0, kb, k0, k0 + kb, ipiv, sP, info, batch, leaf, stream);
+    return panel_rec<R>(A, lda, sA, N, k0, kb, k0, k0 + kb, ipiv, sP, info, batch, leaf, nt, stream);
 }
 
 template <typename R>
@@ -1445,8 +1458,9 @@ int getrf_steps(cx<R>* A, int N, int64_t lda, int64_t strideA, int* ipiv, int64_
     int e;
     const int nb = getrf_block(N, aux ? aux->block : 0);
     const int leaf = aux ? aux->leaf : 16;
+    const int pnt = aux ? aux->panel_threads : 512;
     const bool ahead = aux && aux->side && aux->lookahead;
-    if ((e = panel_dispatch<R>(A, lda, strideA, N, 0, min(nb, N), ipiv, strideP, info, batch, leaf, stream))) return e;
+    if ((e = panel_dispatch<R>(A, lda, strideA, N, 0, min(nb, N), ipiv, strideP, info, batch, leaf, pnt, stream))) return e;
     for (int k0 = 0; k0 < N; k0 += nb) {
         const int kb = min(nb, N - k0);
         // panel k0 is factored (on `stream`, or on the side stream joined below)
@@ -1470,7 +1484,7 @@ int getrf_steps(cx<R>* A, int N, int64_t lda, int64_t strideA, int* ipiv, int64_
             // the trailing matrix is updated on `stream` (disjoint columns)
             if (hipEventRecord(aux->ready, stream) != hipSuccess) return (int)hipErrorUnknown;
             if (hipStreamWaitEvent(aux->side, aux->ready, 0) != hipSuccess) return (int)hipErrorUnknown;
-            if ((e = panel_dispatch<R>(A, lda, strideA, N, k0 + kb, kn, ipiv, strideP, info, batch, leaf, aux->side)))
+            if ((e = panel_dispatch<R>(A, lda, strideA, N, k0 + kb, kn, ipiv, strideP, info, batch, leaf, pnt, aux->side)))
                 return e;
             if (hipEventRecord(aux->done, aux->side) != hipSuccess) return (int)hipErrorUnknown;
         }
@@ -1480,7 +1494,7 @@ int getrf_steps(cx<R>* A, int N, int64_t lda, int64_t strideA, int* ipiv, int64_
         if (ahead) {
             if (hipStreamWaitEvent(stream, aux->done, 0) != hipSuccess) return (int)hipErrorUnknown;
         } else if ((e = panel_dispatch<R>(A, lda, strideA, N, k0 + kb, kn, ipiv, strideP, info, batch, leaf,
-                                          stream))) {
+                                          pnt, stream))) {
             return e;
         }
     }

@@ -311,8 +311,11 @@ enum {
                                     one wavefront per SIMD; 3: the Gauss form on 128 x 64.
                                     complex128 products always use 128 x 64, four products;
                                     default 0 */
-    CPT_TUNE_LOOKAHEAD_ACTIVE = 6 /* read only (cpt_get_tuning): the look-ahead mode the last
-                                     factorisation ran, 1 where mode 2 found no hardware queue */
+    CPT_TUNE_LOOKAHEAD_ACTIVE = 6, /* read only (cpt_get_tuning): the look-ahead mode the last
+                                      factorisation ran, 1 where mode 2 found no hardware queue */
+    CPT_TUNE_PANEL_THREADS = 7     /* threads of a panel leaf's work-group: 512 (default) or 256
+                                      (one wavefront per SIMD, up to 8192 rows: fits on a CU beside
+                                      one work-group of the tile product) */
 };
 int cpt_set_tuning(cpt_handle h, int param, int value);
 int cpt_get_tuning(cpt_handle h, int param, int* value);

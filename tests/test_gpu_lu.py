@@ -278,8 +278,9 @@ def test_lu_tunings_agree(block, leaf, lookahead, tiles):
         _lib.set_tuning(ctx, **saved)
 
 
-@pytest.mark.parametrize("block,leaf", [(256, 16), (256, 8), (128, 8), (64, 16)])
-def test_lu_pivoting_every_block_width(block, leaf):
+@pytest.mark.parametrize("block,leaf,nt", [(256, 16, 512), (256, 8, 512), (128, 8, 512), (64, 16, 512),
+                                           (256, 16, 256), (128, 8, 256)])
+def test_lu_pivoting_every_block_width(block, leaf, nt):
     """Matrices with no diagonal weight (a row interchange at nearly every
     column, rows from every part of the panel): the interchanges of each
     leaf reach all the panel's other columns, 256 of them at block width
@@ -288,11 +289,11 @@ def test_lu_pivoting_every_block_width(block, leaf):
     only."""
     from scipy.linalg import lu_factor, lu_solve
     from capytaine_amd import _lib
-    print(f"[gpu] LU pivoting block={block} leaf={leaf}", flush=True)
+    print(f"[gpu] LU pivoting block={block} leaf={leaf} panel threads={nt}", flush=True)
     ctx = _lib.Context.get(device())
     saved = _lib.get_tuning(ctx)
     try:
-        _lib.set_tuning(ctx, lu_block=block, panel_leaf=leaf, lookahead=0)
+        _lib.set_tuning(ctx, lu_block=block, panel_leaf=leaf, lookahead=0, panel_threads=nt)
         rng = np.random.default_rng(block * 7 + leaf)
         n = 700
         As, Bs = zip(*[_random_system(rng, n, 4, diag=0.0) for _ in range(2)])
@@ -312,11 +313,48 @@ def test_lu_pivoting_every_block_width(block, leaf):
         _lib.set_tuning(ctx, **saved)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_lu_panel_threads_agree_large(precision):
+    """256- and 512-thread panel leaves (every row count class up to 4352:
+    1, 2, 4, 8, 16 and 32 rows per thread) pick the same pivots and give
+    the same factor to rounding (the same operations in the same order per
+    row; the instantiations may contract products into FMAs differently)."""
+    from capytaine_amd import _lib
+    print(f"[gpu] LU panel threads 256 vs 512, n=4352, {precision}", flush=True)
+    ctx = _lib.Context.get(device())
+    saved = _lib.get_tuning(ctx)
+    rng = np.random.default_rng(11)
+    n = 4352
+    As, Bs = zip(*[_random_system(rng, n, 3) for _ in range(2)])
+    As, Bs = np.array(As), np.array(Bs)
+    out = []
+    try:
+        for nt in (512, 256):
+            _lib.set_tuning(ctx, lu_block=256, panel_leaf=16, lookahead=1, panel_threads=nt)
+            out.append(_factor_solve(As, Bs) if precision == "fp64" else _mixed(As, Bs))
+    finally:
+        _lib.set_tuning(ctx, **saved)
+    if precision == "fp64":
+        (X5, info5, piv5, LU5), (X2, info2, piv2, LU2) = out
+        assert not info5.any() and not info2.any()
+        assert np.array_equal(piv5, piv2)
+        assert np.abs(LU5 - LU2).max() <= 1e-13 * np.abs(LU5).max()
+        assert np.linalg.norm(X5 - X2) <= 1e-12 * np.linalg.norm(X5)
+        for X in (X5, X2):
+            r = np.linalg.norm(np.einsum("bij,bjr->bir", As, X) - Bs) / np.linalg.norm(Bs)
+            assert r < 1e-12
+    else:
+        (X5, _, r5), (X2, _, r2) = out
+        assert r5 <= 1e-12 and r2 <= 1e-12
+        assert np.linalg.norm(X5 - X2) <= 1e-12 * np.linalg.norm(X5)
+
+
 def test_tuning_rejects_bad_values():
     from capytaine_amd import _lib
     ctx = _lib.Context.get(device())
     for name, bad in [("lu_block", 96), ("lu_block", 512), ("panel_leaf", 0), ("panel_leaf", 65), ("lookahead", 3),
-                      ("assembly_chunk", -1), ("panel_cus", 0), ("panel_cus", 129), ("gemm_tiles", 4)]:
+                      ("assembly_chunk", -1), ("panel_cus", 0), ("panel_cus", 129), ("gemm_tiles", 4),
+                      ("panel_threads", 128), ("panel_threads", 1024), ("lookahead_active", 1)]:
         with pytest.raises(_lib.CptError):
             _lib.set_tuning(ctx, **{name: bad})
     with pytest.raises(_lib.CptError):
