@@ -40,6 +40,7 @@ sys.path.insert(0, ROOT)
 METRIC = "panel-pairs/s (S,K assembly) + BEM solves/s, mixed vs FP64 at 1e-6; p99 latency"
 MI355X_FP64_MFMA_TFLOPS = 78.6
 MI355X_FP32_MFMA_TFLOPS = 157.3
+MI355X_FP16_MFMA_TFLOPS = 2500.0  # dense, FP16 inputs, FP32 accumulation
 MI355X_HBM_GBS = 8000.0
 
 
@@ -524,7 +525,7 @@ def main():
     ap.add_argument("--no-symmetric", action="store_true", help="skip the reflection-symmetric barge measurement")
     ap.add_argument("--gemm-tiles", type=int, default=None,
                     help="force the complex64 tile product variant of the mixed form (cpt_set_tuning "
-                         "gemm_tiles, 0-3) after tuning, for profiler passes of one variant")
+                         "gemm_tiles, 0-4) after tuning, for profiler passes of one variant")
     ap.add_argument("--panel-threads", type=int, default=None, choices=(256, 512),
                     help="force the panel leaf work-group size of both forms (cpt_set_tuning "
                          "panel_threads) after tuning, for profiler passes of one variant")
@@ -683,9 +684,12 @@ def main():
     # the complex64 product the tuner chose for the mixed form
     ctiles = _lib.get_tuning(ctx)["gemm_tiles"]
     cname = {0: "gemm_tiled_kernel<float,64>", 1: "gemm_tiled_kernel<float,128>",
-             2: "gemm_tiled_gauss_w1_kernel<float,128>", 3: "gemm_tiled_kernel<float,64,true>"}[ctiles]
+             2: "gemm_tiled_gauss_w1_kernel<float,128>", 3: "gemm_tiled_kernel<float,64,true>",
+             4: "gemm_tiled_kernel<float,64,true,true>"}[ctiles]
+    cnote = (", three real products per complex one" if ctiles >= 2 else "") + \
+        (", FP16 inputs, FP32 accumulation" if ctiles == 4 else "")
     names = {"zgemm": "gemm_tiled_kernel<double,64> (complex128 MFMA tile product)",
-             "cgemm": f"{cname} (complex64 MFMA tile product{', three real products per complex one' if ctiles >= 2 else ''})",
+             "cgemm": f"{cname} (complex64 MFMA tile product{cnote})",
              "skinny_z": "gemm_skinny_kernel<double> (complex128 product, <= 32 columns)",
              "skinny_c": "gemm_skinny_kernel<float> (complex64 product, <= 32 columns)"}
     if dom.startswith("skinny"):
@@ -695,7 +699,8 @@ def main():
                     "work_per_launch": "bytes of A, B, C per product, summed over the batch"}
     else:
         achieved = kd["work"] / (kd["ms"] / 1e3) / 1e12
-        peak = MI355X_FP32_MFMA_TFLOPS if dom == "cgemm" else MI355X_FP64_MFMA_TFLOPS
+        peak = MI355X_FP64_MFMA_TFLOPS if dom != "cgemm" else \
+            MI355X_FP16_MFMA_TFLOPS if ctiles == 4 else MI355X_FP32_MFMA_TFLOPS
         roofline = {"bound": "mfma", "kernel": names[dom], "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                     "frac": achieved / peak, "traffic": None,
                     "work_per_launch": "8 M N K flop per complex M x N x K product, summed over the batch"}
@@ -734,7 +739,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c64 factor + f64 refinement (mixed); f64 reported beside",
+        "dtype": "c64 factor" + (" (FP16-input tile products)" if ctiles == 4 else "") +
+                 " + f64 refinement (mixed); f64 reported beside",
         "data": "synthetic",
         "config": {"workload": "barge 20x8x2 m, 4352 panels, 40 frequencies 0.2-2.5 rad/s, 6 dof + 8 headings "
                                "(560 problems), infinite depth; mixed-precision batched form vs FP64 plain form",

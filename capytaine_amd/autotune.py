@@ -31,9 +31,11 @@ PANEL_LEAVES = (8, 16)
 # from a partial mask; it remains available through cpt_set_tuning.
 LOOKAHEAD = (0, 1)
 PANEL_CUS = (16, 32)         # reserved CUs tried with LOOKAHEAD = 2
-# complex64 product tiles: 128 x 64, 128 x 128, and the three-product (Gauss)
-# forms 128 x 128 (one wavefront per SIMD) and 128 x 64
-GEMM_TILES = (0, 1, 2, 3)
+# complex64 product tiles: 128 x 64, 128 x 128, the three-product (Gauss)
+# forms 128 x 128 (one wavefront per SIMD) and 128 x 64, and the Gauss
+# 128 x 64 form with FP16 inputs (more refinement sweeps; a candidate whose
+# refinement misses the tolerance is discarded)
+GEMM_TILES = (0, 1, 2, 3, 4)
 # threads of a panel leaf's work-group timed with look-ahead 1 (the only
 # mode where a leaf runs beside the trailing update).  256 (one wavefront
 # per SIMD) fits on a CU beside one work-group of the tile product, but
@@ -162,6 +164,11 @@ def tune_lu(n, batch, precision="fp64", device=None, reps=2, nrhs=14, candidates
             _lib.set_tuning(ctx, lu_block=blk, panel_leaf=leaf, lookahead=la, panel_cus=cus, gemm_tiles=tl,
                             panel_threads=nt)
             t = _time(run, device, reps)
+            if precision == "mixed" and not res.value <= 1e-12:
+                if log:
+                    log(f"[tune] n={n} batch={batch} mixed gemm_tiles={tl}: refinement stopped at residual "
+                        f"{res.value:.2e} after {it.value} sweeps, discarded")
+                continue
             timings[f"{blk}/{leaf}/{la}/{cus}/{tl}/{nt}"] = t
             if log:
                 ran = _lib.lookahead_active(ctx)

@@ -1067,7 +1067,7 @@ int cpt_set_tuning(cpt_handle h, int param, int value)
         h->lu_streams.panel_cus = value;
         return CPT_OK;
     case CPT_TUNE_GEMM_TILES:
-        if (value < 0 || value > 3) break;
+        if (value < 0 || value > 4) break;
         h->gemm_tiles = value;
         return CPT_OK;
     case CPT_TUNE_ASSEMBLY_CHUNK:

@@ -37,6 +37,7 @@ namespace {
 
 typedef double dvec4 __attribute__((ext_vector_type(4)));
 typedef float fvec4 __attribute__((ext_vector_type(4)));
+typedef _Float16 hvec4 __attribute__((ext_vector_type(4)));
 
 template <typename R>
 struct cx {
@@ -1053,7 +1054,7 @@ constexpr int TBM = 128, TKC = 16;
 // GAUSS: three real products per complex one, P1 = Ar Br, P2 = Ai Bi,
 // P3 = (Ar + Ai)(Br + Bi), Re = P1 - P2, Im = P3 - P1 - P2 (6 M N K real
 // MFMA flops for the 8 M N K of the product; one more accumulator set).
-template <typename R, int TBN, bool GAUSS>
+template <typename R, int TBN, bool GAUSS, bool H16 = false>
 __device__ __forceinline__ void gemm_tiled_body(int M, int N, int K, R alpha, const cx<R>* __restrict__ Abase,
                                                 int64_t lda, int64_t strideA, const cx<R>* __restrict__ Bbase,
                                                 int64_t ldb, int64_t strideB, cx<R>* __restrict__ Cbase, int64_t ldc,
@@ -1115,6 +1116,42 @@ __device__ __forceinline__ void gemm_tiled_body(int M, int N, int K, R alpha, co
         }
         __syncthreads();
         if (kc + TKC < K) CPT_LOAD_CHUNK(kc + TKC);
+        if constexpr (H16) {
+            // FP16 inputs, FP32 accumulation: one 16-deep MFMA per product
+            // and tile pair, lane (li, lk) holding k = 4 lk .. 4 lk + 3 of
+            // row li of A and of column li of B; the output layout is the
+            // FP32 instruction's.  Gauss form: Re, Im and Re + Im rounded
+            // from FP32.
+            static_assert(sizeof(R) == 4 && GAUSS && TKC == 16, "FP16 tile product: complex64, Gauss form");
+            hvec4 ar[4], ai[4], as[4], br[FB], bi[FB], bs[FB];
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const Cx v = sA[4 * lk + j][wm * 64 + a * 16 + li];
+                    ar[a][j] = (_Float16)v.re;
+                    ai[a][j] = (_Float16)v.im;
+                    as[a][j] = (_Float16)(v.re + v.im);
+                }
+#pragma unroll
+            for (int b = 0; b < FB; ++b)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const Cx v = sB[wn * (TBN / 2) + b * 16 + li][4 * lk + j];
+                    br[b][j] = (_Float16)v.re;
+                    bi[b][j] = (_Float16)v.im;
+                    bs[b][j] = (_Float16)(v.re + v.im);
+                }
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < FB; ++b) {
+                    cr[a][b] = __builtin_amdgcn_mfma_f32_16x16x16f16(ar[a], br[b], cr[a][b], 0, 0, 0);
+                    ci[a][b] = __builtin_amdgcn_mfma_f32_16x16x16f16(ai[a], bi[b], ci[a][b], 0, 0, 0);
+                    cs[GAUSS ? a : 0][GAUSS ? b : 0] = __builtin_amdgcn_mfma_f32_16x16x16f16(
+                        as[a], bs[b], cs[GAUSS ? a : 0][GAUSS ? b : 0], 0, 0, 0);
+                }
+        } else
 #pragma unroll
         for (int ks = 0; ks < TKC; ks += 4) {
             R ar[4], ai[4], br[FB], bi[FB];
@@ -1193,14 +1230,14 @@ __device__ __forceinline__ void gemm_tiled_body(int M, int N, int K, R alpha, co
             }
 }
 
-template <typename R, int TBN, bool GAUSS = false>
+template <typename R, int TBN, bool GAUSS = false, bool H16 = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 gemm_tiled_kernel(int M, int N, int K, R alpha, const cx<R>* __restrict__ Abase, int64_t lda, int64_t strideA,
                   const cx<R>* __restrict__ Bbase, int64_t ldb, int64_t strideB, cx<R>* __restrict__ Cbase,
                   int64_t ldc, int64_t strideC, int beta_zero)
 {
-    gemm_tiled_body<R, TBN, GAUSS>(M, N, K, alpha, Abase, lda, strideA, Bbase, ldb, strideB, Cbase, ldc, strideC,
-                                   beta_zero);
+    gemm_tiled_body<R, TBN, GAUSS, H16>(M, N, K, alpha, Abase, lda, strideA, Bbase, ldb, strideB, Cbase, ldc,
+                                        strideC, beta_zero);
 }
 
 // The Gauss form with one wavefront per SIMD: its three accumulator sets fit
@@ -1357,6 +1394,12 @@ int gemm(int M, int N, int K, R alpha, const cx<R>* A, int64_t lda, int64_t sA, 
                 dim3 grid((M + TBM - 1) / TBM, (N + 63) / 64, batch);
                 hipLaunchKernelGGL((gemm_tiled_kernel<R, 64, true>), grid, dim3(256), 0, stream, M, N, K, alpha, A,
                                    lda, sA, B, ldb, sB, C, ldc, sC, beta_zero);
+                return (int)hipGetLastError();
+            }
+            if (active_gemm_tiles == 4) {
+                dim3 grid((M + TBM - 1) / TBM, (N + 63) / 64, batch);
+                hipLaunchKernelGGL((gemm_tiled_kernel<R, 64, true, true>), grid, dim3(256), 0, stream, M, N, K,
+                                   alpha, A, lda, sA, B, ldb, sB, C, ldc, sC, beta_zero);
                 return (int)hipGetLastError();
             }
         }

@@ -197,9 +197,11 @@ def test_mixed_precision_solve(n, nb):
         assert e < 1e-12, e                  # refined
 
 
-def test_mixed_precision_on_bem_matrix(default_table):
-    """Sphere (450 panels): mixed engine gives the FP64 engine's sources."""
-    from capytaine_amd import DefaultMatrixEngine, mesh_sphere
+@pytest.mark.parametrize("tiles", [None, 4])
+def test_mixed_precision_on_bem_matrix(default_table, tiles):
+    """Sphere (450 panels): mixed engine gives the FP64 engine's sources,
+    also with the FP16-input tile products (tiles 4)."""
+    from capytaine_amd import DefaultMatrixEngine, mesh_sphere, _lib
     from gpu_helpers import ref_table_gf
     mesh = mesh_sphere(radius=1.0, resolution=(30, 30)).immersed_part()
     gf = ref_table_gf(default_table)
@@ -209,10 +211,42 @@ def test_mixed_precision_on_bem_matrix(default_table):
     e64 = DefaultMatrixEngine(green_function=gf)
     _, K = e64.build_matrices(mesh, mesh, **params)
     x64 = e64.linear_solver(K, b)
-    emx = DefaultMatrixEngine(green_function=gf, linear_solver="mixed_precision")
-    _, K = emx.build_matrices(mesh, mesh, **params)
-    xmx = emx.linear_solver(K, b)
+    ctx = _lib.Context.get(device())
+    saved = _lib.get_tuning(ctx)
+    try:
+        if tiles is not None:
+            _lib.set_tuning(ctx, gemm_tiles=tiles)
+        emx = DefaultMatrixEngine(green_function=gf, linear_solver="mixed_precision")
+        _, K = emx.build_matrices(mesh, mesh, **params)
+        xmx = emx.linear_solver(K, b)
+    finally:
+        _lib.set_tuning(ctx, **saved)
     assert np.linalg.norm(xmx - x64) / np.linalg.norm(x64) < 1e-11
+
+
+def test_mixed_fp16_products_large():
+    """4352 unknowns, FP16-input tile products in the complex64 factor: the
+    FP64 refinement reaches 1e-12 within 10 sweeps and the solution equals
+    the FP32-product one's to that level."""
+    from capytaine_amd import _lib
+    print("[gpu] mixed solve with FP16 tile products, n=4352", flush=True)
+    ctx = _lib.Context.get(device())
+    saved = _lib.get_tuning(ctx)
+    rng = np.random.default_rng(5)
+    n = 4352
+    As, Bs = zip(*[_random_system(rng, n, 4) for _ in range(2)])
+    As, Bs = np.array(As), np.array(Bs)
+    out = {}
+    try:
+        for tiles in (3, 4):
+            _lib.set_tuning(ctx, lu_block=256, panel_leaf=16, lookahead=1, gemm_tiles=tiles)
+            out[tiles] = _mixed(As, Bs)
+    finally:
+        _lib.set_tuning(ctx, **saved)
+    (X3, it3, r3), (X4, it4, r4) = out[3], out[4]
+    print(f"[gpu] refinement sweeps: FP32 products {it3}, FP16 products {it4}", flush=True)
+    assert r3 <= 1e-12 and r4 <= 1e-12 and it4 <= 10
+    assert np.linalg.norm(X4 - X3) <= 1e-11 * np.linalg.norm(X3)
 
 
 def test_kernel_timing():
@@ -247,11 +281,14 @@ def test_kernel_timing():
 @pytest.mark.parametrize("block,leaf,lookahead,tiles", [(64, 8, 0, 1), (64, 32, 1, 0), (128, 16, 1, 1),
                                                         (128, 64, 0, 0), (128, 16, 2, 1), (64, 8, 2, 0),
                                                         (256, 16, 0, 0), (256, 8, 1, 0), (256, 16, 1, 2),
-                                                        (128, 16, 1, 3), (256, 8, 0, 3)])
+                                                        (128, 16, 1, 3), (256, 8, 0, 3), (256, 16, 1, 4),
+                                                        (64, 8, 0, 4)])
 def test_lu_tunings_agree(block, leaf, lookahead, tiles):
     """Every tuning of the blocked LU solves to LAPACK's accuracy, the
-    complex64 three-product (Gauss) tile products included (tiles 2 and 3;
-    the FP64 factorisation keeps the four-product form under any setting)."""
+    complex64 three-product (Gauss) tile products included (tiles 2 and 3,
+    and 4 with FP16 inputs, where the FP64 refinement recovers the
+    accuracy; the FP64 factorisation keeps the four-product form under any
+    setting)."""
     from scipy.linalg import lu_factor, lu_solve
     from capytaine_amd import _lib
     print(f"[gpu] LU tuning block={block} leaf={leaf} lookahead={lookahead} tiles={tiles}", flush=True)
@@ -353,7 +390,7 @@ def test_tuning_rejects_bad_values():
     from capytaine_amd import _lib
     ctx = _lib.Context.get(device())
     for name, bad in [("lu_block", 96), ("lu_block", 512), ("panel_leaf", 0), ("panel_leaf", 65), ("lookahead", 3),
-                      ("assembly_chunk", -1), ("panel_cus", 0), ("panel_cus", 129), ("gemm_tiles", 4),
+                      ("assembly_chunk", -1), ("panel_cus", 0), ("panel_cus", 129), ("gemm_tiles", 5),
                       ("panel_threads", 128), ("panel_threads", 1024), ("lookahead_active", 1)]:
         with pytest.raises(_lib.CptError):
             _lib.set_tuning(ctx, **{name: bad})
