@@ -1054,7 +1054,7 @@ constexpr int TBM = 128, TKC = 16;
 // GAUSS: three real products per complex one, P1 = Ar Br, P2 = Ai Bi,
 // P3 = (Ar + Ai)(Br + Bi), Re = P1 - P2, Im = P3 - P1 - P2 (6 M N K real
 // MFMA flops for the 8 M N K of the product; one more accumulator set).
-template <typename R, int TBN, bool GAUSS, bool H16 = false>
+template <typename R, int TBN, bool GAUSS>
 __device__ __forceinline__ void gemm_tiled_body(int M, int N, int K, R alpha, const cx<R>* __restrict__ Abase,
                                                 int64_t lda, int64_t strideA, const cx<R>* __restrict__ Bbase,
                                                 int64_t ldb, int64_t strideB, cx<R>* __restrict__ Cbase, int64_t ldc,
@@ -1116,42 +1116,6 @@ __device__ __forceinline__ void gemm_tiled_body(int M, int N, int K, R alpha, co
         }
         __syncthreads();
         if (kc + TKC < K) CPT_LOAD_CHUNK(kc + TKC);
-        if constexpr (H16) {
-            // FP16 inputs, FP32 accumulation: one 16-deep MFMA per product
-            // and tile pair, lane (li, lk) holding k = 4 lk .. 4 lk + 3 of
-            // row li of A and of column li of B; the output layout is the
-            // FP32 instruction's.  Gauss form: Re, Im and Re + Im rounded
-            // from FP32.
-            static_assert(sizeof(R) == 4 && GAUSS && TKC == 16, "FP16 tile product: complex64, Gauss form");
-            hvec4 ar[4], ai[4], as[4], br[FB], bi[FB], bs[FB];
-#pragma unroll
-            for (int a = 0; a < 4; ++a)
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const Cx v = sA[4 * lk + j][wm * 64 + a * 16 + li];
-                    ar[a][j] = (_Float16)v.re;
-                    ai[a][j] = (_Float16)v.im;
-                    as[a][j] = (_Float16)(v.re + v.im);
-                }
-#pragma unroll
-            for (int b = 0; b < FB; ++b)
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const Cx v = sB[wn * (TBN / 2) + b * 16 + li][4 * lk + j];
-                    br[b][j] = (_Float16)v.re;
-                    bi[b][j] = (_Float16)v.im;
-                    bs[b][j] = (_Float16)(v.re + v.im);
-                }
-#pragma unroll
-            for (int a = 0; a < 4; ++a)
-#pragma unroll
-                for (int b = 0; b < FB; ++b) {
-                    cr[a][b] = __builtin_amdgcn_mfma_f32_16x16x16f16(ar[a], br[b], cr[a][b], 0, 0, 0);
-                    ci[a][b] = __builtin_amdgcn_mfma_f32_16x16x16f16(ai[a], bi[b], ci[a][b], 0, 0, 0);
-                    cs[GAUSS ? a : 0][GAUSS ? b : 0] = __builtin_amdgcn_mfma_f32_16x16x16f16(
-                        as[a], bs[b], cs[GAUSS ? a : 0][GAUSS ? b : 0], 0, 0, 0);
-                }
-        } else
 #pragma unroll
         for (int ks = 0; ks < TKC; ks += 4) {
             R ar[4], ai[4], br[FB], bi[FB];
@@ -1230,14 +1194,14 @@ __device__ __forceinline__ void gemm_tiled_body(int M, int N, int K, R alpha, co
             }
 }
 
-template <typename R, int TBN, bool GAUSS = false, bool H16 = false>
+template <typename R, int TBN, bool GAUSS = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 gemm_tiled_kernel(int M, int N, int K, R alpha, const cx<R>* __restrict__ Abase, int64_t lda, int64_t strideA,
                   const cx<R>* __restrict__ Bbase, int64_t ldb, int64_t strideB, cx<R>* __restrict__ Cbase,
                   int64_t ldc, int64_t strideC, int beta_zero)
 {
-    gemm_tiled_body<R, TBN, GAUSS, H16>(M, N, K, alpha, Abase, lda, strideA, Bbase, ldb, strideB, Cbase, ldc,
-                                        strideC, beta_zero);
+    gemm_tiled_body<R, TBN, GAUSS>(M, N, K, alpha, Abase, lda, strideA, Bbase, ldb, strideB, Cbase, ldc, strideC,
+                                   beta_zero);
 }
 
 // The Gauss form with one wavefront per SIMD: its three accumulator sets fit
@@ -1252,6 +1216,147 @@ gemm_tiled_gauss_w1_kernel(int M, int N, int K, R alpha, const cx<R>* __restrict
                                   beta_zero);
 }
 
+
+// Complex64 tile product with FP16 inputs and FP32 accumulation, Gauss
+// form (P1 = Ar Br, P2 = Ai Bi, P3 = (Ar + Ai)(Br + Bi); Re = P1 - P2,
+// Im = P3 - P1 - P2), 128 x 64 tiles.  A and B are converted once per
+// work-group as they are staged: three FP16 planes (Re, Im, Re + Im) per
+// operand, k contiguous, so a lane reads the four k values of an MFMA
+// operand (row li of A or column li of B, k = 4 lk .. 4 lk + 3) with one
+// 8-byte LDS read.  Staging: a thread loads 8 consecutive k of one row of A
+// (lanes on consecutive rows: coalesced) and 4 consecutive k of one column
+// of B.
+constexpr int H16_KP = TKC + 4;  // k stride of a plane row in halfs (40 bytes: 8-byte aligned, 2-way writes)
+
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+gemm_h16_kernel(int M, int N, int K, float alpha, const cx<float>* __restrict__ Abase, int64_t lda, int64_t strideA,
+                const cx<float>* __restrict__ Bbase, int64_t ldb, int64_t strideB, cx<float>* __restrict__ Cbase,
+                int64_t ldc, int64_t strideC, int beta_zero)
+{
+    typedef cx<float> Cx;
+    constexpr int TBN = 64, FB = TBN / 32, KP = H16_KP;
+    static_assert(TKC == 16 && TBM == 128, "staging maps 256 threads onto 128 x 16 of A and 16 x 64 of B");
+    __shared__ __attribute__((aligned(16))) _Float16 sA[3][TBM * KP];
+    __shared__ __attribute__((aligned(16))) _Float16 sB[3][TBN * KP];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int row0 = blockIdx.x * TBM, col0 = blockIdx.y * TBN;
+    const Cx* A = Abase + (size_t)blockIdx.z * strideA;
+    const Cx* B = Bbase + (size_t)blockIdx.z * strideB;
+    Cx* C = Cbase + (size_t)blockIdx.z * strideC;
+
+    // staging roles: A row ar_ (k = 8 akh .. + 7 of the chunk); B column bc_ (k = 4 bkq .. + 3)
+    const int ar_ = tid & 127, akh = tid >> 7;
+    const int bc_ = tid >> 2, bkq = tid & 3;
+    const bool a_ok = row0 + ar_ < M, b_ok = col0 + bc_ < N;
+    const Cx* Arow = A + (row0 + ar_);
+    const Cx* Bcol = B + (size_t)(col0 + bc_) * ldb;
+    Cx ra[8], rb[4];
+
+    fvec4 cr[4][FB], ci[4][FB], cs[4][FB];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < FB; ++b) {
+            cr[a][b] = fvec4{0, 0, 0, 0};
+            ci[a][b] = fvec4{0, 0, 0, 0};
+            cs[a][b] = fvec4{0, 0, 0, 0};
+        }
+
+    auto load = [&](int kc) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int k = kc + 8 * akh + i;
+            if (a_ok && k < K) ra[i] = Arow[(size_t)k * lda];
+            else ra[i].re = ra[i].im = 0.0f;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int k = kc + 4 * bkq + j;
+            if (b_ok && k < K) rb[j] = Bcol[k];
+            else rb[j].re = rb[j].im = 0.0f;
+        }
+    };
+
+    load(0);
+    for (int kc = 0; kc < K; kc += TKC) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            hvec4 hr, hi, hs;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const Cx v = ra[4 * q + j];
+                hr[j] = (_Float16)v.re;
+                hi[j] = (_Float16)v.im;
+                hs[j] = (_Float16)(v.re + v.im);
+            }
+            const int off = ar_ * KP + 8 * akh + 4 * q;
+            *reinterpret_cast<hvec4*>(&sA[0][off]) = hr;
+            *reinterpret_cast<hvec4*>(&sA[1][off]) = hi;
+            *reinterpret_cast<hvec4*>(&sA[2][off]) = hs;
+        }
+        {
+            hvec4 hr, hi, hs;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                hr[j] = (_Float16)rb[j].re;
+                hi[j] = (_Float16)rb[j].im;
+                hs[j] = (_Float16)(rb[j].re + rb[j].im);
+            }
+            const int off = bc_ * KP + 4 * bkq;
+            *reinterpret_cast<hvec4*>(&sB[0][off]) = hr;
+            *reinterpret_cast<hvec4*>(&sB[1][off]) = hi;
+            *reinterpret_cast<hvec4*>(&sB[2][off]) = hs;
+        }
+        __syncthreads();
+        if (kc + TKC < K) load(kc + TKC);
+        hvec4 xr[4], xi[4], xs[4], yr[FB], yi[FB], ys[FB];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            const int off = (wm * 64 + a * 16 + li) * KP + 4 * lk;
+            xr[a] = *reinterpret_cast<const hvec4*>(&sA[0][off]);
+            xi[a] = *reinterpret_cast<const hvec4*>(&sA[1][off]);
+            xs[a] = *reinterpret_cast<const hvec4*>(&sA[2][off]);
+        }
+#pragma unroll
+        for (int b = 0; b < FB; ++b) {
+            const int off = (wn * (TBN / 2) + b * 16 + li) * KP + 4 * lk;
+            yr[b] = *reinterpret_cast<const hvec4*>(&sB[0][off]);
+            yi[b] = *reinterpret_cast<const hvec4*>(&sB[1][off]);
+            ys[b] = *reinterpret_cast<const hvec4*>(&sB[2][off]);
+        }
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < FB; ++b) {
+                cr[a][b] = __builtin_amdgcn_mfma_f32_16x16x16f16(xr[a], yr[b], cr[a][b], 0, 0, 0);
+                ci[a][b] = __builtin_amdgcn_mfma_f32_16x16x16f16(xi[a], yi[b], ci[a][b], 0, 0, 0);
+                cs[a][b] = __builtin_amdgcn_mfma_f32_16x16x16f16(xs[a], ys[b], cs[a][b], 0, 0, 0);
+            }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < FB; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const float p1 = cr[a][b][v], p2 = ci[a][b][v];
+                const int row = row0 + wm * 64 + 16 * a + Mfma<float>::row(lane, v);
+                const int col = col0 + wn * (TBN / 2) + 16 * b + li;
+                if (row < M && col < N) {
+                    Cx* c = C + row + (size_t)col * ldc;
+                    float re = beta_zero ? 0.0f : c->re;
+                    float im = beta_zero ? 0.0f : c->im;
+                    re += alpha * (p1 - p2);
+                    im += alpha * (cs[a][b][v] - p1 - p2);
+                    c->re = re;
+                    c->im = im;
+                }
+            }
+}
 
 template <typename R, int NT, int RPT, int SB>
 void launch_panel(cx<R>* A, int64_t lda, int64_t sA, int N, int k0, int kb, int sw_lo, int sw_hi, int* ipiv,
@@ -1398,8 +1503,8 @@ int gemm(int M, int N, int K, R alpha, const cx<R>* A, int64_t lda, int64_t sA, 
             }
             if (active_gemm_tiles == 4) {
                 dim3 grid((M + TBM - 1) / TBM, (N + 63) / 64, batch);
-                hipLaunchKernelGGL((gemm_tiled_kernel<R, 64, true, true>), grid, dim3(256), 0, stream, M, N, K,
-                                   alpha, A, lda, sA, B, ldb, sB, C, ldc, sC, beta_zero);
+                hipLaunchKernelGGL(gemm_h16_kernel, grid, dim3(256), 0, stream, M, N, K, alpha, A, lda, sA, B, ldb,
+                                   sB, C, ldc, sC, beta_zero);
                 return (int)hipGetLastError();
             }
         }
