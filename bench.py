@@ -685,7 +685,7 @@ def main():
     ctiles = _lib.get_tuning(ctx)["gemm_tiles"]
     cname = {0: "gemm_tiled_kernel<float,64>", 1: "gemm_tiled_kernel<float,128>",
              2: "gemm_tiled_gauss_w1_kernel<float,128>", 3: "gemm_tiled_kernel<float,64,true>",
-             4: "gemm_h16_kernel"}[ctiles]
+             4: "gemm_h16_kernel<16>"}[ctiles]
     cnote = (", three real products per complex one" if ctiles >= 2 else "") + \
         (", FP16 inputs, FP32 accumulation" if ctiles == 4 else "")
     names = {"zgemm": "gemm_tiled_kernel<double,64> (complex128 MFMA tile product)",

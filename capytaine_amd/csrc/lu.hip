@@ -1219,23 +1219,39 @@ gemm_tiled_gauss_w1_kernel(int M, int N, int K, R alpha, const cx<R>* __restrict
 
 // Complex64 tile product with FP16 inputs and FP32 accumulation, Gauss
 // form (P1 = Ar Br, P2 = Ai Bi, P3 = (Ar + Ai)(Br + Bi); Re = P1 - P2,
-// Im = P3 - P1 - P2), 128 x 64 tiles.  A and B are converted once per
-// work-group as they are staged: three FP16 planes (Re, Im, Re + Im) per
-// operand, k contiguous, so a lane reads the four k values of an MFMA
-// operand (row li of A or column li of B, k = 4 lk .. 4 lk + 3) with one
-// 8-byte LDS read.  Staging: a thread loads 8 consecutive k of one row of A
-// (lanes on consecutive rows: coalesced) and 4 consecutive k of one column
-// of B.
-constexpr int H16_KP = TKC + 4;  // k stride of a plane row in halfs (40 bytes: 8-byte aligned, 2-way writes)
+// Im = P3 - P1 - P2), 128 x 64 tiles, k in chunks of KC = 16.  A and
+// B are converted once per work-group as they are staged: three FP16 planes
+// (Re, Im, Re + Im) per operand, k contiguous, so a lane reads the KC / 4
+// k values of an MFMA operand (row li of A or column li of B, k = KC/4 lk
+// and on) with one LDS read.  A and B fill the instruction's k slots the
+// same way, so the product sums over every k of the chunk.  Staging: a
+// thread loads KC / 2 consecutive k of one row of A (lanes on consecutive
+// rows: coalesced) and KC / 4 consecutive k of one column of B.
+template <int KC>
+struct H16Op;
+template <>
+struct H16Op<16> {
+    typedef hvec4 v;
+    static constexpr int KP = 20;  // k stride of a plane row in halfs: 40 bytes, 8-byte aligned
+    static __device__ __forceinline__ fvec4 mma(v a, v b, fvec4 c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
+    }
+};
+// (a 32-deep chunk on v_mfma_f32_16x16x32_f16 needs 208 VGPRs and 46 KB of
+// LDS, two wavefronts per SIMD instead of three, and measured 3 % slower)
 
+template <int KC>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 gemm_h16_kernel(int M, int N, int K, float alpha, const cx<float>* __restrict__ Abase, int64_t lda, int64_t strideA,
                 const cx<float>* __restrict__ Bbase, int64_t ldb, int64_t strideB, cx<float>* __restrict__ Cbase,
                 int64_t ldc, int64_t strideC, int beta_zero)
 {
     typedef cx<float> Cx;
-    constexpr int TBN = 64, FB = TBN / 32, KP = H16_KP;
-    static_assert(TKC == 16 && TBM == 128, "staging maps 256 threads onto 128 x 16 of A and 16 x 64 of B");
+    typedef typename H16Op<KC>::v hv;
+    constexpr int TBN = 64, FB = TBN / 32, KP = H16Op<KC>::KP;
+    constexpr int AK = KC / 2, BK = KC / 4, LK = KC / 4;  // k per thread of A, of B; k per lane of an operand
+    static_assert(TBM == 128 && KP >= KC, "staging maps 256 threads onto 128 x KC of A and KC x 64 of B");
     __shared__ __attribute__((aligned(16))) _Float16 sA[3][TBM * KP];
     __shared__ __attribute__((aligned(16))) _Float16 sB[3][TBN * KP];
     const int tid = threadIdx.x;
@@ -1247,13 +1263,13 @@ gemm_h16_kernel(int M, int N, int K, float alpha, const cx<float>* __restrict__ 
     const Cx* B = Bbase + (size_t)blockIdx.z * strideB;
     Cx* C = Cbase + (size_t)blockIdx.z * strideC;
 
-    // staging roles: A row ar_ (k = 8 akh .. + 7 of the chunk); B column bc_ (k = 4 bkq .. + 3)
+    // staging roles: A row ar_ (k = AK akh and on); B column bc_ (k = BK bkq and on)
     const int ar_ = tid & 127, akh = tid >> 7;
     const int bc_ = tid >> 2, bkq = tid & 3;
     const bool a_ok = row0 + ar_ < M, b_ok = col0 + bc_ < N;
     const Cx* Arow = A + (row0 + ar_);
     const Cx* Bcol = B + (size_t)(col0 + bc_) * ldb;
-    Cx ra[8], rb[4];
+    Cx ra[AK], rb[BK];
 
     fvec4 cr[4][FB], ci[4][FB], cs[4][FB];
 #pragma unroll
@@ -1267,74 +1283,74 @@ gemm_h16_kernel(int M, int N, int K, float alpha, const cx<float>* __restrict__ 
 
     auto load = [&](int kc) {
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const int k = kc + 8 * akh + i;
+        for (int i = 0; i < AK; ++i) {
+            const int k = kc + AK * akh + i;
             if (a_ok && k < K) ra[i] = Arow[(size_t)k * lda];
             else ra[i].re = ra[i].im = 0.0f;
         }
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int k = kc + 4 * bkq + j;
+        for (int j = 0; j < BK; ++j) {
+            const int k = kc + BK * bkq + j;
             if (b_ok && k < K) rb[j] = Bcol[k];
             else rb[j].re = rb[j].im = 0.0f;
         }
     };
 
     load(0);
-    for (int kc = 0; kc < K; kc += TKC) {
+    for (int kc = 0; kc < K; kc += KC) {
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            hvec4 hr, hi, hs;
+        for (int q = 0; q < AK / LK; ++q) {
+            hv hr, hi, hs;
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const Cx v = ra[4 * q + j];
+            for (int j = 0; j < LK; ++j) {
+                const Cx v = ra[LK * q + j];
                 hr[j] = (_Float16)v.re;
                 hi[j] = (_Float16)v.im;
                 hs[j] = (_Float16)(v.re + v.im);
             }
-            const int off = ar_ * KP + 8 * akh + 4 * q;
-            *reinterpret_cast<hvec4*>(&sA[0][off]) = hr;
-            *reinterpret_cast<hvec4*>(&sA[1][off]) = hi;
-            *reinterpret_cast<hvec4*>(&sA[2][off]) = hs;
+            const int off = ar_ * KP + AK * akh + LK * q;
+            *reinterpret_cast<hv*>(&sA[0][off]) = hr;
+            *reinterpret_cast<hv*>(&sA[1][off]) = hi;
+            *reinterpret_cast<hv*>(&sA[2][off]) = hs;
         }
-        {
-            hvec4 hr, hi, hs;
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                hr[j] = (_Float16)rb[j].re;
-                hi[j] = (_Float16)rb[j].im;
-                hs[j] = (_Float16)(rb[j].re + rb[j].im);
+        for (int q = 0; q < BK / LK; ++q) {
+            hv hr, hi, hs;
+#pragma unroll
+            for (int j = 0; j < LK; ++j) {
+                const Cx v = rb[LK * q + j];
+                hr[j] = (_Float16)v.re;
+                hi[j] = (_Float16)v.im;
+                hs[j] = (_Float16)(v.re + v.im);
             }
-            const int off = bc_ * KP + 4 * bkq;
-            *reinterpret_cast<hvec4*>(&sB[0][off]) = hr;
-            *reinterpret_cast<hvec4*>(&sB[1][off]) = hi;
-            *reinterpret_cast<hvec4*>(&sB[2][off]) = hs;
+            const int off = bc_ * KP + BK * bkq + LK * q;
+            *reinterpret_cast<hv*>(&sB[0][off]) = hr;
+            *reinterpret_cast<hv*>(&sB[1][off]) = hi;
+            *reinterpret_cast<hv*>(&sB[2][off]) = hs;
         }
         __syncthreads();
-        if (kc + TKC < K) load(kc + TKC);
-        hvec4 xr[4], xi[4], xs[4], yr[FB], yi[FB], ys[FB];
-#pragma unroll
-        for (int a = 0; a < 4; ++a) {
-            const int off = (wm * 64 + a * 16 + li) * KP + 4 * lk;
-            xr[a] = *reinterpret_cast<const hvec4*>(&sA[0][off]);
-            xi[a] = *reinterpret_cast<const hvec4*>(&sA[1][off]);
-            xs[a] = *reinterpret_cast<const hvec4*>(&sA[2][off]);
-        }
+        if (kc + KC < K) load(kc + KC);
+        hv yr[FB], yi[FB], ys[FB];
 #pragma unroll
         for (int b = 0; b < FB; ++b) {
-            const int off = (wn * (TBN / 2) + b * 16 + li) * KP + 4 * lk;
-            yr[b] = *reinterpret_cast<const hvec4*>(&sB[0][off]);
-            yi[b] = *reinterpret_cast<const hvec4*>(&sB[1][off]);
-            ys[b] = *reinterpret_cast<const hvec4*>(&sB[2][off]);
+            const int off = (wn * (TBN / 2) + b * 16 + li) * KP + LK * lk;
+            yr[b] = *reinterpret_cast<const hv*>(&sB[0][off]);
+            yi[b] = *reinterpret_cast<const hv*>(&sB[1][off]);
+            ys[b] = *reinterpret_cast<const hv*>(&sB[2][off]);
         }
 #pragma unroll
-        for (int a = 0; a < 4; ++a)
+        for (int a = 0; a < 4; ++a) {
+            const int off = (wm * 64 + a * 16 + li) * KP + LK * lk;
+            const hv xr = *reinterpret_cast<const hv*>(&sA[0][off]);
+            const hv xi = *reinterpret_cast<const hv*>(&sA[1][off]);
+            const hv xs = *reinterpret_cast<const hv*>(&sA[2][off]);
 #pragma unroll
             for (int b = 0; b < FB; ++b) {
-                cr[a][b] = __builtin_amdgcn_mfma_f32_16x16x16f16(xr[a], yr[b], cr[a][b], 0, 0, 0);
-                ci[a][b] = __builtin_amdgcn_mfma_f32_16x16x16f16(xi[a], yi[b], ci[a][b], 0, 0, 0);
-                cs[a][b] = __builtin_amdgcn_mfma_f32_16x16x16f16(xs[a], ys[b], cs[a][b], 0, 0, 0);
+                cr[a][b] = H16Op<KC>::mma(xr, yr[b], cr[a][b]);
+                ci[a][b] = H16Op<KC>::mma(xi, yi[b], ci[a][b]);
+                cs[a][b] = H16Op<KC>::mma(xs, ys[b], cs[a][b]);
             }
+        }
         __syncthreads();
     }
 #pragma unroll
@@ -1503,7 +1519,7 @@ int gemm(int M, int N, int K, R alpha, const cx<R>* A, int64_t lda, int64_t sA, 
             }
             if (active_gemm_tiles == 4) {
                 dim3 grid((M + TBM - 1) / TBM, (N + 63) / 64, batch);
-                hipLaunchKernelGGL(gemm_h16_kernel, grid, dim3(256), 0, stream, M, N, K, alpha, A, lda, sA, B, ldb,
+                hipLaunchKernelGGL(gemm_h16_kernel<16>, grid, dim3(256), 0, stream, M, N, K, alpha, A, lda, sA, B, ldb,
                                    sB, C, ldc, sC, beta_zero);
                 return (int)hipGetLastError();
             }
