@@ -998,6 +998,18 @@ int cpt_lu_solve_mixed(cpt_handle h, const void* K, int n, int64_t lda, int64_t 
                                                  strideB, (cplx*)X, ldx, strideX, max_iter, tol, work,
                                                  &h->lu_streams, iterations, residual, stream),
                   "mixed solve");
+    if (h->gemm_tiles == 4 && !(*residual <= tol)) {
+        // the FP16-input products left the refinement short of tol: solve
+        // again (X is rewritten, K and B are untouched) with FP32-input
+        // products, and report the sweeps of both
+        const int sweeps = *iterations;
+        CallScope fp32_scope(&h->timer, 3);
+        CPT_CHECK_HIP((hipError_t)launch_mixed_solve((const cplx*)K, n, lda, strideK, batch, (const cplx*)B, nrhs,
+                                                     ldb, strideB, (cplx*)X, ldx, strideX, max_iter, tol, work,
+                                                     &h->lu_streams, iterations, residual, stream),
+                      "mixed solve (FP32-input products)");
+        *iterations += sweeps;
+    }
     return CPT_OK;
 }
 
@@ -1228,6 +1240,42 @@ int cpt_lu_solve_mixed_grouped(cpt_handle h, int ngroups, const cpt_lu_group* gr
         return CPT_ERR_HIP;
     }
     CPT_CHECK_HIP((hipError_t)e, "grouped mixed solve");
+    if (h->gemm_tiles == 4) {
+        // groups the FP16-input products left short of tol are solved again
+        // with FP32-input products; their sweeps add up
+        std::vector<MixedGroup> again;
+        std::vector<int> idx;
+        for (int i = 0; i < ngroups; ++i)
+            if (!(residual[i] <= tol)) {
+                again.push_back(g[i]);
+                idx.push_back(i);
+            }
+        if (!again.empty()) {
+            CallScope fp32_scope(&h->timer, 3);
+            std::vector<int> it2(again.size());
+            std::vector<double> res2(again.size());
+            void* work2 = mixed_workspace(h, mixed_grouped_bytes(again.data(), (int)again.size()));
+            if (!work2) {
+                set_error("cpt_lu_solve_mixed_grouped: workspace allocation failed");
+                return CPT_ERR_ALLOC;
+            }
+            if (fork_lanes(h, stream)) {
+                set_error("cpt_lu_solve_mixed_grouped: could not set up the lane streams");
+                return CPT_ERR_HIP;
+            }
+            const int e2 =
+                launch_mixed_grouped(again.data(), (int)again.size(), work2, &tune, max_iter, tol, it2.data(), res2.data());
+            if (join_lanes(h, stream)) {
+                set_error("cpt_lu_solve_mixed_grouped: could not join the lane streams");
+                return CPT_ERR_HIP;
+            }
+            CPT_CHECK_HIP((hipError_t)e2, "grouped mixed solve (FP32-input products)");
+            for (size_t k = 0; k < again.size(); ++k) {
+                iterations[idx[k]] += it2[k];
+                residual[idx[k]] = res2[k];
+            }
+        }
+    }
     return CPT_OK;
 }
 

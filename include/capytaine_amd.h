@@ -173,7 +173,10 @@ int cpt_lu_solve(cpt_handle h, const void* LU, int n, int64_t lda, int64_t strid
  * (against the unrounded complex128 K) and complex64 corrections until
  * max_j ||r_j||_2 / ||b_j||_2 <= tol or max_iter refinement sweeps.  K and B
  * are not modified.  Outputs the sweeps used and the final relative
- * residual.  Synchronises `stream` once per sweep.
+ * residual.  Synchronises `stream` once per sweep.  With FP16-input tile
+ * products (CPT_TUNE_GEMM_TILES 4), a solve left short of tol is done again
+ * with FP32-input products; the sweeps of both are reported (the grouped
+ * variant does the same per group).
  * Replaces, with the north-star's mixed-precision scheme, the factor + solve
  * of DefaultMatrixEngine.linear_solver (src/capytaine/bem/engines.py:326-338,
  * LAPACK zgetrf/zgetrs through block_circulant_matrices.py:374-385). */

@@ -142,3 +142,49 @@ def test_grouped_lu_random():
             Xr = lu_solve(lu_factor(As[i][b]), Bs[i][b])
             assert np.linalg.norm(Xg[b] - Xr) / np.linalg.norm(Xr) < 1e-10
             assert np.linalg.norm(Xm[b] - Xr) / np.linalg.norm(Xr) < 1e-10
+
+
+def test_grouped_mixed_fp16_products_fall_back_to_fp32():
+    """cpt_lu_solve_mixed_grouped with FP16-input products and one sweep
+    allowed: the groups left short of tol (orders 300 and 700, whose
+    factors use the tile products) are solved again with FP32-input
+    products; every group meets tol and matches scipy."""
+    import torch
+    from scipy.linalg import lu_factor, lu_solve
+    from capytaine_amd import _lib
+    from capytaine_amd._lib import CptLuGroup, check, ptr, stream_handle
+    print("[gpu] grouped mixed: FP16 products, one sweep, FP32 retry", flush=True)
+    rng = np.random.default_rng(8)
+    dev = device()
+    sizes, batches, r = [65, 300, 700], [2, 2, 1], 4
+    As, Bs = [], []
+    for n, nb in zip(sizes, batches):
+        A = rng.normal(size=(nb, n, n)) + 1j * rng.normal(size=(nb, n, n))
+        As.append(A / np.sqrt(n) + 3 * np.eye(n))
+        Bs.append(rng.normal(size=(nb, n, r)) + 1j * rng.normal(size=(nb, n, r)))
+    t = lambda a: torch.as_tensor(np.ascontiguousarray(np.swapaxes(a, 1, 2))).to(dev)
+    K, B = [t(A) for A in As], [t(b) for b in Bs]
+    X = [torch.empty_like(b) for b in B]
+    gm = (CptLuGroup * len(sizes))()
+    for i, (n, nb) in enumerate(zip(sizes, batches)):
+        gm[i] = CptLuGroup(n=n, batch=nb, A=K[i].data_ptr(), lda=n, strideA=n * n, B=B[i].data_ptr(), nrhs=r,
+                           ldb=n, strideB=n * r, X=X[i].data_ptr(), ldx=n, strideX=n * r)
+    lib = _lib.load()
+    ctx = _lib.Context.get(dev)
+    saved = _lib.get_tuning(ctx)
+    its = np.zeros(len(sizes), dtype=np.int32)
+    res = np.zeros(len(sizes))
+    try:
+        _lib.set_tuning(ctx, gemm_tiles=4)
+        check(lib.cpt_lu_solve_mixed_grouped(ctx.handle, len(sizes), gm, 1, 1e-12, ptr(its), ptr(res),
+                                             stream_handle(dev)), "mixed")
+        torch.cuda.synchronize()
+    finally:
+        _lib.set_tuning(ctx, **saved)
+    assert np.all(res <= 1e-12)
+    assert its[1] == 2 and its[2] == 2
+    for i in range(len(sizes)):
+        Xm = np.swapaxes(X[i].cpu().numpy(), 1, 2)
+        for b in range(batches[i]):
+            Xr = lu_solve(lu_factor(As[i][b]), Bs[i][b])
+            assert np.linalg.norm(Xm[b] - Xr) / np.linalg.norm(Xr) < 1e-10

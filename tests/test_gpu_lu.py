@@ -386,6 +386,30 @@ def test_lu_panel_threads_agree_large(precision):
         assert np.linalg.norm(X5 - X2) <= 1e-12 * np.linalg.norm(X5)
 
 
+def test_mixed_fp16_products_fall_back_to_fp32():
+    """When the FP16-input products leave the refinement short of tol (here
+    with one sweep allowed), cpt_lu_solve_mixed solves again with FP32-input
+    products: the residual meets tol and the sweeps of both are reported."""
+    from scipy.linalg import lu_factor, lu_solve
+    from capytaine_amd import _lib
+    print("[gpu] mixed solve: FP16 products, one sweep, FP32 retry", flush=True)
+    ctx = _lib.Context.get(device())
+    saved = _lib.get_tuning(ctx)
+    rng = np.random.default_rng(21)
+    n = 1100
+    As, Bs = zip(*[_random_system(rng, n, 3) for _ in range(2)])
+    As, Bs = np.array(As), np.array(Bs)
+    try:
+        _lib.set_tuning(ctx, gemm_tiles=4)
+        X, it, res = _mixed(As, Bs, max_iter=1)
+    finally:
+        _lib.set_tuning(ctx, **saved)
+    assert res <= 1e-12 and it == 2
+    for b in range(2):
+        Xr = lu_solve(lu_factor(As[b]), Bs[b])
+        assert np.linalg.norm(X[b] - Xr) / np.linalg.norm(Xr) < 1e-10
+
+
 def test_tuning_rejects_bad_values():
     from capytaine_amd import _lib
     ctx = _lib.Context.get(device())
