@@ -38,6 +38,7 @@ namespace {
 typedef double dvec4 __attribute__((ext_vector_type(4)));
 typedef float fvec4 __attribute__((ext_vector_type(4)));
 typedef _Float16 hvec4 __attribute__((ext_vector_type(4)));
+typedef float fvec4u __attribute__((ext_vector_type(4), aligned(8)));  // 16 bytes at 8-byte alignment
 
 template <typename R>
 struct cx {
@@ -1353,25 +1354,43 @@ gemm_h16_kernel(int M, int N, int K, float alpha, const cx<float>* __restrict__ 
         }
         __syncthreads();
     }
+    // a lane holds rows 4 lk .. 4 lk + 3 of one column of each 16 x 16 tile
+    // (Mfma<float>::row): 32 contiguous bytes of C, read and written as two
+    // 16-byte vectors (8-byte alignment asked: ldc and offsets may be odd);
+    // tiles cut by the last row take one row at a time
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int b = 0; b < FB; ++b)
+        for (int b = 0; b < FB; ++b) {
+            const int rowb = row0 + wm * 64 + 16 * a + 4 * lk;
+            const int col = col0 + wn * (TBN / 2) + 16 * b + li;
+            if (col >= N || rowb >= M) continue;
+            float re[4], im[4];
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const float p1 = cr[a][b][v], p2 = ci[a][b][v];
-                const int row = row0 + wm * 64 + 16 * a + Mfma<float>::row(lane, v);
-                const int col = col0 + wn * (TBN / 2) + 16 * b + li;
-                if (row < M && col < N) {
-                    Cx* c = C + row + (size_t)col * ldc;
-                    float re = beta_zero ? 0.0f : c->re;
-                    float im = beta_zero ? 0.0f : c->im;
-                    re += alpha * (p1 - p2);
-                    im += alpha * (cs[a][b][v] - p1 - p2);
-                    c->re = re;
-                    c->im = im;
-                }
+                re[v] = alpha * (p1 - p2);
+                im[v] = alpha * (cs[a][b][v] - p1 - p2);
             }
+            Cx* c = C + rowb + (size_t)col * ldc;
+            if (rowb + 3 < M) {
+                fvec4u* cv = reinterpret_cast<fvec4u*>(c);
+                fvec4u lo = {re[0], im[0], re[1], im[1]}, hi = {re[2], im[2], re[3], im[3]};
+                if (!beta_zero) {
+                    lo += cv[0];
+                    hi += cv[1];
+                }
+                cv[0] = lo;
+                cv[1] = hi;
+            } else {
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    if (rowb + v < M) {
+                        c[v].re = (beta_zero ? 0.0f : c[v].re) + re[v];
+                        c[v].im = (beta_zero ? 0.0f : c[v].im) + im[v];
+                    }
+            }
+        }
 }
 
 template <typename R, int NT, int RPT, int SB>
