@@ -1175,24 +1175,57 @@ __device__ __forceinline__ void gemm_tiled_body(int M, int N, int K, R alpha, co
                     ci[a][b][v] = cs[GAUSS ? a : 0][GAUSS ? b : 0][v] - p1 - p2;
                 }
     }
+    if constexpr (sizeof(R) == 4) {
+        // complex64: a lane holds rows 4 lk .. 4 lk + 3 of one column of a
+        // 16 x 16 tile, 32 contiguous bytes of C, as two 16-byte vectors
+        // (8-byte alignment asked); tiles cut by the last row one row at a time
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+        for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int b = 0; b < FB; ++b)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int row = row0 + wm * 64 + 16 * a + Mfma<R>::row(lane, v);
+            for (int b = 0; b < FB; ++b) {
+                const int rowb = row0 + wm * 64 + 16 * a + 4 * lk;
                 const int col = col0 + wn * (TBN / 2) + 16 * b + li;
-                if (row < M && col < N) {
-                    Cx* c = C + row + (size_t)col * ldc;
-                    R re = beta_zero ? R(0) : c->re;
-                    R im = beta_zero ? R(0) : c->im;
-                    re += alpha * cr[a][b][v];
-                    im += alpha * ci[a][b][v];
-                    c->re = re;
-                    c->im = im;
+                if (col >= N || rowb >= M) continue;
+                Cx* c = C + rowb + (size_t)col * ldc;
+                if (rowb + 3 < M) {
+                    fvec4u* cv = reinterpret_cast<fvec4u*>(c);
+                    fvec4u lo = {alpha * cr[a][b][0], alpha * ci[a][b][0], alpha * cr[a][b][1], alpha * ci[a][b][1]};
+                    fvec4u hi = {alpha * cr[a][b][2], alpha * ci[a][b][2], alpha * cr[a][b][3], alpha * ci[a][b][3]};
+                    if (!beta_zero) {
+                        lo += cv[0];
+                        hi += cv[1];
+                    }
+                    cv[0] = lo;
+                    cv[1] = hi;
+                } else {
+#pragma unroll
+                    for (int v = 0; v < 4; ++v)
+                        if (rowb + v < M) {
+                            c[v].re = (beta_zero ? R(0) : c[v].re) + alpha * cr[a][b][v];
+                            c[v].im = (beta_zero ? R(0) : c[v].im) + alpha * ci[a][b][v];
+                        }
                 }
             }
+    } else {
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < FB; ++b)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int row = row0 + wm * 64 + 16 * a + Mfma<R>::row(lane, v);
+                    const int col = col0 + wn * (TBN / 2) + 16 * b + li;
+                    if (row < M && col < N) {
+                        Cx* c = C + row + (size_t)col * ldc;
+                        R re = beta_zero ? R(0) : c->re;
+                        R im = beta_zero ? R(0) : c->im;
+                        re += alpha * cr[a][b][v];
+                        im += alpha * ci[a][b][v];
+                        c->re = re;
+                        c->im = im;
+                    }
+                }
+    }
 }
 
 template <typename R, int TBN, bool GAUSS = false>
