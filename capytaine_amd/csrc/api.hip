@@ -15,8 +15,8 @@
 
 namespace cpt {
 
-int launch_tabulation(const double* r_range, int nr, const double* z_range, int nz, int npts, double* tab,
-                      hipStream_t stream);
+int tabulate_on_device(const double* r_range, int nr, const double* z_range, int nz, int npts, double* tab,
+                       hipStream_t stream);
 int launch_table_relayout(const double* src, int nr, int nz, double* dst, hipStream_t stream);
 int launch_zgemm(int M, int N, int K, double alpha, const cplx* A, int64_t lda, int64_t sA, const cplx* B,
                  int64_t ldb, int64_t sB, cplx* C, int64_t ldc, int64_t sC, int beta_zero, int batch,
@@ -468,9 +468,8 @@ int cpt_tabulate(cpt_handle h, const double* r_range, int nr, const double* z_ra
         return CPT_ERR_ARG;
     }
     CPT_CHECK_HIP(hipSetDevice(h->device), "hipSetDevice");
-    int e = launch_tabulation(r_range, nr, z_range, nz, nb_integration_points, tab, stream);
-    CPT_CHECK_HIP((hipError_t)e, "tabulate kernel launch");
-    return CPT_OK;
+    const int e = tabulate_on_device(r_range, nr, z_range, nz, nb_integration_points, tab, stream);
+    return e == 0 ? CPT_OK : (e == -3 ? CPT_ERR_ALLOC : CPT_ERR_HIP);
 }
 
 int cpt_set_tabulation(cpt_handle h, const double* r_range, int nr, const double* z_range, int nz,
