@@ -48,11 +48,17 @@ SIGNATURES = {
     "cpt_set_tuning": (_I, [_P, _I, _I]),
     "cpt_get_tuning": (_I, [_P, _I, ctypes.POINTER(ctypes.c_int)]),
     "cpt_block_transform": (_I, [_P, _I, _I, _I, _I, _I, _I, _P, _L, _L, _L, _P, _L, _L, _L, _P]),
+    "cpt_mixed_factor_size": (_I, [_I, _I, ctypes.POINTER(ctypes.c_size_t)]),
+    "cpt_mixed_factor": (_I, [_P, _P, _I, _L, _L, _I, _P, ctypes.c_size_t, _P]),
+    "cpt_mixed_refine": (_I, [_P, _P, _I, _L, _L, _I, _P, ctypes.c_size_t, _P, _I, _L, _L, _P, _L, _L, _I, _D,
+                              ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double), _P]),
 }
 
-# Tuning parameters of cpt_set_tuning, in the order of the CPT_TUNE_* values.
-TUNING_PARAMS = ("lu_block", "panel_leaf", "lookahead", "assembly_chunk", "panel_cus", "gemm_tiles",
-                 "lookahead_active", "panel_threads")
+# Tuning parameters of cpt_set_tuning, in the order of the CPT_TUNE_* values
+# (None: a retired value).
+TUNING_PARAMS = ("lu_block", "panel_leaf", "lookahead", "assembly_chunk", None, "gemm_tiles",
+                 "lookahead_active", "panel_threads", "mixed_retries")
+READ_ONLY_TUNING = ("lookahead_active", "mixed_retries")
 
 # Kernel classes of cpt_profile_end, in the order of the CPT_TIMED_* values.
 TIMED_KERNELS = ("zgemm", "cgemm", "panel_z", "panel_c", "assembly", "skinny_z", "skinny_c")
@@ -80,6 +86,8 @@ _lock = threading.RLock()  # Context.get -> Context.__init__ -> load() re-enters
 
 
 CPT_ERR_ALLOC = -3
+CPT_ERR_UNSUPPORTED = -5
+CPT_ERR_NOT_CONVERGED = -6
 
 
 class CptError(RuntimeError):
@@ -88,6 +96,13 @@ class CptError(RuntimeError):
     def __init__(self, msg, status=None):
         super().__init__(msg)
         self.status = status
+
+
+class ConvergenceError(CptError):
+    """An iterative solve (mixed-precision refinement, GMRES) ended above its
+    tolerance (CPT_ERR_NOT_CONVERGED).  This can be due to overlapping panels
+    or irregular frequencies, as the reference says of its GMRES
+    (engines.py:84-86)."""
 
 
 def call_with_memory_retry(fn, what):
@@ -121,7 +136,8 @@ def load():
 def check(status, what):
     if status != 0:
         msg = load().cpt_last_error().decode(errors="replace")
-        raise CptError(f"{what} failed with status {status}: {msg}", status)
+        cls = ConvergenceError if status == CPT_ERR_NOT_CONVERGED else CptError
+        raise cls(f"{what} failed with status {status}: {msg}", status)
 
 
 def ptr(t):
@@ -205,20 +221,28 @@ def set_tuning(ctx, **params):
         check(lib.cpt_set_tuning(ctx.handle, TUNING_PARAMS.index(name), int(value)), f"cpt_set_tuning({name})")
 
 
-def lookahead_active(ctx):
-    """Look-ahead mode the context's last factorisation ran (mode 2 runs as 1
-    when the device has no hardware queue for its CU-masked streams)."""
+def _get_one(ctx, name):
     v = ctypes.c_int()
-    check(load().cpt_get_tuning(ctx.handle, TUNING_PARAMS.index("lookahead_active"), ctypes.byref(v)),
-          "cpt_get_tuning")
+    check(load().cpt_get_tuning(ctx.handle, TUNING_PARAMS.index(name), ctypes.byref(v)), "cpt_get_tuning")
     return v.value
+
+
+def lookahead_active(ctx):
+    """Look-ahead mode the context's factorisations run."""
+    return _get_one(ctx, "lookahead_active")
+
+
+def mixed_retries(ctx):
+    """Mixed solves (or groups) of the context factored again with
+    FP32-input products after FP16-input products fell short."""
+    return _get_one(ctx, "mixed_retries")
 
 
 def get_tuning(ctx):
     lib = load()
     out = {}
     for i, name in enumerate(TUNING_PARAMS):
-        if name == "lookahead_active":  # read only
+        if name is None or name in READ_ONLY_TUNING:
             continue
         v = ctypes.c_int()
         check(lib.cpt_get_tuning(ctx.handle, i, ctypes.byref(v)), "cpt_get_tuning")

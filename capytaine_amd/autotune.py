@@ -25,16 +25,13 @@ from ._lib import check, ptr, stream_handle
 
 LU_BLOCKS = (128, 256)      # 64 loses at every measured size above 1000 rows
 PANEL_LEAVES = (8, 16)
-# look-ahead modes timed: 0 and 1.  Mode 2 (CU-masked streams) is not
-# tried: on MI355X under ROCm 7.2 the masked streams are refused ("out of
-# memory") and a fresh process has crashed inside the runtime creating one
-# from a partial mask; it remains available through cpt_set_tuning.
+# look-ahead modes: 0 (in order) and 1 (next panel on a side stream)
 LOOKAHEAD = (0, 1)
-PANEL_CUS = (16, 32)         # reserved CUs tried with LOOKAHEAD = 2
 # complex64 product tiles: 128 x 64, 128 x 128, the three-product (Gauss)
 # forms 128 x 128 (one wavefront per SIMD) and 128 x 64, and the Gauss
-# 128 x 64 form with FP16 inputs (more refinement sweeps; a candidate whose
-# refinement misses the tolerance is discarded)
+# 128 x 64 form with FP16 inputs (more refinement sweeps).  A candidate whose
+# solve was factored again with FP32-input products (the context's
+# mixed_retries count grew) or missed the tolerance is discarded.
 GEMM_TILES = (0, 1, 2, 3, 4)
 # threads of a panel leaf's work-group timed with look-ahead 1 (the only
 # mode where a leaf runs beside the trailing update).  256 (one wavefront
@@ -117,8 +114,8 @@ def _time(fn, device, reps):
 
 
 def tune_lu(n, batch, precision="fp64", device=None, reps=2, nrhs=14, candidates=None, store=True, log=None):
-    """Time every (lu_block, panel_leaf, lookahead, panel_cus, gemm_tiles,
-    panel_threads) candidate on `batch` synthetic n x n matrices and return
+    """Time every (lu_block, panel_leaf, lookahead, gemm_tiles, panel_threads)
+    candidate on `batch` synthetic n x n matrices and return
     the fastest as a dict, with the timings of all candidates under
     "timings_ms"."""
     import torch
@@ -138,48 +135,51 @@ def tune_lu(n, batch, precision="fp64", device=None, reps=2, nrhs=14, candidates
     import ctypes
     it, res = ctypes.c_int(), ctypes.c_double()
 
+    failed = []
+
     def run():
         if precision == "fp64":
             A.copy_(A0)
             check(lib.cpt_lu_factor(ctx.handle, ptr(A), n, n, n * n, ptr(ipiv), n, ptr(info), batch, st),
                   "cpt_lu_factor")
         else:
-            check(lib.cpt_lu_solve_mixed(ctx.handle, ptr(A0), n, n, n * n, batch, ptr(B), nrhs, n, n * nrhs,
-                                         ptr(X), n, n * nrhs, 10, 1e-12, ctypes.byref(it), ctypes.byref(res), st),
-                  "cpt_lu_solve_mixed")
+            try:
+                check(lib.cpt_lu_solve_mixed(ctx.handle, ptr(A0), n, n, n * n, batch, ptr(B), nrhs, n, n * nrhs,
+                                             ptr(X), n, n * nrhs, 10, 1e-12, ctypes.byref(it), ctypes.byref(res),
+                                             st), "cpt_lu_solve_mixed")
+            except _lib.ConvergenceError:
+                failed.append(True)
 
     timings = {}
     if candidates is None:
         tiles = GEMM_TILES if precision == "mixed" else (0,)
-        candidates = [(blk, leaf, la, cus, tl, nt)
+        candidates = [(blk, leaf, la, tl, nt)
                       for blk, leaf, la, tl in itertools.product(LU_BLOCKS, PANEL_LEAVES, LOOKAHEAD, tiles)
-                      for cus in (PANEL_CUS if la == 2 else PANEL_CUS[:1])
                       for nt in (PANEL_THREADS if la == 1 else PANEL_THREADS[:1])]
     try:
         for cand in candidates:
-            blk, leaf, la, cus, tl = cand[:5]
-            nt = cand[5] if len(cand) > 5 else 512
+            blk, leaf, la, tl = cand[:4]
+            nt = cand[4] if len(cand) > 4 else 512
             if leaf > blk // 2:
                 continue
-            _lib.set_tuning(ctx, lu_block=blk, panel_leaf=leaf, lookahead=la, panel_cus=cus, gemm_tiles=tl,
-                            panel_threads=nt)
+            _lib.set_tuning(ctx, lu_block=blk, panel_leaf=leaf, lookahead=la, gemm_tiles=tl, panel_threads=nt)
+            retries0 = _lib.mixed_retries(ctx)
+            del failed[:]
             t = _time(run, device, reps)
-            if precision == "mixed" and not res.value <= 1e-12:
+            if precision == "mixed" and (failed or _lib.mixed_retries(ctx) != retries0):
                 if log:
-                    log(f"[tune] n={n} batch={batch} mixed gemm_tiles={tl}: refinement stopped at residual "
-                        f"{res.value:.2e} after {it.value} sweeps, discarded")
+                    log(f"[tune] n={n} batch={batch} mixed gemm_tiles={tl}: the refinement needed the FP32 "
+                        f"retry or missed 1e-12 (residual {res.value:.2e}, {it.value} sweeps), discarded")
                 continue
-            timings[f"{blk}/{leaf}/{la}/{cus}/{tl}/{nt}"] = t
+            timings[f"{blk}/{leaf}/{la}/{tl}/{nt}"] = t
             if log:
-                ran = _lib.lookahead_active(ctx)
                 log(f"[tune] n={n} batch={batch} {precision} block={blk} leaf={leaf} lookahead={la}"
-                    f"{'' if ran == la else f' (ran as {ran})'} panel_cus={cus} gemm_tiles={tl}"
-                    f" panel_threads={nt}: {t:.2f} ms")
+                    f" gemm_tiles={tl} panel_threads={nt}: {t:.2f} ms")
     finally:
         _lib.set_tuning(ctx, **saved)
     best_key = min(timings, key=timings.get)
-    blk, leaf, la, cus, tl, nt = (int(x) for x in best_key.split("/"))
-    best = {"lu_block": blk, "panel_leaf": leaf, "lookahead": la, "panel_cus": cus, "gemm_tiles": tl,
+    blk, leaf, la, tl, nt = (int(x) for x in best_key.split("/"))
+    best = {"lu_block": blk, "panel_leaf": leaf, "lookahead": la, "gemm_tiles": tl,
             "panel_threads": nt, "ms": timings[best_key], "timings_ms": timings}
     if store:
         _store(_key(device, "lu", precision, n, batch), best)
@@ -228,8 +228,8 @@ def apply(ctx, n, batch, precision="fp64", nb_wavenumbers=None):
     applied = {}
     best = lookup(device, "lu", precision, n, batch)
     if best:
-        applied.update(lu_block=best["lu_block"], panel_leaf=best["panel_leaf"], lookahead=best["lookahead"],
-                       panel_cus=best.get("panel_cus", 16), gemm_tiles=best.get("gemm_tiles", 1),
+        applied.update(lu_block=best["lu_block"], panel_leaf=best["panel_leaf"],
+                       lookahead=min(int(best["lookahead"]), 1), gemm_tiles=best.get("gemm_tiles", 1),
                        panel_threads=best.get("panel_threads", 512))
     best = lookup(device, "assembly", "fp64", n, nb_wavenumbers or batch)
     if best:

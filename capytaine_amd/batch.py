@@ -177,7 +177,7 @@ class BatchedSweep:
                 "cpt_lu_solve_mixed_grouped")
         finally:
             check(lib.cpt_set_workspace(self.ctx.handle, None, 0), "cpt_set_workspace")
-        self.iterations, self.residual = its, res
+            self.iterations, self.residual = its, res
 
     def post(self):
         lib = _lib.load()

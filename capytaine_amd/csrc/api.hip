@@ -27,13 +27,10 @@ size_t getrf_workspace_bytes(int N, int batch);
 size_t mixed_workspace_bytes(int n, int nrhs, int batch);
 size_t mixed_grouped_bytes(const MixedGroup* g, int ng);
 int launch_mixed_grouped(const MixedGroup* g, int ng, void* work, const LuSetup* tune, int max_iter, double tol,
-                         int* iterations, double* residual);
+                         int* iterations, double* residual, int* info_out);
 int launch_getrf_lanes(const LuGroupArgs* g, int ng, const LuSetup* tune, void* work);
 int launch_getrs_lanes(const LuGroupArgs* g, int ng, void* work);
 size_t getrs_workspace_bytes(int N, int batch);
-int launch_mixed_solve(const cplx* K, int n, int64_t lda, int64_t strideK, int batch, const cplx* B, int nrhs,
-                       int64_t ldb, int64_t strideB, cplx* X, int64_t ldx, int64_t strideX, int max_iter, double tol,
-                       void* work, const LuSetup* aux, int* iterations, double* residual, hipStream_t stream);
 int launch_getrs(const cplx* LU, int N, int64_t lda, int64_t strideA, const int* ipiv, int64_t strideP, cplx* B,
                  int nrhs, int64_t ldb, int64_t strideB, int batch, cplx* work, hipStream_t stream);
 int launch_block_transform(int xor_kind, int n, int inverse, int R, int C, int nbatch, const cplx* in, int64_t sbi,
@@ -136,9 +133,8 @@ struct cpt_context {
     cpt::DevBuf lu_work, mixed_work, solve_work;  // solve_work: inverted diagonal blocks of the solves
     cpt::LuSetup lu_streams;
     int asm_chunk = 0;  // panels per assembly work-group, 0 = by size
-    int part_cus = 0;   // CUs of the current partitioned look-ahead streams
     int gemm_tiles = 0; // 1: complex64 products on 128 x 128 tiles, 0: 128 x 64
-    bool part_failed = false;  // the device refused CU-masked streams
+    long mixed_retries = 0;    // FP32-input re-factorisations after FP16-input products fell short
     void* ext_work = nullptr;  // caller-owned workspace of the mixed solves
     size_t ext_bytes = 0;
     // lanes of the grouped LU: default-priority streams, a fork event and
@@ -221,69 +217,8 @@ __global__ void zero_int_kernel(int* __restrict__ p, int n)
 }
 
 // Side stream (default priority) and events of the LU look-ahead.
-// Streams of the CU-partitioned look-ahead: panel_cus CUs spread evenly
-// over the device for the panels, the other CUs for the rest.  Recreated
-// when panel_cus changes.
-int ensure_partition(cpt_context* h)
-{
-    LuSetup& L = h->lu_streams;
-    if (L.lookahead != 2 || h->part_failed) return 0;
-    // CU-masked streams only on request: on MI355X under ROCm 7.2 they are
-    // refused ("out of memory"), and creating one from a partial mask has
-    // crashed a process inside the runtime
-    const char* opt = std::getenv("CAPYTAINE_AMD_CU_MASK");
-    if (!opt || std::strcmp(opt, "1") != 0) {
-        set_error("CU-masked streams are off (CAPYTAINE_AMD_CU_MASK=1 enables them)");
-        h->part_failed = true;
-        return -1;
-    }
-    int ncu = 0;
-    hipError_t err = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->device);
-    if (err != hipSuccess || ncu < 8) {
-        set_error(std::string("CU count query failed: ") + hipGetErrorString(err));
-        h->part_failed = true;
-        return -1;
-    }
-    const int want = std::max(1, std::min(L.panel_cus, ncu / 2));
-    if (L.part_main && h->part_cus == want) return 0;
-    if (L.part_main) (void)hipStreamDestroy(L.part_main);
-    if (L.part_side) (void)hipStreamDestroy(L.part_side);
-    L.part_main = L.part_side = nullptr;
-    const int words = (ncu + 31) / 32;
-    std::vector<uint32_t> side(words, 0u), rest(words, 0u);
-    for (int c = 0; c < ncu; ++c) {
-        const bool panel = (c * want) / ncu != ((c + 1) * want) / ncu;  // want CUs, evenly spaced
-        (panel ? side : rest)[c / 32] |= 1u << (c % 32);
-    }
-    if ((err = hipExtStreamCreateWithCUMask(&L.part_side, words, side.data())) != hipSuccess ||
-        (err = hipExtStreamCreateWithCUMask(&L.part_main, words, rest.data())) != hipSuccess) {
-        set_error(std::string("hipExtStreamCreateWithCUMask (") + std::to_string(ncu) + " CUs, " +
-                  std::to_string(want) + " for the panels): " + hipGetErrorString(err));
-        if (L.part_side) (void)hipStreamDestroy(L.part_side);
-        L.part_side = nullptr;
-        (void)hipGetLastError();  // do not leave the failure for the next launch check
-        h->part_failed = true;
-        return -1;
-    }
-    if ((!L.enter && hipEventCreateWithFlags(&L.enter, hipEventDisableTiming) != hipSuccess) ||
-        (!L.leave && hipEventCreateWithFlags(&L.leave, hipEventDisableTiming) != hipSuccess)) {
-        set_error("look-ahead events could not be created");
-        h->part_failed = true;
-        return -1;
-    }
-    h->part_cus = want;
-    return 0;
-}
-
 int ensure_lu_streams(cpt_context* h)
 {
-    // A CU-masked stream needs a hardware queue of its own; when the device
-    // has none left (GPU_MAX_HW_QUEUES), mode 2 runs as mode 1.
-    if (ensure_partition(h)) {
-        h->lu_streams.part_main = nullptr;
-        // once per context: say why mode 2 runs as mode 1
-        std::fprintf(stderr, "capytaine_amd: look-ahead mode 2 runs as mode 1: %s\n", cpt_last_error());
-    }
     if (h->lu_streams.side) return 0;
     if (hipStreamCreateWithFlags(&h->lu_streams.side, hipStreamNonBlocking) != hipSuccess) return -1;
     if (hipEventCreateWithFlags(&h->lu_streams.ready, hipEventDisableTiming) != hipSuccess) return -1;
@@ -377,6 +312,70 @@ void* mixed_workspace(cpt_context* h, size_t bytes)
     return h->mixed_work.p;
 }
 
+// Header of a caller-owned complex64 factor (cpt_mixed_factor): magic, n,
+// batch and the product form that factored it, in the first 256 bytes.
+constexpr int FACTOR_MAGIC = 0x4650434d;
+constexpr size_t FACTOR_HEADER = 256;
+
+__global__ void factor_header_kernel(int* p, int magic, int n, int batch, int tiles)
+{
+    if (threadIdx.x == 0) {
+        p[0] = magic;
+        p[1] = n;
+        p[2] = batch;
+        p[3] = tiles;
+    }
+}
+
+// Refinement of one group with its factor F, factored first when `factor`.
+// *tiles is the product form of F (set when factoring here).  When the
+// FP16-input products (form 4) leave the refinement short of tol, F is
+// factored again with FP32-input products and the refinement repeated; the
+// sweeps of both are reported.  CPT_ERR_NOT_CONVERGED when the final
+// residual exceeds tol or the complex64 factor is exactly singular; X,
+// iterations and residual are written in every case.
+int mixed_solve_one(cpt_context* h, const MixedGroup& g, const MixedFactor& F, void* fws, void* rws, bool factor,
+                    int* tiles, int max_iter, double tol, int* iterations, double* residual, const char* who)
+{
+    int info = 0, e;
+    if (factor) {
+        *tiles = h->gemm_tiles;
+        if ((e = mixed_factor(g.K, g.n, g.lda, g.strideK, g.batch, F, fws, &h->lu_streams, g.s))) {
+            set_error(std::string(who) + ": complex64 factorisation: " + hipGetErrorString((hipError_t)e));
+            return CPT_ERR_HIP;
+        }
+    }
+    if ((e = mixed_refine(&g, &F, 1, rws, max_iter, tol, iterations, residual, &info))) {
+        set_error(std::string(who) + ": refinement: " + hipGetErrorString((hipError_t)e));
+        return CPT_ERR_HIP;
+    }
+    if (*tiles == 4 && !(*residual <= tol)) {
+        const int sweeps = *iterations;
+        CallScope fp32_scope(&h->timer, 3);
+        if ((e = mixed_factor(g.K, g.n, g.lda, g.strideK, g.batch, F, fws, &h->lu_streams, g.s)) ||
+            (e = mixed_refine(&g, &F, 1, rws, max_iter, tol, iterations, residual, &info))) {
+            set_error(std::string(who) + ": FP32-input retry: " + hipGetErrorString((hipError_t)e));
+            return CPT_ERR_HIP;
+        }
+        *tiles = 3;
+        ++h->mixed_retries;
+        *iterations += sweeps;
+    }
+    if (info != 0) {
+        set_error(std::string(who) + ": the complex64 factor is exactly singular (pivot " + std::to_string(info) +
+                  " is zero); relative residual " + std::to_string(*residual));
+        return CPT_ERR_NOT_CONVERGED;
+    }
+    if (!(*residual <= tol)) {
+        char buf[160];
+        std::snprintf(buf, sizeof buf, ": the refinement stopped at relative residual %.3e > tol %.3e after %d sweeps",
+                      *residual, tol, *iterations);
+        set_error(std::string(who) + buf);
+        return CPT_ERR_NOT_CONVERGED;
+    }
+    return CPT_OK;
+}
+
 Table table_of(const cpt_context* h)
 {
     Table T;
@@ -448,10 +447,6 @@ int cpt_destroy(cpt_handle h)
     if (h->lu_streams.side) (void)hipStreamDestroy(h->lu_streams.side);
     if (h->lu_streams.ready) (void)hipEventDestroy(h->lu_streams.ready);
     if (h->lu_streams.done) (void)hipEventDestroy(h->lu_streams.done);
-    if (h->lu_streams.part_main) (void)hipStreamDestroy(h->lu_streams.part_main);
-    if (h->lu_streams.part_side) (void)hipStreamDestroy(h->lu_streams.part_side);
-    if (h->lu_streams.enter) (void)hipEventDestroy(h->lu_streams.enter);
-    if (h->lu_streams.leave) (void)hipEventDestroy(h->lu_streams.leave);
     h->timer.release();
     for (hipStream_t l : h->lanes) (void)hipStreamDestroy(l);
     for (hipEvent_t ev : h->lane_done) (void)hipEventDestroy(ev);
@@ -962,9 +957,9 @@ int cpt_lu_solve_mixed(cpt_handle h, const void* K, int n, int64_t lda, int64_t 
                        int nrhs, int64_t ldb, int64_t strideB, void* X, int64_t ldx, int64_t strideX, int max_iter,
                        double tol, int* iterations, double* residual, hipStream_t stream)
 {
-    if (!h || n < 0 || nrhs < 0 || batch < 0 || lda < n || ldb < n || ldx < n || max_iter < 0 ||
+    if (!h || n < 0 || nrhs < 0 || batch < 0 || lda < n || ldb < n || ldx < n || max_iter < 0 || !(tol >= 0) ||
         (batch > 1 && (strideK < lda * n || strideB < ldb * nrhs || strideX < ldx * nrhs))) {
-        set_error("cpt_lu_solve_mixed: invalid sizes or strides");
+        set_error("cpt_lu_solve_mixed: invalid sizes, strides, max_iter or tol");
         return CPT_ERR_ARG;
     }
     if (!iterations || !residual) {
@@ -984,6 +979,9 @@ int cpt_lu_solve_mixed(cpt_handle h, const void* K, int n, int64_t lda, int64_t 
     }
     CPT_CHECK_HIP(hipSetDevice(h->device), "hipSetDevice");
     CallScope call_scope(&h->timer, h->gemm_tiles);
+    const MixedGroup g{(const cplx*)K, n, lda, strideK, batch, (const cplx*)B, nrhs, ldb, strideB, (cplx*)X, ldx,
+                       strideX, stream};
+    const size_t fb = mixed_factor_bytes(n, batch), wb = mixed_factor_ws_bytes(n, batch);
     void* work = mixed_workspace(h, mixed_workspace_bytes(n, nrhs, batch));
     if (!work) {
         set_error("cpt_lu_solve_mixed: workspace allocation failed");
@@ -993,23 +991,117 @@ int cpt_lu_solve_mixed(cpt_handle h, const void* K, int n, int64_t lda, int64_t 
         set_error("cpt_lu_solve_mixed: could not create the look-ahead streams: " + std::string(cpt_last_error()));
         return CPT_ERR_HIP;
     }
-    CPT_CHECK_HIP((hipError_t)launch_mixed_solve((const cplx*)K, n, lda, strideK, batch, (const cplx*)B, nrhs, ldb,
-                                                 strideB, (cplx*)X, ldx, strideX, max_iter, tol, work,
-                                                 &h->lu_streams, iterations, residual, stream),
-                  "mixed solve");
-    if (h->gemm_tiles == 4 && !(*residual <= tol)) {
-        // the FP16-input products left the refinement short of tol: solve
-        // again (X is rewritten, K and B are untouched) with FP32-input
-        // products, and report the sweeps of both
-        const int sweeps = *iterations;
-        CallScope fp32_scope(&h->timer, 3);
-        CPT_CHECK_HIP((hipError_t)launch_mixed_solve((const cplx*)K, n, lda, strideK, batch, (const cplx*)B, nrhs,
-                                                     ldb, strideB, (cplx*)X, ldx, strideX, max_iter, tol, work,
-                                                     &h->lu_streams, iterations, residual, stream),
-                      "mixed solve (FP32-input products)");
-        *iterations += sweeps;
+    char* base = (char*)(((uintptr_t)work + 255) & ~(uintptr_t)255);
+    const MixedFactor F = mixed_factor_at(base, n, batch);
+    int tiles = 0;
+    return mixed_solve_one(h, g, F, base + fb, base + fb + wb, true, &tiles, max_iter, tol, iterations, residual,
+                           "cpt_lu_solve_mixed");
+}
+
+int cpt_mixed_factor_size(int n, int batch, size_t* bytes)
+{
+    if (n < 0 || batch < 0 || !bytes) {
+        set_error("cpt_mixed_factor_size: invalid arguments");
+        return CPT_ERR_ARG;
     }
+    *bytes = FACTOR_HEADER + mixed_factor_bytes(n, batch);
     return CPT_OK;
+}
+
+int cpt_mixed_factor(cpt_handle h, const void* K, int n, int64_t lda, int64_t strideK, int batch, void* factor,
+                     size_t factor_bytes, hipStream_t stream)
+{
+    if (!h || n < 1 || batch < 1 || lda < n || (batch > 1 && strideK < lda * n) || !K || !factor ||
+        ((uintptr_t)factor & 255) != 0) {
+        set_error("cpt_mixed_factor: invalid sizes, strides or pointers (the factor must be 256-byte aligned)");
+        return CPT_ERR_ARG;
+    }
+    if (factor_bytes < FACTOR_HEADER + mixed_factor_bytes(n, batch)) {
+        set_error("cpt_mixed_factor: factor buffer smaller than cpt_mixed_factor_size");
+        return CPT_ERR_ARG;
+    }
+    if (n > 24576) {
+        set_error("cpt_mixed_factor: n > 24576 is not supported");
+        return CPT_ERR_UNSUPPORTED;
+    }
+    CPT_CHECK_HIP(hipSetDevice(h->device), "hipSetDevice");
+    CallScope call_scope(&h->timer, h->gemm_tiles);
+    if (h->lu_work.reserve(mixed_factor_ws_bytes(n, batch))) {
+        set_error("cpt_mixed_factor: workspace allocation failed");
+        return CPT_ERR_ALLOC;
+    }
+    if (ensure_lu_streams(h)) {
+        set_error("cpt_mixed_factor: could not create the look-ahead streams: " + std::string(cpt_last_error()));
+        return CPT_ERR_HIP;
+    }
+    const MixedFactor F = mixed_factor_at((char*)factor + FACTOR_HEADER, n, batch);
+    hipLaunchKernelGGL(factor_header_kernel, dim3(1), dim3(64), 0, stream, (int*)factor, FACTOR_MAGIC, n, batch,
+                       h->gemm_tiles);
+    CPT_CHECK_HIP(hipGetLastError(), "factor header");
+    CPT_CHECK_HIP((hipError_t)mixed_factor((const cplx*)K, n, lda, strideK, batch, F, h->lu_work.p, &h->lu_streams,
+                                           stream),
+                  "cpt_mixed_factor");
+    return CPT_OK;
+}
+
+int cpt_mixed_refine(cpt_handle h, const void* K, int n, int64_t lda, int64_t strideK, int batch, void* factor,
+                     size_t factor_bytes, const void* B, int nrhs, int64_t ldb, int64_t strideB, void* X, int64_t ldx,
+                     int64_t strideX, int max_iter, double tol, int* iterations, double* residual, hipStream_t stream)
+{
+    if (!h || n < 1 || batch < 1 || nrhs < 0 || lda < n || ldb < n || ldx < n || max_iter < 0 || !(tol >= 0) ||
+        (batch > 1 && (strideK < lda * n || strideB < ldb * nrhs || strideX < ldx * nrhs)) || !K || !factor ||
+        ((uintptr_t)factor & 255) != 0) {
+        set_error("cpt_mixed_refine: invalid sizes, strides, pointers, max_iter or tol");
+        return CPT_ERR_ARG;
+    }
+    if (!iterations || !residual) {
+        set_error("cpt_mixed_refine: null output pointer");
+        return CPT_ERR_ARG;
+    }
+    *iterations = 0;
+    *residual = 0.0;
+    if (factor_bytes < FACTOR_HEADER + mixed_factor_bytes(n, batch)) {
+        set_error("cpt_mixed_refine: factor buffer smaller than cpt_mixed_factor_size");
+        return CPT_ERR_ARG;
+    }
+    if (nrhs == 0) return CPT_OK;
+    if (!B || !X) {
+        set_error("cpt_mixed_refine: null pointer argument");
+        return CPT_ERR_ARG;
+    }
+    CPT_CHECK_HIP(hipSetDevice(h->device), "hipSetDevice");
+    CallScope call_scope(&h->timer, h->gemm_tiles);
+    int head[4];
+    CPT_CHECK_HIP(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    CPT_CHECK_HIP(hipMemcpy(head, factor, sizeof head, hipMemcpyDeviceToHost), "reading the factor header");
+    if (head[0] != FACTOR_MAGIC || head[1] != n || head[2] != batch) {
+        set_error("cpt_mixed_refine: the factor buffer does not hold a cpt_mixed_factor of this order and batch");
+        return CPT_ERR_ARG;
+    }
+    const MixedGroup g{(const cplx*)K, n, lda, strideK, batch, (const cplx*)B, nrhs, ldb, strideB, (cplx*)X, ldx,
+                       strideX, stream};
+    const size_t wb = mixed_factor_ws_bytes(n, batch);
+    void* work = mixed_workspace(h, 256 + wb + mixed_refine_bytes(&g, 1));
+    if (!work) {
+        set_error("cpt_mixed_refine: workspace allocation failed");
+        return CPT_ERR_ALLOC;
+    }
+    if (ensure_lu_streams(h)) {
+        set_error("cpt_mixed_refine: could not create the look-ahead streams: " + std::string(cpt_last_error()));
+        return CPT_ERR_HIP;
+    }
+    char* base = (char*)(((uintptr_t)work + 255) & ~(uintptr_t)255);
+    const MixedFactor F = mixed_factor_at((char*)factor + FACTOR_HEADER, n, batch);
+    int tiles = head[3];
+    const int status = mixed_solve_one(h, g, F, base, base + wb, false, &tiles, max_iter, tol, iterations, residual,
+                                       "cpt_mixed_refine");
+    if (tiles != head[3]) {
+        // factored again with FP32-input products: later refinements start there
+        hipLaunchKernelGGL(factor_header_kernel, dim3(1), dim3(64), 0, stream, (int*)factor, FACTOR_MAGIC, n, batch,
+                           tiles);
+        CPT_CHECK_HIP(hipGetLastError(), "factor header");
+    }
+    return status;
 }
 
 }  // extern "C"
@@ -1070,12 +1162,14 @@ int cpt_set_tuning(cpt_handle h, int param, int value)
         h->lu_streams.leaf = value;
         return CPT_OK;
     case CPT_TUNE_LOOKAHEAD:
-        if (value < 0 || value > 2) break;
+        if (value == 2) {
+            // CU-masked streams: refused by the runtime on MI355X, and a
+            // process creating one from a partial mask crashed inside it
+            set_error("cpt_set_tuning: look-ahead mode 2 (CU-masked streams) is not supported");
+            return CPT_ERR_UNSUPPORTED;
+        }
+        if (value < 0 || value > 1) break;
         h->lu_streams.lookahead = value;
-        return CPT_OK;
-    case CPT_TUNE_PANEL_CUS:
-        if (value < 1 || value > 128) break;
-        h->lu_streams.panel_cus = value;
         return CPT_OK;
     case CPT_TUNE_GEMM_TILES:
         if (value < 0 || value > 4) break;
@@ -1108,16 +1202,11 @@ int cpt_get_tuning(cpt_handle h, int param, int* value)
     case CPT_TUNE_LU_BLOCK: *value = h->lu_streams.block; return CPT_OK;
     case CPT_TUNE_PANEL_LEAF: *value = h->lu_streams.leaf; return CPT_OK;
     case CPT_TUNE_LOOKAHEAD: *value = h->lu_streams.lookahead; return CPT_OK;
-    case CPT_TUNE_PANEL_CUS: *value = h->lu_streams.panel_cus; return CPT_OK;
     case CPT_TUNE_GEMM_TILES: *value = h->gemm_tiles; return CPT_OK;
-    case CPT_TUNE_LOOKAHEAD_ACTIVE:
-        // mode 2 runs as mode 1 when its CU-masked streams could not be made
-        *value = (h->lu_streams.lookahead == 2 && (h->part_failed || !h->lu_streams.part_main))
-                     ? (h->part_failed ? 1 : 2)
-                     : h->lu_streams.lookahead;
-        return CPT_OK;
+    case CPT_TUNE_LOOKAHEAD_ACTIVE: *value = h->lu_streams.lookahead; return CPT_OK;
     case CPT_TUNE_ASSEMBLY_CHUNK: *value = h->asm_chunk; return CPT_OK;
     case CPT_TUNE_PANEL_THREADS: *value = h->lu_streams.panel_threads; return CPT_OK;
+    case CPT_TUNE_MIXED_RETRIES: *value = (int)std::min<long>(h->mixed_retries, 0x7fffffff); return CPT_OK;
     default:
         set_error("cpt_get_tuning: unknown parameter " + std::to_string(param));
         return CPT_ERR_ARG;
@@ -1233,7 +1322,8 @@ int cpt_lu_solve_mixed_grouped(cpt_handle h, int ngroups, const cpt_lu_group* gr
     LuSetup tune = h->lu_streams;
     tune.side = nullptr;
     tune.lookahead = 0;
-    const int e = launch_mixed_grouped(g.data(), ngroups, work, &tune, max_iter, tol, iterations, residual);
+    std::vector<int> info(ngroups, 0);
+    const int e = launch_mixed_grouped(g.data(), ngroups, work, &tune, max_iter, tol, iterations, residual, info.data());
     if (join_lanes(h, stream)) {
         set_error("cpt_lu_solve_mixed_grouped: could not join the lane streams");
         return CPT_ERR_HIP;
@@ -1251,7 +1341,7 @@ int cpt_lu_solve_mixed_grouped(cpt_handle h, int ngroups, const cpt_lu_group* gr
             }
         if (!again.empty()) {
             CallScope fp32_scope(&h->timer, 3);
-            std::vector<int> it2(again.size());
+            std::vector<int> it2(again.size()), info2(again.size());
             std::vector<double> res2(again.size());
             void* work2 = mixed_workspace(h, mixed_grouped_bytes(again.data(), (int)again.size()));
             if (!work2) {
@@ -1262,8 +1352,8 @@ int cpt_lu_solve_mixed_grouped(cpt_handle h, int ngroups, const cpt_lu_group* gr
                 set_error("cpt_lu_solve_mixed_grouped: could not set up the lane streams");
                 return CPT_ERR_HIP;
             }
-            const int e2 =
-                launch_mixed_grouped(again.data(), (int)again.size(), work2, &tune, max_iter, tol, it2.data(), res2.data());
+            const int e2 = launch_mixed_grouped(again.data(), (int)again.size(), work2, &tune, max_iter, tol,
+                                                it2.data(), res2.data(), info2.data());
             if (join_lanes(h, stream)) {
                 set_error("cpt_lu_solve_mixed_grouped: could not join the lane streams");
                 return CPT_ERR_HIP;
@@ -1272,7 +1362,26 @@ int cpt_lu_solve_mixed_grouped(cpt_handle h, int ngroups, const cpt_lu_group* gr
             for (size_t k = 0; k < again.size(); ++k) {
                 iterations[idx[k]] += it2[k];
                 residual[idx[k]] = res2[k];
+                info[idx[k]] = info2[k];
             }
+            h->mixed_retries += (long)again.size();
+        }
+    }
+    for (int i = 0; i < ngroups; ++i) {
+        if (info[i] != 0 || !(residual[i] <= tol)) {
+            char buf[240];
+            if (info[i] != 0)
+                std::snprintf(buf, sizeof buf,
+                              "cpt_lu_solve_mixed_grouped: group %d (order %d): the complex64 factor is exactly "
+                              "singular (pivot %d is zero)",
+                              i, g[i].n, info[i]);
+            else
+                std::snprintf(buf, sizeof buf,
+                              "cpt_lu_solve_mixed_grouped: group %d (order %d): the refinement stopped at relative "
+                              "residual %.3e > tol %.3e after %d sweeps",
+                              i, g[i].n, residual[i], tol, iterations[i]);
+            set_error(buf);
+            return CPT_ERR_NOT_CONVERGED;
         }
     }
     return CPT_OK;

@@ -56,12 +56,7 @@ struct LuSetup {
     int block = 0;      // block width: 64, 128 or 256, 0 = by size (256 from 2048 rows, 128 from 1024)
     int leaf = 16;      // panel columns factored by one work-group per matrix
     int panel_threads = 512;  // threads of a panel leaf's work-group: 512 or 256
-    int lookahead = 1;  // 1: next panel on the side stream; 2: on CUs reserved for it
-    // CU-partitioned look-ahead: panels on `part_side` (panel_cus CUs), the
-    // rest of the factorisation on `part_main` (the other CUs)
-    int panel_cus = 16;
-    hipStream_t part_main = nullptr, part_side = nullptr;
-    hipEvent_t enter = nullptr, leave = nullptr;
+    int lookahead = 1;  // 1: next panel on the side stream, 0: in order
 };
 
 // One group of the grouped (variable-size) LU: `batch` matrices of order n,
@@ -78,6 +73,24 @@ struct MixedGroup {
     int64_t ldx, strideX;
     hipStream_t s;
 };
+// Complex64 factor of the mixed solve (complex64 values as interleaved
+// float pairs): LU factors (batch n^2, leading dimension n), pivots and info,
+// inverted 64 x 64 diagonal blocks (n x 64 per matrix).
+struct MixedFactor {
+    void* K32;
+    int* ipiv;
+    int* info;
+    void* D;
+};
+size_t mixed_factor_bytes(int n, int batch);
+MixedFactor mixed_factor_at(void* base, int n, int batch);
+size_t mixed_factor_ws_bytes(int n, int batch);
+size_t mixed_refine_bytes(const MixedGroup* g, int ng);
+int mixed_factor(const cplx* K, int n, int64_t lda, int64_t strideK, int batch, const MixedFactor& F, void* ws,
+                 const LuSetup* tune, hipStream_t s);
+int mixed_refine(const MixedGroup* g, const MixedFactor* F, int ng, void* work, int max_iter, double tol,
+                 int* iterations, double* residual, int* info_out);
+
 struct LuGroupArgs {
     cplx* A;
     int n, batch;

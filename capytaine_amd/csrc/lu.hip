@@ -1652,19 +1652,7 @@ int getrf(cx<R>* A, int N, int64_t lda, int64_t strideA, int* ipiv, int64_t stri
           cx<R>* work, const LuSetup* aux, hipStream_t stream)
 {
     if (N <= 0 || batch <= 0) return 0;
-    if (!aux || aux->lookahead != 2 || !aux->part_main) return getrf_steps<R>(A, N, lda, strideA, ipiv, strideP, info,
-                                                                             batch, work, aux, stream);
-    // CU-partitioned look-ahead: the factorisation runs on the two masked
-    // streams, ordered after the caller's stream, which then waits for it
-    LuSetup part = *aux;
-    part.side = aux->part_side;
-    if (hipEventRecord(aux->enter, stream) != hipSuccess) return (int)hipErrorUnknown;
-    if (hipStreamWaitEvent(aux->part_main, aux->enter, 0) != hipSuccess) return (int)hipErrorUnknown;
-    if (hipStreamWaitEvent(aux->part_side, aux->enter, 0) != hipSuccess) return (int)hipErrorUnknown;
-    const int e = getrf_steps<R>(A, N, lda, strideA, ipiv, strideP, info, batch, work, &part, aux->part_main);
-    if (hipEventRecord(aux->leave, aux->part_main) != hipSuccess) return (int)hipErrorUnknown;
-    if (hipStreamWaitEvent(stream, aux->leave, 0) != hipSuccess) return (int)hipErrorUnknown;
-    return e;
+    return getrf_steps<R>(A, N, lda, strideA, ipiv, strideP, info, batch, work, aux, stream);
 }
 
 template <typename R>
@@ -1848,58 +1836,103 @@ int launch_getrs(const cplx* LU, int N, int64_t lda, int64_t strideA, const int*
                          strideB, batch, stream);
 }
 
-// Workspace of the mixed solve, in bytes: complex64 factors (batch n^2),
-// pivots and info, FP64 residuals (batch n nrhs), complex64 corrections,
-// the complex64 getrf workspace, and the norm buffers.
-// Workspace of one group of the mixed solve: complex64 copy of K, pivots,
-// info, FP64 residuals, complex64 corrections (256-byte aligned parts).
+// ---------------------------------------------------------------------------
+// Mixed precision.  A complex64 factor of `batch` matrices of order n lives
+// in one buffer: the LU factors (batch n^2, column-major, leading dimension
+// n), the pivots (batch n) and info (batch), and the inverted 64 x 64
+// diagonal blocks of L and U (a strip of n x PB per matrix), each part
+// 256-byte aligned.  The refinement takes the factor and its own workspace:
+// FP64 residuals and complex64 corrections (batch n nrhs each) and the
+// squared column norms.
+// ---------------------------------------------------------------------------
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-size_t mixed_group_bytes(int n, int nrhs, int batch)
+size_t mixed_factor_bytes(int n, int batch)
 {
     const size_t b = (size_t)batch;
-    size_t bytes = align256(sizeof(cx<float>) * b * n * n);
-    bytes += align256(sizeof(int) * (b * n + b));
-    bytes += align256(sizeof(cx<double>) * b * n * nrhs);
-    bytes += align256(sizeof(cx<float>) * b * n * nrhs);
-    bytes += align256(sizeof(cx<float>) * getrf_ws_elems<float>(n, batch));
-    return bytes;
+    return align256(sizeof(cx<float>) * b * n * n) + align256(sizeof(int) * (b * n + b)) +
+           align256(sizeof(cx<float>) * b * n * PB);
 }
 
-size_t mixed_workspace_bytes(int n, int nrhs, int batch)
+MixedFactor mixed_factor_at(void* base, int n, int batch)
 {
-    return mixed_group_bytes(n, nrhs, batch) + align256(2 * sizeof(double) * (size_t)batch * nrhs) + 256;
+    const size_t b = (size_t)batch;
+    char* p = (char*)base;
+    MixedFactor F;
+    F.K32 = p;
+    p += align256(sizeof(cx<float>) * b * n * n);
+    F.ipiv = (int*)p;
+    F.info = F.ipiv + b * n;
+    p += align256(sizeof(int) * (b * n + b));
+    F.D = p;
+    return F;
 }
 
-size_t mixed_grouped_bytes(const MixedGroup* g, int ng)
+size_t mixed_factor_ws_bytes(int n, int batch) { return align256(sizeof(cx<float>) * getrf_ws_elems<float>(n, batch)); }
+
+size_t mixed_refine_bytes(const MixedGroup* g, int ng)
 {
-    size_t bytes = 256, norms = 0;
+    size_t bytes = 0, norms = 0;
     for (int i = 0; i < ng; ++i) {
-        bytes += mixed_group_bytes(g[i].n, g[i].nrhs, g[i].batch);
+        const size_t br = (size_t)g[i].batch * g[i].n * g[i].nrhs;
+        bytes += align256(sizeof(cx<double>) * br) + align256(sizeof(cx<float>) * br);
         norms += (size_t)g[i].batch * g[i].nrhs;
     }
     return bytes + align256(2 * sizeof(double) * norms);
 }
 
-// Mixed-precision solve of K X = B for groups of complex128 systems (one
-// order per group), each group on its own stream: K is rounded to
-// complex64 and factorised on FP32 MFMA; X is refined with FP64 residuals
-// r = B - K X (FP64 MFMA product against the unrounded K) and complex64
-// corrections until max_j ||r_j|| / ||b_j|| <= tol or max_iter sweeps, per
+// Workspace of factor and refinement of groups factored inside one call:
+// per group its factor and getrf workspace, then the refinement's.
+size_t mixed_grouped_bytes(const MixedGroup* g, int ng)
+{
+    size_t bytes = 256;
+    for (int i = 0; i < ng; ++i) bytes += mixed_factor_bytes(g[i].n, g[i].batch) + mixed_factor_ws_bytes(g[i].n, g[i].batch);
+    return bytes + mixed_refine_bytes(g, ng);
+}
+
+size_t mixed_workspace_bytes(int n, int nrhs, int batch)
+{
+    MixedGroup g{};
+    g.n = n;
+    g.nrhs = nrhs;
+    g.batch = batch;
+    return mixed_grouped_bytes(&g, 1);
+}
+
+// Complex64 factor of K on stream s: K32 = K rounded, partial-pivoting LU
+// on FP32 MFMA (or the product form of the calling thread), inverted
+// diagonal blocks for the substitutions.  `ws`: mixed_factor_ws_bytes.
+int mixed_factor(const cplx* K, int n, int64_t lda, int64_t strideK, int batch, const MixedFactor& F, void* ws,
+                 const LuSetup* tune, hipStream_t s)
+{
+    if (n <= 0 || batch <= 0) return 0;
+    typedef cx<float> c64;
+    const int64_t sK32 = (int64_t)n * n;
+    const dim3 blk(256), gK((n + 255) / 256, n, batch);
+    hipLaunchKernelGGL(z2c_kernel, gK, blk, 0, s, (const cx<double>*)K, lda, strideK, (c64*)F.K32, (int64_t)n, sK32,
+                       n, n);
+    hipLaunchKernelGGL(zero_ints_kernel, dim3((batch + 255) / 256), blk, 0, s, F.info, batch);
+    int e;
+    if ((e = getrf<float>((c64*)F.K32, n, n, sK32, F.ipiv, n, F.info, batch, (c64*)ws, tune, s))) return e;
+    return invert_diag<float>((const c64*)F.K32, n, n, sK32, (c64*)F.D, n, (int64_t)n * PB, 0, batch, s);
+}
+
+// Refinement of K X = B for groups of complex128 systems with their complex64
+// factors, each group on its own stream: x0 = LU32 \ b, then sweeps
+// r = b - K x (FP64 MFMA product against the unrounded K) and
+// x += LU32 \ r until max_j ||r_j|| / ||b_j|| <= tol or max_iter sweeps, per
 // group.  The host waits for all streams once per sweep to read the
-// residual norms of every group at once.
-int launch_mixed_grouped(const MixedGroup* g, int ng, void* work, const LuSetup* tune, int max_iter, double tol,
-                         int* iterations, double* residual)
+// residual norms of every group at once; at the first wait it also reads
+// each factor's info: info_out[i] = first nonzero info of group i's
+// matrices (1 + column of an exactly-zero complex64 pivot), else 0.
+int mixed_refine(const MixedGroup* g, const MixedFactor* F, int ng, void* work, int max_iter, double tol,
+                 int* iterations, double* residual, int* info_out)
 {
     typedef cx<float> c64;
     typedef cx<double> c128;
     struct State {
-        c64* K32;
-        int* ipiv;
-        int* info;
         c128* Rz;
         c64* D;
-        c64* W;  // getrf workspace
         double* nb2;
         double* nr2;
         size_t norm_off;
@@ -1909,20 +1942,13 @@ int launch_mixed_grouped(const MixedGroup* g, int ng, void* work, const LuSetup*
     char* p = (char*)(((uintptr_t)work + 255) & ~(uintptr_t)255);
     size_t nnorm = 0;
     for (int i = 0; i < ng; ++i) {
-        const size_t b = (size_t)g[i].batch, n = (size_t)g[i].n, r = (size_t)g[i].nrhs;
-        st[i].K32 = (c64*)p;
-        p += align256(sizeof(c64) * b * n * n);
-        st[i].ipiv = (int*)p;
-        st[i].info = st[i].ipiv + b * n;
-        p += align256(sizeof(int) * (b * n + b));
+        const size_t br = (size_t)g[i].batch * g[i].n * g[i].nrhs;
         st[i].Rz = (c128*)p;
-        p += align256(sizeof(c128) * b * n * r);
+        p += align256(sizeof(c128) * br);
         st[i].D = (c64*)p;
-        p += align256(sizeof(c64) * b * n * r);
-        st[i].W = (c64*)p;
-        p += align256(sizeof(c64) * getrf_ws_elems<float>((int)n, (int)b));
+        p += align256(sizeof(c64) * br);
         st[i].norm_off = nnorm;
-        nnorm += b * r;
+        nnorm += (size_t)g[i].batch * g[i].nrhs;
     }
     double* nb2_all = (double*)p;
     double* nr2_all = nb2_all + nnorm;
@@ -1937,28 +1963,24 @@ int launch_mixed_grouped(const MixedGroup* g, int ng, void* work, const LuSetup*
         S.active = G.n > 0 && G.batch > 0 && G.nrhs > 0;
         iterations[i] = 0;
         residual[i] = 0.0;
+        info_out[i] = 0;
         if (!S.active) continue;
         const int n = G.n, r = G.nrhs, batch = G.batch;
         const int64_t sK32 = (int64_t)n * n, sRD = (int64_t)n * r;
-        const dim3 gK((n + 255) / 256, n, batch), gB((n + 255) / 256, r, batch);
-        hipLaunchKernelGGL(z2c_kernel, gK, blk, 0, G.s, (const c128*)G.K, G.lda, G.strideK, S.K32, (int64_t)n, sK32,
-                           n, n);
-        hipLaunchKernelGGL(zero_ints_kernel, dim3((batch + 255) / 256), blk, 0, G.s, S.info, batch);
-        if ((e = getrf<float>(S.K32, n, n, sK32, S.ipiv, n, S.info, batch, S.W, tune, G.s))) return e;
-        // the complex64 factor is internal: its diagonal blocks are replaced
-        // by their inverses for the substitutions
-        if ((e = invert_diag<float>(S.K32, n, n, sK32, S.K32, n, sK32, 1, batch, G.s))) return e;
+        const dim3 gB((n + 255) / 256, r, batch);
         // x0 = LU32 \ b
         hipLaunchKernelGGL(z2c_kernel, gB, blk, 0, G.s, (const c128*)G.B, G.ldb, G.strideB, S.D, (int64_t)n, sRD, n,
                            r);
-        if ((e = getrs<float>(S.K32, n, n, sK32, S.ipiv, n, S.K32, n, sK32, 1, S.D, r, n, sRD, batch, G.s))) return e;
+        if ((e = getrs<float>((const c64*)F[i].K32, n, n, sK32, F[i].ipiv, n, (const c64*)F[i].D, n, (int64_t)n * PB,
+                              0, S.D, r, n, sRD, batch, G.s)))
+            return e;
         hipLaunchKernelGGL(c2z_add_kernel, gB, blk, 0, G.s, S.D, (int64_t)n, sRD, (c128*)G.X, G.ldx, G.strideX, n, r,
                            0);
         hipLaunchKernelGGL(colnorm2_kernel, dim3(r, batch), blk, 0, G.s, (const c128*)G.B, G.ldb, G.strideB, n, r,
                            S.nb2);
         if ((e = (int)hipGetLastError())) return e;
     }
-    bool have_b = false;
+    bool first = true;
     for (;;) {
         bool any = false;
         for (int i = 0; i < ng; ++i) {
@@ -1981,10 +2003,21 @@ int launch_mixed_grouped(const MixedGroup* g, int ng, void* work, const LuSetup*
         if (!any) break;
         for (int i = 0; i < ng; ++i)
             if (st[i].active && hipStreamSynchronize(g[i].s) != hipSuccess) return (int)hipErrorUnknown;
-        if (!have_b) {
+        if (first) {
             if (hipMemcpy(hb.data(), nb2_all, sizeof(double) * nnorm, hipMemcpyDeviceToHost) != hipSuccess)
                 return (int)hipErrorUnknown;
-            have_b = true;
+            for (int i = 0; i < ng; ++i) {
+                if (!st[i].active) continue;
+                std::vector<int> inf(g[i].batch);
+                if (hipMemcpy(inf.data(), F[i].info, sizeof(int) * g[i].batch, hipMemcpyDeviceToHost) != hipSuccess)
+                    return (int)hipErrorUnknown;
+                for (int v : inf)
+                    if (v != 0) {
+                        info_out[i] = v;
+                        break;
+                    }
+            }
+            first = false;
         }
         if (hipMemcpy(hr.data(), nr2_all, sizeof(double) * nnorm, hipMemcpyDeviceToHost) != hipSuccess)
             return (int)hipErrorUnknown;
@@ -1998,7 +2031,8 @@ int launch_mixed_grouped(const MixedGroup* g, int ng, void* work, const LuSetup*
                 worst = std::isnan(rel) ? INFINITY : std::max(worst, rel);
             }
             residual[i] = worst;
-            if (worst <= tol || iterations[i] >= max_iter) {
+            // a singular complex64 factor cannot correct anything
+            if (worst <= tol || iterations[i] >= max_iter || info_out[i] != 0) {
                 S.active = false;
                 continue;
             }
@@ -2007,7 +2041,9 @@ int launch_mixed_grouped(const MixedGroup* g, int ng, void* work, const LuSetup*
             const int64_t sK32 = (int64_t)n * n, sRD = (int64_t)n * r;
             const dim3 gB((n + 255) / 256, r, batch);
             hipLaunchKernelGGL(z2c_kernel, gB, blk, 0, G.s, S.Rz, (int64_t)n, sRD, S.D, (int64_t)n, sRD, n, r);
-            if ((e = getrs<float>(S.K32, n, n, sK32, S.ipiv, n, S.K32, n, sK32, 1, S.D, r, n, sRD, batch, G.s))) return e;
+            if ((e = getrs<float>((const c64*)F[i].K32, n, n, sK32, F[i].ipiv, n, (const c64*)F[i].D, n,
+                                  (int64_t)n * PB, 0, S.D, r, n, sRD, batch, G.s)))
+                return e;
             hipLaunchKernelGGL(c2z_add_kernel, gB, blk, 0, G.s, S.D, (int64_t)n, sRD, (c128*)G.X, G.ldx, G.strideX,
                                n, r, 1);
             if ((e = (int)hipGetLastError())) return e;
@@ -2017,12 +2053,25 @@ int launch_mixed_grouped(const MixedGroup* g, int ng, void* work, const LuSetup*
     return 0;
 }
 
-int launch_mixed_solve(const cplx* K, int n, int64_t lda, int64_t strideK, int batch, const cplx* B, int nrhs,
-                       int64_t ldb, int64_t strideB, cplx* X, int64_t ldx, int64_t strideX, int max_iter, double tol,
-                       void* work, const LuSetup* aux, int* iterations, double* residual, hipStream_t stream)
+// Factor and refine groups inside one call: the factors and getrf
+// workspaces at the start of `work` (mixed_grouped_bytes), each group's
+// factorisation on its own stream.
+int launch_mixed_grouped(const MixedGroup* g, int ng, void* work, const LuSetup* tune, int max_iter, double tol,
+                         int* iterations, double* residual, int* info_out)
 {
-    MixedGroup g{K, n, lda, strideK, batch, B, nrhs, ldb, strideB, X, ldx, strideX, stream};
-    return launch_mixed_grouped(&g, 1, work, aux, max_iter, tol, iterations, residual);
+    std::vector<MixedFactor> F(ng);
+    char* p = (char*)(((uintptr_t)work + 255) & ~(uintptr_t)255);
+    int e;
+    for (int i = 0; i < ng; ++i) {
+        F[i] = mixed_factor_at(p, g[i].n, g[i].batch);
+        p += mixed_factor_bytes(g[i].n, g[i].batch);
+        void* ws = p;
+        p += mixed_factor_ws_bytes(g[i].n, g[i].batch);
+        if (g[i].n > 0 && g[i].batch > 0 && g[i].nrhs > 0 &&
+            (e = mixed_factor(g[i].K, g[i].n, g[i].lda, g[i].strideK, g[i].batch, F[i], ws, tune, g[i].s)))
+            return e;
+    }
+    return mixed_refine(g, F.data(), ng, p, max_iter, tol, iterations, residual, info_out);
 }
 
 // One factorisation workspace per group, back to back in `work`

@@ -149,10 +149,66 @@ class LUDecomposedDeviceMatrix:
         return x
 
 
+class MixedFactor:
+    """Complex64 LU factor of a batch of complex128 matrices K (torch tensor
+    (nb, n, n), column-major blocks), in a caller-owned device buffer
+    (cpt_mixed_factor), refined against K by cpt_mixed_refine.  K must stay
+    unchanged while the factor is used."""
+
+    def __init__(self, K, stream=None):
+        import ctypes
+        import torch
+        self.K = K
+        nb, n, _ = K.shape
+        self.n, self.nb = n, nb
+        size = ctypes.c_size_t()
+        check(_lib.load().cpt_mixed_factor_size(n, nb, ctypes.byref(size)), "cpt_mixed_factor_size")
+        # torch's allocator returns 512-byte aligned blocks; the library asks 256
+        self.buf = torch.empty(size.value, dtype=torch.uint8, device=K.device)
+        self.nbytes = size.value
+        self.ctx = _lib.Context.get(K.device)
+        stream = stream_handle(K.device) if stream is None else stream
+        _lib.call_with_memory_retry(lambda: _lib.load().cpt_mixed_factor(
+            self.ctx.handle, ptr(K), n, n, n * n, nb, ptr(self.buf), self.nbytes, stream), "cpt_mixed_factor")
+
+    def refine(self, b, x, max_iter, tol, stream=None):
+        """x = K^-1 b for device tensors b, x of shape (nb, nrhs, n); returns
+        (sweeps, final relative residual).  Raises ConvergenceError when the
+        residual stays above tol."""
+        import ctypes
+        nb, nrhs, n = b.shape
+        it, res = ctypes.c_int(), ctypes.c_double()
+        stream = stream_handle(b.device) if stream is None else stream
+        try:
+            _lib.call_with_memory_retry(lambda: _lib.load().cpt_mixed_refine(
+                self.ctx.handle, ptr(self.K), n, n, n * n, nb, ptr(self.buf), self.nbytes, ptr(b), nrhs, n,
+                n * nrhs, ptr(x), n, n * nrhs, max_iter, tol, ctypes.byref(it), ctypes.byref(res), stream),
+                "cpt_mixed_refine")
+        finally:
+            self.iterations, self.residual = it.value, res.value
+        return it.value, res.value
+
+    def factors(self):
+        """Host copies (lu (nb, n, n) complex64 row-major views of the
+        column-major factors transposed back, ipiv (nb, n), info (nb)) of the
+        buffer's parts, for inspection."""
+        import numpy as np
+        n, nb = self.n, self.nb
+        raw = self.buf.cpu().numpy()
+        a256 = lambda x: (x + 255) & ~255
+        off = 256
+        lu = raw[off:off + 8 * nb * n * n].view(np.complex64).reshape(nb, n, n).transpose(0, 2, 1)
+        off += a256(8 * nb * n * n)
+        ints = raw[off:off + 4 * (nb * n + nb)].view(np.int32)
+        return lu, ints[:nb * n].reshape(nb, n), ints[nb * n:]
+
+
 class MixedPrecisionSolver:
     """Complex64 LU on FP32 MFMA with FP64 iterative refinement against the
-    FP64 matrix (cpt_lu_solve_mixed).  Keeps A; factorises at every solve
-    call (the factor lives in the library workspace)."""
+    FP64 matrix.  The factor is computed once, at the first solve, and kept
+    for every later right-hand side, as the reference keeps the LU of K
+    (engines.py:326-338).  Raises ConvergenceError when the refinement
+    stays above tol."""
 
     def __init__(self, A: DeviceMatrix, max_iter=10, tol=1e-12):
         self.A = A
@@ -160,21 +216,22 @@ class MixedPrecisionSolver:
         self.tol = tol
         self.iterations = None
         self.residual = None
+        self.factor = None
+        self.factorisations = 0
 
     shape = property(lambda self: self.A.shape)
     dtype = property(lambda self: self.A.dtype)
 
     def solve_device(self, b):
-        import ctypes
         import torch
-        nb, nrhs, n = b.shape
+        if self.factor is None:
+            self.factor = MixedFactor(self.A.data)
+            self.factorisations += 1
         x = torch.empty_like(b)
-        it, res = ctypes.c_int(), ctypes.c_double()
-        ctx = _lib.Context.get(b.device)
-        check(_lib.load().cpt_lu_solve_mixed(ctx.handle, ptr(self.A.data), n, n, n * n, nb, ptr(b), nrhs, n, n * nrhs,
-                                             ptr(x), n, n * nrhs, self.max_iter, self.tol, ctypes.byref(it),
-                                             ctypes.byref(res), stream_handle(b.device)), "cpt_lu_solve_mixed")
-        self.iterations, self.residual = it.value, res.value
+        try:
+            self.factor.refine(b, x, self.max_iter, self.tol)
+        finally:
+            self.iterations, self.residual = self.factor.iterations, self.factor.residual
         return x
 
     def solve(self, b):

@@ -6,8 +6,10 @@ problems already on the device, ``FrequencySweep.run`` does what capytaine's
 _solve :145-215 per problem), batched:
 
   1. one assembly launch for every wavenumber (cpt_build_matrices),
-  2. one batched LU of the K matrices, in place (cpt_lu_factor),
-  3. one batched solve for all right-hand sides (cpt_lu_solve),
+  2. one batched LU of the K matrices, in place (cpt_lu_factor), or their
+     complex64 factors (cpt_mixed_factor) in the mixed form,
+  3. one batched solve for all right-hand sides (cpt_lu_solve), or the FP64
+     refinement with those factors (cpt_mixed_refine),
   4. potentials Phi = S @ sources for all of them (cpt_zgemm),
   5. generalised forces W^T Phi with W = -(dof . n) A (cpt_zgemm), to be
      scaled by i omega rho per frequency (bodies.py:238-250).
@@ -75,9 +77,12 @@ class FrequencySweep:
             check(lib.cpt_set_assembly_precision(self.ctx.handle, 0), "assembly precision")
 
     def factor(self):
-        """FP64: LU of K in place.  Mixed: nothing here (the complex64 factor
-        is computed inside the mixed solve, K is kept for the residuals)."""
+        """FP64: LU of K in place.  Mixed: complex64 factor of K
+        (cpt_mixed_factor) into a buffer of its own; K is kept for the FP64
+        residuals of the refinement."""
         if self.precision == "mixed":
+            from .engines import MixedFactor
+            self.mixed_factor = MixedFactor(self.K, stream=self.stream)
             return
         n = self.n
         check(_lib.load().cpt_lu_factor(self.ctx.handle, ptr(self.K), n, n, n * n, ptr(self.ipiv), n,
@@ -88,12 +93,10 @@ class FrequencySweep:
         n = self.n
         lib = _lib.load()
         if self.precision == "mixed":
-            import ctypes
-            it, res = ctypes.c_int(), ctypes.c_double()
-            check(lib.cpt_lu_solve_mixed(self.ctx.handle, ptr(self.K), n, n, n * n, self.nb, ptr(bc), self.nrhs, n,
-                                         n * self.nrhs, ptr(self.X), n, n * self.nrhs, self.max_iter, self.tol,
-                                         ctypes.byref(it), ctypes.byref(res), self.stream), "cpt_lu_solve_mixed")
-            self.iterations, self.residual = it.value, res.value
+            try:
+                self.mixed_factor.refine(bc, self.X, self.max_iter, self.tol, stream=self.stream)
+            finally:
+                self.iterations, self.residual = self.mixed_factor.iterations, self.mixed_factor.residual
             return
         self.X.copy_(bc)
         _lib.call_with_memory_retry(lambda: lib.cpt_lu_solve(

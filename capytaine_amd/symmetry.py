@@ -256,8 +256,8 @@ class LUDecomposedSymmetricMatrix:
 class MixedSymmetricSolver:
     """The mixed-precision solve of a BlockSymmetricDeviceMatrix: right-hand
     sides transformed, the n transformed blocks solved by the complex64 LU
-    with FP64 refinement against the complex128 blocks
-    (cpt_lu_solve_mixed), solutions transformed back."""
+    with FP64 refinement against the complex128 blocks (cpt_mixed_factor once,
+    cpt_mixed_refine per call), solutions transformed back."""
 
     def __init__(self, A, max_iter=10, tol=1e-12):
         from .engines import DeviceMatrix, MixedPrecisionSolver

@@ -36,7 +36,8 @@ enum {
     CPT_ERR_HIP = -2,         /* HIP runtime error */
     CPT_ERR_ALLOC = -3,       /* device allocation failed */
     CPT_ERR_STATE = -4,       /* call order violated (e.g. no tabulation) */
-    CPT_ERR_UNSUPPORTED = -5  /* option of the reference not implemented here */
+    CPT_ERR_UNSUPPORTED = -5, /* option of the reference not implemented here */
+    CPT_ERR_NOT_CONVERGED = -6 /* iterative solve left above its tolerance (outputs still written) */
 };
 
 /* Values of the integer options, equal to the reference's Fortran
@@ -174,15 +175,46 @@ int cpt_lu_solve(cpt_handle h, const void* LU, int n, int64_t lda, int64_t strid
  * max_j ||r_j||_2 / ||b_j||_2 <= tol or max_iter refinement sweeps.  K and B
  * are not modified.  Outputs the sweeps used and the final relative
  * residual.  Synchronises `stream` once per sweep.  With FP16-input tile
- * products (CPT_TUNE_GEMM_TILES 4), a solve left short of tol is done again
- * with FP32-input products; the sweeps of both are reported (the grouped
- * variant does the same per group).
+ * products (CPT_TUNE_GEMM_TILES 4), a solve left short of tol is factored
+ * and refined again with FP32-input products; the sweeps of both are
+ * reported and the context's CPT_TUNE_MIXED_RETRIES count grows by one.
+ * Returns CPT_ERR_NOT_CONVERGED when the final residual exceeds tol (NaN
+ * included) or when the complex64 factor has an exactly-zero pivot; X,
+ * iterations and residual are written in that case too.  tol must be >= 0.
  * Replaces, with the north-star's mixed-precision scheme, the factor + solve
  * of DefaultMatrixEngine.linear_solver (src/capytaine/bem/engines.py:326-338,
  * LAPACK zgetrf/zgetrs through block_circulant_matrices.py:374-385). */
 int cpt_lu_solve_mixed(cpt_handle h, const void* K, int n, int64_t lda, int64_t strideK, int batch, const void* B,
                        int nrhs, int64_t ldb, int64_t strideB, void* X, int64_t ldx, int64_t strideX, int max_iter,
                        double tol, int* iterations, double* residual, hipStream_t stream);
+
+/* The two halves of cpt_lu_solve_mixed, for a factor reused across calls,
+ * as the reference keeps the LU of K in place of K and solves each later
+ * right-hand side with it (src/capytaine/bem/engines.py:326-338).
+ *
+ * cpt_mixed_factor_size: bytes of the factor buffer of `batch` matrices of
+ * order n.  Layout (for inspection; offsets from the buffer): a 256-byte
+ * header (int32 magic, n, batch, product form), then the complex64 LU
+ * factors (batch x n x n, column-major, leading dimension n, L unit lower
+ * and U upper as LAPACK cgetrf leaves them), then the pivots (batch x n
+ * int32, 0-based) and info (batch int32), then the inverted 64 x 64 diagonal
+ * blocks (n x 64 complex64 per matrix); each part starts on a 256-byte
+ * boundary.
+ * cpt_mixed_factor: factor K (complex128, read) into `factor` (caller-owned
+ * device memory, 256-byte aligned, at least cpt_mixed_factor_size bytes),
+ * asynchronously on `stream`.
+ * cpt_mixed_refine: the refinement of cpt_lu_solve_mixed with that factor
+ * (same K, unchanged since), same outputs, tolerance and errors; when
+ * FP16-input products leave it short of tol, the factor is recomputed in
+ * place with FP32-input products and later refinements use it.  Reads the
+ * factor's header (synchronises `stream` first); CPT_ERR_ARG if the buffer
+ * holds no factor of this n and batch. */
+int cpt_mixed_factor_size(int n, int batch, size_t* bytes);
+int cpt_mixed_factor(cpt_handle h, const void* K, int n, int64_t lda, int64_t strideK, int batch, void* factor,
+                     size_t factor_bytes, hipStream_t stream);
+int cpt_mixed_refine(cpt_handle h, const void* K, int n, int64_t lda, int64_t strideK, int batch, void* factor,
+                     size_t factor_bytes, const void* B, int nrhs, int64_t ldb, int64_t strideB, void* X, int64_t ldx,
+                     int64_t strideX, int max_iter, double tol, int* iterations, double* residual, hipStream_t stream);
 
 /* One group of the grouped (variable-size) LU: `batch` matrices of order n
  * with the strides of cpt_lu_factor / cpt_lu_solve / cpt_lu_solve_mixed.
@@ -206,7 +238,10 @@ typedef struct {
  * the context's lane streams (longest first), ordered after the work
  * already on `stream`, which waits for all of them.  The mixed form
  * synchronises once per refinement sweep for all groups together and
- * writes iterations[g] and residual[g] per group.  Replaces the per-problem
+ * writes iterations[g] and residual[g] per group; it retries groups with
+ * FP32-input products like cpt_lu_solve_mixed and returns
+ * CPT_ERR_NOT_CONVERGED if any group ends above tol or with a singular
+ * complex64 factor.  Replaces the per-problem
  * LAPACK calls of DefaultMatrixEngine.linear_solver
  * (src/capytaine/bem/engines.py:300-346) over the bodies and frequencies of
  * BEMSolver.solve_all (src/capytaine/bem/solver.py:268-372). */
@@ -303,12 +338,11 @@ enum {
     CPT_TUNE_LU_BLOCK = 0,       /* block width of the LU: 64, 128, 256, or 0 = by size (256 from
                                     2048 rows, 128 from 1024, else 64) */
     CPT_TUNE_PANEL_LEAF = 1,     /* panel columns factored by one work-group per matrix, 1..64 */
-    CPT_TUNE_LOOKAHEAD = 2,      /* 0: in order; 1: next panel on a side stream; 2: next panel on
-                                    CUs reserved for it (streams with CU masks, tried only when the
-                                    environment sets CAPYTAINE_AMD_CU_MASK=1; runs as 1 otherwise
-                                    or when the runtime refuses them) */
+    CPT_TUNE_LOOKAHEAD = 2,      /* 0: in order; 1: next panel on a side stream.  2 (the next panel
+                                    on CU-masked streams) returns CPT_ERR_UNSUPPORTED: the runtime
+                                    refuses such streams on MI355X */
     CPT_TUNE_ASSEMBLY_CHUNK = 3, /* panels per assembly work-group, 0 = by size, else 1..1024 */
-    CPT_TUNE_PANEL_CUS = 4,      /* CUs reserved for the panels when CPT_TUNE_LOOKAHEAD = 2, 1..128 */
+    /* 4 is unused (it was the CU count of look-ahead mode 2) */
     CPT_TUNE_GEMM_TILES = 5,     /* complex64 tiled products: 0: 128 x 64 tiles; 1: 128 x 128;
                                     2: 128 x 128, three real products per complex one (Gauss),
                                     one wavefront per SIMD; 3: the Gauss form on 128 x 64;
@@ -316,11 +350,14 @@ enum {
                                     mixed solve's FP64 refinement takes more sweeps).
                                     complex128 products always use 128 x 64, four products;
                                     default 0 */
-    CPT_TUNE_LOOKAHEAD_ACTIVE = 6, /* read only (cpt_get_tuning): the look-ahead mode the last
-                                      factorisation ran, 1 where mode 2 found no hardware queue */
-    CPT_TUNE_PANEL_THREADS = 7     /* threads of a panel leaf's work-group: 512 (default) or 256
+    CPT_TUNE_LOOKAHEAD_ACTIVE = 6, /* read only (cpt_get_tuning): the look-ahead mode factorisations
+                                      run */
+    CPT_TUNE_PANEL_THREADS = 7,    /* threads of a panel leaf's work-group: 512 (default) or 256
                                       (one wavefront per SIMD, up to 8192 rows: fits on a CU beside
                                       one work-group of the tile product) */
+    CPT_TUNE_MIXED_RETRIES = 8     /* read only: mixed solves (or groups) of this context factored
+                                      again with FP32-input products after the FP16-input products
+                                      left the refinement short of its tolerance */
 };
 int cpt_set_tuning(cpt_handle h, int param, int value);
 int cpt_get_tuning(cpt_handle h, int param, int* value);

@@ -155,7 +155,9 @@ def test_lu_on_bem_matrix(default_table):
     assert np.linalg.norm(x - xr) / np.linalg.norm(xr) < 1e-12
 
 
-def _mixed(As, Bs, max_iter=10, tol=1e-12):
+def _mixed(As, Bs, max_iter=10, tol=1e-12, allow_unconverged=False):
+    """cpt_lu_solve_mixed; with allow_unconverged, a CPT_ERR_NOT_CONVERGED
+    return still yields the outputs it wrote."""
     import ctypes
     import torch
     from capytaine_amd import _lib
@@ -167,9 +169,13 @@ def _mixed(As, Bs, max_iter=10, tol=1e-12):
     X = torch.empty_like(B)
     it, res = ctypes.c_int(), ctypes.c_double()
     ctx = _lib.Context.get(device())
-    check(_lib.load().cpt_lu_solve_mixed(ctx.handle, ptr(A), n, n, n * n, nb, ptr(B), r, n, n * r, ptr(X), n, n * r,
-                                         max_iter, tol, ctypes.byref(it), ctypes.byref(res),
-                                         stream_handle(device())), "mixed")
+    try:
+        check(_lib.load().cpt_lu_solve_mixed(ctx.handle, ptr(A), n, n, n * n, nb, ptr(B), r, n, n * r, ptr(X), n,
+                                             n * r, max_iter, tol, ctypes.byref(it), ctypes.byref(res),
+                                             stream_handle(device())), "mixed")
+    except _lib.ConvergenceError:
+        if not allow_unconverged:
+            raise
     torch.cuda.synchronize()
     # K and B are left untouched
     assert np.array_equal(np.swapaxes(A.cpu().numpy(), 1, 2), As)
@@ -186,7 +192,7 @@ def test_mixed_precision_solve(n, nb):
     Bs = np.empty((nb, n, 6), dtype=complex)
     for b in range(nb):
         As[b], Bs[b] = _random_system(rng, n, 6)
-    X0, it0, res0 = _mixed(As, Bs, max_iter=0)
+    X0, it0, res0 = _mixed(As, Bs, max_iter=0, allow_unconverged=True)
     X, it, res = _mixed(As, Bs)
     assert it <= 6 and res <= 1e-12
     for b in range(nb):
@@ -279,7 +285,7 @@ def test_kernel_timing():
 
 
 @pytest.mark.parametrize("block,leaf,lookahead,tiles", [(64, 8, 0, 1), (64, 32, 1, 0), (128, 16, 1, 1),
-                                                        (128, 64, 0, 0), (128, 16, 2, 1), (64, 8, 2, 0),
+                                                        (128, 64, 0, 0), (128, 16, 0, 1), (64, 8, 1, 0),
                                                         (256, 16, 0, 0), (256, 8, 1, 0), (256, 16, 1, 2),
                                                         (128, 16, 1, 3), (256, 8, 0, 3), (256, 16, 1, 4),
                                                         (64, 8, 0, 4)])
@@ -303,10 +309,12 @@ def test_lu_tunings_agree(block, leaf, lookahead, tiles):
         As, Bs = np.array(As), np.array(Bs)
         X, info, _, _ = _factor_solve(As, Bs)
         assert not info.any()
-        # mode 2 runs as mode 1 where the device has no queue for CU masks
-        assert _lib.lookahead_active(ctx) in ((1, 2) if lookahead == 2 else (lookahead,))
+        assert _lib.lookahead_active(ctx) == lookahead
+        retries = _lib.mixed_retries(ctx)
         Xm, _, res = _mixed(As, Bs)
         assert res <= 1e-12
+        # the product form under test did the work: no FP32-input retry
+        assert _lib.mixed_retries(ctx) == retries
         for b in range(3):
             Xr = lu_solve(lu_factor(As[b]), Bs[b])
             assert np.linalg.norm(X[b] - Xr) / np.linalg.norm(Xr) < 1e-10
@@ -401,7 +409,9 @@ def test_mixed_fp16_products_fall_back_to_fp32():
     As, Bs = np.array(As), np.array(Bs)
     try:
         _lib.set_tuning(ctx, gemm_tiles=4)
+        retries = _lib.mixed_retries(ctx)
         X, it, res = _mixed(As, Bs, max_iter=1)
+        assert _lib.mixed_retries(ctx) == retries + 1
     finally:
         _lib.set_tuning(ctx, **saved)
     assert res <= 1e-12 and it == 2
@@ -414,12 +424,17 @@ def test_tuning_rejects_bad_values():
     from capytaine_amd import _lib
     ctx = _lib.Context.get(device())
     for name, bad in [("lu_block", 96), ("lu_block", 512), ("panel_leaf", 0), ("panel_leaf", 65), ("lookahead", 3),
-                      ("assembly_chunk", -1), ("panel_cus", 0), ("panel_cus", 129), ("gemm_tiles", 5),
-                      ("panel_threads", 128), ("panel_threads", 1024), ("lookahead_active", 1)]:
+                      ("assembly_chunk", -1), ("gemm_tiles", 5), ("panel_threads", 128), ("panel_threads", 1024),
+                      ("lookahead_active", 1), ("mixed_retries", 0)]:
         with pytest.raises(_lib.CptError):
             _lib.set_tuning(ctx, **{name: bad})
-    with pytest.raises(_lib.CptError):
-        _lib.check(_lib.load().cpt_set_tuning(ctx.handle, 99, 0), "cpt_set_tuning")
+    for param in (99, 4):  # unknown, and the retired CU count of look-ahead mode 2
+        with pytest.raises(_lib.CptError):
+            _lib.check(_lib.load().cpt_set_tuning(ctx.handle, param, 16), "cpt_set_tuning")
+    # look-ahead on CU-masked streams is refused, not attempted
+    with pytest.raises(_lib.CptError) as err:
+        _lib.set_tuning(ctx, lookahead=2)
+    assert err.value.status == _lib.CPT_ERR_UNSUPPORTED
 
 
 @pytest.mark.parametrize("n,block,leaf", [(640, 128, 64), (768, 256, 64), (300, 256, 32)])
@@ -442,3 +457,130 @@ def test_lu_last_panel_wider_than_leaf_kernel(n, block, leaf):
         assert np.linalg.norm(X[0] - Xr) / np.linalg.norm(Xr) < 1e-10
     finally:
         _lib.set_tuning(ctx, **saved)
+
+
+def test_mixed_unconverged_is_an_error():
+    """A refinement left above tol returns CPT_ERR_NOT_CONVERGED (raised as
+    ConvergenceError) with its outputs written: max_iter 0 and tol 1e-14,
+    single and grouped entry points, and the FP16-input retry path."""
+    import ctypes
+    import torch
+    from capytaine_amd import _lib
+    from capytaine_amd._lib import check, ptr, stream_handle
+    print("[gpu] mixed solve left above tol", flush=True)
+    rng = np.random.default_rng(31)
+    n = 300
+    As, Bs = zip(*[_random_system(rng, n, 3) for _ in range(2)])
+    As, Bs = np.array(As), np.array(Bs)
+    ctx = _lib.Context.get(device())
+    saved = _lib.get_tuning(ctx)
+    try:
+        for tiles in (0, 4):
+            _lib.set_tuning(ctx, gemm_tiles=tiles)
+            with pytest.raises(_lib.ConvergenceError, match="residual"):
+                _mixed(As, Bs, max_iter=0, tol=1e-14)
+            X, it, res = _mixed(As, Bs, max_iter=0, tol=1e-14, allow_unconverged=True)
+            assert it == 0 and 1e-14 < res < 1e-3
+        # grouped: one group converges, the other gets no sweep
+        A1, B1 = _dev(np.swapaxes(As, 1, 2)), _dev(np.swapaxes(Bs, 1, 2))
+        X1 = torch.empty_like(B1)
+        g = (_lib.CptLuGroup * 1)()
+        g[0] = _lib.CptLuGroup(n, 2, A1.data_ptr(), n, n * n, None, 0, None, B1.data_ptr(), 3, n, n * 3,
+                               X1.data_ptr(), n, n * 3)
+        its, res = np.zeros(1, np.int32), np.zeros(1)
+        _lib.set_tuning(ctx, gemm_tiles=0)
+        with pytest.raises(_lib.ConvergenceError, match="group 0"):
+            check(_lib.load().cpt_lu_solve_mixed_grouped(ctx.handle, 1, g, 0, 1e-14, ptr(its), ptr(res),
+                                                         stream_handle(device())), "grouped")
+        assert its[0] == 0 and res[0] > 1e-14
+        # bad tolerances are argument errors
+        for tol in (float("nan"), -1.0):
+            it, r = ctypes.c_int(), ctypes.c_double()
+            assert _lib.load().cpt_lu_solve_mixed(ctx.handle, ptr(A1), n, n, n * n, 2, ptr(B1), 3, n, n * 3,
+                                                  ptr(X1), n, n * 3, 5, tol, ctypes.byref(it), ctypes.byref(r),
+                                                  stream_handle(device())) == -1
+    finally:
+        _lib.set_tuning(ctx, **saved)
+
+
+def test_mixed_singular_factor_is_an_error():
+    """An exactly singular matrix: the complex64 factor has a zero pivot and
+    the mixed solve reports it instead of returning a meaningless X."""
+    from capytaine_amd import _lib
+    n = 40
+    A = np.eye(n, dtype=complex)
+    A[:, 5] = 0.0
+    with pytest.raises(_lib.ConvergenceError, match="singular"):
+        _mixed(A[None], np.ones((1, n, 2), dtype=complex))
+
+
+@pytest.mark.parametrize("n", [517, 518, 519])
+@pytest.mark.parametrize("tiles", [0, 1, 2, 3, 4])
+def test_complex64_factor_direct(n, tiles):
+    """The complex64 factor itself (cpt_mixed_factor, read back from its
+    buffer) for every tile product: ||P A - L U|| / ||A|| at the unit
+    round-off of the product's inputs (FP32: 6e-8, FP16 inputs: 4.9e-4)
+    times a small factor, with row counts 1, 2 and 3 mod 4 so that the
+    vector and per-row epilogues of the products run; pivots equal to the
+    FP64 factor's on this diagonally weighted matrix."""
+    from scipy.linalg import lu_factor
+    from capytaine_amd import _lib
+    from capytaine_amd.engines import MixedFactor
+    print(f"[gpu] complex64 factor n={n} tiles={tiles}", flush=True)
+    rng = np.random.default_rng(n * 10 + tiles)
+    A, _ = _random_system(rng, n, 1)
+    ctx = _lib.Context.get(device())
+    saved = _lib.get_tuning(ctx)
+    try:
+        _lib.set_tuning(ctx, lu_block=128, panel_leaf=16, lookahead=1, gemm_tiles=tiles)
+        F = MixedFactor(_dev(A.T[None]))
+        lu, ipiv, info = F.factors()
+    finally:
+        _lib.set_tuning(ctx, **saved)
+    assert info[0] == 0
+    lu = lu[0].astype(np.complex128)
+    L = np.tril(lu, -1) + np.eye(n)
+    U = np.triu(lu)
+    PA = A.copy()
+    for i, p in enumerate(ipiv[0]):  # LAPACK order of interchanges
+        PA[[i, p]] = PA[[p, i]]
+    err = np.linalg.norm(PA - L @ U) / np.linalg.norm(A)
+    bound = 50 * 4.9e-4 / np.sqrt(n) if tiles == 4 else 50 * 6e-8
+    print(f"[gpu]   ||PA - LU|| / ||A|| = {err:.2e} (bound {bound:.1e})", flush=True)
+    assert err < bound
+    assert np.array_equal(ipiv[0], lu_factor(A)[1])
+
+
+def test_mixed_factor_reused_across_solves(default_table):
+    """BEMSolver-style use: the engine's mixed solver factors K once and each
+    of 14 right-hand sides solved one call at a time reuses the factor
+    (one factorisation's panel launches in total), every solution at the
+    FP64 engine's."""
+    from capytaine_amd import DefaultMatrixEngine, mesh_sphere, _lib
+    from gpu_helpers import ref_table_gf
+    mesh = mesh_sphere(radius=1.0, resolution=(20, 20)).immersed_part()
+    gf = ref_table_gf(default_table)
+    params = dict(free_surface=0.0, water_depth=np.inf, wavenumber=1.3, adjoint_double_layer=True,
+                  diagonal_term_in_double_layer=True)
+    rng = np.random.default_rng(2)
+    bs = [rng.normal(size=mesh.nb_faces) + 1j * rng.normal(size=mesh.nb_faces) for _ in range(14)]
+    e64 = DefaultMatrixEngine(green_function=gf)
+    _, K = e64.build_matrices(mesh, mesh, **params)
+    x64 = [e64.linear_solver(e64.last_computed_matrices[1], b) for b in bs]
+    emx = DefaultMatrixEngine(green_function=gf, linear_solver="mixed_precision")
+    ctx = _lib.Context.get(device())
+    _lib.profile_begin(ctx)
+    xs = []
+    for b in bs:
+        _, K = emx.build_matrices(mesh, mesh, **params)  # cache hit after the first
+        xs.append(emx.linear_solver(K, b))
+    prof = _lib.profile_end(ctx)
+    solver = emx.last_computed_matrices[1]
+    assert solver.factorisations == 1
+    _lib.profile_begin(ctx)
+    from capytaine_amd.engines import MixedFactor
+    MixedFactor(solver.A.data)
+    one = _lib.profile_end(ctx)
+    assert prof["panel_c"]["launches"] == one["panel_c"]["launches"] > 0
+    for x, xr in zip(xs, x64):
+        assert np.linalg.norm(x - xr) / np.linalg.norm(xr) < 1e-11
