@@ -48,6 +48,7 @@ SIGNATURES = {
     "cpt_set_tuning": (_I, [_P, _I, _I]),
     "cpt_get_tuning": (_I, [_P, _I, ctypes.POINTER(ctypes.c_int)]),
     "cpt_block_transform": (_I, [_P, _I, _I, _I, _I, _I, _I, _P, _L, _L, _L, _P, _L, _L, _L, _P]),
+    "cpt_lu_solve_noalloc": (_I, [_P, _P, _I, _L, _L, _P, _L, _P, _I, _L, _L, _I, _P]),
     "cpt_mixed_factor_size": (_I, [_I, _I, ctypes.POINTER(ctypes.c_size_t)]),
     "cpt_mixed_factor": (_I, [_P, _P, _I, _L, _L, _I, _P, ctypes.c_size_t, _P]),
     "cpt_mixed_refine": (_I, [_P, _P, _I, _L, _L, _I, _P, ctypes.c_size_t, _P, _I, _L, _L, _P, _L, _L, _I, _D,

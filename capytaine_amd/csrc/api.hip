@@ -8,10 +8,18 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 #include "cpt_internal.hpp"
 #include "../../include/capytaine_amd.h"
+
+// Scratch of cpt_lu_solve_noalloc: the inverted 64 x 64 diagonal blocks of
+// the factors (a strip of n x 64 complex128 per matrix) for at most
+// CPT_LU_SOLVE_NOALLOC_MAX_ROWS rows over the batch: 32 MiB, held by every
+// device the library runs on until the process ends.  Its device address is
+// taken once per device (scratch_address) and only that address is used.
+static __device__ __align__(256) double scratch[2 * 64 * (size_t)CPT_LU_SOLVE_NOALLOC_MAX_ROWS];
 
 namespace cpt {
 
@@ -39,6 +47,34 @@ int block_transform_max_blocks();
 
 static thread_local std::string g_last_error;
 void set_error(const std::string& msg) { g_last_error = msg; }
+
+// Device address of `scratch` per device, looked up once with that device
+// current.
+constexpr int MAX_DEVICES = 64;
+static void* g_scratch_addr[MAX_DEVICES];
+static std::mutex g_scratch_mutex;
+
+int scratch_init(int device)
+{
+    if (device < 0 || device >= MAX_DEVICES) {
+        set_error("scratch_init: device index beyond " + std::to_string(MAX_DEVICES));
+        return -1;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_addr[device]) return 0;
+    int prev = 0;
+    if (hipGetDevice(&prev) != hipSuccess) prev = device;
+    hipError_t e = hipSetDevice(device);
+    void* p = nullptr;
+    if (e == hipSuccess) e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    (void)hipSetDevice(prev);
+    if (e != hipSuccess || !p) {
+        set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        return -1;
+    }
+    g_scratch_addr[device] = p;
+    return 0;
+}
 
 thread_local KernelTimer* active_timer = nullptr;
 thread_local int active_gemm_tiles = 0;
@@ -416,6 +452,7 @@ int cpt_create(int device, cpt_handle* out)
         set_error("cpt_create: invalid device index " + std::to_string(device));
         return CPT_ERR_ARG;
     }
+    if (scratch_init(device)) return CPT_ERR_HIP;
     cpt_context* h = new cpt_context();
     h->device = device;
     *out = h;
@@ -893,6 +930,39 @@ int cpt_lu_solve(cpt_handle h, const void* LU, int n, int64_t lda, int64_t strid
     }
     CPT_CHECK_HIP((hipError_t)launch_getrs((const cplx*)LU, n, lda, strideA, ipiv, strideP, (cplx*)B, nrhs, ldb,
                                            strideB, batch, (cplx*)h->solve_work.p, stream),
+                  "getrs launch");
+    return CPT_OK;
+}
+
+int cpt_lu_solve_noalloc(cpt_handle h, const void* LU, int n, int64_t lda, int64_t strideA, const int* ipiv,
+                         int64_t strideP, void* B, int nrhs, int64_t ldb, int64_t strideB, int batch,
+                         hipStream_t stream)
+{
+    if (!h || n < 0 || nrhs < 0 || batch < 0 || lda < n || ldb < n ||
+        (batch > 1 && (strideA < lda * n || strideP < n || strideB < ldb * nrhs))) {
+        set_error("cpt_lu_solve_noalloc: invalid sizes or strides");
+        return CPT_ERR_ARG;
+    }
+    if ((int64_t)n * batch > CPT_LU_SOLVE_NOALLOC_MAX_ROWS) {
+        set_error("cpt_lu_solve_noalloc: n x batch = " + std::to_string((int64_t)n * batch) +
+                  " exceeds the static scratch (" + std::to_string(CPT_LU_SOLVE_NOALLOC_MAX_ROWS) +
+                  " rows); use cpt_lu_solve");
+        return CPT_ERR_ARG;
+    }
+    if (n == 0 || nrhs == 0 || batch == 0) return CPT_OK;
+    if (!LU || !ipiv || !B) {
+        set_error("cpt_lu_solve_noalloc: null pointer argument");
+        return CPT_ERR_ARG;
+    }
+    void* ptr = (h->device >= 0 && h->device < MAX_DEVICES) ? g_scratch_addr[h->device] : nullptr;
+    if (!ptr) {
+        set_error("cpt_lu_solve_noalloc: no scratch address for this device");
+        return CPT_ERR_STATE;
+    }
+    CPT_CHECK_HIP(hipSetDevice(h->device), "hipSetDevice");
+    CallScope call_scope(nullptr, h->gemm_tiles);  // no timing events
+    CPT_CHECK_HIP((hipError_t)launch_getrs((const cplx*)LU, n, lda, strideA, ipiv, strideP, (cplx*)B, nrhs, ldb,
+                                           strideB, batch, (cplx*)ptr, stream),
                   "getrs launch");
     return CPT_OK;
 }

@@ -168,6 +168,20 @@ int cpt_lu_factor(cpt_handle h, void* A, int n, int64_t lda, int64_t strideA, in
 int cpt_lu_solve(cpt_handle h, const void* LU, int n, int64_t lda, int64_t strideA, const int* ipiv,
                  int64_t strideP, void* B, int nrhs, int64_t ldb, int64_t strideB, int batch, hipStream_t stream);
 
+/* cpt_lu_solve without any device allocation: the inverted diagonal blocks
+ * go to a static device array of the library (32 MiB per device, held until
+ * the process ends) instead of the context's workspace.  No counterpart in
+ * the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * n x batch must not exceed CPT_LU_SOLVE_NOALLOC_MAX_ROWS (CPT_ERR_ARG
+ * otherwise).  Every call on one device shares that array: calls that
+ * overlap on one device (other streams, other threads, other contexts) race
+ * and must be ordered by the caller.  Same results as cpt_lu_solve. */
+#define CPT_LU_SOLVE_NOALLOC_MAX_ROWS 32768
+int cpt_lu_solve_noalloc(cpt_handle h, const void* LU, int n, int64_t lda, int64_t strideA, const int* ipiv,
+                         int64_t strideP, void* B, int nrhs, int64_t ldb, int64_t strideB, int batch,
+                         hipStream_t stream);
+
 /* Mixed-precision solve of K X = B for `batch` complex128 systems with nrhs
  * right-hand sides each: K is rounded to complex64 and factorised with FP32
  * MFMA trailing updates; X is then refined with FP64 residuals r = B - K X
