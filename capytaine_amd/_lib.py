@@ -58,7 +58,7 @@ SIGNATURES = {
 # Tuning parameters of cpt_set_tuning, in the order of the CPT_TUNE_* values
 # (None: a retired value).
 TUNING_PARAMS = ("lu_block", "panel_leaf", "lookahead", "assembly_chunk", None, "gemm_tiles",
-                 "lookahead_active", "panel_threads", "mixed_retries")
+                 "lookahead_active", "panel_threads", "mixed_retries", "panel_wgs", "panel_spin")
 READ_ONLY_TUNING = ("lookahead_active", "mixed_retries")
 
 # Kernel classes of cpt_profile_end, in the order of the CPT_TIMED_* values.

@@ -397,6 +397,10 @@ int mixed_solve_one(cpt_context* h, const MixedGroup& g, const MixedFactor& F, v
         ++h->mixed_retries;
         *iterations += sweeps;
     }
+    if (info < 0) {
+        set_error(std::string(who) + ": the complex64 panel factorisation gave up waiting for its work-groups");
+        return CPT_ERR_NOT_CONVERGED;
+    }
     if (info != 0) {
         set_error(std::string(who) + ": the complex64 factor is exactly singular (pivot " + std::to_string(info) +
                   " is zero); relative residual " + std::to_string(*residual));
@@ -1253,6 +1257,13 @@ int cpt_set_tuning(cpt_handle h, int param, int value)
         if (value != 256 && value != 512) break;
         h->lu_streams.panel_threads = value;
         return CPT_OK;
+    case CPT_TUNE_PANEL_WGS:
+        if (value != 0 && value != 1) break;
+        h->lu_streams.panel_mw = value;
+        return CPT_OK;
+    case CPT_TUNE_PANEL_SPIN:
+        h->lu_streams.panel_spin = value;
+        return CPT_OK;
     default:
         set_error("cpt_set_tuning: unknown parameter " + std::to_string(param));
         return CPT_ERR_ARG;
@@ -1276,6 +1287,8 @@ int cpt_get_tuning(cpt_handle h, int param, int* value)
     case CPT_TUNE_LOOKAHEAD_ACTIVE: *value = h->lu_streams.lookahead; return CPT_OK;
     case CPT_TUNE_ASSEMBLY_CHUNK: *value = h->asm_chunk; return CPT_OK;
     case CPT_TUNE_PANEL_THREADS: *value = h->lu_streams.panel_threads; return CPT_OK;
+    case CPT_TUNE_PANEL_WGS: *value = h->lu_streams.panel_mw; return CPT_OK;
+    case CPT_TUNE_PANEL_SPIN: *value = h->lu_streams.panel_spin; return CPT_OK;
     case CPT_TUNE_MIXED_RETRIES: *value = (int)std::min<long>(h->mixed_retries, 0x7fffffff); return CPT_OK;
     default:
         set_error("cpt_get_tuning: unknown parameter " + std::to_string(param));
@@ -1314,6 +1327,8 @@ int cpt_lu_factor_grouped(cpt_handle h, int ngroups, const cpt_lu_group* groups,
     LuSetup tune = h->lu_streams;
     tune.side = nullptr;  // no look-ahead inside a lane: the lanes overlap instead
     tune.lookahead = 0;
+    // up to NUM_LANES multi-work-group leaves at once must fit the device together
+    tune.mw_budget = 256 / NUM_LANES;
     const int e = launch_getrf_lanes(g.data(), ngroups, &tune, h->lu_work.p);
     if (join_lanes(h, stream)) {
         set_error("cpt_lu_factor_grouped: could not join the lane streams");
@@ -1392,6 +1407,7 @@ int cpt_lu_solve_mixed_grouped(cpt_handle h, int ngroups, const cpt_lu_group* gr
     LuSetup tune = h->lu_streams;
     tune.side = nullptr;
     tune.lookahead = 0;
+    tune.mw_budget = 256 / NUM_LANES;
     std::vector<int> info(ngroups, 0);
     const int e = launch_mixed_grouped(g.data(), ngroups, work, &tune, max_iter, tol, iterations, residual, info.data());
     if (join_lanes(h, stream)) {

@@ -56,6 +56,9 @@ struct LuSetup {
     int block = 0;      // block width: 64, 128 or 256, 0 = by size (256 from 2048 rows, 128 from 1024)
     int leaf = 16;      // panel columns factored by one work-group per matrix
     int panel_threads = 512;  // threads of a panel leaf's work-group: 512 or 256
+    int panel_mw = 1;         // leaves over several work-groups per matrix where they fit
+    int panel_spin = 0;       // polls a multi-work-group leaf waits per column, 0: default; < 0: give up at once
+    int mw_budget = 256;      // work-groups of one multi-work-group leaf launch over the batch
     int lookahead = 1;  // 1: next panel on the side stream, 0: in order
 };
 

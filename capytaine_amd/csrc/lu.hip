@@ -27,8 +27,10 @@
 // The solve inverts the factor's 64 x 64 diagonal blocks once and runs one
 // substitution launch per block and direction (MFMA products only).
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <climits>
 #include <cmath>
+#include <utility>
 #include <vector>
 #include "cpt_internal.hpp"
 
@@ -422,14 +424,15 @@ __global__ void __launch_bounds__(NT) panel_kernel(cx<R>* __restrict__ Abase, in
             const int cols_per_chunk = (NT * PPT) / nslot;
             for (int cb = 0; cb < ncol; cb += cols_per_chunk) {
                 const int npairs = min(cols_per_chunk, ncol - cb) * nslot;
-                C val[PPT];
-                int dst[PPT];
-                C* colp[PPT];
+                // values as scalars and destinations as offsets (-1: none),
+                // all in registers
+                R vre[PPT], vim[PPT];
+                int64_t dst[PPT];
 #pragma unroll
                 for (int u = 0; u < PPT; ++u) {
                     const int pair = tid + NT * u;
-                    colp[u] = nullptr;
-                    dst[u] = 0;
+                    vre[u] = vim[u] = R(0);
+                    dst[u] = -1;
                     if (pair < npairs) {
                         const int col = sw_lo + cb + pair / nslot, j = pair % nslot;
                         if (col < c0 || col >= c0 + sbw) {
@@ -439,20 +442,391 @@ __global__ void __launch_bounds__(NT) panel_kernel(cx<R>* __restrict__ Abase, in
                                 if (x == t0 + s) x = sPiv[s];
                                 else if (x == sPiv[s]) x = t0 + s;
                             }
-                            colp[u] = A + k0 + (size_t)col * lda;
-                            dst[u] = r;
-                            val[u] = colp[u][x];
+                            const int64_t colo = k0 + (int64_t)col * lda;
+                            const C val = A[colo + x];
+                            vre[u] = val.re;
+                            vim[u] = val.im;
+                            dst[u] = colo + r;
                         }
                     }
                 }
                 __syncthreads();
 #pragma unroll
                 for (int u = 0; u < PPT; ++u)
-                    if (colp[u]) colp[u][dst[u]] = val[u];
+                    if (dst[u] >= 0) A[dst[u]] = C{vre[u], vim[u]};
             }
         }
         __syncthreads();
     }
+}
+
+// ---------------------------------------------------------------------------
+// 1b. Panel leaf over several work-groups per matrix.
+//
+// P work-groups of NT threads (512 or 1024) per matrix each own a slab of NT
+// rows of the leaf, one row per thread with all W leaf columns in registers:
+// the leaf is loaded once and stored once.  Row interchanges move no data: each register slot
+// carries the (leaf-relative) row it currently holds, an interchange of rows
+// j and p swaps two labels, and the final store writes each slot to its
+// row.  Per column every work-group finds its best candidate (largest
+// |Re|+|Im|, first row on ties) and publishes it -- value, row, and the
+// row's W entries -- in an exchange record; wave 0 of every work-group then
+// reads all P records of its matrix, picks the same winner as every other
+// work-group and stages the pivot row in LDS.  Each 64-bit word of a record
+// holds 32 bits of data and, in its high half, the column's sequence number:
+// a record is complete when every word shows the current sequence, so one
+// round of loads suffices and no fence is needed (agent-scope atomic loads
+// and stores, coherent across the XCDs).  Records alternate between two
+// parities: a work-group writes the record of column j + 2 only after it has
+// read every record of column j + 1, which the other work-groups write only
+// after reading those of column j.  Sequence numbers are unique within a
+// factorisation, whose start clears the records.  The wait is bounded (spin
+// polls; a negative spin gives up at once, a test hook): a work-group that
+// gives up sets the matrix's info to -1 and returns, so that every
+// work-group ends.  Register indices stay static: the leaf's columns rotate
+// through the slots, the pivot column always in slot 0.
+// ---------------------------------------------------------------------------
+constexpr int MW_PMAX = 32;    // work-groups per matrix
+constexpr int MW_HDR = 4;      // header words: best (1 or 2 words), row, padding
+constexpr int MW_ROWW = 64;    // words of one row of W entries (2 W for float pairs, 4 W for double pairs)
+constexpr int MW_REC = 72;     // words per record: header, candidate row, padding
+constexpr int MW_XWORDS = 2 * MW_PMAX * MW_REC;  // words per matrix: two parities of MW_PMAX records
+constexpr int MW_LEAF_BUDGET = 256;  // work-groups of one leaf launch over the batch
+
+template <typename R>
+constexpr int mw_width() { return sizeof(R) == 4 ? 32 : 16; }
+
+// v[s] <- v[s + 1], v[W - 1] <- v[0], as straight assignments (a loop of
+// them would be recognised as a memory move and keep v out of registers).
+template <typename T, int W, int... S>
+__device__ __forceinline__ void rotate_slots(T (&v)[W], std::integer_sequence<int, S...>)
+{
+    const T f = v[0];
+    ((v[S] = v[S + 1]), ...);
+    v[W - 1] = f;
+}
+
+__device__ __forceinline__ void mw_store(uint64_t* p, uint32_t seq, uint32_t bits)
+{
+    __hip_atomic_store(p, ((uint64_t)seq << 32) | bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t mw_load(const uint64_t* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// A real as record words (low halves) and back.
+template <typename R>
+struct MwWords;
+template <>
+struct MwWords<float> {
+    static constexpr int N = 1;
+    static __device__ __forceinline__ void put(uint64_t* p, uint32_t seq, float v) { mw_store(p, seq, __float_as_uint(v)); }
+    static __device__ __forceinline__ float get(const uint64_t* w) { return __uint_as_float((uint32_t)w[0]); }
+};
+template <>
+struct MwWords<double> {
+    static constexpr int N = 2;
+    static __device__ __forceinline__ void put(uint64_t* p, uint32_t seq, double v)
+    {
+        mw_store(p, seq, (uint32_t)__double2loint(v));
+        mw_store(p + 1, seq, (uint32_t)__double2hiint(v));
+    }
+    static __device__ __forceinline__ double get(const uint64_t* w)
+    {
+        return __hiloint2double((int)(uint32_t)w[1], (int)(uint32_t)w[0]);
+    }
+};
+
+template <typename R, int W, int MW_NT>
+__global__ void __launch_bounds__(MW_NT) panel_mw_kernel(cx<R>* __restrict__ Abase, int64_t lda, int64_t strideA,
+                                                         int N, int k0, int kb, int sw_lo, int sw_hi,
+                                                         int* __restrict__ ipiv_base, int64_t strideP,
+                                                         int* __restrict__ info_base, uint64_t* __restrict__ xbase,
+                                                         int P, uint32_t seq0, int spin)
+{
+    typedef cx<R> C;
+    typedef MwWords<R> MW;
+    constexpr int NV = MW::N;
+    constexpr int PER = MW_HDR + 2 * NV * W;  // words read per record
+    static_assert(2 * NV * W <= MW_ROWW, "leaf row does not fit a record");
+    constexpr int NW = MW_NT / 64;
+    constexpr int RPT = 1;  // one row per thread: the leaf's W entries in registers
+    constexpr int Rw = MW_NT * RPT;
+    const int g = blockIdx.x, b = blockIdx.y;
+    C* A = Abase + (size_t)blockIdx.y * strideA;
+    int* ipiv = ipiv_base + (size_t)b * strideP;
+    uint64_t* X = xbase + (size_t)b * MW_XWORDS;
+    const int M = N - k0;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int row0 = g * Rw;
+    const int rend = min(M, row0 + Rw);
+
+    __shared__ C sCand[NW][W];
+    __shared__ C sP[W];
+    __shared__ R sBest[NW];
+    __shared__ int sIdx[NW];
+    __shared__ int sPiv[W];
+    __shared__ int sWin, sAbort;
+    __shared__ R sWinBest;
+
+    C v[RPT][W];
+    int lab[RPT];  // leaf row held by slot q (INT_MAX: no row)
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+        const int t = row0 + tid + MW_NT * q;
+        lab[q] = t < rend ? t : INT_MAX;
+#pragma unroll
+        for (int s = 0; s < W; ++s) {
+            if (t < rend && s < kb) v[q][s] = A[(k0 + t) + (size_t)(k0 + s) * lda];
+            else v[q][s] = C{R(0), R(0)};
+        }
+    }
+
+    // after j steps, register slot s holds leaf column (s + j) mod W: the
+    // pivot column is always slot 0, and each step rotates the slots by one,
+    // the finished column moving to the end.  The loop is not unrolled: the
+    // code of one step fits the instruction cache.
+    bool quit = false;
+    int jend = kb;  // set to the current step when giving up: one exit only
+#pragma unroll 1
+    for (int j = 0; j < jend; ++j) {
+        {
+            const uint32_t seq = seq0 + (uint32_t)j;
+            R best = R(-1);
+            int bidx = INT_MAX;
+#pragma unroll
+            for (int q = 0; q < RPT; ++q) {
+                if (lab[q] != INT_MAX && lab[q] >= j) {
+                    const R a = cabs1(v[q][0]);
+                    if (a > best || (a == best && lab[q] < bidx)) {
+                        best = a;
+                        bidx = lab[q];
+                    }
+                }
+            }
+            wave_argmax(best, bidx);
+            if (lane == 0) {
+                sBest[wave] = best;
+                sIdx[wave] = bidx;
+            }
+#pragma unroll
+            for (int q = 0; q < RPT; ++q) {
+                if (lab[q] == bidx) {
+#pragma unroll
+                    for (int s = 0; s < W; ++s) sCand[wave][s] = v[q][s];
+                }
+            }
+            __syncthreads();
+            uint64_t* recs = X + (size_t)(j & 1) * MW_PMAX * MW_REC;
+            // entries of the pivot row beyond the active columns (the
+            // finished ones, rotated to the end) enter the update as zeros
+            const int active = W - j;
+            if (wave == 0) {
+                R bg = sBest[0];
+                int pg = sIdx[0], wg = 0;
+#pragma unroll
+                for (int w = 1; w < NW; ++w) {
+                    const R bw = sBest[w];
+                    const int iw = sIdx[w];
+                    if (bw > bg || (bw == bg && iw < pg)) {
+                        bg = bw;
+                        pg = iw;
+                        wg = w;
+                    }
+                }
+                int ab = 0;
+                if (P == 1) {
+                    if (lane < W) sP[lane] = lane < active ? sCand[wg][lane] : C{R(0), R(0)};
+                    if (lane == 0) {
+                        sWin = pg;
+                        sWinBest = bg;
+                    }
+                } else {
+                    // this work-group's record: the row first, and once its
+                    // stores are complete the header, so that a header
+                    // showing the column's sequence implies a complete row
+                    uint64_t* rec = recs + (size_t)g * MW_REC;
+                    if (lane < W) {
+                        const C c = sCand[wg][lane];
+                        uint64_t* pc = rec + MW_HDR + 2 * NV * lane;
+                        MW::put(pc, seq, c.re);
+                        MW::put(pc + NV, seq, c.im);
+                    }
+                    if (lane == 0) {
+                        // release: the wave's row stores precede it
+                        __hip_atomic_store(rec + 2, ((uint64_t)seq << 32) | (uint32_t)pg, __ATOMIC_RELEASE,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+                        MW::put(rec, seq, bg);
+                        if (NV == 1) mw_store(rec + 1, seq, 0u);
+                        mw_store(rec + 3, seq, 0u);
+                    }
+                    // headers of the P records (lane r: record r)
+                    R rb = R(-1);
+                    int ri = INT_MAX;
+                    for (int it = 0;; ++it) {
+                        bool ok = true;
+                        if (lane < P && spin >= 0) {
+                            const uint64_t* h = recs + (size_t)lane * MW_REC;
+                            uint64_t hw[MW_HDR];
+#pragma unroll
+                            for (int u = 0; u < MW_HDR; ++u) hw[u] = mw_load(h + u);
+#pragma unroll
+                            for (int u = 0; u < MW_HDR; ++u) ok = ok && (uint32_t)(hw[u] >> 32) == seq;
+                            rb = MW::get(hw);
+                            ri = (int)(uint32_t)hw[2];
+                        } else if (lane < P) {
+                            ok = false;
+                        }
+                        if (__all(ok)) break;
+                        if (spin < 0 || it >= spin) {
+                            ab = 1;
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(1);
+                    }
+                    if (!ab) {
+                        // pairs with the writers' release: their rows are visible
+                        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                        // the winner over the records: rows are distinct, the
+                        // lowest record lane holding the winning row is its owner
+                        int idx = ri;
+                        R bb = rb;
+                        wave_argmax(bb, idx);
+                        const unsigned long long hit = __ballot(lane < P && ri == idx);
+                        const int rr = hit ? __builtin_ctzll(hit) : 0;
+                        // its row: lane s < W reads entry s (checked too)
+                        for (int it = 0;; ++it) {
+                            bool ok = true;
+                            if (lane < W) {
+                                const uint64_t* pc = recs + (size_t)rr * MW_REC + MW_HDR + 2 * NV * lane;
+                                uint64_t ew[2 * NV];
+#pragma unroll
+                                for (int u = 0; u < 2 * NV; ++u) ew[u] = mw_load(pc + u);
+#pragma unroll
+                                for (int u = 0; u < 2 * NV; ++u) ok = ok && (uint32_t)(ew[u] >> 32) == seq;
+                                if (ok) sP[lane] = lane < active ? C{MW::get(ew), MW::get(ew + NV)} : C{R(0), R(0)};
+                            }
+                            if (__all(ok)) break;
+                            if (it >= std::max(spin, 0)) {
+                                ab = 1;
+                                break;
+                            }
+                            __builtin_amdgcn_s_sleep(1);
+                        }
+                        if (lane == 0) {
+                            sWin = idx;
+                            sWinBest = bb;
+                        }
+                    }
+                }
+                if (lane == 0) sAbort = ab;
+            }
+            __syncthreads();
+            const int abort = sAbort;
+
+            if (abort) {  // uniform: every wave read the same verdicts
+                // the columns left unfinished get no interchange, so that the
+                // kernels after this one find valid rows in ipiv
+                if (g == 0)
+                    for (int jj = j + tid; jj < kb; jj += MW_NT) ipiv[k0 + jj] = k0 + jj;
+                if (tid == 0) info_base[b] = -1;
+                quit = true;
+                jend = j;
+            }
+            // no nonzero candidate (a zero column, or one of NaNs, which no
+            // comparison selects): no interchange, singularity reported; after
+            // giving up nothing more is written
+            const bool pivoted = !quit && sWinBest > R(0);
+            const int p = pivoted ? sWin : j;
+            if (tid == 0 && !quit) {
+                sPiv[j] = p;
+                if (g == 0) {
+                    if (!pivoted && info_base[b] == 0) info_base[b] = k0 + j + 1;
+                    ipiv[k0 + j] = k0 + p;
+                }
+            }
+            // rows j and p trade places: their labels
+#pragma unroll
+            for (int q = 0; q < RPT; ++q) {
+                if (lab[q] == p) lab[q] = j;
+                else if (lab[q] == j) lab[q] = p;
+            }
+            const C piv = pivoted ? sP[0] : C{R(0), R(0)};
+            const bool nonzero = (piv.re != R(0) || piv.im != R(0));
+            const C rcp = nonzero ? zrcp(piv) : C{R(1), R(0)};
+#pragma unroll
+            for (int q = 0; q < RPT; ++q) {
+                // rows below the pivot take l and the rank-1 update, the
+                // others a zero multiplier (no branch: one version of the
+                // registers across the loop); the pivot row is zero on the
+                // finished columns
+                // (selects of values: a select of the two operands'
+                // addresses would keep v in memory)
+                const bool below = lab[q] != INT_MAX && lab[q] > j;
+                const C cur = v[q][0];
+                const C lq = nonzero ? zmul(cur, rcp) : cur;
+                const C l{below ? lq.re : R(0), below ? lq.im : R(0)};
+                v[q][0] = C{below ? lq.re : cur.re, below ? lq.im : cur.im};
+#pragma unroll
+                for (int s = 1; s < W; ++s) v[q][s] = zsub(v[q][s], zmul(l, sP[s]));
+            }
+        }
+        // rotate: the finished column to the last slot
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) rotate_slots(v[q], std::make_integer_sequence<int, W - 1>{});
+    }
+    if (quit) return;
+    // slot s holds column (s + kb) mod W
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+        if (lab[q] != INT_MAX)
+#pragma unroll
+            for (int s = 0; s < W; ++s) {
+                const int c = (s + kb) % W;
+                if (c < kb) A[(k0 + lab[q]) + (size_t)(k0 + c) * lda] = v[q][s];
+            }
+    }
+    __syncthreads();  // sPiv complete
+
+    // the leaf's interchanges on the panel's other columns [sw_lo, sw_hi):
+    // column c goes to work-group (c - sw_lo) mod P; one thread per
+    // (column, involved row) finds, walking the interchanges backwards,
+    // which original row ends in that row, loads it, and after a barrier
+    // stores it
+    {
+        const int nslot = 2 * kb;
+        const int ncol = sw_hi - sw_lo;
+        const int mycols = ncol > g ? (ncol - g + P - 1) / P : 0;
+        const int cols_per_chunk = MW_NT / nslot;
+        for (int cb = 0; cb < mycols; cb += cols_per_chunk) {
+            C val{R(0), R(0)};
+            int64_t dst = -1;
+            const int ci = cb + tid / nslot;
+            if (tid < cols_per_chunk * nslot && ci < mycols) {
+                const int col = sw_lo + g + ci * P, js = tid % nslot;
+                if (col < k0 || col >= k0 + kb) {
+                    const int r = (js & 1) ? sPiv[js >> 1] : (js >> 1);
+                    int x = r;
+                    for (int s = kb - 1; s >= 0; --s) {
+                        if (x == s) x = sPiv[s];
+                        else if (x == sPiv[s]) x = s;
+                    }
+                    const int64_t colo = k0 + (int64_t)col * lda;
+                    val = A[colo + x];
+                    dst = colo + r;
+                }
+            }
+            __syncthreads();
+            if (dst >= 0) A[dst] = val;
+        }
+    }
+}
+
+__global__ void zero_words_kernel(uint64_t* __restrict__ p, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -1434,37 +1808,104 @@ void launch_panel(cx<R>* A, int64_t lda, int64_t sA, int N, int k0, int kb, int 
                        sw_hi, ipiv, sP, info);
 }
 
+// Panel settings of one factorisation: single-work-group leaf width and
+// threads, and for the multi-work-group leaves the exchange records (null:
+// single-work-group leaves only), the work-groups one launch may take over
+// the batch, the spin bound of their waits and the next sequence number.
+struct PanelCfg {
+    int leaf = 16, nt = 512;
+    bool mw = true;  // multi-work-group leaves where they fit
+    uint64_t* xbuf = nullptr;
+    int budget = MW_LEAF_BUDGET;
+    int spin = 1 << 20;
+    uint32_t seq = 1;
+};
+
+// Threads per work-group (512 or 1024, one row each), work-groups per
+// matrix and matrices per launch of a multi-work-group leaf of M rows: one
+// launch of at most `budget` work-groups when the batch fits (512 threads
+// up to 16 work-groups per matrix, else 1024 up to 32), else the batch in
+// chunks of budget / P matrices with 1024 threads; nt = 0 when M exceeds
+// 32 x 1024 rows.
+inline void mw_shape(int M, int batch, const PanelCfg& pc, int& P, int& nt, int& per_launch)
+{
+    const int pmax = std::max(1, std::min(MW_PMAX, pc.budget / std::max(batch, 1)));
+    if (M <= 512 * std::min(pmax, 16)) nt = 512;
+    else if (M <= 1024 * pmax) nt = 1024;
+    else nt = M <= 1024 * MW_PMAX ? 1024 : 0;
+    P = nt ? (M + nt - 1) / nt : 0;
+    per_launch = P ? std::max(1, std::min(batch, pc.budget / P)) : 0;
+}
+
+template <typename R, int NT>
+void launch_panel_mw(cx<R>* A, int64_t lda, int64_t sA, int N, int k0, int kb, int sw_lo, int sw_hi, int* ipiv,
+                     int64_t sP, int* info, int batch, int P, const PanelCfg& pc, uint32_t seq, hipStream_t stream)
+{
+    hipLaunchKernelGGL((panel_mw_kernel<R, mw_width<R>(), NT>), dim3(P, batch), dim3(NT), 0, stream, A, lda, sA, N,
+                       k0, kb, sw_lo, sw_hi, ipiv, sP, info, pc.xbuf, P, seq, pc.spin);
+}
+
+// Factor columns [k0, k0 + kb), kb <= mw_width, of rows [k0, N) over
+// several work-groups per matrix; sets `done` when the leaf fits them.
+template <typename R>
+int panel_leaf_mw(cx<R>* A, int64_t lda, int64_t sA, int N, int k0, int kb, int sw_lo, int sw_hi, int* ipiv,
+                  int64_t sP, int* info, int batch, PanelCfg& pc, bool& done, hipStream_t stream)
+{
+    done = false;
+    if (!pc.xbuf || kb > mw_width<R>()) return 0;
+    int P, nt, per;
+    mw_shape(N - k0, batch, pc, P, nt, per);
+    if (!nt) return 0;
+    const uint32_t seq = pc.seq;
+    pc.seq += 64;
+    PanelCfg part = pc;
+    for (int b0 = 0; b0 < batch; b0 += per) {
+        const int nb = std::min(per, batch - b0);
+        cx<R>* Ab = A + (size_t)b0 * sA;
+        int* pb = ipiv + (size_t)b0 * sP;
+        int* ib = info + b0;
+        part.xbuf = pc.xbuf + (size_t)b0 * MW_XWORDS;
+        if (nt == 512) launch_panel_mw<R, 512>(Ab, lda, sA, N, k0, kb, sw_lo, sw_hi, pb, sP, ib, nb, P, part, seq, stream);
+        else launch_panel_mw<R, 1024>(Ab, lda, sA, N, k0, kb, sw_lo, sw_hi, pb, sP, ib, nb, P, part, seq, stream);
+    }
+    done = true;
+    return (int)hipGetLastError();
+}
+
 // Factor columns [k0, k0 + kb) of rows [k0, N) with one work-group per
 // matrix; interchanges are applied to the columns [sw_lo, sw_hi).
 template <typename R>
 int panel_leaf(cx<R>* A, int64_t lda, int64_t sA, int N, int k0, int kb, int sw_lo, int sw_hi, int* ipiv, int64_t sP,
                int* info, int batch, int nt, hipStream_t stream)
 {
-    // 512 threads (two wavefronts per SIMD, up to 256 VGPRs each); the
-    // register-resident sub-block holds RPT rows x SB columns per thread,
-    // RPT * SB <= 32 complex values up to 8192 rows.  256 threads (one
-    // wavefront per SIMD) up to 8192 rows: a work-group fits on a CU beside
-    // one work-group of the tile product, so a leaf factored while a large
-    // product runs starts when one of the product's work-groups ends
-    // rather than when a whole CU has drained.
+    // 512 threads (two wavefronts per SIMD, up to 256 VGPRs each) or 256
+    // (one wavefront per SIMD, up to 512); the register-resident sub-block
+    // holds RPT rows x SB columns per thread.  A 256-thread work-group fits
+    // on a CU beside one work-group of the tile product (the tuning option
+    // CPT_TUNE_PANEL_THREADS).
     const int M = N - k0;
-    if (nt == 256 && M <= 8192) {
+    // every instantiation launched here keeps its rows in registers without
+    // scratch: 512 threads up to 2048 rows (complex128) or 8192 (complex64),
+    // 256 threads up to 8192 rows; taller leaves are factored by several
+    // work-groups per matrix
+    constexpr bool dbl = sizeof(R) == 8;
+    if (nt == 256 || M > (dbl ? 2048 : 8192)) {
         if (M <= 256) launch_panel<R, 256, 1, 8>(A, lda, sA, N, k0, kb, sw_lo, sw_hi, ipiv, sP, info, batch, stream);
         else if (M <= 512) launch_panel<R, 256, 2, 8>(A, lda, sA, N, k0, kb, sw_lo, sw_hi, ipiv, sP, info, batch, stream);
         else if (M <= 1024) launch_panel<R, 256, 4, 8>(A, lda, sA, N, k0, kb, sw_lo, sw_hi, ipiv, sP, info, batch, stream);
         else if (M <= 2048) launch_panel<R, 256, 8, 4>(A, lda, sA, N, k0, kb, sw_lo, sw_hi, ipiv, sP, info, batch, stream);
         else if (M <= 4096) launch_panel<R, 256, 16, 2>(A, lda, sA, N, k0, kb, sw_lo, sw_hi, ipiv, sP, info, batch, stream);
-        else launch_panel<R, 256, 32, 1>(A, lda, sA, N, k0, kb, sw_lo, sw_hi, ipiv, sP, info, batch, stream);
+        else if (M <= 8192) launch_panel<R, 256, 32, 1>(A, lda, sA, N, k0, kb, sw_lo, sw_hi, ipiv, sP, info, batch, stream);
+        else return (int)hipErrorInvalidValue;  // taller leaves go to the multi-work-group kernel
         return (int)hipGetLastError();
     }
     if (M <= 512) launch_panel<R, 512, 1, 8>(A, lda, sA, N, k0, kb, sw_lo, sw_hi, ipiv, sP, info, batch, stream);
     else if (M <= 1024) launch_panel<R, 512, 2, 8>(A, lda, sA, N, k0, kb, sw_lo, sw_hi, ipiv, sP, info, batch, stream);
     else if (M <= 2048) launch_panel<R, 512, 4, 8>(A, lda, sA, N, k0, kb, sw_lo, sw_hi, ipiv, sP, info, batch, stream);
-    else if (M <= 4096) launch_panel<R, 512, 8, 4>(A, lda, sA, N, k0, kb, sw_lo, sw_hi, ipiv, sP, info, batch, stream);
-    else if (M <= 8192) launch_panel<R, 512, 16, 2>(A, lda, sA, N, k0, kb, sw_lo, sw_hi, ipiv, sP, info, batch, stream);
-    else if (M <= 16384) launch_panel<R, 512, 32, 1>(A, lda, sA, N, k0, kb, sw_lo, sw_hi, ipiv, sP, info, batch, stream);
-    else if (M <= 24576) launch_panel<R, 512, 48, 1>(A, lda, sA, N, k0, kb, sw_lo, sw_hi, ipiv, sP, info, batch, stream);
-    else return (int)hipErrorInvalidValue;
+    else if constexpr (!dbl) {
+        if (M <= 4096) launch_panel<R, 512, 8, 4>(A, lda, sA, N, k0, kb, sw_lo, sw_hi, ipiv, sP, info, batch, stream);
+        else launch_panel<R, 512, 16, 2>(A, lda, sA, N, k0, kb, sw_lo, sw_hi, ipiv, sP, info, batch, stream);
+    }
     return (int)hipGetLastError();
 }
 
@@ -1500,15 +1941,23 @@ int trsm_lower(const cx<R>* L, int64_t lda, int64_t sA, int kb, cx<R>* B, int64_
 // half.  The one-work-group leaves stream only `leaf` columns.
 template <typename R>
 int panel_rec(cx<R>* A, int64_t lda, int64_t sA, int N, int c0, int w, int sw_lo, int sw_hi, int* ipiv, int64_t sP,
-              int* info, int batch, int leaf, int nt, hipStream_t stream)
+              int* info, int batch, PanelCfg& pc, hipStream_t stream)
 {
     const int M = N - c0;
-    // the leaf kernel holds at most PB columns (its LDS triangle)
-    if (w <= PB && (w <= leaf || M <= 2 * leaf))
-        return panel_leaf<R>(A, lda, sA, N, c0, w, sw_lo, sw_hi, ipiv, sP, info, batch, nt, stream);
-    const int h = w / 2;
     int e;
-    if ((e = panel_rec<R>(A, lda, sA, N, c0, h, sw_lo, sw_hi, ipiv, sP, info, batch, leaf, nt, stream))) return e;
+    // multi-work-group leaves where enabled, and always above the 8192 rows
+    // the one-work-group kernel keeps in registers
+    if (w <= mw_width<R>() && pc.xbuf && (pc.mw || M > 8192)) {
+        bool done;
+        if ((e = panel_leaf_mw<R>(A, lda, sA, N, c0, w, sw_lo, sw_hi, ipiv, sP, info, batch, pc, done, stream)) || done)
+            return e;
+    }
+    // the one-work-group leaf kernel holds at most PB columns (its LDS
+    // triangle) and 8192 rows; taller panels split down to the other's width
+    if (w <= PB && (w <= pc.leaf || M <= 2 * pc.leaf) && M <= 8192)
+        return panel_leaf<R>(A, lda, sA, N, c0, w, sw_lo, sw_hi, ipiv, sP, info, batch, pc.nt, stream);
+    const int h = w / 2;
+    if ((e = panel_rec<R>(A, lda, sA, N, c0, h, sw_lo, sw_hi, ipiv, sP, info, batch, pc, stream))) return e;
     cx<R>* A11 = A + c0 + (size_t)c0 * lda;
     cx<R>* A12 = A + c0 + (size_t)(c0 + h) * lda;
     cx<R>* A21 = A + (c0 + h) + (size_t)c0 * lda;
@@ -1516,20 +1965,20 @@ int panel_rec(cx<R>* A, int64_t lda, int64_t sA, int N, int c0, int w, int sw_lo
     if ((e = trsm_lower<R>(A11, lda, sA, h, A12, lda, sA, w - h, batch, stream))) return e;
     if (M > h && (e = gemm<R>(M - h, w - h, h, R(-1), A21, lda, sA, A12, lda, sA, A22, lda, sA, 0, batch, stream)))
         return e;
-    return panel_rec<R>(A, lda, sA, N, c0 + h, w - h, sw_lo, sw_hi, ipiv, sP, info, batch, leaf, nt, stream);
+    return panel_rec<R>(A, lda, sA, N, c0 + h, w - h, sw_lo, sw_hi, ipiv, sP, info, batch, pc, stream);
 }
 
 // LU with partial pivoting of the panel columns [k0, k0 + kb), rows [k0, N);
 // interchanges applied within the panel columns only.
 template <typename R>
 int panel_dispatch(cx<R>* A, int64_t lda, int64_t sA, int N, int k0, int kb, int* ipiv, int64_t sP, int* info,
-                   int batch, int leaf, int nt, hipStream_t stream)
+                   int batch, PanelCfg& pc, hipStream_t stream)
 {
     const int M = N - k0;
     // complex LU of an M x kb panel: 4 (M kb^2 - kb^3 / 3) real flops
     TimedLaunch timed(stream, sizeof(R) == 8 ? TK_PANEL_Z : TK_PANEL_C,
                       4.0 * ((double)M * kb * kb - (double)kb * kb * kb / 3.0) * batch);
-    return panel_rec<R>(A, lda, sA, N, k0, kb, k0, k0 + kb, ipiv, sP, info, batch, leaf, nt, stream);
+    return panel_rec<R>(A, lda, sA, N, k0, kb, k0, k0 + kb, ipiv, sP, info, batch, pc, stream);
 }
 
 template <typename R>
@@ -1628,11 +2077,18 @@ int trsm_lower(const cx<R>* L, int64_t lda, int64_t sA, int kb, cx<R>* B, int64_
 inline int getrf_block(int N, int tuned) { return tuned ? tuned : (N >= 2048 ? 4 * PB : N >= 1024 ? 2 * PB : PB); }
 
 // Workspace of the factorisation (elements): the inverted diagonal blocks
-// of L11 for U12 = L11^-1 A12, kb x PB per matrix for the widest block.
+// of L11 for U12 = L11^-1 A12, kb x PB per matrix for the widest block,
+// then the exchange records of the multi-work-group panel leaves.
+template <typename R>
+size_t getrf_inv_elems(int batch)
+{
+    return (size_t)4 * PB * PB * (size_t)batch;
+}
 template <typename R>
 size_t getrf_ws_elems(int, int batch)
 {
-    return (size_t)4 * PB * PB * (size_t)batch;
+    const size_t xbytes = sizeof(uint64_t) * (size_t)MW_XWORDS * batch;
+    return getrf_inv_elems<R>(batch) + (xbytes + sizeof(cx<R>) - 1) / sizeof(cx<R>);
 }
 
 // Blocked right-looking LU with partial pivoting (LAPACK zgetrf semantics),
@@ -1661,10 +2117,22 @@ int getrf_steps(cx<R>* A, int N, int64_t lda, int64_t strideA, int* ipiv, int64_
 {
     int e;
     const int nb = getrf_block(N, aux ? aux->block : 0);
-    const int leaf = aux ? aux->leaf : 16;
-    const int pnt = aux ? aux->panel_threads : 512;
+    PanelCfg pc;
+    pc.leaf = aux ? aux->leaf : 16;
+    pc.nt = aux ? aux->panel_threads : 512;
+    if (aux && aux->mw_budget > 0) pc.budget = aux->mw_budget;
+    if (aux && aux->panel_spin != 0) pc.spin = aux->panel_spin;
+    pc.mw = !aux || aux->panel_mw;
+    if (work) {
+        // the exchange records start cleared: sequence numbers are unique
+        // within this factorisation only
+        pc.xbuf = (uint64_t*)(work + getrf_inv_elems<R>(batch));
+        const size_t nwords = (size_t)MW_XWORDS * batch;
+        hipLaunchKernelGGL(zero_words_kernel, dim3((unsigned)((nwords + 255) / 256)), dim3(256), 0, stream, pc.xbuf,
+                           nwords);
+    }
     const bool ahead = aux && aux->side && aux->lookahead;
-    if ((e = panel_dispatch<R>(A, lda, strideA, N, 0, min(nb, N), ipiv, strideP, info, batch, leaf, pnt, stream))) return e;
+    if ((e = panel_dispatch<R>(A, lda, strideA, N, 0, min(nb, N), ipiv, strideP, info, batch, pc, stream))) return e;
     for (int k0 = 0; k0 < N; k0 += nb) {
         const int kb = min(nb, N - k0);
         // panel k0 is factored (on `stream`, or on the side stream joined below)
@@ -1688,7 +2156,7 @@ int getrf_steps(cx<R>* A, int N, int64_t lda, int64_t strideA, int* ipiv, int64_
             // the trailing matrix is updated on `stream` (disjoint columns)
             if (hipEventRecord(aux->ready, stream) != hipSuccess) return (int)hipErrorUnknown;
             if (hipStreamWaitEvent(aux->side, aux->ready, 0) != hipSuccess) return (int)hipErrorUnknown;
-            if ((e = panel_dispatch<R>(A, lda, strideA, N, k0 + kb, kn, ipiv, strideP, info, batch, leaf, pnt, aux->side)))
+            if ((e = panel_dispatch<R>(A, lda, strideA, N, k0 + kb, kn, ipiv, strideP, info, batch, pc, aux->side)))
                 return e;
             if (hipEventRecord(aux->done, aux->side) != hipSuccess) return (int)hipErrorUnknown;
         }
@@ -1697,8 +2165,8 @@ int getrf_steps(cx<R>* A, int N, int64_t lda, int64_t strideA, int* ipiv, int64_
             return e;
         if (ahead) {
             if (hipStreamWaitEvent(stream, aux->done, 0) != hipSuccess) return (int)hipErrorUnknown;
-        } else if ((e = panel_dispatch<R>(A, lda, strideA, N, k0 + kb, kn, ipiv, strideP, info, batch, leaf,
-                                          pnt, stream))) {
+        } else if ((e = panel_dispatch<R>(A, lda, strideA, N, k0 + kb, kn, ipiv, strideP, info, batch, pc,
+                                          stream))) {
             return e;
         }
     }

@@ -150,7 +150,9 @@ int cpt_wave_part_infinite_depth(cpt_handle h, int np, const double* x0i, const 
 /* Batched LU factorisation with partial pivoting of `batch` complex128
  * n x n matrices A + b*strideA (column-major, leading dimension lda), in
  * place.  ipiv[b*strideP + i] (0-based) and info[b] as LAPACK zgetrf
- * (info = 1 + index of the first exactly-zero pivot, 0 if none).
+ * (info = 1 + index of the first exactly-zero pivot, 0 if none); info = -1
+ * when a multi-work-group panel leaf gave up waiting for its work-groups
+ * (CPT_TUNE_PANEL_SPIN), the factor then being incomplete.
  * Replaces scipy.linalg.lu_factor (LAPACK zgetrf) as called by
  * src/capytaine/tools/block_circulant_matrices.py:361-378 from
  * src/capytaine/bem/engines.py:326-335. */
@@ -369,9 +371,17 @@ enum {
     CPT_TUNE_PANEL_THREADS = 7,    /* threads of a panel leaf's work-group: 512 (default) or 256
                                       (one wavefront per SIMD, up to 8192 rows: fits on a CU beside
                                       one work-group of the tile product) */
-    CPT_TUNE_MIXED_RETRIES = 8     /* read only: mixed solves (or groups) of this context factored
+    CPT_TUNE_MIXED_RETRIES = 8,    /* read only: mixed solves (or groups) of this context factored
                                       again with FP32-input products after the FP16-input products
                                       left the refinement short of its tolerance */
+    CPT_TUNE_PANEL_WGS = 9,        /* 1 (default): panel leaves of 32 (complex64) or 16 (complex128)
+                                      columns factored by up to 32 work-groups per matrix that
+                                      exchange each column's pivot candidates through device memory,
+                                      where the leaf's rows fit their registers; 0: one work-group
+                                      per matrix */
+    CPT_TUNE_PANEL_SPIN = 10       /* polls a multi-work-group leaf waits for each column's candidates
+                                      before giving up (then info = -1): 0 = default (2^20); a
+                                      negative value gives up at once (a test of that path) */
 };
 int cpt_set_tuning(cpt_handle h, int param, int value);
 int cpt_get_tuning(cpt_handle h, int param, int* value);
