@@ -463,48 +463,50 @@ __global__ void __launch_bounds__(NT) panel_kernel(cx<R>* __restrict__ Abase, in
 // ---------------------------------------------------------------------------
 // 1b. Panel leaf over several work-groups per matrix.
 //
-// P work-groups of NT threads (512 or 1024) per matrix each own a slab of NT
-// rows of the leaf, one row per thread with all W leaf columns in registers:
-// the leaf is loaded once and stored once.  Row interchanges move no data: each register slot
-// carries the (leaf-relative) row it currently holds, an interchange of rows
-// j and p swaps two labels, and the final store writes each slot to its
-// row.  Per column every work-group finds its best candidate (largest
-// |Re|+|Im|, first row on ties) and publishes it -- value, row, and the
-// row's W entries -- in an exchange record; wave 0 of every work-group then
-// reads all P records of its matrix, picks the same winner as every other
-// work-group and stages the pivot row in LDS.  Each 64-bit word of a record
-// holds 32 bits of data and, in its high half, the column's sequence number:
-// a record is complete when every word shows the current sequence, so one
-// round of loads suffices and no fence is needed (agent-scope atomic loads
-// and stores, coherent across the XCDs).  Records alternate between two
-// parities: a work-group writes the record of column j + 2 only after it has
-// read every record of column j + 1, which the other work-groups write only
-// after reading those of column j.  Sequence numbers are unique within a
-// factorisation, whose start clears the records.  The wait is bounded (spin
-// polls; a negative spin gives up at once, a test hook): a work-group that
-// gives up sets the matrix's info to -1 and returns, so that every
-// work-group ends.  Register indices stay static: the leaf's columns rotate
-// through the slots, the pivot column always in slot 0.
+// P work-groups of 512 threads per matrix each own a slab of 512 or 1024
+// rows of the leaf, one or two rows per thread with all W leaf columns in
+// registers: the leaf is loaded once and stored once.  Row interchanges move no data:
+// each register slot carries the (leaf-relative) row it currently holds, an
+// interchange of rows j and p swaps two labels, and the final store writes
+// each slot to its row.
+//
+// Per column, one barrier inside the work-group: every wave finds its best
+// candidate (largest |Re|+|Im|, first row on ties) with a DPP reduction and
+// stages the candidate's row, value and reciprocal in LDS (two buffers, by
+// column parity, so that the next column's staging never meets a slow
+// wave's reads); after the barrier every wave combines the staged entries
+// itself (same inputs, same result), so no second barrier hands the winner
+// around.  With P > 1 the work-group's winner is published -- value, row,
+// reciprocal and the row's W entries -- in an exchange record, and the
+// whole work-group reads all P records of its matrix in one round of loads
+// into LDS; one more barrier tells every wave that each of its words showed
+// the column's sequence number.  Each 64-bit word of a record holds 32 bits
+// of data and, in its high half, the sequence number: a record is complete
+// when every word shows the current sequence, so no fence is needed
+// (agent-scope atomic loads and stores, coherent across the XCDs).  Records
+// alternate between two parities: a work-group writes the record of column
+// j + 2 only after it has read every record of column j + 1, which the other
+// work-groups write only after reading those of column j.  Sequence numbers
+// are unique within a factorisation, whose start clears the records.  The
+// wait is bounded (spin polls; a negative spin gives up at once, a test
+// hook): a work-group that gives up sets the matrix's info to -1 and
+// returns, so that every work-group ends.  Register indices stay static:
+// the update of column j writes each slot's result one slot down, so the
+// pivot column is always slot 0 and the finished column moves to the end.
 // ---------------------------------------------------------------------------
 constexpr int MW_PMAX = 32;    // work-groups per matrix
-constexpr int MW_HDR = 4;      // header words: best (1 or 2 words), row, padding
 constexpr int MW_ROWW = 64;    // words of one row of W entries (2 W for float pairs, 4 W for double pairs)
-constexpr int MW_REC = 72;     // words per record: header, candidate row, padding
+constexpr int MW_REC = 72;     // words per record: header, candidate row
 constexpr int MW_XWORDS = 2 * MW_PMAX * MW_REC;  // words per matrix: two parities of MW_PMAX records
 constexpr int MW_LEAF_BUDGET = 256;  // work-groups of one leaf launch over the batch
 
 template <typename R>
 constexpr int mw_width() { return sizeof(R) == 4 ? 32 : 16; }
 
-// v[s] <- v[s + 1], v[W - 1] <- v[0], as straight assignments (a loop of
-// them would be recognised as a memory move and keep v out of registers).
-template <typename T, int W, int... S>
-__device__ __forceinline__ void rotate_slots(T (&v)[W], std::integer_sequence<int, S...>)
-{
-    const T f = v[0];
-    ((v[S] = v[S + 1]), ...);
-    v[W - 1] = f;
-}
+// Header of a record, in 32-bit data words: best (1 or 2 words) at 0, row at
+// 2, reciprocal of the pivot (re, im) from 4; the row's entries follow.
+template <typename R>
+constexpr int mw_hdr() { return 4 + 2 * (int)(sizeof(R) / 4); }
 
 __device__ __forceinline__ void mw_store(uint64_t* p, uint32_t seq, uint32_t bits)
 {
@@ -515,30 +517,98 @@ __device__ __forceinline__ uint64_t mw_load(const uint64_t* p)
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// A real as record words (low halves) and back.
+// One step of the argmax with a DPP move restricted to the rows in ROWMASK;
+// lanes of the other rows compare with themselves and keep their values.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ int dpp_keep(int v)
+{
+    return __builtin_amdgcn_update_dpp(v, v, CTRL, ROWMASK, 0xF, false);
+}
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ float dpp_keep(float v)
+{
+    return __int_as_float(dpp_keep<CTRL, ROWMASK>(__float_as_int(v)));
+}
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_keep(double v)
+{
+    const int lo = dpp_keep<CTRL, ROWMASK>(__double2loint(v)), hi = dpp_keep<CTRL, ROWMASK>(__double2hiint(v));
+    return __hiloint2double(hi, lo);
+}
+// Full-row moves (every source lane valid) need no old value.
+template <int CTRL>
+__device__ __forceinline__ int dpp_any(int v)
+{
+    return __builtin_amdgcn_mov_dpp(v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_any(float v)
+{
+    return __int_as_float(dpp_any<CTRL>(__float_as_int(v)));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_any(double v)
+{
+    const int lo = dpp_any<CTRL>(__double2loint(v)), hi = dpp_any<CTRL>(__double2hiint(v));
+    return __hiloint2double(hi, lo);
+}
 template <typename R>
-struct MwWords;
-template <>
-struct MwWords<float> {
-    static constexpr int N = 1;
-    static __device__ __forceinline__ void put(uint64_t* p, uint32_t seq, float v) { mw_store(p, seq, __float_as_uint(v)); }
-    static __device__ __forceinline__ float get(const uint64_t* w) { return __uint_as_float((uint32_t)w[0]); }
-};
-template <>
-struct MwWords<double> {
-    static constexpr int N = 2;
-    static __device__ __forceinline__ void put(uint64_t* p, uint32_t seq, double v)
-    {
-        mw_store(p, seq, (uint32_t)__double2loint(v));
-        mw_store(p + 1, seq, (uint32_t)__double2hiint(v));
-    }
-    static __device__ __forceinline__ double get(const uint64_t* w)
-    {
-        return __hiloint2double((int)(uint32_t)w[1], (int)(uint32_t)w[0]);
-    }
-};
+__device__ __forceinline__ void argmax_take(R& best, int& idx, R ob, int oi)
+{
+    // all three comparisons evaluated (no short circuit, no branches)
+    const bool take = (ob > best) | ((ob == best) & (oi < idx));
+    best = take ? ob : best;
+    idx = take ? oi : idx;
+}
+template <typename R, int CTRL, int ROWMASK>
+__device__ __forceinline__ void argmax_keep(R& best, int& idx)
+{
+    if constexpr (ROWMASK == 0xF)
+        argmax_take(best, idx, dpp_any<CTRL>(best), dpp_any<CTRL>(idx));
+    else
+        argmax_take(best, idx, dpp_keep<CTRL, ROWMASK>(best), dpp_keep<CTRL, ROWMASK>(idx));
+}
+constexpr int DPP_ROW_BCAST15 = 0x142, DPP_ROW_BCAST31 = 0x143;
 
-template <typename R, int W, int MW_NT>
+// (best, idx) over the lanes of row 0 (lanes 0..15), uniform: the largest
+// best, the smallest idx among equal values.
+template <typename R>
+__device__ __forceinline__ void row0_argmax(R& best, int& idx)
+{
+    argmax_keep<R, DPP_XOR1, 0xF>(best, idx);
+    argmax_keep<R, DPP_XOR2, 0xF>(best, idx);
+    argmax_keep<R, DPP_HALF_MIRROR, 0xF>(best, idx);
+    argmax_keep<R, DPP_MIRROR, 0xF>(best, idx);
+    best = lane_bcast(best, 0);
+    idx = __builtin_amdgcn_readlane(idx, 0);
+}
+// The same over the whole wavefront: rows reduced by the mirrors, then row
+// 0 into row 1 and row 2 into row 3 (broadcast of lane 15 of each row), then
+// rows 0-1 into row 3 (broadcast of lane 31); lane 63 holds the result.
+template <typename R>
+__device__ __forceinline__ void wave_argmax_bcast(R& best, int& idx)
+{
+    argmax_keep<R, DPP_XOR1, 0xF>(best, idx);
+    argmax_keep<R, DPP_XOR2, 0xF>(best, idx);
+    argmax_keep<R, DPP_HALF_MIRROR, 0xF>(best, idx);
+    argmax_keep<R, DPP_MIRROR, 0xF>(best, idx);
+    argmax_keep<R, DPP_ROW_BCAST15, 0xA>(best, idx);
+    argmax_keep<R, DPP_ROW_BCAST31, 0xC>(best, idx);
+    best = lane_bcast(best, 63);
+    idx = __builtin_amdgcn_readlane(idx, 63);
+}
+
+__device__ __forceinline__ float rfma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double rfma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// Word k of a real as 32-bit data (k < sizeof(R) / 4).
+__device__ __forceinline__ uint32_t real_word(float v, int) { return __float_as_uint(v); }
+__device__ __forceinline__ uint32_t real_word(double v, int k)
+{
+    return (uint32_t)(k ? __double2hiint(v) : __double2loint(v));
+}
+
+template <typename R, int W, int MW_NT, int RPT>
 __global__ void __launch_bounds__(MW_NT) panel_mw_kernel(cx<R>* __restrict__ Abase, int64_t lda, int64_t strideA,
                                                          int N, int k0, int kb, int sw_lo, int sw_hi,
                                                          int* __restrict__ ipiv_base, int64_t strideP,
@@ -546,13 +616,15 @@ __global__ void __launch_bounds__(MW_NT) panel_mw_kernel(cx<R>* __restrict__ Aba
                                                          int P, uint32_t seq0, int spin)
 {
     typedef cx<R> C;
-    typedef MwWords<R> MW;
-    constexpr int NV = MW::N;
-    constexpr int PER = MW_HDR + 2 * NV * W;  // words read per record
-    static_assert(2 * NV * W <= MW_ROWW, "leaf row does not fit a record");
+    constexpr int NV = sizeof(R) / 4;         // 32-bit words per real
+    constexpr int HDR = mw_hdr<R>();
+    constexpr int PER = HDR + 2 * NV * W;     // words read per record
+    constexpr int WPT = (MW_PMAX * PER + MW_NT - 1) / MW_NT;  // record words per thread
+    static_assert(2 * NV * W <= MW_ROWW && PER <= MW_REC, "leaf row does not fit a record");
+    static_assert(HDR % (2 * NV) == 0 && (MW_REC * 4) % 16 == 0, "record data misaligned for its reals");
     constexpr int NW = MW_NT / 64;
-    constexpr int RPT = 1;  // one row per thread: the leaf's W entries in registers
-    constexpr int Rw = MW_NT * RPT;
+    static_assert(NW <= 16, "the wave entries are combined within one row of 16 lanes");
+    constexpr int Rw = MW_NT * RPT;  // RPT rows per thread, each with the leaf's W entries in registers
     const int g = blockIdx.x, b = blockIdx.y;
     C* A = Abase + (size_t)blockIdx.y * strideA;
     int* ipiv = ipiv_base + (size_t)b * strideP;
@@ -562,13 +634,19 @@ __global__ void __launch_bounds__(MW_NT) panel_mw_kernel(cx<R>* __restrict__ Aba
     const int row0 = g * Rw;
     const int rend = min(M, row0 + Rw);
 
-    __shared__ C sCand[NW][W];
-    __shared__ C sP[W];
-    __shared__ R sBest[NW];
-    __shared__ int sIdx[NW];
+    __shared__ C sCand[2][NW][W];  // each wave's candidate row, finished columns as zeros
+    __shared__ C sRcp[2][NW];      // its reciprocal
+    __shared__ R sBest[2][NW];
+    __shared__ int sIdx[2][NW];
+    __shared__ int sOk[2][NW];     // exchange polls: the wave saw every word of its share
+    __shared__ C sZero[W];         // the pivot row of a column without pivot
     __shared__ int sPiv[W];
-    __shared__ int sWin, sAbort;
-    __shared__ R sWinBest;
+    __shared__ int sSrc[2 * W], sDst[2 * W];
+    // the records of one column as read: the data halves of their words,
+    // record r from word r * MW_REC (16-byte aligned)
+    __shared__ __align__(16) uint32_t sXd[MW_PMAX * MW_REC + 1];
+
+    if (tid < W) sZero[tid] = C{R(0), R(0)};
 
     C v[RPT][W];
     int lab[RPT];  // leaf row held by slot q (INT_MAX: no row)
@@ -583,198 +661,188 @@ __global__ void __launch_bounds__(MW_NT) panel_mw_kernel(cx<R>* __restrict__ Aba
         }
     }
 
-    // after j steps, register slot s holds leaf column (s + j) mod W: the
-    // pivot column is always slot 0, and each step rotates the slots by one,
-    // the finished column moving to the end.  The loop is not unrolled: the
-    // code of one step fits the instruction cache.
+    // after j steps, register slot s holds leaf column (s + j) mod W.  The
+    // loop is not unrolled: the code of one step fits the instruction cache.
     bool quit = false;
     int jend = kb;  // set to the current step when giving up: one exit only
 #pragma unroll 1
     for (int j = 0; j < jend; ++j) {
+        const int buf = j & 1;
+        const uint32_t seq = seq0 + (uint32_t)j;
+        // entries of the pivot row beyond the active columns (the finished
+        // ones, at the end of the slots) enter the update as zeros
+        const int active = W - j;
         {
-            const uint32_t seq = seq0 + (uint32_t)j;
             R best = R(-1);
             int bidx = INT_MAX;
+            C cb = v[0][0];  // the thread's best entry
 #pragma unroll
             for (int q = 0; q < RPT; ++q) {
-                if (lab[q] != INT_MAX && lab[q] >= j) {
-                    const R a = cabs1(v[q][0]);
-                    if (a > best || (a == best && lab[q] < bidx)) {
-                        best = a;
-                        bidx = lab[q];
-                    }
-                }
+                const R a = cabs1(v[q][0]);
+                const bool take = (lab[q] != INT_MAX) & (lab[q] >= j) & ((a > best) | ((a == best) & (lab[q] < bidx)));
+                best = take ? a : best;
+                bidx = take ? lab[q] : bidx;
+                cb = C{take ? v[q][0].re : cb.re, take ? v[q][0].im : cb.im};
             }
-            wave_argmax(best, bidx);
-            if (lane == 0) {
-                sBest[wave] = best;
-                sIdx[wave] = bidx;
-            }
+            // the reciprocal of the thread's best entry, computed beside the
+            // reduction: the winner's is staged with its row
+            const C myrcp = (cb.re != R(0) || cb.im != R(0)) ? zrcp(cb) : C{R(1), R(0)};
+            wave_argmax_bcast(best, bidx);
 #pragma unroll
             for (int q = 0; q < RPT; ++q) {
                 if (lab[q] == bidx) {
 #pragma unroll
-                    for (int s = 0; s < W; ++s) sCand[wave][s] = v[q][s];
+                    for (int s = 0; s < W; ++s) sCand[buf][wave][s] = v[q][s];
+                    sRcp[buf][wave] = myrcp;
                 }
             }
-            __syncthreads();
-            uint64_t* recs = X + (size_t)(j & 1) * MW_PMAX * MW_REC;
-            // entries of the pivot row beyond the active columns (the
-            // finished ones, rotated to the end) enter the update as zeros
-            const int active = W - j;
-            if (wave == 0) {
-                R bg = sBest[0];
-                int pg = sIdx[0], wg = 0;
-#pragma unroll
-                for (int w = 1; w < NW; ++w) {
-                    const R bw = sBest[w];
-                    const int iw = sIdx[w];
-                    if (bw > bg || (bw == bg && iw < pg)) {
-                        bg = bw;
-                        pg = iw;
-                        wg = w;
-                    }
-                }
-                int ab = 0;
-                if (P == 1) {
-                    if (lane < W) sP[lane] = lane < active ? sCand[wg][lane] : C{R(0), R(0)};
-                    if (lane == 0) {
-                        sWin = pg;
-                        sWinBest = bg;
-                    }
-                } else {
-                    // this work-group's record: the row first, and once its
-                    // stores are complete the header, so that a header
-                    // showing the column's sequence implies a complete row
-                    uint64_t* rec = recs + (size_t)g * MW_REC;
-                    if (lane < W) {
-                        const C c = sCand[wg][lane];
-                        uint64_t* pc = rec + MW_HDR + 2 * NV * lane;
-                        MW::put(pc, seq, c.re);
-                        MW::put(pc + NV, seq, c.im);
-                    }
-                    if (lane == 0) {
-                        // release: the wave's row stores precede it
-                        __hip_atomic_store(rec + 2, ((uint64_t)seq << 32) | (uint32_t)pg, __ATOMIC_RELEASE,
-                                           __HIP_MEMORY_SCOPE_AGENT);
-                        MW::put(rec, seq, bg);
-                        if (NV == 1) mw_store(rec + 1, seq, 0u);
-                        mw_store(rec + 3, seq, 0u);
-                    }
-                    // headers of the P records (lane r: record r)
-                    R rb = R(-1);
-                    int ri = INT_MAX;
-                    for (int it = 0;; ++it) {
-                        bool ok = true;
-                        if (lane < P && spin >= 0) {
-                            const uint64_t* h = recs + (size_t)lane * MW_REC;
-                            uint64_t hw[MW_HDR];
-#pragma unroll
-                            for (int u = 0; u < MW_HDR; ++u) hw[u] = mw_load(h + u);
-#pragma unroll
-                            for (int u = 0; u < MW_HDR; ++u) ok = ok && (uint32_t)(hw[u] >> 32) == seq;
-                            rb = MW::get(hw);
-                            ri = (int)(uint32_t)hw[2];
-                        } else if (lane < P) {
-                            ok = false;
-                        }
-                        if (__all(ok)) break;
-                        if (spin < 0 || it >= spin) {
-                            ab = 1;
-                            break;
-                        }
-                        __builtin_amdgcn_s_sleep(1);
-                    }
-                    if (!ab) {
-                        // pairs with the writers' release: their rows are visible
-                        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-                        // the winner over the records: rows are distinct, the
-                        // lowest record lane holding the winning row is its owner
-                        int idx = ri;
-                        R bb = rb;
-                        wave_argmax(bb, idx);
-                        const unsigned long long hit = __ballot(lane < P && ri == idx);
-                        const int rr = hit ? __builtin_ctzll(hit) : 0;
-                        // its row: lane s < W reads entry s (checked too)
-                        for (int it = 0;; ++it) {
-                            bool ok = true;
-                            if (lane < W) {
-                                const uint64_t* pc = recs + (size_t)rr * MW_REC + MW_HDR + 2 * NV * lane;
-                                uint64_t ew[2 * NV];
-#pragma unroll
-                                for (int u = 0; u < 2 * NV; ++u) ew[u] = mw_load(pc + u);
-#pragma unroll
-                                for (int u = 0; u < 2 * NV; ++u) ok = ok && (uint32_t)(ew[u] >> 32) == seq;
-                                if (ok) sP[lane] = lane < active ? C{MW::get(ew), MW::get(ew + NV)} : C{R(0), R(0)};
-                            }
-                            if (__all(ok)) break;
-                            if (it >= std::max(spin, 0)) {
-                                ab = 1;
-                                break;
-                            }
-                            __builtin_amdgcn_s_sleep(1);
-                        }
-                        if (lane == 0) {
-                            sWin = idx;
-                            sWinBest = bb;
-                        }
-                    }
-                }
-                if (lane == 0) sAbort = ab;
-            }
-            __syncthreads();
-            const int abort = sAbort;
-
-            if (abort) {  // uniform: every wave read the same verdicts
-                // the columns left unfinished get no interchange, so that the
-                // kernels after this one find valid rows in ipiv
-                if (g == 0)
-                    for (int jj = j + tid; jj < kb; jj += MW_NT) ipiv[k0 + jj] = k0 + jj;
-                if (tid == 0) info_base[b] = -1;
-                quit = true;
-                jend = j;
-            }
-            // no nonzero candidate (a zero column, or one of NaNs, which no
-            // comparison selects): no interchange, singularity reported; after
-            // giving up nothing more is written
-            const bool pivoted = !quit && sWinBest > R(0);
-            const int p = pivoted ? sWin : j;
-            if (tid == 0 && !quit) {
-                sPiv[j] = p;
-                if (g == 0) {
-                    if (!pivoted && info_base[b] == 0) info_base[b] = k0 + j + 1;
-                    ipiv[k0 + j] = k0 + p;
-                }
-            }
-            // rows j and p trade places: their labels
-#pragma unroll
-            for (int q = 0; q < RPT; ++q) {
-                if (lab[q] == p) lab[q] = j;
-                else if (lab[q] == j) lab[q] = p;
-            }
-            const C piv = pivoted ? sP[0] : C{R(0), R(0)};
-            const bool nonzero = (piv.re != R(0) || piv.im != R(0));
-            const C rcp = nonzero ? zrcp(piv) : C{R(1), R(0)};
-#pragma unroll
-            for (int q = 0; q < RPT; ++q) {
-                // rows below the pivot take l and the rank-1 update, the
-                // others a zero multiplier (no branch: one version of the
-                // registers across the loop); the pivot row is zero on the
-                // finished columns
-                // (selects of values: a select of the two operands'
-                // addresses would keep v in memory)
-                const bool below = lab[q] != INT_MAX && lab[q] > j;
-                const C cur = v[q][0];
-                const C lq = nonzero ? zmul(cur, rcp) : cur;
-                const C l{below ? lq.re : R(0), below ? lq.im : R(0)};
-                v[q][0] = C{below ? lq.re : cur.re, below ? lq.im : cur.im};
-#pragma unroll
-                for (int s = 1; s < W; ++s) v[q][s] = zsub(v[q][s], zmul(l, sP[s]));
+            // after the owner's stores (one wave's LDS operations complete
+            // in order)
+            if (lane >= active && lane < W) sCand[buf][wave][lane] = C{R(0), R(0)};
+            if (lane == 0) {
+                sBest[buf][wave] = best;
+                sIdx[buf][wave] = bidx;
             }
         }
-        // rotate: the finished column to the last slot
+        __syncthreads();
+        // the work-group's candidate, combined by every wave
+        const int wl = lane < NW ? lane : 0;  // indices stay inside the arrays
+        R bg = lane < NW ? sBest[buf][wl] : R(-1);
+        int pg = lane < NW ? sIdx[buf][wl] : INT_MAX;
+        const int mine = pg;
+        row0_argmax(bg, pg);
+        const unsigned long long whit = __ballot(lane < NW && mine == pg);
+        const int wg = whit ? __builtin_ctzll(whit) : 0;
+        R wbest = bg;
+        int pwin = pg;
+        C rcp = sRcp[buf][wg];
+        const C* prow = &sCand[buf][wg][0];
+        int ab = 0;  // uniform: every wave reads the same flags
+        if (P > 1) {
+            uint64_t* recs = X + (size_t)buf * MW_PMAX * MW_REC;
+            if (wave == 0) {
+                // this work-group's record of the column
+                uint64_t* rec = recs + (size_t)g * MW_REC;
+                if (lane < 2 * NV * W) {
+                    const int e = lane / (2 * NV), k = lane % (2 * NV);
+                    const C c = prow[e];
+                    mw_store(rec + HDR + lane, seq, real_word(k < NV ? c.re : c.im, k % NV));
+                }
+                if (lane < HDR) {
+                    uint32_t bits = 0;
+                    if (lane < NV) bits = real_word(bg, lane);
+                    else if (lane == 2) bits = (uint32_t)pg;
+                    else if (lane >= 4) {
+                        const int k = lane - 4;
+                        bits = real_word(k < NV ? rcp.re : rcp.im, k % NV);
+                    }
+                    mw_store(rec + lane, seq, bits);
+                }
+            }
+            // all P records in one round of loads by the whole work-group
+            // (word w: record w / PER, its word w % PER), repeated until
+            // every word shows this column's sequence
+            const int nw = P * PER;
+            for (int it = 0;; ++it) {
+                // every load issued before the first wait: words past the
+                // records read word 0 again, and are not kept
+                uint64_t x[WPT];
+                int at[WPT];
 #pragma unroll
-        for (int q = 0; q < RPT; ++q) rotate_slots(v[q], std::make_integer_sequence<int, W - 1>{});
+                for (int u = 0; u < WPT; ++u) {
+                    const int w = tid + MW_NT * u;
+                    const int r = w / PER, o = w - r * PER;
+                    at[u] = w < nw ? r * MW_REC + o : -1;
+                    x[u] = mw_load(recs + max(at[u], 0));
+                }
+                bool ok = spin >= 0;
+#pragma unroll
+                for (int u = 0; u < WPT; ++u) {
+                    ok = ok & ((at[u] < 0) | ((uint32_t)(x[u] >> 32) == seq));
+                    sXd[at[u] >= 0 ? at[u] : MW_PMAX * MW_REC] = (uint32_t)x[u];  // past the end: discarded
+                }
+                const bool wok = __all(ok);
+                if (lane == 0) sOk[it & 1][wave] = wok;
+                __syncthreads();
+                if (__all(lane >= NW || sOk[it & 1][wl])) break;
+                if (spin < 0 || it >= spin) {
+                    ab = 1;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+            if (!ab) {
+                // the winner over the records: rows are distinct, the
+                // lowest record holding the winning row is its owner
+                R rb = R(-1);
+                int ri = INT_MAX;
+                if (lane < P) {
+                    rb = *reinterpret_cast<const R*>(&sXd[lane * MW_REC]);
+                    ri = (int)sXd[lane * MW_REC + 2];
+                }
+                const int rmine = ri;
+                wave_argmax_bcast(rb, ri);
+                const unsigned long long hit = __ballot(lane < P && rmine == ri);
+                const int rr = hit ? __builtin_ctzll(hit) : 0;
+                wbest = rb;
+                pwin = ri;
+                rcp = *reinterpret_cast<const C*>(&sXd[rr * MW_REC + 4]);
+                prow = reinterpret_cast<const C*>(&sXd[rr * MW_REC + HDR]);
+            }
+        }
+        if (ab) {
+            // the columns left unfinished get no interchange, so that the
+            // kernels after this one find valid rows in ipiv
+            if (g == 0)
+                for (int jj = j + tid; jj < kb; jj += MW_NT) ipiv[k0 + jj] = k0 + jj;
+            if (tid == 0) info_base[b] = -1;
+            quit = true;
+            jend = j;
+        }
+        // no nonzero candidate (a zero column, or one of NaNs, which no
+        // comparison selects): no interchange, singularity reported; after
+        // giving up nothing more is written
+        const bool pivoted = !quit && wbest > R(0);
+        const int p = pivoted ? pwin : j;
+        if (tid == 0 && !quit) {
+            sPiv[j] = p;
+            if (g == 0) {
+                if (!pivoted && info_base[b] == 0) info_base[b] = k0 + j + 1;
+                ipiv[k0 + j] = k0 + p;
+            }
+        }
+        // rows j and p trade places: their labels
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+            if (lab[q] == p) lab[q] = j;
+            else if (lab[q] == j) lab[q] = p;
+        }
+        if (!pivoted) {
+            prow = sZero;
+            rcp = C{R(1), R(0)};
+        }
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+            // rows below the pivot take l and the rank-1 update, the others
+            // a zero multiplier (no branch: one version of the registers
+            // across the loop); each result goes one slot down, the finished
+            // column to the last slot
+            // (selects of values: a select of the two operands' addresses
+            // would keep v in memory)
+            const bool below = lab[q] != INT_MAX && lab[q] > j;
+            const C cur = v[q][0];
+            const C lq = pivoted ? zmul(cur, rcp) : cur;
+            const C l{below ? lq.re : R(0), below ? lq.im : R(0)};
+            const C fin{below ? lq.re : cur.re, below ? lq.im : cur.im};
+#pragma unroll
+            for (int s = 1; s < W; ++s) {
+                // v - l p in four fused operations
+                const C pv = prow[s], x = v[q][s];
+                v[q][s - 1] = C{rfma(-l.re, pv.re, rfma(l.im, pv.im, x.re)),
+                                rfma(-l.re, pv.im, rfma(-l.im, pv.re, x.im))};
+            }
+            v[q][W - 1] = fin;
+        }
     }
     if (quit) return;
     // slot s holds column (s + kb) mod W
@@ -788,37 +856,51 @@ __global__ void __launch_bounds__(MW_NT) panel_mw_kernel(cx<R>* __restrict__ Aba
             }
     }
     __syncthreads();  // sPiv complete
-
-    // the leaf's interchanges on the panel's other columns [sw_lo, sw_hi):
-    // column c goes to work-group (c - sw_lo) mod P; one thread per
-    // (column, involved row) finds, walking the interchanges backwards,
-    // which original row ends in that row, loads it, and after a barrier
-    // stores it
+    // the leaf's interchanges on the panel's other columns [sw_lo, sw_hi),
+    // column c to work-group (c - sw_lo) mod P: the rows involved (j and its
+    // pivot, for each column j) and, walking the interchanges backwards,
+    // the original row each ends with, once in LDS; then every (column,
+    // row) pair is loaded, and after a barrier stored
     {
         const int nslot = 2 * kb;
+        if (tid < nslot) {
+            const int r = (tid & 1) ? sPiv[tid >> 1] : (tid >> 1);
+            int x = r;
+            for (int s = kb - 1; s >= 0; --s) {
+                if (x == s) x = sPiv[s];
+                else if (x == sPiv[s]) x = s;
+            }
+            sDst[tid] = r;
+            sSrc[tid] = x;
+        }
+        __syncthreads();
         const int ncol = sw_hi - sw_lo;
         const int mycols = ncol > g ? (ncol - g + P - 1) / P : 0;
-        const int cols_per_chunk = MW_NT / nslot;
-        for (int cb = 0; cb < mycols; cb += cols_per_chunk) {
-            C val{R(0), R(0)};
-            int64_t dst = -1;
-            const int ci = cb + tid / nslot;
-            if (tid < cols_per_chunk * nslot && ci < mycols) {
-                const int col = sw_lo + g + ci * P, js = tid % nslot;
-                if (col < k0 || col >= k0 + kb) {
-                    const int r = (js & 1) ? sPiv[js >> 1] : (js >> 1);
-                    int x = r;
-                    for (int s = kb - 1; s >= 0; --s) {
-                        if (x == s) x = sPiv[s];
-                        else if (x == sPiv[s]) x = s;
+        const int npairs = mycols * nslot;
+        constexpr int PPT = 8;  // pairs per thread in flight
+        // whole columns per chunk: a column's loads all precede its stores
+        const int chunk = nslot ? (MW_NT * PPT / nslot) * nslot : 1;
+        for (int p0 = 0; p0 < npairs; p0 += chunk) {
+            C val[PPT];
+            int64_t dst[PPT];
+#pragma unroll
+            for (int u = 0; u < PPT; ++u) {
+                const int pr = p0 + tid + MW_NT * u;
+                val[u] = C{R(0), R(0)};
+                dst[u] = -1;
+                if (tid + MW_NT * u < chunk && pr < npairs) {
+                    const int col = sw_lo + g + (pr / nslot) * P, js = pr % nslot;
+                    if (col < k0 || col >= k0 + kb) {
+                        const int64_t colo = k0 + (int64_t)col * lda;
+                        val[u] = A[colo + sSrc[js]];
+                        dst[u] = colo + sDst[js];
                     }
-                    const int64_t colo = k0 + (int64_t)col * lda;
-                    val = A[colo + x];
-                    dst = colo + r;
                 }
             }
             __syncthreads();
-            if (dst >= 0) A[dst] = val;
+#pragma unroll
+            for (int u = 0; u < PPT; ++u)
+                if (dst[u] >= 0) A[dst[u]] = val[u];
         }
     }
 }
@@ -1812,6 +1894,8 @@ void launch_panel(cx<R>* A, int64_t lda, int64_t sA, int N, int k0, int kb, int 
 // threads, and for the multi-work-group leaves the exchange records (null:
 // single-work-group leaves only), the work-groups one launch may take over
 // the batch, the spin bound of their waits and the next sequence number.
+constexpr int PANEL_SINGLE_MAX_ROWS = 8192;  // rows of the one-work-group leaf kernels
+
 struct PanelCfg {
     int leaf = 16, nt = 512;
     bool mw = true;  // multi-work-group leaves where they fit
@@ -1821,27 +1905,25 @@ struct PanelCfg {
     uint32_t seq = 1;
 };
 
-// Threads per work-group (512 or 1024, one row each), work-groups per
-// matrix and matrices per launch of a multi-work-group leaf of M rows: one
-// launch of at most `budget` work-groups when the batch fits (512 threads
-// up to 16 work-groups per matrix, else 1024 up to 32), else the batch in
-// chunks of budget / P matrices with 1024 threads; nt = 0 when M exceeds
-// 32 x 1024 rows.
+// Rows per work-group (512 threads with one row each, or two: 512 or 1024
+// rows), work-groups per matrix and matrices per launch of a
+// multi-work-group leaf of M rows: 512 rows when they hold the leaf, else
+// 1024 (fewer work-groups, fewer exchanges to wait for), up to 32
+// work-groups per matrix; one launch of at most `budget` work-groups when
+// the batch fits, else the batch in chunks of budget / P matrices; nt = 0
+// when M exceeds 32 x 1024 rows.
 inline void mw_shape(int M, int batch, const PanelCfg& pc, int& P, int& nt, int& per_launch)
 {
-    const int pmax = std::max(1, std::min(MW_PMAX, pc.budget / std::max(batch, 1)));
-    if (M <= 512 * std::min(pmax, 16)) nt = 512;
-    else if (M <= 1024 * pmax) nt = 1024;
-    else nt = M <= 1024 * MW_PMAX ? 1024 : 0;
+    nt = M <= 512 ? 512 : M <= 1024 * MW_PMAX ? 1024 : 0;
     P = nt ? (M + nt - 1) / nt : 0;
     per_launch = P ? std::max(1, std::min(batch, pc.budget / P)) : 0;
 }
 
-template <typename R, int NT>
+template <typename R, int NT, int RPT>
 void launch_panel_mw(cx<R>* A, int64_t lda, int64_t sA, int N, int k0, int kb, int sw_lo, int sw_hi, int* ipiv,
                      int64_t sP, int* info, int batch, int P, const PanelCfg& pc, uint32_t seq, hipStream_t stream)
 {
-    hipLaunchKernelGGL((panel_mw_kernel<R, mw_width<R>(), NT>), dim3(P, batch), dim3(NT), 0, stream, A, lda, sA, N,
+    hipLaunchKernelGGL((panel_mw_kernel<R, mw_width<R>(), NT, RPT>), dim3(P, batch), dim3(NT), 0, stream, A, lda, sA, N,
                        k0, kb, sw_lo, sw_hi, ipiv, sP, info, pc.xbuf, P, seq, pc.spin);
 }
 
@@ -1865,8 +1947,8 @@ int panel_leaf_mw(cx<R>* A, int64_t lda, int64_t sA, int N, int k0, int kb, int 
         int* pb = ipiv + (size_t)b0 * sP;
         int* ib = info + b0;
         part.xbuf = pc.xbuf + (size_t)b0 * MW_XWORDS;
-        if (nt == 512) launch_panel_mw<R, 512>(Ab, lda, sA, N, k0, kb, sw_lo, sw_hi, pb, sP, ib, nb, P, part, seq, stream);
-        else launch_panel_mw<R, 1024>(Ab, lda, sA, N, k0, kb, sw_lo, sw_hi, pb, sP, ib, nb, P, part, seq, stream);
+        if (nt == 512) launch_panel_mw<R, 512, 1>(Ab, lda, sA, N, k0, kb, sw_lo, sw_hi, pb, sP, ib, nb, P, part, seq, stream);
+        else launch_panel_mw<R, 512, 2>(Ab, lda, sA, N, k0, kb, sw_lo, sw_hi, pb, sP, ib, nb, P, part, seq, stream);
     }
     done = true;
     return (int)hipGetLastError();
@@ -1947,14 +2029,14 @@ int panel_rec(cx<R>* A, int64_t lda, int64_t sA, int N, int c0, int w, int sw_lo
     int e;
     // multi-work-group leaves where enabled, and always above the 8192 rows
     // the one-work-group kernel keeps in registers
-    if (w <= mw_width<R>() && pc.xbuf && (pc.mw || M > 8192)) {
+    if (w <= mw_width<R>() && pc.xbuf && (pc.mw || M > PANEL_SINGLE_MAX_ROWS)) {
         bool done;
         if ((e = panel_leaf_mw<R>(A, lda, sA, N, c0, w, sw_lo, sw_hi, ipiv, sP, info, batch, pc, done, stream)) || done)
             return e;
     }
     // the one-work-group leaf kernel holds at most PB columns (its LDS
     // triangle) and 8192 rows; taller panels split down to the other's width
-    if (w <= PB && (w <= pc.leaf || M <= 2 * pc.leaf) && M <= 8192)
+    if (w <= PB && (w <= pc.leaf || M <= 2 * pc.leaf) && M <= PANEL_SINGLE_MAX_ROWS)
         return panel_leaf<R>(A, lda, sA, N, c0, w, sw_lo, sw_hi, ipiv, sP, info, batch, pc.nt, stream);
     const int h = w / 2;
     if ((e = panel_rec<R>(A, lda, sA, N, c0, h, sw_lo, sw_hi, ipiv, sP, info, batch, pc, stream))) return e;
@@ -2151,19 +2233,27 @@ int getrf_steps(cx<R>* A, int N, int64_t lda, int64_t strideA, int* ipiv, int64_
         if ((e = gemm<R>(rest, kn, kb, R(-1), A21, lda, strideA, A12, lda, strideA, A22, lda, strideA, 0, batch,
                          stream)))
             return e;
-        if (ahead) {
+        // a panel beside the trailing update gets one work-group per matrix
+        // per leaf: the work-groups of a multi-work-group leaf must all be
+        // resident at once, which a concurrent update of the whole device
+        // delays until it drains; panels taller than one work-group's reach
+        // are factored in order
+        const bool side = ahead && rest <= PANEL_SINGLE_MAX_ROWS;
+        if (side) {
             // ... factor the next panel on the side stream while the rest of
             // the trailing matrix is updated on `stream` (disjoint columns)
             if (hipEventRecord(aux->ready, stream) != hipSuccess) return (int)hipErrorUnknown;
             if (hipStreamWaitEvent(aux->side, aux->ready, 0) != hipSuccess) return (int)hipErrorUnknown;
-            if ((e = panel_dispatch<R>(A, lda, strideA, N, k0 + kb, kn, ipiv, strideP, info, batch, pc, aux->side)))
+            PanelCfg one = pc;
+            one.mw = false;
+            if ((e = panel_dispatch<R>(A, lda, strideA, N, k0 + kb, kn, ipiv, strideP, info, batch, one, aux->side)))
                 return e;
             if (hipEventRecord(aux->done, aux->side) != hipSuccess) return (int)hipErrorUnknown;
         }
         if (rest > kn && (e = gemm<R>(rest, rest - kn, kb, R(-1), A21, lda, strideA, A12 + (size_t)kn * lda, lda,
                                       strideA, A22 + (size_t)kn * lda, lda, strideA, 0, batch, stream)))
             return e;
-        if (ahead) {
+        if (side) {
             if (hipStreamWaitEvent(stream, aux->done, 0) != hipSuccess) return (int)hipErrorUnknown;
         } else if ((e = panel_dispatch<R>(A, lda, strideA, N, k0 + kb, kn, ipiv, strideP, info, batch, pc,
                                           stream))) {
