@@ -8,11 +8,14 @@ device with the boundary conditions already in HBM: assembly of S and K for
 the 40 wavenumbers (one launch), the 40 factorisations (batched), the 560
 solves, the potentials S @ sources and the generalised forces.
 
-Two forms are timed in the same run: the plain FP64 form (complex128 LU on
-FP64 MFMA) and the mixed-precision batched form (complex64 LU on FP32 MFMA,
-refined with FP64 residuals against the FP64 matrix).  `value` is the mixed
-form's throughput; the FP64 throughput, the ratio and the largest relative
-difference of any coefficient between the two forms are reported beside it.
+Three forms are timed in the same run: the plain FP64 form (complex128 LU
+on FP64 MFMA), the mixed-precision batched form (far panel pairs read from a
+float copy of the table; complex64 LU with FP16-input tile products, refined
+with FP64 residuals against the complex128 matrix), and the same mixed form
+on the FP64 assembly.  `value` is the mixed form's throughput; the FP64
+throughput, the ratio and the error of every coefficient family against the
+FP64 form (and, at the CPU sample's frequencies, against the oracle) are
+reported beside it.
 
 Prints one JSON line (see the repository's bench contract).
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline] [--no-fleet] [--no-serving]
@@ -66,25 +69,63 @@ def problems_by_frequency(body, omegas, headings, rho, g):
 
 def cpu_baseline(mesh, body, pbs_by_freq, table, n_freq=6):
     """The oracle (C restatement of the reference core, OpenMP on all host
-    cores) + scipy LU, on a bounded sample: the first and last frequencies
-    of the sweep, every problem of those frequencies."""
+    cores) + scipy LU, on a bounded sample: `n_freq` frequencies spread over
+    the sweep, every problem of those frequencies.  Returns the baseline
+    object and the oracle's forces (n_freq, nproblems, ndof) at the sample's
+    frequency indices."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from oracle import Oracle, mesh_arrays, solve_coefficients
     orc = Oracle()
     arr = mesh_arrays(mesh)
     idx = list(np.linspace(0, len(pbs_by_freq) - 1, n_freq).round().astype(int))
+    dofs = list(body.dofs)
+    forces = np.zeros((len(idx), len(pbs_by_freq[0]), len(dofs)), dtype=complex)
     t0 = time.perf_counter()
     nprob = 0
-    for i in idx:
+    for s, i in enumerate(idx):
         pbs = pbs_by_freq[i]
         S, K = orc.build_matrices(**arr, wavenumber=pbs[0].wavenumber, depth=np.inf, table=table)
-        solve_coefficients(S, K, body, pbs)
+        for r, f in enumerate(solve_coefficients(S, K, body, pbs)):
+            forces[s, r] = [f[d] for d in dofs]
         nprob += len(pbs)
     dt = time.perf_counter() - t0
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return dict(value=nprob / dt, unit="problems/s", cores=threads, kind="port",
-                sample=f"{len(idx)} of 40 frequencies ({nprob} problems), oracle assembly + scipy LU, "
-                       f"{dt:.1f} s")
+                sample=f"{len(idx)} of {len(pbs_by_freq)} frequencies ({nprob} problems), oracle assembly + "
+                       f"scipy LU, {dt:.1f} s"), idx, forces
+
+
+def coefficient_errors(f, ref, omegas, ndof):
+    """Errors of every hydrodynamic coefficient of `f` against `ref`, both
+    (frequencies, problems, ndof) forces with the ndof radiation problems
+    first (radiating dof order) and the diffraction problems after them.
+    Added mass Re F / omega^2, radiation damping Im F / omega and the
+    diffraction force: each coefficient's |error| relative to the largest
+    |coefficient| of its family at its frequency; `by_dof` is the largest
+    over frequencies and problems for each influenced dof, `diagonal_own`
+    the error of each diagonal added mass and damping relative to itself
+    (where it is at least 1e-3 of its family's largest)."""
+    w = np.asarray(omegas, float)[:, None, None]
+    fams = {"added_mass": (f[:, :ndof].real / w ** 2, ref[:, :ndof].real / w ** 2),
+            "radiation_damping": (f[:, :ndof].imag / w, ref[:, :ndof].imag / w),
+            "diffraction_force": (f[:, ndof:], ref[:, ndof:])}
+    out = {}
+    for name, (x, x0) in fams.items():
+        if x0.shape[1] == 0:
+            continue
+        scale = np.abs(x0).max(axis=(1, 2), keepdims=True)
+        e = np.abs(x - x0) / np.maximum(scale, 1e-300)
+        row = {"max": float(e.max()), "by_dof": [float(v) for v in e.max(axis=(0, 1))]}
+        if name != "diffraction_force":
+            # each diagonal coefficient against itself, where it is at least
+            # 1e-3 of its family's largest (a vanishing coefficient, such as
+            # a yaw damping, has no relative error to speak of)
+            d = np.arange(ndof)
+            big = np.abs(x0[:, d, d]) >= 1e-3 * scale[:, :, 0]
+            own = np.where(big, np.abs(x[:, d, d] - x0[:, d, d]) / np.maximum(np.abs(x0[:, d, d]), 1e-300), 0.0)
+            row["diagonal_own"] = [float(v) for v in own.max(axis=0)]
+        out[name] = row
+    return out
 
 
 def measured_traffic(kernel):
@@ -188,40 +229,42 @@ def fleet_benchmark(gf, dev, rho=1025.0, g=9.81, seed=0, budget_gb=110.0, log=No
 
 
 def serving_benchmark(gf, dev, seed=1, requests=40, loads=(0.5, 0.8, 1.0, 1.2), log=None):
-    """Poisson request stream against the continuous-batching server:
-    saturation throughput from a burst of `requests` requests, then
-    latency percentiles at arrival rates of `loads` x saturation."""
+    """Poisson request stream against the continuous-batching server.  One
+    mix of `requests` requests is drawn once: its burst (all submitted at
+    once) gives the saturation throughput, and the same mix, in the same
+    order, arrives at Poisson rates of `loads` x saturation for the latency
+    percentiles (p50, p90, p99 and the largest latency of each load point)."""
     import threading
     from capytaine_amd import workloads
     from capytaine_amd.serving import BEMServer
-
-    def make(rng, count):
-        return [workloads.serving_request(rng) for _ in range(count)]
 
     def problems(reqs):
         return sum(len(om) * (len(b.dofs) + len(hd)) for b, om, hd in reqs)
 
     rng = np.random.default_rng(seed)
+    mix = [workloads.serving_request(rng) for _ in range(requests)]
+    sizes = np.array([b.mesh.nb_faces for b, _, _ in mix])
     out = {"workload": f"hull requests (seed {seed}): 500-12000 panels log-uniform, 1-32 frequencies in 0.2-3.0 "
-                       "rad/s, random dof subset, 0-8 headings; mixed precision; Poisson arrivals"}
+                       "rad/s, random dof subset, 0-8 headings; mixed precision; Poisson arrivals",
+           "requests_per_load_point": requests,
+           "mix": {"panels_median": int(np.median(sizes)), "panels_max": int(sizes.max()),
+                   "problems": problems(mix)}}
     with BEMServer(gf, rho=1025.0) as srv:
         # warm: first use of the slot contexts
         b, om, hd = workloads.serving_request(np.random.default_rng(seed + 1000), n_range=(500, 800))
         srv.submit(b, om[:1], hd).result()
-        burst = make(rng, requests)
         t0 = time.perf_counter()
-        futs = [srv.submit(*r) for r in burst]
+        futs = [srv.submit(*r) for r in mix]
         for f in futs:
             f.result()
         span = time.perf_counter() - t0
         sat = requests / span
         out["saturation_requests_per_s"] = sat
-        out["saturation_solves_per_s"] = problems(burst) / span
+        out["saturation_solves_per_s"] = problems(mix) / span
         if log:
             log(f"[serving] saturation {sat:.2f} requests/s ({span:.2f} s for {requests})")
         out["loads"] = []
         for load in loads:
-            reqs = make(rng, requests)
             gaps = rng.exponential(1.0 / (load * sat), size=requests)
             arrivals = np.cumsum(gaps) - gaps[0]
             n_before = len(srv.latencies)
@@ -229,7 +272,7 @@ def serving_benchmark(gf, dev, seed=1, requests=40, loads=(0.5, 0.8, 1.0, 1.2), 
 
             def client():
                 start = time.perf_counter()
-                for r, t in zip(reqs, arrivals):
+                for r, t in zip(mix, arrivals):
                     delay = start + t - time.perf_counter()
                     if delay > 0:
                         time.sleep(delay)
@@ -243,23 +286,24 @@ def serving_benchmark(gf, dev, seed=1, requests=40, loads=(0.5, 0.8, 1.0, 1.2), 
                 f.result()
             span = time.perf_counter() - t0
             lat = np.array(srv.latencies[n_before:])
-            row = {"load": load, "requests": requests, "p50_s": float(np.percentile(lat, 50)),
-                   "p99_s": float(np.percentile(lat, 99)), "requests_per_s": requests / span,
-                   "solves_per_s": problems(reqs) / span}
+            row = {"load": load, "requests": requests, "offered_requests_per_s": load * sat,
+                   "p50_s": float(np.percentile(lat, 50)), "p90_s": float(np.percentile(lat, 90)),
+                   "p99_s": float(np.percentile(lat, 99)), "max_s": float(lat.max()),
+                   "requests_per_s": requests / span, "solves_per_s": problems(mix) / span}
             out["loads"].append(row)
             if log:
-                log(f"[serving] load {load}: p50 {row['p50_s']:.3f} s p99 {row['p99_s']:.3f} s, "
-                    f"{row['solves_per_s']:.0f} solves/s")
+                log(f"[serving] load {load}: p50 {row['p50_s']:.3f} s p90 {row['p90_s']:.3f} s p99 "
+                    f"{row['p99_s']:.3f} s max {row['max_s']:.3f} s, {row['solves_per_s']:.0f} solves/s")
         out["max_inflight_batches"] = srv.max_inflight_batches
     return out
 
 
-def serving_child(timeout_s=900):
+def serving_child(timeout_s=900, requests=40):
     """serving_benchmark in a child process (same interpreter, this file
     with --workload serving): its JSON line, or the exit status when the
     child ends without one.  Its progress lines go to this process's stderr."""
     import subprocess
-    cmd = [sys.executable, "-u", os.path.abspath(__file__), "--workload", "serving"]
+    cmd = [sys.executable, "-u", os.path.abspath(__file__), "--workload", "serving", "--requests", str(requests)]
     try:
         p = subprocess.run(cmd, stdout=subprocess.PIPE, text=True, timeout=timeout_s)
     except subprocess.TimeoutExpired:
@@ -529,6 +573,11 @@ def main():
     ap.add_argument("--panel-threads", type=int, default=None, choices=(256, 512),
                     help="force the panel leaf work-group size of both forms (cpt_set_tuning "
                          "panel_threads) after tuning, for profiler passes of one variant")
+    ap.add_argument("--requests", type=int, default=40,
+                    help="requests of the serving mix (drawn once; the saturation burst and every load point "
+                         "use the same mix)")
+    ap.add_argument("--loads", type=str, default="0.5,0.8,1.0,1.2",
+                    help="arrival rates of the serving measurement, as fractions of its saturation throughput")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner (a stored tuning of this size class is still applied)")
     ap.add_argument("--workload", choices=("barge", "cylinders", "copy", "serving", "post", "symmetric"),
@@ -554,7 +603,9 @@ def main():
         import capytaine_amd as cpa
         dev = torch.device("cuda", 0)
         torch.cuda.set_device(dev)
-        res = serving_benchmark(cpa.Delhommeau(device=dev), dev, log=lambda m: print(m, file=sys.stderr, flush=True))
+        res = serving_benchmark(cpa.Delhommeau(device=dev), dev, requests=args.requests,
+                                loads=tuple(float(v) for v in args.loads.split(",")),
+                                log=lambda m: print(m, file=sys.stderr, flush=True))
         print(json.dumps(res), flush=True)
         return
 
@@ -607,7 +658,7 @@ def main():
         tuning["assembly"] = autotune.tune_assembly(gf, mesh, ks, log=log)
         torch.cuda.empty_cache()
 
-    def run(precision, far_field_fp32=False):
+    def run(precision, far_field_fp32=None):
         applied = autotune.apply(ctx, n, len(ks), precision, nb_wavenumbers=len(ks))
         if args.gemm_tiles is not None and precision == "mixed":
             _lib.set_tuning(ctx, gemm_tiles=args.gemm_tiles)
@@ -660,8 +711,8 @@ def main():
         print(f"[bench] {args.only}: {r['seconds'] / args.steps * 1e3:.1f} ms per step", file=sys.stderr, flush=True)
         return
     r64 = run("fp64")
-    rmx = run("mixed")
-    rff = run("mixed", far_field_fp32=True)
+    rmx = run("mixed")  # far pairs from the float table: the mixed form's default
+    rfa = run("mixed", far_field_fp32=False)  # the same on the FP64 assembly
     asm_err = assembly_error(gf, mesh, [ks[0], ks[len(ks) // 2], ks[-1]], _lib)
     nb = len(ks)
     nprob = nb * nrhs
@@ -671,7 +722,7 @@ def main():
     f64, fmx = r64["forces"], rmx["forces"]
     scale = np.abs(f64).max(axis=2, keepdims=True)  # per problem
     mixed_err = float(np.max(np.abs(fmx - f64) / scale))
-    ff_err = float(np.max(np.abs(rff["forces"] - f64) / scale))
+    fa_err = float(np.max(np.abs(rfa["forces"] - f64) / scale))
     t_asm = r64["phases"]["assembly"]
     t_lu = r64["phases"]["lu_factor"]
     pairs_per_s = nb * n * n / t_asm
@@ -740,7 +791,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "c64 factor" + (" (FP16-input tile products)" if ctiles == 4 else "") +
-                 " + f64 refinement (mixed); f64 reported beside",
+                 " + f64 refinement, far pairs from an f32 table (mixed); f64 reported beside",
         "data": "synthetic",
         "config": {"workload": "barge 20x8x2 m, 4352 panels, 40 frequencies 0.2-2.5 rad/s, 6 dof + 8 headings "
                                "(560 problems), infinite depth; mixed-precision batched form vs FP64 plain form",
@@ -749,13 +800,15 @@ def main():
         "fp64_ms_per_step": r64["seconds"] / args.steps * 1e3,
         "mixed_over_fp64": valuemx / value64,
         "mixed_max_rel_err_vs_fp64": mixed_err,
-        "mixed_fp32_far_field": {
-            "problems_per_s": nprob * args.steps / rff["seconds"],
-            "phase_ms_per_step": {k: v * 1e3 for k, v in rff["phases"].items()},
-            "assembly_panel_pairs_per_s": nb * n * n / rff["phases"]["assembly"],
-            "max_rel_err_vs_fp64": ff_err,
-            "assembly_max_rel_err": {"S": asm_err["S"], "K": asm_err["K"],
-                                     "wavenumbers": "first, middle and last of the sweep"}},
+        "coefficient_errors_mixed_vs_fp64": coefficient_errors(fmx, f64, omegas_arr, len(body.dofs)),
+        "far_field_assembly_max_rel_err": {"S": asm_err["S"], "K": asm_err["K"],
+                                           "wavenumbers": "first, middle and last of the sweep",
+                                           "assembly_panel_pairs_per_s": nb * n * n / rmx["phases"]["assembly"]},
+        "mixed_fp64_assembly": {
+            "problems_per_s": nprob * args.steps / rfa["seconds"],
+            "phase_ms_per_step": {k: v * 1e3 for k, v in rfa["phases"].items()},
+            "assembly_panel_pairs_per_s": nb * n * n / rfa["phases"]["assembly"],
+            "max_rel_err_vs_fp64": fa_err},
         "mixed_refinement_sweeps": rmx["iterations"],
         "mixed_final_rel_residual": rmx["residual"],
         "assembly_panel_pairs_per_s": pairs_per_s,
@@ -781,12 +834,16 @@ def main():
         _lib.check(_lib.load().cpt_release_workspace(ctx.handle), "cpt_release_workspace")
         torch.cuda.synchronize()
         torch.cuda.empty_cache()
-        out["serving"] = serving_child()
+        out["serving"] = serving_child(requests=args.requests)
         loads = out["serving"].get("loads", [])
         out["p99_latency_s_at_saturation"] = next((r["p99_s"] for r in loads if r["load"] == 1.0), None)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(mesh, body, pbs_by_freq,
-                                           (gf.tabulated_r_range, gf.tabulated_z_range, gf.tabulated_integrals))
+        out["cpu_baseline"], idx, f_orc = cpu_baseline(
+            mesh, body, pbs_by_freq, (gf.tabulated_r_range, gf.tabulated_z_range, gf.tabulated_integrals))
+        out["coefficient_errors_vs_oracle"] = {
+            "frequencies_rad_s": [float(omegas_arr[i]) for i in idx],
+            "fp64": coefficient_errors(f64[idx], f_orc, omegas_arr[idx], len(body.dofs)),
+            "mixed": coefficient_errors(fmx[idx], f_orc, omegas_arr[idx], len(body.dofs))}
     print(json.dumps(out), flush=True)
 
 

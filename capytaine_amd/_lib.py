@@ -58,8 +58,9 @@ SIGNATURES = {
 # Tuning parameters of cpt_set_tuning, in the order of the CPT_TUNE_* values
 # (None: a retired value).
 TUNING_PARAMS = ("lu_block", "panel_leaf", "lookahead", "assembly_chunk", None, "gemm_tiles",
-                 "lookahead_active", "panel_threads", "mixed_retries", "panel_wgs", "panel_spin")
-READ_ONLY_TUNING = ("lookahead_active", "mixed_retries")
+                 "lookahead_active", "panel_threads", "mixed_retries", "panel_wgs", "panel_spin",
+                 "pivot_speculation", "spec_repairs")
+READ_ONLY_TUNING = ("lookahead_active", "mixed_retries", "spec_repairs")
 
 # Kernel classes of cpt_profile_end, in the order of the CPT_TIMED_* values.
 TIMED_KERNELS = ("zgemm", "cgemm", "panel_z", "panel_c", "assembly", "skinny_z", "skinny_c")
@@ -237,6 +238,13 @@ def mixed_retries(ctx):
     """Mixed solves (or groups) of the context factored again with
     FP32-input products after FP16-input products fell short."""
     return _get_one(ctx, "mixed_retries")
+
+
+def spec_repairs(ctx):
+    """Panels of the context's factorisations factored again with partial
+    pivoting after the speculative panel's check failed (waits for the
+    device)."""
+    return _get_one(ctx, "spec_repairs")
 
 
 def get_tuning(ctx):

@@ -6,7 +6,8 @@ problems (solver.py:268-372 -> _solve :145-215 per problem), on the device:
 
   1. one persistent assembly launch over every (body, wavenumber) pair
      (cpt_build_matrices_batched: a work-list of (body, row block, panel
-     chunk) tiles, most expensive first), the far field optionally in FP32;
+     chunk) tiles, most expensive first), far pairs of the mixed form from the
+     float table;
   2. a variable-size batched LU: one group per body (its matrices share the
      body's order), the groups running concurrently on the library's lane
      streams -- complex128 (cpt_lu_factor_grouped + cpt_lu_solve_grouped) or
@@ -58,7 +59,7 @@ class BatchedSweep:
     """
 
     def __init__(self, green_function, bodies, wavenumbers, nrhs, *, precision="mixed", max_iter=10, tol=1e-12,
-                 free_surface=0.0, context=None, far_field_fp32=False):
+                 free_surface=0.0, context=None, far_field_fp32=None):
         import torch
         if precision not in ("fp64", "mixed"):
             raise ValueError(f"precision must be 'fp64' or 'mixed', got {precision!r}")
@@ -73,8 +74,10 @@ class BatchedSweep:
         if len(self.ks) != nbody or len(self.nrhs) != nbody:
             raise ValueError("one wavenumber list and one nrhs per body")
         self.precision = precision
-        # FP32 far field on request (mixed form only)
-        self.far_field_fp32 = bool(far_field_fp32) and precision == "mixed"
+        # mixed form: far pairs read the float copy of the table (FP64
+        # arithmetic; coefficients within ~1e-7 of the FP64 assembly) unless
+        # far_field_fp32 is False
+        self.far_field_fp32 = precision == "mixed" and far_field_fp32 is not False
         self.max_iter, self.tol = max_iter, tol
         self.free_surface = free_surface
         self.meshes = [b.mesh_including_lid for b in self.bodies]

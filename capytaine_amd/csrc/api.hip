@@ -157,7 +157,7 @@ struct cpt_context {
     int device = 0;
     // tabulation
     cpt::DevBuf r_range, z_range, table;
-    cpt::DevBuf r_range_f, z_range_f, table_f;  // float copies (FP32 far field)
+    cpt::DevBuf r_range_f, z_range_f, table_f;  // float copies (far field of the mixed form)
     int far_fp32 = 0;                           // mixed-precision assembly
     int nr = 0, nz = 0, grid = 1, npts = 1001;
     double rmax = 0, zmax_abs = 0, dz = 0;
@@ -459,6 +459,14 @@ int cpt_create(int device, cpt_handle* out)
     if (scratch_init(device)) return CPT_ERR_HIP;
     cpt_context* h = new cpt_context();
     h->device = device;
+    // device counter of speculative panels factored again (CPT_TUNE_SPEC_REPAIRS)
+    if (hipSetDevice(device) != hipSuccess || hipMalloc((void**)&h->lu_streams.spec_repairs, sizeof(int)) != hipSuccess ||
+        hipMemset(h->lu_streams.spec_repairs, 0, sizeof(int)) != hipSuccess) {
+        set_error("cpt_create: could not allocate the context's counters");
+        if (h->lu_streams.spec_repairs) (void)hipFree(h->lu_streams.spec_repairs);
+        delete h;
+        return CPT_ERR_HIP;
+    }
     *out = h;
     return CPT_OK;
 }
@@ -488,6 +496,7 @@ int cpt_destroy(cpt_handle h)
     if (h->lu_streams.side) (void)hipStreamDestroy(h->lu_streams.side);
     if (h->lu_streams.ready) (void)hipEventDestroy(h->lu_streams.ready);
     if (h->lu_streams.done) (void)hipEventDestroy(h->lu_streams.done);
+    if (h->lu_streams.spec_repairs) (void)hipFree(h->lu_streams.spec_repairs);
     h->timer.release();
     for (hipStream_t l : h->lanes) (void)hipStreamDestroy(l);
     for (hipEvent_t ev : h->lane_done) (void)hipEventDestroy(ev);
@@ -1264,6 +1273,10 @@ int cpt_set_tuning(cpt_handle h, int param, int value)
     case CPT_TUNE_PANEL_SPIN:
         h->lu_streams.panel_spin = value;
         return CPT_OK;
+    case CPT_TUNE_PIVOT_SPECULATION:
+        if (value != 0 && value != 1) break;
+        h->lu_streams.speculate = value;
+        return CPT_OK;
     default:
         set_error("cpt_set_tuning: unknown parameter " + std::to_string(param));
         return CPT_ERR_ARG;
@@ -1290,6 +1303,13 @@ int cpt_get_tuning(cpt_handle h, int param, int* value)
     case CPT_TUNE_PANEL_WGS: *value = h->lu_streams.panel_mw; return CPT_OK;
     case CPT_TUNE_PANEL_SPIN: *value = h->lu_streams.panel_spin; return CPT_OK;
     case CPT_TUNE_MIXED_RETRIES: *value = (int)std::min<long>(h->mixed_retries, 0x7fffffff); return CPT_OK;
+    case CPT_TUNE_PIVOT_SPECULATION: *value = h->lu_streams.speculate; return CPT_OK;
+    case CPT_TUNE_SPEC_REPAIRS:
+        // the device counter: waits for the device's work
+        CPT_CHECK_HIP(hipSetDevice(h->device), "hipSetDevice");
+        CPT_CHECK_HIP(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        CPT_CHECK_HIP(hipMemcpy(value, h->lu_streams.spec_repairs, sizeof(int), hipMemcpyDeviceToHost), "hipMemcpy");
+        return CPT_OK;
     default:
         set_error("cpt_get_tuning: unknown parameter " + std::to_string(param));
         return CPT_ERR_ARG;

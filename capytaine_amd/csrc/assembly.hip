@@ -12,7 +12,6 @@
 //     computed once per pair and reused for every wavenumber of the batch,
 //     with the same order of additions as the reference for each entry.
 #include "green.hpp"
-#include "green_f32.hpp"
 #include "cpt_internal.hpp"
 
 namespace cpt {
@@ -28,7 +27,7 @@ struct WaveOut {
 // (z -> zsign*z + zshift) used by the finite-depth images.
 __device__ __forceinline__ void wave_panel(const Table& T, double k, int gfs, bool deriv_first, double x0, double x1,
                                            double x2, double cx, double cy, double cz, double area, const double* qp,
-                                           int nq, double zsign, double zshift, WaveOut& o)
+                                           int nq, double zsign, double zshift, WaveOut& o, bool ftab = false)
 {
     const double czi = zsign * cz + zshift;
     const double dxc = x0 - cx, dyc = x1 - cy;
@@ -43,7 +42,7 @@ __device__ __forceinline__ void wave_panel(const Table& T, double k, int gfs, bo
         for (int q = 0; q < nq; ++q) {
             const double qx = qp[4 * q], qy = qp[4 * q + 1], qz = zsign * qp[4 * q + 2] + zshift, w = qp[4 * q + 3];
             cplx g, ng[3];
-            wave_point(T, k, gfs, x0 - qx, x1 - qy, x2 + qz, g, ng);
+            wave_point(T, k, gfs, x0 - qx, x1 - qy, x2 + qz, g, ng, ftab);
             o.G.re += g.re * w;
             o.G.im += g.im * w;
 #pragma unroll
@@ -109,90 +108,6 @@ __device__ __forceinline__ void wave_panel_finite_depth(const Table& T, double k
     }
 }
 
-// Far pair (r0 > 7 R, infinite depth) of the mixed-precision assembly: the
-// one-point Rankine and reflected-Rankine terms and the wave term in FP32,
-// summed per wavenumber in the order of the FP64 path and stored in
-// complex128.  Coordinate differences are taken in FP64 before rounding.
-__device__ __forceinline__ void far_pair_f32(const AssemblyArgs& A, int I, int J, bool active, double x0, double x1,
-                                             double x2, double cx, double cy, double cz, double area,
-                                             const double* qp, double nx, double ny, double nz, bool deriv_first)
-{
-    const float af = (float)area;
-    float gd, gdx, gdy, gdz, gr, grx, gry, grz;
-    one_point_rankine_f(0.0f, 0.0f, 0.0f, (float)(cx - x0), (float)(cy - x1), (float)(cz - x2), af, gd, gdx, gdy,
-                        gdz);
-    // image point (x, y, -z): Green_Rankine.f90:202-338 with a = -1, b = 0
-    one_point_rankine_f(0.0f, 0.0f, 0.0f, (float)(cx - x0), (float)(cy - x1), (float)(cz + x2), af, gr, grx, gry,
-                        grz);
-    grz = -grz;
-    if (!deriv_first) {
-        gdx = -gdx;
-        gdy = -gdy;
-        gdz = -gdz;
-        grx = -grx;
-        gry = -gry;
-    }
-    for (int b = 0; b < A.nb; ++b) {
-        const FreqArgs& P = A.freq[b];
-        const float sgn = (P.gfs == HIGH_FREQ) ? -1.0f : 1.0f;
-        float Gre = gd + sgn * gr, Gim = 0.0f;
-        float n0r = gdx + sgn * grx, n1r = gdy + sgn * gry, n2r = gdz + sgn * grz;
-        float n0i = 0.0f, n1i = 0.0f, n2i = 0.0f;
-        if (P.gfs == LOW_FREQ_WITH_RANKINE_PART) n2r += 2 * (float)P.k * gr;
-        if (P.wave) {
-            float wGr = 0.0f, wGi = 0.0f, w0r = 0.0f, w0i = 0.0f, w1r = 0.0f, w1i = 0.0f, w2r = 0.0f, w2i = 0.0f;
-            for (int q = 0; q < A.nq; ++q) {
-                const double qx = qp[4 * q], qy = qp[4 * q + 1], qz = qp[4 * q + 2];
-                const float w = (float)qp[4 * q + 3];
-                cplxf g, ng[3];
-                wave_point_f(A.T, P.k, P.gfs, x0 - qx, x1 - qy, x2 + qz, g, ng);
-                wGr += g.re * w;
-                wGi += g.im * w;
-                w0r += ng[0].re * w;
-                w0i += ng[0].im * w;
-                w1r += ng[1].re * w;
-                w1i += ng[1].im * w;
-                w2r += ng[2].re * w;
-                w2i += ng[2].im * w;
-            }
-            if (!deriv_first) {
-                w0r = -w0r;
-                w0i = -w0i;
-                w1r = -w1r;
-                w1i = -w1i;
-            }
-            Gre += wGr;
-            Gim += wGi;
-            n0r += w0r;
-            n0i += w0i;
-            n1r += w1r;
-            n1i += w1i;
-            n2r += w2r;
-            n2i += w2i;
-        }
-        if (active) {
-            const size_t ij = (size_t)I + (size_t)A.ldS * J;
-            cplx* S = A.S + (size_t)b * A.strideS;
-            S[ij].re = M1_4PI * (double)Gre;
-            S[ij].im = M1_4PI * (double)Gim;
-            cplx* K = A.K + (size_t)b * A.strideK;
-            if (A.nk == 1) {
-                const size_t kij = (size_t)I + (size_t)A.ldK * J;
-                K[kij].re = M1_4PI * (nx * (double)n0r + ny * (double)n1r + nz * (double)n2r);
-                K[kij].im = M1_4PI * (nx * (double)n0i + ny * (double)n1i + nz * (double)n2i);
-            } else {
-                const size_t kij = 3 * ((size_t)I + (size_t)A.ldK * J);
-                K[kij + 0].re = M1_4PI * (double)n0r;
-                K[kij + 0].im = M1_4PI * (double)n0i;
-                K[kij + 1].re = M1_4PI * (double)n1r;
-                K[kij + 1].im = M1_4PI * (double)n1i;
-                K[kij + 2].re = M1_4PI * (double)n2r;
-                K[kij + 2].im = M1_4PI * (double)n2i;
-            }
-        }
-    }
-}
-
 // One tile: collocation points iblk * 256 + threadIdx.x against panels
 // jc * j_chunk .. + j_chunk, for every wavenumber of A.
 template <bool FINITE_DEPTH, bool FAR32 = false>
@@ -211,7 +126,7 @@ __global__ void __launch_bounds__(256) assemble_finite_depth_kernel(AssemblyArgs
     assemble_tile<true>(A, blockIdx.x, blockIdx.y);
 }
 
-// Mixed-precision form: far pairs in FP32.
+// Mixed-precision form: far pairs read the float table.
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 assemble_infinite_depth_mixed_kernel(AssemblyArgs A)
 {
@@ -257,16 +172,15 @@ __device__ __forceinline__ void assemble_tile(const AssemblyArgs& A, int iblk, i
             ny = A.dpn[3 * (size_t)J + 1];
             nz = A.dpn[3 * (size_t)J + 2];
         }
+        // mixed form: a far pair (outside the reference's near-field radius
+        // for both the direct and the reflected Rankine terms,
+        // Green_Rankine.f90:50 and :224) reads the float table
+        bool ftab = false;
         if (FAR32) {
-            // the reference's near-field tests (Green_Rankine.f90:50 and
-            // its reflected form), with the FP64 path's expressions
             const double r0 = norm3(x0 - cx, x1 - cy, x2 - cz);
             const double r1 = sqrt((x0 - cx) * (x0 - cx) + (x1 - cy) * (x1 - cy) + (-x2 - cz) * (-x2 - cz));
-            if (far_direct(x0 - cx, x1 - cy, x2 - cz, r0, radius) &&
-                far_image(x0 - cx, x1 - cy, -x2 - cz, r1, radius)) {
-                far_pair_f32(A, I, J, active, x0, x1, x2, cx, cy, cz, area, qp, nx, ny, nz, deriv_first);
-                continue;
-            }
+            ftab = far_direct(x0 - cx, x1 - cy, x2 - cz, r0, radius) &&
+                   far_image(x0 - cx, x1 - cy, -x2 - cz, r1, radius);
         }
         // Rankine term (matrices.f90:122-134)
         double gd, gdx, gdy, gdz;
@@ -311,7 +225,7 @@ __device__ __forceinline__ void assemble_tile(const AssemblyArgs& A, int iblk, i
                 if (P.wave) {
                     WaveOut w;
                     wave_panel(A.T, P.k, P.gfs, deriv_first, x0, x1, x2, cx, cy, cz, area, qp, A.nq,
-                               1.0, 0.0, w);
+                               1.0, 0.0, w, ftab);
                     acc(o, w, false);
                 }
             } else {

@@ -338,11 +338,99 @@ __device__ __forceinline__ void asymptotic(double r, double z, double* out)
     out[4] = 2 * (-ez * (sk + ck * oor / 2));
 }
 
+// ---------------------------------------------------------------------------
+// Far field of the mixed-precision assembly: the wave term of a far panel
+// pair read from an FP32 copy of the table.
+//
+// For a panel pair outside the reference's near-field radius (r0 > 7 R and
+// r1 > 7 R, where Green_Rankine.f90:50 and :224 switch to the one-point
+// formulas) the infinite-depth wave term (Green_wave.f90:162-271 with
+// Delhommeau_integrals.f90:274-402) gathers its 3 x 3 x 5 stencil from the
+// float table (half the bytes of the double one) and finds the stencil with
+// FP32 logarithms.  The Lagrange weights and the interpolation are FP64:
+// an error census of the 4352-panel barge (FP32 variants of each far-pair
+// term against the FP64 pipeline, 4 frequencies) put the coefficient error
+// of FP32 weights and sums at 1.1e-6 (the weights of a 0.2-wide r cell from
+// an FP32 r near 13 are wrong in the 6th digit), and that of float table
+// values with FP64 arithmetic at 1e-7.
+//
+// The stencil is the reference's: a nearest-index position whose FP32 value
+// lies within 1e-3 of a rounding boundary is recomputed in FP64.
+
+// Nearest index of Fortran NINT / INT of a position p computed in FP32,
+// recomputed from the FP64 position when p is within 1e-3 of where the
+// rounding changes.
+__device__ __forceinline__ bool near_boundary(float p, bool nint)
+{
+    const float f = p - floorf(p);
+    return nint ? fabsf(f - 0.5f) < 1e-3f : (f < 1e-3f || f > 1.0f - 1e-3f);
+}
+
+__device__ __forceinline__ void table_index_f(const Table& T, double r, double z, int& i, int& j)
+{
+    const float rf = (float)r, azf = fabsf((float)z);
+    if (T.grid == LEGACY_GRID) {
+        if (r < (double)1e-6f) {
+            i = 1;
+        } else if (r < 1.0) {
+            const float p = 5 * (log10f(rf) + 6) + 1;
+            i = near_boundary(p, false) ? f_int(5 * (log10(r) + 6) + 1) : (int)p;
+        } else {
+            i = f_int(3 * r + 28);
+        }
+        const double az = fabs(z);
+        if (az > (double)1e-2f) {
+            const float p = 8 * (log10f(azf) + 4.5f);
+            j = near_boundary(p, false) ? f_int(8 * (log10(az) + 4.5)) : (int)p;
+        } else {
+            const float p = 5 * (log10f(azf) + 6);
+            j = near_boundary(p, false) ? f_int(5 * (log10(az) + 6)) : (int)p;
+        }
+    } else {
+        if (r < (double)1e-10f) {
+            i = 1;
+        } else if (r < 1.0) {
+            const float p = (log10f(rf) / 10.0f + 1.0f) * (float)(T.index_of_1 - 1) + 1.0f;
+            i = near_boundary(p, true) ? f_nint((log10(r) / 10.0 + 1.0) * (T.index_of_1 - 1) + 1)
+                                       : (int)roundf(p);
+        } else {
+            i = f_nint((r - 1) * (T.nr - T.index_of_1) / (T.rmax - 1) + T.index_of_1);
+        }
+        const float p = (log10f(azf) + 10.0f) / (float)T.dz;
+        j = near_boundary(p, true) ? f_nint((log10(fabs(z)) + 10) / T.dz) : (int)roundf(p);
+    }
+    i = max(2, min(T.nr - 1, i)) - 1;
+    j = max(2, min(T.nz - 1, j)) - 1;
+}
+
+// pick_in_table with the stencil's values read from the float table.
+__device__ __forceinline__ void pick_in_float_table(const Table& T, double r, double z, double* out)
+{
+    int i, j;
+    table_index_f(T, r, z, i, j);
+    const double r0 = T.r_range[i - 1], r1 = T.r_range[i], r2 = T.r_range[i + 1];
+    const double z0 = T.z_range[j - 1], z1 = T.z_range[j], z2 = T.z_range[j + 1];
+    const double xl0 = pl2(r1, r2, r0, r), xl1 = pl2(r2, r0, r1, r), xl2 = pl2(r0, r1, r2, r);
+    const double zl[3] = {pl2(z1, z2, z0, z), pl2(z2, z0, z1, z), pl2(z0, z1, z2, z)};
+#pragma unroll
+    for (int c = 0; c < 5; ++c) out[c] = 0.0;
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+        const float* t = T.tab_f + ((size_t)(j - 1 + b) * T.nr + (i - 1)) * 5;
+        const double w = zl[b];
+#pragma unroll
+        for (int c = 0; c < 5; ++c)
+            out[c] += w * (xl0 * (double)t[c] + xl1 * (double)t[5 + c] + xl2 * (double)t[10 + c]);
+    }
+}
+
 // WAVE_PART_INFINITE_DEPTH, Green_wave.f90:162-271, for a point x and a
 // source point xi (already differenced in the horizontal plane): dx, dy are
 // x - xi, zsum is x3 + xi3.  Gradient with respect to x.
+// With ftab, the table's values come from its float copy (far pairs of the
+// mixed-precision assembly).
 __device__ __forceinline__ void wave_point(const Table& T, double k, int gfs, double dx, double dy, double zsum,
-                                           cplx& G, cplx* nG)
+                                           cplx& G, cplx* nG, bool ftab = false)
 {
     const double r = k * sqrt(dx * dx + dy * dy);
     const double z = k * zsum;
@@ -353,8 +441,12 @@ __device__ __forceinline__ void wave_point(const Table& T, double k, int gfs, do
         drdx2 = k * k * dy / r;
     }
     double I[5];
-    if (fabs(z) < T.zmax_abs && r < T.rmax) pick_in_table(T, r, z, I);
-    else asymptotic(r, z, I);
+    if (fabs(z) < T.zmax_abs && r < T.rmax) {
+        if (ftab) pick_in_float_table(T, r, z, I);
+        else pick_in_table(T, r, z, I);
+    } else {
+        asymptotic(r, z, I);
+    }
     double gre, gim = I[2];
     double n0r, n1r, n2r, n2i;
     if (gfs == HIGH_FREQ) {

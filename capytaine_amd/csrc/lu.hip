@@ -1882,6 +1882,234 @@ gemm_h16_kernel(int M, int N, int K, float alpha, const cx<float>* __restrict__ 
         }
 }
 
+// ---------------------------------------------------------------------------
+// 1c. Speculative panel: the factorisation with the pivots it would take
+//     known in advance, checked against LAPACK's pivot rule.
+//
+// On the BEM influence matrices (second-kind operators, K = I/2 + a compact
+// part) partial pivoting interchanges no rows: at every column j of the
+// elimination the diagonal entry has the largest |Re|+|Im| of the column
+// (on the configured bodies the largest entry below it stays under 0.35 of
+// it).  A panel is then factored with the diagonal as pivots: no pivot
+// search, and the rows below each leaf's leading w x w block are
+// independent (L21 = A21 U11^-1, an MFMA-free but row-parallel product).
+// Every entry a_ij^(j) the elimination meets below a pivot u_jj is compared
+// with it exactly as the pivot search compares candidates: cabs1(a_ij^(j)) >
+// cabs1(u_jj), or a zero or non-number u_jj, means the search would have
+// taken another row.  A matrix whose panel fails the check has the panel
+// restored from a copy taken before the panel step and factored again with
+// partial pivoting by the repair kernel (LAPACK's rule column by column,
+// ties to the first row, zero columns reported in info), so the factor is
+// LAPACK's P A = L U in both cases, up to rounding.
+// ---------------------------------------------------------------------------
+constexpr int SPEC_KB_MAX = 4 * PB;  // widest panel (block width) the copy holds
+
+// copy of the panel columns [k0, k0 + kb), rows [k0, N), leading dimension
+// N - k0, per matrix
+template <typename R>
+__global__ void spec_backup_kernel(const cx<R>* __restrict__ Abase, int64_t lda, int64_t strideA, int N, int k0,
+                                   int kb, cx<R>* __restrict__ bak, int64_t strideBak)
+{
+    const int M = N - k0;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i >= M) return;
+    const size_t b = blockIdx.z;
+    bak[b * strideBak + i + (size_t)j * M] = Abase[b * strideA + (k0 + i) + (size_t)(k0 + j) * lda];
+}
+
+// Leading w x w block (w <= PB) of the leaf at column c0: factored without
+// interchanges in LDS, checked, written back with ipiv = identity; U11^-1
+// (upper part, diagonal included) into Ubase (PB x PB per matrix).
+template <typename R>
+__global__ void __launch_bounds__(256) spec_top_kernel(cx<R>* __restrict__ Abase, int64_t lda, int64_t strideA,
+                                                       int c0, int w, int* __restrict__ ipiv_base, int64_t strideP,
+                                                       int* __restrict__ flags, cx<R>* __restrict__ Ubase)
+{
+    typedef cx<R> C;
+    __shared__ C sT[PB * PB];
+    __shared__ int sBad;
+    const int b = blockIdx.x;
+    if (flags[b]) return;  // the panel is factored again anyway
+    C* A = Abase + (size_t)b * strideA + c0 + (size_t)c0 * lda;
+    const int tid = threadIdx.x;
+    if (tid == 0) sBad = 0;
+    for (int e = tid; e < w * w; e += 256) sT[(e % w) + PB * (e / w)] = A[(e % w) + (size_t)(e / w) * lda];
+    __syncthreads();
+    for (int j = 0; j < w; ++j) {
+        const C d = sT[j + PB * j];
+        const R ad = cabs1(d);
+        if (tid > j && tid < w) {
+            const C a = sT[tid + PB * j];
+            if (!(ad > R(0)) || cabs1(a) > ad) sBad = 1;
+            sT[tid + PB * j] = zmul(a, zrcp(d));
+        }
+        if (tid == 0 && !(ad > R(0))) sBad = 1;
+        __syncthreads();
+        const int m = w - j - 1;
+        for (int e = tid; e < m * m; e += 256) {
+            const int i = j + 1 + e % m, k = j + 1 + e / m;
+            sT[i + PB * k] = zsub(sT[i + PB * k], zmul(sT[i + PB * j], sT[j + PB * k]));
+        }
+        __syncthreads();
+    }
+    if (sBad) {
+        if (tid == 0) flags[b] = 1;
+        return;
+    }
+    for (int e = tid; e < w * w; e += 256) A[(e % w) + (size_t)(e / w) * lda] = sT[(e % w) + PB * (e / w)];
+    int* ipiv = ipiv_base + (size_t)b * strideP;
+    if (tid < w) ipiv[c0 + tid] = c0 + tid;
+    __syncthreads();
+    // U11 as the upper triangle with reciprocal diagonal, then its inverse
+    for (int e = tid; e < w * w; e += 256) {
+        const int i = e % w, k = e / w;
+        if (i > k) sT[i + PB * k] = C{R(0), R(0)};
+        else if (i == k) sT[i + PB * k] = zrcp(sT[i + PB * k]);
+    }
+    __syncthreads();
+    invert_triangle<R, true>(sT, w, tid & 63, tid >> 6, Ubase + (size_t)b * PB * PB, PB);
+}
+
+// Rows [c0 + w, N) of the leaf's columns: L21 = A21 U11^-1 in place, each
+// entry checked as l_ij u_jj against u_jj.  64 rows per work-group, a
+// thread per (row, group of 16 columns).
+template <typename R>
+__global__ void __launch_bounds__(256) spec_rows_kernel(cx<R>* __restrict__ Abase, int64_t lda, int64_t strideA,
+                                                        int N, int c0, int w, int* __restrict__ flags,
+                                                        const cx<R>* __restrict__ Ubase)
+{
+    typedef cx<R> C;
+    __shared__ C sX[PB * PB];
+    __shared__ C sD[PB];
+    __shared__ R sAd[PB];
+    const int b = blockIdx.y;
+    if (flags[b]) return;
+    const int r0 = c0 + w + blockIdx.x * 64;
+    C* A = Abase + (size_t)b * strideA;
+    const C* X = Ubase + (size_t)b * PB * PB;
+    const int tid = threadIdx.x;
+    for (int e = tid; e < PB * PB; e += 256) {
+        const int i = e % PB, k = e / PB;
+        sX[e] = (i <= k && k < w) ? X[e] : C{R(0), R(0)};
+    }
+    if (tid < w) {
+        const C d = A[(c0 + tid) + (size_t)(c0 + tid) * lda];
+        sD[tid] = d;
+        sAd[tid] = cabs1(d);
+    }
+    __syncthreads();
+    const int r = r0 + (tid & 63), cg = tid >> 6;
+    C acc[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) acc[q] = C{R(0), R(0)};
+    const int kend = min(w, 16 * cg + 16);
+    if (r < N) {
+        for (int k = 0; k < kend; ++k) {
+            const C a = A[r + (size_t)(c0 + k) * lda];
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const C x = sX[k + PB * (16 * cg + q)];
+                acc[q].re += a.re * x.re - a.im * x.im;
+                acc[q].im += a.re * x.im + a.im * x.re;
+            }
+        }
+    }
+    __syncthreads();  // every thread of the row has read it before any writes
+    if (r >= N) return;
+    bool bad = false;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int c = 16 * cg + q;
+        if (c < w) {
+            A[r + (size_t)(c0 + c) * lda] = acc[q];
+            if (cabs1(zmul(acc[q], sD[c])) > sAd[c]) bad = true;
+        }
+    }
+    if (bad) atomicOr(flags + b, 1);
+}
+
+// Partial pivoting of the panel columns [k0, k0 + kb), rows [k0, N), of the
+// matrices whose check failed: the panel back from the copy, then LAPACK's
+// rule column by column (largest |Re|+|Im|, first row on ties; a column
+// without a nonzero candidate keeps its row and sets info), interchanges
+// within the panel, the scaled column and the rank-1 update of the panel's
+// remaining columns.  One work-group per matrix, global memory: the rare
+// path.  Clears the matrix's flag and counts the repair.
+template <typename R>
+__global__ void __launch_bounds__(512) spec_repair_kernel(cx<R>* __restrict__ Abase, int64_t lda, int64_t strideA,
+                                                          int N, int k0, int kb, const cx<R>* __restrict__ bak,
+                                                          int64_t strideBak, int* __restrict__ ipiv_base,
+                                                          int64_t strideP, int* __restrict__ info_base,
+                                                          int* __restrict__ flags, int* __restrict__ repairs)
+{
+    typedef cx<R> C;
+    constexpr int NT = 512, NW = NT / 64;
+    __shared__ R sBest[NW];
+    __shared__ int sIdx[NW];
+    const int b = blockIdx.x;
+    if (!flags[b]) return;
+    const int M = N - k0;
+    C* P = Abase + (size_t)b * strideA + k0 + (size_t)k0 * lda;
+    const C* S = bak + (size_t)b * strideBak;
+    int* ipiv = ipiv_base + (size_t)b * strideP;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (size_t e = tid; e < (size_t)M * kb; e += NT) P[(e % M) + (e / M) * lda] = S[e];
+    __syncthreads();
+    for (int j = 0; j < kb; ++j) {
+        R best = R(-1);
+        int bidx = INT_MAX;
+        for (int t = j + tid; t < M; t += NT) {
+            const R a = cabs1(P[t + (size_t)j * lda]);
+            if (a > best) {
+                best = a;
+                bidx = t;
+            }
+        }
+        wave_argmax(best, bidx);
+        if (lane == 0) {
+            sBest[wave] = best;
+            sIdx[wave] = bidx;
+        }
+        __syncthreads();
+        R bb = sBest[0];
+        int p = sIdx[0];
+#pragma unroll
+        for (int v = 1; v < NW; ++v)
+            if (sBest[v] > bb || (sBest[v] == bb && sIdx[v] < p)) {
+                bb = sBest[v];
+                p = sIdx[v];
+            }
+        const bool pivoted = bb > R(0);
+        if (!pivoted) p = j;
+        if (tid == 0) {
+            if (!pivoted && info_base[b] == 0) info_base[b] = k0 + j + 1;
+            ipiv[k0 + j] = k0 + p;
+        }
+        if (p != j)
+            for (int c = tid; c < kb; c += NT) {
+                const C x = P[j + (size_t)c * lda];
+                P[j + (size_t)c * lda] = P[p + (size_t)c * lda];
+                P[p + (size_t)c * lda] = x;
+            }
+        __syncthreads();
+        const C piv = P[j + (size_t)j * lda];
+        const bool nonzero = piv.re != R(0) || piv.im != R(0);
+        const C rr = nonzero ? zrcp(piv) : C{R(1), R(0)};
+        for (int t = j + 1 + tid; t < M; t += NT) {
+            C l = P[t + (size_t)j * lda];
+            if (nonzero) l = zmul(l, rr);
+            P[t + (size_t)j * lda] = l;
+            for (int c = j + 1; c < kb; ++c)
+                P[t + (size_t)c * lda] = zsub(P[t + (size_t)c * lda], zmul(l, P[j + (size_t)c * lda]));
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        flags[b] = 0;
+        if (repairs) atomicAdd(repairs, 1);
+    }
+}
+
 template <typename R, int NT, int RPT, int SB>
 void launch_panel(cx<R>* A, int64_t lda, int64_t sA, int N, int k0, int kb, int sw_lo, int sw_hi, int* ipiv,
                   int64_t sP, int* info, int batch, hipStream_t stream)
@@ -1903,6 +2131,13 @@ struct PanelCfg {
     int budget = MW_LEAF_BUDGET;
     int spin = 1 << 20;
     uint32_t seq = 1;
+    // speculative panels (1c): U11^-1 per matrix, the panel copy, the
+    // per-matrix check flags and the context's repair counter (may be null)
+    bool spec = false;
+    void* spec_uinv = nullptr;
+    void* spec_bak = nullptr;
+    int* spec_flags = nullptr;
+    int* spec_repairs = nullptr;
 };
 
 // Rows per work-group (512 threads with one row each, or two: 512 or 1024
@@ -2050,8 +2285,41 @@ int panel_rec(cx<R>* A, int64_t lda, int64_t sA, int N, int c0, int w, int sw_lo
     return panel_rec<R>(A, lda, sA, N, c0 + h, w - h, sw_lo, sw_hi, ipiv, sP, info, batch, pc, stream);
 }
 
+// Speculative factorisation (1c) of columns [c0, c0 + w) of rows [c0, N):
+// leaves of at most PB columns (leading block in LDS, rows below by the
+// inverse of U11), joined as in panel_rec by trsm + MFMA GEMM.
+template <typename R>
+int panel_rec_spec(cx<R>* A, int64_t lda, int64_t sA, int N, int c0, int w, int* ipiv, int64_t sP, int batch,
+                   const PanelCfg& pc, hipStream_t stream)
+{
+    int e;
+    if (w <= PB) {
+        cx<R>* U = (cx<R>*)pc.spec_uinv;
+        hipLaunchKernelGGL(spec_top_kernel<R>, dim3(batch), dim3(256), 0, stream, A, lda, sA, c0, w, ipiv, sP,
+                           pc.spec_flags, U);
+        const int rows = N - c0 - w;
+        if (rows > 0)
+            hipLaunchKernelGGL(spec_rows_kernel<R>, dim3((rows + 63) / 64, batch), dim3(256), 0, stream, A, lda, sA, N,
+                               c0, w, pc.spec_flags, (const cx<R>*)U);
+        return (int)hipGetLastError();
+    }
+    const int h = w / 2;
+    const int M = N - c0;
+    if ((e = panel_rec_spec<R>(A, lda, sA, N, c0, h, ipiv, sP, batch, pc, stream))) return e;
+    cx<R>* A11 = A + c0 + (size_t)c0 * lda;
+    cx<R>* A12 = A + c0 + (size_t)(c0 + h) * lda;
+    cx<R>* A21 = A + (c0 + h) + (size_t)c0 * lda;
+    cx<R>* A22 = A + (c0 + h) + (size_t)(c0 + h) * lda;
+    if ((e = trsm_lower<R>(A11, lda, sA, h, A12, lda, sA, w - h, batch, stream))) return e;
+    if (M > h && (e = gemm<R>(M - h, w - h, h, R(-1), A21, lda, sA, A12, lda, sA, A22, lda, sA, 0, batch, stream)))
+        return e;
+    return panel_rec_spec<R>(A, lda, sA, N, c0 + h, w - h, ipiv, sP, batch, pc, stream);
+}
+
 // LU with partial pivoting of the panel columns [k0, k0 + kb), rows [k0, N);
-// interchanges applied within the panel columns only.
+// interchanges applied within the panel columns only.  Speculative panels:
+// copy, factorisation with the diagonal pivots and its check, then the
+// repair of the matrices whose check failed.
 template <typename R>
 int panel_dispatch(cx<R>* A, int64_t lda, int64_t sA, int N, int k0, int kb, int* ipiv, int64_t sP, int* info,
                    int batch, PanelCfg& pc, hipStream_t stream)
@@ -2060,6 +2328,18 @@ int panel_dispatch(cx<R>* A, int64_t lda, int64_t sA, int N, int k0, int kb, int
     // complex LU of an M x kb panel: 4 (M kb^2 - kb^3 / 3) real flops
     TimedLaunch timed(stream, sizeof(R) == 8 ? TK_PANEL_Z : TK_PANEL_C,
                       4.0 * ((double)M * kb * kb - (double)kb * kb * kb / 3.0) * batch);
+    if (pc.spec && kb <= SPEC_KB_MAX) {
+        cx<R>* bak = (cx<R>*)pc.spec_bak;
+        const int64_t sBak = (int64_t)N * SPEC_KB_MAX;
+        hipLaunchKernelGGL(spec_backup_kernel<R>, dim3((M + 255) / 256, kb, batch), dim3(256), 0, stream,
+                           (const cx<R>*)A, lda, sA, N, k0, kb, bak, sBak);
+        int e;
+        if ((e = (int)hipGetLastError())) return e;
+        if ((e = panel_rec_spec<R>(A, lda, sA, N, k0, kb, ipiv, sP, batch, pc, stream))) return e;
+        hipLaunchKernelGGL(spec_repair_kernel<R>, dim3(batch), dim3(512), 0, stream, A, lda, sA, N, k0, kb,
+                           (const cx<R>*)bak, sBak, ipiv, sP, info, pc.spec_flags, pc.spec_repairs);
+        return (int)hipGetLastError();
+    }
     return panel_rec<R>(A, lda, sA, N, k0, kb, k0, k0 + kb, ipiv, sP, info, batch, pc, stream);
 }
 
@@ -2167,10 +2447,20 @@ size_t getrf_inv_elems(int batch)
     return (size_t)4 * PB * PB * (size_t)batch;
 }
 template <typename R>
-size_t getrf_ws_elems(int, int batch)
+size_t getrf_xbuf_elems(int batch)
 {
     const size_t xbytes = sizeof(uint64_t) * (size_t)MW_XWORDS * batch;
-    return getrf_inv_elems<R>(batch) + (xbytes + sizeof(cx<R>) - 1) / sizeof(cx<R>);
+    return (xbytes + sizeof(cx<R>) - 1) / sizeof(cx<R>);
+}
+// ... then for the speculative panels: U11^-1 (PB x PB per matrix), the
+// panel copy (N x SPEC_KB_MAX per matrix) and one 64-bit word of flag per
+// matrix
+template <typename R>
+size_t getrf_ws_elems(int N, int batch)
+{
+    const size_t flag_elems = (sizeof(uint64_t) * (size_t)batch + sizeof(cx<R>) - 1) / sizeof(cx<R>);
+    return getrf_inv_elems<R>(batch) + getrf_xbuf_elems<R>(batch) + (size_t)PB * PB * batch +
+           (size_t)N * SPEC_KB_MAX * batch + flag_elems;
 }
 
 // Blocked right-looking LU with partial pivoting (LAPACK zgetrf semantics),
@@ -2212,6 +2502,16 @@ int getrf_steps(cx<R>* A, int N, int64_t lda, int64_t strideA, int* ipiv, int64_
         const size_t nwords = (size_t)MW_XWORDS * batch;
         hipLaunchKernelGGL(zero_words_kernel, dim3((unsigned)((nwords + 255) / 256)), dim3(256), 0, stream, pc.xbuf,
                            nwords);
+        if (aux && aux->speculate) {
+            cx<R>* sp = work + getrf_inv_elems<R>(batch) + getrf_xbuf_elems<R>(batch);
+            pc.spec = true;
+            pc.spec_uinv = sp;
+            pc.spec_bak = sp + (size_t)PB * PB * batch;
+            pc.spec_flags = (int*)(sp + (size_t)PB * PB * batch + (size_t)N * SPEC_KB_MAX * batch);
+            pc.spec_repairs = aux ? aux->spec_repairs : nullptr;
+            hipLaunchKernelGGL(zero_words_kernel, dim3((unsigned)((batch + 255) / 256)), dim3(256), 0, stream,
+                               (uint64_t*)pc.spec_flags, (size_t)batch);
+        }
     }
     const bool ahead = aux && aux->side && aux->lookahead;
     if ((e = panel_dispatch<R>(A, lda, strideA, N, 0, min(nb, N), ipiv, strideP, info, batch, pc, stream))) return e;

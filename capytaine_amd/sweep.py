@@ -24,7 +24,7 @@ from ._lib import check, ptr, stream_handle
 
 class FrequencySweep:
     def __init__(self, green_function, body, wavenumbers, nrhs, *, free_surface=0.0, water_depth=np.inf,
-                 precision="fp64", max_iter=10, tol=1e-12, far_field_fp32=False):
+                 precision="fp64", max_iter=10, tol=1e-12, far_field_fp32=None):
         import torch
         self.gf = green_function
         self.body = body
@@ -39,9 +39,10 @@ class FrequencySweep:
         if precision not in ("fp64", "mixed"):
             raise ValueError(f"precision must be 'fp64' or 'mixed', got {precision!r}")
         self.precision = precision
-        # FP32 far field on request (mixed form): coefficients then move by
-        # ~1e-6 relative; the default keeps the FP64 assembly
-        self.far_field_fp32 = bool(far_field_fp32) and precision == "mixed"
+        # mixed form: far pairs read the float copy of the table (FP64
+        # arithmetic; coefficients within ~1e-7 of the FP64 assembly) unless
+        # far_field_fp32 is False
+        self.far_field_fp32 = precision == "mixed" and far_field_fp32 is not False
         self.max_iter = max_iter
         self.tol = tol
         self.iterations = 0

@@ -318,15 +318,14 @@ int cpt_profile_end(cpt_handle h, double* ms, double* work, int64_t* launches, d
 
 /* Precision of the following assemblies of `h` (cpt_build_matrices,
  * cpt_build_matrices_batched), for the mixed-precision form:
- * far_field_fp32 = 1 evaluates the panel pairs outside the near-field radius
- * (r0 > 7 R, the reference's switch to the one-point Rankine formula,
- * Green_Rankine.f90:50) in FP32 -- one-point Rankine and reflected Rankine,
- * and the infinite-depth wave term interpolated in a float copy of the
- * table with the FP64 path's branch and stencil decisions; near pairs, the
- * diagonal term and the free-surface self term stay FP64, and S and K are
- * stored in complex128.  Entries then differ from the FP64 assembly by
- * FP32 rounding (~1e-7 of max |S|, |K|).  Infinite depth only (ignored in
- * finite depth).
+ * far_field_fp32 = 1 reads, for the panel pairs outside the near-field
+ * radius (r0 > 7 R and r1 > 7 R, the reference's switch to the one-point
+ * Rankine formulas, Green_Rankine.f90:50 and :224), the infinite-depth wave
+ * term's 3 x 3 stencil from a float copy of the table, with the reference's
+ * stencil; the Lagrange weights, the interpolation and every other term are
+ * FP64, and S and K are stored in complex128.  Far entries then differ from
+ * the FP64 assembly by the table's float rounding (~1e-8 of max |S|, |K|);
+ * near pairs are bit-equal.  Infinite depth only (ignored in finite depth).
  * 0 (default) is the plain FP64 assembly.  No reference counterpart: the
  * reference is FP64 (or FP32 throughout when built in single precision). */
 int cpt_set_assembly_precision(cpt_handle h, int far_field_fp32);
@@ -379,9 +378,20 @@ enum {
                                       exchange each column's pivot candidates through device memory,
                                       where the leaf's rows fit their registers; 0: one work-group
                                       per matrix */
-    CPT_TUNE_PANEL_SPIN = 10       /* polls a multi-work-group leaf waits for each column's candidates
+    CPT_TUNE_PANEL_SPIN = 10,      /* polls a multi-work-group leaf waits for each column's candidates
                                       before giving up (then info = -1): 0 = default (2^20); a
                                       negative value gives up at once (a test of that path) */
+    CPT_TUNE_PIVOT_SPECULATION = 11, /* 1: each panel is factored with its diagonal entries
+                                        as pivots (no pivot search; the rows below a leaf's leading
+                                        block solved at once) and every entry the elimination meets
+                                        is checked against LAPACK's rule (|Re|+|Im| of an entry below
+                                        the pivot larger than the pivot's, or a zero pivot, fails);
+                                        a failing matrix's panel is restored and factored again with
+                                        partial pivoting, so the factor is LAPACK's either way.
+                                        0 (default): pivot searches in every panel */
+    CPT_TUNE_SPEC_REPAIRS = 12     /* read only: panels of this context factored again after a failed
+                                      check (one per matrix and panel); reading it waits for the
+                                      device */
 };
 int cpt_set_tuning(cpt_handle h, int param, int value);
 int cpt_get_tuning(cpt_handle h, int param, int* value);

@@ -1,12 +1,14 @@
-"""FP32 far field of the mixed-precision assembly against the FP64
-assembly: near pairs (r0 <= 7 R) bit-equal, far pairs within FP32 rounding,
+"""Far field of the mixed-precision assembly (far pairs read the float copy
+of the table, FP64 arithmetic) against the FP64 assembly: near pairs
+(r0 <= 7 R) bit-equal, far pairs within the float table's rounding,
 coefficients within the 1e-6 relative target.
 
 Tolerances (written here): max |dS| / max |S| and max |dK| / max |K| below
-1e-6, normwise below 1e-6 (FP32 rounding through the ~50 operations of a
-far pair gives ~2e-7); forces of every problem within 1e-6 of its
-largest FP64 force (the mixed solve refines to a 1e-12 residual against
-the mixed-assembled matrix)."""
+1e-7, normwise below 1e-7 (a stencil of float values carries 6e-8 relative
+rounding); forces of every problem within 1e-6 of its largest FP64 force
+(the mixed solve refines to a 1e-12 residual against the mixed-assembled
+matrix), on a 640-panel barge and on the 4352-panel benchmark barge at
+three frequencies (0.2, 1.4 and 2.5 rad/s, the last the hardest)."""
 import numpy as np
 import pytest
 
@@ -55,19 +57,27 @@ def test_far_field_fp32_matrices(default_table, case):
             err = np.abs(A32 - A64).max() / scale
             nerr = np.linalg.norm(A32 - A64) / np.linalg.norm(A64)
             print(f"[gpu]   k={ks[b]} {name}: max {err:.2e} normwise {nerr:.2e}", flush=True)
-            assert err < 1e-6 and nerr < 1e-6, (name, b, err, nerr)
+            assert err < 1e-7 and nerr < 1e-7, (name, b, err, nerr)
 
 
-def test_mixed_sweep_coefficients(default_table):
+@pytest.mark.parametrize("case", ["barge640", "barge4352"])
+def test_mixed_sweep_coefficients(default_table, case):
     import capytaine_amd as cpa
     from capytaine_amd import workloads
     from capytaine_amd.sweep import FrequencySweep, boundary_conditions
-    print("[gpu] mixed sweep (FP32 far field) coefficients vs FP64", flush=True)
+    print(f"[gpu] mixed sweep (float-table far field) coefficients vs FP64, {case}", flush=True)
     gf = ref_table_gf(default_table)
-    mesh = cpa.mesh_parallelepiped(size=(10, 4, 2), center=(0, 0, -1), resolution=(20, 8, 4), missing_sides={"top"})
-    body = cpa.FloatingBody(mesh, name="b").add_all_rigid_body_dofs(rotation_center=(0, 0, 0))
-    om = np.array([0.3, 0.8, 1.5, 2.4])
-    pbs = workloads.problems_by_frequency(body, om, np.arange(4) * np.pi / 2, rho=1025.0)
+    if case == "barge640":
+        mesh = cpa.mesh_parallelepiped(size=(10, 4, 2), center=(0, 0, -1), resolution=(20, 8, 4),
+                                       missing_sides={"top"})
+        body = cpa.FloatingBody(mesh, name="b").add_all_rigid_body_dofs(rotation_center=(0, 0, 0))
+        om = np.array([0.3, 0.8, 1.5, 2.4])
+        headings = np.arange(4) * np.pi / 2
+    else:
+        body, _, headings = workloads.barge()
+        mesh = body.mesh
+        om = np.array([0.2, 1.4, 2.5])
+    pbs = workloads.problems_by_frequency(body, om, headings, rho=1025.0)
     r = len(pbs[0])
     bc = boundary_conditions(pbs, r, mesh.nb_faces, device())
     out = {}
