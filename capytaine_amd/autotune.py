@@ -153,9 +153,13 @@ def tune_lu(n, batch, precision="fp64", device=None, reps=2, nrhs=14, candidates
     timings = {}
     if candidates is None:
         tiles = GEMM_TILES if precision == "mixed" else (0,)
+        # speculative panels (the context's default) have no pivot-search
+        # leaves: their leaf width and threads are fixed
+        spec = saved.get("pivot_speculation", 0)
+        leaves, threads = ((16,), (512,)) if spec else (PANEL_LEAVES, PANEL_THREADS)
         candidates = [(blk, leaf, la, tl, nt)
-                      for blk, leaf, la, tl in itertools.product(LU_BLOCKS, PANEL_LEAVES, LOOKAHEAD, tiles)
-                      for nt in (PANEL_THREADS if la == 1 else PANEL_THREADS[:1])]
+                      for blk, leaf, la, tl in itertools.product(LU_BLOCKS, leaves, LOOKAHEAD, tiles)
+                      for nt in (threads if la == 1 else threads[:1])]
     try:
         for cand in candidates:
             blk, leaf, la, tl = cand[:4]

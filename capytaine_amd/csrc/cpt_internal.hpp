@@ -60,7 +60,7 @@ struct LuSetup {
     int panel_spin = 0;       // polls a multi-work-group leaf waits per column, 0: default; < 0: give up at once
     int mw_budget = 256;      // work-groups of one multi-work-group leaf launch over the batch
     int lookahead = 1;  // 1: next panel on the side stream, 0: in order
-    int speculate = 0;  // panels factored with the diagonal as pivots, checked (else pivot searches)
+    int speculate = 1;  // panels factored with the diagonal as pivots, checked (else pivot searches)
     int* spec_repairs = nullptr;  // device counter of panels factored again after a failed check
 };
 

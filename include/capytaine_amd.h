@@ -381,14 +381,14 @@ enum {
     CPT_TUNE_PANEL_SPIN = 10,      /* polls a multi-work-group leaf waits for each column's candidates
                                       before giving up (then info = -1): 0 = default (2^20); a
                                       negative value gives up at once (a test of that path) */
-    CPT_TUNE_PIVOT_SPECULATION = 11, /* 1: each panel is factored with its diagonal entries
+    CPT_TUNE_PIVOT_SPECULATION = 11, /* 1 (default): each panel is factored with its diagonal entries
                                         as pivots (no pivot search; the rows below a leaf's leading
                                         block solved at once) and every entry the elimination meets
                                         is checked against LAPACK's rule (|Re|+|Im| of an entry below
                                         the pivot larger than the pivot's, or a zero pivot, fails);
                                         a failing matrix's panel is restored and factored again with
                                         partial pivoting, so the factor is LAPACK's either way.
-                                        0 (default): pivot searches in every panel */
+                                        0: pivot searches in every panel */
     CPT_TUNE_SPEC_REPAIRS = 12     /* read only: panels of this context factored again after a failed
                                       check (one per matrix and panel); reading it waits for the
                                       device */

@@ -93,14 +93,23 @@ def test_lu_requires_pivoting():
     assert np.linalg.norm(A @ X[0] - B) / (np.linalg.norm(A) * np.linalg.norm(X[0])) < 50 * n * EPS
 
 
-def test_lu_singular_info():
+@pytest.mark.parametrize("speculation", [0, 1])
+def test_lu_singular_info(speculation):
+    from capytaine_amd import _lib
     n = 40
     A = np.zeros((n, n), dtype=complex)
     A[np.arange(n), np.arange(n)] = 1.0
     A[:, 5] = 0.0
     A[5, 5] = 0.0
-    _, info, _, _ = _factor_solve(A[None], np.ones((1, n, 1), dtype=complex))
+    ctx = _lib.Context.get(device())
+    saved = _lib.get_tuning(ctx)
+    try:
+        _lib.set_tuning(ctx, pivot_speculation=speculation)
+        _, info, ipiv, _ = _factor_solve(A[None], np.ones((1, n, 1), dtype=complex))
+    finally:
+        _lib.set_tuning(ctx, **saved)
     assert info[0] == 6  # 1 + index of the first zero pivot (LAPACK convention)
+    assert np.array_equal(ipiv[0], np.arange(n))
 
 
 def test_lu_batched():
@@ -301,7 +310,8 @@ def test_lu_tunings_agree(block, leaf, lookahead, tiles):
     ctx = _lib.Context.get(device())
     saved = _lib.get_tuning(ctx)
     try:
-        _lib.set_tuning(ctx, lu_block=block, panel_leaf=leaf, lookahead=lookahead, gemm_tiles=tiles)
+        _lib.set_tuning(ctx, lu_block=block, panel_leaf=leaf, lookahead=lookahead, gemm_tiles=tiles,
+                        pivot_speculation=0)
         assert _lib.get_tuning(ctx)["lu_block"] == block
         rng = np.random.default_rng(block + leaf)
         n = 517
@@ -338,7 +348,7 @@ def test_lu_pivoting_every_block_width(block, leaf, nt):
     ctx = _lib.Context.get(device())
     saved = _lib.get_tuning(ctx)
     try:
-        _lib.set_tuning(ctx, lu_block=block, panel_leaf=leaf, lookahead=0, panel_threads=nt)
+        _lib.set_tuning(ctx, lu_block=block, panel_leaf=leaf, lookahead=0, panel_threads=nt, pivot_speculation=0)
         rng = np.random.default_rng(block * 7 + leaf)
         n = 700
         As, Bs = zip(*[_random_system(rng, n, 4, diag=0.0) for _ in range(2)])
@@ -375,7 +385,7 @@ def test_lu_panel_threads_agree_large(precision):
     out = []
     try:
         for nt in (512, 256):
-            _lib.set_tuning(ctx, lu_block=256, panel_leaf=16, lookahead=1, panel_threads=nt)
+            _lib.set_tuning(ctx, lu_block=256, panel_leaf=16, lookahead=1, panel_threads=nt, pivot_speculation=0)
             out.append(_factor_solve(As, Bs) if precision == "fp64" else _mixed(As, Bs))
     finally:
         _lib.set_tuning(ctx, **saved)
@@ -425,7 +435,8 @@ def test_tuning_rejects_bad_values():
     ctx = _lib.Context.get(device())
     for name, bad in [("lu_block", 96), ("lu_block", 512), ("panel_leaf", 0), ("panel_leaf", 65), ("lookahead", 3),
                       ("assembly_chunk", -1), ("gemm_tiles", 5), ("panel_threads", 128), ("panel_threads", 1024),
-                      ("lookahead_active", 1), ("mixed_retries", 0)]:
+                      ("lookahead_active", 1), ("mixed_retries", 0), ("pivot_speculation", 2),
+                      ("spec_repairs", 0)]:
         with pytest.raises(_lib.CptError):
             _lib.set_tuning(ctx, **{name: bad})
     for param in (99, 4):  # unknown, and the retired CU count of look-ahead mode 2
@@ -448,7 +459,7 @@ def test_lu_last_panel_wider_than_leaf_kernel(n, block, leaf):
     ctx = _lib.Context.get(device())
     saved = _lib.get_tuning(ctx)
     try:
-        _lib.set_tuning(ctx, lu_block=block, panel_leaf=leaf, lookahead=0)
+        _lib.set_tuning(ctx, lu_block=block, panel_leaf=leaf, lookahead=0, pivot_speculation=0)
         rng = np.random.default_rng(n)
         A, B = _random_system(rng, n, 3)
         X, info, _, _ = _factor_solve(A[None], B[None])
@@ -606,7 +617,7 @@ def test_multi_workgroup_leaves_agree(precision, n, nb):
     out = {}
     try:
         for wgs in (0, 1):
-            _lib.set_tuning(ctx, lu_block=256, panel_leaf=16, lookahead=1, panel_wgs=wgs)
+            _lib.set_tuning(ctx, lu_block=256, panel_leaf=16, lookahead=1, panel_wgs=wgs, pivot_speculation=0)
             out[wgs] = _factor_solve(As, Bs) if precision == "fp64" else _mixed(As, Bs)
     finally:
         _lib.set_tuning(ctx, **saved)
@@ -656,8 +667,9 @@ def test_multi_workgroup_leaf_gives_up():
         _lib.set_tuning(ctx, **saved)
 
 
+@pytest.mark.parametrize("speculation", [0, 1])
 @pytest.mark.parametrize("n,nb", [(300, 1), (1500, 2)])
-def test_nan_pivot_column_stays_in_bounds(n, nb):
+def test_nan_pivot_column_stays_in_bounds(n, nb, speculation):
     """A pivot column of NaNs (no |Re|+|Im| compares larger than anything)
     takes no interchange, like LAPACK's first index: every pivot stays a row
     of the matrix, the factorisation ends (info names the column) and the
@@ -665,7 +677,7 @@ def test_nan_pivot_column_stays_in_bounds(n, nb):
     as the pivot of such a column, with one work-group (300 rows) and with
     three per matrix (1500 rows)."""
     from capytaine_amd import _lib
-    print(f"[gpu] NaN pivot column n={n} batch={nb}", flush=True)
+    print(f"[gpu] NaN pivot column n={n} batch={nb} speculation={speculation}", flush=True)
     rng = np.random.default_rng(n)
     As, Bs = zip(*[_random_system(rng, n, 1) for _ in range(nb)])
     As, Bs = np.array(As), np.array(Bs)
@@ -673,10 +685,129 @@ def test_nan_pivot_column_stays_in_bounds(n, nb):
     ctx = _lib.Context.get(device())
     saved = _lib.get_tuning(ctx)
     try:
-        _lib.set_tuning(ctx, panel_wgs=1)
+        _lib.set_tuning(ctx, panel_wgs=1, pivot_speculation=speculation)
         X, info, ipiv, _ = _factor_solve(As, Bs)
     finally:
         _lib.set_tuning(ctx, **saved)
     assert np.all((ipiv >= 0) & (ipiv < n))
     assert ipiv[0, 5] == 5
     assert np.isnan(X).any()
+
+
+# ---------------------------------------------------------------------------
+# Speculative panels (CPT_TUNE_PIVOT_SPECULATION): each panel factored with
+# its diagonal as pivots and checked against LAPACK's rule; a failing
+# matrix's panel restored and factored again with partial pivoting.
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("n,nb", [(517, 3), (2300, 2), (4352, 2)])
+def test_speculative_panels_without_interchanges(precision, n, nb):
+    """Matrices on which partial pivoting interchanges no rows (diagonal
+    weight 3 against entries of about 1/sqrt(n), as on the BEM matrices):
+    no panel is repaired, the pivots are LAPACK's (the identity) and the
+    factor equals the pivot-search panels' to rounding."""
+    from scipy.linalg import lu_factor, lu_solve
+    from capytaine_amd import _lib
+    print(f"[gpu] speculative panels, no interchanges, n={n} batch={nb} {precision}", flush=True)
+    rng = np.random.default_rng(n + 7 * nb)
+    As, Bs = zip(*[_random_system(rng, n, 3, diag=3.0) for _ in range(nb)])
+    As, Bs = np.array(As), np.array(Bs)
+    refs = [lu_factor(As[b]) for b in range(nb)]
+    for lu, piv in refs:
+        assert np.array_equal(piv, np.arange(n))  # the case interchanges no rows
+    ctx = _lib.Context.get(device())
+    saved = _lib.get_tuning(ctx)
+    out = {}
+    try:
+        for spec in (0, 1):
+            _lib.set_tuning(ctx, lu_block=256, lookahead=1, pivot_speculation=spec)
+            repairs = _lib.spec_repairs(ctx)
+            out[spec] = _factor_solve(As, Bs) if precision == "fp64" else _mixed(As, Bs)
+            assert _lib.spec_repairs(ctx) == repairs
+    finally:
+        _lib.set_tuning(ctx, **saved)
+    if precision == "fp64":
+        (X0, info0, piv0, LU0), (X1, info1, piv1, LU1) = out[0], out[1]
+        assert not info1.any()
+        assert np.array_equal(piv1, np.broadcast_to(np.arange(n), piv1.shape))
+        assert np.linalg.norm(LU1 - LU0) <= 1e-12 * np.linalg.norm(LU0)
+        for b in range(nb):
+            Xr = lu_solve(refs[b], Bs[b])
+            assert np.linalg.norm(X1[b] - Xr) / np.linalg.norm(Xr) < 1e-12
+            be = np.linalg.norm(As[b] @ X1[b] - Bs[b]) / (np.linalg.norm(As[b]) * np.linalg.norm(X1[b]))
+            assert be < 50 * n * EPS
+    else:
+        (X0, _, r0), (X1, _, r1) = out[0], out[1]
+        assert r0 <= 1e-12 and r1 <= 1e-12
+        assert np.linalg.norm(X1 - X0) <= 1e-11 * np.linalg.norm(X0)
+
+
+@pytest.mark.parametrize("block", [64, 128, 256])
+def test_speculation_repairs_every_panel(block):
+    """No diagonal weight: partial pivoting interchanges rows at nearly
+    every column, so every panel's check fails and every panel is repaired
+    (one repair per matrix and panel): LAPACK's pivots, scipy's solutions,
+    in FP64 and in the mixed form."""
+    from scipy.linalg import lu_factor, lu_solve
+    from capytaine_amd import _lib
+    print(f"[gpu] speculation repairs every panel, block={block}", flush=True)
+    rng = np.random.default_rng(block + 3)
+    n, nb = 700, 2
+    As, Bs = zip(*[_random_system(rng, n, 4, diag=0.0) for _ in range(nb)])
+    As, Bs = np.array(As), np.array(Bs)
+    ctx = _lib.Context.get(device())
+    saved = _lib.get_tuning(ctx)
+    try:
+        _lib.set_tuning(ctx, lu_block=block, lookahead=0, pivot_speculation=1)
+        repairs = _lib.spec_repairs(ctx)
+        X, info, ipiv, _ = _factor_solve(As, Bs)
+        assert _lib.spec_repairs(ctx) - repairs == nb * ((n + block - 1) // block)
+        Xm, _, res = _mixed(As, Bs)
+        assert res <= 1e-12
+    finally:
+        _lib.set_tuning(ctx, **saved)
+    assert not info.any()
+    for b in range(nb):
+        lu, piv = lu_factor(As[b])
+        assert np.mean(piv != np.arange(n)) > 0.9  # the case pivots
+        assert np.mean(ipiv[b] == piv) > 0.99
+        Xr = lu_solve((lu, piv), Bs[b])
+        assert np.linalg.norm(X[b] - Xr) / np.linalg.norm(Xr) < 1e-10
+        assert np.linalg.norm(Xm[b] - Xr) / np.linalg.norm(Xr) < 1e-10
+
+
+def test_speculation_repairs_one_late_panel():
+    """Diagonally weighted except for one zero diagonal entry in the third
+    panel (column 600, panels of 256): the first two panels pass the check,
+    and exactly the panels where LAPACK interchanges rows are repaired --
+    the third, and the fourth too when the row that leaves column 600's
+    diagonal lands there; the interchanges are LAPACK's and the solutions
+    scipy's."""
+    from scipy.linalg import lu_factor, lu_solve
+    from capytaine_amd import _lib
+    print("[gpu] speculation repairs the third panel only", flush=True)
+    rng = np.random.default_rng(600)
+    n, nb = 900, 2
+    As, Bs = zip(*[_random_system(rng, n, 3, diag=3.0) for _ in range(nb)])
+    As, Bs = np.array(As), np.array(Bs)
+    As[:, 600, 600] = 0.0
+    ctx = _lib.Context.get(device())
+    saved = _lib.get_tuning(ctx)
+    try:
+        _lib.set_tuning(ctx, lu_block=256, lookahead=0, pivot_speculation=1)
+        repairs = _lib.spec_repairs(ctx)
+        X, info, ipiv, _ = _factor_solve(As, Bs)
+        repaired = _lib.spec_repairs(ctx) - repairs
+    finally:
+        _lib.set_tuning(ctx, **saved)
+    assert not info.any()
+    expected = 0
+    for b in range(nb):
+        lu, piv = lu_factor(As[b])
+        assert piv[600] != 600 and np.array_equal(piv[:600], np.arange(600))
+        expected += sum(bool(np.any(piv[k0:k0 + 256] != np.arange(k0, min(n, k0 + 256)))) for k0 in range(0, n, 256))
+        assert np.array_equal(ipiv[b], piv)
+        Xr = lu_solve((lu, piv), Bs[b])
+        assert np.linalg.norm(X[b] - Xr) / np.linalg.norm(Xr) < 1e-12
+    assert nb <= expected <= 2 * nb and repaired == expected
