@@ -570,6 +570,9 @@ def main():
     ap.add_argument("--gemm-tiles", type=int, default=None,
                     help="force the complex64 tile product variant of the mixed form (cpt_set_tuning "
                          "gemm_tiles, 0-4) after tuning, for profiler passes of one variant")
+    ap.add_argument("--lu-block", type=int, default=None, choices=(128, 256, 512, 1024),
+                    help="force the LU block width of both forms (cpt_set_tuning lu_block) after tuning, for "
+                         "profiler passes of one variant")
     ap.add_argument("--panel-threads", type=int, default=None, choices=(256, 512),
                     help="force the panel leaf work-group size of both forms (cpt_set_tuning "
                          "panel_threads) after tuning, for profiler passes of one variant")
@@ -666,6 +669,9 @@ def main():
         if args.panel_threads is not None:
             _lib.set_tuning(ctx, panel_threads=args.panel_threads)
             applied = dict(applied or {}, panel_threads=args.panel_threads)
+        if args.lu_block is not None:
+            _lib.set_tuning(ctx, lu_block=args.lu_block)
+            applied = dict(applied or {}, lu_block=args.lu_block)
         if applied:
             print(f"[bench] {precision}: tuning {applied}", file=sys.stderr, flush=True)
         sweep = FrequencySweep(gf, body, ks, nrhs, precision=precision, max_iter=10, tol=1e-12,
@@ -730,7 +736,7 @@ def main():
     # roofline of the dominant kernel of the reported (mixed) form, from the
     # live HIP-event timing of every launch of it in the timed region
     kern = rmx["kernels"]
-    dom = max(("zgemm", "cgemm", "skinny_z", "skinny_c"), key=lambda k: kern[k]["ms"])
+    dom = max(("zgemm", "cgemm", "skinny_z", "skinny_c", "cgemm_packed"), key=lambda k: kern[k]["ms"])
     kd = kern[dom]
     # the complex64 product the tuner chose for the mixed form
     ctiles = _lib.get_tuning(ctx)["gemm_tiles"]
@@ -742,7 +748,9 @@ def main():
     names = {"zgemm": "gemm_tiled_kernel<double,64> (complex128 MFMA tile product)",
              "cgemm": f"{cname} (complex64 MFMA tile product{cnote})",
              "skinny_z": "gemm_skinny_kernel<double> (complex128 product, <= 32 columns)",
-             "skinny_c": "gemm_skinny_kernel<float> (complex64 product, <= 32 columns)"}
+             "skinny_c": "gemm_skinny_kernel<float> (complex64 product, <= 32 columns)",
+             "cgemm_packed": "gemm_h16p_kernel (complex64 trailing update on FP16 operands packed once per block "
+                             "step, v_mfma_f32_16x16x32_f16, three real products per complex one, FP32 accumulation)"}
     if dom.startswith("skinny"):
         achieved = kd["work"] / (kd["ms"] / 1e3) / 1e9
         roofline = {"bound": "hbm", "kernel": names[dom], "achieved": achieved, "peak": MI355X_HBM_GBS,
@@ -750,15 +758,15 @@ def main():
                     "work_per_launch": "bytes of A, B, C per product, summed over the batch"}
     else:
         achieved = kd["work"] / (kd["ms"] / 1e3) / 1e12
-        peak = MI355X_FP64_MFMA_TFLOPS if dom != "cgemm" else \
-            MI355X_FP16_MFMA_TFLOPS if ctiles == 4 else MI355X_FP32_MFMA_TFLOPS
+        peak = MI355X_FP64_MFMA_TFLOPS if dom == "zgemm" else MI355X_FP16_MFMA_TFLOPS \
+            if dom == "cgemm_packed" or ctiles == 4 else MI355X_FP32_MFMA_TFLOPS
         roofline = {"bound": "mfma", "kernel": names[dom], "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                     "frac": achieved / peak, "traffic": None,
                     "work_per_launch": "8 M N K flop per complex M x N x K product, summed over the batch"}
     roofline.update(launches=kd["launches"], avg_launch_us=kd["ms"] / kd["launches"] * 1e3,
                     work_per_launch_avg=kd["work"] / kd["launches"],
                     algorithmic_bytes_per_launch=kd["bytes"] / kd["launches"])
-    if dom == "cgemm" and ctiles >= 2:
+    if dom == "cgemm_packed" or (dom == "cgemm" and ctiles >= 2):
         roofline["mfma_flops_note"] = ("the Gauss form executes 6 M N K real MFMA flops for the 8 M N K counted in "
                                        "`achieved`; its MFMA pipes run at 3/4 of `achieved`")
     traffic = measured_traffic(names[dom].split(" ")[0])

@@ -63,7 +63,7 @@ TUNING_PARAMS = ("lu_block", "panel_leaf", "lookahead", "assembly_chunk", None, 
 READ_ONLY_TUNING = ("lookahead_active", "mixed_retries", "spec_repairs")
 
 # Kernel classes of cpt_profile_end, in the order of the CPT_TIMED_* values.
-TIMED_KERNELS = ("zgemm", "cgemm", "panel_z", "panel_c", "assembly", "skinny_z", "skinny_c")
+TIMED_KERNELS = ("zgemm", "cgemm", "panel_z", "panel_c", "assembly", "skinny_z", "skinny_c", "cgemm_packed")
 
 
 

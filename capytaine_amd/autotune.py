@@ -24,6 +24,9 @@ from . import _lib
 from ._lib import check, ptr, stream_handle
 
 LU_BLOCKS = (128, 256)      # 64 loses at every measured size above 1000 rows
+# wider blocks, for speculative panels (their panels and triangular solves
+# recurse into deep, compute-bound products)
+SPEC_LU_BLOCKS = (512, 1024)
 PANEL_LEAVES = (8, 16)
 # look-ahead modes: 0 (in order) and 1 (next panel on a side stream)
 LOOKAHEAD = (0, 1)
@@ -157,8 +160,9 @@ def tune_lu(n, batch, precision="fp64", device=None, reps=2, nrhs=14, candidates
         # leaves: their leaf width and threads are fixed
         spec = saved.get("pivot_speculation", 0)
         leaves, threads = ((16,), (512,)) if spec else (PANEL_LEAVES, PANEL_THREADS)
+        blocks = LU_BLOCKS + SPEC_LU_BLOCKS if spec else LU_BLOCKS
         candidates = [(blk, leaf, la, tl, nt)
-                      for blk, leaf, la, tl in itertools.product(LU_BLOCKS, leaves, LOOKAHEAD, tiles)
+                      for blk, leaf, la, tl in itertools.product(blocks, leaves, LOOKAHEAD, tiles)
                       for nt in (threads if la == 1 else threads[:1])]
     try:
         for cand in candidates:

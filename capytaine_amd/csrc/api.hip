@@ -1237,7 +1237,7 @@ int cpt_set_tuning(cpt_handle h, int param, int value)
     }
     switch (param) {
     case CPT_TUNE_LU_BLOCK:
-        if (value != 0 && value != 64 && value != 128 && value != 256) break;
+        if (value != 0 && value != 64 && value != 128 && value != 256 && value != 512 && value != 1024) break;
         h->lu_streams.block = value;
         return CPT_OK;
     case CPT_TUNE_PANEL_LEAF:

@@ -125,7 +125,8 @@ enum TimedKernel {
     TK_ASSEMBLY = 4,
     TK_SKINNY_Z = 5,
     TK_SKINNY_C = 6,
-    TK_COUNT = 7
+    TK_CGEMM_PACKED = 7,
+    TK_COUNT = 8
 };
 struct KernelTimer {
     struct Rec {

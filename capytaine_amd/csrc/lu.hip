@@ -112,6 +112,8 @@ struct Mfma<float> {
 
 constexpr int PB = 64;  // widest triangle the panel and trsm kernels hold (one wavefront)
 
+inline int round_up_host(int x, int m) { return (x + m - 1) / m * m; }
+
 // The panel kernel's sub-block update loads L for groups of panel_qg rows
 // and panel_jc columns per round trip: at most `cap` complex values per
 // thread in flight; panel_jc divides SB and panel_qg divides RPT.
@@ -913,14 +915,18 @@ __global__ void zero_words_kernel(uint64_t* __restrict__ p, size_t n)
 
 // ---------------------------------------------------------------------------
 // 2. Row interchanges ipiv[r0 .. r1) applied to the columns [0, c_lo) and
-//    [c_hi, ncols) of a column-major array (LAPACK zlaswp).
+//    [c_hi, ncols) of a column-major array (LAPACK zlaswp); with `swapped`,
+//    only for the matrices whose flag is set.
 // ---------------------------------------------------------------------------
 template <typename R>
 __global__ void laswp_kernel(cx<R>* __restrict__ Abase, int64_t lda, int64_t strideA, int ncols, int c_lo, int c_hi,
-                             const int* __restrict__ ipiv_base, int64_t strideP, int r0, int r1)
+                             const int* __restrict__ ipiv_base, int64_t strideP, int r0, int r1,
+                             const int* __restrict__ swapped)
 {
     const int col = blockIdx.x * blockDim.x + threadIdx.x;
     if (col >= ncols || (col >= c_lo && col < c_hi)) return;
+    // a speculative panel that passed its check interchanged no rows
+    if (swapped && !swapped[blockIdx.y]) return;
     cx<R>* A = Abase + (size_t)blockIdx.y * strideA + (size_t)col * lda;
     const int* ipiv = ipiv_base + (size_t)blockIdx.y * strideP;
     for (int i = r0; i < r1; ++i) {
@@ -1892,7 +1898,7 @@ gemm_h16_kernel(int M, int N, int K, float alpha, const cx<float>* __restrict__ 
 // (on the configured bodies the largest entry below it stays under 0.35 of
 // it).  A panel is then factored with the diagonal as pivots: no pivot
 // search, and the rows below each leaf's leading w x w block are
-// independent (L21 = A21 U11^-1, an MFMA-free but row-parallel product).
+// independent (each solves l U11 = a by itself).
 // Every entry a_ij^(j) the elimination meets below a pivot u_jj is compared
 // with it exactly as the pivot search compares candidates: cabs1(a_ij^(j)) >
 // cabs1(u_jj), or a zero or non-number u_jj, means the search would have
@@ -1902,129 +1908,147 @@ gemm_h16_kernel(int M, int N, int K, float alpha, const cx<float>* __restrict__ 
 // ties to the first row, zero columns reported in info), so the factor is
 // LAPACK's P A = L U in both cases, up to rounding.
 // ---------------------------------------------------------------------------
-constexpr int SPEC_KB_MAX = 4 * PB;  // widest panel (block width) the copy holds
+constexpr int SPEC_KB_MAX = 16 * PB;  // widest panel (block width) the copy holds
 
-// copy of the panel columns [k0, k0 + kb), rows [k0, N), leading dimension
-// N - k0, per matrix
+// columns of the panel copy per matrix
+inline int spec_bak_cols(int N) { return N < SPEC_KB_MAX ? N : SPEC_KB_MAX; }
+
+// Leaf width of the speculative panels: 32 columns, so that a thread holds
+// one row of a leaf in registers (32 complex values) with straight-line code
+// that stays in the instruction cache.
 template <typename R>
-__global__ void spec_backup_kernel(const cx<R>* __restrict__ Abase, int64_t lda, int64_t strideA, int N, int k0,
-                                   int kb, cx<R>* __restrict__ bak, int64_t strideBak)
+constexpr int spec_width()
 {
-    const int M = N - k0;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
-    if (i >= M) return;
-    const size_t b = blockIdx.z;
-    bak[b * strideBak + i + (size_t)j * M] = Abase[b * strideA + (k0 + i) + (size_t)(k0 + j) * lda];
+    return 32;
 }
 
-// Leading w x w block (w <= PB) of the leaf at column c0: factored without
-// interchanges in LDS, checked, written back with ipiv = identity; U11^-1
-// (upper part, diagonal included) into Ubase (PB x PB per matrix).
+// copy of the panel columns [k0, k0 + kb), rows [k0, N), leading dimension
+// N - k0, per matrix (four rows per thread, 1024 per work-group and
+// column); clears the matrix's interchange flag of this panel
 template <typename R>
-__global__ void __launch_bounds__(256) spec_top_kernel(cx<R>* __restrict__ Abase, int64_t lda, int64_t strideA,
-                                                       int c0, int w, int* __restrict__ ipiv_base, int64_t strideP,
-                                                       int* __restrict__ flags, cx<R>* __restrict__ Ubase)
+__global__ void __launch_bounds__(256) spec_backup_kernel(const cx<R>* __restrict__ Abase, int64_t lda,
+                                                          int64_t strideA, int N, int k0, int kb,
+                                                          cx<R>* __restrict__ bak, int64_t strideBak,
+                                                          int* __restrict__ swapped)
+{
+    const int M = N - k0;
+    const int j = blockIdx.y;
+    const size_t b = blockIdx.z;
+    if (blockIdx.x == 0 && j == 0 && threadIdx.x == 0) swapped[b] = 0;
+    const cx<R>* src = Abase + b * strideA + k0 + (size_t)(k0 + j) * lda;
+    cx<R>* dst = bak + b * strideBak + (size_t)j * M;
+    cx<R> v[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int i = blockIdx.x * 1024 + q * 256 + threadIdx.x;
+        if (i < M) v[q] = src[i];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int i = blockIdx.x * 1024 + q * 256 + threadIdx.x;
+        if (i < M) dst[i] = v[q];
+    }
+}
+
+// Leading w x w block (w <= spec_width) of the leaf at column c0, factored
+// without interchanges by one wavefront per matrix in LDS: lane (i, h) owns
+// row i's columns of parity h (rows and columns beyond w padded with the
+// identity, which leaves the leading block's factor unchanged).  At column j
+// every lane forms l_ij from the column and the pivot (both final since the
+// last barrier), the owner of column j stores it apart, and each lane
+// updates its own columns beyond j with row j (which it updated itself):
+// one barrier per column.  Each entry a_ij^(j) met below the pivot is
+// checked; the block is written back with ipiv = identity.
+template <typename R>
+__global__ void __launch_bounds__(64) spec_top_kernel(cx<R>* __restrict__ Abase, int64_t lda, int64_t strideA,
+                                                      int c0, int w, int* __restrict__ ipiv_base, int64_t strideP,
+                                                      int* __restrict__ flags)
 {
     typedef cx<R> C;
-    __shared__ C sT[PB * PB];
-    __shared__ int sBad;
+    constexpr int SW = spec_width<R>();
+    constexpr int LD = SW + 1;  // padded row stride: lanes of one column hit different banks
+    __shared__ C sT[SW * LD];   // sT[i * LD + k]: the working block, row-major
+    __shared__ C sL[SW * LD];   // the multipliers l_ij
     const int b = blockIdx.x;
     if (flags[b]) return;  // the panel is factored again anyway
     C* A = Abase + (size_t)b * strideA + c0 + (size_t)c0 * lda;
-    const int tid = threadIdx.x;
-    if (tid == 0) sBad = 0;
-    for (int e = tid; e < w * w; e += 256) sT[(e % w) + PB * (e / w)] = A[(e % w) + (size_t)(e / w) * lda];
+    const int lane = threadIdx.x, i = lane & (SW - 1), h = lane >> 5;
+    for (int k = h; k < SW; k += 2) sT[i * LD + k] = (i < w && k < w) ? A[i + (size_t)k * lda] : C{R(i == k ? 1 : 0), R(0)};
     __syncthreads();
-    for (int j = 0; j < w; ++j) {
-        const C d = sT[j + PB * j];
+    bool bad = false;
+    for (int j = 0; j < SW; ++j) {
+        const C d = sT[j * LD + j];
         const R ad = cabs1(d);
-        if (tid > j && tid < w) {
-            const C a = sT[tid + PB * j];
-            if (!(ad > R(0)) || cabs1(a) > ad) sBad = 1;
-            sT[tid + PB * j] = zmul(a, zrcp(d));
-        }
-        if (tid == 0 && !(ad > R(0))) sBad = 1;
-        __syncthreads();
-        const int m = w - j - 1;
-        for (int e = tid; e < m * m; e += 256) {
-            const int i = j + 1 + e % m, k = j + 1 + e / m;
-            sT[i + PB * k] = zsub(sT[i + PB * k], zmul(sT[i + PB * j], sT[j + PB * k]));
-        }
+        const C a = sT[i * LD + j];
+        const bool below = i > j;
+        bad = bad || !(ad > R(0)) || (below && cabs1(a) > ad);
+        const C l = zmul(a, zrcp(d));
+        if (below && h == (j & 1)) sL[i * LD + j] = l;
+        if (below)
+            for (int k = j + 1 + ((j + 1 + h) & 1); k < SW; k += 2)
+                sT[i * LD + k] = zsub(sT[i * LD + k], zmul(l, sT[j * LD + k]));
         __syncthreads();
     }
-    if (sBad) {
-        if (tid == 0) flags[b] = 1;
+    if (__ballot(bad)) {
+        if (lane == 0) flags[b] = 1;
         return;
     }
-    for (int e = tid; e < w * w; e += 256) A[(e % w) + (size_t)(e / w) * lda] = sT[(e % w) + PB * (e / w)];
-    int* ipiv = ipiv_base + (size_t)b * strideP;
-    if (tid < w) ipiv[c0 + tid] = c0 + tid;
-    __syncthreads();
-    // U11 as the upper triangle with reciprocal diagonal, then its inverse
-    for (int e = tid; e < w * w; e += 256) {
-        const int i = e % w, k = e / w;
-        if (i > k) sT[i + PB * k] = C{R(0), R(0)};
-        else if (i == k) sT[i + PB * k] = zrcp(sT[i + PB * k]);
+    if (i < w) {
+        for (int k = h; k < w; k += 2) A[i + (size_t)k * lda] = k < i ? sL[i * LD + k] : sT[i * LD + k];
+        if (h == 0) ipiv_base[(size_t)b * strideP + c0 + i] = c0 + i;
     }
-    __syncthreads();
-    invert_triangle<R, true>(sT, w, tid & 63, tid >> 6, Ubase + (size_t)b * PB * PB, PB);
 }
 
-// Rows [c0 + w, N) of the leaf's columns: L21 = A21 U11^-1 in place, each
-// entry checked as l_ij u_jj against u_jj.  64 rows per work-group, a
-// thread per (row, group of 16 columns).
+// Rows [c0 + w, N) of the leaf's columns: l U11 = a solved row by row
+// (forward substitution over the columns, U11 in LDS with the reciprocals of
+// its diagonal), a thread per row holding it in registers; each entry
+// a_rc^(c) met before its division is checked against u_cc.
 template <typename R>
 __global__ void __launch_bounds__(256) spec_rows_kernel(cx<R>* __restrict__ Abase, int64_t lda, int64_t strideA,
-                                                        int N, int c0, int w, int* __restrict__ flags,
-                                                        const cx<R>* __restrict__ Ubase)
+                                                        int N, int c0, int w, int* __restrict__ flags)
 {
     typedef cx<R> C;
-    __shared__ C sX[PB * PB];
-    __shared__ C sD[PB];
-    __shared__ R sAd[PB];
+    constexpr int SW = spec_width<R>();
+    __shared__ C sU[SW * SW];
+    __shared__ R sAd[SW];
     const int b = blockIdx.y;
     if (flags[b]) return;
-    const int r0 = c0 + w + blockIdx.x * 64;
     C* A = Abase + (size_t)b * strideA;
-    const C* X = Ubase + (size_t)b * PB * PB;
     const int tid = threadIdx.x;
-    for (int e = tid; e < PB * PB; e += 256) {
-        const int i = e % PB, k = e / PB;
-        sX[e] = (i <= k && k < w) ? X[e] : C{R(0), R(0)};
-    }
-    if (tid < w) {
-        const C d = A[(c0 + tid) + (size_t)(c0 + tid) * lda];
-        sD[tid] = d;
-        sAd[tid] = cabs1(d);
-    }
-    __syncthreads();
-    const int r = r0 + (tid & 63), cg = tid >> 6;
-    C acc[16];
 #pragma unroll
-    for (int q = 0; q < 16; ++q) acc[q] = C{R(0), R(0)};
-    const int kend = min(w, 16 * cg + 16);
-    if (r < N) {
-        for (int k = 0; k < kend; ++k) {
-            const C a = A[r + (size_t)(c0 + k) * lda];
-#pragma unroll
-            for (int q = 0; q < 16; ++q) {
-                const C x = sX[k + PB * (16 * cg + q)];
-                acc[q].re += a.re * x.re - a.im * x.im;
-                acc[q].im += a.re * x.im + a.im * x.re;
+    for (int e0 = 0; e0 < SW * SW; e0 += 256) {
+        const int e = e0 + tid, i = e % SW, k = e / SW;
+        if (k < w && i <= k) {
+            const C u = A[(c0 + i) + (size_t)(c0 + k) * lda];
+            if (i == k) {
+                sU[e] = zrcp(u);
+                sAd[k] = cabs1(u);
+            } else {
+                sU[e] = u;
             }
         }
     }
-    __syncthreads();  // every thread of the row has read it before any writes
+    __syncthreads();
+    const int r = c0 + w + blockIdx.x * 256 + tid;
     if (r >= N) return;
+    C x[SW];
+#pragma unroll
+    for (int c = 0; c < SW; ++c)
+        if (c < w) x[c] = A[r + (size_t)(c0 + c) * lda];
     bool bad = false;
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-        const int c = 16 * cg + q;
+    for (int c = 0; c < SW; ++c) {
         if (c < w) {
-            A[r + (size_t)(c0 + c) * lda] = acc[q];
-            if (cabs1(zmul(acc[q], sD[c])) > sAd[c]) bad = true;
+            C t = x[c];
+#pragma unroll
+            for (int k = 0; k < c; ++k) t = zsub(t, zmul(x[k], sU[k + SW * c]));
+            bad = bad || cabs1(t) > sAd[c];
+            x[c] = zmul(t, sU[c + SW * c]);
         }
     }
+#pragma unroll
+    for (int c = 0; c < SW; ++c)
+        if (c < w) A[r + (size_t)(c0 + c) * lda] = x[c];
     if (bad) atomicOr(flags + b, 1);
 }
 
@@ -2034,13 +2058,15 @@ __global__ void __launch_bounds__(256) spec_rows_kernel(cx<R>* __restrict__ Abas
 // without a nonzero candidate keeps its row and sets info), interchanges
 // within the panel, the scaled column and the rank-1 update of the panel's
 // remaining columns.  One work-group per matrix, global memory: the rare
-// path.  Clears the matrix's flag and counts the repair.
+// path.  Clears the matrix's flag, marks an interchange for the rows
+// outside the panel, and counts the repair.
 template <typename R>
 __global__ void __launch_bounds__(512) spec_repair_kernel(cx<R>* __restrict__ Abase, int64_t lda, int64_t strideA,
                                                           int N, int k0, int kb, const cx<R>* __restrict__ bak,
                                                           int64_t strideBak, int* __restrict__ ipiv_base,
                                                           int64_t strideP, int* __restrict__ info_base,
-                                                          int* __restrict__ flags, int* __restrict__ repairs)
+                                                          int* __restrict__ flags, int* __restrict__ swapped,
+                                                          int* __restrict__ repairs)
 {
     typedef cx<R> C;
     constexpr int NT = 512, NW = NT / 64;
@@ -2084,6 +2110,7 @@ __global__ void __launch_bounds__(512) spec_repair_kernel(cx<R>* __restrict__ Ab
         if (tid == 0) {
             if (!pivoted && info_base[b] == 0) info_base[b] = k0 + j + 1;
             ipiv[k0 + j] = k0 + p;
+            if (p != j) swapped[b] = 1;
         }
         if (p != j)
             for (int c = tid; c < kb; c += NT) {
@@ -2110,6 +2137,279 @@ __global__ void __launch_bounds__(512) spec_repair_kernel(cx<R>* __restrict__ Ab
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// 5c. Packed FP16 tile product for the complex64 trailing updates.
+//
+// C += alpha A B with A (M x K) and B (K x N) complex64, in the Gauss form
+// of gemm_h16_kernel (three real products Ar Br, Ai Bi, (Ar + Ai)(Br + Bi),
+// FP16 inputs, FP32 accumulation), but with the operands packed to FP16
+// once per product instead of in every tile: pack_a_h16_kernel and
+// pack_b_h16_kernel write the three planes (re, im, re + im) with k
+// contiguous per row of A and per column of B (zero beyond M, N and K, K
+// padded to 32), so that the tile kernel stages them to LDS with plain
+// 16-byte loads and feeds v_mfma_f32_16x16x32_f16 (eight k per lane).
+// Tiles of 128 x 128 per work-group of four wavefronts (64 x 64 each), the
+// 32-deep chunks double-buffered in LDS.  The batch is spread so that every work-group of one matrix
+// runs on the same XCD (work-group ids go round-robin over the 8 XCDs), and
+// the matrix's packed A and B are read into one L2 only.
+// ---------------------------------------------------------------------------
+typedef _Float16 hvec8 __attribute__((ext_vector_type(8)));
+constexpr int HP_BM = 128, HP_BK = 32, HP_LDK = HP_BK + 8;  // LDS row stride in halves (80 bytes)
+constexpr int HP_GROUP = 4;  // row panels per group of tiles
+
+// A (M x K, column-major) -> Ap: plane p, row m at Ap + (p Mp + m) Kp, k
+// contiguous.  64 rows x 32 k per work-group, transposed through LDS.
+__global__ void __launch_bounds__(256) pack_a_h16_kernel(const cx<float>* __restrict__ Abase, int64_t lda,
+                                                         int64_t strideA, int M, int K, _Float16* __restrict__ Apbase,
+                                                         int Mp, int Kp, int64_t strideAp)
+{
+    __shared__ cx<float> t[32][65];
+    const int m0 = blockIdx.x * 64, k0 = blockIdx.y * 32, tid = threadIdx.x;
+    const cx<float>* A = Abase + (size_t)blockIdx.z * strideA;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int k = (tid >> 6) + 4 * q, m = tid & 63;
+        cx<float> v{0.0f, 0.0f};
+        if (m0 + m < M && k0 + k < K) v = A[(m0 + m) + (size_t)(k0 + k) * lda];
+        t[k][m] = v;
+    }
+    __syncthreads();
+    const int m = tid >> 2, seg = tid & 3;
+    hvec8 re, im, sm;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const cx<float> v = t[8 * seg + j][m];
+        re[j] = (_Float16)v.re;
+        im[j] = (_Float16)v.im;
+        sm[j] = (_Float16)(v.re + v.im);
+    }
+    const size_t plane = (size_t)Mp * Kp;
+    _Float16* P = Apbase + (size_t)blockIdx.z * strideAp + (size_t)(m0 + m) * Kp + k0 + 8 * seg;
+    *reinterpret_cast<hvec8*>(P) = re;
+    *reinterpret_cast<hvec8*>(P + plane) = im;
+    *reinterpret_cast<hvec8*>(P + 2 * plane) = sm;
+}
+
+// B (K x N, column-major) -> Bp: plane p, column n at Bp + (p Np + n) Kp, k
+// contiguous (B's own order): eight k per thread.
+__global__ void __launch_bounds__(256) pack_b_h16_kernel(const cx<float>* __restrict__ Bbase, int64_t ldb,
+                                                         int64_t strideB, int K, int N, _Float16* __restrict__ Bpbase,
+                                                         int Np, int Kp, int64_t strideBp)
+{
+    const int segs = Kp / 8;
+    const int id = blockIdx.x * 256 + threadIdx.x;
+    if (id >= Np * segs) return;
+    const int n = id / segs, seg = id % segs;
+    const cx<float>* B = Bbase + (size_t)blockIdx.y * strideB + (size_t)n * ldb;
+    hvec8 re, im, sm;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const int k = 8 * seg + j;
+        cx<float> v{0.0f, 0.0f};
+        if (n < N && k < K) v = B[k];
+        re[j] = (_Float16)v.re;
+        im[j] = (_Float16)v.im;
+        sm[j] = (_Float16)(v.re + v.im);
+    }
+    const size_t plane = (size_t)Np * Kp;
+    _Float16* P = Bpbase + (size_t)blockIdx.y * strideBp + (size_t)n * Kp + 8 * seg;
+    *reinterpret_cast<hvec8*>(P) = re;
+    *reinterpret_cast<hvec8*>(P + plane) = im;
+    *reinterpret_cast<hvec8*>(P + 2 * plane) = sm;
+}
+
+// C (M x N of the columns [c_lo, c_lo + N) of the packed B) += alpha A B;
+// one-dimensional grid of Mt Nt batch work-groups, Mt = ceil(M / 128),
+// Nt = ceil(N / 128).  Two LDS stages: chunk i is multiplied from one while
+// chunk i + 1, loaded into registers during chunk i - 1, is stored to the
+// other, and chunk i + 2 is loaded: one barrier per chunk.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
+gemm_h16p_kernel(int M, int N, int Kp, float alpha, const _Float16* __restrict__ Apbase, int Mp, int64_t strideAp,
+                 const _Float16* __restrict__ Bpbase, int Np, int c_lo, int64_t strideBp, cx<float>* __restrict__ Cbase,
+                 int64_t ldc, int64_t strideC, int batch)
+{
+    typedef cx<float> Cx;
+    constexpr int STAGE = 3 * HP_BM * HP_LDK;  // halves of one operand's stage
+    __shared__ __attribute__((aligned(16))) _Float16 sA[2 * STAGE];
+    __shared__ __attribute__((aligned(16))) _Float16 sB[2 * STAGE];
+    const int Mt = (M + HP_BM - 1) / HP_BM, Nt = (N + HP_BM - 1) / HP_BM, T = Mt * Nt;
+    // work-group -> (matrix, tile): all tiles of a matrix on one XCD when the
+    // batch is a multiple of 8
+    int z, t;
+    const int L = blockIdx.x;
+    if (batch % 8 == 0) {
+        const int x = L % 8, j = L / 8;
+        z = x + 8 * (j / T);
+        t = j % T;
+    } else {
+        z = L / T;
+        t = L % T;
+    }
+    // tiles in groups of HP_GROUP row panels, column panels outer within a
+    // group: the work-groups of an XCD in flight at once share a few A and B
+    // panels in its L2
+    const int gsz = HP_GROUP * Nt, g = t / gsz, first = g * HP_GROUP;
+    const int grows = min(HP_GROUP, Mt - first), tg = t % gsz;
+    const int row0 = (first + tg % grows) * HP_BM, col0 = (tg / grows) * HP_BM;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int wm = wave & 1, wn = wave >> 1;
+    const size_t planeA = (size_t)Mp * Kp, planeB = (size_t)Np * Kp;
+    const _Float16* Ap = Apbase + (size_t)z * strideAp + (size_t)row0 * Kp;
+    const _Float16* Bp = Bpbase + (size_t)z * strideBp + (size_t)(c_lo + col0) * Kp;
+    Cx* C = Cbase + (size_t)z * strideC;
+    // staging role: rows (or columns) sr and sr + 64, k = 8 sseg .. + 8
+    const int sr = tid >> 2, sseg = tid & 3;
+    hvec8 ra[3][2], rb[3][2];
+    auto load = [&](int kc) {
+#pragma unroll
+        for (int p = 0; p < 3; ++p)
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                ra[p][i] = *reinterpret_cast<const hvec8*>(Ap + p * planeA + (size_t)(sr + 64 * i) * Kp + kc + 8 * sseg);
+                rb[p][i] = *reinterpret_cast<const hvec8*>(Bp + p * planeB + (size_t)(sr + 64 * i) * Kp + kc + 8 * sseg);
+            }
+    };
+    auto store = [&](int st) {
+#pragma unroll
+        for (int p = 0; p < 3; ++p)
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                const int off = st * STAGE + p * HP_BM * HP_LDK + (sr + 64 * i) * HP_LDK + 8 * sseg;
+                *reinterpret_cast<hvec8*>(&sA[off]) = ra[p][i];
+                *reinterpret_cast<hvec8*>(&sB[off]) = rb[p][i];
+            }
+    };
+    fvec4 cr[4][4], ci[4][4], cs[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            cr[a][b] = fvec4{0, 0, 0, 0};
+            ci[a][b] = fvec4{0, 0, 0, 0};
+            cs[a][b] = fvec4{0, 0, 0, 0};
+        }
+    const int nk = Kp / HP_BK;
+    load(0);
+    store(0);
+    if (nk > 1) load(HP_BK);
+    for (int c = 0; c < nk; ++c) {
+        // stage c & 1 holds chunk c; the registers hold chunk c + 1
+        __syncthreads();
+        const int st = c & 1;
+        hvec8 yr[4], yi[4], ys[4];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int off = st * STAGE + (wn * 64 + b * 16 + li) * HP_LDK + 8 * lk;
+            yr[b] = *reinterpret_cast<const hvec8*>(&sB[off]);
+            yi[b] = *reinterpret_cast<const hvec8*>(&sB[off + HP_BM * HP_LDK]);
+            ys[b] = *reinterpret_cast<const hvec8*>(&sB[off + 2 * HP_BM * HP_LDK]);
+        }
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            const int off = st * STAGE + (wm * 64 + a * 16 + li) * HP_LDK + 8 * lk;
+            const hvec8 xr = *reinterpret_cast<const hvec8*>(&sA[off]);
+            const hvec8 xi = *reinterpret_cast<const hvec8*>(&sA[off + HP_BM * HP_LDK]);
+            const hvec8 xs = *reinterpret_cast<const hvec8*>(&sA[off + 2 * HP_BM * HP_LDK]);
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                cr[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xr, yr[b], cr[a][b], 0, 0, 0);
+                ci[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xi, yi[b], ci[a][b], 0, 0, 0);
+                cs[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xs, ys[b], cs[a][b], 0, 0, 0);
+            }
+        }
+        // chunk c + 1 into the other stage (its last readers, of chunk c - 1,
+        // passed this iteration's barrier), then load chunk c + 2
+        if (c + 1 < nk) {
+            store(st ^ 1);
+            if (c + 2 < nk) load((c + 2) * HP_BK);
+        }
+    }
+    // a lane holds rows 4 lk .. 4 lk + 3 of one column of each 16 x 16 block:
+    // 32 contiguous bytes of C, read and written as two 16-byte vectors
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int rowb = row0 + wm * 64 + 16 * a + 4 * lk;
+            const int col = col0 + wn * 64 + 16 * b + li;
+            if (col >= N || rowb >= M) continue;
+            float re[4], im[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const float p1 = cr[a][b][v], p2 = ci[a][b][v];
+                re[v] = alpha * (p1 - p2);
+                im[v] = alpha * (cs[a][b][v] - p1 - p2);
+            }
+            Cx* cp = C + rowb + (size_t)col * ldc;
+            if (rowb + 3 < M) {
+                fvec4u* cv = reinterpret_cast<fvec4u*>(cp);
+                fvec4u lo = {re[0], im[0], re[1], im[1]}, hi = {re[2], im[2], re[3], im[3]};
+                lo += cv[0];
+                hi += cv[1];
+                cv[0] = lo;
+                cv[1] = hi;
+            } else {
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    if (rowb + v < M) {
+                        cp[v].re += re[v];
+                        cp[v].im += im[v];
+                    }
+            }
+        }
+}
+
+// Halves of the packed operands of products with M, N <= n and K <= kmax:
+// A and B, three planes each (B with one tile of columns to spare), per
+// matrix.
+inline size_t h16p_pack_halves(int n, int kmax)
+{
+    const size_t np = (size_t)round_up_host(n, HP_BM) + HP_BM, kp = (size_t)round_up_host(kmax, HP_BK);
+    return 2 * 3 * np * kp;
+}
+
+// C += alpha A B (complex64, column-major, batched) through the packed
+// operands in `pack` (h16p_pack_halves(max(M, N), K) halves per matrix).
+// The columns [0, n_first) of C and B are multiplied first, then the rest,
+// with `between` called in between (the look-ahead's fork).
+template <typename F>
+int gemm_h16p(int M, int N, int K, float alpha, const cx<float>* A, int64_t lda, int64_t sA, const cx<float>* B,
+              int64_t ldb, int64_t sB, cx<float>* C, int64_t ldc, int64_t sC, int batch, _Float16* pack,
+              int64_t sPack, int n_first, F between, hipStream_t stream)
+{
+    if (M <= 0 || N <= 0 || K <= 0 || batch <= 0) return 0;
+    // B's packed columns reach one tile beyond N: the second launch starts
+    // at a column that need not be a multiple of 128
+    const int Mp = round_up_host(M, HP_BM), Np = round_up_host(N, HP_BM) + HP_BM, Kp = round_up_host(K, HP_BK);
+    _Float16* Ap = pack;
+    _Float16* Bp = pack + (size_t)3 * Mp * Kp;
+    hipLaunchKernelGGL(pack_a_h16_kernel, dim3(Mp / 64, Kp / 32, batch), dim3(256), 0, stream, A, lda, sA, M, K, Ap,
+                       Mp, Kp, sPack);
+    const int segs = Kp / 8;
+    hipLaunchKernelGGL(pack_b_h16_kernel, dim3((Np * segs + 255) / 256, batch), dim3(256), 0, stream, B, ldb, sB, K, N,
+                       Bp, Np, Kp, sPack);
+    int e;
+    if ((e = (int)hipGetLastError())) return e;
+    auto launch = [&](int c_lo, int n) {
+        if (n <= 0) return;
+        // algorithmic bytes: the packed operands (three FP16 planes) read
+        // once, C read and written
+        TimedLaunch timed(stream, TK_CGEMM_PACKED, 8.0 * M * n * K * batch,
+                          (6.0 * ((double)M * K + (double)K * n) + 16.0 * M * n) * batch);
+        const int Mt = (M + HP_BM - 1) / HP_BM, Nt = (n + HP_BM - 1) / HP_BM;
+        hipLaunchKernelGGL(gemm_h16p_kernel, dim3(Mt * Nt * batch), dim3(256), 0, stream, M, n, Kp, alpha,
+                           (const _Float16*)Ap, Mp, sPack, (const _Float16*)Bp, Np, c_lo, sPack, C + (size_t)c_lo * ldc,
+                           ldc, sC, batch);
+    };
+    const int n1 = n_first > 0 && n_first < N ? n_first : N;
+    launch(0, n1);
+    if ((e = between())) return e;
+    launch(n1, N - n1);
+    return (int)hipGetLastError();
+}
+
 template <typename R, int NT, int RPT, int SB>
 void launch_panel(cx<R>* A, int64_t lda, int64_t sA, int N, int k0, int kb, int sw_lo, int sw_hi, int* ipiv,
                   int64_t sP, int* info, int batch, hipStream_t stream)
@@ -2131,12 +2431,12 @@ struct PanelCfg {
     int budget = MW_LEAF_BUDGET;
     int spin = 1 << 20;
     uint32_t seq = 1;
-    // speculative panels (1c): U11^-1 per matrix, the panel copy, the
-    // per-matrix check flags and the context's repair counter (may be null)
+    // speculative panels (1c): the panel copy, the per-matrix check flags
+    // and the context's repair counter (may be null)
     bool spec = false;
-    void* spec_uinv = nullptr;
     void* spec_bak = nullptr;
     int* spec_flags = nullptr;
+    int* spec_swapped = nullptr;  // per matrix: the panel interchanged rows (repair)
     int* spec_repairs = nullptr;
 };
 
@@ -2250,6 +2550,9 @@ int invert_diag(const cx<R>* LU, int N, int64_t lda, int64_t strideA, cx<R>* D, 
 template <typename R>
 int trsm_lower(const cx<R>* L, int64_t lda, int64_t sA, int kb, cx<R>* B, int64_t ldb, int64_t sB, int ncols,
                int batch, hipStream_t stream, cx<R>* inv = nullptr);
+template <typename R>
+int trsm_rec(const cx<R>* L, int64_t lda, int64_t sA, int kb, cx<R>* B, int64_t ldb, int64_t sB, int ncols, int batch,
+             hipStream_t stream, cx<R>* inv);
 
 // Recursive factorisation of columns [c0, c0 + w) of rows [c0, N) inside the
 // panel [sw_lo, sw_hi): factor the left half, solve its unit-lower triangle
@@ -2286,21 +2589,21 @@ int panel_rec(cx<R>* A, int64_t lda, int64_t sA, int N, int c0, int w, int sw_lo
 }
 
 // Speculative factorisation (1c) of columns [c0, c0 + w) of rows [c0, N):
-// leaves of at most PB columns (leading block in LDS, rows below by the
-// inverse of U11), joined as in panel_rec by trsm + MFMA GEMM.
+// leaves of at most spec_width columns (leading block by one wavefront,
+// rows below by forward substitution), joined as in panel_rec by trsm +
+// MFMA GEMM.
 template <typename R>
 int panel_rec_spec(cx<R>* A, int64_t lda, int64_t sA, int N, int c0, int w, int* ipiv, int64_t sP, int batch,
                    const PanelCfg& pc, hipStream_t stream)
 {
     int e;
-    if (w <= PB) {
-        cx<R>* U = (cx<R>*)pc.spec_uinv;
-        hipLaunchKernelGGL(spec_top_kernel<R>, dim3(batch), dim3(256), 0, stream, A, lda, sA, c0, w, ipiv, sP,
-                           pc.spec_flags, U);
+    if (w <= spec_width<R>()) {
+        hipLaunchKernelGGL(spec_top_kernel<R>, dim3(batch), dim3(64), 0, stream, A, lda, sA, c0, w, ipiv, sP,
+                           pc.spec_flags);
         const int rows = N - c0 - w;
         if (rows > 0)
-            hipLaunchKernelGGL(spec_rows_kernel<R>, dim3((rows + 63) / 64, batch), dim3(256), 0, stream, A, lda, sA, N,
-                               c0, w, pc.spec_flags, (const cx<R>*)U);
+            hipLaunchKernelGGL(spec_rows_kernel<R>, dim3((rows + 255) / 256, batch), dim3(256), 0, stream, A, lda, sA,
+                               N, c0, w, pc.spec_flags);
         return (int)hipGetLastError();
     }
     const int h = w / 2;
@@ -2310,7 +2613,7 @@ int panel_rec_spec(cx<R>* A, int64_t lda, int64_t sA, int N, int c0, int w, int*
     cx<R>* A12 = A + c0 + (size_t)(c0 + h) * lda;
     cx<R>* A21 = A + (c0 + h) + (size_t)c0 * lda;
     cx<R>* A22 = A + (c0 + h) + (size_t)(c0 + h) * lda;
-    if ((e = trsm_lower<R>(A11, lda, sA, h, A12, lda, sA, w - h, batch, stream))) return e;
+    if ((e = trsm_rec<R>(A11, lda, sA, h, A12, lda, sA, w - h, batch, stream, nullptr))) return e;
     if (M > h && (e = gemm<R>(M - h, w - h, h, R(-1), A21, lda, sA, A12, lda, sA, A22, lda, sA, 0, batch, stream)))
         return e;
     return panel_rec_spec<R>(A, lda, sA, N, c0 + h, w - h, ipiv, sP, batch, pc, stream);
@@ -2330,14 +2633,15 @@ int panel_dispatch(cx<R>* A, int64_t lda, int64_t sA, int N, int k0, int kb, int
                       4.0 * ((double)M * kb * kb - (double)kb * kb * kb / 3.0) * batch);
     if (pc.spec && kb <= SPEC_KB_MAX) {
         cx<R>* bak = (cx<R>*)pc.spec_bak;
-        const int64_t sBak = (int64_t)N * SPEC_KB_MAX;
-        hipLaunchKernelGGL(spec_backup_kernel<R>, dim3((M + 255) / 256, kb, batch), dim3(256), 0, stream,
-                           (const cx<R>*)A, lda, sA, N, k0, kb, bak, sBak);
+        const int64_t sBak = (int64_t)N * spec_bak_cols(N);
+        hipLaunchKernelGGL(spec_backup_kernel<R>, dim3((M + 1023) / 1024, kb, batch), dim3(256), 0, stream,
+                           (const cx<R>*)A, lda, sA, N, k0, kb, bak, sBak, pc.spec_swapped);
         int e;
         if ((e = (int)hipGetLastError())) return e;
         if ((e = panel_rec_spec<R>(A, lda, sA, N, k0, kb, ipiv, sP, batch, pc, stream))) return e;
         hipLaunchKernelGGL(spec_repair_kernel<R>, dim3(batch), dim3(512), 0, stream, A, lda, sA, N, k0, kb,
-                           (const cx<R>*)bak, sBak, ipiv, sP, info, pc.spec_flags, pc.spec_repairs);
+                           (const cx<R>*)bak, sBak, ipiv, sP, info, pc.spec_flags, pc.spec_swapped,
+                           pc.spec_repairs);
         return (int)hipGetLastError();
     }
     return panel_rec<R>(A, lda, sA, N, k0, kb, k0, k0 + kb, ipiv, sP, info, batch, pc, stream);
@@ -2433,6 +2737,23 @@ int trsm_lower(const cx<R>* L, int64_t lda, int64_t sA, int kb, cx<R>* B, int64_
     return 0;
 }
 
+// L^-1 B for a unit lower kb x kb block L of any size: halves of L
+// recursively (X1 = L11^-1 B1, B2 -= L21 X1 on MFMA with K = kb / 2, X2 =
+// L22^-1 B2) down to blocks of 256, which trsm_lower solves (with `inv`, room
+// for 256 x PB per matrix, or the wavefront triangle kernel): the products
+// keep a depth that makes them compute-bound.
+template <typename R>
+int trsm_rec(const cx<R>* L, int64_t lda, int64_t sA, int kb, cx<R>* B, int64_t ldb, int64_t sB, int ncols, int batch,
+             hipStream_t stream, cx<R>* inv)
+{
+    if (kb <= 4 * PB) return trsm_lower<R>(L, lda, sA, kb, B, ldb, sB, ncols, batch, stream, inv);
+    const int h = ((kb / 2 + PB - 1) / PB) * PB;
+    int e;
+    if ((e = trsm_rec<R>(L, lda, sA, h, B, ldb, sB, ncols, batch, stream, inv))) return e;
+    if ((e = gemm<R>(kb - h, ncols, h, R(-1), L + h, lda, sA, B, ldb, sB, B + h, ldb, sB, 0, batch, stream))) return e;
+    return trsm_rec<R>(L + h + (size_t)h * lda, lda, sA, kb - h, B + h, ldb, sB, ncols, batch, stream, inv);
+}
+
 // Block width of the right-looking factorisation: 128 columns halve the
 // trailing-matrix traffic per flop against 64 where the matrices are large
 // (256 halves it again, at the price of wider panels: a tuning choice).
@@ -2452,15 +2773,27 @@ size_t getrf_xbuf_elems(int batch)
     const size_t xbytes = sizeof(uint64_t) * (size_t)MW_XWORDS * batch;
     return (xbytes + sizeof(cx<R>) - 1) / sizeof(cx<R>);
 }
-// ... then for the speculative panels: U11^-1 (PB x PB per matrix), the
-// panel copy (N x SPEC_KB_MAX per matrix) and one 64-bit word of flag per
-// matrix
+// ... then for the speculative panels: the panel copy (N x min(N,
+// SPEC_KB_MAX) per matrix), the check flags and the interchange flags (a
+// 64-bit word each per matrix)
+template <typename R>
+size_t getrf_spec_elems(int N, int batch)
+{
+    const size_t flag_elems = (2 * sizeof(uint64_t) * (size_t)batch + sizeof(cx<R>) - 1) / sizeof(cx<R>);
+    return (size_t)N * spec_bak_cols(N) * batch + flag_elems;
+}
+// ... and, for complex64, the packed FP16 operands of the trailing updates
+template <typename R>
+size_t getrf_pack_elems(int N, int batch)
+{
+    if (sizeof(R) != 4) return 0;
+    return (h16p_pack_halves(N, spec_bak_cols(N)) * sizeof(_Float16) * batch + sizeof(cx<R>) - 1) / sizeof(cx<R>);
+}
 template <typename R>
 size_t getrf_ws_elems(int N, int batch)
 {
-    const size_t flag_elems = (sizeof(uint64_t) * (size_t)batch + sizeof(cx<R>) - 1) / sizeof(cx<R>);
-    return getrf_inv_elems<R>(batch) + getrf_xbuf_elems<R>(batch) + (size_t)PB * PB * batch +
-           (size_t)N * SPEC_KB_MAX * batch + flag_elems;
+    return getrf_inv_elems<R>(batch) + getrf_xbuf_elems<R>(batch) + getrf_spec_elems<R>(N, batch) +
+           getrf_pack_elems<R>(N, batch);
 }
 
 // Blocked right-looking LU with partial pivoting (LAPACK zgetrf semantics),
@@ -2488,7 +2821,7 @@ int getrf_steps(cx<R>* A, int N, int64_t lda, int64_t strideA, int* ipiv, int64_
                 cx<R>* work, const LuSetup* aux, hipStream_t stream)
 {
     int e;
-    const int nb = getrf_block(N, aux ? aux->block : 0);
+    int nb = getrf_block(N, aux ? aux->block : 0);
     PanelCfg pc;
     pc.leaf = aux ? aux->leaf : 16;
     pc.nt = aux ? aux->panel_threads : 512;
@@ -2502,16 +2835,28 @@ int getrf_steps(cx<R>* A, int N, int64_t lda, int64_t strideA, int* ipiv, int64_
         const size_t nwords = (size_t)MW_XWORDS * batch;
         hipLaunchKernelGGL(zero_words_kernel, dim3((unsigned)((nwords + 255) / 256)), dim3(256), 0, stream, pc.xbuf,
                            nwords);
-        if (aux && aux->speculate) {
+        if (aux && aux->speculate && nb <= SPEC_KB_MAX) {
             cx<R>* sp = work + getrf_inv_elems<R>(batch) + getrf_xbuf_elems<R>(batch);
             pc.spec = true;
-            pc.spec_uinv = sp;
-            pc.spec_bak = sp + (size_t)PB * PB * batch;
-            pc.spec_flags = (int*)(sp + (size_t)PB * PB * batch + (size_t)N * SPEC_KB_MAX * batch);
+            pc.spec_bak = sp;
+            pc.spec_flags = (int*)(sp + (size_t)N * spec_bak_cols(N) * batch);
+            pc.spec_swapped = pc.spec_flags + 2 * (size_t)batch;
             pc.spec_repairs = aux ? aux->spec_repairs : nullptr;
-            hipLaunchKernelGGL(zero_words_kernel, dim3((unsigned)((batch + 255) / 256)), dim3(256), 0, stream,
-                               (uint64_t*)pc.spec_flags, (size_t)batch);
+            hipLaunchKernelGGL(zero_words_kernel, dim3((unsigned)((2 * batch + 255) / 256)), dim3(256), 0, stream,
+                               (uint64_t*)pc.spec_flags, 2 * (size_t)batch);
         }
+    }
+    // blocks wider than 256 columns: speculative panels only (the pivot
+    // searches' trsm and leaves hold at most 256)
+    if (!pc.spec && nb > 4 * PB) nb = 4 * PB;
+    // complex64 with FP16-input products: packed operands for the trailing
+    // updates (blocks up to the panel copy's width)
+    _Float16* pack = nullptr;
+    int64_t sPack = 0;
+    if (sizeof(R) == 4 && work && active_gemm_tiles == 4 && nb <= spec_bak_cols(N)) {
+        pack = (_Float16*)(work + getrf_inv_elems<R>(batch) + getrf_xbuf_elems<R>(batch) +
+                           getrf_spec_elems<R>(N, batch));
+        sPack = (int64_t)h16p_pack_halves(N, spec_bak_cols(N));
     }
     const bool ahead = aux && aux->side && aux->lookahead;
     if ((e = panel_dispatch<R>(A, lda, strideA, N, 0, min(nb, N), ipiv, strideP, info, batch, pc, stream))) return e;
@@ -2519,7 +2864,8 @@ int getrf_steps(cx<R>* A, int N, int64_t lda, int64_t strideA, int* ipiv, int64_
         const int kb = min(nb, N - k0);
         // panel k0 is factored (on `stream`, or on the side stream joined below)
         hipLaunchKernelGGL(laswp_kernel<R>, dim3((N + 255) / 256, batch), dim3(256), 0, stream, A, lda, strideA, N, k0,
-                           k0 + kb, ipiv, strideP, k0, k0 + kb);
+                           k0 + kb, ipiv, strideP, k0, k0 + kb,
+                           pc.spec && kb <= SPEC_KB_MAX ? pc.spec_swapped : nullptr);
         if ((e = (int)hipGetLastError())) return e;
         const int rest = N - k0 - kb;
         if (rest <= 0) break;
@@ -2527,32 +2873,44 @@ int getrf_steps(cx<R>* A, int N, int64_t lda, int64_t strideA, int* ipiv, int64_
         cx<R>* A12 = A + k0 + (size_t)(k0 + kb) * lda;
         cx<R>* A21 = A + (k0 + kb) + (size_t)k0 * lda;
         cx<R>* A22 = A + (k0 + kb) + (size_t)(k0 + kb) * lda;
-        if ((e = trsm_lower<R>(A11, lda, strideA, kb, A12, lda, strideA, rest, batch, stream, work))) return e;
+        if ((e = trsm_rec<R>(A11, lda, strideA, kb, A12, lda, strideA, rest, batch, stream, work))) return e;
         const int kn = min(nb, rest);  // width of the next panel
-        // trailing update of the next panel's columns first ...
-        if ((e = gemm<R>(rest, kn, kb, R(-1), A21, lda, strideA, A12, lda, strideA, A22, lda, strideA, 0, batch,
-                         stream)))
-            return e;
         // a panel beside the trailing update gets one work-group per matrix
         // per leaf: the work-groups of a multi-work-group leaf must all be
         // resident at once, which a concurrent update of the whole device
         // delays until it drains; panels taller than one work-group's reach
         // are factored in order
         const bool side = ahead && rest <= PANEL_SINGLE_MAX_ROWS;
-        if (side) {
+        auto fork = [&]() -> int {
+            if (!side) return 0;
             // ... factor the next panel on the side stream while the rest of
             // the trailing matrix is updated on `stream` (disjoint columns)
             if (hipEventRecord(aux->ready, stream) != hipSuccess) return (int)hipErrorUnknown;
             if (hipStreamWaitEvent(aux->side, aux->ready, 0) != hipSuccess) return (int)hipErrorUnknown;
             PanelCfg one = pc;
             one.mw = false;
-            if ((e = panel_dispatch<R>(A, lda, strideA, N, k0 + kb, kn, ipiv, strideP, info, batch, one, aux->side)))
-                return e;
+            int e2;
+            if ((e2 = panel_dispatch<R>(A, lda, strideA, N, k0 + kb, kn, ipiv, strideP, info, batch, one, aux->side)))
+                return e2;
             if (hipEventRecord(aux->done, aux->side) != hipSuccess) return (int)hipErrorUnknown;
+            return 0;
+        };
+        if (pack) {
+            // complex64 with FP16-input products: operands packed once for
+            // both halves of the update (the next panel's columns first)
+            if ((e = gemm_h16p(rest, rest, kb, -1.0f, (const cx<float>*)A21, lda, strideA, (const cx<float>*)A12, lda,
+                               strideA, (cx<float>*)A22, lda, strideA, batch, pack, sPack, kn, fork, stream)))
+                return e;
+        } else {
+            // trailing update of the next panel's columns first ...
+            if ((e = gemm<R>(rest, kn, kb, R(-1), A21, lda, strideA, A12, lda, strideA, A22, lda, strideA, 0, batch,
+                             stream)))
+                return e;
+            if ((e = fork())) return e;
+            if (rest > kn && (e = gemm<R>(rest, rest - kn, kb, R(-1), A21, lda, strideA, A12 + (size_t)kn * lda, lda,
+                                          strideA, A22 + (size_t)kn * lda, lda, strideA, 0, batch, stream)))
+                return e;
         }
-        if (rest > kn && (e = gemm<R>(rest, rest - kn, kb, R(-1), A21, lda, strideA, A12 + (size_t)kn * lda, lda,
-                                      strideA, A22 + (size_t)kn * lda, lda, strideA, 0, batch, stream)))
-            return e;
         if (side) {
             if (hipStreamWaitEvent(stream, aux->done, 0) != hipSuccess) return (int)hipErrorUnknown;
         } else if ((e = panel_dispatch<R>(A, lda, strideA, N, k0 + kb, kn, ipiv, strideP, info, batch, pc,
@@ -2572,7 +2930,7 @@ int getrs(const cx<R>* LU, int N, int64_t lda, int64_t strideA, const int* ipiv,
 {
     if (N <= 0 || nrhs <= 0 || batch <= 0) return 0;
     hipLaunchKernelGGL(laswp_kernel<R>, dim3((nrhs + 255) / 256, batch), dim3(256), 0, stream, B, ldb, strideB, nrhs, 0,
-                       0, ipiv, strideP, 0, N);
+                       0, ipiv, strideP, 0, N, nullptr);
     // forward substitution with the unit lower factor: the rows below each
     // diagonal block, then every diagonal block in place
     const int nblk = (N + PB - 1) / PB;

@@ -311,7 +311,9 @@ enum {
     CPT_TIMED_ASSEMBLY = 4,  /* S/K assembly */
     CPT_TIMED_SKINNY_Z = 5,  /* complex128 product with <= 32 columns (solves, residuals, S X) */
     CPT_TIMED_SKINNY_C = 6,  /* complex64 product with <= 32 columns */
-    CPT_TIMED_KERNELS = 7
+    CPT_TIMED_CGEMM_PACKED = 7, /* complex64 trailing update on FP16 operands packed once per block
+                                   step (mixed precision, tile setting 4) */
+    CPT_TIMED_KERNELS = 8
 };
 int cpt_profile_begin(cpt_handle h);
 int cpt_profile_end(cpt_handle h, double* ms, double* work, int64_t* launches, double* bytes);
@@ -350,8 +352,9 @@ int cpt_release_workspace(cpt_handle h);
  * on them only through rounding: every value factors with LAPACK's partial
  * pivoting rule. */
 enum {
-    CPT_TUNE_LU_BLOCK = 0,       /* block width of the LU: 64, 128, 256, or 0 = by size (256 from
-                                    2048 rows, 128 from 1024, else 64) */
+    CPT_TUNE_LU_BLOCK = 0,       /* block width of the LU: 64, 128, 256, 512, 1024, or 0 = by size
+                                    (256 from 2048 rows, 128 from 1024, else 64); 512 and 1024 with
+                                    speculative panels only (else 256) */
     CPT_TUNE_PANEL_LEAF = 1,     /* panel columns factored by one work-group per matrix, 1..64 */
     CPT_TUNE_LOOKAHEAD = 2,      /* 0: in order; 1: next panel on a side stream.  2 (the next panel
                                     on CU-masked streams) returns CPT_ERR_UNSUPPORTED: the runtime

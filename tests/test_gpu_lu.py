@@ -433,7 +433,7 @@ def test_mixed_fp16_products_fall_back_to_fp32():
 def test_tuning_rejects_bad_values():
     from capytaine_amd import _lib
     ctx = _lib.Context.get(device())
-    for name, bad in [("lu_block", 96), ("lu_block", 512), ("panel_leaf", 0), ("panel_leaf", 65), ("lookahead", 3),
+    for name, bad in [("lu_block", 96), ("lu_block", 2048), ("panel_leaf", 0), ("panel_leaf", 65), ("lookahead", 3),
                       ("assembly_chunk", -1), ("gemm_tiles", 5), ("panel_threads", 128), ("panel_threads", 1024),
                       ("lookahead_active", 1), ("mixed_retries", 0), ("pivot_speculation", 2),
                       ("spec_repairs", 0)]:
@@ -701,15 +701,16 @@ def test_nan_pivot_column_stays_in_bounds(n, nb, speculation):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-@pytest.mark.parametrize("n,nb", [(517, 3), (2300, 2), (4352, 2)])
-def test_speculative_panels_without_interchanges(precision, n, nb):
+@pytest.mark.parametrize("n,nb,block", [(517, 3, 256), (2300, 2, 256), (4352, 2, 256), (2300, 2, 1024),
+                                        (4352, 2, 512), (4352, 2, 1024)])
+def test_speculative_panels_without_interchanges(precision, n, nb, block):
     """Matrices on which partial pivoting interchanges no rows (diagonal
     weight 3 against entries of about 1/sqrt(n), as on the BEM matrices):
     no panel is repaired, the pivots are LAPACK's (the identity) and the
     factor equals the pivot-search panels' to rounding."""
     from scipy.linalg import lu_factor, lu_solve
     from capytaine_amd import _lib
-    print(f"[gpu] speculative panels, no interchanges, n={n} batch={nb} {precision}", flush=True)
+    print(f"[gpu] speculative panels, no interchanges, n={n} batch={nb} block={block} {precision}", flush=True)
     rng = np.random.default_rng(n + 7 * nb)
     As, Bs = zip(*[_random_system(rng, n, 3, diag=3.0) for _ in range(nb)])
     As, Bs = np.array(As), np.array(Bs)
@@ -721,7 +722,8 @@ def test_speculative_panels_without_interchanges(precision, n, nb):
     out = {}
     try:
         for spec in (0, 1):
-            _lib.set_tuning(ctx, lu_block=256, lookahead=1, pivot_speculation=spec)
+            # the pivot-search panels take blocks of at most 256
+            _lib.set_tuning(ctx, lu_block=256 if spec == 0 else block, lookahead=1, pivot_speculation=spec)
             repairs = _lib.spec_repairs(ctx)
             out[spec] = _factor_solve(As, Bs) if precision == "fp64" else _mixed(As, Bs)
             assert _lib.spec_repairs(ctx) == repairs
@@ -743,7 +745,7 @@ def test_speculative_panels_without_interchanges(precision, n, nb):
         assert np.linalg.norm(X1 - X0) <= 1e-11 * np.linalg.norm(X0)
 
 
-@pytest.mark.parametrize("block", [64, 128, 256])
+@pytest.mark.parametrize("block", [64, 128, 256, 512, 1024])
 def test_speculation_repairs_every_panel(block):
     """No diagonal weight: partial pivoting interchanges rows at nearly
     every column, so every panel's check fails and every panel is repaired
