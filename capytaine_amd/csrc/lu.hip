@@ -923,18 +923,28 @@ __global__ void laswp_kernel(cx<R>* __restrict__ Abase, int64_t lda, int64_t str
                              const int* __restrict__ ipiv_base, int64_t strideP, int r0, int r1,
                              const int* __restrict__ swapped)
 {
-    const int col = blockIdx.x * blockDim.x + threadIdx.x;
-    if (col >= ncols || (col >= c_lo && col < c_hi)) return;
     // a speculative panel that passed its check interchanged no rows
     if (swapped && !swapped[blockIdx.y]) return;
+    __shared__ int sp[256];
+    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool mine = col < ncols && !(col >= c_lo && col < c_hi);
     cx<R>* A = Abase + (size_t)blockIdx.y * strideA + (size_t)col * lda;
     const int* ipiv = ipiv_base + (size_t)blockIdx.y * strideP;
-    for (int i = r0; i < r1; ++i) {
-        const int p = ipiv[i];
-        if (p != i) {
-            const cx<R> x = A[i], y = A[p];
-            A[i] = y;
-            A[p] = x;
+    // the pivots in chunks of 256 through LDS (one load per thread), then
+    // every column walks them in order
+    for (int i0 = r0; i0 < r1; i0 += 256) {
+        const int n = min(256, r1 - i0);
+        __syncthreads();
+        if ((int)threadIdx.x < n) sp[threadIdx.x] = ipiv[i0 + threadIdx.x];
+        __syncthreads();
+        if (!mine) continue;
+        for (int u = 0; u < n; ++u) {
+            const int i = i0 + u, p = sp[u];
+            if (p != i) {
+                const cx<R> x = A[i], y = A[p];
+                A[i] = y;
+                A[p] = x;
+            }
         }
     }
 }
@@ -1411,6 +1421,61 @@ __global__ void __launch_bounds__(256) gemm_small_kernel(int M, int N, int K, R 
 //     tiles are summed through LDS.  The stream of A from HBM bounds it.
 // ---------------------------------------------------------------------------
 constexpr int SKM = 64, SKN = 16, SKK = 16;
+
+// C = alpha A B (+ C) for a product with few rows (M <= 16) and a long K
+// (the generalised forces W^T Phi: 6 dof x nrhs over the panels): one
+// work-group per (column, matrix), each thread summing every row over
+// k = tid, tid + 256, ..., then a reduction across the work-group.
+constexpr int FEW_ROWS = 16;
+
+template <typename R>
+__global__ void __launch_bounds__(256) gemm_fewrows_kernel(int M, int K, R alpha, const cx<R>* __restrict__ Abase,
+                                                           int64_t lda, int64_t strideA, const cx<R>* __restrict__ Bbase,
+                                                           int64_t ldb, int64_t strideB, cx<R>* __restrict__ Cbase,
+                                                           int64_t ldc, int64_t strideC, int beta_zero)
+{
+    typedef cx<R> C;
+    __shared__ C red[4][FEW_ROWS];
+    const int col = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const C* A = Abase + (size_t)blockIdx.y * strideA;
+    const C* B = Bbase + (size_t)blockIdx.y * strideB + (size_t)col * ldb;
+    C* Cc = Cbase + (size_t)blockIdx.y * strideC + (size_t)col * ldc;
+    C acc[FEW_ROWS];
+#pragma unroll
+    for (int i = 0; i < FEW_ROWS; ++i) acc[i] = C{R(0), R(0)};
+    for (int k = tid; k < K; k += 256) {
+        const C x = B[k];
+#pragma unroll
+        for (int i = 0; i < FEW_ROWS; ++i)
+            if (i < M) {
+                const C a = A[i + (size_t)k * lda];
+                acc[i].re += a.re * x.re - a.im * x.im;
+                acc[i].im += a.re * x.im + a.im * x.re;
+            }
+    }
+#pragma unroll
+    for (int i = 0; i < FEW_ROWS; ++i) {
+        R re = acc[i].re, im = acc[i].im;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            re += __shfl_xor(re, off);
+            im += __shfl_xor(im, off);
+        }
+        if (lane == 0) red[wave][i] = C{re, im};
+    }
+    __syncthreads();
+    if (tid < M) {
+        R re = R(0), im = R(0);
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            re += red[w][tid].re;
+            im += red[w][tid].im;
+        }
+        C* c = Cc + tid;
+        c->re = (beta_zero ? R(0) : c->re) + alpha * re;
+        c->im = (beta_zero ? R(0) : c->im) + alpha * im;
+    }
+}
 
 template <typename R>
 __global__ void __launch_bounds__(256) gemm_skinny_kernel(int M, int N, int K, R alpha,
@@ -2652,6 +2717,11 @@ int gemm(int M, int N, int K, R alpha, const cx<R>* A, int64_t lda, int64_t sA, 
          int64_t sB, cx<R>* C, int64_t ldc, int64_t sC, int beta_zero, int batch, hipStream_t stream)
 {
     if (M <= 0 || N <= 0 || batch <= 0) return 0;
+    if (M <= FEW_ROWS && K >= 256) {
+        hipLaunchKernelGGL(gemm_fewrows_kernel<R>, dim3(N, batch), dim3(256), 0, stream, M, K, alpha, A, lda, sA, B,
+                           ldb, sB, C, ldc, sC, beta_zero);
+        return (int)hipGetLastError();
+    }
     if (N <= 2 * SKN && M >= SKM && K >= 4 * SKK) {
         dim3 grid((M + SKM - 1) / SKM, (N + SKN - 1) / SKN, batch);
         const double nbytes =
