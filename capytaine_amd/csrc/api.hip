@@ -228,14 +228,16 @@ __global__ void pack_faces_kernel(const double* __restrict__ vertices, int nv, c
     }
 }
 
+template <bool UNTAB>
 __global__ void wave_points_kernel(Table T, int np, const double* __restrict__ x0i, const double* __restrict__ x0j,
                                    double k, int gfs, cplx* __restrict__ g, cplx* __restrict__ ng)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= np) return;
     cplx G, nG[3];
-    wave_point(T, k, gfs, x0i[3 * (size_t)p] - x0j[3 * (size_t)p], x0i[3 * (size_t)p + 1] - x0j[3 * (size_t)p + 1],
-               x0i[3 * (size_t)p + 2] + x0j[3 * (size_t)p + 2], G, nG);
+    wave_point<UNTAB>(T, k, gfs, x0i[3 * (size_t)p] - x0j[3 * (size_t)p],
+                      x0i[3 * (size_t)p + 1] - x0j[3 * (size_t)p + 1], x0i[3 * (size_t)p + 2] + x0j[3 * (size_t)p + 2],
+                      G, nG);
     g[p] = G;
     for (int c = 0; c < 3; ++c) ng[3 * (size_t)p + c] = nG[c];
 }
@@ -529,8 +531,22 @@ int cpt_set_tabulation(cpt_handle h, const double* r_range, int nr, const double
         return CPT_ERR_ARG;
     }
     if (nr <= 1 || nz <= 1) {
-        set_error("cpt_set_tabulation: untabulated evaluation (nr or nz <= 1) is not supported on the device");
-        return CPT_ERR_UNSUPPORTED;
+        // no table: the wave term integrates over theta for each point
+        // (Green_wave.f90:226-228), with nb_integration_points Simpson points
+        if (nb_integration_points < 1) {
+            set_error("cpt_set_tabulation: nb_integration_points must be positive");
+            return CPT_ERR_ARG;
+        }
+        h->nr = nr;
+        h->nz = nz;
+        h->grid = grid_shape;
+        h->npts = nb_integration_points;
+        h->rmax = 0.0;
+        h->zmax_abs = 0.0;
+        h->dz = 0.0;
+        h->index_of_1 = 0;
+        h->has_table = true;
+        return CPT_OK;
     }
     CPT_CHECK_HIP(hipSetDevice(h->device), "hipSetDevice");
     if (h->r_range.reserve(sizeof(double) * nr) || h->z_range.reserve(sizeof(double) * nz) ||
@@ -860,7 +876,8 @@ int cpt_build_matrices_batched(cpt_handle h, int nb_bodies, const cpt_body* bodi
     (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->device);
     const int grid = (int)std::min<size_t>(sorted.size(), (size_t)ncu * 2);
     const int e = launch_assembly_worklist((const AssemblyArgs*)h->bargs.p, (const AsmTile*)h->btiles.p,
-                                           (int)sorted.size(), grid, pairs, h->far_fp32, stream);
+                                           (int)sorted.size(), grid, pairs, h->far_fp32, h->nr <= 1 || h->nz <= 1,
+                                           stream);
     CPT_CHECK_HIP((hipError_t)e, "batched assembly kernel launch");
     // The host tables above are copied from pageable memory: wait for the
     // copies before they go out of scope.
@@ -882,8 +899,12 @@ int cpt_wave_part_infinite_depth(cpt_handle h, int np, const double* x0i, const 
     if (np == 0) return CPT_OK;
     CPT_CHECK_HIP(hipSetDevice(h->device), "hipSetDevice");
     CallScope call_scope(&h->timer, h->gemm_tiles);
-    hipLaunchKernelGGL(wave_points_kernel, dim3((np + 255) / 256), dim3(256), 0, stream, table_of(h), np, x0i, x0j,
-                       wavenumber, gf_singularities, (cplx*)g, (cplx*)nablag);
+    if (h->nr <= 1 || h->nz <= 1)
+        hipLaunchKernelGGL(wave_points_kernel<true>, dim3((np + 255) / 256), dim3(256), 0, stream, table_of(h), np,
+                           x0i, x0j, wavenumber, gf_singularities, (cplx*)g, (cplx*)nablag);
+    else
+        hipLaunchKernelGGL(wave_points_kernel<false>, dim3((np + 255) / 256), dim3(256), 0, stream, table_of(h), np,
+                           x0i, x0j, wavenumber, gf_singularities, (cplx*)g, (cplx*)nablag);
     CPT_CHECK_HIP(hipGetLastError(), "wave kernel launch");
     return CPT_OK;
 }

@@ -25,6 +25,7 @@ struct WaveOut {
 // (x, xi) = collocation point and panel; qp points to nq records (x,y,z,w);
 // zsign/zshift map the panel's z coordinates for the sea-bottom image
 // (z -> zsign*z + zshift) used by the finite-depth images.
+template <bool UNTAB = false>
 __device__ __forceinline__ void wave_panel(const Table& T, double k, int gfs, bool deriv_first, double x0, double x1,
                                            double x2, double cx, double cy, double cz, double area, const double* qp,
                                            int nq, double zsign, double zshift, WaveOut& o, bool ftab = false)
@@ -42,7 +43,7 @@ __device__ __forceinline__ void wave_panel(const Table& T, double k, int gfs, bo
         for (int q = 0; q < nq; ++q) {
             const double qx = qp[4 * q], qy = qp[4 * q + 1], qz = zsign * qp[4 * q + 2] + zshift, w = qp[4 * q + 3];
             cplx g, ng[3];
-            wave_point(T, k, gfs, x0 - qx, x1 - qy, x2 + qz, g, ng, ftab);
+            wave_point<UNTAB>(T, k, gfs, x0 - qx, x1 - qy, x2 + qz, g, ng, ftab);
             o.G.re += g.re * w;
             o.G.im += g.im * w;
 #pragma unroll
@@ -78,6 +79,7 @@ __device__ __forceinline__ void acc(WaveOut& a, const WaveOut& b, bool flip_z)
 }
 
 // integral_of_wave_parts_finite_depth, Green_wave.f90:311-436.
+template <bool UNTAB = false>
 __device__ __forceinline__ void wave_panel_finite_depth(const Table& T, double k, double depth, int gfs,
                                                         bool deriv_first, double x0, double x1, double x2, double cx,
                                                         double cy, double cz, double area, const double* qp, int nq,
@@ -88,13 +90,13 @@ __device__ __forceinline__ void wave_panel_finite_depth(const Table& T, double k
 #pragma unroll
     for (int c = 0; c < 3; ++c) o.nG[c].re = o.nG[c].im = 0.0;
     const double xs2 = -x2 - 2 * depth;
-    wave_panel(T, k, gfs, deriv_first, x0, x1, x2, cx, cy, cz, area, qp, nq, 1.0, 0.0, t);
+    wave_panel<UNTAB>(T, k, gfs, deriv_first, x0, x1, x2, cx, cy, cz, area, qp, nq, 1.0, 0.0, t);
     acc(o, t, false);
-    wave_panel(T, k, gfs, deriv_first, x0, x1, xs2, cx, cy, cz, area, qp, nq, 1.0, 0.0, t);
+    wave_panel<UNTAB>(T, k, gfs, deriv_first, x0, x1, xs2, cx, cy, cz, area, qp, nq, 1.0, 0.0, t);
     acc(o, t, deriv_first);
-    wave_panel(T, k, gfs, deriv_first, x0, x1, x2, cx, cy, cz, area, qp, nq, -1.0, -2 * depth, t);
+    wave_panel<UNTAB>(T, k, gfs, deriv_first, x0, x1, x2, cx, cy, cz, area, qp, nq, -1.0, -2 * depth, t);
     acc(o, t, !deriv_first);
-    wave_panel(T, k, gfs, deriv_first, x0, x1, xs2, cx, cy, cz, area, qp, nq, -1.0, -2 * depth, t);
+    wave_panel<UNTAB>(T, k, gfs, deriv_first, x0, x1, xs2, cx, cy, cz, area, qp, nq, -1.0, -2 * depth, t);
     acc(o, t, true);
     const double amh = k * depth;
     const double akh = amh * tanh(amh);
@@ -110,20 +112,23 @@ __device__ __forceinline__ void wave_panel_finite_depth(const Table& T, double k
 
 // One tile: collocation points iblk * 256 + threadIdx.x against panels
 // jc * j_chunk .. + j_chunk, for every wavenumber of A.
-template <bool FINITE_DEPTH, bool FAR32 = false>
+template <bool FINITE_DEPTH, bool FAR32 = false, bool UNTAB = false>
 __device__ __forceinline__ void assemble_tile(const AssemblyArgs& A, int iblk, int jc);
 
 // Infinite depth: two wavefronts per SIMD (about 220 VGPRs, no spills of the
 // main path); finite depth carries more live state and runs at one.
+// UNTAB: the wave term integrated for each point instead of read from a table.
+template <bool UNTAB>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 assemble_infinite_depth_kernel(AssemblyArgs A)
 {
-    assemble_tile<false>(A, blockIdx.x, blockIdx.y);
+    assemble_tile<false, false, UNTAB>(A, blockIdx.x, blockIdx.y);
 }
 
+template <bool UNTAB>
 __global__ void __launch_bounds__(256) assemble_finite_depth_kernel(AssemblyArgs A)
 {
-    assemble_tile<true>(A, blockIdx.x, blockIdx.y);
+    assemble_tile<true, false, UNTAB>(A, blockIdx.x, blockIdx.y);
 }
 
 // Mixed-precision form: far pairs read the float table.
@@ -136,17 +141,17 @@ assemble_infinite_depth_mixed_kernel(AssemblyArgs A)
 // Persistent form over many bodies: each work-group walks the work-list of
 // (body, row block, panel chunk) tiles with a grid stride.  The tiles are
 // ordered by decreasing cost on the host so the stride balances the load.
-template <bool FAR32>
+template <bool FAR32, bool UNTAB>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 assemble_worklist_kernel(const AssemblyArgs* __restrict__ bodies, const AsmTile* __restrict__ tiles, int ntiles)
 {
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const AsmTile tl = tiles[t];
-        assemble_tile<false, FAR32>(bodies[tl.body], tl.iblk, tl.jc);
+        assemble_tile<false, FAR32, UNTAB>(bodies[tl.body], tl.iblk, tl.jc);
     }
 }
 
-template <bool FINITE_DEPTH, bool FAR32>
+template <bool FINITE_DEPTH, bool FAR32, bool UNTAB>
 __device__ __forceinline__ void assemble_tile(const AssemblyArgs& A, int iblk, int jc)
 {
     const int I = iblk * blockDim.x + threadIdx.x;
@@ -224,8 +229,8 @@ __device__ __forceinline__ void assemble_tile(const AssemblyArgs& A, int iblk, i
             if (!FINITE_DEPTH) {
                 if (P.wave) {
                     WaveOut w;
-                    wave_panel(A.T, P.k, P.gfs, deriv_first, x0, x1, x2, cx, cy, cz, area, qp, A.nq,
-                               1.0, 0.0, w, ftab);
+                    wave_panel<UNTAB>(A.T, P.k, P.gfs, deriv_first, x0, x1, x2, cx, cy, cz, area, qp, A.nq,
+                                      1.0, 0.0, w, ftab);
                     acc(o, w, false);
                 }
             } else {
@@ -242,8 +247,8 @@ __device__ __forceinline__ void assemble_tile(const AssemblyArgs& A, int iblk, i
                 }
                 if (P.wave) {
                     WaveOut w;
-                    wave_panel_finite_depth(A.T, P.k, A.depth, P.gfs, deriv_first, x0, x1, x2, cx, cy,
-                                            cz, area, qp, A.nq, w);
+                    wave_panel_finite_depth<UNTAB>(A.T, P.k, A.depth, P.gfs, deriv_first, x0, x1, x2, cx,
+                                                   cy, cz, area, qp, A.nq, w);
                     acc(o, w, false);
                 }
                 // integral_of_prony_decomp_finite_depth, Green_wave.f90:438-519
@@ -308,21 +313,36 @@ int launch_assembly(const AssemblyArgs& A, hipStream_t stream)
     dim3 grid(gx, gy);
     // stores of S and K: 32 bytes per pair and wavenumber
     TimedLaunch timed(stream, TK_ASSEMBLY, (double)A.n * A.m * A.nb, 32.0 * A.n * A.m * A.nb);
-    if (A.finite_depth) hipLaunchKernelGGL(assemble_finite_depth_kernel, grid, block, 0, stream, A);
-    else if (A.far_fp32) hipLaunchKernelGGL(assemble_infinite_depth_mixed_kernel, grid, block, 0, stream, A);
-    else hipLaunchKernelGGL(assemble_infinite_depth_kernel, grid, block, 0, stream, A);
+    // without a table there is no float copy either: the FP64 integration
+    // serves both forms
+    const bool untab = A.T.nr <= 1 || A.T.nz <= 1;
+    if (A.finite_depth) {
+        if (untab) hipLaunchKernelGGL(assemble_finite_depth_kernel<true>, grid, block, 0, stream, A);
+        else hipLaunchKernelGGL(assemble_finite_depth_kernel<false>, grid, block, 0, stream, A);
+    } else if (untab) {
+        hipLaunchKernelGGL(assemble_infinite_depth_kernel<true>, grid, block, 0, stream, A);
+    } else if (A.far_fp32) {
+        hipLaunchKernelGGL(assemble_infinite_depth_mixed_kernel, grid, block, 0, stream, A);
+    } else {
+        hipLaunchKernelGGL(assemble_infinite_depth_kernel<false>, grid, block, 0, stream, A);
+    }
     return (int)hipGetLastError();
 }
 
 int launch_assembly_worklist(const AssemblyArgs* bodies, const AsmTile* tiles, int ntiles, int grid, double pairs,
-                             int far_fp32, hipStream_t stream)
+                             int far_fp32, bool untabulated, hipStream_t stream)
 {
     if (ntiles <= 0) return 0;
     TimedLaunch timed(stream, TK_ASSEMBLY, pairs, 32.0 * pairs);
-    if (far_fp32)
-        hipLaunchKernelGGL(assemble_worklist_kernel<true>, dim3(grid), dim3(256), 0, stream, bodies, tiles, ntiles);
+    if (untabulated)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(assemble_worklist_kernel<false, true>), dim3(grid), dim3(256), 0, stream,
+                           bodies, tiles, ntiles);
+    else if (far_fp32)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(assemble_worklist_kernel<true, false>), dim3(grid), dim3(256), 0, stream,
+                           bodies, tiles, ntiles);
     else
-        hipLaunchKernelGGL(assemble_worklist_kernel<false>, dim3(grid), dim3(256), 0, stream, bodies, tiles, ntiles);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(assemble_worklist_kernel<false, false>), dim3(grid), dim3(256), 0, stream,
+                           bodies, tiles, ntiles);
     return (int)hipGetLastError();
 }
 

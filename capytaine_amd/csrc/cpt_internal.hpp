@@ -46,7 +46,7 @@ struct AsmTile {
     int body, iblk, jc, pad;
 };
 int launch_assembly_worklist(const AssemblyArgs* bodies, const AsmTile* tiles, int ntiles, int grid, double pairs,
-                             int far_fp32, hipStream_t stream);
+                             int far_fp32, bool untabulated, hipStream_t stream);
 
 // Tuning of the blocked LU and the side stream / events of its look-ahead
 // (owned by a context).

@@ -11,6 +11,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include "integrals.hpp"
 
 namespace cpt {
 
@@ -428,7 +429,10 @@ __device__ __forceinline__ void pick_in_float_table(const Table& T, double r, do
 // source point xi (already differenced in the horizontal plane): dx, dy are
 // x - xi, zsum is x3 + xi3.  Gradient with respect to x.
 // With ftab, the table's values come from its float copy (far pairs of the
-// mixed-precision assembly).
+// mixed-precision assembly).  UNTAB: no table, the integrals are evaluated
+// for each point (Green_wave.f90:226-228), in kernels of their own so that
+// the inlined integration does not weigh on the tabulated kernels' registers.
+template <bool UNTAB = false>
 __device__ __forceinline__ void wave_point(const Table& T, double k, int gfs, double dx, double dy, double zsum,
                                            cplx& G, cplx* nG, bool ftab = false)
 {
@@ -441,7 +445,9 @@ __device__ __forceinline__ void wave_point(const Table& T, double k, int gfs, do
         drdx2 = k * k * dy / r;
     }
     double I[5];
-    if (fabs(z) < T.zmax_abs && r < T.rmax) {
+    if (UNTAB) {
+        numerical_integration_device(r, z, T.npts, I);
+    } else if (fabs(z) < T.zmax_abs && r < T.rmax) {
         if (ftab) pick_in_float_table(T, r, z, I);
         else pick_in_table(T, r, z, I);
     } else {

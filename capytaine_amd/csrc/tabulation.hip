@@ -25,6 +25,7 @@
 #include <thread>
 #include <vector>
 #include "cpt_internal.hpp"
+#include "integrals.hpp"
 
 // The C library's float complex logarithm (C99 clogf).
 extern "C" float _Complex clogf(float _Complex);
@@ -33,104 +34,6 @@ extern "C" float _Complex clogf(float _Complex);
 
 namespace cpt {
 namespace {
-
-struct cf { float re, im; };
-struct cdd { double re, im; };
-
-__device__ __forceinline__ cf mk(float a, float b) { return cf{a, b}; }
-__device__ __forceinline__ cf add(cf a, cf b) { return mk(a.re + b.re, a.im + b.im); }
-__device__ __forceinline__ cf addr(float s, cf a) { return mk(s + a.re, a.im); }
-__device__ __forceinline__ cf mul(cf a, cf b) { return mk(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re); }
-__device__ __forceinline__ cf mulr(float s, cf a) { return mk(s * a.re, s * a.im); }
-
-__device__ __forceinline__ cf cdiv(cf a, cf b)
-{
-    int ilogbw = 0;
-    float c = b.re, d = b.im;
-    const float logbw = logbf(fmaxf(fabsf(c), fabsf(d)));
-    if (isfinite(logbw)) {
-        ilogbw = (int)logbw;
-        c = scalbnf(c, -ilogbw);
-        d = scalbnf(d, -ilogbw);
-    }
-    const float denom = c * c + d * d;
-    return mk(scalbnf((a.re * c + a.im * d) / denom, -ilogbw), scalbnf((a.im * c - a.re * d) / denom, -ilogbw));
-}
-
-__device__ __forceinline__ cdd ddiv(cdd a, cdd b)
-{
-    int ilogbw = 0;
-    double c = b.re, d = b.im;
-    const double logbw = logb(fmax(fabs(c), fabs(d)));
-    if (isfinite(logbw)) {
-        ilogbw = (int)logbw;
-        c = scalbn(c, -ilogbw);
-        d = scalbn(d, -ilogbw);
-    }
-    const double denom = c * c + d * d;
-    return cdd{scalbn((a.re * c + a.im * d) / denom, -ilogbw), scalbn((a.im * c - a.re * d) / denom, -ilogbw)};
-}
-
-// exp(z) E1(z) in float32, Delhommeau_integrals.f90:104-151.  `lz` points
-// at log(z) for the logarithmic branch (z.re > -0.5, |z.im| <= 10).
-__device__ cf exp_e1(double zr, double zi, const float2* lz_ptr)
-{
-    const cf z = mk((float)zr, (float)zi);
-    if (z.re < -16.0f) {
-        const cf y = cdiv(mk(1.0f, 0.0f), z);
-        cf t = addr(24.0f, mulr(-120.0f, y));
-        t = addr(-6.0f, mul(y, t));
-        t = addr(2.0f, mul(y, t));
-        t = addr(-1.0f, mul(y, t));
-        t = addr(1.0f, mul(y, t));
-        return mul(y, t);
-    }
-    if (fabsf(z.im) > 10.0f) {
-        return add(add(cdiv(mk(0.711093f, 0.0f), addr(0.415775f, z)), cdiv(mk(0.278518f, 0.0f), addr(2.29428f, z))),
-                   cdiv(mk(0.010389f, 0.0f), addr(6.2900f, z)));
-    }
-    if (z.re > -0.5f) {
-        const float2 l = *lz_ptr;
-        const cf lz = mk(l.x, l.y);
-        cf p1 = addr(0.763297e-03f, mulr(0.97087007e-05f, z));
-        p1 = addr(0.206543e-01f, mul(z, p1));
-        p1 = addr(0.23721365e+00f, mul(z, p1));
-        p1 = addr(0.1e+01f, mul(z, p1));
-        cf p2 = addr(0.19387339e-02f, mulr(-0.51801555e-03f, z));
-        p2 = addr(-0.3000591e-01f, mul(z, p2));
-        p2 = addr(0.41806426e-01f, mul(z, p2));
-        p2 = addr(-0.149545886e+01f, mul(z, p2));
-        p2 = addr(0.99999207e+00f, mul(z, p2));
-        cf q = addr(-0.8700861e-03f, mulr(0.2989204e-03f, z));
-        q = addr(0.12982719e-01f, mul(z, q));
-        q = addr(-0.66786033e-01f, mul(z, q));
-        q = addr(0.28388363e+00f, mul(z, q));
-        q = addr(-0.76273617e+00f, mul(z, q));
-        q = addr(0.1e+01f, mul(z, q));
-        const cf r = cdiv(add(mul(lz, p1), mulr(0.5772156649e+00f, p2)), q);
-        return mk(-r.re, -r.im);
-    }
-    // Rational fit in the left half plane; conjugate coefficients for Im z >= 0.
-    const float s = (z.im < 0.0f) ? 1.0f : -1.0f;
-    cf num = mul(mk(1.000000f, s * 1.3935496e-06f), z);
-    num = mul(add(num, mk(15.82958f, s * -20.14222f)), z);
-    num = mul(add(num, mk(-70.52863f, s * -227.9511f)), z);
-    num = mul(add(num, mk(-985.4221f, s * -226.6272f)), z);
-    num = mul(add(num, mk(-1202.318f, s * 1580.907f)), z);
-    num = mul(add(num, mk(953.2441f, s * 1342.447f)), z);
-    num = mul(add(num, mk(417.3716f, s * -196.6665f)), z);
-    num = add(num, mk(-9.881266f, s * -24.24952f));
-    cf den = mul(mk(1.000000f, s * 0.0f), z);
-    den = mul(add(den, mk(16.83184f, s * -20.14481f)), z);
-    den = mul(add(den, mk(-55.66969f, s * -248.1167f)), z);
-    den = mul(add(den, mk(-1068.640f, s * -434.4707f)), z);
-    den = mul(add(den, mk(-2082.250f, s * 1522.471f)), z);
-    den = mul(add(den, mk(383.3455f, s * 2730.378f)), z);
-    den = mul(add(den, mk(1216.791f, s * 351.7189f)), z);
-    den = mul(add(den, mk(115.3926f, s * -161.2647f)), z);
-    den = add(den, mk(-3.777369f, s * -4.510900f));
-    return cdiv(num, den);
-}
 
 // Writes the rows `rows` (nrows z indices) of the table in the reference
 // layout (nr, nz, 5), column-major.  Host-library values: ct (n), ez (nz,
@@ -176,7 +79,7 @@ __global__ void __launch_bounds__(256) tabulate_kernel(const double* __restrict_
         // the host supplies log(z) exactly where exp_e1 takes its log branch
         const int u = yidx[ki];
         const float2* lz = (lrow && u >= 0) ? lrow + u : &nan2;
-        const cf g = exp_e1(zr, zi, lz);
+        const cf g = exp_e1(zr, zi, [lz](float, float) { return *lz; });
         // ii*pi*exp_zeta = (0, pi) * exp_zeta
         const double pr = 0.0 * er - PI * ei, pi_ = 0.0 * ei + PI * er;
         const double jr = (double)g.re + pr, ji = (double)g.im + pi_;

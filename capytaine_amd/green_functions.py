@@ -107,6 +107,8 @@ def default_z_spacing(nz, zmin, method):
         if nz == 46:
             out[45] = -16.0
         return out
+    if nz == 0:
+        return out
     dz = (_libm.log10(abs(zmin)) + 10.0) / nz
     for j in range(1, nz + 1):
         out[j - 1] = -min(_libm.pow(10.0, dz * j - 10.0), abs(zmin))
@@ -226,9 +228,6 @@ class Delhommeau(AbstractGreenFunction):
         if tabulation_cache_dir == "default":
             tabulation_cache_dir = cache_directory()
         self.tabulation_cache_dir = tabulation_cache_dir
-        if tabulation_nr <= 1 or tabulation_nz <= 1:
-            raise NotImplementedError("Untabulated evaluation (tabulation_nr or tabulation_nz <= 1) "
-                                      "is not supported by the device kernels.")
         self._create_or_load_tabulation(tabulation_nr, float(tabulation_rmax), tabulation_nz,
                                         float(tabulation_zmin), tabulation_nb_integration_points,
                                         tabulation_cache_dir)
@@ -290,6 +289,12 @@ class Delhommeau(AbstractGreenFunction):
         import torch
         r = default_r_spacing(nr, rmax, self.tabulation_grid_shape_index)
         z = default_z_spacing(nz, zmin, self.tabulation_grid_shape_index)
+        if nr <= 1 or nz <= 1:
+            # no usable table: the device integrates the wave term for each
+            # point (Green_wave.f90:226-228); the table keeps the reference's
+            # shape, its values unused
+            self._set_tabulation(r, z, np.zeros((nr, nz, 5), order="F"))
+            return
         ctx = _lib.Context.get(self.device)
         r_d = torch.as_tensor(r, dtype=torch.float64).to(self.device)
         z_d = torch.as_tensor(z, dtype=torch.float64).to(self.device)

@@ -72,7 +72,11 @@ int cpt_tabulate(cpt_handle h, const double* r_range, int nr, const double* z_ra
  * Takes the arguments the reference passes to build_matrices as
  * `*self.all_tabulation_parameters` (delhommeau.py:263-267, 505):
  * nb_integration_points, grid_shape, r_range, z_range, tabulated_integrals.
- * Synchronises `stream`. */
+ * nr <= 1 or nz <= 1 selects the untabulated evaluation (the pointers may
+ * then be NULL): the wave term integrates over theta with
+ * nb_integration_points Simpson points for each point, as the reference
+ * does for such a table (Green_wave.f90:226-228), with the device's
+ * transcendental functions.  Synchronises `stream`. */
 int cpt_set_tabulation(cpt_handle h, const double* r_range, int nr, const double* z_range, int nz,
                        const double* tab, int grid_shape, int nb_integration_points, hipStream_t stream);
 
