@@ -17,6 +17,7 @@ CPT_LEGACY_GRID, CPT_SCALED_NEMOH3_GRID = 0, 1
 CPT_LEGACY_FINITE_DEPTH, CPT_NEWER_FINITE_DEPTH = 0, 1
 CPT_HIGH_FREQ, CPT_LOW_FREQ, CPT_LOW_FREQ_WITH_RANKINE_PART = 0, 1, 2
 CPT_BLOCKS_CYCLIC, CPT_BLOCKS_XOR = 0, 1
+CPT_ASSEMBLY_FAR_FIELD_FP32, CPT_ASSEMBLY_COMPLEX64 = 1, 2
 
 # Every symbol of include/capytaine_amd.h with its argument types.
 _P, _I, _L, _D = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_double
@@ -33,6 +34,9 @@ SIGNATURES = {
     "cpt_lu_factor": (_I, [_P, _P, _I, _L, _L, _P, _L, _P, _I, _P]),
     "cpt_lu_solve": (_I, [_P, _P, _I, _L, _L, _P, _L, _P, _I, _L, _L, _I, _P]),
     "cpt_zgemm": (_I, [_P, _I, _I, _I, _D, _P, _L, _L, _P, _L, _L, _P, _L, _L, _I, _I, _P]),
+    "cpt_clu_factor": (_I, [_P, _P, _I, _L, _L, _P, _L, _P, _I, _P]),
+    "cpt_clu_solve": (_I, [_P, _P, _I, _L, _L, _P, _L, _P, _I, _L, _L, _I, _P]),
+    "cpt_cgemm": (_I, [_P, _I, _I, _I, ctypes.c_float, _P, _L, _L, _P, _L, _L, _P, _L, _L, _I, _I, _P]),
     "cpt_lu_solve_mixed": (_I, [_P, _P, _I, _L, _L, _I, _P, _I, _L, _L, _P, _L, _L, _I, _D,
                                 ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double), _P]),
     "cpt_profile_begin": (_I, [_P]),

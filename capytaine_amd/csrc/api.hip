@@ -41,6 +41,15 @@ int launch_getrs_lanes(const LuGroupArgs* g, int ng, void* work);
 size_t getrs_workspace_bytes(int N, int batch);
 int launch_getrs(const cplx* LU, int N, int64_t lda, int64_t strideA, const int* ipiv, int64_t strideP, cplx* B,
                  int nrhs, int64_t ldb, int64_t strideB, int batch, cplx* work, hipStream_t stream);
+int launch_cgemm(int M, int N, int K, float alpha, const void* A, int64_t lda, int64_t sA, const void* B,
+                 int64_t ldb, int64_t sB, void* C, int64_t ldc, int64_t sC, int beta_zero, int batch,
+                 hipStream_t stream);
+size_t cgetrf_workspace_bytes(int N, int batch);
+int launch_cgetrf(void* A, int N, int64_t lda, int64_t strideA, int* ipiv, int64_t strideP, int* info, int batch,
+                  void* work, const LuSetup* aux, hipStream_t stream);
+size_t cgetrs_workspace_bytes(int N, int batch);
+int launch_cgetrs(const void* LU, int N, int64_t lda, int64_t strideA, const int* ipiv, int64_t strideP, void* B,
+                  int nrhs, int64_t ldb, int64_t strideB, int batch, void* work, hipStream_t stream);
 int launch_block_transform(int xor_kind, int n, int inverse, int R, int C, int nbatch, const cplx* in, int64_t sbi,
                            int64_t sj, int64_t ldi, cplx* out, int64_t sbo, int64_t sk, int64_t ldo, hipStream_t stream);
 int block_transform_max_blocks();
@@ -159,6 +168,7 @@ struct cpt_context {
     cpt::DevBuf r_range, z_range, table;
     cpt::DevBuf r_range_f, z_range_f, table_f;  // float copies (far field of the mixed form)
     int far_fp32 = 0;                           // mixed-precision assembly
+    int out_c64 = 0;                            // S and K stored as complex64
     int nr = 0, nz = 0, grid = 1, npts = 1001;
     double rmax = 0, zmax_abs = 0, dz = 0;
     int index_of_1 = 81;
@@ -712,6 +722,7 @@ int cpt_build_matrices(cpt_handle h, int nb_collocation_points, const double* co
     int chunk = (int)std::max<int64_t>(1, ((int64_t)gx * m) / 4096);
     A.j_chunk = h->asm_chunk ? h->asm_chunk : std::min(chunk, 64);
     A.far_fp32 = (h->far_fp32 && !finite_depth && !rankine_only) ? 1 : 0;
+    A.c64 = h->out_c64;
     A.freq = (const FreqArgs*)h->freq.p;
     A.S = (cplx*)S;
     A.ldS = ldS;
@@ -835,6 +846,7 @@ int cpt_build_matrices_batched(cpt_handle h, int nb_bodies, const cpt_body* bodi
         A.nb = nb;
         A.j_chunk = chunk;
         A.far_fp32 = h->far_fp32;
+        A.c64 = h->out_c64;
         A.freq = freq_d + ob;
         A.S = (cplx*)B.S;
         A.ldS = B.ldS;
@@ -1019,6 +1031,91 @@ int cpt_zgemm(cpt_handle h, int M, int N, int K, double alpha, const void* A, in
     CPT_CHECK_HIP((hipError_t)launch_zgemm(M, N, K, alpha, (const cplx*)A, lda, strideA, (const cplx*)B, ldb, strideB,
                                            (cplx*)C, ldc, strideC, beta_zero, batch, stream),
                   "zgemm launch");
+    return CPT_OK;
+}
+
+// complex64 products and LU run the FP32-input tile products: the FP16-input
+// ones (tile setting 4) serve the mixed solve, whose refinement restores the
+// digits they lose, and are replaced here by the 128 x 128 FP32 tiles.
+static int fp32_tiles(const cpt_context* h) { return h->gemm_tiles == 4 ? 1 : h->gemm_tiles; }
+
+int cpt_cgemm(cpt_handle h, int M, int N, int K, float alpha, const void* A, int64_t lda, int64_t strideA,
+              const void* B, int64_t ldb, int64_t strideB, void* C, int64_t ldc, int64_t strideC, int beta_zero,
+              int batch, hipStream_t stream)
+{
+    if (!h || M < 0 || N < 0 || K < 0 || batch < 0 || lda < M || ldb < K || ldc < M) {
+        set_error("cpt_cgemm: invalid sizes");
+        return CPT_ERR_ARG;
+    }
+    if (M == 0 || N == 0 || batch == 0) return CPT_OK;
+    if (!A || !B || !C) {
+        set_error("cpt_cgemm: null pointer argument");
+        return CPT_ERR_ARG;
+    }
+    CPT_CHECK_HIP(hipSetDevice(h->device), "hipSetDevice");
+    CallScope call_scope(&h->timer, fp32_tiles(h));
+    CPT_CHECK_HIP((hipError_t)launch_cgemm(M, N, K, alpha, A, lda, strideA, B, ldb, strideB, C, ldc, strideC,
+                                           beta_zero, batch, stream),
+                  "cgemm launch");
+    return CPT_OK;
+}
+
+int cpt_clu_factor(cpt_handle h, void* A, int n, int64_t lda, int64_t strideA, int* ipiv, int64_t strideP,
+                   int* info, int batch, hipStream_t stream)
+{
+    if (!h || n < 0 || batch < 0 || lda < n || (batch > 1 && (strideA < lda * n || strideP < n))) {
+        set_error("cpt_clu_factor: invalid sizes or strides");
+        return CPT_ERR_ARG;
+    }
+    if (n == 0 || batch == 0) return CPT_OK;
+    if (!A || !ipiv || !info) {
+        set_error("cpt_clu_factor: null pointer argument");
+        return CPT_ERR_ARG;
+    }
+    if (n > 24576) {
+        set_error("cpt_clu_factor: n > 24576 is not supported");
+        return CPT_ERR_UNSUPPORTED;
+    }
+    CPT_CHECK_HIP(hipSetDevice(h->device), "hipSetDevice");
+    CallScope call_scope(&h->timer, fp32_tiles(h));
+    hipLaunchKernelGGL(zero_int_kernel, dim3((batch + 255) / 256), dim3(256), 0, stream, info, batch);
+    CPT_CHECK_HIP(hipGetLastError(), "zero info launch");
+    if (h->lu_work.reserve(cgetrf_workspace_bytes(n, batch))) {
+        set_error("cpt_clu_factor: workspace allocation failed");
+        return CPT_ERR_ALLOC;
+    }
+    if (ensure_lu_streams(h)) {
+        set_error("cpt_clu_factor: could not create the look-ahead streams: " + std::string(cpt_last_error()));
+        return CPT_ERR_HIP;
+    }
+    CPT_CHECK_HIP((hipError_t)launch_cgetrf(A, n, lda, strideA, ipiv, strideP, info, batch, h->lu_work.p,
+                                            &h->lu_streams, stream),
+                  "cgetrf launch");
+    return CPT_OK;
+}
+
+int cpt_clu_solve(cpt_handle h, const void* LU, int n, int64_t lda, int64_t strideA, const int* ipiv,
+                  int64_t strideP, void* B, int nrhs, int64_t ldb, int64_t strideB, int batch, hipStream_t stream)
+{
+    if (!h || n < 0 || nrhs < 0 || batch < 0 || lda < n || ldb < n ||
+        (batch > 1 && (strideA < lda * n || strideP < n || strideB < ldb * nrhs))) {
+        set_error("cpt_clu_solve: invalid sizes or strides");
+        return CPT_ERR_ARG;
+    }
+    if (n == 0 || nrhs == 0 || batch == 0) return CPT_OK;
+    if (!LU || !ipiv || !B) {
+        set_error("cpt_clu_solve: null pointer argument");
+        return CPT_ERR_ARG;
+    }
+    CPT_CHECK_HIP(hipSetDevice(h->device), "hipSetDevice");
+    CallScope call_scope(&h->timer, fp32_tiles(h));
+    if (h->solve_work.reserve(cgetrs_workspace_bytes(n, batch))) {
+        set_error("cpt_clu_solve: workspace allocation failed");
+        return CPT_ERR_ALLOC;
+    }
+    CPT_CHECK_HIP((hipError_t)launch_cgetrs(LU, n, lda, strideA, ipiv, strideP, B, nrhs, ldb, strideB, batch,
+                                            h->solve_work.p, stream),
+                  "cgetrs launch");
     return CPT_OK;
 }
 
@@ -1562,12 +1659,13 @@ int cpt_mixed_workspace_size(cpt_handle h, int ngroups, const cpt_lu_group* grou
     return CPT_OK;
 }
 
-int cpt_set_assembly_precision(cpt_handle h, int far_field_fp32)
+int cpt_set_assembly_precision(cpt_handle h, int flags)
 {
-    if (!h || (far_field_fp32 != 0 && far_field_fp32 != 1)) {
+    if (!h || flags < 0 || flags > (CPT_ASSEMBLY_FAR_FIELD_FP32 | CPT_ASSEMBLY_COMPLEX64)) {
         set_error("cpt_set_assembly_precision: invalid arguments");
         return CPT_ERR_ARG;
     }
-    h->far_fp32 = far_field_fp32;
+    h->far_fp32 = (flags & CPT_ASSEMBLY_FAR_FIELD_FP32) ? 1 : 0;
+    h->out_c64 = (flags & CPT_ASSEMBLY_COMPLEX64) ? 1 : 0;
     return CPT_OK;
 }

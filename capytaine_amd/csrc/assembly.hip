@@ -115,6 +115,18 @@ __device__ __forceinline__ void wave_panel_finite_depth(const Table& T, double k
 template <bool FINITE_DEPTH, bool FAR32 = false, bool UNTAB = false>
 __device__ __forceinline__ void assemble_tile(const AssemblyArgs& A, int iblk, int jc);
 
+// One entry of S or K: complex128, or complex64 for the single-precision
+// Green function (the FP64 value rounded once, at the store).
+__device__ __forceinline__ void store_entry(cplx* base, size_t idx, double re, double im, bool c64)
+{
+    if (c64) {
+        reinterpret_cast<float2*>(base)[idx] = make_float2((float)re, (float)im);
+    } else {
+        base[idx].re = re;
+        base[idx].im = im;
+    }
+}
+
 // Infinite depth: two wavefronts per SIMD (about 220 VGPRs, no spills of the
 // main path); finite depth carries more live state and runs at one.
 // UNTAB: the wave term integrated for each point instead of read from a table.
@@ -275,17 +287,15 @@ __device__ __forceinline__ void assemble_tile(const AssemblyArgs& A, int iblk, i
             }
             if (active) {
                 const size_t ij = (size_t)I + (size_t)A.ldS * J;
-                cplx* S = A.S + (size_t)b * A.strideS;
-                S[ij].re = M1_4PI * o.G.re;
-                S[ij].im = M1_4PI * o.G.im;
-                cplx* K = A.K + (size_t)b * A.strideK;
+                const bool c64 = A.c64;
+                store_entry(A.S, (size_t)b * A.strideS + ij, M1_4PI * o.G.re, M1_4PI * o.G.im, c64);
+                const size_t k0 = (size_t)b * A.strideK;
                 if (A.nk == 1) {
                     const size_t kij = (size_t)I + (size_t)A.ldK * J;
                     double kre = M1_4PI * (nx * o.nG[0].re + ny * o.nG[1].re + nz * o.nG[2].re);
                     const double kim = M1_4PI * (nx * o.nG[0].im + ny * o.nG[1].im + nz * o.nG[2].im);
                     if (on_diag) kre += diag_coef;
-                    K[kij].re = kre;
-                    K[kij].im = kim;
+                    store_entry(A.K, k0 + kij, kre, kim, c64);
                 } else {
                     // add_diagonal_term with early_dot_product=False adds the
                     // coefficient times the dot-product normal of row I (== J).
@@ -295,8 +305,7 @@ __device__ __forceinline__ void assemble_tile(const AssemblyArgs& A, int iblk, i
                     for (int c = 0; c < 3; ++c) {
                         double kre = M1_4PI * o.nG[c].re;
                         if (on_diag) kre += diag_coef * dn[c];
-                        K[kij + c].re = kre;
-                        K[kij + c].im = M1_4PI * o.nG[c].im;
+                        store_entry(A.K, k0 + kij + c, kre, M1_4PI * o.nG[c].im, c64);
                     }
                 }
             }
@@ -311,8 +320,8 @@ int launch_assembly(const AssemblyArgs& A, hipStream_t stream)
     const int gx = (A.n + 255) / 256;
     const int gy = (A.m + A.j_chunk - 1) / A.j_chunk;
     dim3 grid(gx, gy);
-    // stores of S and K: 32 bytes per pair and wavenumber
-    TimedLaunch timed(stream, TK_ASSEMBLY, (double)A.n * A.m * A.nb, 32.0 * A.n * A.m * A.nb);
+    // stores of S and K: 32 bytes per pair and wavenumber (16 in complex64)
+    TimedLaunch timed(stream, TK_ASSEMBLY, (double)A.n * A.m * A.nb, (A.c64 ? 16.0 : 32.0) * A.n * A.m * A.nb);
     // without a table there is no float copy either: the FP64 integration
     // serves both forms
     const bool untab = A.T.nr <= 1 || A.T.nz <= 1;

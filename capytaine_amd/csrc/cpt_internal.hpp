@@ -31,9 +31,10 @@ struct AssemblyArgs {
     int diag, n_diag;
     double free_surface;
     int nb, j_chunk;
-    int far_fp32, pad_;    // FP32 far field (infinite depth, mixed form)
+    int far_fp32;          // FP32 far field (infinite depth, mixed form)
+    int c64;               // S and K stored as complex64 (single-precision Green function)
     const FreqArgs* freq;  // device, nb
-    cplx* S;
+    cplx* S;               // complex128, or complex64 when c64 (strides in entries)
     int64_t ldS, strideS;
     cplx* K;
     int64_t ldK, strideK;

@@ -3122,6 +3122,38 @@ int launch_getrs(const cplx* LU, int N, int64_t lda, int64_t strideA, const int*
                          strideB, batch, stream);
 }
 
+// complex64 counterparts (the single-precision Green function's matrices):
+// the same kernels instantiated on float.
+int launch_cgemm(int M, int N, int K, float alpha, const void* A, int64_t lda, int64_t sA, const void* B,
+                 int64_t ldb, int64_t sB, void* C, int64_t ldc, int64_t sC, int beta_zero, int batch,
+                 hipStream_t stream)
+{
+    return gemm<float>(M, N, K, alpha, (const cx<float>*)A, lda, sA, (const cx<float>*)B, ldb, sB, (cx<float>*)C, ldc,
+                       sC, beta_zero, batch, stream);
+}
+
+size_t cgetrf_workspace_bytes(int N, int batch) { return sizeof(cx<float>) * getrf_ws_elems<float>(N, batch); }
+
+int launch_cgetrf(void* A, int N, int64_t lda, int64_t strideA, int* ipiv, int64_t strideP, int* info, int batch,
+                  void* work, const LuSetup* aux, hipStream_t stream)
+{
+    return getrf<float>((cx<float>*)A, N, lda, strideA, ipiv, strideP, info, batch, (cx<float>*)work, aux, stream);
+}
+
+size_t cgetrs_workspace_bytes(int N, int batch) { return sizeof(cx<float>) * (size_t)N * PB * (size_t)batch; }
+
+int launch_cgetrs(const void* LU, int N, int64_t lda, int64_t strideA, const int* ipiv, int64_t strideP, void* B,
+                  int nrhs, int64_t ldb, int64_t strideB, int batch, void* work, hipStream_t stream)
+{
+    if (N <= 0 || nrhs <= 0 || batch <= 0) return 0;
+    int e;
+    cx<float>* D = (cx<float>*)work;
+    const int64_t sD = (int64_t)N * PB;
+    if ((e = invert_diag<float>((const cx<float>*)LU, N, lda, strideA, D, N, sD, 0, batch, stream))) return e;
+    return getrs<float>((const cx<float>*)LU, N, lda, strideA, ipiv, strideP, D, N, sD, 0, (cx<float>*)B, nrhs, ldb,
+                        strideB, batch, stream);
+}
+
 // ---------------------------------------------------------------------------
 // Mixed precision.  A complex64 factor of `batch` matrices of order n lives
 // in one buffer: the LU factors (batch n^2, column-major, leading dimension

@@ -24,9 +24,10 @@ LOG = logging.getLogger(__name__)
 
 
 class DeviceMatrix:
-    """Dense complex128 matrix (or batch of matrices) on the GPU.
+    """Dense complex128 (or complex64, for the single-precision Green
+    function) matrix or batch of matrices on the GPU.
 
-    `data` is a torch complex128 tensor of shape (nb, cols, rows) (or
+    `data` is a torch complex tensor of shape (nb, cols, rows) (or
     (cols, rows)): each matrix is stored column-major, leading dimension
     `rows`, like LAPACK and the reference's Fortran arrays.
     """
@@ -46,8 +47,17 @@ class DeviceMatrix:
         return (self.batch, self.data.shape[2], self.data.shape[1])
 
     @property
+    def single(self):
+        import torch
+        return self.data.dtype == torch.complex64
+
+    @property
     def dtype(self):
-        return np.dtype(np.complex128)
+        return np.dtype(np.complex64 if self.single else np.complex128)
+
+    @property
+    def torch_dtype(self):
+        return self.data.dtype
 
     ndim = property(lambda self: len(self.shape))
 
@@ -66,11 +76,14 @@ class DeviceMatrix:
         import torch
         nb, cols, rows = self.data.shape
         nrhs = x.shape[1]
-        y = torch.empty((nb, nrhs, rows), dtype=torch.complex128, device=self.data.device)
+        if x.dtype != self.data.dtype:
+            raise ValueError(f"matmul_device: {x.dtype} operand for a {self.data.dtype} matrix")
+        y = torch.empty((nb, nrhs, rows), dtype=self.data.dtype, device=self.data.device)
         ctx = _lib.Context.get(self.data.device)
-        check(_lib.load().cpt_zgemm(ctx.handle, rows, nrhs, cols, 1.0, ptr(self.data), rows, rows * cols, ptr(x),
-                                    cols, cols * nrhs, ptr(y), rows, rows * nrhs, 1, nb,
-                                    stream_handle(self.data.device)), "cpt_zgemm")
+        name = "cpt_cgemm" if self.single else "cpt_zgemm"
+        check(getattr(_lib.load(), name)(ctx.handle, rows, nrhs, cols, 1.0, ptr(self.data), rows, rows * cols, ptr(x),
+                                         cols, cols * nrhs, ptr(y), rows, rows * nrhs, 1, nb,
+                                         stream_handle(self.data.device)), name)
         return y
 
     def __matmul__(self, x):
@@ -79,10 +92,10 @@ class DeviceMatrix:
         vec = x.ndim == 1
         xb = x.reshape(x.shape[0], -1) if self.batch == 1 else x
         if self.batch == 1:
-            xd = torch.as_tensor(np.ascontiguousarray(xb.T), dtype=torch.complex128).to(self.data.device)[None]
+            xd = torch.as_tensor(np.ascontiguousarray(xb.T), dtype=self.data.dtype).to(self.data.device)[None]
         else:
             xd = torch.as_tensor(np.ascontiguousarray(np.swapaxes(xb, -1, -2)),
-                                 dtype=torch.complex128).to(self.data.device)
+                                 dtype=self.data.dtype).to(self.data.device)
         y = self.matmul_device(xd).cpu().numpy()
         y = np.swapaxes(y, -1, -2)
         if self.batch == 1:
@@ -93,7 +106,8 @@ class DeviceMatrix:
 
 class LUDecomposedDeviceMatrix:
     """LU factors P A = L U of a batch of DeviceMatrix, computed by
-    cpt_lu_factor; `solve` replaces LUDecomposedMatrix.solve
+    cpt_lu_factor (cpt_clu_factor for complex64 matrices, as LAPACK's cgetrf
+    serves scipy for them); `solve` replaces LUDecomposedMatrix.solve
     (tools/block_circulant_matrices.py:374-385)."""
 
     def __init__(self, A: DeviceMatrix, overwrite_a=False):
@@ -107,8 +121,9 @@ class LUDecomposedDeviceMatrix:
         self.ipiv = torch.empty((nb, rows), dtype=torch.int32, device=data.device)
         self.info = torch.empty((nb,), dtype=torch.int32, device=data.device)
         ctx = _lib.Context.get(data.device)
-        check(_lib.load().cpt_lu_factor(ctx.handle, ptr(data), rows, rows, rows * rows, ptr(self.ipiv), rows,
-                                        ptr(self.info), nb, stream_handle(data.device)), "cpt_lu_factor")
+        name = "cpt_clu_factor" if self.lu.single else "cpt_lu_factor"
+        check(getattr(_lib.load(), name)(ctx.handle, ptr(data), rows, rows, rows * rows, ptr(self.ipiv), rows,
+                                         ptr(self.info), nb, stream_handle(data.device)), name)
         self._checked = False
 
     shape = property(lambda self: self.lu.shape)
@@ -124,10 +139,13 @@ class LUDecomposedDeviceMatrix:
     def solve_device(self, b):
         """Solve in place for a device tensor b of shape (nb, nrhs, n)."""
         nb, nrhs, n = b.shape
+        if b.dtype != self.lu.data.dtype:
+            raise ValueError(f"solve_device: {b.dtype} right-hand sides for a {self.lu.data.dtype} factor")
         ctx = _lib.Context.get(b.device)
-        _lib.call_with_memory_retry(lambda: _lib.load().cpt_lu_solve(
+        name = "cpt_clu_solve" if self.lu.single else "cpt_lu_solve"
+        _lib.call_with_memory_retry(lambda: getattr(_lib.load(), name)(
             ctx.handle, ptr(self.lu.data), n, n, n * n, ptr(self.ipiv), n, ptr(b), nrhs, n, n * nrhs, nb,
-            stream_handle(b.device)), "cpt_lu_solve")
+            stream_handle(b.device)), name)
         return b
 
     def solve(self, b):
@@ -136,10 +154,10 @@ class LUDecomposedDeviceMatrix:
         vec = b.ndim == 1
         bb = b.reshape(b.shape[0], -1) if self.lu.batch == 1 else b
         if self.lu.batch == 1:
-            bd = torch.as_tensor(np.ascontiguousarray(bb.T), dtype=torch.complex128).to(self.lu.device)[None]
+            bd = torch.as_tensor(np.ascontiguousarray(bb.T), dtype=self.lu.torch_dtype).to(self.lu.device)[None]
         else:
             bd = torch.as_tensor(np.ascontiguousarray(np.swapaxes(bb, -1, -2)),
-                                 dtype=torch.complex128).to(self.lu.device)
+                                 dtype=self.lu.torch_dtype).to(self.lu.device)
         x = self.solve_device(bd).cpu().numpy()
         self.check_info()
         x = np.swapaxes(x, -1, -2)
@@ -341,7 +359,11 @@ class DefaultMatrixEngine:
             normals = points.faces_normals if gradient else None
             points = points.faces_centers
         points = np.ascontiguousarray(points, dtype=np.float64)
-        src = np.asarray(sources, dtype=np.complex128)
+        # complex64 matrices and products for the single-precision Green
+        # function, as its matrices are (abstract_green_function.py:60-70)
+        f32 = gf.floating_point_precision == "float32"
+        cdt, ndt = (torch.complex64, np.complex64) if f32 else (torch.complex128, np.complex128)
+        src = np.asarray(sources, dtype=ndt)
         single = src.ndim == 1
         src = src.reshape(src.shape[0], -1)
         n, m, r = points.shape[0], mesh.nb_faces, src.shape[1]
@@ -349,17 +371,17 @@ class DefaultMatrixEngine:
             raise ValueError(f"{src.shape[0]} sources for a mesh of {m} faces")
         dev = gf.device
         nk = 3 if gradient else 1
-        per_point = 16 * m * (1 + nk)
+        per_point = (8 if f32 else 16) * m * (1 + nk)
         if max_chunk_bytes is None:
             free, _ = torch.cuda.mem_get_info(dev)
             max_chunk_bytes = min(8 << 30, free // 4)
         chunk = int(max(1, min(n, max_chunk_bytes // per_point)))
         chunk = -(-n // -(-n // chunk))  # chunks of equal size: the product's grid is one row slab per chunk
-        S_buf = torch.empty(m * chunk, dtype=torch.complex128, device=dev)
-        K_buf = torch.empty(m * chunk * nk, dtype=torch.complex128, device=dev)
-        X = torch.as_tensor(np.ascontiguousarray(src.T), dtype=torch.complex128).to(dev)[None]  # (1, r, m)
+        S_buf = torch.empty(m * chunk, dtype=cdt, device=dev)
+        K_buf = torch.empty(m * chunk * nk, dtype=cdt, device=dev)
+        X = torch.as_tensor(np.ascontiguousarray(src.T), dtype=cdt).to(dev)[None]  # (1, r, m)
         P = torch.as_tensor(points).to(dev)  # one upload: a pageable copy per chunk would wait for the device
-        out = torch.empty((r, n, nk), dtype=torch.complex128, device=dev)
+        out = torch.empty((r, n, nk), dtype=cdt, device=dev)
         for p0 in range(0, n, chunk):
             nc = min(chunk, n - p0)
             S = S_buf[:m * nc].view(1, m, nc)
@@ -378,6 +400,7 @@ class DefaultMatrixEngine:
             fs = gf_params.get("free_surface", 0.0)
             on_fs = np.abs(mesh.faces_centers[:nd, 2] - fs) < float(np.float32(1e-8))
             coef = np.where(on_fs, 1.0, 0.5)[:, None] * normals[:nd]  # (nd, 3)
+            coef = coef.astype(np.float32 if f32 else np.float64)
             out[:, :nd, :] += (torch.as_tensor(coef, device=dev)[None] * X[0, :, :nd, None])
         host = torch.empty(out.shape, dtype=out.dtype, pin_memory=True)
         host.copy_(out)
@@ -395,6 +418,10 @@ class DefaultMatrixEngine:
             return S, K
         from .symmetry import evaluate_symmetric, symmetry_of
         sym = symmetry_of(mesh1) if mesh1 is mesh2 else None
+        if sym is not None and self.green_function.floating_point_precision == "float32":
+            raise NotImplementedError("The single-precision Green function is not available for symmetric "
+                                      "meshes on the device; use mesh.merged() or floating_point_precision="
+                                      "'float64'.")
         if sym is not None:
             # engines.py:178-247: the first block column, all collocation
             # points against the reference block
@@ -422,6 +449,16 @@ class DefaultMatrixEngine:
         self.last_computed_matrices = (S, K)
         return self.last_computed_matrices
 
+    def _lu_solve(self, A, b, overwrite_a):
+        """engines.py:326-338: factor once, cached in place of K."""
+        if not has_been_lu_decomposed(A):
+            luA = lu_decompose(A, overwrite_a=overwrite_a)
+            if self.last_computed_matrices is not None and A is self.last_computed_matrices[1]:
+                self.last_computed_matrices = (self.last_computed_matrices[0], luA)
+        else:
+            luA = A
+        return luA.solve(b)
+
     def linear_solver(self, A, b):
         """engines.py:300-346."""
         if not isinstance(self._linear_solver, str):
@@ -431,16 +468,14 @@ class DefaultMatrixEngine:
                                  f"does not match the expected shape ({b.shape})")
             return x
         if self._linear_solver in ("lu_decomposition", "lu_decomposition_with_overwrite"):
-            overwrite_a = (self._linear_solver == "lu_decomposition_with_overwrite")
-            if not has_been_lu_decomposed(A):
-                luA = lu_decompose(A, overwrite_a=overwrite_a)
-                if self.last_computed_matrices is not None and A is self.last_computed_matrices[1]:
-                    self.last_computed_matrices = (self.last_computed_matrices[0], luA)
-            else:
-                luA = A
-            return luA.solve(b)
+            return self._lu_solve(A, b, overwrite_a=(self._linear_solver == "lu_decomposition_with_overwrite"))
         if self._linear_solver == "mixed_precision":
             from .symmetry import BlockSymmetricDeviceMatrix, MixedSymmetricSolver
+            if isinstance(A, (DeviceMatrix, LUDecomposedDeviceMatrix)) and A.dtype == np.complex64:
+                # complex64 matrices (single-precision Green function): the
+                # complex64 LU is the whole solve, there is no FP64 matrix to
+                # refine against
+                return self._lu_solve(A, b, overwrite_a=False)
             if not isinstance(A, (MixedPrecisionSolver, MixedSymmetricSolver)):
                 solver = MixedSymmetricSolver(A) if isinstance(A, BlockSymmetricDeviceMatrix) else \
                     MixedPrecisionSolver(A)
@@ -455,7 +490,8 @@ class DefaultMatrixEngine:
         """Device memory estimate in GB (engines.py:348-398, plain meshes)."""
         nb_faces = problem.body.mesh.nb_faces
         nb_matrices = 3 if self._linear_solver == "lu_decomposition" else 2
-        return nb_faces ** 2 * nb_matrices * 16 / 1e9
+        nb_bytes = 8 if self.green_function.floating_point_precision == "float32" else 16
+        return nb_faces ** 2 * nb_matrices * nb_bytes / 1e9
 
 
 BasicMatrixEngine = DefaultMatrixEngine

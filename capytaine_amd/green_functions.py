@@ -204,9 +204,8 @@ class Delhommeau(AbstractGreenFunction):
                  floating_point_precision=_default_parameters["floating_point_precision"],
                  gf_singularities=_default_parameters["gf_singularities"],
                  device=None):
-        if floating_point_precision != "float64":
-            raise NotImplementedError("The plain device path computes in float64; "
-                                      "use the engine's mixed-precision mode for float32 work.")
+        if floating_point_precision not in ("float32", "float64"):
+            raise NotImplementedError(f"Unsupported floating point precision: {floating_point_precision}")
         if tabulation_grid_shape not in _GRID:
             raise KeyError(tabulation_grid_shape)
         if gf_singularities not in _SING:
@@ -271,7 +270,8 @@ class Delhommeau(AbstractGreenFunction):
                     loaded = np.load(filepath)  # allow_pickle=False
                     r, z, v = loaded["r_range"], loaded["z_range"], loaded["values"]
                     if v.shape == (nr, nz, 5):
-                        self._set_tabulation(np.asarray(r, float), np.asarray(z, float), np.asarray(v, float))
+                        self._set_tabulation(np.asarray(r, self._real), np.asarray(z, self._real),
+                                             np.asarray(v, self._real))
                         self.tabulation_source = filepath
                         return filename
                 except (EOFError, FileNotFoundError, KeyError, ValueError, OSError):
@@ -285,15 +285,24 @@ class Delhommeau(AbstractGreenFunction):
                 LOG.warning("Could not save the tabulation in %s", cache_dir)
         return filename
 
+    @property
+    def _real(self):
+        return np.float32 if self.floating_point_precision == "float32" else np.float64
+
     def _create_tabulation(self, nr, rmax, nz, zmin, npts):
+        """The table on the device (bit-identical to the reference's in
+        float64).  In float32, the grid is rounded to float32 first and the
+        integrals are tabulated in FP64 at those points, then rounded: a table
+        consistent with its grid, within float rounding of the exact values
+        (the reference's float32 core sums them in float32 instead)."""
         import torch
-        r = default_r_spacing(nr, rmax, self.tabulation_grid_shape_index)
-        z = default_z_spacing(nz, zmin, self.tabulation_grid_shape_index)
+        r = default_r_spacing(nr, rmax, self.tabulation_grid_shape_index).astype(self._real)
+        z = default_z_spacing(nz, zmin, self.tabulation_grid_shape_index).astype(self._real)
         if nr <= 1 or nz <= 1:
             # no usable table: the device integrates the wave term for each
             # point (Green_wave.f90:226-228); the table keeps the reference's
             # shape, its values unused
-            self._set_tabulation(r, z, np.zeros((nr, nz, 5), order="F"))
+            self._set_tabulation(r, z, np.zeros((nr, nz, 5), dtype=self._real, order="F"))
             return
         ctx = _lib.Context.get(self.device)
         r_d = torch.as_tensor(r, dtype=torch.float64).to(self.device)
@@ -303,9 +312,14 @@ class Delhommeau(AbstractGreenFunction):
         check(_lib.load().cpt_tabulate(ctx.handle, ptr(r_d), nr, ptr(z_d), nz, npts, ptr(tab_d),
                                        stream_handle(self.device)), "cpt_tabulate")
         values = tab_d.cpu().numpy().transpose(2, 1, 0)  # (nr, nz, 5), Fortran-ordered view
-        self._set_tabulation(r, z, np.asfortranarray(values), device_copies=(r_d, z_d, tab_d))
+        if self._real is np.float32:
+            self._set_tabulation(r, z, np.asfortranarray(values, dtype=np.float32))
+        else:
+            self._set_tabulation(r, z, np.asfortranarray(values), device_copies=(r_d, z_d, tab_d))
 
     def _set_tabulation(self, r, z, values, device_copies=None):
+        """Host copies in the Green function's precision; the device copies
+        are FP64 (float32 values widened exactly)."""
         import torch
         self.tabulated_r_range = r
         self.tabulated_z_range = z
@@ -384,7 +398,8 @@ class Delhommeau(AbstractGreenFunction):
         """Assemble S and K on the device for one wavenumber or a sequence of
         wavenumbers.
 
-        Returns (S, K) as torch complex128 tensors holding column-major
+        Returns (S, K) as torch complex128 tensors (complex64 for
+        floating_point_precision="float32") holding column-major
         matrices: S has shape (nb, m, n) (that is S[b].T is the (n, m)
         matrix), K has shape (nb, m, n) if early_dot_product else
         (nb, m, n, 3).  A scalar wavenumber gives nb = 1.
@@ -433,12 +448,20 @@ class Delhommeau(AbstractGreenFunction):
                 prony[b, :, :d.shape[1]] = d  # missing terms have a = 0
             prony = np.ascontiguousarray(prony)
         nk = 1 if early_dot_product else 3
+        single = self.floating_point_precision == "float32"
+        cdtype = torch.complex64 if single else torch.complex128
         if out is None:
-            S = torch.empty((nb, m, n), dtype=torch.complex128, device=dev)
-            K = torch.empty((nb, m, n) if nk == 1 else (nb, m, n, 3), dtype=torch.complex128, device=dev)
+            S = torch.empty((nb, m, n), dtype=cdtype, device=dev)
+            K = torch.empty((nb, m, n) if nk == 1 else (nb, m, n, 3), dtype=cdtype, device=dev)
         else:
             S, K = out
+            if S.dtype != cdtype or K.dtype != cdtype:
+                raise ValueError(f"out must hold {cdtype} matrices for {self.floating_point_precision}")
         ctx = self._context()
+        # complex64 stores for the single-precision form; the flag is reset
+        # after the call, the context being shared
+        check(lib.cpt_set_assembly_precision(ctx.handle, _lib.CPT_ASSEMBLY_COMPLEX64 if single else 0),
+              "cpt_set_assembly_precision")
         status = lib.cpt_build_matrices(
             ctx.handle, n, ptr(pts_d), ptr(dpn_d), mesh2.nb_vertices, ptr(md["vertices"]), m, ptr(md["faces"]),
             ptr(md["centers"]), ptr(md["normals"]), ptr(md["areas"]), ptr(md["radii"]), md["nq"],
@@ -446,6 +469,8 @@ class Delhommeau(AbstractGreenFunction):
             self.finite_depth_method_index, nexp, ptr(prony), int(bool(adjoint_double_layer)),
             int(bool(early_dot_product)), int(bool(diagonal_term_in_double_layer)), float(free_surface),
             ptr(S), n, n * m, ptr(K), n, n * m * nk, stream_handle(dev))
+        if single:
+            check(lib.cpt_set_assembly_precision(ctx.handle, 0), "cpt_set_assembly_precision")
         check(status, "cpt_build_matrices")
         return S, K
 

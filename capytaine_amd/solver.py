@@ -31,6 +31,19 @@ from .problems_checks import check_problems
 LOG = logging.getLogger(__name__)
 
 
+def _complex_dtype(A):
+    """Complex type of the right-hand side for the matrix A (solver.py:52-57,
+    184, 195): complex64 for the single-precision Green function's
+    matrices, complex128 otherwise."""
+    return np.complex64 if np.dtype(getattr(A, "dtype", np.complex128)) in (np.float32, np.complex64) \
+        else np.complex128
+
+
+def _single_precision(engine):
+    gf = getattr(engine, "green_function", None)
+    return getattr(gf, "floating_point_precision", "float64") == "float32"
+
+
 class Timer:
     """Accumulated wall time per step, with the reference's step names
     (tools/timer.py, solver.py:176-198)."""
@@ -100,17 +113,19 @@ class BEMSolver:
             with self.timer("Matrix-vector product"):
                 rhs = S @ problem.boundary_condition
             with self.timer("Linear solver"):
-                potential = self.engine.linear_solver(D, rhs.astype(np.complex128))
+                potential = self.engine.linear_solver(D, rhs.astype(_complex_dtype(D)))
             sources = None
         else:
             with self.timer("Green function"):
                 S, K = self.engine.build_matrices(mesh, mesh, **gf_params, adjoint_double_layer=True,
                                                   diagonal_term_in_double_layer=True)
             with self.timer("Linear solver"):
-                sources = self.engine.linear_solver(K, problem.boundary_condition.astype(np.complex128))
+                sources = self.engine.linear_solver(K, problem.boundary_condition.astype(_complex_dtype(K)))
             with self.timer("Matrix-vector product"):
                 potential = S @ sources
-        pressure = 1j * omega * problem.rho * potential
+        # in the potential's precision (complex64 for the single-precision
+        # Green function), whatever the scalars' types
+        pressure = (1j * omega * problem.rho * potential).astype(potential.dtype, copy=False)
         if problem.forward_speed != 0.0:
             # solver.py:200-204: + rho U d(phi)/dx, the gradient on the panels
             # of the mesh itself (with the diagonal term of build_fullK_matrix)
@@ -148,7 +163,9 @@ class BEMSolver:
         signature (solver.py:268) and unused: the device batches instead."""
         method = method if method is not None else self.method
         check_problems(problems)
-        if method != "indirect":
+        # the batched paths hold complex128 matrices: the single-precision
+        # Green function solves problem by problem, as the reference does
+        if method != "indirect" or _single_precision(self.engine):
             return [self._solve_and_catch_errors(pb, method=method, keep_details=keep_details,
                                                  _check_wavelength=False) for pb in problems]
         results = [None] * len(problems)
@@ -185,6 +202,9 @@ class BEMSolver:
         on the panels for the pressure term rho U d(phi)/dx
         (solver.py:200-204)."""
         import torch
+        if _single_precision(self.engine):
+            raise NotImplementedError("solve_batched holds complex128 matrices; with the single-precision Green "
+                                      "function use solve or solve_all")
         body = problems[0].body
         mesh = body.mesh_including_lid
         gf = self.engine.green_function

@@ -277,6 +277,24 @@ int cpt_zgemm(cpt_handle h, int M, int N, int K, double alpha, const void* A, in
               const void* B, int64_t ldb, int64_t strideB, void* C, int64_t ldc, int64_t strideC, int beta_zero,
               int batch, hipStream_t stream);
 
+/* complex64 counterparts for the single-precision Green function
+ * (floating_point_precision="float32", whose S and K are complex64:
+ * abstract_green_function.py:60-70).  cpt_clu_factor / cpt_clu_solve:
+ * cpt_lu_factor / cpt_lu_solve on complex64 matrices (LAPACK cgetrf /
+ * cgetrs, which scipy.linalg.lu_factor / lu_solve call for a complex64
+ * matrix from block_circulant_matrices.py:361-385; the same pivot rule and
+ * info); workspace N x 64 complex64 per matrix for the solve.  cpt_cgemm:
+ * cpt_zgemm on complex64 (S @ sources, solver.py:197-198).  The tile
+ * products take FP32 inputs whatever CPT_TUNE_GEMM_TILES says (its FP16
+ * setting serves the mixed solve only). */
+int cpt_clu_factor(cpt_handle h, void* A, int n, int64_t lda, int64_t strideA, int* ipiv, int64_t strideP,
+                   int* info, int batch, hipStream_t stream);
+int cpt_clu_solve(cpt_handle h, const void* LU, int n, int64_t lda, int64_t strideA, const int* ipiv,
+                  int64_t strideP, void* B, int nrhs, int64_t ldb, int64_t strideB, int batch, hipStream_t stream);
+int cpt_cgemm(cpt_handle h, int M, int N, int K, float alpha, const void* A, int64_t lda, int64_t strideA,
+              const void* B, int64_t ldb, int64_t strideB, void* C, int64_t ldc, int64_t strideC, int beta_zero,
+              int batch, hipStream_t stream);
+
 /* Group Fourier transform over the n blocks of a symmetric mesh's
  * matrices or vectors: out_k = s * sum_j chi(k, j) in_j, with
  * chi(k, j) = exp(-2 pi i j k / n) for rotations (kind CPT_BLOCKS_CYCLIC;
@@ -323,18 +341,28 @@ int cpt_profile_begin(cpt_handle h);
 int cpt_profile_end(cpt_handle h, double* ms, double* work, int64_t* launches, double* bytes);
 
 /* Precision of the following assemblies of `h` (cpt_build_matrices,
- * cpt_build_matrices_batched), for the mixed-precision form:
- * far_field_fp32 = 1 reads, for the panel pairs outside the near-field
+ * cpt_build_matrices_batched); `flags` is a sum of the following, 0 (the
+ * default) being the plain FP64 assembly in complex128:
+ *
+ * CPT_ASSEMBLY_COMPLEX64: S and K are stored as complex64 (strides and
+ * leading dimensions in entries), the FP64 value of each entry rounded once;
+ * the single-precision Green function (Delhommeau with
+ * floating_point_precision="float32", delhommeau.py:39, :130, whose matrices
+ * are complex64: abstract_green_function.py:60-70).
+ *
+ * CPT_ASSEMBLY_FAR_FIELD_FP32, for the mixed-precision form: reads, for the
+ * panel pairs outside the near-field
  * radius (r0 > 7 R and r1 > 7 R, the reference's switch to the one-point
  * Rankine formulas, Green_Rankine.f90:50 and :224), the infinite-depth wave
  * term's 3 x 3 stencil from a float copy of the table, with the reference's
  * stencil; the Lagrange weights, the interpolation and every other term are
- * FP64, and S and K are stored in complex128.  Far entries then differ from
+ * FP64.  Far entries then differ from
  * the FP64 assembly by the table's float rounding (~1e-8 of max |S|, |K|);
  * near pairs are bit-equal.  Infinite depth only (ignored in finite depth).
- * 0 (default) is the plain FP64 assembly.  No reference counterpart: the
- * reference is FP64 (or FP32 throughout when built in single precision). */
-int cpt_set_assembly_precision(cpt_handle h, int far_field_fp32);
+ * No reference counterpart: the reference is FP64 (or FP32 throughout when
+ * built in single precision). */
+enum { CPT_ASSEMBLY_FAR_FIELD_FP32 = 1, CPT_ASSEMBLY_COMPLEX64 = 2 };
+int cpt_set_assembly_precision(cpt_handle h, int flags);
 
 /* Caller-owned device workspace for cpt_lu_solve_mixed and
  * cpt_lu_solve_mixed_grouped (the complex64 factors, pivots, residuals and

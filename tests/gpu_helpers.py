@@ -41,9 +41,13 @@ def build_raw(gf, arr, wavenumbers, *, depth=np.inf, prony=None, gfs=1, finite_d
         nexp = prony.shape[1]
         pr = np.ascontiguousarray(np.broadcast_to(prony, (nb, 2, nexp)))
     nk = 1 if early_dot_product else 3
-    S = torch.empty((nb, m, n), dtype=torch.complex128, device=dev)
-    K = torch.empty((nb, m, n) if nk == 1 else (nb, m, n, 3), dtype=torch.complex128, device=dev)
+    single = gf.floating_point_precision == "float32"  # complex64 stores
+    cdt = torch.complex64 if single else torch.complex128
+    S = torch.empty((nb, m, n), dtype=cdt, device=dev)
+    K = torch.empty((nb, m, n) if nk == 1 else (nb, m, n, 3), dtype=cdt, device=dev)
     ctx = gf._context()
+    check(_lib.load().cpt_set_assembly_precision(ctx.handle, _lib.CPT_ASSEMBLY_COMPLEX64 if single else 0),
+          "cpt_set_assembly_precision")
     keep = [t(arr["vertices"]), t(arr["faces"], torch.int32), t(arr["centers"]), t(arr["normals"]),
             t(arr["areas"]), t(arr["radii"]), t(arr["quad_points"]), t(qw)]
     check(_lib.load().cpt_build_matrices(
@@ -52,6 +56,7 @@ def build_raw(gf, arr, wavenumbers, *, depth=np.inf, prony=None, gfs=1, finite_d
         nb, ptr(ks), ptr(gfs_arr), float(depth), finite_depth_method, nexp, ptr(pr), int(adjoint),
         int(early_dot_product), int(diagonal_term), float(free_surface), ptr(S), n, n * m, ptr(K), n, n * m * nk,
         stream_handle(dev)), "cpt_build_matrices")
+    check(_lib.load().cpt_set_assembly_precision(ctx.handle, 0), "cpt_set_assembly_precision")
     torch.cuda.synchronize()
     S = S.cpu().numpy().transpose(0, 2, 1)
     K = K.cpu().numpy().transpose(0, 2, 1) if nk == 1 else K.cpu().numpy().transpose(0, 3, 2, 1)

@@ -1,0 +1,191 @@
+"""The single-precision Green function (floating_point_precision="float32",
+delhommeau.py:39, :130; its S and K are complex64, abstract_green_function.py:
+60-70) and the complex64 linear algebra that follows it (scipy's lu_factor /
+lu_solve call LAPACK cgetrf / cgetrs for a complex64 matrix,
+block_circulant_matrices.py:361-385).
+
+The device evaluates each entry in FP64 from the float32 table and rounds it
+once at the store, so S and K are within the table's float rounding of the
+FP64 oracle's matrices on the FP64 table: normwise 1e-6 (the reference's
+float32 core sums in float32 throughout and lands further from the FP64
+values; its own test, test_bem_solver.py:134-138, checks the dtypes)."""
+import numpy as np
+import pytest
+from conftest import rel_err
+
+from gpu_helpers import build_raw, device
+
+pytestmark = pytest.mark.gpu
+
+EPS32 = float(np.finfo(np.float32).eps)
+
+
+@pytest.fixture(scope="module")
+def gf32():
+    from capytaine_amd import Delhommeau
+    print("[gpu] float32 Delhommeau", flush=True)
+    return Delhommeau(floating_point_precision="float32", tabulation_cache_dir=None, device=device())
+
+
+def test_float32_tabulation(gf32, default_table):
+    """test_green_functions_fortran_core.py:115-118 (dtypes), and the values:
+    the FP64 table at the float32 grid, rounded, within 1e-6 of the largest
+    entry of the reference's FP64 table."""
+    assert gf32.tabulated_integrals.dtype == np.float32
+    assert gf32.tabulated_r_range.dtype == np.float32 and gf32.tabulated_z_range.dtype == np.float32
+    assert np.array_equal(gf32.tabulated_r_range, default_table[0].astype(np.float32))
+    t32, t64 = gf32.tabulated_integrals.astype(np.float64), default_table[2]
+    err = np.abs(t32 - t64).max(axis=(0, 1)) / np.abs(t64).max(axis=(0, 1))
+    print(f"[gpu]   float32 table vs FP64 reference table, per integral: {err}", flush=True)
+    assert np.all(err <= 1e-6)
+
+
+def test_float32_matrices_vs_oracle(gf32, default_table, orc):
+    """S and K of the 450-panel sphere, complex64, against the oracle's FP64
+    matrices on the reference table."""
+    from capytaine_amd import mesh_sphere
+    from oracle import mesh_arrays
+    mesh = mesh_sphere(radius=1.0, resolution=(30, 30)).immersed_part()
+    arr = mesh_arrays(mesh)
+    for k in (0.1, 0.64, 2.0):
+        S, K = gf32.evaluate(mesh, mesh, wavenumber=k)
+        assert S.dtype == np.complex64 and K.dtype == np.complex64
+        Sr, Kr = orc.build_matrices(**arr, wavenumber=k, depth=np.inf, table=default_table)
+        eS, eK = rel_err(S, Sr), rel_err(K, Kr)
+        print(f"[gpu]   k {k}: normwise S {eS:.1e}, K {eK:.1e}", flush=True)
+        assert eS <= 1e-6 and eK <= 1e-6
+    # finite depth (5 m), with the full gradient (early_dot_product False)
+    k, h = 0.64, 5.0
+    prony = gf32.find_best_exponential_decomposition(k * h)
+    S, K = build_raw(gf32, arr, k, depth=h, prony=prony, finite_depth_method=gf32.finite_depth_method_index,
+                     early_dot_product=False)
+    Sr, Kr = orc.build_matrices(**arr, wavenumber=k, depth=h, table=default_table, prony=prony,
+                                finite_depth_method=gf32.finite_depth_method_index, early_dot_product=False)
+    eS, eK = rel_err(S[0], Sr), rel_err(K[0], Kr)
+    print(f"[gpu]   finite depth, k {k}: normwise S {eS:.1e}, K (3 components) {eK:.1e}", flush=True)
+    assert eS <= 1e-6 and eK <= 1e-6
+
+
+def test_float32_rankine_only_is_real(gf32):
+    from capytaine_amd import mesh_sphere
+    mesh = mesh_sphere(radius=1.0, resolution=(10, 10)).immersed_part()
+    S, K = gf32.evaluate(mesh, mesh, free_surface=np.inf, water_depth=np.inf)
+    assert S.dtype == np.float32 and K.dtype == np.float32
+
+
+def test_float32_solver(gf32):
+    """test_bem_solver.py:134-138, plus the forces of every rigid-body dof
+    against the FP64 solver's (relative 1e-5 of the largest force), through
+    solve and solve_all."""
+    import capytaine_amd as cpa
+    print("[gpu] float32 solver", flush=True)
+    mesh = cpa.mesh_sphere(radius=1.0, center=(0, 0, -2.0), resolution=(16, 16))
+    body = cpa.FloatingBody(mesh).add_all_rigid_body_dofs()
+    pb = cpa.RadiationProblem(body=body, radiating_dof="Surge", omega=1.0)
+    res = cpa.BEMSolver(green_function=gf32).solve(pb)
+    assert res.pressure.dtype == np.complex64 and res.potential.dtype == np.complex64
+    gf64 = cpa.Delhommeau(tabulation_cache_dir=None, device=device())
+    ref = cpa.BEMSolver(green_function=gf64).solve(pb)
+    f32 = np.array([res.forces[d] for d in body.dofs])
+    f64 = np.array([ref.forces[d] for d in body.dofs])
+    err = np.abs(f32 - f64).max() / np.abs(f64).max()
+    print(f"[gpu]   forces vs FP64: {err:.1e}", flush=True)
+    assert err <= 1e-5
+    pbs = [cpa.RadiationProblem(body=body, radiating_dof=d, omega=1.0) for d in ("Surge", "Heave")]
+    pbs.append(cpa.DiffractionProblem(body=body, wave_direction=0.3, omega=1.0))
+    for one, r in zip(pbs, cpa.BEMSolver(green_function=gf32).solve_all(pbs, keep_details=True)):
+        single = cpa.BEMSolver(green_function=gf32).solve(one)
+        assert r.pressure.dtype == np.complex64
+        assert all(abs(r.forces[d] - single.forces[d]) <= 1e-6 * max(abs(single.forces[x]) for x in body.dofs)
+                   for d in body.dofs)
+
+
+def _clu(As, Bs):
+    import torch
+    from capytaine_amd import _lib
+    from capytaine_amd._lib import check, ptr, stream_handle
+    nb, n, _ = As.shape
+    r = Bs.shape[2]
+    dev = device()
+    A = torch.as_tensor(np.ascontiguousarray(np.swapaxes(As, 1, 2)), dtype=torch.complex64).to(dev)
+    B = torch.as_tensor(np.ascontiguousarray(np.swapaxes(Bs, 1, 2)), dtype=torch.complex64).to(dev)
+    ipiv = torch.empty((nb, n), dtype=torch.int32, device=dev)
+    info = torch.empty(nb, dtype=torch.int32, device=dev)
+    ctx = _lib.Context.get(dev)
+    lib = _lib.load()
+    st = stream_handle(dev)
+    check(lib.cpt_clu_factor(ctx.handle, ptr(A), n, n, n * n, ptr(ipiv), n, ptr(info), nb, st), "cpt_clu_factor")
+    check(lib.cpt_clu_solve(ctx.handle, ptr(A), n, n, n * n, ptr(ipiv), n, ptr(B), r, n, n * r, nb, st),
+          "cpt_clu_solve")
+    torch.cuda.synchronize()
+    return (np.swapaxes(B.cpu().numpy(), 1, 2), info.cpu().numpy(), ipiv.cpu().numpy(),
+            np.swapaxes(A.cpu().numpy(), 1, 2))
+
+
+@pytest.mark.parametrize("n", [150, 517, 1100])
+def test_clu_factor_vs_lapack(n):
+    """cpt_clu_factor / cpt_clu_solve on complex64 systems of two matrices:
+    the pivots of LAPACK's cgetrf (scipy.linalg.lu_factor of the complex64
+    matrix) on these diagonally weighted matrices, ||P A - L U|| / ||A||
+    <= 50 eps32, backward error of the solve <= 50 n eps32, info 0."""
+    from scipy.linalg import lu_factor
+    print(f"[gpu] complex64 LU n={n}", flush=True)
+    rng = np.random.default_rng(n)
+    As = np.empty((2, n, n), dtype=np.complex64)
+    Bs = np.empty((2, n, 3), dtype=np.complex64)
+    for b in range(2):
+        A = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+        A = A / np.sqrt(n) + 2.0 * np.diag(rng.uniform(0.5, 1.5, n) * np.exp(1j * rng.uniform(0, 6, n)))
+        As[b] = A
+        Bs[b] = rng.normal(size=(n, 3)) + 1j * rng.normal(size=(n, 3))
+    X, info, ipiv, LU = _clu(As, Bs)
+    assert np.all(info == 0)
+    for b in range(2):
+        A = As[b].astype(np.complex128)
+        lu_ref, piv_ref = lu_factor(As[b])
+        assert lu_ref.dtype == np.complex64
+        assert np.array_equal(ipiv[b], piv_ref)
+        lu = LU[b].astype(np.complex128)
+        L = np.tril(lu, -1) + np.eye(n)
+        U = np.triu(lu)
+        PA = A.copy()
+        for i, p in enumerate(ipiv[b]):
+            PA[[i, p]] = PA[[p, i]]
+        err = np.linalg.norm(PA - L @ U) / np.linalg.norm(A)
+        x = X[b].astype(np.complex128)
+        bwd = np.linalg.norm(A @ x - Bs[b]) / (np.linalg.norm(A) * np.linalg.norm(x))
+        print(f"[gpu]   ||PA - LU|| / ||A|| {err:.1e}, backward error {bwd:.1e}", flush=True)
+        assert err <= 50 * EPS32 and bwd <= 50 * n * EPS32
+
+
+def test_clu_singular_info():
+    n = 40
+    A = np.zeros((1, n, n), dtype=np.complex64)
+    A[0] = np.eye(n)
+    A[0, 7, 7] = 0.0
+    _, info, _, _ = _clu(A, np.ones((1, n, 1), dtype=np.complex64))
+    assert info[0] == 8  # LAPACK: 1 + index of the first exactly-zero pivot
+
+
+@pytest.mark.parametrize("shape", [(300, 4, 300), (600, 200, 500), (37, 5, 1000)])
+def test_cgemm(shape):
+    import torch
+    from capytaine_amd import _lib
+    from capytaine_amd._lib import check, ptr, stream_handle
+    M, N, K = shape
+    rng = np.random.default_rng(M + N + K)
+    A = (rng.normal(size=(M, K)) + 1j * rng.normal(size=(M, K))).astype(np.complex64)
+    B = (rng.normal(size=(K, N)) + 1j * rng.normal(size=(K, N))).astype(np.complex64)
+    dev = device()
+    Ad = torch.as_tensor(np.ascontiguousarray(A.T)).to(dev)
+    Bd = torch.as_tensor(np.ascontiguousarray(B.T)).to(dev)
+    Cd = torch.empty((N, M), dtype=torch.complex64, device=dev)
+    ctx = _lib.Context.get(dev)
+    check(_lib.load().cpt_cgemm(ctx.handle, M, N, K, 1.0, ptr(Ad), M, M * K, ptr(Bd), K, K * N, ptr(Cd), M, M * N, 1,
+                                1, stream_handle(dev)), "cpt_cgemm")
+    torch.cuda.synchronize()
+    C = Cd.cpu().numpy().T
+    ref = A.astype(np.complex128) @ B.astype(np.complex128)
+    err = np.abs(C - ref).max() / np.abs(ref).max()
+    print(f"[gpu] cgemm {shape}: {err:.1e}", flush=True)
+    assert err <= 10 * np.sqrt(K) * EPS32
