@@ -37,6 +37,8 @@ SIGNATURES = {
     "cpt_clu_factor": (_I, [_P, _P, _I, _L, _L, _P, _L, _P, _I, _P]),
     "cpt_clu_solve": (_I, [_P, _P, _I, _L, _L, _P, _L, _P, _I, _L, _L, _I, _P]),
     "cpt_cgemm": (_I, [_P, _I, _I, _I, ctypes.c_float, _P, _L, _L, _P, _L, _L, _P, _L, _L, _I, _I, _P]),
+    "cpt_zgmres": (_I, [_P, _P, _I, _L, _P, _I, _L, _P, _L, _I, _I, _D, _D, _P, _P, _P]),
+    "cpt_cgmres": (_I, [_P, _P, _I, _L, _P, _I, _L, _P, _L, _I, _I, _D, _D, _P, _P, _P]),
     "cpt_lu_solve_mixed": (_I, [_P, _P, _I, _L, _L, _I, _P, _I, _L, _L, _P, _L, _L, _I, _D,
                                 ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double), _P]),
     "cpt_profile_begin": (_I, [_P]),

@@ -50,6 +50,11 @@ int launch_cgetrf(void* A, int N, int64_t lda, int64_t strideA, int* ipiv, int64
 size_t cgetrs_workspace_bytes(int N, int batch);
 int launch_cgetrs(const void* LU, int N, int64_t lda, int64_t strideA, const int* ipiv, int64_t strideP, void* B,
                   int nrhs, int64_t ldb, int64_t strideB, int batch, void* work, hipStream_t stream);
+size_t gmres_workspace_bytes(int n, int restart, bool single);
+int gmres_max_restart();
+int run_gmres(bool single, const void* A, int n, int64_t lda, const void* B, int nrhs, int64_t ldb, void* X,
+              int64_t ldx, int restart, int maxiter, double rtol, double atol, void* work, hipStream_t stream,
+              int* iterations, double* residual, int* not_converged);
 int launch_block_transform(int xor_kind, int n, int inverse, int R, int C, int nbatch, const cplx* in, int64_t sbi,
                            int64_t sj, int64_t ldi, cplx* out, int64_t sbo, int64_t sk, int64_t ldo, hipStream_t stream);
 int block_transform_max_blocks();
@@ -177,6 +182,7 @@ struct cpt_context {
     cpt::DevBuf faces, quad, freq, prony;
     cpt::DevBuf bfaces, bquad, bfreq, bargs, btiles;  // multi-body assembly
     cpt::DevBuf lu_work, mixed_work, solve_work;  // solve_work: inverted diagonal blocks of the solves
+    cpt::DevBuf gmres_work;                       // Krylov basis and reductions of the GMRES solves
     cpt::LuSetup lu_streams;
     int asm_chunk = 0;  // panels per assembly work-group, 0 = by size
     int gemm_tiles = 0; // 1: complex64 products on 128 x 128 tiles, 0: 128 x 64
@@ -503,6 +509,7 @@ int cpt_destroy(cpt_handle h)
     h->freq.release();
     h->prony.release();
     h->lu_work.release();
+    h->gmres_work.release();
     h->mixed_work.release();
     h->solve_work.release();
     if (h->lu_streams.side) (void)hipStreamDestroy(h->lu_streams.side);
@@ -1119,6 +1126,58 @@ int cpt_clu_solve(cpt_handle h, const void* LU, int n, int64_t lda, int64_t stri
     return CPT_OK;
 }
 
+static int gmres_entry(cpt_handle h, bool single, const char* name, const void* A, int n, int64_t lda, const void* B,
+                       int nrhs, int64_t ldb, void* X, int64_t ldx, int restart, int maxiter, double rtol,
+                       double atol, int* iterations, double* residual, hipStream_t stream)
+{
+    const std::string who(name);
+    if (!h || n < 0 || nrhs < 0 || lda < n || ldb < n || ldx < n || !(rtol >= 0.0) || !(atol >= 0.0) ||
+        restart > gmres_max_restart()) {
+        set_error(who + ": invalid arguments");
+        return CPT_ERR_ARG;
+    }
+    if (n == 0 || nrhs == 0) return CPT_OK;
+    if (!A || !B || !X || !iterations || !residual) {
+        set_error(who + ": null pointer argument");
+        return CPT_ERR_ARG;
+    }
+    // scipy.sparse.linalg.gmres defaults: restart 20 (at most n), 10 n cycles
+    if (restart <= 0) restart = 20;
+    restart = std::min(restart, n);
+    if (maxiter <= 0) maxiter = (int)std::min<int64_t>(10 * (int64_t)n, 1 << 30);
+    CPT_CHECK_HIP(hipSetDevice(h->device), "hipSetDevice");
+    CallScope call_scope(&h->timer, fp32_tiles(h));
+    if (h->gmres_work.reserve(gmres_workspace_bytes(n, restart, single))) {
+        set_error(who + ": workspace allocation failed");
+        return CPT_ERR_ALLOC;
+    }
+    int not_converged = 0;
+    CPT_CHECK_HIP((hipError_t)run_gmres(single, A, n, lda, B, nrhs, ldb, X, ldx, restart, maxiter, rtol, atol,
+                                        h->gmres_work.p, stream, iterations, residual, &not_converged),
+                  name);
+    if (not_converged) {
+        set_error(who + ": " + std::to_string(not_converged) + " right-hand side(s) did not converge");
+        return CPT_ERR_NOT_CONVERGED;
+    }
+    return CPT_OK;
+}
+
+int cpt_zgmres(cpt_handle h, const void* A, int n, int64_t lda, const void* B, int nrhs, int64_t ldb, void* X,
+               int64_t ldx, int restart, int maxiter, double rtol, double atol, int* iterations, double* residual,
+               hipStream_t stream)
+{
+    return gmres_entry(h, false, "cpt_zgmres", A, n, lda, B, nrhs, ldb, X, ldx, restart, maxiter, rtol, atol,
+                       iterations, residual, stream);
+}
+
+int cpt_cgmres(cpt_handle h, const void* A, int n, int64_t lda, const void* B, int nrhs, int64_t ldb, void* X,
+               int64_t ldx, int restart, int maxiter, double rtol, double atol, int* iterations, double* residual,
+               hipStream_t stream)
+{
+    return gmres_entry(h, true, "cpt_cgmres", A, n, lda, B, nrhs, ldb, X, ldx, restart, maxiter, rtol, atol,
+                       iterations, residual, stream);
+}
+
 int cpt_block_transform(cpt_handle h, int kind, int n, int inverse, int rows, int cols, int batch, const void* in,
                         int64_t in_batch_stride, int64_t in_block_stride, int64_t ldi, void* out,
                         int64_t out_batch_stride, int64_t out_block_stride, int64_t ldo, hipStream_t stream)
@@ -1631,6 +1690,7 @@ int cpt_release_workspace(cpt_handle h)
     h->lu_work.release();
     h->mixed_work.release();
     h->solve_work.release();
+    h->gmres_work.release();
     return CPT_OK;
 }
 

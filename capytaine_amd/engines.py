@@ -262,6 +262,69 @@ class MixedPrecisionSolver:
         return x[:, 0] if vec else x
 
 
+def gmres_device(mats, b, *, rtol=1e-5, atol=1e-6, restart=None, maxiter=None):
+    """Device GMRES (cpt_zgmres / cpt_cgmres) for the column-major matrices
+    `mats` (torch (nb, n, n)) and right-hand sides b (torch (nb, r, n)):
+    returns x (nb, r, n), the Arnoldi steps and final relative residuals
+    ((nb, r) numpy).  Raises RuntimeError with the reference's message when
+    a right-hand side does not converge (engines.py:84-86)."""
+    import torch
+    nb, n, _ = mats.shape
+    r = b.shape[1]
+    if b.dtype != mats.dtype:
+        raise ValueError(f"gmres: {b.dtype} right-hand sides for {mats.dtype} matrices")
+    single = mats.dtype == torch.complex64
+    name = "cpt_cgmres" if single else "cpt_zgmres"
+    fn = getattr(_lib.load(), name)
+    x = torch.empty_like(b)
+    its = np.zeros((nb, r), dtype=np.int32)
+    res = np.zeros((nb, r))
+    ctx = _lib.Context.get(mats.device)
+    atol = np.broadcast_to(np.asarray(atol, dtype=float), (nb,))
+    for i in range(nb):
+        status = fn(ctx.handle, ptr(mats[i]), n, n, ptr(b[i]), r, n, ptr(x[i]), n, int(restart or 0),
+                    int(maxiter or 0), float(rtol), float(atol[i]), its[i].ctypes.data, res[i].ctypes.data,
+                    stream_handle(mats.device))
+        if status == _lib.CPT_ERR_NOT_CONVERGED:
+            raise RuntimeError(f"No convergence of the GMRES after {int(its[i].max())} iterations.\n"
+                               "This can be due to overlapping panels or irregular frequencies.")
+        check(status, name)
+    LOG.info("End of GMRES after %s iterations.", its.tolist())
+    return x, its, res
+
+
+def solve_gmres(A, b):
+    """engines.py:74-89 on the device: scipy.sparse.linalg.gmres(A, b,
+    atol=1e-6) for each column of b, with its defaults (rtol 1e-5, restart
+    20, 10 n restart cycles).  A symmetric mesh's matrix is solved block by
+    block in the transformed space, each block to the absolute tolerance
+    max(1e-6, 1e-5 ||b||) of the whole system: the transform multiplies
+    norms by sqrt(n) and each block's residual is at most that tolerance, so
+    the residual of the whole system is too."""
+    import torch
+    from .symmetry import BlockSymmetricDeviceMatrix, transform_vectors
+    b = np.asarray(b)
+    vec = b.ndim == 1
+    bb = b.reshape(b.shape[0], -1)
+    if isinstance(A, BlockSymmetricDeviceMatrix):
+        if A.batch != 1:
+            raise ValueError("gmres: one matrix at a time")
+        bd = torch.as_tensor(np.ascontiguousarray(bb.T), dtype=torch.complex128).to(A.device)[None]
+        tau = max(1e-6, 1e-5 * float(np.linalg.norm(bb, axis=0).max()))
+        bh = transform_vectors(A.sym, bd)  # (n, r, w): one system per block
+        xh, _, _ = gmres_device(A.diagonal_blocks(), bh.contiguous(), rtol=0.0, atol=tau)
+        x = transform_vectors(A.sym, xh, inverse=True)[0].cpu().numpy().T
+    elif isinstance(A, DeviceMatrix):
+        if A.batch != 1:
+            raise ValueError("gmres: one matrix at a time")
+        bd = torch.as_tensor(np.ascontiguousarray(bb.T), dtype=A.torch_dtype).to(A.device)[None]
+        xd, _, _ = gmres_device(A.data, bd)
+        x = xd[0].cpu().numpy().T
+    else:
+        raise TypeError(f"gmres: unsupported matrix {type(A).__name__}")
+    return x[:, 0] if vec else x
+
+
 def has_been_lu_decomposed(A):
     from .symmetry import LUDecomposedSymmetricMatrix
     return isinstance(A, (LUDecomposedDeviceMatrix, LUDecomposedSymmetricMatrix))
@@ -484,6 +547,8 @@ class DefaultMatrixEngine:
             else:
                 solver = A
             return solver.solve(b)
+        if self._linear_solver == "gmres":
+            return solve_gmres(A, b)
         raise NotImplementedError(f"Unknown `linear_solver` in DefaultMatrixEngine: {self._linear_solver}")
 
     def compute_ram_estimation(self, problem):

@@ -295,6 +295,27 @@ int cpt_cgemm(cpt_handle h, int M, int N, int K, float alpha, const void* A, int
               const void* B, int64_t ldb, int64_t strideB, void* C, int64_t ldc, int64_t strideC, int beta_zero,
               int batch, hipStream_t stream);
 
+/* Restarted GMRES without preconditioner, A x = b for each of the nrhs
+ * columns of B (column-major, ldb) in turn, x0 = 0, into X (ldx): the
+ * reference's linear_solver="gmres" (src/capytaine/bem/engines.py:74-89,
+ * 340-341: scipy.sparse.linalg.gmres(A, b, atol=1e-6) and its defaults
+ * rtol=1e-5, restart=20, maxiter=10 n restart cycles).  Stops when
+ * ||b - A x||_2 <= max(atol, rtol ||b||_2), the true residual checked after
+ * each restart cycle (the Arnoldi estimate ends a cycle early).  restart <= 0
+ * and maxiter <= 0 take those defaults; restart is at most 128 and at most n.
+ * iterations[j] = Arnoldi steps, residual[j] = final ||b - A x|| / ||b|| of
+ * column j (host arrays of nrhs).  Returns CPT_ERR_NOT_CONVERGED when a
+ * column misses the tolerance (the reference raises RuntimeError); X holds
+ * the last iterates then.  Synchronises `stream` once per Arnoldi step.
+ * cpt_zgmres: complex128; cpt_cgmres: complex64 (the single-precision Green
+ * function's matrices), reductions in FP64 for both. */
+int cpt_zgmres(cpt_handle h, const void* A, int n, int64_t lda, const void* B, int nrhs, int64_t ldb, void* X,
+               int64_t ldx, int restart, int maxiter, double rtol, double atol, int* iterations, double* residual,
+               hipStream_t stream);
+int cpt_cgmres(cpt_handle h, const void* A, int n, int64_t lda, const void* B, int nrhs, int64_t ldb, void* X,
+               int64_t ldx, int restart, int maxiter, double rtol, double atol, int* iterations, double* residual,
+               hipStream_t stream);
+
 /* Group Fourier transform over the n blocks of a symmetric mesh's
  * matrices or vectors: out_k = s * sum_j chi(k, j) in_j, with
  * chi(k, j) = exp(-2 pi i j k / n) for rotations (kind CPT_BLOCKS_CYCLIC;
