@@ -30,6 +30,8 @@ SIGNATURES = {
     "cpt_set_tabulation": (_I, [_P, _P, _I, _P, _I, _P, _I, _I, _P]),
     "cpt_build_matrices": (_I, [_P, _I, _P, _P, _I, _P, _I, _P, _P, _P, _P, _P, _I, _P, _P, _I, _P, _P, _D, _I, _I,
                                 _P, _I, _I, _I, _D, _P, _L, _L, _P, _L, _L, _P]),
+    "cpt_build_matrices_dev": (_I, [_P, _I, _P, _P, _I, _P, _I, _P, _P, _P, _P, _P, _I, _P, _P, _I, _P, _P, _D, _I, _I,
+                                    _P, _I, _I, _I, _D, _P, _L, _L, _P, _L, _L, _P]),
     "cpt_wave_part_infinite_depth": (_I, [_P, _I, _P, _P, _D, _I, _P, _P, _P]),
     "cpt_lu_factor": (_I, [_P, _P, _I, _L, _L, _P, _L, _P, _I, _P]),
     "cpt_lu_solve": (_I, [_P, _P, _I, _L, _L, _P, _L, _P, _I, _L, _L, _I, _P]),

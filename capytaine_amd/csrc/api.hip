@@ -210,6 +210,25 @@ using namespace cpt;
 
 namespace {
 
+// Frequency table from device arrays (cpt_build_matrices_dev): prony holds
+// nb x (lambda[nexp], a[nexp]).
+__global__ void freq_args_kernel(int nb, const double* __restrict__ k, const int* __restrict__ gfs,
+                                 const double* __restrict__ prony, int nexp, int rankine_only,
+                                 FreqArgs* __restrict__ out)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    FreqArgs f;
+    f.k = k[b];
+    f.gfs = gfs[b];
+    f.wave = (!rankine_only && 0.0 < f.k && isfinite(f.k)) ? 1 : 0;
+    f.nexp = nexp;
+    f.pad = 0;
+    f.prony_lambda = prony ? prony + (size_t)b * 2 * nexp : nullptr;
+    f.prony_a = prony ? prony + (size_t)b * 2 * nexp + nexp : nullptr;
+    out[b] = f;
+}
+
 __global__ void pack_faces_kernel(const double* __restrict__ vertices, int nv, const int32_t* __restrict__ faces,
                                   const double* __restrict__ centers, const double* __restrict__ normals,
                                   const double* __restrict__ areas, const double* __restrict__ radii,
@@ -610,15 +629,16 @@ int cpt_set_tabulation(cpt_handle h, const double* r_range, int nr, const double
     return CPT_OK;
 }
 
-int cpt_build_matrices(cpt_handle h, int nb_collocation_points, const double* collocation_points,
-                       const double* dot_product_normals, int nb_vertices, const double* vertices, int nb_faces,
-                       const int32_t* faces, const double* centers, const double* normals, const double* areas,
-                       const double* radiuses, int nb_quad_points, const double* quad_points,
-                       const double* quad_weights, int nb_wavenumbers, const double* wavenumbers,
-                       const int* gf_singularities, double depth, int finite_depth_method, int nexp,
-                       const double* prony, int adjoint_double_layer, int early_dot_product,
-                       int diagonal_term_in_double_layer, double free_surface, void* S, int64_t ldS,
-                       int64_t strideS, void* K, int64_t ldK, int64_t strideK, hipStream_t stream)
+static int build_matrices_impl(bool device_params, cpt_handle h, int nb_collocation_points,
+                               const double* collocation_points,
+                               const double* dot_product_normals, int nb_vertices, const double* vertices, int nb_faces,
+                               const int32_t* faces, const double* centers, const double* normals, const double* areas,
+                               const double* radiuses, int nb_quad_points, const double* quad_points,
+                               const double* quad_weights, int nb_wavenumbers, const double* wavenumbers,
+                               const int* gf_singularities, double depth, int finite_depth_method, int nexp,
+                               const double* prony, int adjoint_double_layer, int early_dot_product,
+                               int diagonal_term_in_double_layer, double free_surface, void* S, int64_t ldS,
+                               int64_t strideS, void* K, int64_t ldK, int64_t strideK, hipStream_t stream)
 {
     if (!h) {
         set_error("cpt_build_matrices: null handle");
@@ -651,8 +671,8 @@ int cpt_build_matrices(cpt_handle h, int nb_collocation_points, const double* co
         set_error("cpt_build_matrices: finite depth needs the Prony decomposition");
         return CPT_ERR_ARG;
     }
-    bool need_table = false;
-    for (int b = 0; b < nb; ++b) {
+    bool need_table = device_params && !rankine_only;  // the wavenumbers are read on the device
+    for (int b = 0; b < nb && !device_params; ++b) {
         const double k = wavenumbers[b];
         const int g = gf_singularities[b];
         if (g != CPT_HIGH_FREQ && g != CPT_LOW_FREQ && g != CPT_LOW_FREQ_WITH_RANKINE_PART) {
@@ -685,24 +705,32 @@ int cpt_build_matrices(cpt_handle h, int nb_collocation_points, const double* co
                        (double*)h->quad.p);
     CPT_CHECK_HIP(hipGetLastError(), "pack kernel launch");
 
-    std::vector<FreqArgs> fa(nb);
-    const double* prony_dev = (const double*)h->prony.p;
-    if (ne > 0)
-        CPT_CHECK_HIP(hipMemcpyAsync(h->prony.p, prony, sizeof(double) * 2 * (size_t)nb * ne, hipMemcpyHostToDevice,
-                                     stream),
-                      "copy prony");
-    for (int b = 0; b < nb; ++b) {
-        const double k = wavenumbers[b];
-        fa[b].k = k;
-        fa[b].gfs = gf_singularities[b];
-        fa[b].wave = (!rankine_only && 0.0 < k && std::isfinite(k)) ? 1 : 0;
-        fa[b].nexp = ne;
-        fa[b].pad = 0;
-        fa[b].prony_lambda = prony_dev + (size_t)b * 2 * ne;
-        fa[b].prony_a = prony_dev + (size_t)b * 2 * ne + ne;
+    if (device_params) {
+        // the frequency table from the device arrays, at run time: nothing
+        // from host memory enters the stream, so the call can be captured
+        hipLaunchKernelGGL(freq_args_kernel, dim3((nb + 63) / 64), dim3(64), 0, stream, nb, wavenumbers,
+                           gf_singularities, ne > 0 ? prony : nullptr, ne, rankine_only ? 1 : 0, (FreqArgs*)h->freq.p);
+        CPT_CHECK_HIP(hipGetLastError(), "frequency table kernel launch");
+    } else {
+        std::vector<FreqArgs> fa(nb);
+        const double* prony_dev = (const double*)h->prony.p;
+        if (ne > 0)
+            CPT_CHECK_HIP(hipMemcpyAsync(h->prony.p, prony, sizeof(double) * 2 * (size_t)nb * ne,
+                                         hipMemcpyHostToDevice, stream),
+                          "copy prony");
+        for (int b = 0; b < nb; ++b) {
+            const double k = wavenumbers[b];
+            fa[b].k = k;
+            fa[b].gfs = gf_singularities[b];
+            fa[b].wave = (!rankine_only && 0.0 < k && std::isfinite(k)) ? 1 : 0;
+            fa[b].nexp = ne;
+            fa[b].pad = 0;
+            fa[b].prony_lambda = prony_dev + (size_t)b * 2 * ne;
+            fa[b].prony_a = prony_dev + (size_t)b * 2 * ne + ne;
+        }
+        CPT_CHECK_HIP(hipMemcpyAsync(h->freq.p, fa.data(), sizeof(FreqArgs) * nb, hipMemcpyHostToDevice, stream),
+                      "copy frequency table");
     }
-    CPT_CHECK_HIP(hipMemcpyAsync(h->freq.p, fa.data(), sizeof(FreqArgs) * nb, hipMemcpyHostToDevice, stream),
-                  "copy frequency table");
 
     AssemblyArgs A;
     std::memset(&A, 0, sizeof(A));
@@ -740,6 +768,40 @@ int cpt_build_matrices(cpt_handle h, int nb_collocation_points, const double* co
     int e = launch_assembly(A, stream);
     CPT_CHECK_HIP((hipError_t)e, "assembly kernel launch");
     return CPT_OK;
+}
+
+int cpt_build_matrices(cpt_handle h, int nb_collocation_points, const double* collocation_points,
+                       const double* dot_product_normals, int nb_vertices, const double* vertices, int nb_faces,
+                       const int32_t* faces, const double* centers, const double* normals, const double* areas,
+                       const double* radiuses, int nb_quad_points, const double* quad_points,
+                       const double* quad_weights, int nb_wavenumbers, const double* wavenumbers,
+                       const int* gf_singularities, double depth, int finite_depth_method, int nexp,
+                       const double* prony, int adjoint_double_layer, int early_dot_product,
+                       int diagonal_term_in_double_layer, double free_surface, void* S, int64_t ldS,
+                       int64_t strideS, void* K, int64_t ldK, int64_t strideK, hipStream_t stream)
+{
+    return build_matrices_impl(false, h, nb_collocation_points, collocation_points, dot_product_normals, nb_vertices,
+                               vertices, nb_faces, faces, centers, normals, areas, radiuses, nb_quad_points,
+                               quad_points, quad_weights, nb_wavenumbers, wavenumbers, gf_singularities, depth,
+                               finite_depth_method, nexp, prony, adjoint_double_layer, early_dot_product,
+                               diagonal_term_in_double_layer, free_surface, S, ldS, strideS, K, ldK, strideK, stream);
+}
+
+int cpt_build_matrices_dev(cpt_handle h, int nb_collocation_points, const double* collocation_points,
+                           const double* dot_product_normals, int nb_vertices, const double* vertices, int nb_faces,
+                           const int32_t* faces, const double* centers, const double* normals, const double* areas,
+                           const double* radiuses, int nb_quad_points, const double* quad_points,
+                           const double* quad_weights, int nb_wavenumbers, const double* wavenumbers,
+                           const int* gf_singularities, double depth, int finite_depth_method, int nexp,
+                           const double* prony, int adjoint_double_layer, int early_dot_product,
+                           int diagonal_term_in_double_layer, double free_surface, void* S, int64_t ldS,
+                           int64_t strideS, void* K, int64_t ldK, int64_t strideK, hipStream_t stream)
+{
+    return build_matrices_impl(true, h, nb_collocation_points, collocation_points, dot_product_normals, nb_vertices,
+                               vertices, nb_faces, faces, centers, normals, areas, radiuses, nb_quad_points,
+                               quad_points, quad_weights, nb_wavenumbers, wavenumbers, gf_singularities, depth,
+                               finite_depth_method, nexp, prony, adjoint_double_layer, early_dot_product,
+                               diagonal_term_in_double_layer, free_surface, S, ldS, strideS, K, ldK, strideK, stream);
 }
 
 int cpt_build_matrices_batched(cpt_handle h, int nb_bodies, const cpt_body* bodies, int adjoint_double_layer,

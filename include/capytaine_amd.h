@@ -110,6 +110,28 @@ int cpt_build_matrices(cpt_handle h, int nb_collocation_points, const double* co
                        int diagonal_term_in_double_layer, double free_surface, void* S, int64_t ldS,
                        int64_t strideS, void* K, int64_t ldK, int64_t strideK, hipStream_t stream);
 
+/* cpt_build_matrices with the per-wavenumber arguments in device memory:
+ * wavenumbers (nb_wavenumbers doubles), gf_singularities (nb_wavenumbers
+ * ints) and prony (nb_wavenumbers x 2 x nexp doubles, as above) are device
+ * pointers read by the kernels when they run.  Nothing from host memory
+ * enters the stream, so a call captured in a HIP graph can be replayed with
+ * new values written to those arrays (the per-frequency graphs of the
+ * sweeps).  The wavenumbers are not checked on the host: each must be 0,
+ * positive or +inf, and gf_singularities one of the CPT_*_FREQ values; a
+ * table must be set unless free_surface is infinite.  The same buffers of
+ * the context as cpt_build_matrices are reused: a first call of the same
+ * sizes outside the capture makes sure the capture allocates nothing.
+ * No counterpart in the reference (its arguments are host arrays). */
+int cpt_build_matrices_dev(cpt_handle h, int nb_collocation_points, const double* collocation_points,
+                           const double* dot_product_normals, int nb_vertices, const double* vertices, int nb_faces,
+                           const int32_t* faces, const double* centers, const double* normals, const double* areas,
+                           const double* radiuses, int nb_quad_points, const double* quad_points,
+                           const double* quad_weights, int nb_wavenumbers, const double* wavenumbers,
+                           const int* gf_singularities, double depth, int finite_depth_method, int nexp,
+                           const double* prony, int adjoint_double_layer, int early_dot_product,
+                           int diagonal_term_in_double_layer, double free_surface, void* S, int64_t ldS,
+                           int64_t strideS, void* K, int64_t ldK, int64_t strideK, hipStream_t stream);
+
 /* One body of cpt_build_matrices_batched: the arrays cpt_build_matrices
  * takes, for a mesh against itself (collocation points and dot-product
  * normals are the panel centres and normals, as BEMSolver uses them). */
