@@ -1778,13 +1778,13 @@ gemm_tiled_gauss_w1_kernel(int M, int N, int K, R alpha, const cx<R>* __restrict
 }
 
 
-// Complex64 tile product with FP16 inputs and FP32 accumulation, Gauss
-// form (P1 = Ar Br, P2 = Ai Bi, P3 = (Ar + Ai)(Br + Bi); Re = P1 - P2,
-// Im = P3 - P1 - P2), 128 x 64 tiles, k in chunks of KC = 16.  A and
-// B are converted once per work-group as they are staged: three FP16 planes
-// (Re, Im, Re + Im) per operand, k contiguous, so a lane reads the KC / 4
-// k values of an MFMA operand (row li of A or column li of B, k = KC/4 lk
-// and on) with one LDS read.  A and B fill the instruction's k slots the
+// Complex64 tile product with FP16 inputs and FP32 accumulation, four-
+// product form (Re += Ar Br - Ai Bi, Im += Ar Bi + Ai Br; -Ai by a sign flip
+// in registers), 128 x 64 tiles, k in chunks of KC = 16.  A and B are
+// converted once per work-group as they are staged: two FP16 planes (Re,
+// Im) per operand, k contiguous, so a lane reads the KC / 4 k values of an
+// MFMA operand (row li of A or column li of B, k = KC/4 lk and on) with one
+// LDS read.  A and B fill the instruction's k slots the
 // same way, so the product sums over every k of the chunk.  Staging: a
 // thread loads KC / 2 consecutive k of one row of A (lanes on consecutive
 // rows: coalesced) and KC / 4 consecutive k of one column of B.
@@ -1813,8 +1813,8 @@ gemm_h16_kernel(int M, int N, int K, float alpha, const cx<float>* __restrict__ 
     constexpr int TBN = 64, FB = TBN / 32, KP = H16Op<KC>::KP;
     constexpr int AK = KC / 2, BK = KC / 4, LK = KC / 4;  // k per thread of A, of B; k per lane of an operand
     static_assert(TBM == 128 && KP >= KC, "staging maps 256 threads onto 128 x KC of A and KC x 64 of B");
-    __shared__ __attribute__((aligned(16))) _Float16 sA[3][TBM * KP];
-    __shared__ __attribute__((aligned(16))) _Float16 sB[3][TBN * KP];
+    __shared__ __attribute__((aligned(16))) _Float16 sA[2][TBM * KP];
+    __shared__ __attribute__((aligned(16))) _Float16 sB[2][TBN * KP];
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int li = lane & 15, lk = lane >> 4;
@@ -1832,14 +1832,13 @@ gemm_h16_kernel(int M, int N, int K, float alpha, const cx<float>* __restrict__ 
     const Cx* Bcol = B + (size_t)(col0 + bc_) * ldb;
     Cx ra[AK], rb[BK];
 
-    fvec4 cr[4][FB], ci[4][FB], cs[4][FB];
+    fvec4 cr[4][FB], ci[4][FB];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < FB; ++b) {
             cr[a][b] = fvec4{0, 0, 0, 0};
             ci[a][b] = fvec4{0, 0, 0, 0};
-            cs[a][b] = fvec4{0, 0, 0, 0};
         }
 
     auto load = [&](int kc) {
@@ -1861,55 +1860,51 @@ gemm_h16_kernel(int M, int N, int K, float alpha, const cx<float>* __restrict__ 
     for (int kc = 0; kc < K; kc += KC) {
 #pragma unroll
         for (int q = 0; q < AK / LK; ++q) {
-            hv hr, hi, hs;
+            hv hr, hi;
 #pragma unroll
             for (int j = 0; j < LK; ++j) {
                 const Cx v = ra[LK * q + j];
                 hr[j] = (_Float16)v.re;
                 hi[j] = (_Float16)v.im;
-                hs[j] = (_Float16)(v.re + v.im);
             }
             const int off = ar_ * KP + AK * akh + LK * q;
             *reinterpret_cast<hv*>(&sA[0][off]) = hr;
             *reinterpret_cast<hv*>(&sA[1][off]) = hi;
-            *reinterpret_cast<hv*>(&sA[2][off]) = hs;
         }
 #pragma unroll
         for (int q = 0; q < BK / LK; ++q) {
-            hv hr, hi, hs;
+            hv hr, hi;
 #pragma unroll
             for (int j = 0; j < LK; ++j) {
                 const Cx v = rb[LK * q + j];
                 hr[j] = (_Float16)v.re;
                 hi[j] = (_Float16)v.im;
-                hs[j] = (_Float16)(v.re + v.im);
             }
             const int off = bc_ * KP + BK * bkq + LK * q;
             *reinterpret_cast<hv*>(&sB[0][off]) = hr;
             *reinterpret_cast<hv*>(&sB[1][off]) = hi;
-            *reinterpret_cast<hv*>(&sB[2][off]) = hs;
         }
         __syncthreads();
         if (kc + KC < K) load(kc + KC);
-        hv yr[FB], yi[FB], ys[FB];
+        hv yr[FB], yi[FB];
 #pragma unroll
         for (int b = 0; b < FB; ++b) {
             const int off = (wn * (TBN / 2) + b * 16 + li) * KP + LK * lk;
             yr[b] = *reinterpret_cast<const hv*>(&sB[0][off]);
             yi[b] = *reinterpret_cast<const hv*>(&sB[1][off]);
-            ys[b] = *reinterpret_cast<const hv*>(&sB[2][off]);
         }
 #pragma unroll
         for (int a = 0; a < 4; ++a) {
             const int off = (wm * 64 + a * 16 + li) * KP + LK * lk;
             const hv xr = *reinterpret_cast<const hv*>(&sA[0][off]);
             const hv xi = *reinterpret_cast<const hv*>(&sA[1][off]);
-            const hv xs = *reinterpret_cast<const hv*>(&sA[2][off]);
+            const hv xn = -xi;
 #pragma unroll
             for (int b = 0; b < FB; ++b) {
                 cr[a][b] = H16Op<KC>::mma(xr, yr[b], cr[a][b]);
-                ci[a][b] = H16Op<KC>::mma(xi, yi[b], ci[a][b]);
-                cs[a][b] = H16Op<KC>::mma(xs, ys[b], cs[a][b]);
+                ci[a][b] = H16Op<KC>::mma(xr, yi[b], ci[a][b]);
+                cr[a][b] = H16Op<KC>::mma(xn, yi[b], cr[a][b]);
+                ci[a][b] = H16Op<KC>::mma(xi, yr[b], ci[a][b]);
             }
         }
         __syncthreads();
@@ -1928,9 +1923,8 @@ gemm_h16_kernel(int M, int N, int K, float alpha, const cx<float>* __restrict__ 
             float re[4], im[4];
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const float p1 = cr[a][b][v], p2 = ci[a][b][v];
-                re[v] = alpha * (p1 - p2);
-                im[v] = alpha * (cs[a][b][v] - p1 - p2);
+                re[v] = alpha * cr[a][b][v];
+                im[v] = alpha * ci[a][b][v];
             }
             Cx* c = C + rowb + (size_t)col * ldc;
             if (rowb + 3 < M) {
@@ -2206,22 +2200,27 @@ __global__ void __launch_bounds__(512) spec_repair_kernel(cx<R>* __restrict__ Ab
 // ---------------------------------------------------------------------------
 // 5c. Packed FP16 tile product for the complex64 trailing updates.
 //
-// C += alpha A B with A (M x K) and B (K x N) complex64, in the Gauss form
-// of gemm_h16_kernel (three real products Ar Br, Ai Bi, (Ar + Ai)(Br + Bi),
-// FP16 inputs, FP32 accumulation), but with the operands packed to FP16
-// once per product instead of in every tile: pack_a_h16_kernel and
-// pack_b_h16_kernel write the three planes (re, im, re + im) with k
-// contiguous per row of A and per column of B (zero beyond M, N and K, K
-// padded to 32), so that the tile kernel stages them to LDS with plain
-// 16-byte loads and feeds v_mfma_f32_16x16x32_f16 (eight k per lane).
+// C += alpha A B with A (M x K) and B (K x N) complex64 in the four-product
+// form (Cr += Ar Br - Ai Bi, Ci += Ar Bi + Ai Br), FP16 inputs and FP32
+// accumulation, with the operands packed to FP16 once per product instead
+// of in every tile: pack_a_h16_kernel and pack_b_h16_kernel write two
+// planes (re, im) with k contiguous per row of A and per column of B (zero
+// beyond M, N and K, K padded to 32), so that the tile kernel stages them to
+// LDS with plain 16-byte loads and feeds v_mfma_f32_16x16x32_f16 (eight k
+// per lane); -Ai is formed in registers by a sign flip.  Two planes and two
+// accumulator sets (against three of each in the three-product form) halve
+// the LDS bytes per matrix-core instruction and fit two wavefronts per SIMD,
+// so that one wave's LDS and global waits overlap the other's products.
 // Tiles of 128 x 128 per work-group of four wavefronts (64 x 64 each), the
-// 32-deep chunks double-buffered in LDS.  The batch is spread so that every work-group of one matrix
-// runs on the same XCD (work-group ids go round-robin over the 8 XCDs), and
-// the matrix's packed A and B are read into one L2 only.
+// 32-deep chunks double-buffered in LDS (80 KiB: two work-groups per CU).
+// The batch is spread so that every work-group of one matrix runs on the
+// same XCD (work-group ids go round-robin over the 8 XCDs), and the
+// matrix's packed A and B are read into one L2 only.
 // ---------------------------------------------------------------------------
 typedef _Float16 hvec8 __attribute__((ext_vector_type(8)));
 constexpr int HP_BM = 128, HP_BK = 32, HP_LDK = HP_BK + 8;  // LDS row stride in halves (80 bytes)
-constexpr int HP_GROUP = 4;  // row panels per group of tiles
+constexpr int HP_GROUP = 4;   // row panels per group of tiles
+constexpr int HP_PLANES = 2;  // re, im
 
 // A (M x K, column-major) -> Ap: plane p, row m at Ap + (p Mp + m) Kp, k
 // contiguous.  64 rows x 32 k per work-group, transposed through LDS.
@@ -2241,19 +2240,17 @@ __global__ void __launch_bounds__(256) pack_a_h16_kernel(const cx<float>* __rest
     }
     __syncthreads();
     const int m = tid >> 2, seg = tid & 3;
-    hvec8 re, im, sm;
+    hvec8 re, im;
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
         const cx<float> v = t[8 * seg + j][m];
         re[j] = (_Float16)v.re;
         im[j] = (_Float16)v.im;
-        sm[j] = (_Float16)(v.re + v.im);
     }
     const size_t plane = (size_t)Mp * Kp;
     _Float16* P = Apbase + (size_t)blockIdx.z * strideAp + (size_t)(m0 + m) * Kp + k0 + 8 * seg;
     *reinterpret_cast<hvec8*>(P) = re;
     *reinterpret_cast<hvec8*>(P + plane) = im;
-    *reinterpret_cast<hvec8*>(P + 2 * plane) = sm;
 }
 
 // B (K x N, column-major) -> Bp: plane p, column n at Bp + (p Np + n) Kp, k
@@ -2267,7 +2264,7 @@ __global__ void __launch_bounds__(256) pack_b_h16_kernel(const cx<float>* __rest
     if (id >= Np * segs) return;
     const int n = id / segs, seg = id % segs;
     const cx<float>* B = Bbase + (size_t)blockIdx.y * strideB + (size_t)n * ldb;
-    hvec8 re, im, sm;
+    hvec8 re, im;
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
         const int k = 8 * seg + j;
@@ -2275,13 +2272,11 @@ __global__ void __launch_bounds__(256) pack_b_h16_kernel(const cx<float>* __rest
         if (n < N && k < K) v = B[k];
         re[j] = (_Float16)v.re;
         im[j] = (_Float16)v.im;
-        sm[j] = (_Float16)(v.re + v.im);
     }
     const size_t plane = (size_t)Np * Kp;
     _Float16* P = Bpbase + (size_t)blockIdx.y * strideBp + (size_t)n * Kp + 8 * seg;
     *reinterpret_cast<hvec8*>(P) = re;
     *reinterpret_cast<hvec8*>(P + plane) = im;
-    *reinterpret_cast<hvec8*>(P + 2 * plane) = sm;
 }
 
 // C (M x N of the columns [c_lo, c_lo + N) of the packed B) += alpha A B;
@@ -2289,13 +2284,13 @@ __global__ void __launch_bounds__(256) pack_b_h16_kernel(const cx<float>* __rest
 // Nt = ceil(N / 128).  Two LDS stages: chunk i is multiplied from one while
 // chunk i + 1, loaded into registers during chunk i - 1, is stored to the
 // other, and chunk i + 2 is loaded: one barrier per chunk.
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 gemm_h16p_kernel(int M, int N, int Kp, float alpha, const _Float16* __restrict__ Apbase, int Mp, int64_t strideAp,
                  const _Float16* __restrict__ Bpbase, int Np, int c_lo, int64_t strideBp, cx<float>* __restrict__ Cbase,
                  int64_t ldc, int64_t strideC, int batch)
 {
     typedef cx<float> Cx;
-    constexpr int STAGE = 3 * HP_BM * HP_LDK;  // halves of one operand's stage
+    constexpr int STAGE = HP_PLANES * HP_BM * HP_LDK;  // halves of one operand's stage
     __shared__ __attribute__((aligned(16))) _Float16 sA[2 * STAGE];
     __shared__ __attribute__((aligned(16))) _Float16 sB[2 * STAGE];
     const int Mt = (M + HP_BM - 1) / HP_BM, Nt = (N + HP_BM - 1) / HP_BM, T = Mt * Nt;
@@ -2326,10 +2321,10 @@ gemm_h16p_kernel(int M, int N, int Kp, float alpha, const _Float16* __restrict__
     Cx* C = Cbase + (size_t)z * strideC;
     // staging role: rows (or columns) sr and sr + 64, k = 8 sseg .. + 8
     const int sr = tid >> 2, sseg = tid & 3;
-    hvec8 ra[3][2], rb[3][2];
+    hvec8 ra[HP_PLANES][2], rb[HP_PLANES][2];
     auto load = [&](int kc) {
 #pragma unroll
-        for (int p = 0; p < 3; ++p)
+        for (int p = 0; p < HP_PLANES; ++p)
 #pragma unroll
             for (int i = 0; i < 2; ++i) {
                 ra[p][i] = *reinterpret_cast<const hvec8*>(Ap + p * planeA + (size_t)(sr + 64 * i) * Kp + kc + 8 * sseg);
@@ -2338,7 +2333,7 @@ gemm_h16p_kernel(int M, int N, int Kp, float alpha, const _Float16* __restrict__
     };
     auto store = [&](int st) {
 #pragma unroll
-        for (int p = 0; p < 3; ++p)
+        for (int p = 0; p < HP_PLANES; ++p)
 #pragma unroll
             for (int i = 0; i < 2; ++i) {
                 const int off = st * STAGE + p * HP_BM * HP_LDK + (sr + 64 * i) * HP_LDK + 8 * sseg;
@@ -2346,14 +2341,13 @@ gemm_h16p_kernel(int M, int N, int Kp, float alpha, const _Float16* __restrict__
                 *reinterpret_cast<hvec8*>(&sB[off]) = rb[p][i];
             }
     };
-    fvec4 cr[4][4], ci[4][4], cs[4][4];
+    fvec4 cr[4][4], ci[4][4];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
             cr[a][b] = fvec4{0, 0, 0, 0};
             ci[a][b] = fvec4{0, 0, 0, 0};
-            cs[a][b] = fvec4{0, 0, 0, 0};
         }
     const int nk = Kp / HP_BK;
     load(0);
@@ -2363,25 +2357,25 @@ gemm_h16p_kernel(int M, int N, int Kp, float alpha, const _Float16* __restrict__
         // stage c & 1 holds chunk c; the registers hold chunk c + 1
         __syncthreads();
         const int st = c & 1;
-        hvec8 yr[4], yi[4], ys[4];
+        hvec8 yr[4], yi[4];
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
             const int off = st * STAGE + (wn * 64 + b * 16 + li) * HP_LDK + 8 * lk;
             yr[b] = *reinterpret_cast<const hvec8*>(&sB[off]);
             yi[b] = *reinterpret_cast<const hvec8*>(&sB[off + HP_BM * HP_LDK]);
-            ys[b] = *reinterpret_cast<const hvec8*>(&sB[off + 2 * HP_BM * HP_LDK]);
         }
 #pragma unroll
         for (int a = 0; a < 4; ++a) {
             const int off = st * STAGE + (wm * 64 + a * 16 + li) * HP_LDK + 8 * lk;
             const hvec8 xr = *reinterpret_cast<const hvec8*>(&sA[off]);
             const hvec8 xi = *reinterpret_cast<const hvec8*>(&sA[off + HP_BM * HP_LDK]);
-            const hvec8 xs = *reinterpret_cast<const hvec8*>(&sA[off + 2 * HP_BM * HP_LDK]);
+            const hvec8 xn = -xi;
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
                 cr[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xr, yr[b], cr[a][b], 0, 0, 0);
-                ci[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xi, yi[b], ci[a][b], 0, 0, 0);
-                cs[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xs, ys[b], cs[a][b], 0, 0, 0);
+                ci[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xr, yi[b], ci[a][b], 0, 0, 0);
+                cr[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xn, yi[b], cr[a][b], 0, 0, 0);
+                ci[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xi, yr[b], ci[a][b], 0, 0, 0);
             }
         }
         // chunk c + 1 into the other stage (its last readers, of chunk c - 1,
@@ -2403,9 +2397,8 @@ gemm_h16p_kernel(int M, int N, int Kp, float alpha, const _Float16* __restrict__
             float re[4], im[4];
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const float p1 = cr[a][b][v], p2 = ci[a][b][v];
-                re[v] = alpha * (p1 - p2);
-                im[v] = alpha * (cs[a][b][v] - p1 - p2);
+                re[v] = alpha * cr[a][b][v];
+                im[v] = alpha * ci[a][b][v];
             }
             Cx* cp = C + rowb + (size_t)col * ldc;
             if (rowb + 3 < M) {
@@ -2427,12 +2420,12 @@ gemm_h16p_kernel(int M, int N, int Kp, float alpha, const _Float16* __restrict__
 }
 
 // Halves of the packed operands of products with M, N <= n and K <= kmax:
-// A and B, three planes each (B with one tile of columns to spare), per
+// A and B, two planes each (B with one tile of columns to spare), per
 // matrix.
 inline size_t h16p_pack_halves(int n, int kmax)
 {
     const size_t np = (size_t)round_up_host(n, HP_BM) + HP_BM, kp = (size_t)round_up_host(kmax, HP_BK);
-    return 2 * 3 * np * kp;
+    return 2 * HP_PLANES * np * kp;
 }
 
 // C += alpha A B (complex64, column-major, batched) through the packed
@@ -2449,7 +2442,7 @@ int gemm_h16p(int M, int N, int K, float alpha, const cx<float>* A, int64_t lda,
     // at a column that need not be a multiple of 128
     const int Mp = round_up_host(M, HP_BM), Np = round_up_host(N, HP_BM) + HP_BM, Kp = round_up_host(K, HP_BK);
     _Float16* Ap = pack;
-    _Float16* Bp = pack + (size_t)3 * Mp * Kp;
+    _Float16* Bp = pack + (size_t)HP_PLANES * Mp * Kp;
     hipLaunchKernelGGL(pack_a_h16_kernel, dim3(Mp / 64, Kp / 32, batch), dim3(256), 0, stream, A, lda, sA, M, K, Ap,
                        Mp, Kp, sPack);
     const int segs = Kp / 8;
@@ -2459,10 +2452,10 @@ int gemm_h16p(int M, int N, int K, float alpha, const cx<float>* A, int64_t lda,
     if ((e = (int)hipGetLastError())) return e;
     auto launch = [&](int c_lo, int n) {
         if (n <= 0) return;
-        // algorithmic bytes: the packed operands (three FP16 planes) read
+        // algorithmic bytes: the packed operands (two FP16 planes) read
         // once, C read and written
         TimedLaunch timed(stream, TK_CGEMM_PACKED, 8.0 * M * n * K * batch,
-                          (6.0 * ((double)M * K + (double)K * n) + 16.0 * M * n) * batch);
+                          (4.0 * ((double)M * K + (double)K * n) + 16.0 * M * n) * batch);
         const int Mt = (M + HP_BM - 1) / HP_BM, Nt = (n + HP_BM - 1) / HP_BM;
         hipLaunchKernelGGL(gemm_h16p_kernel, dim3(Mt * Nt * batch), dim3(256), 0, stream, M, n, Kp, alpha,
                            (const _Float16*)Ap, Mp, sPack, (const _Float16*)Bp, Np, c_lo, sPack, C + (size_t)c_lo * ldc,

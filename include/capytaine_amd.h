@@ -439,8 +439,10 @@ enum {
     CPT_TUNE_GEMM_TILES = 5,     /* complex64 tiled products: 0: 128 x 64 tiles; 1: 128 x 128;
                                     2: 128 x 128, three real products per complex one (Gauss),
                                     one wavefront per SIMD; 3: the Gauss form on 128 x 64;
-                                    4: that form with FP16 inputs and FP32 accumulation (the
-                                    mixed solve's FP64 refinement takes more sweeps).
+                                    4: FP16 inputs and FP32 accumulation, four real products
+                                    per complex one on two FP16 planes per operand (the
+                                    trailing updates on operands packed once per block step;
+                                    the mixed solve's FP64 refinement takes more sweeps).
                                     complex128 products always use 128 x 64, four products;
                                     default 0 */
     CPT_TUNE_LOOKAHEAD_ACTIVE = 6, /* read only (cpt_get_tuning): the look-ahead mode factorisations
