@@ -458,10 +458,12 @@ class Delhommeau(AbstractGreenFunction):
             if S.dtype != cdtype or K.dtype != cdtype:
                 raise ValueError(f"out must hold {cdtype} matrices for {self.floating_point_precision}")
         ctx = self._context()
-        # complex64 stores for the single-precision form; the flag is reset
-        # after the call, the context being shared
-        check(lib.cpt_set_assembly_precision(ctx.handle, _lib.CPT_ASSEMBLY_COMPLEX64 if single else 0),
-              "cpt_set_assembly_precision")
+        # complex64 stores for the single-precision form, reset after the
+        # call (the context is shared); the FP64 form leaves the context's
+        # setting alone (the mixed sweeps set its far field before calling)
+        if single:
+            check(lib.cpt_set_assembly_precision(ctx.handle, _lib.CPT_ASSEMBLY_COMPLEX64),
+                  "cpt_set_assembly_precision")
         status = lib.cpt_build_matrices(
             ctx.handle, n, ptr(pts_d), ptr(dpn_d), mesh2.nb_vertices, ptr(md["vertices"]), m, ptr(md["faces"]),
             ptr(md["centers"]), ptr(md["normals"]), ptr(md["areas"]), ptr(md["radii"]), md["nq"],
