@@ -50,6 +50,21 @@ def chunks_by_memory(sizes, budget):
     return out
 
 
+def _arena(shapes, dtype, dev):
+    """Views of the given shapes into one device allocation, each starting
+    on a 256-byte boundary."""
+    import torch
+    item = torch.empty((), dtype=dtype).element_size()
+    align = max(1, 256 // item)
+    sizes = [int(np.prod(sh)) for sh in shapes]
+    offs, total = [], 0
+    for n in sizes:
+        offs.append(total)
+        total += -(-n // align) * align
+    buf = torch.empty(max(total, 1), dtype=dtype, device=dev)
+    return [buf[o:o + n].view(sh) for o, n, sh in zip(offs, sizes, shapes)]
+
+
 class BatchedSweep:
     """Device-resident solve of many bodies, each at its own wavenumbers.
 
@@ -84,13 +99,17 @@ class BatchedSweep:
         self.n = [m.nb_faces for m in self.meshes]
         cplx = torch.complex128
         dev = self.dev
-        self.S = [torch.empty((len(k), n, n), dtype=cplx, device=dev) for k, n in zip(self.ks, self.n)]
-        self.K = [torch.empty((len(k), n, n), dtype=cplx, device=dev) for k, n in zip(self.ks, self.n)]
-        self.X = [torch.empty((len(k), r, n), dtype=cplx, device=dev) for k, r, n in zip(self.ks, self.nrhs, self.n)]
-        self.Phi = [torch.empty_like(x) for x in self.X]
+        # each kind of array in one allocation, the bodies' arrays as views
+        # into it: one contiguous range per kind, whatever the allocator's
+        # cache holds (separate allocations per body measured up to 2.5x
+        # slower solves when they came back scattered from the cache)
+        self.S = _arena([(len(k), n, n) for k, n in zip(self.ks, self.n)], cplx, dev)
+        self.K = _arena([(len(k), n, n) for k, n in zip(self.ks, self.n)], cplx, dev)
+        self.X = _arena([(len(k), r, n) for k, r, n in zip(self.ks, self.nrhs, self.n)], cplx, dev)
+        self.Phi = _arena([tuple(x.shape) for x in self.X], cplx, dev)
         if precision == "fp64":
-            self.ipiv = [torch.empty((len(k), n), dtype=torch.int32, device=dev) for k, n in zip(self.ks, self.n)]
-            self.info = [torch.empty((len(k),), dtype=torch.int32, device=dev) for k in self.ks]
+            self.ipiv = _arena([(len(k), n) for k, n in zip(self.ks, self.n)], torch.int32, dev)
+            self.info = _arena([(len(k),) for k in self.ks], torch.int32, dev)
         self.W, self.G = [], []
         for b, mesh, k, r in zip(self.bodies, self.meshes, self.ks, self.nrhs):
             W = -(b.dof_normal_matrix(including_lid=True) * mesh.faces_areas[None, :]).T
