@@ -757,14 +757,15 @@ def main():
     cname = {0: "gemm_tiled_kernel<float,64>", 1: "gemm_tiled_kernel<float,128>",
              2: "gemm_tiled_gauss_w1_kernel<float,128>", 3: "gemm_tiled_kernel<float,64,true>",
              4: "gemm_h16_kernel<16>"}[ctiles]
-    cnote = (", three real products per complex one" if ctiles >= 2 else "") + \
-        (", FP16 inputs, FP32 accumulation" if ctiles == 4 else "")
+    cnote = (", three real products per complex one" if ctiles in (2, 3) else "") + \
+        (", FP16 inputs on two planes, four real products per complex one, FP32 accumulation" if ctiles == 4 else "")
     names = {"zgemm": "gemm_tiled_kernel<double,64> (complex128 MFMA tile product)",
              "cgemm": f"{cname} (complex64 MFMA tile product{cnote})",
              "skinny_z": "gemm_skinny_kernel<double> (complex128 product, <= 32 columns)",
              "skinny_c": "gemm_skinny_kernel<float> (complex64 product, <= 32 columns)",
              "cgemm_packed": "gemm_h16p_kernel (complex64 trailing update on FP16 operands packed once per block "
-                             "step, v_mfma_f32_16x16x32_f16, three real products per complex one, FP32 accumulation)"}
+                             "step as two planes, v_mfma_f32_16x16x32_f16, four real products per complex one, "
+                             "FP32 accumulation)"}
     if dom.startswith("skinny"):
         achieved = kd["work"] / (kd["ms"] / 1e3) / 1e9
         roofline = {"bound": "hbm", "kernel": names[dom], "achieved": achieved, "peak": MI355X_HBM_GBS,
@@ -780,7 +781,7 @@ def main():
     roofline.update(launches=kd["launches"], avg_launch_us=kd["ms"] / kd["launches"] * 1e3,
                     work_per_launch_avg=kd["work"] / kd["launches"],
                     algorithmic_bytes_per_launch=kd["bytes"] / kd["launches"])
-    if dom == "cgemm_packed" or (dom == "cgemm" and ctiles >= 2):
+    if dom == "cgemm" and ctiles in (2, 3):
         roofline["mfma_flops_note"] = ("the Gauss form executes 6 M N K real MFMA flops for the 8 M N K counted in "
                                        "`achieved`; its MFMA pipes run at 3/4 of `achieved`")
     traffic = measured_traffic(names[dom].split(" ")[0])
