@@ -189,3 +189,28 @@ def test_cgemm(shape):
     err = np.abs(C - ref).max() / np.abs(ref).max()
     print(f"[gpu] cgemm {shape}: {err:.1e}", flush=True)
     assert err <= 10 * np.sqrt(K) * EPS32
+
+
+def test_float32_untabulated():
+    """float32 without a table: complex64 S from the per-point integration,
+    within 1e-6 of the float64 untabulated evaluation."""
+    from capytaine_amd import Delhommeau, mesh_sphere
+    mesh = mesh_sphere(radius=1.0, center=(0, 0, -1.5), resolution=(8, 16)).immersed_part()
+    kw = dict(tabulation_nr=0, tabulation_nz=0, tabulation_cache_dir=None, device=device())
+    S32, K32 = Delhommeau(floating_point_precision="float32", **kw).evaluate(mesh, mesh, wavenumber=0.8)
+    S64, K64 = Delhommeau(**kw).evaluate(mesh, mesh, wavenumber=0.8)
+    assert S32.dtype == np.complex64
+    print(f"[gpu]   untabulated float32 vs float64: S {rel_err(S32, S64):.1e}, K {rel_err(K32, K64):.1e}",
+          flush=True)
+    assert rel_err(S32, S64) <= 1e-6 and rel_err(K32, K64) <= 1e-6
+
+
+def test_float32_symmetric_mesh_is_refused(gf32):
+    """Symmetric meshes hold complex128 block matrices: the single-precision
+    form says so instead of mixing precisions."""
+    import capytaine_amd as cpa
+    mesh = cpa.mesh_parallelepiped(size=(2.0, 1.0, 1.0), center=(0.0, 0.0, -0.5), resolution=(4, 2, 2),
+                                   missing_sides={"top"}, reflection_symmetry=True)
+    body = cpa.FloatingBody(mesh).add_translation_dof(name="Heave")
+    with pytest.raises(NotImplementedError, match="symmetric"):
+        cpa.BEMSolver(green_function=gf32).solve(cpa.RadiationProblem(body=body, radiating_dof="Heave", omega=1.0))
