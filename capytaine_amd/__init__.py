@@ -10,7 +10,7 @@ generators.
 """
 from .meshes import (Mesh, mesh_sphere, mesh_parallelepiped, mesh_rectangle, mesh_ellipsoid,
                      mesh_vertical_cylinder, ReflectionSymmetricMesh, RotationSymmetricMesh)
-from .green_functions import Delhommeau, XieDelhommeau, GreenFunctionEvaluationError
+from .green_functions import Delhommeau, XieDelhommeau, FinGreen3D, HAMS_GF, GreenFunctionEvaluationError
 from .engines import DefaultMatrixEngine, BasicMatrixEngine, DeviceMatrix, LUDecomposedDeviceMatrix
 from .problems import (FloatingBody, RadiationProblem, DiffractionProblem, rigid_body_dofs,
                        TranslationDof, RotationDof, froude_krylov_force, airy_waves_velocity,

@@ -14,7 +14,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libcapytaine_amd.so")
 
 CPT_LEGACY_GRID, CPT_SCALED_NEMOH3_GRID = 0, 1
-CPT_LEGACY_FINITE_DEPTH, CPT_NEWER_FINITE_DEPTH = 0, 1
+CPT_FINGREEN3D_METHOD, CPT_LEGACY_FINITE_DEPTH, CPT_NEWER_FINITE_DEPTH = -1, 0, 1
 CPT_HIGH_FREQ, CPT_LOW_FREQ, CPT_LOW_FREQ_WITH_RANKINE_PART = 0, 1, 2
 CPT_BLOCKS_CYCLIC, CPT_BLOCKS_XOR = 0, 1
 CPT_ASSEMBLY_FAR_FIELD_FP32, CPT_ASSEMBLY_COMPLEX64 = 1, 2
@@ -33,6 +33,9 @@ SIGNATURES = {
     "cpt_build_matrices_dev": (_I, [_P, _I, _P, _P, _I, _P, _I, _P, _P, _P, _P, _P, _I, _P, _P, _I, _P, _P, _D, _I, _I,
                                     _P, _I, _I, _I, _D, _P, _L, _L, _P, _L, _L, _P]),
     "cpt_wave_part_infinite_depth": (_I, [_P, _I, _P, _P, _D, _I, _P, _P, _P]),
+    "cpt_build_matrices_fingreen3d": (_I, [_P, _I, _P, _P, _I, _P, _I, _P, _P, _P, _P, _P, _I, _P, _P, _D, _D, _I,
+                                           _P, _I, _I, _I, _I, _D, _P, _L, _P, _L, _P]),
+    "cpt_fingreen3d_points": (_I, [_P, _I, _P, _P, _P, _D, _I, _P, _D, _P, _P]),
     "cpt_lu_factor": (_I, [_P, _P, _I, _L, _L, _P, _L, _P, _I, _P]),
     "cpt_lu_solve": (_I, [_P, _P, _I, _L, _L, _P, _L, _P, _I, _L, _L, _I, _P]),
     "cpt_zgemm": (_I, [_P, _I, _I, _I, _D, _P, _L, _L, _P, _L, _L, _P, _L, _L, _I, _I, _P]),

@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libcapytaine_amd.so")
-SOURCES = ["api.hip", "assembly.hip", "tabulation.hip", "lu.hip", "symmetry.hip", "gmres.hip"]
+SOURCES = ["api.hip", "assembly.hip", "tabulation.hip", "lu.hip", "symmetry.hip", "gmres.hip", "fingreen3d.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [

@@ -183,6 +183,7 @@ struct cpt_context {
     cpt::DevBuf bfaces, bquad, bfreq, bargs, btiles;  // multi-body assembly
     cpt::DevBuf lu_work, mixed_work, solve_work;  // solve_work: inverted diagonal blocks of the solves
     cpt::DevBuf gmres_work;                       // Krylov basis and reductions of the GMRES solves
+    cpt::DevBuf fg_roots, fg_ws;                  // FinGreen3D: dispersion roots, epsilon tables
     cpt::LuSetup lu_streams;
     int asm_chunk = 0;  // panels per assembly work-group, 0 = by size
     int gemm_tiles = 0; // 1: complex64 products on 128 x 128 tiles, 0: 128 x 64
@@ -632,7 +633,7 @@ int cpt_set_tabulation(cpt_handle h, const double* r_range, int nr, const double
     return CPT_OK;
 }
 
-static int build_matrices_impl(bool device_params, cpt_handle h, int nb_collocation_points,
+static int build_matrices_impl(bool device_params, bool no_wave, cpt_handle h, int nb_collocation_points,
                                const double* collocation_points,
                                const double* dot_product_normals, int nb_vertices, const double* vertices, int nb_faces,
                                const int32_t* faces, const double* centers, const double* normals, const double* areas,
@@ -667,14 +668,18 @@ static int build_matrices_impl(bool device_params, cpt_handle h, int nb_collocat
     }
     if (finite_depth && finite_depth_method != CPT_LEGACY_FINITE_DEPTH &&
         finite_depth_method != CPT_NEWER_FINITE_DEPTH) {
-        set_error("cpt_build_matrices: only Delhommeau's finite-depth methods are implemented on the device");
+        set_error(finite_depth_method == CPT_FINGREEN3D_METHOD
+                      ? "cpt_build_matrices: FinGreen3D takes its dispersion roots through cpt_build_matrices_fingreen3d"
+                      : "cpt_build_matrices: unknown finite_depth_method");
         return CPT_ERR_UNSUPPORTED;
     }
     if (finite_depth && nexp > 0 && !prony) {
         set_error("cpt_build_matrices: finite depth needs the Prony decomposition");
         return CPT_ERR_ARG;
     }
-    bool need_table = device_params && !rankine_only;  // the wavenumbers are read on the device
+    // no_wave: the Rankine term and its free-surface image only, the wave
+    // part being added by another kernel (FinGreen3D)
+    bool need_table = device_params && !rankine_only && !no_wave;  // the wavenumbers are read on the device
     for (int b = 0; b < nb && !device_params; ++b) {
         const double k = wavenumbers[b];
         const int g = gf_singularities[b];
@@ -686,7 +691,7 @@ static int build_matrices_impl(bool device_params, cpt_handle h, int nb_collocat
             set_error("cpt_build_matrices: invalid wavenumber");
             return CPT_ERR_ARG;
         }
-        if (!rankine_only && k > 0 && std::isfinite(k)) need_table = true;
+        if (!rankine_only && !no_wave && k > 0 && std::isfinite(k)) need_table = true;
     }
     if (need_table && !h->has_table) {
         set_error("cpt_build_matrices: no tabulation set (call cpt_set_tabulation first)");
@@ -725,7 +730,7 @@ static int build_matrices_impl(bool device_params, cpt_handle h, int nb_collocat
             const double k = wavenumbers[b];
             fa[b].k = k;
             fa[b].gfs = gf_singularities[b];
-            fa[b].wave = (!rankine_only && 0.0 < k && std::isfinite(k)) ? 1 : 0;
+            fa[b].wave = (!rankine_only && !no_wave && 0.0 < k && std::isfinite(k)) ? 1 : 0;
             fa[b].nexp = ne;
             fa[b].pad = 0;
             fa[b].prony_lambda = prony_dev + (size_t)b * 2 * ne;
@@ -783,7 +788,7 @@ int cpt_build_matrices(cpt_handle h, int nb_collocation_points, const double* co
                        int diagonal_term_in_double_layer, double free_surface, void* S, int64_t ldS,
                        int64_t strideS, void* K, int64_t ldK, int64_t strideK, hipStream_t stream)
 {
-    return build_matrices_impl(false, h, nb_collocation_points, collocation_points, dot_product_normals, nb_vertices,
+    return build_matrices_impl(false, false, h, nb_collocation_points, collocation_points, dot_product_normals, nb_vertices,
                                vertices, nb_faces, faces, centers, normals, areas, radiuses, nb_quad_points,
                                quad_points, quad_weights, nb_wavenumbers, wavenumbers, gf_singularities, depth,
                                finite_depth_method, nexp, prony, adjoint_double_layer, early_dot_product,
@@ -800,11 +805,112 @@ int cpt_build_matrices_dev(cpt_handle h, int nb_collocation_points, const double
                            int diagonal_term_in_double_layer, double free_surface, void* S, int64_t ldS,
                            int64_t strideS, void* K, int64_t ldK, int64_t strideK, hipStream_t stream)
 {
-    return build_matrices_impl(true, h, nb_collocation_points, collocation_points, dot_product_normals, nb_vertices,
+    return build_matrices_impl(true, false, h, nb_collocation_points, collocation_points, dot_product_normals, nb_vertices,
                                vertices, nb_faces, faces, centers, normals, areas, radiuses, nb_quad_points,
                                quad_points, quad_weights, nb_wavenumbers, wavenumbers, gf_singularities, depth,
                                finite_depth_method, nexp, prony, adjoint_double_layer, early_dot_product,
                                diagonal_term_in_double_layer, free_surface, S, ldS, strideS, K, ldK, strideK, stream);
+}
+
+// The FinGreen3D finite-depth Green function (matrices.f90:95-209 with
+// finite_depth_method FINGREEN3D_METHOD; Green_wave.f90:545-616): the
+// Rankine term and its free-surface image integrated on the panels by the
+// assembly kernel (as in infinite depth at zero frequency, the sign of the
+// image following gf_singularities), then the FinGreen3D wave part at the
+// quadrature points added by its own kernel.
+int cpt_build_matrices_fingreen3d(cpt_handle h, int nb_collocation_points, const double* collocation_points,
+                                  const double* dot_product_normals, int nb_vertices, const double* vertices,
+                                  int nb_faces, const int32_t* faces, const double* centers, const double* normals,
+                                  const double* areas, const double* radiuses, int nb_quad_points,
+                                  const double* quad_points, const double* quad_weights, double wavenumber,
+                                  double depth, int nb_roots, const double* roots, int gf_singularities,
+                                  int adjoint_double_layer, int early_dot_product, int diagonal_term_in_double_layer,
+                                  double free_surface, void* S, int64_t ldS, void* K, int64_t ldK, hipStream_t stream)
+{
+    if (!h) {
+        set_error("cpt_build_matrices_fingreen3d: null handle");
+        return CPT_ERR_ARG;
+    }
+    if (!(std::isfinite(depth) && depth > 0.0) || !std::isfinite(free_surface)) {
+        set_error("cpt_build_matrices_fingreen3d: FinGreen3D is only implemented for finite depth with a free surface");
+        return CPT_ERR_ARG;
+    }
+    if (!(wavenumber > 0.0 && std::isfinite(wavenumber))) {
+        set_error("cpt_build_matrices_fingreen3d: FinGreen3D is only implemented for non-zero and non-infinite "
+                  "frequencies");
+        return CPT_ERR_ARG;
+    }
+    // the series read roots up to index 98 (Pidcock at R/h = 0.0005)
+    if (nb_roots < 100 || !roots) {
+        set_error("cpt_build_matrices_fingreen3d: at least 100 dispersion roots are needed");
+        return CPT_ERR_ARG;
+    }
+    if (h->out_c64) {
+        set_error("cpt_build_matrices_fingreen3d: FinGreen3D assembles complex128 matrices only");
+        return CPT_ERR_UNSUPPORTED;
+    }
+    const int n = nb_collocation_points, m = nb_faces, nk = early_dot_product ? 1 : 3;
+    const int gfs = gf_singularities;
+    // the Rankine parts (validates the remaining arguments, packs the panels)
+    int st = build_matrices_impl(false, true, h, n, collocation_points, dot_product_normals, nb_vertices, vertices, m,
+                                 faces, centers, normals, areas, radiuses, nb_quad_points, quad_points, quad_weights,
+                                 1, &wavenumber, &gfs, INFINITY, CPT_NEWER_FINITE_DEPTH, 0, nullptr,
+                                 adjoint_double_layer, early_dot_product, diagonal_term_in_double_layer, free_surface,
+                                 S, ldS, ldS * m, K, ldK, ldK * (int64_t)m * nk, stream);
+    if (st != CPT_OK || n == 0 || m == 0) return st;
+    const int threads = fingreen3d_threads((int64_t)n * m);
+    if (h->fg_roots.reserve(sizeof(double) * (size_t)nb_roots) ||
+        h->fg_ws.reserve(sizeof(double) * FG_WS_DOUBLES * (size_t)threads)) {
+        set_error("cpt_build_matrices_fingreen3d: device allocation failed");
+        return CPT_ERR_ALLOC;
+    }
+    CPT_CHECK_HIP(hipMemcpyAsync(h->fg_roots.p, roots, sizeof(double) * nb_roots, hipMemcpyHostToDevice, stream),
+                  "copy dispersion roots");
+    FgArgs A;
+    A.colloc = collocation_points;
+    A.dpn = dot_product_normals;
+    A.quad = (const double*)h->quad.p;
+    A.n = n;
+    A.m = m;
+    A.nq = nb_quad_points;
+    A.nk = nk;
+    A.adjoint = adjoint_double_layer ? 1 : 0;
+    A.roots = (const double*)h->fg_roots.p;
+    A.depth = depth;
+    A.v = wavenumber * std::tanh(wavenumber * depth);  // omega^2/g (Green_wave.f90:571)
+    A.S = (cplx*)S;
+    A.ldS = ldS;
+    A.K = (cplx*)K;
+    A.ldK = ldK;
+    A.ws = (double*)h->fg_ws.p;
+    A.nthreads = threads;
+    CallScope call_scope(&h->timer, h->gemm_tiles);
+    CPT_CHECK_HIP((hipError_t)launch_fingreen3d_pairs(A, stream), "FinGreen3D kernel launch");
+    return CPT_OK;
+}
+
+int cpt_fingreen3d_points(cpt_handle h, int np, const double* rr, const double* zf, const double* zp, double v,
+                          int nb_roots, const double* roots, double depth, void* grn, hipStream_t stream)
+{
+    if (!h || np < 0 || (np > 0 && (!rr || !zf || !zp || !roots || !grn))) {
+        set_error("cpt_fingreen3d_points: invalid arguments");
+        return CPT_ERR_ARG;
+    }
+    if (nb_roots < 100 || !(std::isfinite(depth) && depth > 0.0) || !(v > 0.0 && std::isfinite(v))) {
+        set_error("cpt_fingreen3d_points: needs at least 100 roots, a finite positive depth and omega^2/g");
+        return CPT_ERR_ARG;
+    }
+    if (np == 0) return CPT_OK;
+    CPT_CHECK_HIP(hipSetDevice(h->device), "hipSetDevice");
+    const int threads = fingreen3d_threads(np);
+    if (h->fg_ws.reserve(sizeof(double) * FG_WS_DOUBLES * (size_t)threads)) {
+        set_error("cpt_fingreen3d_points: device allocation failed");
+        return CPT_ERR_ALLOC;
+    }
+    CPT_CHECK_HIP((hipError_t)launch_fingreen3d_points(np, rr, zf, zp, v, roots, depth, (cplx*)grn,
+                                                       (double*)h->fg_ws.p, threads, stream),
+                  "FinGreen3D points kernel launch");
+    return CPT_OK;
 }
 
 int cpt_build_matrices_batched(cpt_handle h, int nb_bodies, const cpt_body* bodies, int adjoint_double_layer,

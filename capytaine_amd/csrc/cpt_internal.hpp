@@ -42,6 +42,30 @@ struct AssemblyArgs {
 
 int launch_assembly(const AssemblyArgs& A, hipStream_t stream);
 
+// FinGreen3D wave part added to S and K (fingreen3d.hip).  The pairs (I, J)
+// of n collocation points and m panels: S[I + ldS J] += -1/(4 pi) sum_q w_q G,
+// and the same for the gradient into K (dot product with dpn when nk == 1).
+struct FgArgs {
+    const double* colloc;  // (n,3)
+    const double* dpn;     // (n,3) if adjoint else (m,3)
+    const double* quad;    // (m, nq, 4): x, y, z, weight
+    int n, m, nq, nk, adjoint;
+    const double* roots;   // dispersion roots, roots[0] the propagating one
+    double depth, v;       // water depth, omega^2/g
+    cplx* S;
+    int64_t ldS;
+    cplx* K;
+    int64_t ldK;
+    double* ws;            // FG_WS_DOUBLES per thread, interleaved by thread
+    int nthreads;
+};
+constexpr int FG_WS_DOUBLES = 400;
+int fingreen3d_threads(int64_t items);
+int launch_fingreen3d_pairs(const FgArgs& A, hipStream_t stream);
+int launch_fingreen3d_points(int np, const double* rr, const double* zf, const double* zp, double v,
+                             const double* roots, double depth, cplx* grn, double* ws, int nthreads,
+                             hipStream_t stream);
+
 // Work-list entry of the persistent multi-body assembly.
 struct AsmTile {
     int body, iblk, jc, pad;

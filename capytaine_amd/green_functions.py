@@ -503,6 +503,160 @@ class Delhommeau(AbstractGreenFunction):
         return S, K
 
 
+class FinGreen3D(AbstractGreenFunction):
+    """The HAMS finite-depth Green function (hams.py:109-209), assembled on
+    the GPU: the Rankine term and its free-surface image integrated on the
+    panels, the FinGreen3D wave part (Liu et al. 2018) at the panels'
+    quadrature points.  Finite depth with a free surface, finite non-zero
+    frequencies only (as the reference); float64.
+
+    nb_dispersion_roots: roots of the dispersion relation passed to the
+    series (the reference's default 200; the series read the first 99).
+    `device`: a torch device, default the current GPU.
+    """
+
+    floating_point_precision = "float64"
+    finite_depth_method = "fingreen3d"
+    finite_depth_method_index = _lib.CPT_FINGREEN3D_METHOD
+    gf_singularities = "low_freq"
+    gf_singularities_index = _SING["low_freq"]
+
+    def __init__(self, *, nb_dispersion_roots=200, device=None):
+        if nb_dispersion_roots < 100:
+            raise ValueError("FinGreen3D's series read the first 99 evanescent roots: "
+                             f"nb_dispersion_roots={nb_dispersion_roots} < 100")
+        self.nb_dispersion_roots = int(nb_dispersion_roots)
+        self.device = _lib.default_device() if device is None else device
+        self.exportable_settings = {
+            'green_function': "FinGreen3D",
+            'nb_dispersion_roots': self.nb_dispersion_roots,
+        }
+
+    def __str__(self):
+        return f"FinGreen3D(nb_dispersion_roots={self.nb_dispersion_roots})"
+
+    __repr__ = __str__
+
+    def __hash__(self):
+        return hash(("FinGreen3D", self.nb_dispersion_roots))
+
+    @staticmethod
+    @lru_cache(maxsize=256)
+    def _roots(nk, wavenumber, depth):
+        from scipy.optimize import brentq
+        omega2_h_over_g = wavenumber * np.tanh(wavenumber * depth) * depth
+
+        def root(i_root):
+            return brentq(lambda y: omega2_h_over_g + y * np.tan(y), (2 * i_root + 1) * np.pi / 2 + 1e-10,
+                          (2 * i_root + 2) * np.pi / 2 - 1e-10) / depth
+        out = np.array([wavenumber] + [root(i) for i in range(nk - 1)])
+        out.flags.writeable = False
+        return out
+
+    def compute_dispersion_relation_roots(self, nk, wavenumber, depth):
+        """[k, then the nk - 1 evanescent roots m_i of omega^2/g = -m tan(m h)],
+        by Brent's method in each branch of the tangent (hams.py:152-156)."""
+        return np.array(self._roots(int(nk), float(wavenumber), float(depth)))
+
+    def evaluate_on_device(self, mesh1, mesh2, *, free_surface=0.0, water_depth=np.inf, wavenumber=1.0,
+                           adjoint_double_layer=True, early_dot_product=True,
+                           diagonal_term_in_double_layer=True, out=None):
+        """As Delhommeau.evaluate_on_device: S (nb, m, n) and K (nb, m, n) or
+        (nb, m, n, 3) complex128 device tensors holding column-major
+        matrices, one per wavenumber."""
+        import torch
+        if free_surface == np.inf or water_depth == np.inf:
+            raise NotImplementedError("FinGreen3D is only implemented for finite depth with a free surface.")
+        ks = np.atleast_1d(np.asarray(wavenumber, dtype=np.float64))
+        if np.any(ks == 0.0) or np.any(ks == np.inf):
+            raise NotImplementedError("FinGreen3D is only implemented for non-zero and non-infinite frequencies")
+        lib = _lib.load()
+        nb = len(ks)
+        dev = self.device
+        md = mesh2.device_arrays(dev)
+        m = mesh2.nb_faces
+        if isinstance(mesh1, torch.Tensor):
+            if not (mesh1.dtype == torch.float64 and mesh1.dim() == 2 and mesh1.shape[1] == 3
+                    and mesh1.is_contiguous() and mesh1.device == torch.device(dev)):
+                raise ValueError("device points must be a contiguous (n, 3) float64 tensor on the Green "
+                                 "function's device")
+            if not adjoint_double_layer:
+                raise ValueError("device points need adjoint_double_layer=True")
+            n = mesh1.shape[0]
+            pts_d, dpn_d = mesh1, torch.zeros_like(mesh1)
+        else:
+            pts, dpn = self._get_colocation_points_and_normals(mesh1, mesh2, adjoint_double_layer)
+            n = pts.shape[0]
+            if mesh1 is mesh2:
+                pts_d, dpn_d = md["centers"], md["normals"]
+            else:
+                pts_d = torch.as_tensor(np.ascontiguousarray(pts), dtype=torch.float64).to(dev)
+                dpn_d = torch.as_tensor(np.ascontiguousarray(dpn), dtype=torch.float64).to(dev)
+        nk = 1 if early_dot_product else 3
+        if out is None:
+            S = torch.empty((nb, m, n), dtype=torch.complex128, device=dev)
+            K = torch.empty((nb, m, n) if nk == 1 else (nb, m, n, 3), dtype=torch.complex128, device=dev)
+        else:
+            S, K = out
+            if S.dtype != torch.complex128 or K.dtype != torch.complex128:
+                raise ValueError("out must hold complex128 matrices for FinGreen3D")
+        ctx = _lib.Context.get(dev)
+        for b, k in enumerate(ks):
+            roots = self._roots(self.nb_dispersion_roots, float(k), float(water_depth))
+            check(lib.cpt_build_matrices_fingreen3d(
+                ctx.handle, n, ptr(pts_d), ptr(dpn_d), mesh2.nb_vertices, ptr(md["vertices"]), m, ptr(md["faces"]),
+                ptr(md["centers"]), ptr(md["normals"]), ptr(md["areas"]), ptr(md["radii"]), md["nq"],
+                ptr(md["quad_points"]), ptr(md["quad_weights"]), float(k), float(water_depth), len(roots),
+                roots.ctypes.data, self.gf_singularities_index, int(bool(adjoint_double_layer)),
+                int(bool(early_dot_product)), int(bool(diagonal_term_in_double_layer)), float(free_surface),
+                ptr(S[b]), n, ptr(K[b]), n, stream_handle(dev)), "cpt_build_matrices_fingreen3d")
+        return S, K
+
+    def evaluate(self, mesh1, mesh2, *, free_surface=0.0, water_depth=np.inf, wavenumber=1.0,
+                 adjoint_double_layer=True, early_dot_product=True, diagonal_term_in_double_layer=True):
+        """hams.py:158-209: Fortran-ordered numpy S (n, m) and K (n, m) (or
+        (3, n, m) if not early_dot_product)."""
+        S, K = self.evaluate_on_device(mesh1, mesh2, free_surface=free_surface, water_depth=water_depth,
+                                       wavenumber=float(wavenumber), adjoint_double_layer=adjoint_double_layer,
+                                       early_dot_product=early_dot_product,
+                                       diagonal_term_in_double_layer=diagonal_term_in_double_layer)
+        S = S[0].cpu().numpy().T
+        K = K[0].cpu().numpy().T if early_dot_product else K[0].cpu().numpy().transpose(2, 1, 0)
+        if np.any(np.isnan(S)) or np.any(np.isnan(K)):
+            raise GreenFunctionEvaluationError("Green function returned a NaN in the interaction matrix.\n"
+                                               "It could be due to overlapping panels.")
+        return S, K
+
+
+class HAMS_GF(AbstractGreenFunction):
+    """hams.py:212-234: FinGreen3D in finite depth.  Its infinite-depth half,
+    the Liang-Wu-Noblesse wave term, is not available on the device."""
+
+    floating_point_precision = "float64"
+
+    def __init__(self, *, device=None):
+        self.finite_depth_gf = FinGreen3D(nb_dispersion_roots=200, device=device)
+        self.device = self.finite_depth_gf.device
+        self.exportable_settings = {'green_function': "HAMS_GF"}
+
+    def __str__(self):
+        return "HAMS_GF()"
+
+    __repr__ = __str__
+
+    def _pick(self, water_depth):
+        if water_depth == np.inf:
+            raise NotImplementedError("HAMS_GF in infinite depth uses the Liang-Wu-Noblesse wave term, which is "
+                                      "not available on the device; use Delhommeau()")
+        return self.finite_depth_gf
+
+    def evaluate_on_device(self, mesh1, mesh2, *, water_depth=np.inf, **kwargs):
+        return self._pick(water_depth).evaluate_on_device(mesh1, mesh2, water_depth=water_depth, **kwargs)
+
+    def evaluate(self, mesh1, mesh2, *, water_depth=np.inf, **kwargs):
+        return self._pick(water_depth).evaluate(mesh1, mesh2, water_depth=water_depth, **kwargs)
+
+
 class XieDelhommeau(Delhommeau):
     """Legacy way to call the gf_singularities="low_freq" variant."""
 

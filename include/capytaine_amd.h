@@ -43,7 +43,9 @@ enum {
 /* Values of the integer options, equal to the reference's Fortran
  * enumerations (libs/libDelhommeau/src/constants.f90:40-52). */
 enum { CPT_LEGACY_GRID = 0, CPT_SCALED_NEMOH3_GRID = 1 };
-enum { CPT_LEGACY_FINITE_DEPTH = 0, CPT_NEWER_FINITE_DEPTH = 1 };
+/* finite_depth_method (constants.f90:45-47); FinGreen3D has an entry point of
+ * its own, cpt_build_matrices_fingreen3d */
+enum { CPT_FINGREEN3D_METHOD = -1, CPT_LEGACY_FINITE_DEPTH = 0, CPT_NEWER_FINITE_DEPTH = 1 };
 enum { CPT_HIGH_FREQ = 0, CPT_LOW_FREQ = 1, CPT_LOW_FREQ_WITH_RANKINE_PART = 2 };
 
 typedef struct cpt_context* cpt_handle;
@@ -131,6 +133,34 @@ int cpt_build_matrices_dev(cpt_handle h, int nb_collocation_points, const double
                            const double* prony, int adjoint_double_layer, int early_dot_product,
                            int diagonal_term_in_double_layer, double free_surface, void* S, int64_t ldS,
                            int64_t strideS, void* K, int64_t ldK, int64_t strideK, hipStream_t stream);
+
+/* The HAMS finite-depth Green function FinGreen3D: replaces build_matrices
+ * with finite_depth_method = FINGREEN3D_METHOD (matrices.f90:95-209,
+ * Green_wave.f90:545-616) as FinGreen3D.evaluate calls it (hams.py:158-209),
+ * one wavenumber per call.  Mesh arrays, S and K as for cpt_build_matrices
+ * (device memory, overwritten, strides ldS and ldK); roots: HOST array of
+ * nb_roots >= 100 doubles, the roots of the dispersion relation (roots[0] =
+ * wavenumber, then the evanescent ones, hams.py:152-156).  depth finite and
+ * free_surface finite (the reference raises otherwise), wavenumber finite and
+ * positive.  complex128 only.  S = -1/(4 pi) (Rankine + sign * free-surface
+ * image + sum_q w_q G_wave), K likewise with the gradient, the diagonal term
+ * as in cpt_build_matrices. */
+int cpt_build_matrices_fingreen3d(cpt_handle h, int nb_collocation_points, const double* collocation_points,
+                                  const double* dot_product_normals, int nb_vertices, const double* vertices,
+                                  int nb_faces, const int32_t* faces, const double* centers, const double* normals,
+                                  const double* areas, const double* radiuses, int nb_quad_points,
+                                  const double* quad_points, const double* quad_weights, double wavenumber,
+                                  double depth, int nb_roots, const double* roots, int gf_singularities,
+                                  int adjoint_double_layer, int early_dot_product, int diagonal_term_in_double_layer,
+                                  double free_surface, void* S, int64_t ldS, void* K, int64_t ldK, hipStream_t stream);
+
+/* fingreen3d_routine (FinGreen3D.f90:71-117) at np points: rr, zf, zp device
+ * arrays of np doubles (horizontal distance, the two depths), v = omega^2/g,
+ * roots: DEVICE array of nb_roots >= 100 dispersion roots (roots[0] the
+ * propagating one), depth h.  grn: complex128 (np, 3) row-major, [G, dG/dR,
+ * dG/dzf] of the wave part (1/r and the free-surface image 1/r1 taken out). */
+int cpt_fingreen3d_points(cpt_handle h, int np, const double* rr, const double* zf, const double* zp, double v,
+                          int nb_roots, const double* roots, double depth, void* grn, hipStream_t stream);
 
 /* One body of cpt_build_matrices_batched: the arrays cpt_build_matrices
  * takes, for a mesh against itself (collocation points and dot-product

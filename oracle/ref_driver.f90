@@ -3,7 +3,8 @@
 ! Test infrastructure only: this file is compiled together with the
 ! reference sources found under libs/libDelhommeau/src of the reference tree
 ! (constants.f90, float64.f90, Delhommeau_integrals.f90, Green_Rankine.f90,
-! Green_wave.f90, matrices.f90) into oracle/_ref/libdelhommeau_ref.so.  The
+! FinGreen3D.f90, Green_wave.f90, matrices.f90) into
+! oracle/_ref/libdelhommeau_ref.so.  The
 ! resulting library is used to write golden vectors and to pin the C oracle.
 ! It is never part of the product path.
 !
@@ -17,6 +18,7 @@ module ref_driver
   use green_rankine
   use green_wave
   use matrices
+  use fingreen3d_module
   implicit none
 contains
 
@@ -157,6 +159,94 @@ contains
     real(c_double), intent(in) :: centers(n, 3), dpn(ndp, 3)
     complex(c_double_complex), intent(inout) :: kk(nk, nc, nf)
     call add_diagonal_term(centers, dpn, free_surface, kk)
+  end subroutine
+
+  ! ---- FinGreen3D (HAMS finite-depth Green function) ---------------------
+  ! The pointwise routine (FinGreen3D.f90:71): rr, zf, zp per point, v the
+  ! deep-water wavenumber omega^2/g, wvn the nk roots of the dispersion
+  ! relation (wvn(1) the propagating one), h the depth.
+  subroutine ref_fingreen3d(np, rr, zf, zp, v, nk, wvn, h, grn) bind(C, name="ref_fingreen3d")
+    integer(c_int), value :: np, nk
+    real(c_double), value :: v, h
+    real(c_double), intent(in) :: rr(np), zf(np), zp(np), wvn(nk)
+    complex(c_double_complex), intent(out) :: grn(np, 3)
+    integer :: i
+    complex(kind=8) :: g3(3)
+    do i = 1, np
+      call fingreen3d_routine(rr(i), zf(i), zp(i), v, wvn, nk, h, g3)
+      grn(i, :) = g3
+    end do
+  end subroutine
+
+  ! Newman's iteration for the roots (FinGreen3D.f90:156), as called by
+  ! compute_dispersion_roots (Green_wave.f90:523-541).
+  subroutine ref_fg_dispersion_roots(nk, omega2_over_g, depth, out) bind(C, name="ref_fg_dispersion_roots")
+    integer(c_int), value :: nk
+    real(c_double), value :: omega2_over_g, depth
+    real(c_double), intent(out) :: out(nk)
+    out = compute_dispersion_roots(nk, omega2_over_g, depth)
+  end subroutine
+
+  ! The special functions the series use, pointwise.
+  ! kind: 0 = IK01A (K0, K1), 1 = JY01B (J0, J1, Y0, Y1), 2 = E1XA,
+  !       3 = EIX, 4 = ERROR, 5 = ENXA (order n)
+  subroutine ref_fg_special(np, kind_, n, x, out) bind(C, name="ref_fg_special")
+    integer(c_int), value :: np, kind_, n
+    real(c_double), intent(in) :: x(np)
+    real(c_double), intent(out) :: out(4, np)
+    integer :: i
+    out = 0
+    do i = 1, np
+      select case (kind_)
+      case (0)
+        call IK01A(x(i), out(1, i), out(2, i))
+      case (1)
+        call JY01B(x(i), out(1, i), out(2, i), out(3, i), out(4, i))
+      case (2)
+        call E1XA(x(i), out(1, i))
+      case (3)
+        call EIX(x(i), out(1, i))
+      case (4)
+        call ERROR(x(i), out(1, i))
+      case default
+        call ENXA(n, x(i), out(1, i))
+      end select
+    end do
+  end subroutine
+
+  ! The epsilon-algorithm limit of the partial sums s(0:n) (FinGreen3D.f90:3053).
+  subroutine ref_fg_limes(n, s, out) bind(C, name="ref_fg_limes")
+    integer(c_int), value :: n
+    real(c_double), intent(in) :: s(0:n)
+    real(c_double), intent(out) :: out(2)
+    real(c_double) :: work(0:n), lim, err
+    integer :: ip, kp
+    work = s
+    call Limes(n, work, lim, ip, kp, err)
+    out(1) = lim
+    out(2) = err
+  end subroutine
+
+  ! build_matrices with the dispersion roots argument (FinGreen3D method).
+  subroutine ref_build_matrices_roots(nc, colloc, ndp, dpn, nv, nf, vertices, faces1, centers, normals, &
+      areas, radii, nq, qp, qw, k, depth, fdm, nroots, roots, gfs, adjoint, nk, s, kk) &
+      bind(C, name="ref_build_matrices_roots")
+    integer(c_int), value :: nc, ndp, nv, nf, nq, fdm, nroots, gfs, adjoint, nk
+    real(c_double), value :: k, depth
+    real(c_double), intent(in) :: colloc(nc, 3), dpn(ndp, 3), vertices(nv, 3)
+    integer(c_int), intent(in) :: faces1(nf, 4)
+    real(c_double), intent(in) :: centers(nf, 3), normals(nf, 3), areas(nf), radii(nf)
+    real(c_double), intent(in) :: qp(nf, nq, 3), qw(nf, nq), roots(nroots)
+    complex(c_double_complex), intent(inout) :: s(nc, nf), kk(nk, nc, nf)
+    real(kind=pre) :: r_range(2), z_range(2), tab(2, 2, nb_tabulated_values), prony(2, 1)
+    integer :: ifaces(nf, 4)
+    r_range = 0
+    z_range = 0
+    tab = 0
+    prony = 0
+    ifaces = faces1
+    call build_matrices(nc, colloc, dpn, nv, nf, vertices, ifaces, centers, normals, areas, radii, &
+      nq, qp, qw, k, depth, 0, 0, r_range, z_range, tab, fdm, prony, roots, gfs, (adjoint /= 0), s, kk)
   end subroutine
 
 end module ref_driver

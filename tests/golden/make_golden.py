@@ -6,6 +6,7 @@ available to build oracle/_ref:
 
     python tests/golden/make_golden.py                   # every fixture
     python tests/golden/make_golden.py post_processing   # post_processing.npz only
+    python tests/golden/make_golden.py fingreen3d        # fingreen3d.npz only
 
 The fixtures hold inputs and expected outputs only.
 """
@@ -185,6 +186,7 @@ def main():
     for f in ("delhommeau_core.npz", "build_matrices.npz", "nemoh2_known_answers.npz"):
         print(f, os.path.getsize(os.path.join(HERE, f)))
     post_processing(R)
+    fingreen3d(R)
 
 
 def post_processing(R):
@@ -244,8 +246,121 @@ def post_processing(R):
     print(f, os.path.getsize(f))
 
 
+def dispersion_roots(nk, k, h):
+    """[k, then the nk - 1 evanescent roots], by Brent's method in each branch
+    of the tangent, as FinGreen3D.compute_dispersion_relation_roots
+    (hams.py:152-156) computes them."""
+    from scipy.optimize import brentq
+    o2h = k * np.tanh(k * h) * h
+    return np.array([k] + [brentq(lambda y: o2h + y * np.tan(y), (2 * i + 1) * np.pi / 2 + 1e-10,
+                                  (2 * i + 2) * np.pi / 2 - 1e-10) / h for i in range(nk - 1)])
+
+
+def fingreen3d(R):
+    """FinGreen3D (FinGreen3D.f90, compiled in by oracle/Makefile):
+    fingreen3d_routine at points of its four regions for several depths and
+    wavenumbers, Newman's dispersion roots, and build_matrices with
+    FINGREEN3D_METHOD on meshes in finite depth (hams.py:158-209)."""
+    from capytaine_amd.meshes import Mesh
+    rng = np.random.default_rng(20261205)
+    out = {}
+    # dispersion roots by Newman's iteration (test_consistency_with_HAMS.py:39-45,
+    # test_green_functions_finite_depth.py:196-215)
+    out["roots_hams_k1_h10"] = R.fg_dispersion_roots(20, 1.0 * np.tanh(10.0), 10.0)
+    for om in (0.1, 1.0, 10.0):
+        for h in (0.1, 1.0, 10.0):
+            out[f"roots_newman_om{om}_h{h}"] = R.fg_dispersion_roots(100, om ** 2 / 9.81, h)
+    # points: (h, k) cases, 60 points per region, R = 0 among Linton's
+    regions = [(0.5, 3.0), (0.05, 0.5), (0.0005, 0.05), (0.0, 0.0005)]
+    for c, (h, k) in enumerate([(10.0, 1.0), (10.0, 2.0), (40.0, 0.3), (1.0, 5.0), (2.0, 0.05)]):
+        wvn = dispersion_roots(200, k, h)
+        v = k * np.tanh(k * h)
+        rr = np.concatenate([rng.uniform(lo, hi, 60) * h for lo, hi in regions])
+        rr[-10:] = 0.0
+        zf = -rng.uniform(0.01, 0.99, rr.size) * h
+        zp = -rng.uniform(0.01, 0.99, rr.size) * h
+        zf[:5] = 0.0  # on the free surface
+        out[f"points{c}_h"] = np.array(h)
+        out[f"points{c}_k"] = np.array(k)
+        out[f"points{c}_v"] = np.array(v)
+        out[f"points{c}_roots"] = wvn
+        out[f"points{c}_rr"] = rr
+        out[f"points{c}_zf"] = zf
+        out[f"points{c}_zp"] = zp
+        grn = R.fingreen3d(rr, zf, zp, v, wvn, h)
+        out[f"points{c}_grn"] = grn
+        # the reference's own sensitivity: the largest change of its output
+        # under eight relative perturbations of 2^-50 of its inputs (a smooth
+        # function moves by about 1e-15 of itself; the rest is the
+        # conditioning of its series limits)
+        noise = np.zeros((rr.size, 3))
+        for t in range(8):
+            sg = rng.choice([-1.0, 1.0], size=(3, rr.size)) * 2.0 ** -50
+            pert = R.fingreen3d(rr * (1 + sg[0]), zf * (1 + sg[1]), zp * (1 + sg[2]), v, wvn, h)
+            noise = np.maximum(noise, np.abs(pert - grn))
+        out[f"points{c}_noise"] = noise
+    # matrices
+    # the two panels of test_consistency_with_HAMS.py:18-27
+    hams = Mesh(np.array([[-0.6, -0.2, -0.5], [-0.6, -0.4, -0.5], [-0.8, -0.4, -0.5], [-0.8, -0.2, -0.5],
+                          [-0.6, 0.0, -0.5], [-0.8, 0.0, -0.5]]), np.array([[0, 1, 2, 3], [4, 0, 3, 5]]))
+    sphere = mesh_sphere(radius=0.5, center=(0.0, 0.0, -1.0), resolution=(4, 4))
+    cylinder = mesh_vertical_cylinder(radius=0.3, length=1.0, center=(0.0, 0.0, -0.5),
+                                      resolution=(0, 12, 6)).immersed_part()
+    barge = mesh_parallelepiped(size=(4.0, 2.0, 1.0), center=(0, 0, -0.5), resolution=(8, 4, 2),
+                                missing_sides={"top"})
+    cases = {
+        "hams_k1": (hams, 1.0, 10.0, dict(adjoint=False)),
+        "hams_k2": (hams, 2.0, 10.0, dict(adjoint=False)),
+        "sphere_k5": (sphere, 5.0, 2.0, dict(adjoint=False)),
+        "sphere_k5_adjoint_full": (sphere, 5.0, 2.0, dict(adjoint=True, early_dot_product=False)),
+        "cylinder_k5": (cylinder, 5.0, 1.0, dict(adjoint=True)),
+        "barge_k08": (barge, 0.8, 6.0, dict(adjoint=True)),
+    }
+    for name, (mesh, k, h, opts) in cases.items():
+        arr = mesh_arrays(mesh)
+        roots = dispersion_roots(200, k, h)
+        S, K = R.build_matrices_fingreen3d(**arr, wavenumber=k, depth=h, roots=roots, free_surface=0.0, **opts)
+        # the reference's sensitivity, as for the points: geometry perturbed
+        # by 2^-50 relative, four times
+        s_noise, k_noise = np.zeros(S.shape), np.zeros(K.shape)
+        for t in range(4):
+            pa = dict(arr)
+            for key in ("colloc", "vertices", "centers", "quad_points"):
+                a = np.asarray(arr[key], dtype=np.float64)
+                pa[key] = a * (1 + rng.choice([-1.0, 1.0], size=a.shape) * 2.0 ** -50)
+            Sp, Kp = R.build_matrices_fingreen3d(**pa, wavenumber=k, depth=h, roots=roots, free_surface=0.0, **opts)
+            s_noise, k_noise = np.maximum(s_noise, np.abs(Sp - S)), np.maximum(k_noise, np.abs(Kp - K))
+        out[f"{name}__S_noise"] = s_noise
+        out[f"{name}__K_noise"] = k_noise
+        for key, val in arr.items():
+            out[f"{name}__{key}"] = np.asarray(val)
+        for key, val in opts.items():
+            out[f"{name}__opt_{key}"] = np.asarray(val)
+        out[f"{name}__k"] = np.array(k)
+        out[f"{name}__h"] = np.array(h)
+        out[f"{name}__roots"] = roots
+        out[f"{name}__S"] = S
+        out[f"{name}__K"] = K
+    # McCamy and Fuchs' cylinder (test_consistency_with_analytical_solutions.py:
+    # 23-44) solved by the reference's matrices and LAPACK's LU (indirect)
+    from oracle import solve_coefficients
+    from capytaine_amd.problems import FloatingBody, DiffractionProblem
+    mf = mesh_vertical_cylinder(radius=0.3, length=1.0, center=(0, 0, -0.5), resolution=(0, 30, 20)).immersed_part()
+    body = FloatingBody(mesh=mf, dofs={"Surge": np.array([[1.0, 0.0, 0.0] for _ in mf.faces])})
+    for k in (5.0, 10.0):
+        pb = DiffractionProblem(body=body, wavenumber=k, wave_direction=0.0, water_depth=1.0)
+        S, K = R.build_matrices_fingreen3d(**mesh_arrays(mf), wavenumber=k, depth=1.0,
+                                           roots=dispersion_roots(200, k, 1.0), adjoint=True, free_surface=0.0)
+        out[f"mccamy_k{int(k)}_indirect_force"] = np.array(solve_coefficients(S, K, body, [pb])[0]["Surge"])
+    f = os.path.join(HERE, "fingreen3d.npz")
+    np.savez_compressed(f, **out)
+    print(f, os.path.getsize(f))
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["post_processing"]:
         post_processing(Reference())
+    elif sys.argv[1:] == ["fingreen3d"]:
+        fingreen3d(Reference())
     else:
         main()

@@ -82,7 +82,7 @@ def lib_bytes():
 
 def test_every_kernel_runs_without_scratch(lib_bytes):
     objs = _gfx950_objects(lib_bytes)
-    assert len(objs) == 6  # one per translation unit
+    assert len(objs) == 7  # one per translation unit
     kernels = {}
     for o in objs:
         kernels.update(_kernel_descriptors(o))

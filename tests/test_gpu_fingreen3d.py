@@ -1,0 +1,198 @@
+"""FinGreen3D on the GPU against the oracle (the reference's FinGreen3D.f90,
+compiled into oracle/_ref; fixtures of tests/golden/fingreen3d.npz) and the
+reference's own tests of it:
+
+  * cpt_build_matrices_fingreen3d (build_matrices with FINGREEN3D_METHOD) and
+    cpt_fingreen3d_points (fingreen3d_routine): the largest difference within
+    1e-10 of the largest value plus ten times the reference's own
+    sensitivity, recorded with the fixtures: the largest change of its
+    output when its inputs move by 2^-50 relative.  A smooth function moves
+    by ~1e-15 of itself there; the reference's epsilon-algorithm limits move
+    by up to 5e-2 of the largest value in Pidcock's region (0.0005 <= R/h <
+    0.05), which bounds what a reimplementation with other elementary
+    functions (the device's exp, log, cos) can match there.  Points: 300 in
+    each of five (depth, wavenumber) cases, region by region.  Matrices:
+    HAMS's two panels, a sphere (direct and adjoint, full gradient), a
+    vertical cylinder (R = 0 pairs: Linton's R < 1e-6 branch), a barge;
+  * HAMS's published S (test_consistency_with_HAMS.py:48-66, rtol 1e-3);
+  * FinGreen3D against Delhommeau in finite depth
+    (test_green_functions_finite_depth.py:186-193, rtol and atol 0.1);
+  * McCamy and Fuchs' analytical diffraction force on a vertical cylinder
+    (test_consistency_with_analytical_solutions.py:23-44, rtol 0.05), direct
+    and indirect, and the indirect force against the reference's own
+    pipeline on the same mesh (its matrices and LAPACK's LU) to 1e-6.  At
+    k = 10 that pipeline itself lands 7.1 % from the analytical force with
+    this mesh generator's cylinder, so the analytical bound is 0.08 there;
+  * the batched solve (solve_all) equal to problem-by-problem solves."""
+import numpy as np
+import pytest
+
+from conftest import load_golden
+from gpu_helpers import device
+
+pytestmark = pytest.mark.gpu
+NCASES = 5
+REGIONS = [("eigen", 0.5, np.inf), ("eigene", 0.05, 0.5), ("pidcock", 0.0005, 0.05), ("linton", -1.0, 0.0005)]
+
+
+@pytest.fixture(scope="module")
+def fg():
+    return load_golden("fingreen3d.npz")
+
+
+def _points(rr, zf, zp, v, roots, h):
+    import torch
+    from capytaine_amd import _lib
+    from capytaine_amd._lib import check, ptr, stream_handle
+    dev = device()
+    t = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64)).to(dev)
+    rr_d, zf_d, zp_d, roots_d = t(rr), t(zf), t(zp), t(roots)
+    out = torch.empty((len(rr), 3), dtype=torch.complex128, device=dev)
+    ctx = _lib.Context.get(dev)
+    check(_lib.load().cpt_fingreen3d_points(ctx.handle, len(rr), ptr(rr_d), ptr(zf_d), ptr(zp_d), float(v), len(roots),
+                                            ptr(roots_d), float(h), ptr(out), stream_handle(dev)),
+          "cpt_fingreen3d_points")
+    return out.cpu().numpy()
+
+
+@pytest.mark.parametrize("c", range(NCASES))
+def test_points_match_oracle(fg, c):
+    g = lambda k: fg[f"points{c}_{k}"]
+    h, k, v = float(g("h")), float(g("k")), float(g("v"))
+    rr, zf, zp, grn, noise = g("rr"), g("zf"), g("zp"), g("grn"), g("noise")
+    print(f"[gpu] FinGreen3D points, h {h} k {k}", flush=True)
+    got = _points(rr, zf, zp, v, g("roots"), h)
+    scale = np.abs(grn).max(axis=0)
+    for name, lo, hi in REGIONS:
+        t = np.abs(rr / h)
+        sel = ((t >= lo) & (t < hi)) if lo >= 0 else (t < hi)
+        a, b = got[sel], grn[sel]
+        d = np.abs(a - b).max(axis=0)
+        tol = 10 * noise[sel].max(axis=0) + 1e-10 * scale
+        print(f"[gpu]   {name}: max err / max |component| {np.array2string(d / scale, precision=1)}, "
+              f"reference sensitivity {np.array2string(noise[sel].max(axis=0) / scale, precision=1)}, "
+              f"bit-equal {np.mean(a == b):.2f}", flush=True)
+        assert np.all(d <= tol)
+
+
+def _build(arr, k, h, roots, *, adjoint=True, early_dot_product=True, diagonal_term=True):
+    import torch
+    from capytaine_amd import _lib
+    from capytaine_amd._lib import check, ptr, stream_handle
+    dev = device()
+    t = lambda a, dt=torch.float64: torch.as_tensor(np.ascontiguousarray(a), dtype=dt).to(dev)
+    n, m = arr["colloc"].shape[0], arr["faces"].shape[0]
+    qw = np.asarray(arr["quad_weights"])
+    nk = 1 if early_dot_product else 3
+    S = torch.empty((m, n), dtype=torch.complex128, device=dev)
+    K = torch.empty((m, n) if nk == 1 else (m, n, 3), dtype=torch.complex128, device=dev)
+    keep = [t(arr["colloc"]), t(arr["dpn"]), t(arr["vertices"]), t(arr["faces"], torch.int32), t(arr["centers"]),
+            t(arr["normals"]), t(arr["areas"]), t(arr["radii"]), t(arr["quad_points"]), t(qw)]
+    roots = np.ascontiguousarray(roots, dtype=np.float64)
+    ctx = _lib.Context.get(dev)
+    check(_lib.load().cpt_build_matrices_fingreen3d(
+        ctx.handle, n, ptr(keep[0]), ptr(keep[1]), arr["vertices"].shape[0], ptr(keep[2]), m, ptr(keep[3]),
+        ptr(keep[4]), ptr(keep[5]), ptr(keep[6]), ptr(keep[7]), qw.shape[1], ptr(keep[8]), ptr(keep[9]), float(k),
+        float(h), len(roots), roots.ctypes.data, 1, int(adjoint), int(early_dot_product), int(diagonal_term), 0.0,
+        ptr(S), n, ptr(K), n, stream_handle(dev)), "cpt_build_matrices_fingreen3d")
+    S = S.cpu().numpy().T
+    K = K.cpu().numpy().T if nk == 1 else K.cpu().numpy().transpose(2, 1, 0)
+    return S, K
+
+
+@pytest.mark.parametrize("name", ["hams_k1", "hams_k2", "sphere_k5", "sphere_k5_adjoint_full", "cylinder_k5",
+                                  "barge_k08"])
+def test_matrices_match_oracle(fg, name):
+    g = lambda key: fg[f"{name}__{key}"]
+    arr = {key: g(key) for key in ("colloc", "dpn", "vertices", "faces", "centers", "normals", "areas", "radii",
+                                   "quad_points", "quad_weights")}
+    opts = {key[len(name) + 6:]: fg[key].item() for key in fg.files if key.startswith(f"{name}__opt_")}
+    print(f"[gpu] FinGreen3D matrices {name} ({arr['faces'].shape[0]} panels) {opts}", flush=True)
+    S, K = _build(arr, float(g("k")), float(g("h")), g("roots"), **opts)
+    Sr, Kr = g("S"), g("K")
+    es = np.abs(S - Sr).max() / np.abs(Sr).max()
+    ek = np.abs(K - Kr).max() / np.abs(Kr).max()
+    ns, nk = g("S_noise").max() / np.abs(Sr).max(), g("K_noise").max() / np.abs(Kr).max()
+    print(f"[gpu]   S {es:.1e}, K {ek:.1e} of the largest entries (reference sensitivity {ns:.1e}, {nk:.1e})",
+          flush=True)
+    assert np.abs(S - Sr).max() <= 10 * g("S_noise").max() + 1e-10 * np.abs(Sr).max()
+    assert np.abs(K - Kr).max() <= 10 * g("K_noise").max() + 1e-10 * np.abs(Kr).max()
+
+
+def _hams_mesh():
+    import capytaine_amd as cpa
+    vertices = np.array([[-0.6, -0.2, -0.5], [-0.6, -0.4, -0.5], [-0.8, -0.4, -0.5], [-0.8, -0.2, -0.5],
+                         [-0.6, 0.0, -0.5], [-0.8, 0.0, -0.5]])
+    return cpa.Mesh(vertices, np.array([[0, 1, 2, 3], [4, 0, 3, 5]]))
+
+
+@pytest.mark.parametrize("k,published", [
+    (1.0, [[0.73247863590752549 + 2.3114545333287739j, -0.43898677720635493 + 2.2883993524884838j],
+           [-0.43898677720635493 + 2.2883993524884838j, 0.73247863590752549 + 2.3114545333287739j]]),
+    (2.0, [[-0.36177831640996905 + 1.7007340213300215j, -1.7129415732152193 + 1.6333866990767409j],
+           [-1.7129415732152193 + 1.6333866990767409j, -0.36177831640996905 + 1.7007340213300215j]]),
+])
+def test_hams_published_S(k, published):
+    import capytaine_amd as cpa
+    mesh = _hams_mesh()
+    S, _ = cpa.FinGreen3D(device=device()).evaluate(mesh, mesh, free_surface=0.0, water_depth=10.0, wavenumber=k,
+                                                    adjoint_double_layer=False)
+    S_r, _ = cpa.Delhommeau(tabulation_cache_dir=None, device=device()).evaluate(
+        mesh, mesh, free_surface=np.inf, water_depth=np.inf, wavenumber=0.0, adjoint_double_layer=False)
+    np.testing.assert_allclose(-4 * np.pi * (S - S_r), np.array(published) * mesh.faces_areas[0], rtol=1e-3)
+
+
+def test_fingreen3d_vs_delhommeau():
+    import capytaine_amd as cpa
+    mesh = cpa.mesh_sphere(radius=0.5, center=(0.0, 0.0, -1.0), resolution=(4, 4))
+    S, D = cpa.FinGreen3D(device=device()).evaluate(mesh, mesh, free_surface=0.0, water_depth=2.0, wavenumber=5.0,
+                                                    adjoint_double_layer=False)
+    S_r, D_r = cpa.Delhommeau(tabulation_cache_dir=None, device=device()).evaluate(
+        mesh, mesh, free_surface=0.0, water_depth=2.0, wavenumber=5.0, adjoint_double_layer=False)
+    np.testing.assert_allclose(S, S_r, rtol=1e-1, atol=1e-1)
+    np.testing.assert_allclose(D, D_r, rtol=1e-1, atol=1e-1)
+
+
+@pytest.mark.parametrize("method", ["direct", "indirect"])
+@pytest.mark.parametrize("k", [5.0, 10.0])
+def test_mccamy_and_fuchs(fg, k, method):
+    from scipy.special import j0, y0, jn, yn
+    import capytaine_amd as cpa
+    h, R = 1.0, 0.3
+    print(f"[gpu] McCamy-Fuchs, FinGreen3D, k {k} {method}", flush=True)
+    mesh = cpa.mesh_vertical_cylinder(radius=R, length=h, center=(0, 0, -h / 2), resolution=(0, 30, 20)).immersed_part()
+    body = cpa.FloatingBody(mesh=mesh, dofs={"Surge": np.array([[1.0, 0.0, 0.0] for _ in mesh.faces])})
+    pb = cpa.DiffractionProblem(body=body, wavenumber=k, wave_direction=0.0, water_depth=h)
+    m0 = pb.omega ** 2 / pb.g
+    dy = (y0(m0 * R) - yn(2, m0 * R))
+    dj = (j0(m0 * R) - jn(2, m0 * R))
+    analytical = (4 * pb.rho * pb.g * h * R) * (-1j * 2 * h / R) * np.tanh(m0 * h) / (m0 * h) ** 2 * \
+        (dy + 1j * dj) / (dy ** 2 + dj ** 2)
+    res = cpa.BEMSolver(method=method, green_function=cpa.FinGreen3D(device=device())).solve(pb)
+    numerical = res.forces["Surge"] + cpa.froude_krylov_force(pb)["Surge"]
+    print(f"[gpu]   numerical {numerical:.4e}, analytical {analytical:.4e}", flush=True)
+    if method == "indirect":
+        ref = complex(fg[f"mccamy_k{int(k)}_indirect_force"])
+        print(f"[gpu]   reference pipeline {ref:.6e}: {abs(res.forces['Surge'] - ref) / abs(ref):.1e}", flush=True)
+        assert abs(res.forces["Surge"] - ref) <= 1e-6 * abs(ref)
+    np.testing.assert_allclose(numerical, analytical, rtol=0.08 if (k, method) == (10.0, "indirect") else 0.05)
+
+
+def test_solve_all_batched_equals_single():
+    import capytaine_amd as cpa
+    print("[gpu] FinGreen3D solve_all vs solve", flush=True)
+    mesh = cpa.mesh_parallelepiped(size=(4.0, 2.0, 1.0), center=(0, 0, -0.5), resolution=(8, 4, 2),
+                                   missing_sides={"top"})
+    body = cpa.FloatingBody(mesh).add_all_rigid_body_dofs(rotation_center=(0, 0, 0))
+    solver = cpa.BEMSolver(green_function=cpa.FinGreen3D(device=device()))
+    pbs = [cpa.RadiationProblem(body=body, radiating_dof=d, omega=om, water_depth=6.0)
+           for om in (0.8, 1.3) for d in ("Heave", "Pitch")]
+    pbs += [cpa.DiffractionProblem(body=body, wave_direction=0.3, omega=1.3, water_depth=6.0)]
+    batched = solver.solve_all(pbs)
+    for pb, rb in zip(pbs, batched):
+        rs = solver.solve(pb)
+        f_s = np.array([rs.forces[d] for d in body.dofs])
+        f_b = np.array([rb.forces[d] for d in body.dofs])
+        err = np.abs(f_b - f_s).max() / np.abs(f_s).max()
+        print(f"[gpu]   {pb}: {err:.1e}", flush=True)
+        assert err <= 1e-10
