@@ -24,6 +24,7 @@ Prints one JSON line (see the repository's bench contract).
     python bench.py --workload serving       # the request-stream measurement alone
     python bench.py --workload post          # the post-processing maps alone
     python bench.py --workload symmetric     # the barge as two reflection planes alone
+    python bench.py --workload fingreen3d    # FinGreen3D assembly of the 20000-panel array
 
 The request-stream measurement runs in a child process (started after the
 headline and fleet legs have released their device memory), so that a
@@ -571,6 +572,93 @@ def cylinders_benchmark(args, rho=1025.0, g=9.81, chunk=4):
     print(json.dumps(out), flush=True)
 
 
+def fingreen3d_benchmark(args, nfreq=4, cpu_points=100):
+    """FinGreen3D (the HAMS finite-depth Green function) assembly of S and K
+    for the 5 x 5 cylinder array (20000 panels, depth 40 m) at `nfreq`
+    wavenumbers, one launch pair per wavenumber, the panels resident in HBM;
+    timed after one untimed evaluation.  The CPU baseline is the reference's
+    own FinGreen3D.f90 (oracle/_ref, OpenMP over the host cores) on the
+    first `cpu_points` collocation rows at the first wavenumber, whose rows
+    are also compared with the device's."""
+    import torch
+    import capytaine_amd as cpa
+    from capytaine_amd import workloads
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    body, omegas, _, depth = workloads.cylinder_array()
+    mesh = body.mesh
+    n = mesh.nb_faces
+    gf = cpa.FinGreen3D(device=dev)
+    sel = np.linspace(0, len(omegas) - 1, nfreq).round().astype(int)
+    from capytaine_amd.problems import wavenumber_from_omega
+    ks = [float(wavenumber_from_omega(float(omegas[i]), 9.81, depth)) for i in sel]
+    S = torch.empty((1, n, n), dtype=torch.complex128, device=dev)
+    K = torch.empty((1, n, n), dtype=torch.complex128, device=dev)
+    gf.evaluate_on_device(mesh, mesh, water_depth=depth, wavenumber=ks[0], out=(S, K))  # untimed
+    torch.cuda.synchronize()
+    times = []
+    for k in ks:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        gf.evaluate_on_device(mesh, mesh, water_depth=depth, wavenumber=k, out=(S, K))
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+        print(f"[fingreen3d] k {k:.4f}: {times[-1]:.3f} s", file=sys.stderr, flush=True)
+    t = float(np.sum(times))
+    out = {"metric": "FinGreen3D panel-pairs/s (S and K assembly)", "value": nfreq * n * n / t,
+           "unit": "panel-pairs/s", "n_gpus": 1, "steps": nfreq, "warmup": 1, "ms_per_step": 1e3 * t / nfreq,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "FinGreen3D S and K of 5 x 5 truncated vertical cylinders, radius 2 m, draft "
+                                  "4 m, 12 m apart, 20000 panels, depth 40 m, adjoint double layer",
+                      "panels": n, "wavenumbers": [float(k) for k in ks], "nb_dispersion_roots": 200}}
+    # the first rows at the first wavenumber, for the comparison with the
+    # reference (S[0] holds the column-major (n, n) matrix: rows are columns)
+    gf.evaluate_on_device(mesh, mesh, water_depth=depth, wavenumber=ks[0], out=(S, K))
+    S_dev = S[0][:, :cpu_points].T.cpu().numpy()
+    K_dev = K[0][:, :cpu_points].T.cpu().numpy()
+    del S, K
+    if args.no_cpu_baseline:
+        print(json.dumps(out), flush=True)
+        return
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    try:
+        from oracle import Reference, mesh_arrays
+        R = Reference()
+    except (ImportError, OSError, FileNotFoundError) as e:
+        out["cpu_baseline"] = {"error": f"oracle/_ref not available: {e}"}
+        print(json.dumps(out), flush=True)
+        return
+    cores = min(os.cpu_count() or 1, 16)
+    arr = mesh_arrays(mesh)
+    arr["colloc"] = arr["colloc"][:cpu_points]
+    arr["dpn"] = arr["dpn"][:cpu_points]
+    roots = cpa.FinGreen3D._roots(200, float(ks[0]), float(depth))
+    t0 = time.perf_counter()
+    S_ref, K_ref = R.build_matrices_fingreen3d(**arr, wavenumber=float(ks[0]), depth=float(depth),
+                                               roots=np.array(roots), adjoint=True, free_surface=0.0)
+    t_cpu = time.perf_counter() - t0
+    out["cpu_baseline"] = {"value": cpu_points * n / t_cpu, "unit": "panel-pairs/s", "cores": cores,
+                           "kind": "reference",
+                           "sample": f"the reference's FinGreen3D.f90 build_matrices (OpenMP, OMP_NUM_THREADS="
+                                     f"{os.environ.get('OMP_NUM_THREADS', 'unset')}) on {cpu_points} collocation "
+                                     f"rows x {n} panels at k = {ks[0]:.4f}: {t_cpu:.2f} s"}
+    out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    out["max_diff_vs_reference"] = {"S": float(np.abs(S_dev - S_ref).max() / np.abs(S_ref).max()),
+                                    "K": float(np.abs(K_dev - K_ref).max() / np.abs(K_ref).max())}
+    # the reference's own sensitivity on the same rows: its change when the
+    # geometry moves by 2^-50 relative (its series limits are not smooth)
+    rng = np.random.default_rng(0)
+    pa = dict(arr)
+    for key in ("colloc", "vertices", "centers", "quad_points"):
+        a = np.asarray(arr[key], dtype=np.float64)
+        pa[key] = a * (1 + rng.choice([-1.0, 1.0], size=a.shape) * 2.0 ** -50)
+    S_p, K_p = R.build_matrices_fingreen3d(**pa, wavenumber=float(ks[0]), depth=float(depth), roots=np.array(roots),
+                                           adjoint=True, free_surface=0.0)
+    out["reference_sensitivity"] = {"S": float(np.abs(S_p - S_ref).max() / np.abs(S_ref).max()),
+                                    "K": float(np.abs(K_p - K_ref).max() / np.abs(K_ref).max())}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -597,7 +685,8 @@ def main():
                     help="arrival rates of the serving measurement, as fractions of its saturation throughput")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner (a stored tuning of this size class is still applied)")
-    ap.add_argument("--workload", choices=("barge", "cylinders", "copy", "serving", "post", "symmetric"),
+    ap.add_argument("--workload", choices=("barge", "cylinders", "copy", "serving", "post", "symmetric",
+                                           "fingreen3d"),
                     default="barge",
                     help="barge: the headline line (plus fleet and serving); cylinders: the 20000-panel "
                          "finite-depth array; copy: a 1 GiB device copy, the calibration of the HBM "
@@ -614,6 +703,9 @@ def main():
         return
     if args.workload == "copy":
         copy_calibration()
+        return
+    if args.workload == "fingreen3d":
+        fingreen3d_benchmark(args)
         return
     if args.workload == "serving":
         import torch
