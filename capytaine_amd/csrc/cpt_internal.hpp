@@ -58,8 +58,13 @@ struct FgArgs {
     int64_t ldK;
     double* ws;            // FG_WS_DOUBLES per thread, interleaved by thread
     int nthreads;
+    unsigned* lists;       // three lists of list_cap pair indices (by series), then 3 counters
+    int list_cap;
 };
 constexpr int FG_WS_DOUBLES = 400;
+// panel pairs per pass of the sorted kernels (bounds the lists' memory)
+constexpr int FG_CHUNK_PAIRS = 1 << 24;
+size_t fingreen3d_list_bytes(int n);
 int fingreen3d_threads(int64_t items);
 int launch_fingreen3d_pairs(const FgArgs& A, hipStream_t stream);
 int launch_fingreen3d_points(int np, const double* rr, const double* zf, const double* zp, double v,

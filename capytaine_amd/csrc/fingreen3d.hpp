@@ -94,17 +94,19 @@ __host__ __device__ inline void bessel_k01(double x, double& k0, double& k1)
     const double x2 = x * x;
     double bi0 = 1.0, bi1 = 1.0;
     if (x <= 18.0) {
+        // the stopping tests |r / I| < 1e-15 written without the division
+        // (the same decision but where the quotient rounds across 1e-15)
         double r = 1.0;
         for (int k = 1; k <= 50; ++k) {
             r = 0.25 * r * x2 / (double)(k * k);
             bi0 += r;
-            if (fabs(r / bi0) < 1.0e-15) break;
+            if (fabs(r) < 1.0e-15 * fabs(bi0)) break;
         }
         r = 1.0;
         for (int k = 1; k <= 50; ++k) {
             r = 0.25 * r * x2 / (double)(k * (k + 1));
             bi1 += r;
-            if (fabs(r / bi1) < 1.0e-15) break;
+            if (fabs(r) < 1.0e-15 * fabs(bi1)) break;
         }
         bi1 = 0.5 * x * bi1;
     } else {
@@ -138,7 +140,7 @@ __host__ __device__ inline void bessel_k01(double x, double& k0, double& k1)
             w0 += 1.0 / k;
             r = 0.25 * r / (double)(k * k) * x2;
             s += r * (w0 + ct);
-            if (fabs((s - prev) / s) < 1.0e-15) break;
+            if (fabs(s - prev) < 1.0e-15 * fabs(s)) break;
             prev = s;
         }
         k0 = s + ct;
@@ -282,28 +284,68 @@ __host__ __device__ inline double derfc(double x) { return -2.0 / sqrt(PI) * exp
 // s and a (n+1 entries, a scratch) are strided by st.
 __host__ __device__ inline double limes(int n, double* s, double* a, size_t st)
 {
+    // The tables live in global memory: each pass loads a block of LB
+    // entries before it stores any of them (every entry a pass reads is one
+    // it has not yet overwritten), so a wavefront waits on memory once per
+    // block instead of once per entry.  The arithmetic and its order are the
+    // sequential algorithm's.
+    constexpr int LB = 8;
     double lim = s[n * st];
     for (int i = 0; i <= n; ++i) a[i * st] = 0.0;
     double amax = 0.0;
     for (int k = 1;; ++k) {
-        if (n - 2 * k + 1 < 0) break;
-        // odd column from the previous even one (cross rule)
-        for (int i = 0; i <= n - 2 * k + 1; ++i) {
-            const double d = s[(i + 1) * st] - s[i * st];
-            const double up = a[(i + 1) * st];
-            a[i * st] = (d != 0.0) ? up + 1.0 / d : up;
-        }
-        if (n - 2 * k < 0) break;
-        for (int i = 0; i <= n - 2 * k; ++i) {
-            const double ai = a[i * st], ai1 = a[(i + 1) * st];
-            const double next = s[(i + 1) * st];
-            const double v = (ai1 != ai) ? next + 1.0 / (ai1 - ai) : next;
-            s[i * st] = v;
-            if (fabs(ai) > amax) {
-                amax = fabs(ai);
-                lim = v;
-                if (next == v) return lim;
+        const int la = n - 2 * k + 1;  // last index of the odd-column pass
+        if (la < 0) break;
+        // odd column from the previous even one (cross rule):
+        // a(i) = a(i+1) + 1 / (s(i+1) - s(i))
+        double sp = s[0];
+        for (int i0 = 0; i0 <= la; i0 += LB) {
+            double sv[LB], av[LB];
+#pragma unroll
+            for (int b = 0; b < LB; ++b) {
+                const int j = i0 + 1 + b;
+                sv[b] = j <= n ? s[j * st] : 0.0;
+                av[b] = j <= n ? a[j * st] : 0.0;
             }
+#pragma unroll
+            for (int b = 0; b < LB; ++b) {
+                const int i = i0 + b;
+                if (i <= la) {
+                    const double d = sv[b] - (b == 0 ? sp : sv[b - 1]);
+                    a[i * st] = (d != 0.0) ? av[b] + 1.0 / d : av[b];
+                }
+            }
+            sp = sv[LB - 1];
+        }
+        const int ls = n - 2 * k;  // last index of the even-column pass
+        if (ls < 0) break;
+        // even column: s(i) = s(i+1) + 1 / (a(i+1) - a(i)); the entry whose
+        // a is the largest so far is the limit
+        double ap = a[0];
+        for (int i0 = 0; i0 <= ls; i0 += LB) {
+            double sv[LB], av[LB];
+#pragma unroll
+            for (int b = 0; b < LB; ++b) {
+                const int j = i0 + 1 + b;
+                sv[b] = j <= n ? s[j * st] : 0.0;
+                av[b] = j <= n ? a[j * st] : 0.0;
+            }
+#pragma unroll
+            for (int b = 0; b < LB; ++b) {
+                const int i = i0 + b;
+                if (i <= ls) {
+                    const double ai = (b == 0 ? ap : av[b - 1]), ai1 = av[b];
+                    const double next = sv[b];
+                    const double v = (ai1 != ai) ? next + 1.0 / (ai1 - ai) : next;
+                    s[i * st] = v;
+                    if (fabs(ai) > amax) {
+                        amax = fabs(ai);
+                        lim = v;
+                        if (next == v) return lim;
+                    }
+                }
+            }
+            ap = av[LB - 1];
         }
     }
     return lim;
@@ -384,13 +426,18 @@ __host__ __device__ inline void eigene(double R, double ZF, double ZP, double WK
     double* gr = ws + LIMES_LEN * st;
     double* gz = ws + 2 * LIMES_LEN * st;
     double* a = ws + 3 * LIMES_LEN * st;
+    // partial sums carried in registers, stored for the epsilon tables
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
     g[0] = gr[0] = gz[0] = 0.0;
     for (int i = 2; i <= nt; ++i) {
         double rm[3];
         evanescent(R, ZF, ZP, wvn[i - 1], H, rm);
-        g[(i - 1) * st] = g[(i - 2) * st] + rm[0];
-        gr[(i - 1) * st] = gr[(i - 2) * st] + rm[1];
-        gz[(i - 1) * st] = gz[(i - 2) * st] + rm[2];
+        s0 = s0 + rm[0];
+        s1 = s1 + rm[1];
+        s2 = s2 + rm[2];
+        g[(i - 1) * st] = s0;
+        gr[(i - 1) * st] = s1;
+        gz[(i - 1) * st] = s2;
     }
     o.g[0].re += limes(nt - 1, g, a, st);
     o.g[1].re += limes(nt - 1, gr, a, st);
@@ -454,6 +501,7 @@ __host__ __device__ inline void pidcock(double R, double ZF, double ZP, double V
     double* gr = ws + LIMES_LEN * st;
     double* gz = ws + 2 * LIMES_LEN * st;
     double* a = ws + 3 * LIMES_LEN * st;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
     g[0] = gr[0] = gz[0] = 0.0;
     for (int i = 2; i <= nt; ++i) {
         const double umt = (i - 1) * PI / H;  // the evanescent roots' deep-water limit m pi / h
@@ -467,9 +515,12 @@ __host__ __device__ inline void pidcock(double R, double ZF, double ZP, double V
         const double amt = cos(umt * (ZF + H)) * ct / H;
         const double dam = -um * ac * sin(um * (ZF + H)) * cu;
         const double damt = -umt * sin(umt * (ZF + H)) * ct / H;
-        g[(i - 1) * st] = g[(i - 2) * st] + (am * K0 - amt * K0t);
-        gr[(i - 1) * st] = gr[(i - 2) * st] + (-um * am * K1 + umt * amt * K1t);
-        gz[(i - 1) * st] = gz[(i - 2) * st] + (dam * K0 - damt * K0t);
+        s0 = s0 + (am * K0 - amt * K0t);
+        s1 = s1 + (-um * am * K1 + umt * amt * K1t);
+        s2 = s2 + (dam * K0 - damt * K0t);
+        g[(i - 1) * st] = s0;
+        gr[(i - 1) * st] = s1;
+        gz[(i - 1) * st] = s2;
     }
     const double l0 = limes(nt - 1, g, a, st), l1 = limes(nt - 1, gr, a, st), l2 = limes(nt - 1, gz, a, st);
     double im[3];
@@ -771,6 +822,31 @@ __host__ __device__ inline void linton(double R, double ZF, double ZP, double V,
     g[1] = g[1] + chp * (chf * (bj1 * ik4));
     g[2] = g[2] - chp * (shf * (bj0 * ik4));
     for (int c = 0; c < 3; ++c) o.g[c] = (-4.0 * PI) * g[c];
+}
+
+// The series fingreen3d_routine picks for R/h: 0 eigenfunction expansion,
+// 1 the same with its epsilon limit, 2 Pidcock's form, 3 Linton's form.
+__host__ __device__ inline int region(double RR, double H)
+{
+    const double rhp = fabs(RR / H);
+    return rhp < 0.0005 ? 3 : (rhp < 0.05 ? 2 : (rhp < 0.5 ? 1 : 0));
+}
+
+// One region's series, for kernels whose pairs were sorted by region
+// (REG = region(RR, H) is the caller's guarantee); the R = 0 convention of
+// point() included.
+template <int REG>
+__host__ __device__ __forceinline__ Grn point_in(double RR, double ZF, double ZP, double V, const double* wvn,
+                                                 double H, double* ws, size_t st)
+{
+    Grn o;
+    const double WK = wvn[0];
+    if (REG == 0) eigen(RR, ZF, ZP, WK, wvn, H, o);
+    else if (REG == 1) eigene(RR, ZF, ZP, WK, wvn, H, ws, st, o);
+    else if (REG == 2) pidcock(RR, ZF, ZP, V, WK, wvn, H, ws, st, o);
+    else linton(RR, ZF, ZP, V, WK, wvn, H, o);
+    if (RR == 0.0) o.g[1] = cmk(0.0, 0.0);
+    return o;
 }
 
 // fingreen3d_routine: the region choice and the R = 0 convention (dG/dR = 0).
