@@ -341,6 +341,32 @@ def fingreen3d(R):
         out[f"{name}__roots"] = roots
         out[f"{name}__S"] = S
         out[f"{name}__K"] = K
+    # points in the fluid around the barge (post-processing: zero normals, the
+    # full gradient), as abstract_green_function.py:42-54 evaluates them
+    pts = []
+    while len(pts) < 25:
+        q = rng.uniform([-3.0, -2.0, -5.5], [3.0, 2.0, -0.05])
+        if np.any(np.abs(q - (0, 0, -0.5)) > (2.05, 1.05, 0.55)):
+            pts.append(q)
+    pts = np.array(pts)
+    arr = dict(mesh_arrays(barge), colloc=pts, dpn=np.zeros_like(pts))
+    roots = dispersion_roots(200, 0.8, 6.0)
+    S, K = R.build_matrices_fingreen3d(**arr, wavenumber=0.8, depth=6.0, roots=roots, free_surface=0.0,
+                                       adjoint=True, early_dot_product=False, diagonal_term=False)
+    s_noise, k_noise = np.zeros(S.shape), np.zeros(K.shape)
+    for t in range(4):
+        pa = dict(arr)
+        for key in ("colloc", "vertices", "centers", "quad_points"):
+            a = np.asarray(arr[key], dtype=np.float64)
+            pa[key] = a * (1 + rng.choice([-1.0, 1.0], size=a.shape) * 2.0 ** -50)
+        Sp, Kp = R.build_matrices_fingreen3d(**pa, wavenumber=0.8, depth=6.0, roots=roots, free_surface=0.0,
+                                             adjoint=True, early_dot_product=False, diagonal_term=False)
+        s_noise, k_noise = np.maximum(s_noise, np.abs(Sp - S)), np.maximum(k_noise, np.abs(Kp - K))
+    for key, val in arr.items():
+        out[f"points_barge_k08__{key}"] = np.asarray(val)
+    out["points_barge_k08__S"], out["points_barge_k08__K"] = S, K
+    out["points_barge_k08__S_noise"], out["points_barge_k08__K_noise"] = s_noise, k_noise
+    out["points_barge_k08__roots"] = roots
     # McCamy and Fuchs' cylinder (test_consistency_with_analytical_solutions.py:
     # 23-44) solved by the reference's matrices and LAPACK's LU (indirect)
     from oracle import solve_coefficients

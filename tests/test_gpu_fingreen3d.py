@@ -196,3 +196,82 @@ def test_solve_all_batched_equals_single():
         err = np.abs(f_b - f_s).max() / np.abs(f_s).max()
         print(f"[gpu]   {pb}: {err:.1e}", flush=True)
         assert err <= 1e-10
+
+
+def test_post_processing_points_match_oracle(fg):
+    """S and the full gradient at points in the fluid (zero normals, no
+    diagonal term), as the post-processing evaluates them from device points
+    (field_at_points), against the oracle within its sensitivity; then the
+    engine's potential and velocity at those points against the fixture
+    matrices times the same sources (within that bound times the sum of the
+    sources' moduli)."""
+    import torch
+    import capytaine_amd as cpa
+    g = lambda key: fg[f"points_barge_k08__{key}"]
+    mesh = cpa.mesh_parallelepiped(size=(4.0, 2.0, 1.0), center=(0, 0, -0.5), resolution=(8, 4, 2),
+                                   missing_sides={"top"})
+    assert np.allclose(mesh.faces_centers, g("centers"))
+    print("[gpu] FinGreen3D post-processing points", flush=True)
+    gf = cpa.FinGreen3D(device=device())
+    pts = torch.as_tensor(g("colloc")).to(device())
+    S, K = gf.evaluate_on_device(pts, mesh, water_depth=6.0, wavenumber=0.8, early_dot_product=False,
+                                 diagonal_term_in_double_layer=False)
+    S = S[0].cpu().numpy().T
+    K = K[0].cpu().numpy().transpose(2, 1, 0)
+    Sr, Kr = g("S"), g("K")
+    es, ek = np.abs(S - Sr).max() / np.abs(Sr).max(), np.abs(K - Kr).max() / np.abs(Kr).max()
+    print(f"[gpu]   S {es:.1e}, gradient {ek:.1e} (sensitivity {g('S_noise').max() / np.abs(Sr).max():.1e}, "
+          f"{g('K_noise').max() / np.abs(Kr).max():.1e})", flush=True)
+    assert np.abs(S - Sr).max() <= 10 * g("S_noise").max() + 1e-10 * np.abs(Sr).max()
+    assert np.abs(K - Kr).max() <= 10 * g("K_noise").max() + 1e-10 * np.abs(Kr).max()
+    engine = cpa.DefaultMatrixEngine(green_function=gf)
+    rng = np.random.default_rng(2)
+    src = rng.normal(size=mesh.nb_faces) + 1j * rng.normal(size=mesh.nb_faces)
+    phi = engine.field_at_points(g("colloc"), mesh, src, water_depth=6.0, wavenumber=0.8)
+    vel = engine.field_at_points(g("colloc"), mesh, src, gradient=True, water_depth=6.0, wavenumber=0.8)
+    # each entry within the matrices' bound above, times the sum of |sources|
+    l1 = np.abs(src).sum()
+    assert np.abs(phi - Sr @ src).max() <= (10 * g("S_noise").max() + 1e-10 * np.abs(Sr).max()) * l1
+    assert np.abs(vel - np.einsum("cij,j->ic", Kr, src)).max() <= (10 * g("K_noise").max() +
+                                                                   1e-10 * np.abs(Kr).max()) * l1
+
+
+def test_symmetric_mesh_equals_merged():
+    """A barge with two reflection planes: the block-symmetric solve (first
+    block column by FinGreen3D, engines.py:178-247) against the merged mesh."""
+    import capytaine_amd as cpa
+    print("[gpu] FinGreen3D symmetric vs merged", flush=True)
+    sym = cpa.mesh_parallelepiped(size=(6.0, 3.0, 1.5), center=(0, 0, -0.75), resolution=(12, 6, 3),
+                                  missing_sides={"top"}, reflection_symmetry=True)
+    gf = cpa.FinGreen3D(device=device())
+    out = {}
+    for name, mesh in (("symmetric", sym), ("merged", sym.merged())):
+        body = cpa.FloatingBody(mesh).add_all_rigid_body_dofs(rotation_center=(0, 0, 0))
+        solver = cpa.BEMSolver(green_function=gf)
+        pbs = [cpa.RadiationProblem(body=body, radiating_dof="Pitch", omega=1.1, water_depth=8.0),
+               cpa.DiffractionProblem(body=body, wave_direction=0.4, omega=1.1, water_depth=8.0)]
+        out[name] = np.array([[r.forces[d] for d in body.dofs] for r in solver.solve_all(pbs)])
+    err = np.abs(out["symmetric"] - out["merged"]).max() / np.abs(out["merged"]).max()
+    print(f"[gpu]   {err:.1e}", flush=True)
+    assert err <= 1e-9
+
+
+def test_forward_speed_batched_equals_single():
+    """Forward speed (the full gradient on the panels, solver.py:200-204)
+    with FinGreen3D: the batched solve against problem-by-problem solves."""
+    import capytaine_amd as cpa
+    print("[gpu] FinGreen3D forward speed", flush=True)
+    mesh = cpa.mesh_parallelepiped(size=(4.0, 2.0, 1.0), center=(0, 0, -0.5), resolution=(8, 4, 2),
+                                   missing_sides={"top"})
+    body = cpa.FloatingBody(mesh).add_all_rigid_body_dofs(rotation_center=(0, 0, 0))
+    solver = cpa.BEMSolver(green_function=cpa.FinGreen3D(device=device()))
+    pbs = [cpa.RadiationProblem(body=body, radiating_dof=d, omega=1.2, forward_speed=0.6, water_depth=6.0)
+           for d in ("Surge", "Heave")]
+    batched = solver.solve_all(pbs)
+    for pb, rb in zip(pbs, batched):
+        rs = solver.solve(pb)
+        f_s = np.array([rs.forces[d] for d in body.dofs])
+        f_b = np.array([rb.forces[d] for d in body.dofs])
+        err = np.abs(f_b - f_s).max() / np.abs(f_s).max()
+        print(f"[gpu]   {pb}: {err:.1e}", flush=True)
+        assert err <= 1e-10
