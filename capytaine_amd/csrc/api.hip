@@ -183,7 +183,7 @@ struct cpt_context {
     cpt::DevBuf bfaces, bquad, bfreq, bargs, btiles;  // multi-body assembly
     cpt::DevBuf lu_work, mixed_work, solve_work;  // solve_work: inverted diagonal blocks of the solves
     cpt::DevBuf gmres_work;                       // Krylov basis and reductions of the GMRES solves
-    cpt::DevBuf fg_roots, fg_ws, fg_lists;        // FinGreen3D: dispersion roots, epsilon tables, pair lists
+    cpt::DevBuf fg_roots, fg_ws, fg_lists, fg_tab; // FinGreen3D: roots, epsilon tables, pair lists, factor tables
     cpt::LuSetup lu_streams;
     int asm_chunk = 0;  // panels per assembly work-group, 0 = by size
     int gemm_tiles = 0; // 1: complex64 products on 128 x 128 tiles, 0: 128 x 64
@@ -861,7 +861,8 @@ int cpt_build_matrices_fingreen3d(cpt_handle h, int nb_collocation_points, const
     const int threads = fingreen3d_threads((int64_t)n * m);
     if (h->fg_roots.reserve(sizeof(double) * (size_t)nb_roots) ||
         h->fg_ws.reserve(sizeof(double) * FG_WS_DOUBLES * (size_t)threads) ||
-        h->fg_lists.reserve(fingreen3d_list_bytes(n))) {
+        h->fg_lists.reserve(fingreen3d_list_bytes(n)) ||
+        h->fg_tab.reserve(fingreen3d_table_bytes((size_t)n + (size_t)m * nb_quad_points))) {
         set_error("cpt_build_matrices_fingreen3d: device allocation failed");
         return CPT_ERR_ALLOC;
     }
@@ -887,6 +888,9 @@ int cpt_build_matrices_fingreen3d(cpt_handle h, int nb_collocation_points, const
     A.nthreads = threads;
     A.lists = (unsigned*)h->fg_lists.p;
     A.list_cap = n > FG_CHUNK_PAIRS ? n : FG_CHUNK_PAIRS;
+    A.npts = (size_t)n + (size_t)m * nb_quad_points;
+    A.tab = (double*)h->fg_tab.p;
+    A.root_tab = A.tab + 4 * (size_t)100 * A.npts;
     CallScope call_scope(&h->timer, h->gemm_tiles);
     CPT_CHECK_HIP((hipError_t)launch_fingreen3d_pairs(A, stream), "FinGreen3D kernel launch");
     return CPT_OK;

@@ -60,7 +60,11 @@ struct FgArgs {
     int nthreads;
     unsigned* lists;       // three lists of list_cap pair indices (by series), then 3 counters
     int list_cap;
+    double* tab;           // per-point factor tables (4 x 100 entries per point, interleaved by point)
+    double* root_tab;      // per-root factors (3 x 100)
+    size_t npts;           // points of the tables: the n collocation points, then the m x nq quadrature points
 };
+size_t fingreen3d_table_bytes(size_t npts);
 constexpr int FG_WS_DOUBLES = 400;
 // panel pairs per pass of the sorted kernels (bounds the lists' memory)
 constexpr int FG_CHUNK_PAIRS = 1 << 24;

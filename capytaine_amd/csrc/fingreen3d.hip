@@ -33,8 +33,12 @@ __device__ __forceinline__ void pair_wave(const FgArgs& A, int I, int J, double*
         // the reference passes the panel's depth first for the adjoint
         // double layer and the collocation point's first otherwise
         const double zf = A.adjoint ? qp[2] : x2, zp = A.adjoint ? x2 : qp[2];
-        const fg::Grn g = CLASS == 3 ? fg::point(r, zf, zp, A.v, A.roots, A.depth, ws, st)
-                                     : fg::point_in<CLASS == 3 ? 0 : CLASS>(r, zf, zp, A.v, A.roots, A.depth, ws, st);
+        // the factor tables of the zf and zp points
+        const size_t pc = (size_t)I, pq = (size_t)A.n + (size_t)J * A.nq + q;
+        const fg::TrigTable tr{A.root_tab, A.tab + (A.adjoint ? pq : pc), A.tab + (A.adjoint ? pc : pq), A.npts};
+        const fg::Grn g = CLASS == 3
+                              ? fg::point(r, zf, zp, A.v, A.roots, A.depth, tr, ws, st)
+                              : fg::point_in<CLASS == 3 ? 0 : CLASS>(r, zf, zp, A.v, A.roots, A.depth, tr, ws, st);
         double drdx1 = 0.0, drdx2 = 0.0;
         if (fabs(r) > 16 * DBL_EPS) {
             drdx1 = A.adjoint ? dx / r : -dx / r;
@@ -63,6 +67,41 @@ __device__ __forceinline__ void pair_wave(const FgArgs& A, int I, int J, double*
             k[c].re += M1_4PI * nre[c];
             k[c].im += M1_4PI * nim[c];
         }
+    }
+}
+
+// The per-root factors of the series (pm, ac, umt by term index).
+__global__ void fingreen3d_root_table_kernel(const double* __restrict__ wvn, double H, double V, double* root)
+{
+    const int i = threadIdx.x;
+    if (i >= fg::TAB_TERMS) return;
+    const fg::TrigDirect d{0.0, 0.0, H, V, wvn};
+    const bool used = i >= 2;
+    root[i] = used ? d.pm(i) : 0.0;
+    root[fg::TAB_TERMS + i] = used ? d.ac(i) : 0.0;
+    root[2 * fg::TAB_TERMS + i] = used ? d.umt(i) : 0.0;
+}
+
+// The per-point factors cos/sin of k (z + h) and of umt (z + h) for every
+// term, with the expressions the series use; points are the collocation
+// points, then the quadrature points.
+__global__ void __launch_bounds__(256) fingreen3d_point_table_kernel(const double* __restrict__ colloc, int n,
+                                                                     const double* __restrict__ quad,
+                                                                     const double* __restrict__ wvn, double H,
+                                                                     double* tab, size_t npts)
+{
+    const size_t total = npts * fg::TAB_TERMS;
+    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (size_t)gridDim.x * blockDim.x) {
+        const size_t p = idx % npts;
+        const int i = (int)(idx / npts);
+        if (i < 2) continue;
+        const double z = p < (size_t)n ? colloc[3 * p + 2] : quad[4 * (p - n) + 2];
+        const fg::TrigDirect d{z, z, H, 0.0, wvn};
+        tab[(4 * (size_t)i + 0) * npts + p] = d.cf(i);
+        tab[(4 * (size_t)i + 1) * npts + p] = d.sf(i);
+        tab[(4 * (size_t)i + 2) * npts + p] = d.ctf(i);
+        tab[(4 * (size_t)i + 3) * npts + p] = d.stf(i);
     }
 }
 
@@ -139,7 +178,8 @@ __global__ void __launch_bounds__(256) fingreen3d_list_kernel(FgArgs A, int j0)
     list_pass<CLASS>(A, j0);
 }
 
-__global__ void __launch_bounds__(256) fingreen3d_points_kernel(int np, const double* __restrict__ rr,
+// all four series inline: one wavefront per SIMD and its whole register file
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) fingreen3d_points_kernel(int np, const double* __restrict__ rr,
                                                                 const double* __restrict__ zf,
                                                                 const double* __restrict__ zp, double v,
                                                                 const double* __restrict__ roots, double depth,
@@ -167,6 +207,11 @@ int fingreen3d_threads(int64_t items)
     return (int)(t < cap ? (t > 256 ? t : 256) : cap);
 }
 
+size_t fingreen3d_table_bytes(size_t npts)
+{
+    return sizeof(double) * (4 * (size_t)fg::TAB_TERMS * npts + 3 * fg::TAB_TERMS);
+}
+
 size_t fingreen3d_list_bytes(int n)
 {
     const size_t cap = (size_t)(n > FG_CHUNK_PAIRS ? n : FG_CHUNK_PAIRS);
@@ -180,6 +225,14 @@ int launch_fingreen3d_pairs(const FgArgs& A, hipStream_t stream)
 {
     if (A.n <= 0 || A.m <= 0) return 0;
     TimedLaunch timed(stream, TK_ASSEMBLY, (double)A.n * A.m, 32.0 * A.n * A.m);
+    hipLaunchKernelGGL(fingreen3d_root_table_kernel, dim3(1), dim3(128), 0, stream, A.roots, A.depth, A.v,
+                       A.root_tab);
+    {
+        const size_t total = A.npts * fg::TAB_TERMS;
+        const size_t blocks = (total + 255) / 256;
+        hipLaunchKernelGGL(fingreen3d_point_table_kernel, dim3((unsigned)(blocks < 8192 ? blocks : 8192)), dim3(256),
+                           0, stream, A.colloc, A.n, A.quad, A.roots, A.depth, A.tab, A.npts);
+    }
     const int cols = A.n >= FG_CHUNK_PAIRS ? 1 : FG_CHUNK_PAIRS / A.n;
     for (int j0 = 0; j0 < A.m; j0 += cols) {
         const int nc = A.m - j0 < cols ? A.m - j0 : cols;
