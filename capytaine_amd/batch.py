@@ -75,6 +75,8 @@ class BatchedSweep:
 
     def __init__(self, green_function, bodies, wavenumbers, nrhs, *, precision="mixed", max_iter=10, tol=1e-12,
                  free_surface=0.0, context=None, far_field_fp32=None):
+        from .green_functions import require_delhommeau
+        require_delhommeau(green_function, "BatchedSweep")
         import torch
         if precision not in ("fp64", "mixed"):
             raise ValueError(f"precision must be 'fp64' or 'mixed', got {precision!r}")

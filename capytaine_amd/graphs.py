@@ -30,6 +30,8 @@ NEXP_MAX = 30
 
 class SweepGraph:
     def __init__(self, green_function, body, nb, nrhs, *, free_surface=0.0, water_depth=np.inf):
+        from .green_functions import require_delhommeau
+        require_delhommeau(green_function, "SweepGraph")
         import torch
         from .green_functions import _SING
         if green_function.floating_point_precision != "float64":

@@ -628,6 +628,15 @@ class FinGreen3D(AbstractGreenFunction):
         return S, K
 
 
+def require_delhommeau(green_function, what):
+    """The sweeps, graphs and the server assemble with the Delhommeau kernels
+    (tables, Prony decompositions, batched launches); other Green functions
+    go through BEMSolver.solve / solve_all."""
+    if not isinstance(green_function, Delhommeau):
+        raise NotImplementedError(f"{what} runs the Delhommeau Green function; with {green_function} use "
+                                  "BEMSolver.solve or BEMSolver.solve_all")
+
+
 class HAMS_GF(AbstractGreenFunction):
     """hams.py:212-234: FinGreen3D in finite depth.  Its infinite-depth half,
     the Liang-Wu-Noblesse wave term, is not available on the device."""

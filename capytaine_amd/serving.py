@@ -172,6 +172,8 @@ class BEMServer:
 
     def __init__(self, green_function, *, precision="mixed", slots=3, memory_budget=None, max_job_bytes=None,
                  max_batch_jobs=16, rho=1000.0, g=9.81, max_iter=10, tol=1e-10):
+        from .green_functions import require_delhommeau
+        require_delhommeau(green_function, "BEMServer")
         # tol: relative residual of the mixed-precision refinement.  Requests
         # reach 12000 panels, where the FP64 residual itself is only good to
         # about 1e-12; 1e-10 keeps each solution within ~1e-9 relative of the

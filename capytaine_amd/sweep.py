@@ -25,6 +25,8 @@ from ._lib import check, ptr, stream_handle
 class FrequencySweep:
     def __init__(self, green_function, body, wavenumbers, nrhs, *, free_surface=0.0, water_depth=np.inf,
                  precision="fp64", max_iter=10, tol=1e-12, far_field_fp32=None):
+        from .green_functions import require_delhommeau
+        require_delhommeau(green_function, "FrequencySweep")
         import torch
         self.gf = green_function
         self.body = body

@@ -158,3 +158,24 @@ def test_struct_layouts_match_header(tmp_path):
         assert got[(cname, "size")] == ctypes.sizeof(cls)
         for fname, _ in cls._fields_:
             assert got[(cname, fname)] == getattr(cls, fname).offset, (cname, fname)
+
+
+def test_delhommeau_only_paths_refuse_other_green_functions():
+    """The sweeps, the per-frequency graphs and the server assemble with the
+    Delhommeau kernels; with FinGreen3D they say so before touching the
+    device."""
+    import pytest
+    import capytaine_amd as cpa
+    from capytaine_amd.sweep import FrequencySweep
+    from capytaine_amd.batch import BatchedSweep
+    from capytaine_amd.graphs import SweepGraph
+    from capytaine_amd.serving import BEMServer
+    gf = cpa.FinGreen3D(device="cpu")
+    with pytest.raises(NotImplementedError, match="FrequencySweep runs the Delhommeau"):
+        FrequencySweep(gf, None, [1.0], 1)
+    with pytest.raises(NotImplementedError, match="BatchedSweep runs the Delhommeau"):
+        BatchedSweep(gf, [], [], 1)
+    with pytest.raises(NotImplementedError, match="SweepGraph runs the Delhommeau"):
+        SweepGraph(gf, None, 1, 1)
+    with pytest.raises(NotImplementedError, match="BEMServer runs the Delhommeau"):
+        BEMServer(gf)
