@@ -19,7 +19,7 @@ reported beside it.
 
 Prints one JSON line (see the repository's bench contract).
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline] [--no-fleet] [--no-serving]
-                    [--no-post] [--no-symmetric]
+                    [--no-post] [--no-symmetric] [--no-fingreen3d]
     python bench.py --workload cylinders     # 20000-panel finite-depth array instead
     python bench.py --workload serving       # the request-stream measurement alone
     python bench.py --workload post          # the post-processing maps alone
@@ -572,17 +572,19 @@ def cylinders_benchmark(args, rho=1025.0, g=9.81, chunk=4):
     print(json.dumps(out), flush=True)
 
 
-def fingreen3d_benchmark(args, nfreq=4, cpu_points=100):
+def fingreen3d_benchmark(no_cpu_baseline=False, nfreq=4, cpu_points=100, log=None):
     """FinGreen3D (the HAMS finite-depth Green function) assembly of S and K
     for the 5 x 5 cylinder array (20000 panels, depth 40 m) at `nfreq`
     wavenumbers, one launch pair per wavenumber, the panels resident in HBM;
     timed after one untimed evaluation.  The CPU baseline is the reference's
     own FinGreen3D.f90 (oracle/_ref, OpenMP over the host cores) on the
     first `cpu_points` collocation rows at the first wavenumber, whose rows
-    are also compared with the device's."""
+    are also compared with the device's.  Returns the measurement as a dict
+    (the `--workload fingreen3d` line, and the default line's "fingreen3d")."""
     import torch
     import capytaine_amd as cpa
     from capytaine_amd import workloads
+    log = log or (lambda m: print(m, file=sys.stderr, flush=True))
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
     body, omegas, _, depth = workloads.cylinder_array()
@@ -603,7 +605,7 @@ def fingreen3d_benchmark(args, nfreq=4, cpu_points=100):
         gf.evaluate_on_device(mesh, mesh, water_depth=depth, wavenumber=k, out=(S, K))
         torch.cuda.synchronize()
         times.append(time.perf_counter() - t0)
-        print(f"[fingreen3d] k {k:.4f}: {times[-1]:.3f} s", file=sys.stderr, flush=True)
+        log(f"[fingreen3d] k {k:.4f}: {times[-1]:.3f} s")
     t = float(np.sum(times))
     out = {"metric": "FinGreen3D panel-pairs/s (S and K assembly)", "value": nfreq * n * n / t,
            "unit": "panel-pairs/s", "n_gpus": 1, "steps": nfreq, "warmup": 1, "ms_per_step": 1e3 * t / nfreq,
@@ -617,17 +619,16 @@ def fingreen3d_benchmark(args, nfreq=4, cpu_points=100):
     S_dev = S[0][:, :cpu_points].T.cpu().numpy()
     K_dev = K[0][:, :cpu_points].T.cpu().numpy()
     del S, K
-    if args.no_cpu_baseline:
-        print(json.dumps(out), flush=True)
-        return
+    torch.cuda.empty_cache()
+    if no_cpu_baseline:
+        return out
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     try:
         from oracle import Reference, mesh_arrays
         R = Reference()
     except (ImportError, OSError, FileNotFoundError) as e:
         out["cpu_baseline"] = {"error": f"oracle/_ref not available: {e}"}
-        print(json.dumps(out), flush=True)
-        return
+        return out
     cores = min(os.cpu_count() or 1, 16)
     arr = mesh_arrays(mesh)
     arr["colloc"] = arr["colloc"][:cpu_points]
@@ -656,7 +657,7 @@ def fingreen3d_benchmark(args, nfreq=4, cpu_points=100):
                                            adjoint=True, free_surface=0.0)
     out["reference_sensitivity"] = {"S": float(np.abs(S_p - S_ref).max() / np.abs(S_ref).max()),
                                     "K": float(np.abs(K_p - K_ref).max() / np.abs(K_ref).max())}
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def main():
@@ -669,6 +670,8 @@ def main():
     ap.add_argument("--no-serving", action="store_true", help="skip the request-stream latency measurement")
     ap.add_argument("--no-post", action="store_true", help="skip the post-processing (elevation, velocity) maps")
     ap.add_argument("--no-symmetric", action="store_true", help="skip the reflection-symmetric barge measurement")
+    ap.add_argument("--no-fingreen3d", action="store_true",
+                    help="skip the FinGreen3D assembly of the 20000-panel cylinder array")
     ap.add_argument("--gemm-tiles", type=int, default=None,
                     help="force the complex64 tile product variant of the mixed form (cpt_set_tuning "
                          "gemm_tiles, 0-4) after tuning, for profiler passes of one variant")
@@ -705,7 +708,7 @@ def main():
         copy_calibration()
         return
     if args.workload == "fingreen3d":
-        fingreen3d_benchmark(args)
+        print(json.dumps(fingreen3d_benchmark(no_cpu_baseline=args.no_cpu_baseline)), flush=True)
         return
     if args.workload == "serving":
         import torch
@@ -945,6 +948,9 @@ def main():
                                                            log=lambda m: print(m, file=sys.stderr, flush=True))
     if not args.no_fleet:
         out["fleet"] = fleet_benchmark(gf, dev, log=lambda m: print(m, file=sys.stderr, flush=True))
+    if not args.no_fingreen3d:
+        # the FinGreen3D finite-depth Green function on the 20000-panel array
+        out["fingreen3d"] = fingreen3d_benchmark(no_cpu_baseline=args.no_cpu_baseline)
     if not args.no_serving:
         _lib.check(_lib.load().cpt_release_workspace(ctx.handle), "cpt_release_workspace")
         torch.cuda.synchronize()
